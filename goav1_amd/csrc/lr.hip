@@ -1,0 +1,340 @@
+// AV1 loop restoration (Wiener and self-guided) on gfx950.
+//
+// Semantics follow goav1's striped libaom flow (tile/restoration_apply.go:526-570,
+// tile/restoration_boundary.go, restoration/wiener.go, restoration/selfguided.go):
+//   - the plane is the visible (rounded) plane; units are offset 8 rows up
+//     (restoration_geometry.go:54-93) and filtered in 64-row stripes
+//     (:97-143) and 64-column processing units (:163-236);
+//   - inside a stripe the filter reads the post-CDEF frame, except the 3 rows
+//     above / below: interior seams read rows saved from the DEBLOCKED frame
+//     (above: y0-2, y0-2, y0-1; below: y1, y1+1, y1+1 with a single saved row
+//     duplicated), the frame top / bottom read the frame edge row
+//     (restoration_boundary.go:192-255, 553-614);
+//   - columns past the plane read the edge sample (3-sample extension of the
+//     data, 4-sample extension of the saved rows: both are edge replication);
+//   - Wiener: H pass with offset 2^(bd+6), round0, clamp to 2^(bd+1+7-round0)-1,
+//     V pass with round1 (wiener.go:132-216);
+//   - SGR: box sums, uint32 p/z/A/B math with LUTs, 6/5 (r=2, every other row)
+//     and 4/3 (r=1) weighted sums, projection with an int16 wrap (selfguided.go).
+//
+// Decomposition: one workgroup per (plane, stripe, 64-column processing unit);
+// every such tile lies inside exactly one restoration unit, so the record is
+// looked up per tile.  The tile and its 3-sample halo are staged in LDS once.
+// The kernel is out of place (post-CDEF frame -> output frame); NONE units and
+// NONE planes copy.
+#include "pf_internal.h"
+
+namespace pf {
+
+#define LR_TW 70          // 64 + 6
+#define LR_AB 66          // 64 + 2
+
+struct LrPlaneGeom {
+    int pw, ph, procw, sh, off, xtiles, stripes;
+};
+
+__device__ __forceinline__ int32_t r2s(int32_t v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
+__device__ __forceinline__ uint32_t r2u(uint32_t v, int n) { return n <= 0 ? v : (v + (1u << (n - 1))) >> n; }
+
+__device__ __constant__ int8_t kSgrR[16][2] = {
+    {2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{0,1},{0,1},{0,1},{0,1},{2,0},{2,0}};
+__device__ __constant__ int16_t kSgrS[16][2] = {
+    {140,3236},{112,2158},{93,1618},{80,1438},{70,1295},{58,1177},{47,1079},{37,996},
+    {30,925},{25,863},{-1,2589},{-1,1618},{-1,1177},{-1,925},{56,-1},{22,-1}};
+
+// AV1's x/(x+1) Q8 table from its closed form (exact: no ties occur)
+__device__ __forceinline__ int32_t x_by_xplus1(uint32_t z)
+{
+    if (z == 0) return 1;
+    if (z >= 255) return 256;
+    return (int32_t)((512u * z + (z + 1)) / (2u * (z + 1)));
+}
+
+template <typename T>
+__device__ __forceinline__ int read_px(const PlaneView &pv, int y, int x)
+{
+    return row_ptr<T>(pv, y)[x];
+}
+
+struct LrLaunchPlane {
+    int type, us, hz, vt;
+    const av1pf_lr_unit *units;
+    const uint16_t *bnd;
+    LrPlaneGeom g;
+    int plane;
+};
+struct LrLaunch { LrLaunchPlane p[3]; int n; int bd; };
+
+template <typename T>
+__global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, LrLaunch L)
+{
+    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ uint16_t tmp[LR_TW * 64];
+    __shared__ int32_t Ab[LR_AB * LR_AB];
+    __shared__ int32_t Bb[LR_AB * LR_AB];
+
+    const LrLaunchPlane &P = L.p[blockIdx.z];
+    const LrPlaneGeom &g = P.g;
+    const int k = blockIdx.x, t = blockIdx.y;
+    if (k >= g.xtiles || t >= g.stripes) return;
+    const int x0 = k * g.procw;
+    const int w = min(g.procw, g.pw - x0);
+    const int y0 = max(0, t * g.sh - g.off);
+    const int y1 = min((t + 1) * g.sh - g.off, g.ph);
+    if (w <= 0 || y0 >= y1) return;
+    const int h = y1 - y0;
+    const int tid = threadIdx.x;
+    const PlaneView &sp = src.p[P.plane];
+    const PlaneView &dp = dst.p[P.plane];
+    const int bd = L.bd;
+
+    av1pf_lr_unit u;
+    u.type = AV1PF_LR_NONE;
+    if (P.type != AV1PF_LR_NONE) {
+        const int uc = min(x0 / P.us, P.hz - 1);
+        const int ur = min((y0 + g.off) / P.us, P.vt - 1);
+        u = P.units[ur * P.hz + uc];
+    }
+    if (u.type != AV1PF_LR_WIENER && u.type != AV1PF_LR_SGRPROJ) {
+        for (int i = tid; i < w * h; i += 256) {
+            const int r = i / w, c = i - r * w;
+            row_ptr<T>(dp, y0 + r)[x0 + c] = row_ptr<T>(sp, y0 + r)[x0 + c];
+        }
+        return;
+    }
+    // ---- stage the tile with its 3-sample halo
+    const int nominal = g.sh - (y0 == 0 ? g.off : 0);
+    const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
+    const int tw = w + 6, th = h + 6;
+    const uint16_t *bnd = P.bnd + (int64_t)t * 4 * g.pw;
+    for (int i = tid; i < tw * th; i += 256) {
+        const int r = i / tw - 3, c = i - (i / tw) * tw - 3;
+        const int x = min(max(x0 + c, 0), g.pw - 1);
+        int v;
+        if (r < 0) {
+            if (copy_above) v = bnd[(r == -1 ? 1 : 0) * g.pw + x];
+            else v = read_px<T>(sp, max(y0 + r, 0), x);
+        } else if (r >= h) {
+            if (copy_below) v = bnd[(2 + min(r - h, 1)) * g.pw + x];
+            else v = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
+        } else {
+            v = read_px<T>(sp, y0 + r, x);
+        }
+        tile[(r + 3) * LR_TW + c + 3] = (uint16_t)v;
+    }
+    __syncthreads();
+    const uint16_t *O = tile + 3 * LR_TW + 3;   // tile origin
+    const int32_t mx = (1 << bd) - 1;
+
+    if (u.type == AV1PF_LR_WIENER) {
+        int round0 = 3, round1 = 11;
+        const int ibr = bd + 7 - round0 + 2;
+        if (ibr > 16) { round0 += ibr - 16; round1 -= ibr - 16; }
+        const int32_t h0 = u.wiener_h[0], h1 = u.wiener_h[1], h2 = u.wiener_h[2];
+        const int32_t h3 = -2 * (h0 + h1 + h2);
+        const int32_t v0 = u.wiener_v[0], v1 = u.wiener_v[1], v2 = u.wiener_v[2];
+        const int32_t v3 = -2 * (v0 + v1 + v2);
+        const int32_t limit = 1 << (bd + 1 + 7 - round0), hoff = 1 << (bd + 6);
+        for (int i = tid; i < w * th; i += 256) {
+            const int r = i / w - 3, c = i - (i / w) * w;
+            const uint16_t *s = O + r * LR_TW + c;
+            int32_t sum = ((int32_t)s[0] << 7) + hoff;
+            sum += h0 * ((int32_t)s[-3] + s[3]) + h1 * ((int32_t)s[-2] + s[2]) + h2 * ((int32_t)s[-1] + s[1]) + h3 * (int32_t)s[0];
+            tmp[(r + 3) * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
+        }
+        __syncthreads();
+        const int32_t voff = 1 << (bd + round1 - 1);
+        for (int i = tid; i < w * h; i += 256) {
+            const int r = i / w, c = i - r * w;
+            const uint16_t *s = tmp + (r + 3) * 64 + c;
+            int32_t sum = ((int32_t)s[0] << 7) - voff;
+            sum += v0 * ((int32_t)s[-3 * 64] + s[3 * 64]) + v1 * ((int32_t)s[-2 * 64] + s[2 * 64]) +
+                   v2 * ((int32_t)s[-64] + s[64]) + v3 * (int32_t)s[0];
+            row_ptr<T>(dp, y0 + r)[x0 + c] = (T)min(max(r2s(sum, round1), 0), mx);
+        }
+        return;
+    }
+
+    // ---- self-guided
+    const int set = u.sgr_set;
+    int xq0, xq1;
+    if (kSgrR[set][0] == 0) { xq0 = 0; xq1 = 128 - u.sgr_xqd[1]; }
+    else if (kSgrR[set][1] == 0) { xq0 = u.sgr_xqd[0]; xq1 = 0; }
+    else { xq0 = u.sgr_xqd[0]; xq1 = 128 - xq0 - u.sgr_xqd[1]; }
+    constexpr int PPT = 16;   // pixels per thread (64x64 / 256)
+    int32_t f0[PPT], f1[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) { f0[j] = 0; f1[j] = 0; }
+    const int aw = w + 2;
+    for (int pass = 0; pass < 2; pass++) {
+        const int r = kSgrR[set][pass];
+        if (r == 0) continue;
+        const int n = (2 * r + 1) * (2 * r + 1);
+        const uint32_t s = (uint32_t)kSgrS[set][pass];
+        const uint32_t recip = (uint32_t)((8192 + n) / (2 * n));
+        const int step = pass == 0 ? 2 : 1;
+        const int nrows = pass == 0 ? (h + 3) / 2 : h + 2;   // rows -1, 1, 3, ... or -1..h
+        __syncthreads();
+        for (int i = tid; i < nrows * aw; i += 256) {
+            const int ri = i / aw;
+            const int y = -1 + ri * step, x = i - ri * aw - 1;
+            int32_t sum = 0, sq = 0;
+            for (int dy = -r; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++) {
+                    const int32_t v = O[(y + dy) * LR_TW + x + dx];
+                    sum += v;
+                    sq += v * v;
+                }
+            const uint32_t a = (uint32_t)r2s(sq, 2 * (bd - 8));
+            const uint32_t b = (uint32_t)r2s(sum, bd - 8);
+            const uint32_t p = a * (uint32_t)n >= b * b ? a * (uint32_t)n - b * b : 0u;
+            uint32_t z = r2u(p * s, 20);
+            z = min(z, 255u);
+            const int32_t aa = x_by_xplus1(z);
+            Ab[(y + 1) * LR_AB + x + 1] = aa;
+            Bb[(y + 1) * LR_AB + x + 1] = (int32_t)r2u((uint32_t)(256 - aa) * (uint32_t)sum * recip, 12);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            const int i = tid + 256 * j;
+            if (i >= w * h) break;
+            const int y = i / w, x = i - y * w;
+            const int32_t uu = O[y * LR_TW + x];
+            const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
+            const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
+            int32_t a, b, v;
+            if (pass == 0) {
+                if ((y & 1) == 0) {
+                    a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
+                    b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
+                    v = r2s(a * uu + b, 9);
+                } else {
+                    a = A[0] * 6 + (A[-1] + A[1]) * 5;
+                    b = B[0] * 6 + (B[-1] + B[1]) * 5;
+                    v = r2s(a * uu + b, 8);
+                }
+                f0[j] = v;
+            } else {
+                a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
+                    (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
+                b = (B[0] + B[-1] + B[1] + B[-LR_AB] + B[LR_AB]) * 4 +
+                    (B[-LR_AB - 1] + B[LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB + 1]) * 3;
+                f1[j] = r2s(a * uu + b, 9);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const int i = tid + 256 * j;
+        if (i >= w * h) break;
+        const int y = i / w, x = i - y * w;
+        const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
+        int32_t v = uu << 7;
+        v += xq0 * (f0[j] - uu);
+        v += xq1 * (f1[j] - uu);
+        const int32_t wv = (int16_t)r2s(v, 11);
+        row_ptr<T>(dp, y0 + y)[x0 + x] = (T)min(max(wv, 0), mx);
+    }
+}
+
+// Save the deblocked context rows of every stripe: bnd[t][0..1] = rows y0-2,
+// y0-1 (t > 0), bnd[t][2..3] = rows y1, y1+1 (duplicated when y1+1 is past
+// the plane), for stripes with an interior lower seam.
+template <typename T>
+__global__ void lr_save_kernel(FrameView f, LrLaunch L)
+{
+    const LrLaunchPlane &P = L.p[blockIdx.z];
+    const LrPlaneGeom &g = P.g;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (x >= g.pw || t >= g.stripes) return;
+    const PlaneView &pv = f.p[P.plane];
+    const int y0 = max(0, t * g.sh - g.off);
+    const int y1 = min((t + 1) * g.sh - g.off, g.ph);
+    uint16_t *b = const_cast<uint16_t *>(P.bnd) + (int64_t)t * 4 * g.pw;
+    if (t > 0) {
+        b[x] = row_ptr<T>(pv, y0 - 2)[x];
+        b[g.pw + x] = row_ptr<T>(pv, y0 - 1)[x];
+    }
+    if (y1 < g.ph) {
+        b[2 * g.pw + x] = row_ptr<T>(pv, y1)[x];
+        b[3 * g.pw + x] = row_ptr<T>(pv, y1 + 1 < g.ph ? y1 + 1 : y1)[x];
+    }
+}
+
+static LrLaunch make_launch(const FrameView &f, const LrArgs &a, bool all_planes, dim3 *grid, int xdiv)
+{
+    LrLaunch L{};
+    L.bd = f.bd;
+    int mx = 0, my = 0;
+    for (int i = 0; i < f.nplanes; i++) {
+        const LrPlaneArgs &pa = a.p[i];
+        if (!all_planes && pa.type == AV1PF_LR_NONE) continue;
+        LrLaunchPlane &p = L.p[L.n++];
+        p.plane = i;
+        p.type = pa.type;
+        p.us = pa.unit_size;
+        p.hz = pa.horz;
+        p.vt = pa.vert;
+        p.units = pa.units;
+        p.bnd = pa.bnd;
+        const int sx = i ? f.ssx : 0, sy = i ? f.ssy : 0;
+        p.g.pw = f.p[i].w;
+        p.g.ph = f.p[i].h;
+        p.g.procw = 64 >> sx;
+        p.g.sh = 64 >> sy;
+        p.g.off = 8 >> sy;
+        p.g.xtiles = (p.g.pw + p.g.procw - 1) / p.g.procw;
+        p.g.stripes = (p.g.ph + p.g.off + p.g.sh - 1) / p.g.sh;
+        mx = max(mx, xdiv ? (p.g.pw + xdiv - 1) / xdiv : p.g.xtiles);
+        my = max(my, p.g.stripes);
+    }
+    *grid = dim3(mx, my, L.n);
+    return L;
+}
+
+hipError_t launch_lr_save(const FrameView &f, const LrArgs &a, hipStream_t s)
+{
+    dim3 grid;
+    LrLaunch L = make_launch(f, a, false, &grid, 256);
+    if (L.n == 0) return hipSuccess;
+    if (f.bd == 8) hipLaunchKernelGGL(lr_save_kernel<uint8_t>, grid, dim3(256), 0, s, f, L);
+    else hipLaunchKernelGGL(lr_save_kernel<uint16_t>, grid, dim3(256), 0, s, f, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_lr(const FrameView &src, const FrameView &dst, const LrArgs &a, hipStream_t s)
+{
+    dim3 grid;
+    LrLaunch L = make_launch(src, a, true, &grid, 0);
+    if (L.n == 0) return hipSuccess;
+    if (src.bd == 8) hipLaunchKernelGGL(lr_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst, L);
+    else hipLaunchKernelGGL(lr_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst, L);
+    return hipGetLastError();
+}
+
+// Plain copy of every plane's aligned extent (used when a stage mix leaves the
+// result in scratch).
+template <typename T>
+__global__ void copy_kernel(FrameView src, FrameView dst)
+{
+    const int pl = blockIdx.z;
+    if (pl >= src.nplanes) return;
+    const PlaneView &s = src.p[pl];
+    const PlaneView &d = dst.p[pl];
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= s.aw || y >= s.ah) return;
+    row_ptr<T>(d, y)[x] = row_ptr<T>(s, y)[x];
+}
+
+hipError_t launch_copy(const FrameView &src, const FrameView &dst, hipStream_t s)
+{
+    dim3 grid((src.p[0].aw + 255) / 256, src.p[0].ah, src.nplanes);
+    if (src.bd == 8) hipLaunchKernelGGL(copy_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst);
+    else hipLaunchKernelGGL(copy_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst);
+    return hipGetLastError();
+}
+
+}  // namespace pf

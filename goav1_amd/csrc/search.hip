@@ -1,0 +1,281 @@
+// Encoder CDEF strength search (SVT search level 1) on gfx950.
+//
+// Semantics follow goav1's cdefSearcher (encoder/cdef_search.go:580-749) and
+// cdef.FilterBlocksU8SubsampledTrusted (cdef/filter_search.go:20-74):
+//   - candidate gi 0..15 = primary gi with secondary 0, then for each primary
+//     p the secondaries 1..3 (level-1 list, cdef_search.go:72-110);
+//   - blocks = 8x8 blocks of the 64x64 filter block with a non-skip MI, in
+//     raster order (computeCDEFList, :410-450);
+//   - the filter input has VeryLarge outside the VISIBLE plane (loadCDEFInput,
+//     :479-505); luma directions come from the reconstruction; chroma takes the
+//     luma direction of the co-located block;
+//   - a trial's luma primary is variance-adjusted per block only when it is
+//     nonzero; a block whose strengths are both zero keeps the reconstruction;
+//   - mse[0][fb][gi] = luma SSE, mse[1][fb][gi] = U SSE + V SSE (uint64), row
+//     step 1 (level 1 subsampling factor).
+//
+// Decomposition: one workgroup per filter block.  Each lane owns 16 luma (then
+// 8 chroma) pixels, loads the centre and its taps once from LDS, and evaluates
+// all 64 trials against them with 64 register accumulators (the batched form
+// of cdef/filter_search_batch.go); the accumulators are reduced across the
+// workgroup with wave shuffles and a 4-way LDS sum.  The work is integer-VALU
+// bound (64 trials x ~70 ops per pixel).
+#include "pf_internal.h"
+
+namespace pf {
+
+#define SVL 0x4000
+#define SP 80   // LDS tile pitch
+
+__device__ __constant__ uint8_t kLevel1[64] = {
+    0, 4, 8, 12, 16, 20, 24, 28, 32, 36, 40, 44, 48, 52, 56, 60,
+    1, 2, 3, 5, 6, 7, 9, 10, 11, 13, 14, 15, 17, 18, 19, 21, 22, 23, 25, 26, 27,
+    29, 30, 31, 33, 34, 35, 37, 38, 39, 41, 42, 43, 45, 46, 47, 49, 50, 51, 53, 54, 55,
+    57, 58, 59, 61, 62, 63 };
+
+__device__ __constant__ int kSOff[12][2] = {
+    { 1 * SP, 2 * SP },         { 1 * SP, 2 * SP - 1 },
+    { -1 * SP + 1, -2 * SP + 2 }, { 1, -1 * SP + 2 },
+    { 1, 2 },                   { 1, 1 * SP + 2 },
+    { 1 * SP + 1, 2 * SP + 2 }, { 1 * SP, 2 * SP + 1 },
+    { 1 * SP, 2 * SP },         { 1 * SP, 2 * SP - 1 },
+    { -1 * SP + 1, -2 * SP + 2 }, { 1, -1 * SP + 2 },
+};
+__device__ __constant__ int kDiv[9] = { 0, 840, 420, 280, 210, 168, 140, 120, 105 };
+
+__device__ __forceinline__ int s_constrain(int diff, int t, int sh)
+{
+    const int a = diff < 0 ? -diff : diff;
+    const int l = min(max(t - (a >> sh), 0), a);
+    return diff < 0 ? -l : l;
+}
+__device__ __forceinline__ int s_ilog2(unsigned v) { return 31 - __clz(v); }
+
+__device__ void s_find_dir(const uint16_t *img, int *dir, int32_t *var)
+{
+    int32_t pa[8][15];
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+#pragma unroll
+        for (int k = 0; k < 15; k++) pa[d][k] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int32_t x = (int32_t)img[i * SP + j] - 128;
+            pa[0][i + j] += x; pa[1][i + (j >> 1)] += x; pa[2][i] += x; pa[3][3 + i - (j >> 1)] += x;
+            pa[4][7 + i - j] += x; pa[5][3 - (i >> 1) + j] += x; pa[6][j] += x; pa[7][(i >> 1) + j] += x;
+        }
+    uint32_t c[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        uint32_t s = 0;
+        if (d == 2 || d == 6) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += (uint32_t)(pa[d][k] * pa[d][k]);
+            s *= 105u;
+        } else if (d == 0 || d == 4) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) s += (uint32_t)(pa[d][k] * pa[d][k] + pa[d][14 - k] * pa[d][14 - k]) * (uint32_t)kDiv[k + 1];
+            s += (uint32_t)(pa[d][7] * pa[d][7]) * 105u;
+        } else {
+#pragma unroll
+            for (int k = 3; k < 8; k++) s += (uint32_t)(pa[d][k] * pa[d][k]);
+            s *= 105u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) s += (uint32_t)(pa[d][k] * pa[d][k] + pa[d][10 - k] * pa[d][10 - k]) * (uint32_t)kDiv[2 * k + 2];
+        }
+        c[d] = s;
+    }
+    int best = 0;
+    int32_t bc = (int32_t)c[0];
+#pragma unroll
+    for (int d = 1; d < 8; d++)
+        if ((int32_t)c[d] > bc) { bc = (int32_t)c[d]; best = d; }
+    *dir = best;
+    *var = (int32_t)((uint32_t)bc - c[(best + 4) & 7]) >> 10;
+}
+
+// Load a w x h region at (x0,y0) plus the 2/8 halo; VeryLarge outside [0,pw)x[0,ph).
+__device__ __forceinline__ void s_load(uint16_t *tile, const PlaneView &pv, int x0, int y0, int w, int h, int tid)
+{
+    const int rows = h + 4, cols = w + 16;
+    for (int i = tid; i < rows * cols; i += 256) {
+        const int r = i / cols, c = i - r * cols;
+        const int y = y0 - 2 + r, x = x0 - 8 + c;
+        uint16_t v = SVL;
+        if (y >= 0 && y < pv.h && x >= 0 && x < pv.w) v = row_ptr<uint8_t>(pv, y)[x];
+        tile[r * SP + c] = v;
+    }
+}
+
+// Evaluate the 64 trials for one pixel and add its squared errors to acc.
+template <bool LUMA>
+__device__ __forceinline__ void s_pixel(uint32_t acc[64], const uint16_t *c, int srcv, int dir, int adj_i,
+                                        bool has_var, int damping)
+{
+    const int x = c[0];
+    const int p0 = kSOff[dir + 2][0], p1 = kSOff[dir + 2][1];
+    const int a0 = c[p0], a1 = c[-p0], a2 = c[p1], a3 = c[-p1];
+    // secondary taps for the block direction (primary trials) and direction 0 (primary-0 trials)
+    int sb[8], sz[8];
+    {
+        const int o[4] = { kSOff[dir + 4][0], kSOff[dir][0], kSOff[dir + 4][1], kSOff[dir][1] };
+#pragma unroll
+        for (int k = 0; k < 4; k++) { sb[2 * k] = c[o[k]]; sb[2 * k + 1] = c[-o[k]]; }
+        const int z[4] = { kSOff[4][0], kSOff[0][0], kSOff[4][1], kSOff[0][1] };
+#pragma unroll
+        for (int k = 0; k < 4; k++) { sz[2 * k] = c[z[k]]; sz[2 * k + 1] = c[-z[k]]; }
+    }
+    const int mk = SVL - 1;
+    int hi = max(max(max(x, a0 & mk), max(a1 & mk, a2 & mk)), a3 & mk);
+    int lo = min(min(min(x, a0), min(a1, a2)), a3);
+#pragma unroll
+    for (int k = 0; k < 8; k++) { hi = max(hi, sb[k] & mk); lo = min(lo, sb[k]); }
+#pragma unroll
+    for (int gi = 0; gi < 64; gi++) {
+        const int packed = gi < 16 ? 4 * gi : 4 * ((gi - 16) / 3) + (gi - 16) % 3 + 1;   // compile-time
+        const int pri = packed >> 2;
+        const int sec = (packed & 3) == 3 ? 4 : (packed & 3);
+        int st = pri;
+        if (LUMA && pri) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
+        int out = x;
+        if (st || sec) {
+            const int ssh = sec ? max(damping - s_ilog2((unsigned)sec), 0) : 0;
+            int sum = 0;
+            if (pri) {
+                const int psh = st ? max(damping - s_ilog2((unsigned)st), 0) : 0;
+                const int t0 = (st & 1) ? 3 : 4, t1 = (st & 1) ? 3 : 2;
+                if (st)
+                    sum = t0 * (s_constrain(a0 - x, st, psh) + s_constrain(a1 - x, st, psh)) +
+                          t1 * (s_constrain(a2 - x, st, psh) + s_constrain(a3 - x, st, psh));
+                if (sec)
+                    sum += 2 * (s_constrain(sb[0] - x, sec, ssh) + s_constrain(sb[1] - x, sec, ssh) +
+                                s_constrain(sb[2] - x, sec, ssh) + s_constrain(sb[3] - x, sec, ssh)) +
+                           s_constrain(sb[4] - x, sec, ssh) + s_constrain(sb[5] - x, sec, ssh) +
+                           s_constrain(sb[6] - x, sec, ssh) + s_constrain(sb[7] - x, sec, ssh);
+                out = x + ((8 + sum - (sum < 0)) >> 4);
+                if (st && sec) out = min(max(out, lo), hi);
+            } else {
+                sum = 2 * (s_constrain(sz[0] - x, sec, ssh) + s_constrain(sz[1] - x, sec, ssh) +
+                           s_constrain(sz[2] - x, sec, ssh) + s_constrain(sz[3] - x, sec, ssh)) +
+                      s_constrain(sz[4] - x, sec, ssh) + s_constrain(sz[5] - x, sec, ssh) +
+                      s_constrain(sz[6] - x, sec, ssh) + s_constrain(sz[7] - x, sec, ssh);
+                out = x + ((8 + sum - (sum < 0)) >> 4);
+            }
+        }
+        const int e = srcv - out;
+        acc[gi] += (uint32_t)(e * e);
+    }
+}
+
+__device__ __forceinline__ void s_reduce(uint32_t acc[64], uint32_t (*part)[64], int tid)
+{
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int gi = 0; gi < 64; gi++) {
+        uint32_t v = acc[gi];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) part[wv][gi] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
+                                                          int damping, uint64_t *mse, int nhfb, int nvfb)
+{
+    __shared__ uint16_t tile[68 * SP];
+    __shared__ uint8_t bdir[64];
+    __shared__ int8_t badj[64];      // adjust index i, -1 = zero variance
+    __shared__ uint8_t blist[64];
+    __shared__ int nblk_s;
+    __shared__ uint32_t part[4][64];
+    const int tid = threadIdx.x;
+    const int fbc = blockIdx.x, fbr = blockIdx.y;
+    const int fb = fbr * nhfb + fbc;
+    const int nfb = nhfb * nvfb;
+    const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
+    const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+    const uint64_t m = noskip ? noskip[fb] : ~0ull;
+    if (tid == 0) {
+        int n = 0;
+        for (int r = 0; r < nvb; r += 2)
+            for (int c = 0; c < nhb; c += 2) {
+                const int b = (r >> 1) * 8 + (c >> 1);
+                if ((m >> b) & 1) blist[n++] = (uint8_t)b;
+            }
+        nblk_s = n;
+    }
+    __syncthreads();
+    const int nblk = nblk_s;
+    uint64_t *m0 = mse + (size_t)fb * 64, *m1 = mse + ((size_t)nfb + fb) * 64;
+    if (nblk == 0) {
+        if (tid < 64) { m0[tid] = 0; m1[tid] = 0; }
+        return;
+    }
+    uint32_t acc[64];
+    // ---- luma
+    {
+        const int x0 = fbc * 64, y0 = fbr * 64, w = nhb * 4, h = nvb * 4;
+        s_load(tile, rec.p[0], x0, y0, w, h, tid);
+        __syncthreads();
+        if (tid < nblk) {
+            const int b = blist[tid];
+            int d;
+            int32_t v;
+            s_find_dir(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
+            bdir[b] = (uint8_t)d;
+            int i = -1;
+            if (v != 0) {
+                i = 0;
+                const int32_t vv = v >> 6;
+                if (vv != 0) i = min(s_ilog2((unsigned)vv), 12);
+            }
+            badj[b] = (int8_t)i;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int gi = 0; gi < 64; gi++) acc[gi] = 0;
+        const int npx = nblk * 64;
+        for (int p = tid; p < npx; p += 256) {
+            const int b = blist[p >> 6], q = p & 63;
+            const int by = b >> 3, bx = b & 7, yy = by * 8 + (q >> 3), xx = bx * 8 + (q & 7);
+            const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
+            const int sv = row_ptr<uint8_t>(src.p[0], y0 + yy)[x0 + xx];
+            s_pixel<true>(acc, c, sv, bdir[b], badj[b], badj[b] >= 0, damping);
+        }
+        s_reduce(acc, part, tid);
+        __syncthreads();
+        if (tid < 64) m0[tid] = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    }
+    // ---- chroma (U + V)
+#pragma unroll
+    for (int gi = 0; gi < 64; gi++) acc[gi] = 0;
+    for (int pl = 1; pl <= 2; pl++) {
+        const int x0 = fbc * 32, y0 = fbr * 32, w = nhb * 2, h = nvb * 2;
+        __syncthreads();
+        s_load(tile, rec.p[pl], x0, y0, w, h, tid);
+        __syncthreads();
+        const int npx = nblk * 16;
+        for (int p = tid; p < npx; p += 256) {
+            const int b = blist[p >> 4], q = p & 15;
+            const int by = b >> 3, bx = b & 7, yy = by * 4 + (q >> 2), xx = bx * 4 + (q & 3);
+            const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
+            const int sv = row_ptr<uint8_t>(src.p[pl], y0 + yy)[x0 + xx];
+            s_pixel<false>(acc, c, sv, bdir[b], 0, true, damping - 1);
+        }
+    }
+    __syncthreads();
+    s_reduce(acc, part, tid);
+    __syncthreads();
+    if (tid < 64) m1[tid] = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+}
+
+hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
+                              int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s)
+{
+    hipLaunchKernelGGL(cdef_search_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse, nhfb, nvfb);
+    return hipGetLastError();
+}
+
+}  // namespace pf

@@ -1,0 +1,219 @@
+"""Host binding of libav1pf.so: the post-filter runner behind the C ABI.
+
+`PostFilterRunner` mirrors goav1's FrameWorkPostFilterRunner
+(internal/av1/decoder/work.go:153-155): one `apply(frame, request)` per
+completed frame, running the remaining post-filter stages in place on the
+frame and writing film grain into a separate display frame
+(decoder/postfilter.go:589-845).  Device memory comes from torch (plumbing
+only); every filter runs in the HIP kernels of libav1pf.so.  There is no CPU
+fallback: if the library or a GPU is missing, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libav1pf.so")
+_lib = None
+
+EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error", "av1pf_apply",
+           "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
+           "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search"]
+
+
+class Av1pfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"av1pf error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libav1pf.so (built by goav1_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python -m goav1_amd.build")
+    lib = C.CDLL(LIB_PATH)
+    lib.av1pf_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    lib.av1pf_destroy.argtypes = [C.c_void_p]
+    lib.av1pf_destroy.restype = None
+    lib.av1pf_version.restype = C.c_char_p
+    lib.av1pf_last_error.argtypes = [C.c_void_p]
+    lib.av1pf_last_error.restype = C.c_char_p
+    lib.av1pf_apply.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
+    lib.av1pf_apply_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
+    lib.av1pf_sync.argtypes = [C.c_void_p, C.c_void_p]
+    lib.av1pf_deblock.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.DeblockParams), C.c_void_p]
+    lib.av1pf_cdef.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.CdefParams), C.c_void_p]
+    lib.av1pf_lr_save_boundaries.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_int, C.c_void_p]
+    lib.av1pf_lr.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_void_p]
+    lib.av1pf_filmgrain.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame),
+                                    C.POINTER(abi.GrainParams), C.c_void_p]
+    lib.av1pf_cdef_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p,
+                                      C.c_int, C.c_void_p, C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the post-filter path runs only on the HIP device")
+    return torch
+
+
+class DeviceFrame:
+    """A frame resident in device memory in the ABI's sample layout."""
+
+    def __init__(self, fr, device="cuda"):
+        torch = _torch()
+        self.meta = fr
+        self.bd = fr.bd
+        self.planes = []
+        for pl in fr.planes:
+            if fr.bd == 8:
+                t = torch.from_numpy(pl.pix.astype(np.uint8)).to(device)
+            else:
+                t = torch.from_numpy(pl.pix.view(np.int16)).to(device)
+            self.planes.append(t)
+        self.abi = self._abi()
+
+    @classmethod
+    def empty_like(cls, fr, device="cuda"):
+        torch = _torch()
+        obj = cls.__new__(cls)
+        obj.meta, obj.bd = fr, fr.bd
+        dt = torch.uint8 if fr.bd == 8 else torch.int16
+        obj.planes = [torch.zeros(pl.pix.shape, dtype=dt, device=device) for pl in fr.planes]
+        obj.abi = obj._abi()
+        return obj
+
+    def _abi(self):
+        fr = self.meta
+        f = abi.Frame()
+        f.width, f.height, f.bit_depth = fr.w, fr.h, fr.bd
+        f.ss_x, f.ss_y, f.mono, f.matrix_identity = fr.ssx, fr.ssy, fr.mono, fr.matrix_identity
+        for i, t in enumerate(self.planes):
+            f.plane[i] = t.data_ptr()
+            f.stride[i] = t.stride(0) * t.element_size()
+        return f
+
+    def to_numpy(self):
+        """Planes back on the host as uint16 arrays (synth.FrameData layout)."""
+        out = []
+        for t in self.planes:
+            a = t.cpu().numpy()
+            out.append(a.astype(np.uint16) if self.bd == 8 else a.view(np.uint16).copy())
+        return out
+
+
+class DeviceSideData:
+    """Per-frame side data uploaded to the device, plus the host request struct."""
+
+    def __init__(self, fr, display=None, device="cuda", stages=None):
+        torch = _torch()
+        self.keep = []
+        req = abi.Request()
+        req.stages = fr.stages if stages is None else stages
+
+        def dev(arr):
+            t = torch.from_numpy(np.ascontiguousarray(arr)).to(device)
+            self.keep.append(t)
+            return t.data_ptr()
+
+        if fr.deblock is not None:
+            d = fr.deblock
+            p = abi.DeblockParams()
+            p.level_y[0], p.level_y[1] = d['level_y']
+            p.level_u, p.level_v, p.sharpness = d['level_u'], d['level_v'], d['sharpness']
+            p.mi_cols, p.mi_rows, p.sb128_w, p.sb128_h = d['mi_cols'], d['mi_rows'], d['sb128_w'], d['sb128_h']
+            p.level_cache = dev(d['level_cache'])
+            p.masks = dev(d['masks'].view(np.int16))
+            self.keep.append(p)
+            req.deblock = C.pointer(p)
+        if fr.cdef is not None:
+            c = fr.cdef
+            p = abi.CdefParams()
+            p.damping, p.strength_count = c['damping'], c['strength_count']
+            for i in range(8):
+                p.y_strength[i], p.uv_strength[i] = int(c['y_strength'][i]), int(c['uv_strength'][i])
+            p.unit_cols, p.unit_rows = c['unit_cols'], c['unit_rows']
+            p.index = dev(c['index'])
+            p.noskip = dev(c['noskip'].view(np.int64))
+            self.keep.append(p)
+            req.cdef = C.pointer(p)
+        if fr.lr is not None:
+            l = fr.lr
+            p = abi.LRParams()
+            for i in range(3):
+                p.frame_type[i] = l['frame_type'][i]
+                p.unit_size[i] = l['unit_size'][i]
+                u = l['units'][i]
+                p.units[i] = dev(u.view(np.uint8)) if u is not None else None
+            self.keep.append(p)
+            req.lr = C.pointer(p)
+        if fr.grain is not None:
+            g = abi.GrainParams()
+            C.memmove(C.byref(g), C.byref(fr.grain), C.sizeof(g))
+            self.keep.append(g)
+            req.grain = C.pointer(g)
+        if display is not None:
+            self.keep.append(display.abi)
+            req.display = C.pointer(display.abi)
+        self.req = req
+
+
+class PostFilterRunner:
+    """One handle per decoder (the reference runner is not reentrant)."""
+
+    def __init__(self, device=-1):
+        _torch()
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.av1pf_create(C.byref(h), device)
+        if rc != abi.OK:
+            raise Av1pfError(rc, "av1pf_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.av1pf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != abi.OK:
+            raise Av1pfError(rc, self.lib.av1pf_last_error(self.h).decode())
+
+    @staticmethod
+    def _stream(stream):
+        if stream is None:
+            import torch
+            return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return C.c_void_p(stream)
+
+    def apply(self, frame, side, stream=None):
+        """Run the request's stages on `frame` (DeviceFrame), asynchronously."""
+        self._check(self.lib.av1pf_apply(self.h, C.byref(frame.abi), C.byref(side.req), self._stream(stream)))
+
+    def apply_batch(self, frames, sides, stream=None):
+        n = len(frames)
+        fa = (abi.Frame * n)(*[f.abi for f in frames])
+        ra = (abi.Request * n)(*[s.req for s in sides])
+        self._check(self.lib.av1pf_apply_batch(self.h, n, fa, ra, self._stream(stream)))
+
+    def sync(self, stream=None):
+        self._check(self.lib.av1pf_sync(self.h, self._stream(stream)))
+
+    def cdef_search(self, src, rec, noskip_dev_ptr, damping, mse_dev_ptr, stream=None):
+        self._check(self.lib.av1pf_cdef_search(self.h, C.byref(src.abi), C.byref(rec.abi), noskip_dev_ptr,
+                                               damping, mse_dev_ptr, self._stream(stream)))

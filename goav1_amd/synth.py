@@ -1,0 +1,415 @@
+"""Seeded synthetic post-filter inputs.
+
+goav1's encoder cannot emit every stage mix the benchmark workloads need (its
+LR keyframe path turns CDEF off, the streaming encoder signals deblock level 0,
+and the high-bit-depth encoders have no CDEF/LR search), so the inputs are
+generated directly: a pre-filter reconstruction plus per-superblock filter
+side data drawn from AV1's legal ranges.  The side data is built the way the
+decoder builds it:
+
+  - a random block partition of each 64x64 superblock with a transform size per
+    block and a skip flag per block;
+  - deblock edge masks in lfmask.FilterMask layout from the transform grid, the
+    filter-width class of an edge being min(class of the two transforms it
+    separates) exactly as dav1d/goav1 mask_edges_{intra,chroma} resolve it
+    (internal/av1/lfmask/mask.go:330-470), so every mask obeys AV1's
+    filter-width-versus-transform-spacing rule;
+  - the 4-byte-per-MI level cache (lfmask/build.go:29-33), chroma levels at
+    chroma MI coordinates (decoder/postfilter_loopfilter_masks.go:170-223);
+  - CDEF strength index per 64x64 unit and the per-unit noskip word (bit by*8+bx
+    set when any MI of the 8x8 block is not skip, postfilter_cdef.go:1276-1320);
+  - loop-restoration unit records with legal Wiener taps and SGR sets/xqd
+    (restoration/wiener.go:11-30, selfguided.go:24-30);
+  - film-grain parameters.
+"""
+import numpy as np
+
+from . import abi
+
+
+def aligned8(v):
+    return (v + 7) & ~7
+
+
+class Plane:
+    """One plane held as uint16 (all bit depths), allocated at the MI-aligned extent."""
+
+    def __init__(self, w, h, alloc_w, alloc_h):
+        self.w, self.h = w, h
+        self.alloc_w, self.alloc_h = alloc_w, alloc_h
+        self.pix = np.zeros((alloc_h, alloc_w), np.uint16)
+
+
+class FrameData:
+    """A frame: geometry plus planes and the per-frame post-filter side data."""
+
+    def __init__(self, w, h, bd, ssx, ssy, mono=False):
+        self.w, self.h, self.bd, self.ssx, self.ssy, self.mono = w, h, bd, ssx, ssy, int(mono)
+        self.matrix_identity = 0
+        aw, ah = aligned8(w), aligned8(h)
+        self.planes = [Plane(w, h, aw, ah)]
+        if not mono:
+            cw, ch = (w + ssx) >> ssx, (h + ssy) >> ssy
+            for _ in range(2):
+                self.planes.append(Plane(cw, ch, aw >> ssx, ah >> ssy))
+        self.deblock = None   # dict
+        self.cdef = None
+        self.lr = None
+        self.grain = None
+        self.stages = 0
+
+    @property
+    def samples(self):
+        return sum(p.w * p.h for p in self.planes)
+
+    @property
+    def bytes_per_sample(self):
+        return 1 if self.bd == 8 else 2
+
+    def copy_pixels(self):
+        out = FrameData.__new__(FrameData)
+        out.__dict__.update(self.__dict__)
+        out.planes = []
+        for p in self.planes:
+            q = Plane(p.w, p.h, p.alloc_w, p.alloc_h)
+            q.pix = p.pix.copy()
+            out.planes.append(q)
+        return out
+
+
+# ---------------------------------------------------------------- pixels
+def fill_blocky(fr, rng, noise_sigma=4.0):
+    """Per-8x8 random DC level in 16..235 + smooth bilinear gradient + Gaussian
+    noise, clipped, scaled to the bit depth (the 720p keyframe recipe)."""
+    scale = 1 << (fr.bd - 8)
+    for pl in fr.planes:
+        H, W = pl.alloc_h, pl.alloc_w
+        dc = rng.integers(16, 236, size=((H + 7) // 8, (W + 7) // 8)).astype(np.float64)
+        dc = np.repeat(np.repeat(dc, 8, 0), 8, 1)[:H, :W]
+        g0, g1, g2, g3 = rng.uniform(-40, 40, size=4)
+        yy, xx = np.mgrid[0:H, 0:W]
+        u, v = xx / max(W - 1, 1), yy / max(H - 1, 1)
+        grad = g0 * (1 - u) * (1 - v) + g1 * u * (1 - v) + g2 * (1 - u) * v + g3 * u * v
+        img = (dc * 0.75 + 32 + grad + rng.normal(0, noise_sigma, size=(H, W))) * scale
+        pl.pix[:] = np.clip(np.rint(img), 0, (1 << fr.bd) - 1).astype(np.uint16)
+
+
+def fill_rectangles(fr, rng, rects, t, sigma=6.0):
+    """Background gradient plus textured rectangles translated by t*velocity
+    (the moving-rectangle sequence recipe; sigma in 10-bit units)."""
+    maxv = (1 << fr.bd) - 1
+    s10 = sigma * (1 << fr.bd) / 1024.0
+    for pi, pl in enumerate(fr.planes):
+        sx = fr.ssx if pi else 0
+        sy = fr.ssy if pi else 0
+        H, W = pl.alloc_h, pl.alloc_w
+        yy, xx = np.mgrid[0:H, 0:W]
+        base = (0.25 + 0.5 * (xx / max(W - 1, 1)) * (0.6 + 0.4 * yy / max(H - 1, 1))) * maxv
+        img = base.copy()
+        for r in rects:
+            x0 = int(round((r['x'] + r['vx'] * t))) >> sx
+            y0 = int(round((r['y'] + r['vy'] * t))) >> sy
+            rw, rh = max(r['w'] >> sx, 1), max(r['h'] >> sy, 1)
+            xa, xb = max(x0, 0), min(x0 + rw, W)
+            ya, yb = max(y0, 0), min(y0 + rh, H)
+            if xa >= xb or ya >= yb:
+                continue
+            lvl = r['level'][pi] * maxv
+            tex = r['tex'][pi]
+            # texture is fixed to the rectangle, so it moves with it
+            ty, tx = ya - y0, xa - x0
+            img[ya:yb, xa:xb] = lvl + tex[ty:ty + (yb - ya), tx:tx + (xb - xa)] * s10
+        pl.pix[:] = np.clip(np.rint(img), 0, maxv).astype(np.uint16)
+
+
+def make_rects(rng, w, h, n=20, sigma=6.0):
+    rects = []
+    for _ in range(n):
+        rw, rh = int(rng.integers(32, max(33, w // 4))), int(rng.integers(32, max(33, h // 4)))
+        r = dict(x=float(rng.integers(0, w)), y=float(rng.integers(0, h)), w=rw, h=rh,
+                 vx=float(rng.uniform(-8, 8)), vy=float(rng.uniform(-8, 8)),
+                 level=rng.uniform(0.1, 0.9, size=3))
+        r['vx'] = float(np.clip(r['vx'], -8, 8))
+        r['tex'] = [rng.uniform(-1.7, 1.7, size=(rh + 1, rw + 1)) for _ in range(3)]
+        rects.append(r)
+    return rects
+
+
+# ---------------------------------------------------------------- block structure
+def _partition(rng, mi_rows, mi_cols, min_blk_log2=1):
+    """Random quadtree partition of each 64x64 superblock.  Returns a list of
+    blocks (y4, x4, log2 size in 4-units) clipped to the MI grid origin."""
+    blocks = []
+    stack = [(y, x, 4) for y in range(0, mi_rows, 16) for x in range(0, mi_cols, 16)]
+    while stack:
+        y, x, l2 = stack.pop()
+        if y >= mi_rows or x >= mi_cols:
+            continue
+        n = 1 << l2
+        must_split = (y + n > mi_rows or x + n > mi_cols) and l2 > min_blk_log2
+        if l2 > min_blk_log2 and (must_split or rng.random() < 0.55):
+            h = n >> 1
+            stack += [(y, x, l2 - 1), (y, x + h, l2 - 1), (y + h, x, l2 - 1), (y + h, x + h, l2 - 1)]
+        else:
+            blocks.append((y, x, l2))
+    return blocks
+
+
+def _cls_luma(w4):
+    return np.minimum(np.log2(np.maximum(w4, 1)).astype(np.int32), 2)
+
+
+def _edge_classes(txid, txsz, luma):
+    """Per-4x4 vertical-edge classes (-1 = no edge) from a transform id grid and
+    transform widths in 4-units along the edge-normal axis."""
+    cls = _cls_luma(txsz) if luma else (txsz > 1).astype(np.int32)
+    e = np.full(txid.shape, -1, np.int32)
+    diff = txid[:, 1:] != txid[:, :-1]
+    c = np.minimum(cls[:, 1:], cls[:, :-1])
+    e[:, 1:] = np.where(diff, c, -1)
+    return e
+
+
+def _pack(ev, eh, nclass, lane_v, lane_h, sbw, sbh, rcw, rch):
+    """Pack per-4x4 edge class grids into [sb][dir][pos][class][half] uint16."""
+    H, W = ev.shape
+    Hp, Wp = sbh * rch, sbw * rcw
+    evp = np.full((Hp, Wp), -1, np.int32)
+    ehp = np.full((Hp, Wp), -1, np.int32)
+    evp[:H, :W] = ev[:Hp, :Wp]
+    ehp[:H, :W] = eh[:Hp, :Wp]
+    out = np.zeros((sbh, sbw, 2, 32, nclass, 2), np.uint16)
+    wv = (np.uint64(1) << np.arange(rch, dtype=np.uint64))
+    wh = (np.uint64(1) << np.arange(rcw, dtype=np.uint64))
+    for k in range(nclass):
+        # vertical edges: pos = column, bits = rows
+        bv = (evp == k).reshape(sbh, rch, sbw, rcw).astype(np.uint64)
+        words = np.einsum('arbc,r->abc', bv, wv)          # (sbh, sbw, rcw)
+        lo = words & np.uint64((1 << lane_v) - 1)
+        hi = words >> np.uint64(lane_v)
+        out[:, :, 0, :rcw, k, 0] = lo.astype(np.uint16)
+        out[:, :, 0, :rcw, k, 1] = hi.astype(np.uint16)
+        bh = (ehp == k).reshape(sbh, rch, sbw, rcw).astype(np.uint64)
+        words = np.einsum('arbc,c->abr', bh, wh)          # (sbh, sbw, rch)
+        lo = words & np.uint64((1 << lane_h) - 1)
+        hi = words >> np.uint64(lane_h)
+        out[:, :, 1, :rch, k, 0] = lo.astype(np.uint16)
+        out[:, :, 1, :rch, k, 1] = hi.astype(np.uint16)
+    return out
+
+
+def build_side_data(fr, rng, *, lf_levels=None, lf_random=False, sharpness=0,
+                    cdef=True, cdef_damping=None, lr_types=(abi.LR_SWITCHABLE,) * 3,
+                    lr_unit=64, skip_prob=0.3, cdef_unread_prob=0.1):
+    """Generate deblock / CDEF / LR side data for `fr` from a random partition."""
+    mi_cols = ((fr.w + 7) >> 3) << 1
+    mi_rows = ((fr.h + 7) >> 3) << 1
+    blocks = _partition(rng, mi_rows, mi_cols)
+    txid = np.zeros((mi_rows, mi_cols), np.int64)
+    txw = np.ones((mi_rows, mi_cols), np.int32)
+    skip = np.zeros((mi_rows, mi_cols), bool)
+    lvl = np.zeros((mi_rows, mi_cols, 4), np.uint8)
+    ssx, ssy = fr.ssx, fr.ssy
+    cmi_rows, cmi_cols = (mi_rows + ssy) >> ssy, (mi_cols + ssx) >> ssx
+    ctxid = np.zeros((cmi_rows, cmi_cols), np.int64)
+    ctxw = np.ones((cmi_rows, cmi_cols), np.int32)
+    tid = 1
+    if lf_levels is None:
+        lf_levels = (int(rng.integers(0, 64)),) * 4
+    base = list(lf_levels)
+    for (y, x, l2) in blocks:
+        n = 1 << l2
+        ye, xe = min(y + n, mi_rows), min(x + n, mi_cols)
+        # luma transform: square, between n/4 and n, at most 16 units (64 px)
+        tl2 = int(rng.integers(max(0, l2 - 2), min(l2, 4) + 1))
+        t = 1 << tl2
+        for ty in range(y, ye, t):
+            for tx in range(x, xe, t):
+                txid[ty:min(ty + t, ye), tx:min(tx + t, xe)] = tid
+                tid += 1
+        txw[y:ye, x:xe] = t
+        skip[y:ye, x:xe] = rng.random() < skip_prob
+        if lf_random:
+            lv = [int(rng.integers(0, 64)) for _ in range(4)]
+        else:
+            lv = base
+        lvl[y:ye, x:xe, 0] = lv[0]
+        lvl[y:ye, x:xe, 1] = lv[1]
+        if not fr.mono:
+            cy, cx = y >> ssy, x >> ssx
+            cn_y, cn_x = max(n >> ssy, 1), max(n >> ssx, 1)
+            cye, cxe = min(cy + cn_y, cmi_rows), min(cx + cn_x, cmi_cols)
+            # chroma transform: the largest square <= the chroma block, <= 32 px
+            ct = min(cn_x, cn_y, 8)
+            for ty in range(cy, cye, ct):
+                for tx in range(cx, cxe, ct):
+                    ctxid[ty:min(ty + ct, cye), tx:min(tx + ct, cxe)] = tid
+                    tid += 1
+            ctxw[cy:cye, cx:cxe] = ct
+            lvl[cy:cye, cx:cxe, 2] = lv[2]
+            lvl[cy:cye, cx:cxe, 3] = lv[3]
+    sbw, sbh = (mi_cols + 31) >> 5, (mi_rows + 31) >> 5
+    ev = _edge_classes(txid, txw, True)
+    eh = _edge_classes(txid.T, txw.T, True).T
+    ymask = _pack(ev, eh, 3, 16, 16, sbw, sbh, 32, 32)
+    masks = np.zeros((sbh, sbw, 1280 // 2), np.uint16)
+    masks[:, :, :384] = ymask.reshape(sbh, sbw, 384)
+    if not fr.mono:
+        cev = _edge_classes(ctxid, ctxw, False)
+        ceh = _edge_classes(ctxid.T, ctxw.T, False).T
+        cm = _pack(cev, ceh, 2, 16 >> ssy, 16 >> ssx, sbw, sbh, 32 >> ssx, 32 >> ssy)
+        masks[:, :, 384:] = cm.reshape(sbh, sbw, 256)
+    fr.deblock = dict(level_y=(base[0], base[1]), level_u=base[2], level_v=base[3],
+                      sharpness=int(sharpness), mi_cols=mi_cols, mi_rows=mi_rows,
+                      sb128_w=sbw, sb128_h=sbh,
+                      level_cache=np.ascontiguousarray(lvl.reshape(-1)),
+                      masks=np.ascontiguousarray(masks.reshape(-1)))
+    fr.skip_mi = skip
+    # ---- CDEF
+    ucols, urows = (fr.w + 63) // 64, (fr.h + 63) // 64
+    nsk = ~skip
+    pad = np.zeros((urows * 16, ucols * 16), bool)
+    pad[:mi_rows, :mi_cols] = nsk
+    b8 = pad.reshape(urows * 8, 2, ucols * 8, 2).any(axis=(1, 3))
+    b8 = b8.reshape(urows, 8, ucols, 8).transpose(0, 2, 1, 3).reshape(urows, ucols, 64)
+    w64 = (b8.astype(np.uint64) << np.arange(64, dtype=np.uint64)).sum(axis=2, dtype=np.uint64)
+    if cdef:
+        bits = int(rng.integers(0, 4))
+        cnt = 1 << bits
+        ys = rng.integers(0, 64, size=8).astype(np.uint8)
+        uvs = rng.integers(0, 64, size=8).astype(np.uint8)
+        idx = rng.integers(0, cnt, size=(urows, ucols)).astype(np.int8)
+        idx[w64 == 0] = -1
+        idx[rng.random((urows, ucols)) < cdef_unread_prob] = -1
+        fr.cdef = dict(damping=int(cdef_damping if cdef_damping is not None else rng.integers(3, 7)),
+                       strength_count=cnt, y_strength=ys, uv_strength=uvs,
+                       unit_cols=ucols, unit_rows=urows,
+                       index=np.ascontiguousarray(idx.reshape(-1)),
+                       noskip=np.ascontiguousarray(w64.reshape(-1)))
+    # ---- loop restoration
+    if lr_types is not None:
+        types = list(lr_types)
+        sizes = [lr_unit, max(lr_unit >> ssx, 32), max(lr_unit >> ssx, 32)]
+        units = []
+        for pi in range(3):
+            if pi >= len(fr.planes):
+                types[pi] = abi.LR_NONE
+            if types[pi] == abi.LR_NONE:
+                units.append(None)
+                continue
+            pw, ph = fr.planes[pi].w, fr.planes[pi].h
+            us = sizes[pi]
+            hz, vt = max((pw + us // 2) // us, 1), max((ph + us // 2) // us, 1)
+            arr = np.zeros(hz * vt, dtype=[('type', 'u1'), ('sgr_set', 'u1'), ('sgr_xqd', 'i1', 2),
+                                           ('wiener_v', 'i1', 3), ('wiener_h', 'i1', 3), ('pad', 'u1', 6)])
+            if types[pi] == abi.LR_SWITCHABLE:
+                ut = rng.integers(0, 3, size=hz * vt)
+            else:
+                ut = np.where(rng.random(hz * vt) < 0.8, types[pi], abi.LR_NONE)
+            arr['type'] = ut
+            arr['sgr_set'] = rng.integers(0, 16, size=hz * vt)
+            arr['sgr_xqd'][:, 0] = rng.integers(-96, 32, size=hz * vt)
+            arr['sgr_xqd'][:, 1] = rng.integers(-32, 96, size=hz * vt)
+            for key in ('wiener_v', 'wiener_h'):
+                arr[key][:, 0] = rng.integers(-5, 11, size=hz * vt)
+                arr[key][:, 1] = rng.integers(-23, 9, size=hz * vt)
+                arr[key][:, 2] = rng.integers(-17, 47, size=hz * vt)
+            units.append(arr)
+        fr.lr = dict(frame_type=types, unit_size=sizes, units=units)
+
+
+def random_grain(rng, bd, lag=3, n_y=8, mono=False, csfl=False):
+    g = abi.GrainParams()
+    g.seed = int(rng.integers(0, 65536))
+    xs = np.sort(rng.choice(256, size=n_y, replace=False))
+    g.num_y_points = n_y
+    for i, x in enumerate(xs):
+        g.y_points[2 * i] = int(x)
+        g.y_points[2 * i + 1] = int(rng.integers(0, 256))
+    if not mono and not csfl:
+        for name, cnt_name in (('cb_points', 'num_cb_points'), ('cr_points', 'num_cr_points')):
+            n = 4
+            xs = np.sort(rng.choice(256, size=n, replace=False))
+            setattr(g, cnt_name, n)
+            arr = getattr(g, name)
+            for i, x in enumerate(xs):
+                arr[2 * i] = int(x)
+                arr[2 * i + 1] = int(rng.integers(0, 256))
+    g.chroma_scaling_from_luma = int(csfl)
+    g.scaling_shift = 10
+    g.ar_coeff_lag = lag
+    g.ar_coeff_shift = 7
+    g.grain_scale_shift = 0
+    g.overlap = 1
+    g.clip_to_restricted_range = 0
+    ny = 2 * lag * (lag + 1)
+    for i in range(ny):
+        g.ar_coeffs_y[i] = int(rng.integers(-128, 128))
+    for i in range(ny + 1):
+        g.ar_coeffs_cb[i] = int(rng.integers(-128, 128))
+        g.ar_coeffs_cr[i] = int(rng.integers(-128, 128))
+    g.cb_mult, g.cb_luma_mult, g.cb_offset = int(rng.integers(0, 256)), int(rng.integers(0, 256)), int(rng.integers(0, 512))
+    g.cr_mult, g.cr_luma_mult, g.cr_offset = int(rng.integers(0, 256)), int(rng.integers(0, 256)), int(rng.integers(0, 512))
+    return g
+
+
+# ---------------------------------------------------------------- workloads
+def filter_level_from_qindex(q):
+    """Deblock level used by the keyframe path: (ac*17563 - 421574 + 2^17) >> 18
+    with ac the 8-bit AC quantizer (encoder/loopfilter_apply.go:43-60).  The AC
+    quantizer values used by the workloads are pinned here from AV1's normative
+    ac_qlookup table: q=120 -> 152, q=160 -> 305."""
+    ac = {120: 152, 160: 305}[q]
+    return max(0, min(63, (ac * 17563 - 421574 + (1 << 17)) >> 18))
+
+
+def keyframe_720p(seed=0, w=1280, h=720, bd=8, ssx=1, ssy=1, stages=abi.STAGES_ALL & ~abi.STAGE_FILMGRAIN):
+    """720p 8-bit 4:2:0 keyframe: blocky pixels, q120 deblock level, random
+    CDEF strengths (damping 3 + (120 >> 6) = 4) and switchable 64-px LR units."""
+    rng = np.random.default_rng(seed)
+    fr = FrameData(w, h, bd, ssx, ssy)
+    fill_blocky(fr, rng)
+    lv = filter_level_from_qindex(120)
+    build_side_data(fr, rng, lf_levels=(lv, lv, lv, lv), cdef_damping=3 + (120 >> 6),
+                    lr_types=(abi.LR_SWITCHABLE,) * 3, lr_unit=64)
+    fr.stages = stages
+    return fr
+
+
+def sequence_1080p10(seed=1, nframes=8, w=1920, h=1080, grain=True):
+    """1080p 10-bit 4:2:0 moving-rectangle sequence with Wiener + SGR units and
+    film grain (lag 3, 8 luma points, scaling shift 10)."""
+    rng = np.random.default_rng(seed)
+    rects = make_rects(rng, w, h)
+    frames = []
+    for t in range(nframes):
+        fr = FrameData(w, h, 10, 1, 1)
+        fill_rectangles(fr, rng, rects, t)
+        build_side_data(fr, rng, lf_random=True, sharpness=int(rng.integers(0, 8)),
+                        lr_types=(abi.LR_SWITCHABLE,) * 3, lr_unit=int(rng.choice([64, 128, 256])))
+        fr.stages = abi.STAGES_ALL if grain else abi.STAGES_ALL & ~abi.STAGE_FILMGRAIN
+        if grain:
+            fr.grain = random_grain(rng, 10)
+        frames.append(fr)
+    return frames
+
+
+BATCH_SIZES = [(426, 240), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def batch_mixed(seed=2, n=64, sizes=BATCH_SIZES):
+    """Mixed batch: sizes x {8,10}-bit x {4:2:0, 4:4:4}, random legal params."""
+    rng = np.random.default_rng(seed)
+    frames = []
+    for _ in range(n):
+        w, h = sizes[int(rng.integers(0, len(sizes)))]
+        bd = int(rng.choice([8, 10]))
+        sub = int(rng.integers(0, 2))
+        fr = FrameData(w, h, bd, sub, sub)
+        fill_blocky(fr, rng)
+        lrt = [int(rng.integers(0, 4)) for _ in range(3)]
+        build_side_data(fr, rng, lf_random=True, sharpness=int(rng.integers(0, 8)),
+                        lr_types=lrt, lr_unit=int(rng.choice([64, 128, 256])))
+        fr.stages = abi.STAGES_ALL & ~abi.STAGE_FILMGRAIN
+        frames.append(fr)
+    return frames

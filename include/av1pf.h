@@ -1,0 +1,220 @@
+/*
+ * av1pf.h — C ABI of the MI355X (gfx950) AV1 post-filter library.
+ *
+ * This is the drop-in boundary that replaces goav1's frame post-filter runner
+ * and the encoder's CDEF / loop-restoration candidate search.  Every entry
+ * point takes plain pointers and sizes; no C++ or framework types cross it.
+ *
+ * Reference interfaces replaced (paths relative to the goav1 tree):
+ *   - FrameWorkPostFilterRunner.Apply(FrameWorkPostFilterContext) error
+ *       internal/av1/decoder/work.go:153-155, called once per completed frame
+ *       from runFrameWorkPostFilterRunnerWithCompact (decoder/work.go:1912-1934)
+ *       through the reusable runner in decoder_postfilter_bind.go:398-431.
+ *   - The stage chain it runs, applySupportedPostFiltersInPlace
+ *       internal/av1/decoder/postfilter.go:589-782 (deblock -> LR boundary save
+ *       -> CDEF -> LR boundary save -> loop restoration -> film grain on a
+ *       display copy, postfilter.go:810-845).
+ *   - cdefSearcher.search  internal/av1/encoder/cdef_search.go:580-634
+ *     (per-64x64 MSE table over the 64 strength pairs of search level 1).
+ *
+ * Memory model.  All pointers handed to av1pf_apply / av1pf_apply_batch /
+ * av1pf_cdef_search are DEVICE pointers on the library's device (frames and
+ * side data resident in HBM).  Calls are asynchronous on the given HIP stream
+ * (NULL = the null stream); av1pf_sync() or a stream synchronisation fences
+ * them, mirroring the reference contract that Apply returns only after Output
+ * is written (decoder_postfilter_bind.go:414-431).
+ *
+ * Sample layout (frame/buffer.go:18-139): 8-bit planes are bytes; 10/12-bit
+ * planes are little-endian uint16.  Each plane must hold the MI-aligned extent
+ * ((W+7)&~7) >> ss_x samples by ((H+7)&~7) >> ss_y rows, because deblocking and
+ * CDEF read and write that padding exactly as the reference does
+ * (postfilter_cdef.go:1390-1412, postfilter_loopfilter.go:945-970).
+ *
+ * Errors are validated on the host before any device write
+ * (postfilter.go:598-616); a nonzero status means nothing was launched.
+ */
+#ifndef AV1PF_H
+#define AV1PF_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define AV1PF_OK                 0
+#define AV1PF_ERR_INVALID_ARG   -1  /* threading.ErrInvalidBatch / frame.ErrInvalidFormat */
+#define AV1PF_ERR_UNSUPPORTED   -2  /* decoder.ErrUnsupportedPostFilter (e.g. superres)   */
+#define AV1PF_ERR_DEVICE        -3  /* HIP runtime failure                                */
+#define AV1PF_ERR_SHORT_BUFFER  -4  /* frame.ErrShortBuffer                               */
+#define AV1PF_ERR_NO_DEVICE     -5  /* no usable gfx950 device                            */
+
+/* ---- stage bits (FrameWorkPostFilter* enum, postfilter.go:13-21) -------- */
+#define AV1PF_STAGE_DEBLOCK    0x01u
+#define AV1PF_STAGE_CDEF       0x02u
+#define AV1PF_STAGE_SUPERRES   0x04u  /* not supported: returns AV1PF_ERR_UNSUPPORTED */
+#define AV1PF_STAGE_LR         0x08u
+#define AV1PF_STAGE_FILMGRAIN  0x10u
+
+/* ---- frame descriptor (frame.Frame / frame.Layout, frame/buffer.go) ----- */
+typedef struct av1pf_frame {
+    int32_t width;            /* visible luma width  (FrameSize.CodedWidth == UpscaledWidth) */
+    int32_t height;           /* visible luma height (FrameSize.Height) */
+    int32_t bit_depth;        /* 8, 10 or 12 */
+    int32_t ss_x, ss_y;       /* chroma subsampling shifts (0 or 1) */
+    int32_t mono;             /* 1 = luma only */
+    int32_t matrix_identity;  /* ColorConfig.MatrixCoefficients == identity (film grain clip) */
+    int32_t reserved;
+    void   *plane[3];         /* Y, U, V sample pointers */
+    int64_t stride[3];        /* row pitch in BYTES */
+} av1pf_frame;
+
+/* ---- deblocking side data ------------------------------------------------
+ * One edge bitmask record per 128x128 region in lfmask.FilterMask layout
+ * (internal/av1/lfmask/mask.go:38-41): y[dir][pos][class][half] with dir 0 =
+ * vertical edges, pos = 4x4 column (dir 0) or row (dir 1) inside the region,
+ * class 0/1/2 = filter width 4/8/16 (14-tap), half = low / high 16 bits;
+ * uv[dir][pos][class][half] with class 0/1 = width 4/6 and the half split at
+ * 16 >> ss (mask.go:406-470).  Masks must follow AV1's transform-spacing rule
+ * (filter widths never reach the next edge of the same direction), which every
+ * decoder-built mask does; edges of one direction are then independent.
+ * level_cache holds four level bytes per 4x4 MI cell (Y vertical, Y
+ * horizontal, U, V; lfmask/build.go:29-33); chroma levels sit at chroma MI
+ * coordinates with the luma row stride mi_cols.                              */
+typedef struct av1pf_lf_mask {
+    uint16_t y[2][32][3][2];
+    uint16_t uv[2][32][2][2];
+} av1pf_lf_mask;
+
+typedef struct av1pf_deblock_params {
+    uint8_t  level_y[2];      /* loop_filter_level[0], [1]: both 0 disables chroma too */
+    uint8_t  level_u, level_v;/* frame chroma levels: 0 skips that plane */
+    uint8_t  sharpness;       /* 0..7 */
+    uint8_t  pad[3];
+    int32_t  mi_cols, mi_rows;/* ((W+7)>>3)<<1, ((H+7)>>3)<<1 */
+    int32_t  sb128_w, sb128_h;/* (mi_cols+31)>>5, (mi_rows+31)>>5 */
+    const uint8_t       *level_cache; /* 4*mi_cols*mi_rows bytes */
+    const av1pf_lf_mask *masks;       /* sb128_w*sb128_h records */
+} av1pf_deblock_params;
+
+/* ---- CDEF side data (parser/cdef.go:14-21, threading/cdef_index.go:13-18) */
+typedef struct av1pf_cdef_params {
+    uint8_t  damping;          /* cdef_damping (3..6) */
+    uint8_t  strength_count;   /* 1 << cdef_bits (1..8) */
+    uint8_t  y_strength[8];    /* packed primary<<2 | secondary code */
+    uint8_t  uv_strength[8];
+    uint8_t  pad[6];
+    int32_t  unit_cols, unit_rows;  /* ceil(W/64), ceil(H/64) */
+    const int8_t   *index;    /* per 64x64 unit: strength index, -1 = not read */
+    const uint64_t *noskip;   /* per unit: bit by*8+bx set when the 8x8 block has a
+                                 non-skip MI cell (dav1d noskip_mask); NULL = all */
+} av1pf_cdef_params;
+
+/* ---- loop restoration side data (parser/restoration.go:15-34,
+ *      tile/restoration_schedule.go:40-48) --------------------------------- */
+#define AV1PF_LR_NONE        0
+#define AV1PF_LR_WIENER      1
+#define AV1PF_LR_SGRPROJ     2
+#define AV1PF_LR_SWITCHABLE  3   /* frame type only */
+
+typedef struct av1pf_lr_unit {
+    uint8_t type;           /* AV1PF_LR_NONE / WIENER / SGRPROJ */
+    uint8_t sgr_set;        /* SGRProj parameter set 0..15 */
+    int8_t  sgr_xqd[2];     /* coded xqd (restoration.DecodeSGRXQ input) */
+    int8_t  wiener_v[3];    /* vertical taps 0..2 (tap 3 = -2*sum, mirrored) */
+    int8_t  wiener_h[3];    /* horizontal taps 0..2 */
+    uint8_t pad[6];
+} av1pf_lr_unit;
+
+typedef struct av1pf_lr_params {
+    uint8_t  frame_type[3];   /* per plane AV1PF_LR_* */
+    uint8_t  pad0;
+    uint16_t unit_size[3];    /* per plane unit size (32..256) */
+    uint16_t pad1;
+    const av1pf_lr_unit *units[3]; /* per plane horz_units*vert_units records, row-major */
+} av1pf_lr_params;
+
+/* ---- film grain parameters (parser/film_grain.go:21-56) ----------------- */
+typedef struct av1pf_grain_params {
+    uint16_t seed;
+    uint8_t  num_y_points, num_cb_points, num_cr_points;
+    uint8_t  chroma_scaling_from_luma;
+    uint8_t  scaling_shift;        /* 8..11 */
+    uint8_t  ar_coeff_lag;         /* 0..3 */
+    uint8_t  ar_coeff_shift;       /* 6..9 */
+    uint8_t  grain_scale_shift;    /* 0..3 */
+    uint8_t  overlap;
+    uint8_t  clip_to_restricted_range;
+    uint8_t  y_points[14][2];      /* (value, scaling) */
+    uint8_t  cb_points[10][2];
+    uint8_t  cr_points[10][2];
+    int8_t   ar_coeffs_y[24];
+    int8_t   ar_coeffs_cb[25];
+    int8_t   ar_coeffs_cr[25];
+    uint8_t  cb_mult, cb_luma_mult;
+    uint16_t cb_offset;
+    uint8_t  cr_mult, cr_luma_mult;
+    uint16_t cr_offset;
+} av1pf_grain_params;
+
+/* ---- one frame's request (FrameWorkPostFilterRequest, postfilter.go:278-284) */
+typedef struct av1pf_request {
+    uint32_t stages;                        /* AV1PF_STAGE_* still to run */
+    uint32_t pad;
+    const av1pf_deblock_params *deblock;    /* host struct, device arrays */
+    const av1pf_cdef_params    *cdef;
+    const av1pf_lr_params      *lr;
+    const av1pf_grain_params   *grain;
+    const av1pf_frame          *display;    /* film-grain output frame (separate buffer) */
+} av1pf_request;
+
+typedef struct av1pf_ctx av1pf_ctx;
+
+/* Library handle: one per decoder (the reference runner is not reentrant,
+ * postfilter_parallel.go:143-146).  device < 0 picks the current device.   */
+int  av1pf_create(av1pf_ctx **out, int device);
+void av1pf_destroy(av1pf_ctx *ctx);
+const char *av1pf_version(void);
+const char *av1pf_last_error(const av1pf_ctx *ctx);
+
+/* Run the remaining post-filter stages of one frame in place on `frame`
+ * (film grain goes to req->display).  Replaces FrameWorkPostFilterRunner.Apply
+ * (decoder/work.go:153-155, decoder_postfilter_bind.go:414).               */
+int  av1pf_apply(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *req, void *stream);
+
+/* Run many frames of mixed size, bit depth and chroma format.  One persistent
+ * launch per stage drains a device work queue of 64x64 tiles across all frames
+ * (no counterpart in the reference, which runs frames one at a time).       */
+int  av1pf_apply_batch(av1pf_ctx *ctx, int n, const av1pf_frame *frames, const av1pf_request *reqs, void *stream);
+
+/* Wait for all work the handle has queued on `stream`.                      */
+int  av1pf_sync(av1pf_ctx *ctx, void *stream);
+
+/* Per-stage entry points (the stage functions the chain calls):
+ *   deblock  -> ApplyLoopFilterEdgesFromMasks  (postfilter_loopfilter_masks.go:33)
+ *   cdef     -> ApplyCDEFPostFilter            (postfilter_cdef.go:295)
+ *   lr       -> ApplyLoopRestorationPostFilter (postfilter.go:1277); needs the two
+ *               boundary saves, made by av1pf_lr_save_boundaries
+ *   grain    -> ApplyFilmGrainPostFilter       (postfilter_filmgrain.go:516)
+ * The chain av1pf_apply composes them; these exist for stage-level callers.  */
+int  av1pf_deblock(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_deblock_params *p, void *stream);
+int  av1pf_cdef(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_cdef_params *p, void *stream);
+int  av1pf_lr_save_boundaries(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, int after_cdef, void *stream);
+int  av1pf_lr(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, void *stream);
+int  av1pf_filmgrain(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *dst, const av1pf_grain_params *p, void *stream);
+
+/* ---- encoder CDEF strength search (cdef_search.go:580-749) ---------------
+ * For every 64x64 filter block that has a non-skip 8x8 block, evaluate the 64
+ * (primary, secondary) strength pairs of search level 1 on luma and on chroma
+ * (U+V summed) and write the SSE (uint64, times the subsampling factor 1) to
+ * mse[2][nfb][64] (plane 0 = luma, 1 = chroma).  src and recon are frames of
+ * identical geometry; damping = 3 + (qindex >> 6) (cdef_search.go:588).    */
+int  av1pf_cdef_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
+                       const uint64_t *noskip, int damping, uint64_t *mse, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AV1PF_H */

@@ -1,0 +1,159 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker (or the timed CPU
+baseline), never part of the product path.
+"""
+import ctypes as C
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+
+from goav1_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.orc_chain.argtypes = [C.c_void_p, C.POINTER(abi.Request), C.c_void_p]
+        for name in ("orc_deblock", "orc_cdef"):
+            getattr(_lib, name).argtypes = [C.c_void_p, C.c_void_p]
+        _lib.orc_cdef_search_mse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _lib.orc_cdef_search_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64,
+                                                C.c_void_p, C.c_void_p, C.c_void_p]
+    return _lib
+
+
+class OrcFrame(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("bd", C.c_int), ("ssx", C.c_int), ("ssy", C.c_int),
+                ("mono", C.c_int), ("matrix_identity", C.c_int),
+                ("p", C.c_void_p * 3), ("stride", C.c_int * 3), ("rows", C.c_int * 3)]
+
+
+def orc_frame(fr):
+    """Wrap a synth.FrameData (uint16 planes) for the oracle; keeps planes alive."""
+    of = OrcFrame()
+    of.w, of.h, of.bd, of.ssx, of.ssy, of.mono = fr.w, fr.h, fr.bd, fr.ssx, fr.ssy, fr.mono
+    of.matrix_identity = fr.matrix_identity
+    for i, pl in enumerate(fr.planes):
+        assert pl.pix.flags.c_contiguous and pl.pix.dtype == np.uint16
+        of.p[i] = pl.pix.ctypes.data
+        of.stride[i] = pl.pix.shape[1]
+        of.rows[i] = pl.pix.shape[0]
+    return of
+
+
+class HostRequest:
+    """Builds an abi.Request whose side-data pointers are host arrays (oracle use)."""
+
+    def __init__(self, fr, stages=None):
+        self.keep = []
+        self.req = abi.Request()
+        self.req.stages = fr.stages if stages is None else stages
+        if fr.deblock is not None:
+            d = fr.deblock
+            p = abi.DeblockParams()
+            p.level_y[0], p.level_y[1] = d['level_y']
+            p.level_u, p.level_v, p.sharpness = d['level_u'], d['level_v'], d['sharpness']
+            p.mi_cols, p.mi_rows, p.sb128_w, p.sb128_h = d['mi_cols'], d['mi_rows'], d['sb128_w'], d['sb128_h']
+            p.level_cache = d['level_cache'].ctypes.data
+            p.masks = d['masks'].ctypes.data
+            self.keep.append(p)
+            self.req.deblock = C.pointer(p)
+        if fr.cdef is not None:
+            c = fr.cdef
+            p = abi.CdefParams()
+            p.damping, p.strength_count = c['damping'], c['strength_count']
+            for i in range(8):
+                p.y_strength[i], p.uv_strength[i] = int(c['y_strength'][i]), int(c['uv_strength'][i])
+            p.unit_cols, p.unit_rows = c['unit_cols'], c['unit_rows']
+            p.index = c['index'].ctypes.data
+            p.noskip = c['noskip'].ctypes.data
+            self.keep.append(p)
+            self.req.cdef = C.pointer(p)
+        if fr.lr is not None:
+            l = fr.lr
+            p = abi.LRParams()
+            for i in range(3):
+                p.frame_type[i] = l['frame_type'][i]
+                p.unit_size[i] = l['unit_size'][i]
+                p.units[i] = l['units'][i].ctypes.data if l['units'][i] is not None else None
+            self.keep.append(p)
+            self.req.lr = C.pointer(p)
+        if fr.grain is not None:
+            self.req.grain = C.pointer(fr.grain)
+
+
+def run_chain(fr, stages=None):
+    """Run the oracle chain on a copy of fr; returns (filtered frame, display frame or None)."""
+    out = fr.copy_pixels()
+    hr = HostRequest(fr, stages)
+    of = orc_frame(out)
+    disp = None
+    dptr = None
+    if hr.req.stages & abi.STAGE_FILMGRAIN:
+        disp = fr.copy_pixels()
+        dptr = C.byref(orc_frame_keep(disp))
+    rc = lib().orc_chain(C.byref(of), C.byref(hr.req), dptr)
+    if rc != 0:
+        raise RuntimeError(f"orc_chain failed: {rc}")
+    return out, disp
+
+
+_keep = []
+
+
+def orc_frame_keep(fr):
+    of = orc_frame(fr)
+    _keep.append(of)
+    if len(_keep) > 64:
+        del _keep[:32]
+    return of
+
+
+def plane_md5(fr):
+    """Per-plane MD5 over the visible samples (8-bit as bytes, else LE uint16)."""
+    out = []
+    for pl in fr.planes:
+        v = pl.pix[:pl.h, :pl.w]
+        b = v.astype(np.uint8).tobytes() if fr.bd == 8 else v.astype('<u2').tobytes()
+        out.append(hashlib.md5(b).hexdigest())
+    return out
+
+
+def cdef_search_mse(src, rec, noskip, damping):
+    mi_cols = ((rec.w + 7) >> 3) << 1
+    mi_rows = ((rec.h + 7) >> 3) << 1
+    nfb = ((mi_cols + 15) // 16) * ((mi_rows + 15) // 16)
+    mse = np.zeros((2, nfb, 64), np.uint64)
+    s, r = orc_frame(src), orc_frame(rec)
+    rc = lib().orc_cdef_search_mse(C.byref(s), C.byref(r), noskip.ctypes.data if noskip is not None else None,
+                                   damping, mse.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_cdef_search_mse failed: {rc}")
+    return mse
+
+
+def cdef_search_finish(mse, skip_fb, lam):
+    nfb = mse.shape[1]
+    ys = np.zeros(8, np.uint8)
+    uvs = np.zeros(8, np.uint8)
+    idx = np.zeros(nfb, np.uint8)
+    sk = np.ascontiguousarray(skip_fb.astype(np.uint8))
+    m = np.ascontiguousarray(mse)
+    bits = lib().orc_cdef_search_finish(m.ctypes.data, nfb, sk.ctypes.data, C.c_uint64(int(lam)),
+                                        ys.ctypes.data, uvs.ctypes.data, idx.ctypes.data)
+    return bits, ys, uvs, idx

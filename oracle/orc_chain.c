@@ -1,0 +1,41 @@
+/*
+ * CPU oracle: the post-filter stage chain.  Test infrastructure only.
+ *
+ * Restates decoder/postfilter.go:589-782 (applySupportedPostFiltersInPlace):
+ *   deblock -> save LR boundary lines from the deblocked frame (afterCDEF=false,
+ *   postfilter.go:683-692) -> CDEF -> save LR boundary lines from the CDEF output
+ *   for the frame-edge stripes (afterCDEF=true, postfilter.go:733-738) -> loop
+ *   restoration -> film grain into the display copy (postfilter.go:810-845).
+ * Superres is rejected (ErrUnsupportedPostFilter, decoder_postfilter_bind.go:272-278).
+ */
+#include "orc.h"
+
+int orc_chain(orc_frame *f, const av1pf_request *req, orc_frame *display)
+{
+    if (!f || !req) return AV1PF_ERR_INVALID_ARG;
+    uint32_t st = req->stages;
+    if (st & AV1PF_STAGE_SUPERRES) return AV1PF_ERR_UNSUPPORTED;
+    if ((st & AV1PF_STAGE_DEBLOCK) && !req->deblock) return AV1PF_ERR_INVALID_ARG;
+    if ((st & AV1PF_STAGE_CDEF) && !req->cdef) return AV1PF_ERR_INVALID_ARG;
+    if ((st & AV1PF_STAGE_LR) && !req->lr) return AV1PF_ERR_INVALID_ARG;
+    if ((st & AV1PF_STAGE_FILMGRAIN) && (!req->grain || !display)) return AV1PF_ERR_INVALID_ARG;
+    int rc;
+    if (st & AV1PF_STAGE_DEBLOCK)
+        if ((rc = orc_deblock(f, req->deblock)) != AV1PF_OK) return rc;
+    orc_lr_bounds b[3] = { { 0 } };
+    if (st & AV1PF_STAGE_LR) {
+        if ((rc = orc_lr_bounds_alloc(f, req->lr, b)) != AV1PF_OK) return rc;
+        orc_lr_save(f, req->lr, b, 0);
+    }
+    if (st & AV1PF_STAGE_CDEF)
+        if ((rc = orc_cdef(f, req->cdef)) != AV1PF_OK) { orc_lr_bounds_free(b); return rc; }
+    if (st & AV1PF_STAGE_LR) {
+        orc_lr_save(f, req->lr, b, 1);
+        rc = orc_lr_apply(f, req->lr, b);
+        orc_lr_bounds_free(b);
+        if (rc != AV1PF_OK) return rc;
+    }
+    if (st & AV1PF_STAGE_FILMGRAIN)
+        if ((rc = orc_filmgrain(f, display, req->grain)) != AV1PF_OK) return rc;
+    return AV1PF_OK;
+}
