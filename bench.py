@@ -1,0 +1,167 @@
+"""Benchmark: bit-exact AV1 post-filter throughput on one MI355X.
+
+Workload (one step): the 1920x1080 10-bit 4:2:0 moving-rectangle sequence, 8
+frames, each through deblock -> CDEF -> loop restoration (Wiener + SGR units)
+-> film grain into a display frame, via the C ABI (libav1pf.so).  Inputs are
+resident in HBM before the timed region: every timed step filters its own
+pristine copy of the 8 pre-filter frames (the chain works in place).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is luma pixels per second (Mpx/s);
+`msamples_per_s` counts luma + chroma samples.  The roofline object is for the
+dominant kernel (most GPU time in the timed region), timed with HIP events
+around each of its launches on the launch stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(frames, seconds):
+    """The oracle (plain-C restatement of goav1's Go path) timed on this host on a
+    bounded sample: one thread over whole frames, repeated until `seconds`."""
+    from oracle import orc
+    fr = frames[0]
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.run_chain(fr)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    px = n * fr.w * fr.h
+    return {"value": round(px / el / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
+            "sample": f"{n} x {fr.w}x{fr.h} 10-bit 4:2:0 frame(s), full chain + film grain, oracle/liborc.so, "
+                      f"1 thread, {el:.1f} s"}
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: every box of this run has a single MI355X, and the "
+              "post-filter path shards by frame with no exchange; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from goav1_amd import abi, synth
+    from goav1_amd.lib import DeviceFrame, DeviceSideData, PostFilterRunner
+
+    frames = synth.sequence_1080p10(seed=1, nframes=args.frames)
+    runner = PostFilterRunner()
+    base = [DeviceFrame(fr) for fr in frames]
+    displays = [DeviceFrame.empty_like(fr) for fr in frames]
+    sides = [DeviceSideData(fr, display=d) for fr, d in zip(frames, displays)]
+    total = args.warmup + args.steps
+    copies = []
+    for s in range(total):
+        row = []
+        for b in base:
+            c = DeviceFrame.__new__(DeviceFrame)
+            c.meta, c.bd = b.meta, b.bd
+            c.planes = [t.clone() for t in b.planes]
+            c.abi = c._abi()
+            row.append(c)
+        copies.append(row)
+    torch.cuda.synchronize()
+
+    for s in range(args.warmup):
+        for f, sd in zip(copies[s], sides):
+            runner.apply(f, sd)
+    runner.sync()
+    torch.cuda.synchronize()
+
+    runner.profile(True)
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for s in range(args.warmup, total):
+        for f, sd in zip(copies[s], sides):
+            runner.apply(f, sd)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    runner.sync()
+    gpu_ms = ev0.elapsed_time(ev1)
+    prof = runner.profile_read()
+    runner.profile(False)
+
+    ms_per_step = gpu_ms / args.steps
+    luma_px = sum(fr.w * fr.h for fr in frames)
+    samples = sum(fr.samples for fr in frames)
+    value = luma_px * args.steps / (gpu_ms / 1e3) / 1e6
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    name, (ms, launches, nbytes) = dom
+    achieved = (nbytes / launches) / ((ms / launches) / 1e3) / 1e9
+    stages = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                  "avg_us": round(1e3 * v[0] / v[1], 2),
+                  "gbs": round((v[2] / v[1]) / ((v[0] / v[1]) / 1e3) / 1e9, 1) if v[2] else None}
+              for k, v in prof.items()}
+    out = {
+        "metric": "post-filter Mpx/s (bit-exact)",
+        "value": round(value, 2),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"1920x1080 10-bit 4:2:0 moving-rectangle sequence, {args.frames} frames per step, "
+                               "deblock+CDEF+LR(Wiener/SGR)+film grain (lag 3)",
+                   "frames_per_step": args.frames, "width": 1920, "height": 1080, "bit_depth": 10,
+                   "chroma": "4:2:0"},
+        "msamples_per_s": round(samples * args.steps / (gpu_ms / 1e3) / 1e6, 2),
+        "wall_s_timed": round(wall, 4),
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(name),
+                     "algorithmic_bytes_per_launch": int(nbytes / launches),
+                     "avg_launch_us": round(1e3 * ms / launches, 2)},
+        "stages": stages,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds)
+        out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    runner.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,7 +27,62 @@ struct av1pf_ctx {
     int16_t *templates = nullptr; // film-grain templates (3 x 73 x 82)
     int *dev_err = nullptr;       // device-side validation flag
     uint64_t *search_tmp = nullptr;
+    // per-stage timing
+    bool profiling = false;
+    struct Rec { int stage; hipEvent_t a, b; double bytes; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double acc_ms[AV1PF_PROFILE_STAGES] = {};
+    int acc_n[AV1PF_PROFILE_STAGES] = {};
+    double acc_bytes[AV1PF_PROFILE_STAGES] = {};
 };
+
+namespace {
+enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH };
+
+hipEvent_t take_event(av1pf_ctx *c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one stage launch with events when profiling is on.
+struct StageTimer {
+    av1pf_ctx *c;
+    int stage;
+    hipStream_t s;
+    double bytes;
+    hipEvent_t a = nullptr;
+    StageTimer(av1pf_ctx *c_, int st, hipStream_t s_, double by) : c(c_), stage(st), s(s_), bytes(by)
+    {
+        if (c->profiling) {
+            a = take_event(c);
+            hipEventRecord(a, s);
+        }
+    }
+    ~StageTimer()
+    {
+        if (c->profiling && a) {
+            hipEvent_t b = take_event(c);
+            hipEventRecord(b, s);
+            c->recs.push_back({ stage, a, b, bytes });
+        }
+    }
+};
+
+double frame_bytes(const pf::FrameView &f)
+{
+    double n = 0;
+    for (int i = 0; i < f.nplanes; i++) n += (double)f.p[i].w * f.p[i].h;
+    return n * (f.bd == 8 ? 1 : 2);
+}
+}  // namespace
 
 namespace {
 
@@ -256,9 +311,9 @@ int run_grain(av1pf_ctx *c, const av1pf_frame *frame, const FrameView &A, const 
     if (!c->templates) HIPCHK(c, hipMalloc(&c->templates, 3 * 73 * 82 * sizeof(int16_t)));
     ga.templates = c->templates;
     // copy the frame into the display buffer first (bindFilmGrainPostFilterOutput), then grain
-    HIPCHK(c, launch_copy(A, D, s));
-    HIPCHK(c, launch_grain_templates(ga, frame->bit_depth, frame->ss_x, frame->ss_y, s));
-    HIPCHK(c, launch_grain_apply(A, D, ga, frame->matrix_identity, s));
+    { StageTimer t(c, ST_COPY, s, 2 * frame_bytes(A)); HIPCHK(c, launch_copy(A, D, s)); }
+    { StageTimer t(c, ST_GRAIN_T, s, 0); HIPCHK(c, launch_grain_templates(ga, frame->bit_depth, frame->ss_x, frame->ss_y, s)); }
+    { StageTimer t(c, ST_GRAIN_A, s, 2 * frame_bytes(A)); HIPCHK(c, launch_grain_apply(A, D, ga, frame->matrix_identity, s)); }
     return AV1PF_OK;
 }
 
@@ -360,13 +415,13 @@ int run_chain(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req, 
             if (p->level_u) a.plane_mask |= 2;
             if (p->level_v) a.plane_mask |= 4;
         }
-        HIPCHK(c, launch_deblock(A, a, s));
+        { StageTimer t(c, ST_DEBLOCK, s, 2 * frame_bytes(A)); HIPCHK(c, launch_deblock(A, a, s)); }
     }
     LrArgs la{};
     bool lr_any = false;
     if (st & AV1PF_STAGE_LR) {
         if ((rc = lr_args(c, frame, req->lr, &la, &lr_any))) return rc;
-        if (lr_any) HIPCHK(c, launch_lr_save(A, la, s));
+        if (lr_any) { StageTimer t(c, ST_LR_SAVE, s, 0); HIPCHK(c, launch_lr_save(A, la, s)); }
     }
     if (st & AV1PF_STAGE_CDEF) {
         const av1pf_cdef_params *p = req->cdef;
@@ -380,16 +435,16 @@ int run_chain(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req, 
             a.strength_count = p->strength_count;
             memcpy(a.y_strength, p->y_strength, 8);
             memcpy(a.uv_strength, p->uv_strength, 8);
-            HIPCHK(c, launch_cdef(A, B, a, s, c->dev_err));
+            { StageTimer t(c, ST_CDEF, s, 2 * frame_bytes(A)); HIPCHK(c, launch_cdef(A, B, a, s, c->dev_err)); }
             cur = &B;
         }
     }
     if (lr_any) {
         const FrameView *other = cur == &A ? &B : &A;
-        HIPCHK(c, launch_lr(*cur, *other, la, s));
+        { StageTimer t(c, ST_LR, s, 2 * frame_bytes(A)); HIPCHK(c, launch_lr(*cur, *other, la, s)); }
         cur = other;
     }
-    if (cur != &A) HIPCHK(c, launch_copy(*cur, A, s));
+    if (cur != &A) { StageTimer t(c, ST_COPY, s, 2 * frame_bytes(A)); HIPCHK(c, launch_copy(*cur, A, s)); }
     if (st & AV1PF_STAGE_FILMGRAIN)
         if ((rc = run_grain(c, frame, A, req->display, req->grain, s))) return rc;
     return AV1PF_OK;
@@ -440,6 +495,8 @@ void av1pf_destroy(av1pf_ctx *c)
     if (c->templates) hipFree(c->templates);
     if (c->dev_err) hipFree(c->dev_err);
     if (c->search_tmp) hipFree(c->search_tmp);
+    for (auto &r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : c->pool) hipEventDestroy(e);
     delete c;
 }
 
@@ -559,8 +616,46 @@ int av1pf_cdef_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *r
     HIPCHK(c, hipSetDevice(c->device));
     const int mi_cols = ((recon->width + 7) >> 3) << 1, mi_rows = ((recon->height + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
-    HIPCHK(c, launch_cdef_search(view_of(src), view_of(recon), noskip, damping, mse, nhfb, nvfb,
+    const FrameView sv = view_of(src);
+    StageTimer t(c, ST_SEARCH, static_cast<hipStream_t>(stream), 2 * frame_bytes(sv));
+    HIPCHK(c, launch_cdef_search(sv, view_of(recon), noskip, damping, mse, nhfb, nvfb,
                                  static_cast<hipStream_t>(stream)));
+    return AV1PF_OK;
+}
+
+int av1pf_profile(av1pf_ctx *c, int enable)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
+    for (auto &r : c->recs) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
+    c->recs.clear();
+    for (int i = 0; i < AV1PF_PROFILE_STAGES; i++) { c->acc_ms[i] = 0; c->acc_n[i] = 0; c->acc_bytes[i] = 0; }
+    c->profiling = enable != 0;
+    return AV1PF_OK;
+}
+
+int av1pf_profile_read(av1pf_ctx *c, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
+                       double bytes[AV1PF_PROFILE_STAGES])
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
+    for (auto &r : c->recs) {
+        float e = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&e, r.a, r.b));
+        c->acc_ms[r.stage] += e;
+        c->acc_n[r.stage] += 1;
+        c->acc_bytes[r.stage] += r.bytes;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    for (int i = 0; i < AV1PF_PROFILE_STAGES; i++) {
+        if (ms) ms[i] = c->acc_ms[i];
+        if (count) count[i] = c->acc_n[i];
+        if (bytes) bytes[i] = c->acc_bytes[i];
+    }
     return AV1PF_OK;
 }
 

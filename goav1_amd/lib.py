@@ -21,7 +21,8 @@ _lib = None
 
 EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error", "av1pf_apply",
            "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
-           "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search"]
+           "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_profile", "av1pf_profile_read"]
+STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search"]
 
 
 class Av1pfError(RuntimeError):
@@ -55,6 +56,8 @@ def load():
                                     C.POINTER(abi.GrainParams), C.c_void_p]
     lib.av1pf_cdef_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p,
                                       C.c_int, C.c_void_p, C.c_void_p]
+    lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
+    lib.av1pf_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     _lib = lib
     return lib
 
@@ -213,6 +216,17 @@ class PostFilterRunner:
 
     def sync(self, stream=None):
         self._check(self.lib.av1pf_sync(self.h, self._stream(stream)))
+
+    def profile(self, enable=True):
+        self._check(self.lib.av1pf_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        """{stage: (total_ms, launches, algorithmic_bytes)} since profile(True)."""
+        ms = (C.c_double * 8)()
+        n = (C.c_int * 8)()
+        by = (C.c_double * 8)()
+        self._check(self.lib.av1pf_profile_read(self.h, ms, n, by))
+        return {STAGE_NAMES[i]: (ms[i], n[i], by[i]) for i in range(8) if n[i]}
 
     def cdef_search(self, src, rec, noskip_dev_ptr, damping, mse_dev_ptr, stream=None):
         self._check(self.lib.av1pf_cdef_search(self.h, C.byref(src.abi), C.byref(rec.abi), noskip_dev_ptr,

@@ -214,6 +214,19 @@ int  av1pf_filmgrain(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *
 int  av1pf_cdef_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
                        const uint64_t *noskip, int damping, uint64_t *mse, void *stream);
 
+/* ---- per-stage timing (no counterpart in the reference, whose closest aid is
+ * the FrameWorkPostFilterResult counters, postfilter.go:309-316) ----------
+ * While enabled, every stage launch is bracketed by HIP events on its stream
+ * and the stage's algorithmic bytes (samples read + written, 2*S*b for a
+ * whole-plane pass) are accumulated.  av1pf_profile_read synchronises the
+ * device and returns, per stage id, total ms, launch count and bytes.
+ * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
+ * 6 grain_apply, 7 cdef_search.                                              */
+#define AV1PF_PROFILE_STAGES 8
+int  av1pf_profile(av1pf_ctx *ctx, int enable);
+int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
+                        double bytes[AV1PF_PROFILE_STAGES]);
+
 #ifdef __cplusplus
 }
 #endif
