@@ -2,16 +2,20 @@
 
 Workload (one step): the 1920x1080 10-bit 4:2:0 moving-rectangle sequence, 8
 frames, each through deblock -> CDEF -> loop restoration (Wiener + SGR units)
--> film grain into a display frame, via the C ABI (libav1pf.so).  Inputs are
-resident in HBM before the timed region: every timed step filters its own
-pristine copy of the 8 pre-filter frames (the chain works in place).
+-> film grain into a display frame, via the C ABI (libav1pf.so), one
+av1pf_apply_batch call per step.  Inputs are resident in HBM before the timed
+region: every timed step filters its own pristine copy of the 8 pre-filter
+frames (the chain works in place).  The same steps run once more with one
+av1pf_apply per frame (per_frame_apply_mpx_s).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
 Prints one JSON line.  `value` is luma pixels per second (Mpx/s);
 `msamples_per_s` counts luma + chroma samples.  The roofline object is for the
-dominant kernel (most GPU time in the timed region), timed with HIP events
-around each of its launches on the launch stream.
+dominant kernel (the HBM-streaming stage with the most GPU time in the timed
+region), timed with HIP events around each of its launches on the launch
+stream.  The CPU baseline times the oracle on the host at up to 16 threads
+(cpu_baseline) and at 1 thread (cpu_baseline_1thread).
 """
 import argparse
 import json
@@ -36,23 +40,27 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(frames, seconds):
+def cpu_baseline(frames, seconds, threads):
     """The oracle (plain-C restatement of goav1's Go path) timed on this host on a
-    bounded sample: one thread over whole frames, repeated until `seconds`."""
+    bounded sample: whole frames of the workload, repeated until `seconds`,
+    with `threads` frames in flight (one per thread; the C code releases the
+    GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import orc
-    fr = frames[0]
-    t0 = time.perf_counter()
     n = 0
-    while True:
-        orc.run_chain(fr)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        while True:
+            list(ex.map(orc.run_chain, [frames[(n + i) % len(frames)] for i in range(threads)]))
+            n += threads
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+    fr = frames[0]
     px = n * fr.w * fr.h
-    return {"value": round(px / el / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
-            "sample": f"{n} x {fr.w}x{fr.h} 10-bit 4:2:0 frame(s), full chain + film grain, oracle/liborc.so, "
-                      f"1 thread, {el:.1f} s"}
+    return {"value": round(px / el / 1e6, 3), "unit": "Mpx/s", "cores": threads, "kind": "port",
+            "sample": f"{n} x {fr.w}x{fr.h} 10-bit 4:2:0 frames of the workload, full chain + film grain, "
+                      f"oracle/liborc.so, {threads} thread(s), {el:.1f} s"}
 
 
 def traffic_for(kernel):
@@ -83,48 +91,60 @@ def main():
     base = [DeviceFrame(fr) for fr in frames]
     displays = [DeviceFrame.empty_like(fr) for fr in frames]
     sides = [DeviceSideData(fr, display=d) for fr, d in zip(frames, displays)]
-    total = args.warmup + args.steps
-    copies = []
-    for s in range(total):
-        row = []
-        for b in base:
-            c = DeviceFrame.__new__(DeviceFrame)
-            c.meta, c.bd = b.meta, b.bd
-            c.planes = [t.clone() for t in b.planes]
-            c.abi = c._abi()
-            row.append(c)
-        copies.append(row)
-    torch.cuda.synchronize()
 
-    for s in range(args.warmup):
-        for f, sd in zip(copies[s], sides):
-            runner.apply(f, sd)
-    runner.sync()
-    torch.cuda.synchronize()
+    def pristine(n):
+        rows = []
+        for _ in range(n):
+            row = []
+            for b in base:
+                c = DeviceFrame.__new__(DeviceFrame)
+                c.meta, c.bd = b.meta, b.bd
+                c.planes = [t.clone() for t in b.planes]
+                c.abi = c._abi()
+                row.append(c)
+            rows.append(row)
+        torch.cuda.synchronize()
+        return rows
 
-    runner.profile(True)
     stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+
+    def timed(rows, batched, profile):
+        """GPU ms over `rows` steps (each step = all frames of the sequence)."""
+        if profile:
+            runner.profile(True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for row in rows:
+            if batched:
+                runner.apply_batch(row, sides)
+            else:
+                for f, sd in zip(row, sides):
+                    runner.apply(f, sd)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        runner.sync()
+        return ev0.elapsed_time(ev1)
+
+    # warm-up, then the timed steps: one apply_batch per step covers the 8 frames
+    timed(pristine(args.warmup), True, False)
+    rows = pristine(args.steps)
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for s in range(args.warmup, total):
-        for f, sd in zip(copies[s], sides):
-            runner.apply(f, sd)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    gpu_ms = timed(rows, True, True)
     wall = time.perf_counter() - t0
-    runner.sync()
-    gpu_ms = ev0.elapsed_time(ev1)
     prof = runner.profile_read()
     runner.profile(False)
+    del rows
+    # the same frames one apply per frame (decoder order, no batching), for reference
+    frame_ms = timed(pristine(args.steps), False, False)
 
     ms_per_step = gpu_ms / args.steps
     luma_px = sum(fr.w * fr.h for fr in frames)
     samples = sum(fr.samples for fr in frames)
     value = luma_px * args.steps / (gpu_ms / 1e3) / 1e6
-    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    # dominant kernel: the HBM-streaming stage with the most GPU time
+    dom = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][0])
     name, (ms, launches, nbytes) = dom
     achieved = (nbytes / launches) / ((ms / launches) / 1e3) / 1e9
     stages = {k: {"ms_total": round(v[0], 3), "launches": v[1],
@@ -149,6 +169,7 @@ def main():
                    "frames_per_step": args.frames, "width": 1920, "height": 1080, "bit_depth": 10,
                    "chroma": "4:2:0"},
         "msamples_per_s": round(samples * args.steps / (gpu_ms / 1e3) / 1e6, 2),
+        "per_frame_apply_mpx_s": round(luma_px * args.steps / (frame_ms / 1e3) / 1e6, 2),
         "wall_s_timed": round(wall, 4),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(name),
@@ -157,8 +178,11 @@ def main():
         "stages": stages,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds)
+        cores = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds / 2, cores)
         out["cpu_baseline"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        one = cpu_baseline(frames, args.cpu_seconds / 2, 1)
+        out["cpu_baseline_1thread"] = one
     runner.close()
     print(json.dumps(out), flush=True)
 

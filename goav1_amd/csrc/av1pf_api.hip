@@ -8,7 +8,16 @@
 //   launched (postfilter.go:598-616); the only check that needs device data,
 //   a CDEF strength index past strength_count (postfilter_cdef.go:942-944), is
 //   flagged by the kernel and reported by av1pf_sync.
+//
+// Batching: a call covers n frames.  The host writes one FrameDesc per frame
+// and one segment table per stage into a pinned staging slot, copies it to the
+// device in one transfer, and launches each stage once for all frames.  Slots
+// form a ring guarded by events, so back-to-back calls never overwrite
+// descriptors a previous call's kernels still read.  Film-grain templates
+// depend only on the grain parameters and run on a side stream that joins the
+// main stream before the grain apply.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,16 +26,31 @@
 
 using namespace pf;
 
+namespace {
+constexpr int kSlots = 4;
+}
+
 struct av1pf_ctx {
     int device = 0;
     std::string err;
-    char *scratch = nullptr;      // second frame buffer (CDEF output / LR ping-pong)
+    char *scratch = nullptr;      // frame-sized second buffers (CDEF output / LR ping-pong)
     size_t scratch_cap = 0;
     uint16_t *lr_bnd = nullptr;   // LR stripe context rows
     size_t lr_bnd_cap = 0;
-    int16_t *templates = nullptr; // film-grain templates (3 x 73 x 82)
+    int16_t *templates = nullptr; // film-grain templates (3 x 73 x 82 per frame)
+    size_t templates_cap = 0;
     int *dev_err = nullptr;       // device-side validation flag
-    uint64_t *search_tmp = nullptr;
+    hipStream_t side = nullptr;   // film-grain template stream
+    hipEvent_t fork = nullptr, join = nullptr;
+    // descriptor ring
+    char *meta_dev[kSlots] = {};
+    char *meta_host[kSlots] = {};
+    size_t meta_cap[kSlots] = {};
+    hipEvent_t meta_done[kSlots] = {};
+    bool meta_used[kSlots] = {};
+    int slot = 0;
+    // stage-level LR: boundaries saved by av1pf_lr_save_boundaries for this frame
+    const void *bnd_frame = nullptr;
     // per-stage timing
     bool profiling = false;
     struct Rec { int stage; hipEvent_t a, b; double bytes; };
@@ -38,7 +62,23 @@ struct av1pf_ctx {
 };
 
 namespace {
+
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH };
+
+int fail(av1pf_ctx *c, int code, const char *msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                       \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);     \
+            return AV1PF_ERR_DEVICE;                                            \
+        }                                                                       \
+    } while (0)
 
 hipEvent_t take_event(av1pf_ctx *c)
 {
@@ -76,34 +116,16 @@ struct StageTimer {
     }
 };
 
-double frame_bytes(const pf::FrameView &f)
-{
-    double n = 0;
-    for (int i = 0; i < f.nplanes; i++) n += (double)f.p[i].w * f.p[i].h;
-    return n * (f.bd == 8 ? 1 : 2);
-}
-}  // namespace
-
-namespace {
-
-int fail(av1pf_ctx *c, int code, const char *msg)
-{
-    if (c) c->err = msg;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                       \
-    do {                                                                        \
-        hipError_t e_ = (expr);                                                 \
-        if (e_ != hipSuccess) {                                                 \
-            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);     \
-            return AV1PF_ERR_DEVICE;                                            \
-        }                                                                       \
-    } while (0)
-
 __global__ void zero_int_kernel(int *p) { *p = 0; }
 
 int bps_of(int bd) { return bd == 8 ? 1 : 2; }
+
+double view_bytes(const FrameView &f)
+{
+    double n = 0;
+    for (int i = 0; i < f.nplanes; i++) n += (double)f.p[i].w * f.p[i].h;
+    return n * bps_of(f.bd);
+}
 
 int validate_frame(av1pf_ctx *c, const av1pf_frame *f)
 {
@@ -147,29 +169,15 @@ FrameView view_of(const av1pf_frame *f)
     return v;
 }
 
-int ensure_scratch(av1pf_ctx *c, const FrameView &f, FrameView *out)
+size_t scratch_bytes(const FrameView &f, int64_t strides[3])
 {
     const int bps = bps_of(f.bd);
     size_t need = 0;
-    int64_t strides[3];
     for (int i = 0; i < f.nplanes; i++) {
         strides[i] = ((int64_t)f.p[i].aw * bps + 255) & ~(int64_t)255;
         need += (size_t)strides[i] * f.p[i].ah;
     }
-    if (need > c->scratch_cap) {
-        if (c->scratch) HIPCHK(c, hipFree(c->scratch));
-        c->scratch = nullptr;
-        HIPCHK(c, hipMalloc(&c->scratch, need));
-        c->scratch_cap = need;
-    }
-    *out = f;
-    size_t off = 0;
-    for (int i = 0; i < f.nplanes; i++) {
-        out->p[i].base = c->scratch + off;
-        out->p[i].stride = strides[i];
-        off += (size_t)strides[i] * f.p[i].ah;
-    }
-    return AV1PF_OK;
+    return (need + 255) & ~(size_t)255;
 }
 
 int validate_deblock(av1pf_ctx *c, const av1pf_frame *f, const av1pf_deblock_params *p)
@@ -224,6 +232,11 @@ int validate_lr(av1pf_ctx *c, const av1pf_frame *f, const av1pf_lr_params *p)
     return AV1PF_OK;
 }
 
+bool grain_noop(const av1pf_grain_params *g)
+{
+    return !g->chroma_scaling_from_luma && !g->num_y_points && !g->num_cb_points && !g->num_cr_points;
+}
+
 int validate_grain(av1pf_ctx *c, const av1pf_frame *f, const av1pf_grain_params *g, const av1pf_frame *d)
 {
     if (!g) return fail(c, AV1PF_ERR_INVALID_ARG, "film grain params missing");
@@ -236,10 +249,10 @@ int validate_grain(av1pf_ctx *c, const av1pf_frame *f, const av1pf_grain_params 
     if (g->num_y_points > 14 || g->num_cb_points > 10 || g->num_cr_points > 10 || g->ar_coeff_lag > 3 ||
         g->grain_scale_shift > 3)
         return fail(c, AV1PF_ERR_INVALID_ARG, "film grain params out of range");
-    const bool noop = !g->chroma_scaling_from_luma && !g->num_y_points && !g->num_cb_points && !g->num_cr_points;
-    if (noop) return AV1PF_OK;
+    if (grain_noop(g)) return AV1PF_OK;
     if (g->scaling_shift < 8 || g->scaling_shift > 11) return fail(c, AV1PF_ERR_INVALID_ARG, "scaling shift out of range");
-    if ((g->ar_coeff_lag || g->num_y_points) && (g->ar_coeff_shift < 6 || g->ar_coeff_shift > 9))
+    if ((g->ar_coeff_lag || g->num_y_points || g->chroma_scaling_from_luma || g->num_cb_points || g->num_cr_points) &&
+        (g->ar_coeff_shift < 6 || g->ar_coeff_shift > 9))
         return fail(c, AV1PF_ERR_INVALID_ARG, "ar coeff shift out of range");
     if (f->mono && (g->chroma_scaling_from_luma || g->num_cb_points || g->num_cr_points))
         return fail(c, AV1PF_ERR_INVALID_ARG, "chroma grain on a monochrome frame");
@@ -255,7 +268,7 @@ int validate_grain(av1pf_ctx *c, const av1pf_frame *f, const av1pf_grain_params 
     return AV1PF_OK;
 }
 
-// AV1 scaling LUT (filmgrain/scaling.go:18-51) -- tiny host work, LUTs travel as kernel args.
+// AV1 scaling LUT (filmgrain/scaling.go:18-51) -- tiny host work.
 void build_scaling_lut(uint8_t lut[256], const uint8_t (*pts)[2], int n)
 {
     memset(lut, 0, 256);
@@ -284,90 +297,6 @@ void lfsr_jump(uint16_t J[16], int steps)
     }
 }
 
-int run_grain(av1pf_ctx *c, const av1pf_frame *frame, const FrameView &A, const av1pf_frame *display,
-              const av1pf_grain_params *g, hipStream_t s)
-{
-    FrameView D = view_of(display);
-    const bool noop = !g->chroma_scaling_from_luma && !g->num_y_points && !g->num_cb_points && !g->num_cr_points;
-    if (noop) {
-        HIPCHK(c, launch_copy(A, D, s));
-        return AV1PF_OK;
-    }
-    GrainArgs ga{};
-    ga.g = *g;
-    ga.active[0] = g->num_y_points != 0;
-    ga.active[1] = !frame->mono && (g->num_cb_points != 0 || g->chroma_scaling_from_luma);
-    ga.active[2] = !frame->mono && (g->num_cr_points != 0 || g->chroma_scaling_from_luma);
-    build_scaling_lut(ga.luts[0], (const uint8_t(*)[2])g->y_points, g->num_y_points);
-    if (g->chroma_scaling_from_luma) {
-        memcpy(ga.luts[1], ga.luts[0], 256);
-        memcpy(ga.luts[2], ga.luts[0], 256);
-    } else {
-        build_scaling_lut(ga.luts[1], (const uint8_t(*)[2])g->cb_points, g->num_cb_points);
-        build_scaling_lut(ga.luts[2], (const uint8_t(*)[2])g->cr_points, g->num_cr_points);
-    }
-    lfsr_jump(ga.jump_luma, 82);
-    lfsr_jump(ga.jump_chroma, frame->ss_x ? 44 : 82);
-    if (!c->templates) HIPCHK(c, hipMalloc(&c->templates, 3 * 73 * 82 * sizeof(int16_t)));
-    ga.templates = c->templates;
-    // copy the frame into the display buffer first (bindFilmGrainPostFilterOutput), then grain
-    { StageTimer t(c, ST_COPY, s, 2 * frame_bytes(A)); HIPCHK(c, launch_copy(A, D, s)); }
-    { StageTimer t(c, ST_GRAIN_T, s, 0); HIPCHK(c, launch_grain_templates(ga, frame->bit_depth, frame->ss_x, frame->ss_y, s)); }
-    { StageTimer t(c, ST_GRAIN_A, s, 2 * frame_bytes(A)); HIPCHK(c, launch_grain_apply(A, D, ga, frame->matrix_identity, s)); }
-    return AV1PF_OK;
-}
-
-int lr_args(av1pf_ctx *c, const av1pf_frame *f, const av1pf_lr_params *p, LrArgs *out, bool *any)
-{
-    *out = LrArgs{};
-    *any = false;
-    const int np = f->mono ? 1 : 3;
-    size_t need = 0;
-    int stripes[3] = { 0, 0, 0 }, pws[3] = { 0, 0, 0 };
-    for (int i = 0; i < np; i++) {
-        LrPlaneArgs &a = out->p[i];
-        a.type = p->frame_type[i];
-        if (a.type == AV1PF_LR_NONE) continue;
-        int pw, ph, hz, vt;
-        lr_geometry(f, p, i, &pw, &ph, &hz, &vt);
-        const int sy = i ? f->ss_y : 0;
-        const int sh = 64 >> sy, off = 8 >> sy;
-        a.unit_size = p->unit_size[i];
-        a.horz = hz;
-        a.vert = vt;
-        a.units = p->units[i];
-        stripes[i] = (ph + off + sh - 1) / sh;
-        pws[i] = pw;
-        a.stripes = stripes[i];
-        need += (size_t)stripes[i] * 4 * pw;
-        *any = true;
-    }
-    if (!*any) return AV1PF_OK;
-    if (need > c->lr_bnd_cap) {
-        if (c->lr_bnd) HIPCHK(c, hipFree(c->lr_bnd));
-        c->lr_bnd = nullptr;
-        HIPCHK(c, hipMalloc(&c->lr_bnd, need * sizeof(uint16_t)));
-        c->lr_bnd_cap = need;
-    }
-    size_t off = 0;
-    for (int i = 0; i < np; i++) {
-        if (out->p[i].type == AV1PF_LR_NONE) continue;
-        out->p[i].bnd = c->lr_bnd + off;
-        off += (size_t)stripes[i] * 4 * pws[i];
-    }
-    return AV1PF_OK;
-}
-
-bool cdef_has_filtering(const av1pf_cdef_params *p, bool mono, int *yf, int *uvf)
-{
-    *yf = *uvf = 0;
-    for (int i = 0; i < p->strength_count && i < 8; i++) {
-        if (p->y_strength[i]) *yf = 1;
-        if (!mono && p->uv_strength[i]) *uvf = 1;
-    }
-    return *yf || *uvf;
-}
-
 int validate_request(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req)
 {
     if (!req) return fail(c, AV1PF_ERR_INVALID_ARG, "null request");
@@ -384,69 +313,283 @@ int validate_request(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request
     return AV1PF_OK;
 }
 
-int run_chain(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req, hipStream_t s)
+struct SegList {
+    std::vector<Seg> v;
+    int total = 0;
+    void add(int frame, int plane, int nx, int ny)
+    {
+        if (nx <= 0 || ny <= 0) return;
+        v.push_back({ frame, plane, nx, total });
+        total += nx * ny;
+    }
+};
+
+struct BatchPlan {
+    std::vector<FrameDesc> descs;
+    SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain;
+    double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0;
+    size_t scratch = 0, bnd = 0;
+    int ngrain = 0;
+};
+
+// Fill descriptors and segment tables; pointers into the arenas are patched
+// once the arena bases are known (offsets kept in the descriptors meanwhile).
+int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, BatchPlan &P,
+               std::vector<size_t> &scr_off, std::vector<size_t> &bnd_off, std::vector<int> &grain_idx,
+               bool skip_lr_save)
 {
-    const uint32_t st = req->stages;
-    FrameView A = view_of(frame);
-    FrameView B;
-    int rc = ensure_scratch(c, A, &B);
+    P.descs.assign(n, FrameDesc{});
+    scr_off.assign(n, 0);
+    bnd_off.assign(n * 3, 0);
+    grain_idx.assign(n, -1);
+    for (int i = 0; i < n; i++) {
+        const av1pf_frame *f = &frames[i];
+        const av1pf_request *r = &reqs[i];
+        FrameDesc &D = P.descs[i];
+        D.v[VIEW_A] = view_of(f);
+        const FrameView &A = D.v[VIEW_A];
+        int64_t strides[3];
+        scr_off[i] = P.scratch;
+        P.scratch += scratch_bytes(A, strides);
+        D.v[VIEW_B] = A;
+        for (int p = 0; p < A.nplanes; p++) D.v[VIEW_B].p[p].stride = strides[p];   // base patched later
+        D.v[VIEW_D] = r->display ? view_of(r->display) : A;
+        D.err = c->dev_err;
+        const uint32_t st = r->stages;
+        int cur = VIEW_A;
+        // ---- deblock
+        if (st & AV1PF_STAGE_DEBLOCK) {
+            const av1pf_deblock_params *p = r->deblock;
+            DbDesc &db = D.db;
+            db.lc = p->level_cache;
+            db.masks = p->masks;
+            db.mi_cols = p->mi_cols;
+            db.sbw = p->sb128_w;
+            for (int l = 0; l < 64; l++) {
+                int shift = (p->sharpness > 0) + (p->sharpness > 4);
+                int lim = l >> shift;
+                if (p->sharpness > 0 && lim > 9 - p->sharpness) lim = 9 - p->sharpness;
+                if (lim < 1) lim = 1;
+                db.I[l] = (uint8_t)lim;
+                db.E[l] = (uint8_t)(2 * (l + 2) + lim);
+            }
+            const bool luma_zero = p->level_y[0] == 0 && p->level_y[1] == 0;
+            for (int pl = 0; pl < A.nplanes; pl++) {
+                if (pl && (luma_zero || (pl == 1 ? p->level_u : p->level_v) == 0)) continue;
+                const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
+                const int w4 = ((((f->width + sx) >> sx) + 3) & ~3) >> 2;
+                const int h4 = ((((f->height + sy) >> sy) + 3) & ~3) >> 2;
+                P.dbv.add(i, pl, (w4 + 63) / 64, h4);
+                P.dbh.add(i, pl, (w4 * 4 + 63) / 64, (h4 + 3) / 4);
+            }
+            P.bytes_db += 2 * view_bytes(A);
+        }
+        // ---- LR boundary rows (from the deblocked frame)
+        bool lr_any = false;
+        if (st & AV1PF_STAGE_LR) {
+            const av1pf_lr_params *p = r->lr;
+            for (int pl = 0; pl < A.nplanes; pl++) {
+                LrPlaneDesc &g = D.lr[pl];
+                const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
+                g.type = p->frame_type[pl];
+                g.pw = A.p[pl].w;
+                g.ph = A.p[pl].h;
+                g.procw = 64 >> sx;
+                g.sh = 64 >> sy;
+                g.off = 8 >> sy;
+                g.xtiles = (g.pw + g.procw - 1) / g.procw;
+                g.stripes = lr_stripes(g.ph, g.sh, g.off);
+                if (g.type == AV1PF_LR_NONE) continue;
+                int pw, ph;
+                lr_geometry(f, p, pl, &pw, &ph, &g.hz, &g.vt);
+                g.us = p->unit_size[pl];
+                g.units = p->units[pl];
+                bnd_off[i * 3 + pl] = P.bnd;
+                P.bnd += (size_t)g.stripes * 4 * g.pw;
+                lr_any = true;
+            }
+            if (lr_any && !skip_lr_save)
+                for (int pl = 0; pl < A.nplanes; pl++)
+                    if (D.lr[pl].type != AV1PF_LR_NONE) P.save.add(i, pl, (D.lr[pl].pw + 255) / 256, D.lr[pl].stripes);
+        }
+        // ---- CDEF
+        if (st & AV1PF_STAGE_CDEF) {
+            const av1pf_cdef_params *p = r->cdef;
+            CdefDesc &cd = D.cd;
+            for (int k = 0; k < p->strength_count && k < 8; k++) {
+                if (p->y_strength[k]) cd.yfilt = 1;
+                if (!f->mono && p->uv_strength[k]) cd.uvfilt = 1;
+            }
+            if (cd.yfilt || cd.uvfilt) {
+                cd.index = p->index;
+                cd.noskip = p->noskip;
+                cd.unit_cols = p->unit_cols;
+                cd.unit_rows = p->unit_rows;
+                cd.damping = p->damping;
+                cd.strength_count = p->strength_count;
+                memcpy(cd.y_strength, p->y_strength, 8);
+                memcpy(cd.uv_strength, p->uv_strength, 8);
+                P.cdef.add(i, -1, p->unit_cols, p->unit_rows);
+                P.bytes_cdef += 2 * view_bytes(A);
+                cur = VIEW_B;
+            }
+        }
+        // ---- LR
+        if (lr_any) {
+            D.lr_src = cur;
+            D.lr_dst = cur == VIEW_A ? VIEW_B : VIEW_A;
+            for (int pl = 0; pl < A.nplanes; pl++) P.lr.add(i, pl, D.lr[pl].xtiles, D.lr[pl].stripes);
+            P.bytes_lr += 2 * view_bytes(A);
+            cur = D.lr_dst;
+        }
+        if (cur == VIEW_B) {
+            for (int pl = 0; pl < A.nplanes; pl++) P.copy_ba.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+            P.bytes_copy += 2 * view_bytes(A);
+        }
+        // ---- film grain
+        if (st & AV1PF_STAGE_FILMGRAIN) {
+            const av1pf_grain_params *g = r->grain;
+            for (int pl = 0; pl < A.nplanes; pl++) P.copy_ad.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+            P.bytes_copy += 2 * view_bytes(A);
+            if (!grain_noop(g)) {
+                GrainDesc &G = D.gr;
+                G.g = *g;
+                G.active[0] = g->num_y_points != 0;
+                G.active[1] = !f->mono && (g->num_cb_points != 0 || g->chroma_scaling_from_luma);
+                G.active[2] = !f->mono && (g->num_cr_points != 0 || g->chroma_scaling_from_luma);
+                build_scaling_lut(G.luts[0], (const uint8_t(*)[2])g->y_points, g->num_y_points);
+                if (g->chroma_scaling_from_luma) {
+                    memcpy(G.luts[1], G.luts[0], 256);
+                    memcpy(G.luts[2], G.luts[0], 256);
+                } else {
+                    build_scaling_lut(G.luts[1], (const uint8_t(*)[2])g->cb_points, g->num_cb_points);
+                    build_scaling_lut(G.luts[2], (const uint8_t(*)[2])g->cr_points, g->num_cr_points);
+                }
+                lfsr_jump(G.jump_luma, 82);
+                lfsr_jump(G.jump_chroma, f->ss_x ? 44 : 82);
+                G.matrix_identity = f->matrix_identity;
+                grain_idx[i] = P.ngrain++;
+                P.tl.add(i, 0, 1, 1);
+                if (!f->mono) {
+                    P.tc.add(i, 1, 1, 1);
+                    P.tc.add(i, 2, 1, 1);
+                }
+                P.grain.add(i, -1, (f->width + 31) / 32, (f->height + 31) / 32);
+                P.bytes_grain += 2 * view_bytes(A);
+            }
+        }
+    }
+    return AV1PF_OK;
+}
+
+int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
+{
+    if (need <= *cap) return AV1PF_OK;
+    HIPCHK(c, hipDeviceSynchronize());   // previous work may still use the old arena
+    if (*buf) HIPCHK(c, hipFree(*buf));
+    *buf = nullptr;
+    need = need + need / 4;
+    HIPCHK(c, hipMalloc(buf, need));
+    *cap = need;
+    return AV1PF_OK;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, hipStream_t s,
+              bool skip_lr_save = false)
+{
+    if (n == 0) return AV1PF_OK;
+    BatchPlan P;
+    std::vector<size_t> scr_off, bnd_off;
+    std::vector<int> grain_idx;
+    int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, skip_lr_save);
     if (rc) return rc;
-    const FrameView *cur = &A;
-    if (st & AV1PF_STAGE_DEBLOCK) {
-        const av1pf_deblock_params *p = req->deblock;
-        DeblockArgs a{};
-        a.level_cache = p->level_cache;
-        a.masks = p->masks;
-        a.mi_cols = p->mi_cols;
-        a.sb128_w = p->sb128_w;
-        for (int l = 0; l < 64; l++) {
-            int shift = 0;
-            if (p->sharpness > 0) shift++;
-            if (p->sharpness > 4) shift++;
-            int lim = l >> shift;
-            if (p->sharpness > 0 && lim > 9 - p->sharpness) lim = 9 - p->sharpness;
-            if (lim < 1) lim = 1;
-            a.I[l] = (uint8_t)lim;
-            a.E[l] = (uint8_t)(2 * (l + 2) + lim);
+    if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
+    if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
+    const size_t tsz = (size_t)3 * 73 * 82 * sizeof(int16_t);
+    if ((rc = ensure(c, (void **)&c->templates, &c->templates_cap, std::max<size_t>(P.ngrain * tsz, 256)))) return rc;
+    for (int i = 0; i < n; i++) {
+        FrameDesc &D = P.descs[i];
+        char *base = c->scratch + scr_off[i];
+        for (int p = 0; p < D.v[VIEW_B].nplanes; p++) {
+            D.v[VIEW_B].p[p].base = base;
+            base += (size_t)D.v[VIEW_B].p[p].stride * D.v[VIEW_B].p[p].ah;
         }
-        const bool luma_zero = p->level_y[0] == 0 && p->level_y[1] == 0;
-        a.plane_mask = 1;
-        if (!frame->mono && !luma_zero) {
-            if (p->level_u) a.plane_mask |= 2;
-            if (p->level_v) a.plane_mask |= 4;
+        for (int p = 0; p < 3; p++)
+            if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = c->lr_bnd + bnd_off[i * 3 + p];
+        if (grain_idx[i] >= 0) D.gr.templates = c->templates + (size_t)grain_idx[i] * 3 * 73 * 82;
+    }
+    // ---- pack descriptors + segment tables into one slot
+    SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain };
+    const int nl = sizeof(lists) / sizeof(lists[0]);
+    size_t off = align256(sizeof(FrameDesc) * n);
+    size_t offs[nl];
+    for (int k = 0; k < nl; k++) {
+        offs[k] = off;
+        off = align256(off + sizeof(Seg) * (lists[k]->v.size() + 1));
+    }
+    const int slot = c->slot;
+    c->slot = (c->slot + 1) % kSlots;
+    if (c->meta_used[slot]) HIPCHK(c, hipEventSynchronize(c->meta_done[slot]));
+    if (off > c->meta_cap[slot]) {
+        if (c->meta_dev[slot]) HIPCHK(c, hipFree(c->meta_dev[slot]));
+        if (c->meta_host[slot]) HIPCHK(c, hipHostFree(c->meta_host[slot]));
+        const size_t cap = off * 2;
+        HIPCHK(c, hipMalloc(&c->meta_dev[slot], cap));
+        HIPCHK(c, hipHostMalloc(&c->meta_host[slot], cap, hipHostMallocDefault));
+        c->meta_cap[slot] = cap;
+    }
+    char *h = c->meta_host[slot];
+    memcpy(h, P.descs.data(), sizeof(FrameDesc) * n);
+    for (int k = 0; k < nl; k++) {
+        Seg sentinel{ 0, 0, 1, lists[k]->total };
+        memcpy(h + offs[k], lists[k]->v.data(), sizeof(Seg) * lists[k]->v.size());
+        memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
+    }
+    char *d = c->meta_dev[slot];
+    HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
+    const FrameDesc *dd = reinterpret_cast<const FrameDesc *>(d);
+    auto segp = [&](int k) { return reinterpret_cast<const Seg *>(d + offs[k]); };
+    auto nseg = [&](int k) { return (int)lists[k]->v.size(); };
+    // ---- film-grain templates on the side stream
+    if (P.tl.total) {
+        HIPCHK(c, hipEventRecord(c->fork, s));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->fork, 0));
+        {
+            StageTimer t(c, ST_GRAIN_T, c->side, 0);
+            HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
         }
-        { StageTimer t(c, ST_DEBLOCK, s, 2 * frame_bytes(A)); HIPCHK(c, launch_deblock(A, a, s)); }
+        HIPCHK(c, hipEventRecord(c->join, c->side));
     }
-    LrArgs la{};
-    bool lr_any = false;
-    if (st & AV1PF_STAGE_LR) {
-        if ((rc = lr_args(c, frame, req->lr, &la, &lr_any))) return rc;
-        if (lr_any) { StageTimer t(c, ST_LR_SAVE, s, 0); HIPCHK(c, launch_lr_save(A, la, s)); }
+    if (P.dbv.total || P.dbh.total) {
+        StageTimer t(c, ST_DEBLOCK, s, P.bytes_db);
+        HIPCHK(c, launch_deblock(dd, segp(0), nseg(0), P.dbv.total, segp(1), nseg(1), P.dbh.total, s));
     }
-    if (st & AV1PF_STAGE_CDEF) {
-        const av1pf_cdef_params *p = req->cdef;
-        CdefArgs a{};
-        if (cdef_has_filtering(p, frame->mono, &a.yfilt, &a.uvfilt)) {
-            a.index = p->index;
-            a.noskip = p->noskip;
-            a.unit_cols = p->unit_cols;
-            a.unit_rows = p->unit_rows;
-            a.damping = p->damping;
-            a.strength_count = p->strength_count;
-            memcpy(a.y_strength, p->y_strength, 8);
-            memcpy(a.uv_strength, p->uv_strength, 8);
-            { StageTimer t(c, ST_CDEF, s, 2 * frame_bytes(A)); HIPCHK(c, launch_cdef(A, B, a, s, c->dev_err)); }
-            cur = &B;
-        }
+    if (P.save.total) {
+        StageTimer t(c, ST_LR_SAVE, s, 0);
+        HIPCHK(c, launch_lr_save(dd, segp(2), nseg(2), P.save.total, s));
     }
-    if (lr_any) {
-        const FrameView *other = cur == &A ? &B : &A;
-        { StageTimer t(c, ST_LR, s, 2 * frame_bytes(A)); HIPCHK(c, launch_lr(*cur, *other, la, s)); }
-        cur = other;
+    if (P.cdef.total) {
+        StageTimer t(c, ST_CDEF, s, P.bytes_cdef);
+        HIPCHK(c, launch_cdef(dd, segp(3), nseg(3), P.cdef.total, s));
     }
-    if (cur != &A) { StageTimer t(c, ST_COPY, s, 2 * frame_bytes(A)); HIPCHK(c, launch_copy(*cur, A, s)); }
-    if (st & AV1PF_STAGE_FILMGRAIN)
-        if ((rc = run_grain(c, frame, A, req->display, req->grain, s))) return rc;
+    if (P.lr.total) {
+        StageTimer t(c, ST_LR, s, P.bytes_lr);
+        HIPCHK(c, launch_lr(dd, segp(4), nseg(4), P.lr.total, s));
+    }
+    if (P.copy_ba.total || P.copy_ad.total) {
+        StageTimer t(c, ST_COPY, s, P.bytes_copy);
+        HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, s));
+        HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, VIEW_A, VIEW_D, s));
+    }
+    if (P.grain.total) {
+        HIPCHK(c, hipStreamWaitEvent(s, c->join, 0));
+        StageTimer t(c, ST_GRAIN_A, s, P.bytes_grain);
+        HIPCHK(c, launch_grain_apply(dd, segp(9), nseg(9), P.grain.total, s));
+    }
+    HIPCHK(c, hipEventRecord(c->meta_done[slot], s));
+    c->meta_used[slot] = true;
     return AV1PF_OK;
 }
 
@@ -454,7 +597,7 @@ int run_chain(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req, 
 
 extern "C" {
 
-const char *av1pf_version(void) { return "av1pf 0.1 gfx950"; }
+const char *av1pf_version(void) { return "av1pf 0.2 gfx950"; }
 
 const char *av1pf_last_error(const av1pf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
@@ -471,14 +614,18 @@ int av1pf_create(av1pf_ctx **out, int device)
     if (hipSetDevice(device) != hipSuccess) return AV1PF_ERR_DEVICE;
     av1pf_ctx *c = new av1pf_ctx();
     c->device = device;
-    if (hipMalloc(&c->dev_err, sizeof(int)) != hipSuccess) {
-        delete c;
-        return AV1PF_ERR_DEVICE;
+    bool ok = hipMalloc(&c->dev_err, sizeof(int)) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kSlots; i++)
+        ok = hipEventCreateWithFlags(&c->meta_done[i], hipEventDisableTiming) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, nullptr, c->dev_err);
+        ok = hipDeviceSynchronize() == hipSuccess;
     }
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, nullptr, c->dev_err);
-    if (hipDeviceSynchronize() != hipSuccess) {
-        hipFree(c->dev_err);
-        delete c;
+    if (!ok) {
+        av1pf_destroy(c);
         return AV1PF_ERR_DEVICE;
     }
     *out = c;
@@ -494,7 +641,14 @@ void av1pf_destroy(av1pf_ctx *c)
     if (c->lr_bnd) hipFree(c->lr_bnd);
     if (c->templates) hipFree(c->templates);
     if (c->dev_err) hipFree(c->dev_err);
-    if (c->search_tmp) hipFree(c->search_tmp);
+    for (int i = 0; i < kSlots; i++) {
+        if (c->meta_dev[i]) hipFree(c->meta_dev[i]);
+        if (c->meta_host[i]) hipHostFree(c->meta_host[i]);
+        if (c->meta_done[i]) hipEventDestroy(c->meta_done[i]);
+    }
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->join) hipEventDestroy(c->join);
+    if (c->side) hipStreamDestroy(c->side);
     for (auto &r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->pool) hipEventDestroy(e);
     delete c;
@@ -506,7 +660,8 @@ int av1pf_apply(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req
     int rc = validate_request(c, frame, req);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    return run_chain(c, frame, req, static_cast<hipStream_t>(stream));
+    if (req->stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;   // the chain's save overwrites saved rows
+    return run_batch(c, 1, frame, req, static_cast<hipStream_t>(stream));
 }
 
 int av1pf_apply_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, void *stream)
@@ -517,11 +672,9 @@ int av1pf_apply_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1p
         if (rc) return rc;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    for (int i = 0; i < n; i++) {
-        int rc = run_chain(c, &frames[i], &reqs[i], static_cast<hipStream_t>(stream));
-        if (rc) return rc;
-    }
-    return AV1PF_OK;
+    for (int i = 0; i < n; i++)
+        if (reqs[i].stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;
+    return run_batch(c, n, frames, reqs, static_cast<hipStream_t>(stream));
 }
 
 int av1pf_sync(av1pf_ctx *c, void *stream)
@@ -529,6 +682,7 @@ int av1pf_sync(av1pf_ctx *c, void *stream)
     if (!c) return AV1PF_ERR_INVALID_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    HIPCHK(c, hipStreamSynchronize(c->side));
     int flag = 0;
     HIPCHK(c, hipMemcpy(&flag, c->dev_err, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) {
@@ -557,48 +711,79 @@ int av1pf_cdef(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_cdef_params *
 
 int av1pf_lr_save_boundaries(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_lr_params *p, int after_cdef, void *stream)
 {
-    // The chain reads frame-edge context straight from the post-CDEF frame, so
-    // only the deblocked seam rows are stored; the after-CDEF save is a no-op.
+    // Frame-edge context is read straight from the post-CDEF frame by the LR
+    // kernel, so only the deblocked interior seam rows are stored; the
+    // after-CDEF save is a no-op.
     if (!c) return AV1PF_ERR_INVALID_ARG;
     int rc = validate_frame(c, frame);
     if (rc) return rc;
     if ((rc = validate_lr(c, frame, p))) return rc;
     if (after_cdef) return AV1PF_OK;
     HIPCHK(c, hipSetDevice(c->device));
-    LrArgs la;
-    bool any;
-    if ((rc = lr_args(c, frame, p, &la, &any))) return rc;
-    if (any) HIPCHK(c, launch_lr_save(view_of(frame), la, static_cast<hipStream_t>(stream)));
+    BatchPlan P;
+    std::vector<size_t> scr_off, bnd_off;
+    std::vector<int> gi;
+    av1pf_request r{};
+    r.stages = AV1PF_STAGE_LR;
+    r.lr = p;
+    // a request with only the save: run the chain's save segment by planning an
+    // LR request and launching just its save list
+    if ((rc = plan_batch(c, 1, frame, &r, P, scr_off, bnd_off, gi, false))) return rc;
+    if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
+    for (int pl = 0; pl < 3; pl++)
+        if (P.descs[0].lr[pl].type != AV1PF_LR_NONE) P.descs[0].lr[pl].bnd = c->lr_bnd + bnd_off[pl];
+    const int slot = c->slot;
+    c->slot = (c->slot + 1) % kSlots;
+    if (c->meta_used[slot]) HIPCHK(c, hipEventSynchronize(c->meta_done[slot]));
+    const size_t off = align256(sizeof(FrameDesc));
+    const size_t need = off + sizeof(Seg) * (P.save.v.size() + 1);
+    if (need > c->meta_cap[slot]) {
+        if (c->meta_dev[slot]) HIPCHK(c, hipFree(c->meta_dev[slot]));
+        if (c->meta_host[slot]) HIPCHK(c, hipHostFree(c->meta_host[slot]));
+        HIPCHK(c, hipMalloc(&c->meta_dev[slot], need * 2));
+        HIPCHK(c, hipHostMalloc(&c->meta_host[slot], need * 2, hipHostMallocDefault));
+        c->meta_cap[slot] = need * 2;
+    }
+    memcpy(c->meta_host[slot], P.descs.data(), sizeof(FrameDesc));
+    Seg sentinel{ 0, 0, 1, P.save.total };
+    memcpy(c->meta_host[slot] + off, P.save.v.data(), sizeof(Seg) * P.save.v.size());
+    memcpy(c->meta_host[slot] + off + sizeof(Seg) * P.save.v.size(), &sentinel, sizeof(Seg));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(c, hipMemcpyAsync(c->meta_dev[slot], c->meta_host[slot], need, hipMemcpyHostToDevice, s));
+    {
+        StageTimer t(c, ST_LR_SAVE, s, 0);
+        HIPCHK(c, launch_lr_save(reinterpret_cast<const FrameDesc *>(c->meta_dev[slot]),
+                                 reinterpret_cast<const Seg *>(c->meta_dev[slot] + off), (int)P.save.v.size(),
+                                 P.save.total, s));
+    }
+    HIPCHK(c, hipEventRecord(c->meta_done[slot], s));
+    c->meta_used[slot] = true;
+    c->bnd_frame = frame->plane[0];
     return AV1PF_OK;
 }
 
 int av1pf_lr(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_lr_params *p, void *stream)
 {
     if (!c) return AV1PF_ERR_INVALID_ARG;
-    int rc = validate_frame(c, frame);
+    av1pf_request r{};
+    r.stages = AV1PF_STAGE_LR;
+    r.lr = p;
+    int rc = validate_request(c, frame, &r);
     if (rc) return rc;
-    if ((rc = validate_lr(c, frame, p))) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    LrArgs la;
-    bool any;
-    if ((rc = lr_args(c, frame, p, &la, &any))) return rc;
-    if (!any) return AV1PF_OK;
-    FrameView A = view_of(frame), B;
-    if ((rc = ensure_scratch(c, A, &B))) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    HIPCHK(c, launch_lr(A, B, la, s));
-    HIPCHK(c, launch_copy(B, A, s));
-    return AV1PF_OK;
+    // boundaries saved by av1pf_lr_save_boundaries for this frame (before CDEF) are used as is
+    const bool saved = c->bnd_frame == frame->plane[0];
+    c->bnd_frame = nullptr;
+    return run_batch(c, 1, frame, &r, static_cast<hipStream_t>(stream), saved);
 }
 
 int av1pf_filmgrain(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *dst, const av1pf_grain_params *p, void *stream)
 {
-    if (!c) return AV1PF_ERR_INVALID_ARG;
-    int rc = validate_frame(c, src);
-    if (rc) return rc;
-    if ((rc = validate_grain(c, src, p, dst))) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
-    return run_grain(c, src, view_of(src), dst, p, static_cast<hipStream_t>(stream));
+    av1pf_request r{};
+    r.stages = AV1PF_STAGE_FILMGRAIN;
+    r.grain = p;
+    r.display = dst;
+    return av1pf_apply(c, src, &r, stream);
 }
 
 int av1pf_cdef_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon,
@@ -617,9 +802,9 @@ int av1pf_cdef_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *r
     const int mi_cols = ((recon->width + 7) >> 3) << 1, mi_rows = ((recon->height + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
     const FrameView sv = view_of(src);
-    StageTimer t(c, ST_SEARCH, static_cast<hipStream_t>(stream), 2 * frame_bytes(sv));
-    HIPCHK(c, launch_cdef_search(sv, view_of(recon), noskip, damping, mse, nhfb, nvfb,
-                                 static_cast<hipStream_t>(stream)));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_SEARCH, s, 2 * view_bytes(sv));
+    HIPCHK(c, launch_cdef_search(sv, view_of(recon), noskip, damping, mse, nhfb, nvfb, s));
     return AV1PF_OK;
 }
 

@@ -150,15 +150,10 @@ __device__ __forceinline__ int str_pri(uint8_t p) { return p >> 2; }
 __device__ __forceinline__ int str_sec(uint8_t p) { const int s = p & 3; return s == 3 ? 4 : s; }
 
 template <typename T>
-__global__ __launch_bounds__(256) void cdef_kernel(FrameView src, FrameView dst, CdefArgs a, int *err)
+__device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView &dst, const CdefDesc &a, int *err,
+                                          int uc, int ur, uint16_t *tile, uint8_t *bdir, int32_t *bvar, uint8_t *bst)
 {
-    __shared__ uint16_t tile[TROWS * TPITCH];
-    __shared__ uint8_t bdir[64];
-    __shared__ int32_t bvar[64];
-    __shared__ uint8_t bst[64];    // per-block primary strength (after luma adjust), 0xff = not listed
-
     const int tid = threadIdx.x;
-    const int uc = blockIdx.x, ur = blockIdx.y;
     const int ui = ur * a.unit_cols + uc;
     int idx = a.index[ui];
     if (idx >= a.strength_count) {
@@ -284,11 +279,26 @@ __global__ __launch_bounds__(256) void cdef_kernel(FrameView src, FrameView dst,
     }
 }
 
-hipError_t launch_cdef(const FrameView &src, const FrameView &dst, const CdefArgs &a, hipStream_t s, int *err)
+__global__ __launch_bounds__(256) void cdef_kernel(const FrameDesc *__restrict__ descs,
+                                                   const Seg *__restrict__ segs, int nseg)
 {
-    dim3 grid(a.unit_cols, a.unit_rows);
-    if (src.bd == 8) hipLaunchKernelGGL(cdef_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst, a, err);
-    else hipLaunchKernelGGL(cdef_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst, a, err);
+    __shared__ uint16_t tile[TROWS * TPITCH];
+    __shared__ uint8_t bdir[64];
+    __shared__ int32_t bvar[64];
+    __shared__ uint8_t bst[64];    // per-block primary strength (after luma adjust), 0xff = not listed
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const int t = blockIdx.x - sg.base;
+    const int uc = t % sg.nx, ur = t / sg.nx;
+    if (D.v[VIEW_A].bd == 8)
+        cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, bst);
+    else
+        cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, bst);
+}
+
+hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(cdef_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
     return hipGetLastError();
 }
 

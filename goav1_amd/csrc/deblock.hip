@@ -104,19 +104,34 @@ __device__ __forceinline__ void lpf_line(T *p, int64_t st, int bd, int E, int I,
     }
 }
 
-struct DbPlane {
-    char *base;
-    int64_t stride;
+// Per-plane constants of one deblock segment.
+struct DbPlaneK {
     int w4, h4;       // MI(4)-aligned position extent, 4x4 units
     int rcw, rch;     // region size in 4x4 units (32 >> ss)
-    int lane;         // mask half split (16 >> ss along the bit axis)
+    int lane_v, lane_h;
     int comp_v, comp_h;
-    int luma;
+    bool luma;
 };
-struct DbPlanes { DbPlane p[3]; int n; };
 
-// Width (4/6/8/16) of the unit whose bit is `bit` in the three class lanes, 0 = no edge.
-__device__ __forceinline__ int unit_wd(uint32_t m0, uint32_t m1, uint32_t m2, int luma)
+__device__ __forceinline__ DbPlaneK db_plane(const FrameView &f, int pl)
+{
+    DbPlaneK k;
+    const int sx = pl ? f.ssx : 0, sy = pl ? f.ssy : 0;
+    const int pw = (f.w + sx) >> sx, ph = (f.h + sy) >> sy;
+    k.w4 = ((pw + 3) & ~3) >> 2;
+    k.h4 = ((ph + 3) & ~3) >> 2;
+    k.rcw = 32 >> sx;
+    k.rch = 32 >> sy;
+    k.luma = pl == 0;
+    k.lane_v = pl ? (16 >> sy) : 16;
+    k.lane_h = pl ? (16 >> sx) : 16;
+    k.comp_v = pl ? pl + 1 : 0;
+    k.comp_h = pl ? pl + 1 : 1;
+    return k;
+}
+
+// Width (4/6/8/16) from the three class bits, 0 = no edge.
+__device__ __forceinline__ int unit_wd(uint32_t m0, uint32_t m1, uint32_t m2, bool luma)
 {
     if (m2) return 16;
     if (m1) return luma ? 8 : 6;
@@ -126,113 +141,86 @@ __device__ __forceinline__ int unit_wd(uint32_t m0, uint32_t m1, uint32_t m2, in
 
 __device__ __forceinline__ uint32_t mask_bit(const uint16_t *halves, int n, int lane)
 {
-    // halves[0] = low lane, halves[1] = high lane
     return n < lane ? (halves[0] >> n) & 1u : (halves[1] >> (n - lane)) & 1u;
 }
 
-// Vertical edges: thread = (pixel row y, 4x4 column x4).
-template <typename T>
-__global__ __launch_bounds__(256) void deblock_v_kernel(DbPlanes planes, const uint8_t *__restrict__ lc,
-                                                        const av1pf_lf_mask *__restrict__ masks,
-                                                        int mi_cols, int sbw, int bd, DeblockArgs tabs)
+// Vertical edges: a 64 x 4 workgroup covers 64 edge columns (x4) by 4 pixel rows.
+__global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restrict__ descs,
+                                                        const Seg *__restrict__ segs, int nseg)
 {
-    const int pi = blockIdx.z;
-    const DbPlane &P = planes.p[pi];
-    const int x4 = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const FrameView &f = D.v[VIEW_A];
+    const int pl = sg.plane;
+    const DbPlaneK P = db_plane(f, pl);
+    const int t = blockIdx.x - sg.base;
+    const int tx = t % sg.nx, ty = t / sg.nx;
+    const int x4 = tx * 64 + threadIdx.x;
+    const int y = ty * 4 + threadIdx.y;
     const int y4 = y >> 2;
     if (x4 == 0 || x4 >= P.w4 || y4 >= P.h4) return;
     const int rr = y4 / P.rch, rc = x4 / P.rcw;
     const int pos = x4 - rc * P.rcw, n = y4 - rr * P.rch;
-    const av1pf_lf_mask &m = masks[rr * sbw + rc];
+    const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
     int wd;
-    if (P.luma) {
-        wd = unit_wd(mask_bit(m.y[0][pos][0], n, 16), mask_bit(m.y[0][pos][1], n, 16),
-                     mask_bit(m.y[0][pos][2], n, 16), 1);
-    } else {
-        wd = unit_wd(mask_bit(m.uv[0][pos][0], n, P.lane), mask_bit(m.uv[0][pos][1], n, P.lane), 0, 0);
-    }
+    if (P.luma)
+        wd = unit_wd(mask_bit(m.y[0][pos][0], n, 16), mask_bit(m.y[0][pos][1], n, 16), mask_bit(m.y[0][pos][2], n, 16), true);
+    else
+        wd = unit_wd(mask_bit(m.uv[0][pos][0], n, P.lane_v), mask_bit(m.uv[0][pos][1], n, P.lane_v), 0, false);
     if (!wd) return;
-    const int li = y4 * mi_cols + x4;
+    const uint8_t *lc = D.db.lc;
+    const int li = y4 * D.db.mi_cols + x4;
     int L = lc[4 * li + P.comp_v];
     if (L == 0) L = lc[4 * (li - 1) + P.comp_v];
     if (L == 0) return;
-    T *row = reinterpret_cast<T *>(P.base + (int64_t)y * P.stride);
-    lpf_line<T>(row + x4 * 4, 1, bd, tabs.E[L], tabs.I[L], L >> 4, wd);
+    const PlaneView &pv = f.p[pl];
+    if (f.bd == 8)
+        lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y) + x4 * 4, 1, 8, D.db.E[L], D.db.I[L], L >> 4, wd);
+    else
+        lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y) + x4 * 4, 1, f.bd, D.db.E[L], D.db.I[L], L >> 4, wd);
 }
 
-// Horizontal edges: thread = (4x4 row y4, pixel column x).
-template <typename T>
-__global__ __launch_bounds__(256) void deblock_h_kernel(DbPlanes planes, const uint8_t *__restrict__ lc,
-                                                        const av1pf_lf_mask *__restrict__ masks,
-                                                        int mi_cols, int sbw, int bd, DeblockArgs tabs)
+// Horizontal edges: a 64 x 4 workgroup covers 64 pixel columns by 4 edge rows (y4).
+__global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restrict__ descs,
+                                                        const Seg *__restrict__ segs, int nseg)
 {
-    const int pi = blockIdx.z;
-    const DbPlane &P = planes.p[pi];
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y4 = blockIdx.y * 4 + threadIdx.y;
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const FrameView &f = D.v[VIEW_A];
+    const int pl = sg.plane;
+    const DbPlaneK P = db_plane(f, pl);
+    const int t = blockIdx.x - sg.base;
+    const int tx = t % sg.nx, ty = t / sg.nx;
+    const int x = tx * 64 + threadIdx.x;
+    const int y4 = ty * 4 + threadIdx.y;
     const int x4 = x >> 2;
     if (y4 == 0 || y4 >= P.h4 || x4 >= P.w4) return;
     const int rr = y4 / P.rch, rc = x4 / P.rcw;
     const int pos = y4 - rr * P.rch, n = x4 - rc * P.rcw;
-    const av1pf_lf_mask &m = masks[rr * sbw + rc];
+    const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
     int wd;
-    if (P.luma) {
-        wd = unit_wd(mask_bit(m.y[1][pos][0], n, 16), mask_bit(m.y[1][pos][1], n, 16),
-                     mask_bit(m.y[1][pos][2], n, 16), 1);
-    } else {
-        wd = unit_wd(mask_bit(m.uv[1][pos][0], n, P.lane), mask_bit(m.uv[1][pos][1], n, P.lane), 0, 0);
-    }
+    if (P.luma)
+        wd = unit_wd(mask_bit(m.y[1][pos][0], n, 16), mask_bit(m.y[1][pos][1], n, 16), mask_bit(m.y[1][pos][2], n, 16), true);
+    else
+        wd = unit_wd(mask_bit(m.uv[1][pos][0], n, P.lane_h), mask_bit(m.uv[1][pos][1], n, P.lane_h), 0, false);
     if (!wd) return;
-    const int li = y4 * mi_cols + x4;
+    const uint8_t *lc = D.db.lc;
+    const int li = y4 * D.db.mi_cols + x4;
     int L = lc[4 * li + P.comp_h];
-    if (L == 0) L = lc[4 * (li - mi_cols) + P.comp_h];
+    if (L == 0) L = lc[4 * (li - D.db.mi_cols) + P.comp_h];
     if (L == 0) return;
-    const int64_t st = P.stride / (int64_t)sizeof(T);
-    T *col = reinterpret_cast<T *>(P.base + (int64_t)(y4 * 4) * P.stride) + x;
-    lpf_line<T>(col, st, bd, tabs.E[L], tabs.I[L], L >> 4, wd);
+    const PlaneView &pv = f.p[pl];
+    if (f.bd == 8)
+        lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4) + x, pv.stride, 8, D.db.E[L], D.db.I[L], L >> 4, wd);
+    else
+        lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4) + x, pv.stride / 2, f.bd, D.db.E[L], D.db.I[L], L >> 4, wd);
 }
 
-hipError_t launch_deblock(const FrameView &f, const DeblockArgs &a, hipStream_t s)
+hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,
+                          hipStream_t s)
 {
-    DbPlanes pl{};
-    int maxw4 = 0, maxh4 = 0;
-    for (int i = 0; i < f.nplanes; i++) {
-        if (!(a.plane_mask & (1 << i))) continue;
-        const int sx = i ? f.ssx : 0, sy = i ? f.ssy : 0;
-        const int pw = (f.w + sx) >> sx, ph = (f.h + sy) >> sy;
-        DbPlane &d = pl.p[pl.n++];
-        d.base = f.p[i].base;
-        d.stride = f.p[i].stride;
-        d.w4 = ((pw + 3) & ~3) >> 2;
-        d.h4 = ((ph + 3) & ~3) >> 2;
-        d.luma = i == 0;
-        d.rcw = 32 >> sx;
-        d.rch = 32 >> sy;
-        d.comp_v = i == 0 ? 0 : i + 1;
-        d.comp_h = i == 0 ? 1 : i + 1;
-        maxw4 = max(maxw4, d.w4);
-        maxh4 = max(maxh4, d.h4);
-        (void)d.lane;
-    }
-    if (pl.n == 0) return hipSuccess;
-    // lanes: vertical edges split the row bits at 16 >> ssy, horizontal at 16 >> ssx
-    DbPlanes plv = pl, plh = pl;
-    for (int i = 0; i < pl.n; i++) {
-        const bool luma = pl.p[i].luma;
-        plv.p[i].lane = luma ? 16 : (16 >> f.ssy);
-        plh.p[i].lane = luma ? 16 : (16 >> f.ssx);
-    }
-    dim3 blk(64, 4);
-    dim3 gv((maxw4 + 63) / 64, (maxh4 * 4 + 3) / 4, pl.n);
-    dim3 gh((maxw4 * 4 + 63) / 64, (maxh4 + 3) / 4, pl.n);
-    if (f.bd == 8) {
-        hipLaunchKernelGGL(deblock_v_kernel<uint8_t>, gv, blk, 0, s, plv, a.level_cache, a.masks, a.mi_cols, a.sb128_w, f.bd, a);
-        hipLaunchKernelGGL(deblock_h_kernel<uint8_t>, gh, blk, 0, s, plh, a.level_cache, a.masks, a.mi_cols, a.sb128_w, f.bd, a);
-    } else {
-        hipLaunchKernelGGL(deblock_v_kernel<uint16_t>, gv, blk, 0, s, plv, a.level_cache, a.masks, a.mi_cols, a.sb128_w, f.bd, a);
-        hipLaunchKernelGGL(deblock_h_kernel<uint16_t>, gh, blk, 0, s, plh, a.level_cache, a.masks, a.mi_cols, a.sb128_w, f.bd, a);
-    }
+    if (totv) hipLaunchKernelGGL(deblock_v_kernel, dim3(totv), dim3(64, 4), 0, s, d, segv, nv);
+    if (toth) hipLaunchKernelGGL(deblock_h_kernel, dim3(toth), dim3(64, 4), 0, s, d, segh, nh);
     return hipGetLastError();
 }
 

@@ -12,14 +12,18 @@
 //     from the pre-grain luma, clip to full or restricted range
 //     (apply.go:72-310,480-549, blend.go, sample.go, scaling.go:55-77).
 //
-// Template kernel: one workgroup.  Each lane fills one template row of
-// Gaussian samples after jumping the LFSR to the row start with a GF(2) matrix
-// power (the LFSR step is linear over GF(2)); the AR recursion, sequential in
-// the reference, runs as a diagonal wavefront: cell (y, x) only needs cells of
-// earlier rows within 3 columns and the 3 cells to its left, so every cell with
-// x + 4y = t is independent and the recursion takes ~350 barrier steps.
-// Apply kernel: one workgroup per 32x32 luma block and its chroma blocks (the
-// offsets of block (row, col) are shared by all planes); scaling LUTs in LDS.
+// Templates: one wave per template.  Each lane fills template rows of Gaussian
+// samples after jumping the LFSR to the row start with a GF(2) matrix power
+// (the LFSR step is linear over GF(2)).  The AR recursion, sequential in the
+// reference, runs as a diagonal wavefront: cell (y, x) only needs earlier rows
+// within 3 columns and the 3 cells to its left, so all cells with x + 4y = t
+// are independent (~350 steps for luma).  A lane owns rows y and y + 35, which
+// are never active in the same step.  The lag is a template parameter, so the
+// taps are unrolled with the coefficients in registers.  Templates depend only
+// on the grain parameters, so the host runs them on a side stream overlapped
+// with deblock / CDEF / LR.
+// Apply: one workgroup per 32x32 luma block and its chroma blocks (the offsets
+// of block (row, col) are shared by all planes); scaling LUTs in LDS.
 #include "pf_internal.h"
 #include "av1pf_gaussian_table.h"
 
@@ -43,94 +47,142 @@ __device__ __forceinline__ uint16_t gf2_apply(const uint16_t *J, uint16_t s)
 {
     uint16_t o = 0;
 #pragma unroll
-    for (int b = 0; b < 16; b++)
-        if ((s >> b) & 1) o ^= J[b];
+    for (int b = 0; b < 16; b++) o ^= (uint16_t)(-(int)((s >> b) & 1) & J[b]);
     return o;
 }
 
-// AR value of one cell (luma.go:60-74, chroma.go:69-87); g is the template in LDS.
-__device__ __forceinline__ void ar_cell(int16_t *g, int y, int x, int lag, const int8_t *coef, int shift,
-                                        int gmin, int gmax, bool chroma, bool luma_term, const int16_t *lg,
-                                        int ssx, int ssy)
+// Gaussian fill of one template (rows [0,h), cols [0,w)) by a 64-lane wave.
+__device__ __forceinline__ void gauss_fill(int16_t *T, uint16_t seed, const uint16_t *Jg, int w, int h, int shift)
 {
-    int sum = 0, pos = 0;
-    for (int dr = -lag; dr <= 0; dr++) {
-        for (int dc = -lag; dc <= lag; dc++) {
-            if (dr == 0 && dc == 0) {
-                if (chroma && luma_term) {
-                    const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
-                    int s = 0;
-                    for (int dy = 0; dy <= ssy; dy++)
-                        for (int dx = 0; dx <= ssx; dx++) s += lg[(ly + dy) * GW + lx + dx];
-                    sum += r2i(s, ssx + ssy) * coef[pos];
-                }
-                goto done;
-            }
-            sum += coef[pos] * g[(y + dr) * GW + x + dc];
+    uint16_t J[16];
+#pragma unroll
+    for (int b = 0; b < 16; b++) J[b] = Jg[b];
+    for (int row = threadIdx.x; row < h; row += 64) {
+        uint16_t s = seed;
+        for (int k = 0; k < row; k++) s = gf2_apply(J, s);
+        for (int x = 0; x < w; x++) {
+            s = lfsr_step(s);
+            T[row * GW + x] = (int16_t)r2i(kGauss[s >> 5], shift);
+        }
+    }
+}
+
+// One AR cell (luma.go:60-74, chroma.go:69-87) with the lag unrolled.
+template <int LAG, bool CHROMA>
+__device__ __forceinline__ void ar_cell(int16_t *T, int y, int x, const int *c, int shift, int gmin, int gmax,
+                                        const int16_t *L, bool luma_term, int ssx, int ssy)
+{
+    int sum = 0;
+    int pos = 0;
+#pragma unroll
+    for (int dr = -LAG; dr <= 0; dr++) {
+#pragma unroll
+        for (int dc = -LAG; dc <= LAG; dc++) {
+            if (dr == 0 && dc == 0) break;
+            sum += c[pos] * T[(y + dr) * GW + x + dc];
             pos++;
         }
     }
-done:
-    g[y * GW + x] = (int16_t)clipi(g[y * GW + x] + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
+    if (CHROMA && luma_term) {
+        const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
+        int s = L[ly * GW + lx];
+        if (ssx) s += L[ly * GW + lx + 1];
+        if (ssy) {
+            s += L[(ly + 1) * GW + lx];
+            if (ssx) s += L[(ly + 1) * GW + lx + 1];
+        }
+        sum += r2i(s, ssx + ssy) * c[pos];
+    }
+    T[y * GW + x] = (int16_t)clipi(T[y * GW + x] + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
 }
 
-__global__ __launch_bounds__(256) void grain_template_kernel(GrainArgs a, int bd, int ssx, int ssy)
+template <int LAG, bool CHROMA>
+__device__ void ar_wavefront(int16_t *T, int w, int h, const int8_t *coef, int shift, int gmin, int gmax,
+                             const int16_t *L, bool luma_term, int ssx, int ssy)
 {
-    __shared__ int16_t T[3][GH * GW];
-    const int tid = threadIdx.x;
-    const av1pf_grain_params &g = a.g;
-    const int shift = 12 - bd + g.grain_scale_shift;
+    constexpr int NC = 2 * LAG * (LAG + 1) + (CHROMA ? 1 : 0);
+    int c[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int i = 0; i < NC; i++) c[i] = coef[i];
+    const int y1 = 3 + threadIdx.x, y2 = y1 + 35;
+    const int tmax = (w - 4) + 4 * (h - 1);
+    for (int t = 15; t <= tmax; t++) {
+        if (threadIdx.x < 35) {
+            int x = t - 4 * y1;
+            if (y1 < h && x >= 3 && x < w - 3)
+                ar_cell<LAG, CHROMA>(T, y1, x, c, shift, gmin, gmax, L, luma_term, ssx, ssy);
+            x = t - 4 * y2;
+            if (y2 < h && x >= 3 && x < w - 3)
+                ar_cell<LAG, CHROMA>(T, y2, x, c, shift, gmin, gmax, L, luma_term, ssx, ssy);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
+{
+    __shared__ int16_t T[GH * GW];
+    const FrameDesc &D = descs[segs[blockIdx.x].frame];
+    const GrainDesc &G = D.gr;
+    const int bd = D.v[VIEW_A].bd;
+    for (int i = threadIdx.x; i < GH * GW; i += 64) T[i] = 0;
+    __syncthreads();
+    if (G.g.num_y_points) {
+        gauss_fill(T, G.g.seed, G.jump_luma, GW, GH, 12 - bd + G.g.grain_scale_shift);
+        __syncthreads();
+        const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
+        switch (G.g.ar_coeff_lag) {
+        case 1: ar_wavefront<1, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
+        case 2: ar_wavefront<2, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
+        case 3: ar_wavefront<3, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
+        default: break;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < GH * GW; i += 64) G.templates[i] = T[i];
+}
+
+__global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
+{
+    __shared__ int16_t T[GH * GW];
+    __shared__ int16_t L[GH * GW];
+    const Seg sg = segs[blockIdx.x];
+    const FrameDesc &D = descs[sg.frame];
+    const GrainDesc &G = D.gr;
+    const FrameView &f = D.v[VIEW_A];
+    const int bd = f.bd, ssx = f.ssx, ssy = f.ssy;
+    const int pl = sg.plane;
     const int cw = ssx ? 44 : GW, ch = ssy ? 38 : GH;
-    for (int i = tid; i < 3 * GH * GW; i += 256) (&T[0][0])[i] = 0;
-    __syncthreads();
-    // ---- Gaussian fill: lane -> (plane, row)
-    {
-        int pl = -1, row = 0;
-        if (tid < GH) { pl = 0; row = tid; }
-        else if (tid < GH + ch) { pl = 1; row = tid - GH; }
-        else if (tid < GH + 2 * ch) { pl = 2; row = tid - GH - ch; }
-        bool on = pl == 0 ? g.num_y_points != 0 : pl > 0;
-        if (on) {
-            uint16_t s = pl == 0 ? g.seed : (uint16_t)(g.seed ^ (pl == 1 ? 0xb524 : 0x49d8));
-            const uint16_t *J = pl == 0 ? a.jump_luma : a.jump_chroma;
-            for (int k = 0; k < row; k++) s = gf2_apply(J, s);
-            const int w = pl == 0 ? GW : cw;
-            for (int x = 0; x < w; x++) {
-                s = lfsr_step(s);
-                T[pl][row * GW + x] = (int16_t)r2i(kGauss[s >> 5], shift);
-            }
-        }
+    const bool luma_term = G.g.num_y_points != 0;
+    for (int i = threadIdx.x; i < GH * GW; i += 64) {
+        T[i] = 0;
+        L[i] = luma_term ? G.templates[i] : (int16_t)0;
     }
     __syncthreads();
-    const int lag = g.ar_coeff_lag;
-    const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
-    // ---- luma AR (diagonal wavefront, cell (y,x) at step x + 4y)
-    if (g.num_y_points && lag) {
-        const int y = 3 + tid;
-        for (int t = 3 + 4 * 3; t <= (GW - 4) + 4 * (GH - 1); t++) {
-            if (tid < GH - 3) {
-                const int x = t - 4 * y;
-                if (x >= 3 && x < GW - 3)
-                    ar_cell(T[0], y, x, lag, g.ar_coeffs_y, g.ar_coeff_shift, gmin, gmax, false, false, nullptr, 0, 0);
-            }
-            __syncthreads();
+    gauss_fill(T, (uint16_t)(G.g.seed ^ (pl == 1 ? 0xb524 : 0x49d8)), G.jump_chroma, cw, ch, 12 - bd + G.g.grain_scale_shift);
+    __syncthreads();
+    const int lag = G.g.ar_coeff_lag;
+    if (lag || luma_term) {
+        const int8_t *coef = pl == 1 ? G.g.ar_coeffs_cb : G.g.ar_coeffs_cr;
+        const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
+        const int sh = G.g.ar_coeff_shift;
+        switch (lag) {
+        case 0: ar_wavefront<0, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
+        case 1: ar_wavefront<1, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
+        case 2: ar_wavefront<2, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
+        default: ar_wavefront<3, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
         }
     }
-    // ---- chroma AR: Cb on lanes 0..127, Cr on lanes 128..255
-    if (lag || g.num_y_points) {
-        const int pl = 1 + (tid >> 7);
-        const int y = 3 + (tid & 127);
-        const int8_t *coef = pl == 1 ? g.ar_coeffs_cb : g.ar_coeffs_cr;
-        for (int t = 3 + 4 * 3; t <= (cw - 4) + 4 * (ch - 1); t++) {
-            if (y < ch) {
-                const int x = t - 4 * y;
-                if (x >= 3 && x < cw - 3)
-                    ar_cell(T[pl], y, x, lag, coef, g.ar_coeff_shift, gmin, gmax, true, g.num_y_points != 0, T[0], ssx, ssy);
-            }
-            __syncthreads();
-        }
-    }
-    for (int i = tid; i < 3 * GH * GW; i += 256) a.templates[i] = (&T[0][0])[i];
+    __syncthreads();
+    int16_t *out = G.templates + pl * GH * GW;
+    for (int i = threadIdx.x; i < GH * GW; i += 64) out[i] = T[i];
+}
+
+hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s)
+{
+    if (nl) hipLaunchKernelGGL(grain_luma_template_kernel, dim3(nl), dim3(64), 0, s, d, luma);
+    if (nc) hipLaunchKernelGGL(grain_chroma_template_kernel, dim3(nc), dim3(64), 0, s, d, chroma);
+    return hipGetLastError();
 }
 
 __device__ __forceinline__ int scale_lut(const uint8_t *lut, int index, int bd)
@@ -150,37 +202,15 @@ __device__ __forceinline__ int blendw(int prev, int cur, int wp, int wc, int bd)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void grain_apply_kernel(FrameView src, FrameView dst, GrainArgs a, int matrix_identity)
+__device__ __forceinline__ void grain_block(const FrameView &src, const FrameView &dst, const GrainDesc &G,
+                                            int bcol, int brow, const uint8_t (*lut)[256], const uint8_t (*offs)[2])
 {
-    __shared__ uint8_t lut[3][256];
-    __shared__ uint8_t offs[2][2];
     const int tid = threadIdx.x;
-    const av1pf_grain_params &g = a.g;
-    const int bcol = blockIdx.x, brow = blockIdx.y;
+    const av1pf_grain_params &g = G.g;
     const int bd = src.bd;
-    for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = a.luts[i >> 8][i & 255];
-    if (tid == 0) {
-        const int nr = g.overlap && brow > 0 ? 2 : 1;
-        for (int i = 0; i < 2; i++) { offs[0][i] = 0; offs[1][i] = 0; }
-        for (int i = 0; i < nr; i++) {
-            const int num = ((brow - i) * 32) >> 5;
-            uint16_t reg = g.seed;
-            reg ^= (uint16_t)(((num * 37 + 178) & 255) << 8);
-            reg ^= (uint16_t)((num * 173 + 105) & 255);
-            uint8_t prev = 0, cur = 0;
-            for (int c = 0; c <= bcol; c++) {
-                reg = lfsr_step(reg);
-                prev = cur;
-                cur = (uint8_t)(reg >> 8);
-            }
-            offs[0][i] = cur;
-            offs[1][i] = (g.overlap && bcol > 0) ? prev : 0;
-        }
-    }
-    __syncthreads();
     const int scaling_shift = g.scaling_shift;
     for (int pl = 0; pl < src.nplanes; pl++) {
-        if (!a.active[pl]) continue;
+        if (!G.active[pl]) continue;
         const int sx = pl ? src.ssx : 0, sy = pl ? src.ssy : 0;
         const PlaneView &sp = src.p[pl];
         const PlaneView &dp = dst.p[pl];
@@ -191,39 +221,37 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(FrameView src, FrameVi
         const int bwid = min(bw, pw - px0), height = min(bh, ph - py0);
         const int ys = (g.overlap && brow > 0) ? min(height, 2 >> sy) : 0;
         const int xs = (g.overlap && bcol > 0) ? min(bwid, 2 >> sx) : 0;
-        const int16_t *tmpl = a.templates + pl * GH * GW;
+        const int16_t *tmpl = G.templates + pl * GH * GW;
         int minv = 0, maxv = (256 << (bd - 8)) - 1;
         if (g.clip_to_restricted_range) {
             minv = 16 << (bd - 8);
-            maxv = (pl == 0 ? 235 : (matrix_identity ? 235 : 240)) << (bd - 8);
+            maxv = (pl == 0 ? 235 : (G.matrix_identity ? 235 : 240)) << (bd - 8);
         }
         const bool odd_luma = pl && sx && (src.p[0].w & 1);
         const int mult = pl == 1 ? g.cb_mult : g.cr_mult;
         const int lmult = pl == 1 ? g.cb_luma_mult : g.cr_luma_mult;
         const int coff = pl == 1 ? g.cb_offset : g.cr_offset;
-        auto goff = [&](int o, bool row) -> int {
-            const int n = row ? (o & 15) : (o >> 4);
-            const int s = row ? sy : sx;
-            return 3 + (2 >> s) * (3 + n);
-        };
+        const bool csub_x = pl && sx, csub_y = pl && sy;
+        // template offsets of the four candidate blocks (sample.go:9-77)
+        const int ox0 = 3 + (2 >> sx) * (3 + (offs[0][0] >> 4)), oy0 = 3 + (2 >> sy) * (3 + (offs[0][0] & 15));
+        const int ox1 = 3 + (2 >> sx) * (3 + (offs[1][0] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1][0] & 15));
+        const int ox2 = 3 + (2 >> sx) * (3 + (offs[0][1] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[0][1] & 15));
+        const int ox3 = 3 + (2 >> sx) * (3 + (offs[1][1] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[1][1] & 15));
         for (int i = tid; i < bwid * height; i += 256) {
             const int y = i / bwid, x = i - y * bwid;
-#define GS(o, bc, br) tmpl[(goff((o), true) + y + bh * (br)) * GW + goff((o), false) + x + bw * (bc)]
-            int gv = GS(offs[0][0], 0, 0);
-            const bool csub_x = pl && sx, csub_y = pl && sy;
+            int gv = tmpl[(oy0 + y) * GW + ox0 + x];
             if (x < xs) {
-                const int left = GS(offs[1][0], 1, 0);
+                const int left = tmpl[(oy1 + y) * GW + ox1 + x + bw];
                 gv = csub_x ? blendw(left, gv, 23, 22, bd) : blendw(left, gv, x == 1 ? 17 : 27, x == 1 ? 27 : 17, bd);
             }
             if (y < ys) {
-                int top = GS(offs[0][1], 0, 1);
+                int top = tmpl[(oy2 + y + bh) * GW + ox2 + x];
                 if (x < xs) {
-                    const int tl = GS(offs[1][1], 1, 1);
+                    const int tl = tmpl[(oy3 + y + bh) * GW + ox3 + x + bw];
                     top = csub_x ? blendw(tl, top, 23, 22, bd) : blendw(tl, top, x == 1 ? 17 : 27, x == 1 ? 27 : 17, bd);
                 }
                 gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
             }
-#undef GS
             const int px = px0 + x, py = py0 + y;
             const int s = row_ptr<T>(sp, py)[px];
             int idx;
@@ -247,18 +275,46 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(FrameView src, FrameVi
     }
 }
 
-hipError_t launch_grain_templates(const GrainArgs &a, int bd, int ssx, int ssy, hipStream_t s)
+__global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
 {
-    hipLaunchKernelGGL(grain_template_kernel, dim3(1), dim3(256), 0, s, a, bd, ssx, ssy);
-    return hipGetLastError();
+    __shared__ uint8_t lut[3][256];
+    __shared__ uint8_t offs[2][2];
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const GrainDesc &G = D.gr;
+    const int t = blockIdx.x - sg.base;
+    const int bcol = t % sg.nx, brow = t / sg.nx;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = G.luts[i >> 8][i & 255];
+    if (tid == 0) {
+        const av1pf_grain_params &g = G.g;
+        const int nr = g.overlap && brow > 0 ? 2 : 1;
+        for (int i = 0; i < 2; i++) { offs[0][i] = 0; offs[1][i] = 0; }
+        for (int i = 0; i < nr; i++) {
+            const int num = brow - i;
+            uint16_t reg = g.seed;
+            reg ^= (uint16_t)(((num * 37 + 178) & 255) << 8);
+            reg ^= (uint16_t)((num * 173 + 105) & 255);
+            uint8_t prev = 0, cur = 0;
+            for (int c = 0; c <= bcol; c++) {
+                reg = lfsr_step(reg);
+                prev = cur;
+                cur = (uint8_t)(reg >> 8);
+            }
+            offs[0][i] = cur;
+            offs[1][i] = (g.overlap && bcol > 0) ? prev : 0;
+        }
+    }
+    __syncthreads();
+    const FrameView &src = D.v[VIEW_A];
+    const FrameView &dst = D.v[VIEW_D];
+    if (src.bd == 8) grain_block<uint8_t>(src, dst, G, bcol, brow, lut, offs);
+    else grain_block<uint16_t>(src, dst, G, bcol, brow, lut, offs);
 }
 
-hipError_t launch_grain_apply(const FrameView &src, const FrameView &dst, const GrainArgs &a,
-                              int matrix_identity, hipStream_t s)
+hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
 {
-    dim3 grid((src.w + 31) / 32, (src.h + 31) / 32);
-    if (src.bd == 8) hipLaunchKernelGGL(grain_apply_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst, a, matrix_identity);
-    else hipLaunchKernelGGL(grain_apply_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst, a, matrix_identity);
+    if (total) hipLaunchKernelGGL(grain_apply_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
     return hipGetLastError();
 }
 

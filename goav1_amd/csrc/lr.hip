@@ -17,21 +17,17 @@
 //   - SGR: box sums, uint32 p/z/A/B math with LUTs, 6/5 (r=2, every other row)
 //     and 4/3 (r=1) weighted sums, projection with an int16 wrap (selfguided.go).
 //
-// Decomposition: one workgroup per (plane, stripe, 64-column processing unit);
-// every such tile lies inside exactly one restoration unit, so the record is
-// looked up per tile.  The tile and its 3-sample halo are staged in LDS once.
-// The kernel is out of place (post-CDEF frame -> output frame); NONE units and
-// NONE planes copy.
+// Decomposition: one workgroup per (frame, plane, stripe, 64-column processing
+// unit); every such tile lies inside exactly one restoration unit, so the
+// record is looked up per tile.  The tile and its 3-sample halo are staged in
+// LDS once.  The kernel is out of place (post-CDEF frame -> output frame); NONE
+// units and NONE planes copy.
 #include "pf_internal.h"
 
 namespace pf {
 
 #define LR_TW 70          // 64 + 6
 #define LR_AB 66          // 64 + 2
-
-struct LrPlaneGeom {
-    int pw, ph, procw, sh, off, xtiles, stripes;
-};
 
 __device__ __forceinline__ int32_t r2s(int32_t v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
 __device__ __forceinline__ uint32_t r2u(uint32_t v, int n) { return n <= 0 ? v : (v + (1u << (n - 1))) >> n; }
@@ -56,26 +52,10 @@ __device__ __forceinline__ int read_px(const PlaneView &pv, int y, int x)
     return row_ptr<T>(pv, y)[x];
 }
 
-struct LrLaunchPlane {
-    int type, us, hz, vt;
-    const av1pf_lr_unit *units;
-    const uint16_t *bnd;
-    LrPlaneGeom g;
-    int plane;
-};
-struct LrLaunch { LrLaunchPlane p[3]; int n; int bd; };
-
 template <typename T>
-__global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, LrLaunch L)
+__device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp, const LrPlaneDesc &g, int bd,
+                                        int k, int t, uint16_t *tile, uint16_t *tmp, int32_t *Ab, int32_t *Bb)
 {
-    __shared__ uint16_t tile[LR_TW * LR_TW];
-    __shared__ uint16_t tmp[LR_TW * 64];
-    __shared__ int32_t Ab[LR_AB * LR_AB];
-    __shared__ int32_t Bb[LR_AB * LR_AB];
-
-    const LrLaunchPlane &P = L.p[blockIdx.z];
-    const LrPlaneGeom &g = P.g;
-    const int k = blockIdx.x, t = blockIdx.y;
     if (k >= g.xtiles || t >= g.stripes) return;
     const int x0 = k * g.procw;
     const int w = min(g.procw, g.pw - x0);
@@ -84,16 +64,13 @@ __global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, L
     if (w <= 0 || y0 >= y1) return;
     const int h = y1 - y0;
     const int tid = threadIdx.x;
-    const PlaneView &sp = src.p[P.plane];
-    const PlaneView &dp = dst.p[P.plane];
-    const int bd = L.bd;
 
     av1pf_lr_unit u;
     u.type = AV1PF_LR_NONE;
-    if (P.type != AV1PF_LR_NONE) {
-        const int uc = min(x0 / P.us, P.hz - 1);
-        const int ur = min((y0 + g.off) / P.us, P.vt - 1);
-        u = P.units[ur * P.hz + uc];
+    if (g.type != AV1PF_LR_NONE) {
+        const int uc = min(x0 / g.us, g.hz - 1);
+        const int ur = min((y0 + g.off) / g.us, g.vt - 1);
+        u = g.units[ur * g.hz + uc];
     }
     if (u.type != AV1PF_LR_WIENER && u.type != AV1PF_LR_SGRPROJ) {
         for (int i = tid; i < w * h; i += 256) {
@@ -106,7 +83,7 @@ __global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, L
     const int nominal = g.sh - (y0 == 0 ? g.off : 0);
     const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
     const int tw = w + 6, th = h + 6;
-    const uint16_t *bnd = P.bnd + (int64_t)t * 4 * g.pw;
+    const uint16_t *bnd = g.bnd + (int64_t)t * 4 * g.pw;
     for (int i = tid; i < tw * th; i += 256) {
         const int r = i / tw - 3, c = i - (i / tw) * tw - 3;
         const int x = min(max(x0 + c, 0), g.pw - 1);
@@ -203,18 +180,17 @@ __global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, L
             const int32_t uu = O[y * LR_TW + x];
             const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
             const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
-            int32_t a, b, v;
+            int32_t a, b;
             if (pass == 0) {
                 if ((y & 1) == 0) {
                     a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
                     b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
-                    v = r2s(a * uu + b, 9);
+                    f0[j] = r2s(a * uu + b, 9);
                 } else {
                     a = A[0] * 6 + (A[-1] + A[1]) * 5;
                     b = B[0] * 6 + (B[-1] + B[1]) * 5;
-                    v = r2s(a * uu + b, 8);
+                    f0[j] = r2s(a * uu + b, 8);
                 }
-                f0[j] = v;
             } else {
                 a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
                     (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
@@ -238,21 +214,36 @@ __global__ __launch_bounds__(256) void lr_kernel(FrameView src, FrameView dst, L
     }
 }
 
+__global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+{
+    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ uint16_t tmp[LR_TW * 64];
+    __shared__ int32_t Ab[LR_AB * LR_AB];
+    __shared__ int32_t Bb[LR_AB * LR_AB];
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const int t = blockIdx.x - sg.base;
+    const int k = t % sg.nx, st = t / sg.nx;
+    const FrameView &src = D.v[D.lr_src];
+    const FrameView &dst = D.v[D.lr_dst];
+    const int pl = sg.plane;
+    if (src.bd == 8)
+        lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab, Bb);
+    else
+        lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab, Bb);
+}
+
 // Save the deblocked context rows of every stripe: bnd[t][0..1] = rows y0-2,
 // y0-1 (t > 0), bnd[t][2..3] = rows y1, y1+1 (duplicated when y1+1 is past
-// the plane), for stripes with an interior lower seam.
+// the plane), for stripes with an interior lower seam.  Workgroup = 256
+// columns of one stripe.
 template <typename T>
-__global__ void lr_save_kernel(FrameView f, LrLaunch L)
+__device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneDesc &g, int x, int t)
 {
-    const LrLaunchPlane &P = L.p[blockIdx.z];
-    const LrPlaneGeom &g = P.g;
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    const int t = blockIdx.y;
     if (x >= g.pw || t >= g.stripes) return;
-    const PlaneView &pv = f.p[P.plane];
     const int y0 = max(0, t * g.sh - g.off);
     const int y1 = min((t + 1) * g.sh - g.off, g.ph);
-    uint16_t *b = const_cast<uint16_t *>(P.bnd) + (int64_t)t * 4 * g.pw;
+    uint16_t *b = g.bnd + (int64_t)t * 4 * g.pw;
     if (t > 0) {
         b[x] = row_ptr<T>(pv, y0 - 2)[x];
         b[g.pw + x] = row_ptr<T>(pv, y0 - 1)[x];
@@ -263,77 +254,52 @@ __global__ void lr_save_kernel(FrameView f, LrLaunch L)
     }
 }
 
-static LrLaunch make_launch(const FrameView &f, const LrArgs &a, bool all_planes, dim3 *grid, int xdiv)
+__global__ __launch_bounds__(256) void lr_save_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
 {
-    LrLaunch L{};
-    L.bd = f.bd;
-    int mx = 0, my = 0;
-    for (int i = 0; i < f.nplanes; i++) {
-        const LrPlaneArgs &pa = a.p[i];
-        if (!all_planes && pa.type == AV1PF_LR_NONE) continue;
-        LrLaunchPlane &p = L.p[L.n++];
-        p.plane = i;
-        p.type = pa.type;
-        p.us = pa.unit_size;
-        p.hz = pa.horz;
-        p.vt = pa.vert;
-        p.units = pa.units;
-        p.bnd = pa.bnd;
-        const int sx = i ? f.ssx : 0, sy = i ? f.ssy : 0;
-        p.g.pw = f.p[i].w;
-        p.g.ph = f.p[i].h;
-        p.g.procw = 64 >> sx;
-        p.g.sh = 64 >> sy;
-        p.g.off = 8 >> sy;
-        p.g.xtiles = (p.g.pw + p.g.procw - 1) / p.g.procw;
-        p.g.stripes = (p.g.ph + p.g.off + p.g.sh - 1) / p.g.sh;
-        mx = max(mx, xdiv ? (p.g.pw + xdiv - 1) / xdiv : p.g.xtiles);
-        my = max(my, p.g.stripes);
-    }
-    *grid = dim3(mx, my, L.n);
-    return L;
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const int t = blockIdx.x - sg.base;
+    const int x = (t % sg.nx) * 256 + threadIdx.x, st = t / sg.nx;
+    const FrameView &f = D.v[VIEW_A];
+    if (f.bd == 8) lr_save_cols<uint8_t>(f.p[sg.plane], D.lr[sg.plane], x, st);
+    else lr_save_cols<uint16_t>(f.p[sg.plane], D.lr[sg.plane], x, st);
 }
 
-hipError_t launch_lr_save(const FrameView &f, const LrArgs &a, hipStream_t s)
+hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
 {
-    dim3 grid;
-    LrLaunch L = make_launch(f, a, false, &grid, 256);
-    if (L.n == 0) return hipSuccess;
-    if (f.bd == 8) hipLaunchKernelGGL(lr_save_kernel<uint8_t>, grid, dim3(256), 0, s, f, L);
-    else hipLaunchKernelGGL(lr_save_kernel<uint16_t>, grid, dim3(256), 0, s, f, L);
+    if (total) hipLaunchKernelGGL(lr_save_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
     return hipGetLastError();
 }
 
-hipError_t launch_lr(const FrameView &src, const FrameView &dst, const LrArgs &a, hipStream_t s)
+hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
 {
-    dim3 grid;
-    LrLaunch L = make_launch(src, a, true, &grid, 0);
-    if (L.n == 0) return hipSuccess;
-    if (src.bd == 8) hipLaunchKernelGGL(lr_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst, L);
-    else hipLaunchKernelGGL(lr_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst, L);
+    if (total) hipLaunchKernelGGL(lr_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
     return hipGetLastError();
 }
 
-// Plain copy of every plane's aligned extent (used when a stage mix leaves the
-// result in scratch).
+// Plain copy of one plane's aligned extent: workgroup = 256 samples of a row.
 template <typename T>
-__global__ void copy_kernel(FrameView src, FrameView dst)
+__device__ __forceinline__ void copy_cols(const PlaneView &s, const PlaneView &d, int x, int y)
 {
-    const int pl = blockIdx.z;
-    if (pl >= src.nplanes) return;
-    const PlaneView &s = src.p[pl];
-    const PlaneView &d = dst.p[pl];
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y = blockIdx.y;
     if (x >= s.aw || y >= s.ah) return;
     row_ptr<T>(d, y)[x] = row_ptr<T>(s, y)[x];
 }
 
-hipError_t launch_copy(const FrameView &src, const FrameView &dst, hipStream_t s)
+__global__ __launch_bounds__(256) void copy_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
+                                                   int nseg, int src, int dst)
 {
-    dim3 grid((src.p[0].aw + 255) / 256, src.p[0].ah, src.nplanes);
-    if (src.bd == 8) hipLaunchKernelGGL(copy_kernel<uint8_t>, grid, dim3(256), 0, s, src, dst);
-    else hipLaunchKernelGGL(copy_kernel<uint16_t>, grid, dim3(256), 0, s, src, dst);
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const int t = blockIdx.x - sg.base;
+    const int x = (t % sg.nx) * 256 + threadIdx.x, y = t / sg.nx;
+    const FrameView &S = D.v[src];
+    if (S.bd == 8) copy_cols<uint8_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);
+    else copy_cols<uint16_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);
+}
+
+hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s)
+{
+    if (total) hipLaunchKernelGGL(copy_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg, src, dst);
     return hipGetLastError();
 }
 

@@ -1,5 +1,12 @@
 // Internal declarations shared by the gfx950 post-filter kernels and the host
 // side of libav1pf.  Not part of the public ABI (include/av1pf.h is).
+//
+// Every stage kernel is batched: one launch covers any number of frames of any
+// size, bit depth and chroma format.  The host uploads one FrameDesc per frame
+// (device memory) and, per launch, a segment table: segment k covers the
+// workgroups [base_k, base_{k+1}) and names the frame and plane they work on.
+// A workgroup finds its segment by binary search, so the grid is one flat
+// range over all frames and the GPU stays full even for small frames.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -28,61 +35,83 @@ __device__ __forceinline__ T *row_ptr(const PlaneView &pv, int y) {
     return reinterpret_cast<T *>(pv.base + (int64_t)y * pv.stride);
 }
 
-// ---- deblocking --------------------------------------------------------
-struct DeblockArgs {
-    const uint8_t *level_cache;   // device
-    const av1pf_lf_mask *masks;   // device
-    int mi_cols, sb128_w;
-    uint8_t E[64], I[64];         // per-level limits for the frame sharpness
-    int plane_mask;               // which planes are filtered
+struct DbDesc {
+    const uint8_t *lc;             // level cache, 4 bytes per MI
+    const av1pf_lf_mask *masks;
+    int mi_cols, sbw;
+    uint8_t E[64], I[64];          // per-level limits for the frame sharpness
 };
-hipError_t launch_deblock(const FrameView &f, const DeblockArgs &a, hipStream_t s);
 
-// ---- CDEF --------------------------------------------------------------
-struct CdefArgs {
-    const int8_t *index;          // device, per unit
-    const uint64_t *noskip;       // device, per unit (may be null = all)
+struct CdefDesc {
+    const int8_t *index;
+    const uint64_t *noskip;        // may be null (all blocks)
     int unit_cols, unit_rows;
-    int damping;
-    int strength_count;
+    int damping, strength_count;
+    int yfilt, uvfilt;
     uint8_t y_strength[8], uv_strength[8];
-    int yfilt, uvfilt;            // any nonzero luma / chroma strength
 };
-// src -> dst (out of place); every sample of the aligned extent of dst is written.
-hipError_t launch_cdef(const FrameView &src, const FrameView &dst, const CdefArgs &a, hipStream_t s, int *err);
 
-// ---- loop restoration ---------------------------------------------------
-struct LrPlaneArgs {
-    int type;                     // frame type (0 = none)
-    int unit_size, horz, vert;    // unit grid
-    const av1pf_lr_unit *units;   // device
-    uint16_t *bnd;                // device: saved deblocked rows [stripes][4][pw] (above0, above1, below0, below1)
-    int stripes;
+struct LrPlaneDesc {
+    int type;                      // frame type, 0 = none (plane copied)
+    int us, hz, vt;                // unit size and grid
+    const av1pf_lr_unit *units;
+    uint16_t *bnd;                 // [stripes][4][pw]: rows y0-2, y0-1, y1, y1+1 of the deblocked frame
+    int pw, ph, procw, sh, off, xtiles, stripes;
 };
-struct LrArgs { LrPlaneArgs p[3]; };
-// save the deblocked context rows of every stripe (called on the deblocked frame)
-hipError_t launch_lr_save(const FrameView &f, const LrArgs &a, hipStream_t s);
-// src (post-CDEF) -> dst
-hipError_t launch_lr(const FrameView &src, const FrameView &dst, const LrArgs &a, hipStream_t s);
 
-// ---- film grain ---------------------------------------------------------
-struct GrainArgs {
+struct GrainDesc {
     av1pf_grain_params g;
     int active[3];
-    int16_t *templates;           // device: 3 x 73 x 82
-    uint16_t jump_luma[16];       // GF(2) LFSR jump matrices (columns) for one template row
-    uint16_t jump_chroma[16];
+    uint16_t jump_luma[16], jump_chroma[16];   // GF(2) LFSR jumps over one template row
     uint8_t luts[3][256];
+    int16_t *templates;            // 3 x 73 x 82
+    int matrix_identity;
 };
-hipError_t launch_grain_templates(const GrainArgs &a, int bd, int ssx, int ssy, hipStream_t s);
-hipError_t launch_grain_apply(const FrameView &src, const FrameView &dst, const GrainArgs &a,
-                              int matrix_identity, hipStream_t s);
 
-// ---- encoder CDEF search -------------------------------------------------
+enum { VIEW_A = 0, VIEW_B = 1, VIEW_D = 2 };
+
+struct FrameDesc {
+    FrameView v[3];                // frame (A), scratch (B), display (D)
+    int lr_src, lr_dst;            // views the LR stage reads / writes
+    DbDesc db;
+    CdefDesc cd;
+    LrPlaneDesc lr[3];
+    GrainDesc gr;
+    int *err;                      // device validation flag
+};
+
+struct Seg {
+    int frame, plane;              // frame index, plane (or -1)
+    int nx;                        // tiles per row of this segment
+    int base;                      // first workgroup of the segment
+};
+
+// Workgroup -> segment (segs sorted by base; segs[n].base = total).
+__device__ __forceinline__ int find_seg(const Seg *__restrict__ segs, int n, int wg)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].base <= wg) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- launches (all batched) ------------------------------------------------
+hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,
+                          hipStream_t s);
+hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
+hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s);
+hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+
+// ---- encoder CDEF search (single frame) ----------------------------------
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s);
 
-// plain copy of the aligned extent of every plane
-hipError_t launch_copy(const FrameView &src, const FrameView &dst, hipStream_t s);
+// tile geometry helpers shared by host and device
+__host__ __device__ inline int lr_stripes(int ph, int sh, int off) { return (ph + off + sh - 1) / sh; }
 
 }  // namespace pf

@@ -214,6 +214,24 @@ class PostFilterRunner:
         ra = (abi.Request * n)(*[s.req for s in sides])
         self._check(self.lib.av1pf_apply_batch(self.h, n, fa, ra, self._stream(stream)))
 
+    # stage-level entry points (the stage functions the chain composes)
+    def deblock(self, frame, side, stream=None):
+        self._check(self.lib.av1pf_deblock(self.h, C.byref(frame.abi), side.req.deblock, self._stream(stream)))
+
+    def cdef(self, frame, side, stream=None):
+        self._check(self.lib.av1pf_cdef(self.h, C.byref(frame.abi), side.req.cdef, self._stream(stream)))
+
+    def lr_save_boundaries(self, frame, side, after_cdef, stream=None):
+        self._check(self.lib.av1pf_lr_save_boundaries(self.h, C.byref(frame.abi), side.req.lr, int(after_cdef),
+                                                      self._stream(stream)))
+
+    def lr(self, frame, side, stream=None):
+        self._check(self.lib.av1pf_lr(self.h, C.byref(frame.abi), side.req.lr, self._stream(stream)))
+
+    def filmgrain(self, src, dst, side, stream=None):
+        self._check(self.lib.av1pf_filmgrain(self.h, C.byref(src.abi), C.byref(dst.abi), side.req.grain,
+                                             self._stream(stream)))
+
     def sync(self, stream=None):
         self._check(self.lib.av1pf_sync(self.h, self._stream(stream)))
 

@@ -184,9 +184,10 @@ const char *av1pf_last_error(const av1pf_ctx *ctx);
  * (decoder/work.go:153-155, decoder_postfilter_bind.go:414).               */
 int  av1pf_apply(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *req, void *stream);
 
-/* Run many frames of mixed size, bit depth and chroma format.  One persistent
- * launch per stage drains a device work queue of 64x64 tiles across all frames
- * (no counterpart in the reference, which runs frames one at a time).       */
+/* Run many frames of mixed size, bit depth and chroma format.  One launch per
+ * stage covers every frame of the batch: each workgroup finds its frame,
+ * plane and tile in a segment table (no counterpart in the reference, which
+ * runs frames one at a time).  Equivalent to n av1pf_apply calls.           */
 int  av1pf_apply_batch(av1pf_ctx *ctx, int n, const av1pf_frame *frames, const av1pf_request *reqs, void *stream);
 
 /* Wait for all work the handle has queued on `stream`.                      */

@@ -103,25 +103,14 @@ def run_chain(fr, stages=None):
     hr = HostRequest(fr, stages)
     of = orc_frame(out)
     disp = None
-    dptr = None
+    dof = None
     if hr.req.stages & abi.STAGE_FILMGRAIN:
         disp = fr.copy_pixels()
-        dptr = C.byref(orc_frame_keep(disp))
-    rc = lib().orc_chain(C.byref(of), C.byref(hr.req), dptr)
+        dof = orc_frame(disp)
+    rc = lib().orc_chain(C.byref(of), C.byref(hr.req), C.byref(dof) if dof is not None else None)
     if rc != 0:
         raise RuntimeError(f"orc_chain failed: {rc}")
     return out, disp
-
-
-_keep = []
-
-
-def orc_frame_keep(fr):
-    of = orc_frame(fr)
-    _keep.append(of)
-    if len(_keep) > 64:
-        del _keep[:32]
-    return of
 
 
 def plane_md5(fr):

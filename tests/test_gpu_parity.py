@@ -120,3 +120,67 @@ def test_sequence_1080p10_first_frames(runner):
         got, gdisp = gpu_run(runner, fr)
         assert_same(got, want, "1080p10 filtered")
         assert_same(gdisp, wdisp, "1080p10 display")
+
+
+def _frames_from(fr, devs):
+    out = fr.copy_pixels()
+    for pl, a in zip(out.planes, devs.to_numpy()):
+        pl.pix[:] = a
+    return out
+
+
+def test_mixed_batch(runner):
+    """One apply_batch over frames of mixed size, depth, chroma format and
+    stage set must equal the oracle frame by frame."""
+    from goav1_amd.lib import DeviceFrame, DeviceSideData
+    import torch
+    rng = np.random.default_rng(4242)
+    sizes = [(200, 136), (136, 72), (66, 66), (426, 240), (90, 50), (320, 180)]
+    stage_sets = STAGE_SETS + [abi.STAGES_ALL, abi.STAGE_FILMGRAIN, abi.STAGE_CDEF | abi.STAGE_FILMGRAIN]
+    frames, devs, disps, sides = [], [], [], []
+    for i in range(14):
+        w, h = sizes[i % len(sizes)]
+        bd = 8 if i % 3 else 10
+        ss = (i // 2) % 2
+        fr = small_frame(9000 + i, w, h, bd, ss, lr_unit=int(rng.choice([64, 128])))
+        fr.grain = synth.random_grain(rng, bd, lag=int(rng.integers(0, 4)))
+        fr.stages = stage_sets[i % len(stage_sets)]
+        d = DeviceFrame(fr)
+        disp = DeviceFrame.empty_like(fr) if fr.stages & abi.STAGE_FILMGRAIN else None
+        frames.append(fr)
+        devs.append(d)
+        disps.append(disp)
+        sides.append(DeviceSideData(fr, display=disp, stages=fr.stages))
+    print(f"mixed batch of {len(frames)} frames", flush=True)
+    runner.apply_batch(devs, sides)
+    runner.sync()
+    torch.cuda.synchronize()
+    for i, fr in enumerate(frames):
+        want, wdisp = orc.run_chain(fr, fr.stages)
+        assert_same(_frames_from(fr, devs[i]), want, f"batch frame {i} stages {fr.stages:#x}")
+        if disps[i] is not None:
+            assert_same(_frames_from(fr, disps[i]), wdisp, f"batch frame {i} display")
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_stage_entry_points(runner, bd):
+    """deblock -> save LR rows -> cdef -> save (after CDEF) -> lr -> film grain
+    through the per-stage entry points equals the chain."""
+    from goav1_amd.lib import DeviceFrame, DeviceSideData
+    import torch
+    fr = small_frame(555 + bd, 264, 168, bd, 1, grain=True)
+    d = DeviceFrame(fr)
+    disp = DeviceFrame.empty_like(fr)
+    side = DeviceSideData(fr, display=disp)
+    print(f"stage entry points bd={bd}", flush=True)
+    runner.deblock(d, side)
+    runner.lr_save_boundaries(d, side, after_cdef=False)
+    runner.cdef(d, side)
+    runner.lr_save_boundaries(d, side, after_cdef=True)
+    runner.lr(d, side)
+    runner.filmgrain(d, disp, side)
+    runner.sync()
+    torch.cuda.synchronize()
+    want, wdisp = orc.run_chain(fr)
+    assert_same(_frames_from(fr, d), want, "stage-level chain")
+    assert_same(_frames_from(fr, disp), wdisp, "stage-level display")
