@@ -14,8 +14,10 @@
 // device in one transfer, and launches each stage once for all frames.  Slots
 // form a ring guarded by events, so back-to-back calls never overwrite
 // descriptors a previous call's kernels still read.  Film-grain templates
-// depend only on the grain parameters and run on a side stream that joins the
-// main stream before the grain apply.
+// depend only on the grain parameters and run on a side stream (which also
+// uploads the descriptors) that joins the main stream before the grain apply;
+// with double-buffered template memory, the templates of call N+1 are built
+// while call N is still filtering.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -28,6 +30,7 @@ using namespace pf;
 
 namespace {
 constexpr int kSlots = 4;
+constexpr size_t kSlotBytes = size_t(1) << 20;   // ~400 frames of descriptors
 }
 
 struct av1pf_ctx {
@@ -37,11 +40,20 @@ struct av1pf_ctx {
     size_t scratch_cap = 0;
     uint16_t *lr_bnd = nullptr;   // LR stripe context rows
     size_t lr_bnd_cap = 0;
-    int16_t *templates = nullptr; // film-grain templates (3 x 73 x 82 per frame)
-    size_t templates_cap = 0;
+    // film-grain templates (3 x 73 x 82 per frame) and block offsets, double
+    // buffered: the side stream fills one pair for call N+1 while the grain
+    // apply of call N reads the other
+    int16_t *templates[2] = {};
+    size_t templates_cap[2] = {};
+    uint8_t *grain_offs[2] = {};
+    size_t grain_offs_cap[2] = {};
+    hipEvent_t tmpl_free[2] = {};  // grain apply done with pair p
+    bool tmpl_free_used[2] = {};
+    int tpar = 0;
+    hipEvent_t uploaded = nullptr;
     int *dev_err = nullptr;       // device-side validation flag
     hipStream_t side = nullptr;   // film-grain template stream
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t join = nullptr;
     // descriptor ring
     char *meta_dev[kSlots] = {};
     char *meta_host[kSlots] = {};
@@ -297,6 +309,14 @@ void lfsr_jump(uint16_t J[16], int steps)
     }
 }
 
+uint16_t gf2_apply(const uint16_t J[16], uint16_t s)
+{
+    uint16_t o = 0;
+    for (int b = 0; b < 16; b++)
+        if ((s >> b) & 1) o ^= J[b];
+    return o;
+}
+
 int validate_request(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req)
 {
     if (!req) return fail(c, AV1PF_ERR_INVALID_ARG, "null request");
@@ -328,7 +348,7 @@ struct BatchPlan {
     std::vector<FrameDesc> descs;
     SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain;
     double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0;
-    size_t scratch = 0, bnd = 0;
+    size_t scratch = 0, bnd = 0, goffs = 0;
     int ngrain = 0;
 };
 
@@ -336,8 +356,9 @@ struct BatchPlan {
 // once the arena bases are known (offsets kept in the descriptors meanwhile).
 int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, BatchPlan &P,
                std::vector<size_t> &scr_off, std::vector<size_t> &bnd_off, std::vector<int> &grain_idx,
-               bool skip_lr_save)
+               std::vector<size_t> &goff, bool skip_lr_save)
 {
+    goff.assign(n, 0);
     P.descs.assign(n, FrameDesc{});
     scr_off.assign(n, 0);
     bnd_off.assign(n * 3, 0);
@@ -449,9 +470,10 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
         // ---- film grain
         if (st & AV1PF_STAGE_FILMGRAIN) {
             const av1pf_grain_params *g = r->grain;
-            for (int pl = 0; pl < A.nplanes; pl++) P.copy_ad.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
-            P.bytes_copy += 2 * view_bytes(A);
-            if (!grain_noop(g)) {
+            if (grain_noop(g)) {
+                for (int pl = 0; pl < A.nplanes; pl++) P.copy_ad.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+                P.bytes_copy += 2 * view_bytes(A);
+            } else {
                 GrainDesc &G = D.gr;
                 G.g = *g;
                 G.active[0] = g->num_y_points != 0;
@@ -465,9 +487,23 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                     build_scaling_lut(G.luts[1], (const uint8_t(*)[2])g->cb_points, g->num_cb_points);
                     build_scaling_lut(G.luts[2], (const uint8_t(*)[2])g->cr_points, g->num_cr_points);
                 }
-                lfsr_jump(G.jump_luma, 82);
-                lfsr_jump(G.jump_chroma, f->ss_x ? 44 : 82);
+                uint16_t jl[16], jc[16];
+                lfsr_jump(jl, 82);
+                lfsr_jump(jc, f->ss_x ? 44 : 82);
+                const uint16_t seeds[3] = { g->seed, (uint16_t)(g->seed ^ 0xb524),
+                                            (uint16_t)(g->seed ^ 0x49d8) };
+                for (int p = 0; p < 3; p++) {
+                    uint16_t st = seeds[p];
+                    for (int r = 0; r < 73; r++) {
+                        G.row_seed[p][r] = st;
+                        st = gf2_apply(p ? jc : jl, st);
+                    }
+                }
                 G.matrix_identity = f->matrix_identity;
+                G.orows = (f->height + 31) / 32;
+                G.ocols = (f->width + 31) / 32;
+                goff[i] = P.goffs;
+                P.goffs += (size_t)G.orows * G.ocols;
                 grain_idx[i] = P.ngrain++;
                 P.tl.add(i, 0, 1, 1);
                 if (!f->mono) {
@@ -501,14 +537,22 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
 {
     if (n == 0) return AV1PF_OK;
     BatchPlan P;
-    std::vector<size_t> scr_off, bnd_off;
+    std::vector<size_t> scr_off, bnd_off, goff;
     std::vector<int> grain_idx;
-    int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, skip_lr_save);
+    int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, goff, skip_lr_save);
     if (rc) return rc;
     if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
     if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
     const size_t tsz = (size_t)3 * 73 * 82 * sizeof(int16_t);
-    if ((rc = ensure(c, (void **)&c->templates, &c->templates_cap, std::max<size_t>(P.ngrain * tsz, 256)))) return rc;
+    const bool side = P.tl.total > 0;
+    const int par = c->tpar;
+    if (side) {
+        c->tpar ^= 1;
+        if ((rc = ensure(c, (void **)&c->templates[par], &c->templates_cap[par], std::max<size_t>(P.ngrain * tsz, 256))))
+            return rc;
+        if ((rc = ensure(c, (void **)&c->grain_offs[par], &c->grain_offs_cap[par], std::max<size_t>(P.goffs, 256))))
+            return rc;
+    }
     for (int i = 0; i < n; i++) {
         FrameDesc &D = P.descs[i];
         char *base = c->scratch + scr_off[i];
@@ -518,7 +562,10 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         }
         for (int p = 0; p < 3; p++)
             if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = c->lr_bnd + bnd_off[i * 3 + p];
-        if (grain_idx[i] >= 0) D.gr.templates = c->templates + (size_t)grain_idx[i] * 3 * 73 * 82;
+        if (grain_idx[i] >= 0) {
+            D.gr.templates = c->templates[par] + (size_t)grain_idx[i] * 3 * 73 * 82;
+            D.gr.offsets = c->grain_offs[par] + goff[i];
+        }
     }
     // ---- pack descriptors + segment tables into one slot
     SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain };
@@ -548,19 +595,25 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
     }
     char *d = c->meta_dev[slot];
-    HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
     const FrameDesc *dd = reinterpret_cast<const FrameDesc *>(d);
     auto segp = [&](int k) { return reinterpret_cast<const Seg *>(d + offs[k]); };
     auto nseg = [&](int k) { return (int)lists[k]->v.size(); };
-    // ---- film-grain templates on the side stream
-    if (P.tl.total) {
-        HIPCHK(c, hipEventRecord(c->fork, s));
-        HIPCHK(c, hipStreamWaitEvent(c->side, c->fork, 0));
+    if (side) {
+        // Film-grain templates depend only on the grain parameters: the side
+        // stream uploads the descriptors and builds them without waiting for
+        // the previous call's filtering, only for the grain apply that last
+        // read this template pair.
+        if (c->tmpl_free_used[par]) HIPCHK(c, hipStreamWaitEvent(c->side, c->tmpl_free[par], 0));
+        HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, c->side));
+        HIPCHK(c, hipEventRecord(c->uploaded, c->side));
+        HIPCHK(c, hipStreamWaitEvent(s, c->uploaded, 0));
         {
             StageTimer t(c, ST_GRAIN_T, c->side, 0);
             HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
         }
         HIPCHK(c, hipEventRecord(c->join, c->side));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
     }
     if (P.dbv.total || P.dbh.total) {
         StageTimer t(c, ST_DEBLOCK, s, P.bytes_db);
@@ -583,10 +636,14 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, s));
         HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, VIEW_A, VIEW_D, s));
     }
+    if (side) HIPCHK(c, hipStreamWaitEvent(s, c->join, 0));
     if (P.grain.total) {
-        HIPCHK(c, hipStreamWaitEvent(s, c->join, 0));
         StageTimer t(c, ST_GRAIN_A, s, P.bytes_grain);
         HIPCHK(c, launch_grain_apply(dd, segp(9), nseg(9), P.grain.total, s));
+    }
+    if (side) {
+        HIPCHK(c, hipEventRecord(c->tmpl_free[par], s));
+        c->tmpl_free_used[par] = true;
     }
     HIPCHK(c, hipEventRecord(c->meta_done[slot], s));
     c->meta_used[slot] = true;
@@ -616,10 +673,18 @@ int av1pf_create(av1pf_ctx **out, int device)
     c->device = device;
     bool ok = hipMalloc(&c->dev_err, sizeof(int)) == hipSuccess &&
               hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->uploaded, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->tmpl_free[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->tmpl_free[1], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess;
-    for (int i = 0; ok && i < kSlots; i++)
-        ok = hipEventCreateWithFlags(&c->meta_done[i], hipEventDisableTiming) == hipSuccess;
+    // descriptor slots sized for large batches up front, so that no call in
+    // steady state allocates
+    for (int i = 0; ok && i < kSlots; i++) {
+        ok = hipEventCreateWithFlags(&c->meta_done[i], hipEventDisableTiming) == hipSuccess &&
+             hipMalloc(&c->meta_dev[i], kSlotBytes) == hipSuccess &&
+             hipHostMalloc(&c->meta_host[i], kSlotBytes, hipHostMallocDefault) == hipSuccess;
+        if (ok) c->meta_cap[i] = kSlotBytes;
+    }
     if (ok) {
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, nullptr, c->dev_err);
         ok = hipDeviceSynchronize() == hipSuccess;
@@ -639,15 +704,19 @@ void av1pf_destroy(av1pf_ctx *c)
     hipDeviceSynchronize();
     if (c->scratch) hipFree(c->scratch);
     if (c->lr_bnd) hipFree(c->lr_bnd);
-    if (c->templates) hipFree(c->templates);
     if (c->dev_err) hipFree(c->dev_err);
     for (int i = 0; i < kSlots; i++) {
         if (c->meta_dev[i]) hipFree(c->meta_dev[i]);
         if (c->meta_host[i]) hipHostFree(c->meta_host[i]);
         if (c->meta_done[i]) hipEventDestroy(c->meta_done[i]);
     }
-    if (c->fork) hipEventDestroy(c->fork);
+    if (c->uploaded) hipEventDestroy(c->uploaded);
     if (c->join) hipEventDestroy(c->join);
+    for (int i = 0; i < 2; i++) {
+        if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
+        if (c->templates[i]) hipFree(c->templates[i]);
+        if (c->grain_offs[i]) hipFree(c->grain_offs[i]);
+    }
     if (c->side) hipStreamDestroy(c->side);
     for (auto &r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->pool) hipEventDestroy(e);
@@ -721,14 +790,14 @@ int av1pf_lr_save_boundaries(av1pf_ctx *c, const av1pf_frame *frame, const av1pf
     if (after_cdef) return AV1PF_OK;
     HIPCHK(c, hipSetDevice(c->device));
     BatchPlan P;
-    std::vector<size_t> scr_off, bnd_off;
+    std::vector<size_t> scr_off, bnd_off, goff;
     std::vector<int> gi;
     av1pf_request r{};
     r.stages = AV1PF_STAGE_LR;
     r.lr = p;
     // a request with only the save: run the chain's save segment by planning an
     // LR request and launching just its save list
-    if ((rc = plan_batch(c, 1, frame, &r, P, scr_off, bnd_off, gi, false))) return rc;
+    if ((rc = plan_batch(c, 1, frame, &r, P, scr_off, bnd_off, gi, goff, false))) return rc;
     if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
     for (int pl = 0; pl < 3; pl++)
         if (P.descs[0].lr[pl].type != AV1PF_LR_NONE) P.descs[0].lr[pl].bnd = c->lr_bnd + bnd_off[pl];
@@ -817,6 +886,13 @@ int av1pf_profile(av1pf_ctx *c, int enable)
     c->recs.clear();
     for (int i = 0; i < AV1PF_PROFILE_STAGES; i++) { c->acc_ms[i] = 0; c->acc_n[i] = 0; c->acc_bytes[i] = 0; }
     c->profiling = enable != 0;
+    // events are created up front: creating them inside a timed region costs
+    // more host time than the launches they bracket
+    while (c->profiling && c->pool.size() < 512) {
+        hipEvent_t e = nullptr;
+        HIPCHK(c, hipEventCreate(&e));
+        c->pool.push_back(e);
+    }
     return AV1PF_OK;
 }
 

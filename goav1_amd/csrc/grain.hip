@@ -13,17 +13,21 @@
 //     (apply.go:72-310,480-549, blend.go, sample.go, scaling.go:55-77).
 //
 // Templates: one wave per template.  Each lane fills template rows of Gaussian
-// samples after jumping the LFSR to the row start with a GF(2) matrix power
-// (the LFSR step is linear over GF(2)).  The AR recursion, sequential in the
-// reference, runs as a diagonal wavefront: cell (y, x) only needs earlier rows
-// within 3 columns and the 3 cells to its left, so all cells with x + 4y = t
-// are independent (~350 steps for luma).  A lane owns rows y and y + 35, which
-// are never active in the same step.  The lag is a template parameter, so the
-// taps are unrolled with the coefficients in registers.  Templates depend only
-// on the grain parameters, so the host runs them on a side stream overlapped
-// with deblock / CDEF / LR.
+// samples from the LFSR state at its row start (the host jumps the LFSR with a
+// GF(2) matrix power; the LFSR step is linear over GF(2)).  The AR recursion,
+// sequential in the reference, runs as a diagonal wavefront: cell (y, x) only
+// needs earlier rows within LAG columns and the LAG cells to its left, so all
+// cells with x + (LAG+1)*y = t are independent.  A lane owns a row and keeps
+// the windows of the rows above in registers, fed by DPP from the lane above,
+// so a step has no LDS round trip and no barrier.  The lag is a template
+// parameter, so the taps are unrolled with the coefficients in registers.
+// Templates depend only on the grain parameters, so the host runs them on a
+// side stream overlapped with deblock / CDEF / LR.
+// The luma template kernel also draws the per-block random offsets (one LFSR
+// walk per 32-line stripe) into a small table.
 // Apply: one workgroup per 32x32 luma block and its chroma blocks (the offsets
-// of block (row, col) are shared by all planes); scaling LUTs in LDS.
+// of block (row, col) are shared by all planes); scaling LUTs in LDS.  Planes
+// without grain are copied, so the display frame is complete.
 #include "pf_internal.h"
 #include "av1pf_gaussian_table.h"
 
@@ -43,23 +47,12 @@ __device__ __forceinline__ uint16_t lfsr_step(uint16_t r)
     return (uint16_t)((r >> 1) | (bit << 15));
 }
 
-__device__ __forceinline__ uint16_t gf2_apply(const uint16_t *J, uint16_t s)
+// Gaussian fill of one template (rows [0,h), cols [0,w)) by a 64-lane wave;
+// lane r starts from the LFSR state at the start of row r (host-computed).
+__device__ __forceinline__ void gauss_fill(int16_t *T, const uint16_t *row_seed, int w, int h, int shift)
 {
-    uint16_t o = 0;
-#pragma unroll
-    for (int b = 0; b < 16; b++) o ^= (uint16_t)(-(int)((s >> b) & 1) & J[b]);
-    return o;
-}
-
-// Gaussian fill of one template (rows [0,h), cols [0,w)) by a 64-lane wave.
-__device__ __forceinline__ void gauss_fill(int16_t *T, uint16_t seed, const uint16_t *Jg, int w, int h, int shift)
-{
-    uint16_t J[16];
-#pragma unroll
-    for (int b = 0; b < 16; b++) J[b] = Jg[b];
     for (int row = threadIdx.x; row < h; row += 64) {
-        uint16_t s = seed;
-        for (int k = 0; k < row; k++) s = gf2_apply(J, s);
+        uint16_t s = row_seed[row];
         for (int x = 0; x < w; x++) {
             s = lfsr_step(s);
             T[row * GW + x] = (int16_t)r2i(kGauss[s >> 5], shift);
@@ -67,54 +60,121 @@ __device__ __forceinline__ void gauss_fill(int16_t *T, uint16_t seed, const uint
     }
 }
 
-// One AR cell (luma.go:60-74, chroma.go:69-87) with the lag unrolled.
-template <int LAG, bool CHROMA>
-__device__ __forceinline__ void ar_cell(int16_t *T, int y, int x, const int *c, int shift, int gmin, int gmax,
-                                        const int16_t *L, bool luma_term, int ssx, int ssy)
+// Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 is overwritten by
+// the caller.
+__device__ __forceinline__ int from_prev_lane(int v)
 {
-    int sum = 0;
-    int pos = 0;
-#pragma unroll
-    for (int dr = -LAG; dr <= 0; dr++) {
-#pragma unroll
-        for (int dc = -LAG; dc <= LAG; dc++) {
-            if (dr == 0 && dc == 0) break;
-            sum += c[pos] * T[(y + dr) * GW + x + dc];
-            pos++;
-        }
-    }
-    if (CHROMA && luma_term) {
-        const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
-        int s = L[ly * GW + lx];
-        if (ssx) s += L[ly * GW + lx + 1];
-        if (ssy) {
-            s += L[(ly + 1) * GW + lx];
-            if (ssx) s += L[(ly + 1) * GW + lx + 1];
-        }
-        sum += r2i(s, ssx + ssy) * c[pos];
-    }
-    T[y * GW + x] = (int16_t)clipi(T[y * GW + x] + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
 }
 
+// Auto-regressive filter (luma.go:60-74, chroma.go:69-87) over rows
+// [y0, y0 + nrows), nrows <= 64, by one wave.  Lane l owns row y0 + l and at
+// step t works on column x = t - (LAG+1)*l: every cell it needs from the rows
+// above was finished at an earlier step, and its own row's left cells are its
+// own previous results.  The (2*LAG+1)-wide windows of the LAG rows above live
+// in registers and advance one column per step; the new right-hand column
+// comes from the lane above through DPP (its newest result and the centres of
+// its own windows), and lane 0 reads it from the rows above the pass in LDS,
+// which are final.  T holds the Gaussian noise on entry and the filtered
+// template on exit for the pass's rows.
 template <int LAG, bool CHROMA>
-__device__ void ar_wavefront(int16_t *T, int w, int h, const int8_t *coef, int shift, int gmin, int gmax,
-                             const int16_t *L, bool luma_term, int ssx, int ssy)
+__device__ void ar_rows(int16_t *T, const int16_t *L, int w, int y0, int nrows, const int *c, int shift, int gmin,
+                        int gmax, bool luma_term, int ssx, int ssy)
+{
+    constexpr int K = LAG + 1, WIN = 2 * LAG + 1;
+    const int lane = threadIdx.x;
+    const int y = y0 + lane;
+    const bool live = lane < nrows;
+    int A[LAG][WIN];
+    int own[LAG];
+#pragma unroll
+    for (int d = 0; d < LAG; d++) {
+#pragma unroll
+        for (int j = 0; j < WIN; j++) A[d][j] = 0;
+        own[d] = 0;
+    }
+    const int t_end = (w - 4) + K * (nrows - 1);
+    for (int t = -K; t <= t_end; t++) {
+        const int x = t - K * lane;
+        int v = (live && x >= 0 && x < w) ? T[y * GW + x] : 0;
+        if (live && x >= 3 && x < w - 3) {
+            int sum = 0, pos = 0;
+#pragma unroll
+            for (int dr = LAG; dr >= 1; dr--) {
+#pragma unroll
+                for (int j = 0; j < WIN; j++) sum += __mul24(c[pos++], A[dr - 1][j]);
+            }
+#pragma unroll
+            for (int j = 0; j < LAG; j++) sum += __mul24(c[pos++], own[j]);
+            if (CHROMA && luma_term) {
+                const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
+                int a = L[ly * GW + lx];
+                if (ssx) a += L[ly * GW + lx + 1];
+                if (ssy) {
+                    a += L[(ly + 1) * GW + lx];
+                    if (ssx) a += L[(ly + 1) * GW + lx + 1];
+                }
+                sum += r2i(a, ssx + ssy) * c[pos];
+            }
+            v = clipi(v + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
+            T[y * GW + x] = (int16_t)v;
+        }
+        // right-hand column (x + 1 + LAG) of the windows for the next step
+        int nv = from_prev_lane(v);
+        int na[LAG > 1 ? LAG - 1 : 1];
+#pragma unroll
+        for (int d = 0; d + 1 < LAG; d++) na[d] = from_prev_lane(A[d][LAG]);
+        if (lane == 0) {
+            const int xx = x + 1 + LAG;
+            const bool ok = xx >= 0 && xx < w;
+            nv = ok ? T[(y - 1) * GW + xx] : 0;
+#pragma unroll
+            for (int d = 0; d + 1 < LAG; d++) na[d] = ok ? T[(y - 2 - d) * GW + xx] : 0;
+        }
+#pragma unroll
+        for (int d = 0; d < LAG; d++) {
+#pragma unroll
+            for (int j = 0; j + 1 < WIN; j++) A[d][j] = A[d][j + 1];
+        }
+        if (LAG > 0) A[0][WIN - 1] = nv;
+#pragma unroll
+        for (int d = 1; d < LAG; d++) A[d][WIN - 1] = na[d - 1];
+#pragma unroll
+        for (int j = 0; j + 1 < LAG; j++) own[j] = own[j + 1];
+        if (LAG > 0) own[LAG - 1] = v;
+    }
+}
+
+// Lag 0 (chroma with only the luma term): every cell is independent.
+__device__ void ar_lag0(int16_t *T, const int16_t *L, int w, int h, int c0, int shift, int gmin, int gmax, int ssx,
+                        int ssy)
+{
+    const int cols = w - 6;
+    for (int i = threadIdx.x; i < (h - 3) * cols; i += 64) {
+        const int y = 3 + i / cols, x = 3 + i % cols;
+        const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
+        int a = L[ly * GW + lx];
+        if (ssx) a += L[ly * GW + lx + 1];
+        if (ssy) {
+            a += L[(ly + 1) * GW + lx];
+            if (ssx) a += L[(ly + 1) * GW + lx + 1];
+        }
+        const int sum = r2i(a, ssx + ssy) * c0;
+        T[y * GW + x] = (int16_t)clipi(T[y * GW + x] + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
+    }
+}
+
+// Rows 3 .. h-1 in passes of up to 64 rows (luma: 3..66, then 67..72).
+template <int LAG, bool CHROMA>
+__device__ void ar_template(int16_t *T, const int16_t *L, int w, int h, const int8_t *coef, int shift, int gmin,
+                            int gmax, bool luma_term, int ssx, int ssy)
 {
     constexpr int NC = 2 * LAG * (LAG + 1) + (CHROMA ? 1 : 0);
     int c[NC > 0 ? NC : 1];
 #pragma unroll
     for (int i = 0; i < NC; i++) c[i] = coef[i];
-    const int y1 = 3 + threadIdx.x, y2 = y1 + 35;
-    const int tmax = (w - 4) + 4 * (h - 1);
-    for (int t = 15; t <= tmax; t++) {
-        if (threadIdx.x < 35) {
-            int x = t - 4 * y1;
-            if (y1 < h && x >= 3 && x < w - 3)
-                ar_cell<LAG, CHROMA>(T, y1, x, c, shift, gmin, gmax, L, luma_term, ssx, ssy);
-            x = t - 4 * y2;
-            if (y2 < h && x >= 3 && x < w - 3)
-                ar_cell<LAG, CHROMA>(T, y2, x, c, shift, gmin, gmax, L, luma_term, ssx, ssy);
-        }
+    for (int y0 = 3; y0 < h; y0 += 64) {
+        ar_rows<LAG, CHROMA>(T, L, w, y0, min(64, h - y0), c, shift, gmin, gmax, luma_term, ssx, ssy);
         __syncthreads();
     }
 }
@@ -122,19 +182,36 @@ __device__ void ar_wavefront(int16_t *T, int w, int h, const int8_t *coef, int s
 __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
 {
     __shared__ int16_t T[GH * GW];
+    // one wave on a serial dependency chain shares its SIMD with the streaming
+    // kernels of the main stream: let it issue first
+    __builtin_amdgcn_s_setprio(3);
     const FrameDesc &D = descs[segs[blockIdx.x].frame];
     const GrainDesc &G = D.gr;
     const int bd = D.v[VIEW_A].bd;
     for (int i = threadIdx.x; i < GH * GW; i += 64) T[i] = 0;
+    // per-block random offsets: one LFSR per 32-line stripe, one draw per
+    // 32-column block (apply.go:43-70, random.go:26-48)
+    for (int r = threadIdx.x; r < G.orows; r += 64) {
+        uint16_t reg = G.g.seed;
+        reg ^= (uint16_t)(((r * 37 + 178) & 255) << 8);
+        reg ^= (uint16_t)((r * 173 + 105) & 255);
+        uint8_t *o = G.offsets + (size_t)r * G.ocols;
+        for (int c = 0; c < G.ocols; c++) {
+            reg = lfsr_step(reg);
+            o[c] = (uint8_t)(reg >> 8);
+        }
+    }
     __syncthreads();
     if (G.g.num_y_points) {
-        gauss_fill(T, G.g.seed, G.jump_luma, GW, GH, 12 - bd + G.g.grain_scale_shift);
+        gauss_fill(T, G.row_seed[0], GW, GH, 12 - bd + G.g.grain_scale_shift);
         __syncthreads();
         const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
+        const int8_t *cf = G.g.ar_coeffs_y;
+        const int sh = G.g.ar_coeff_shift;
         switch (G.g.ar_coeff_lag) {
-        case 1: ar_wavefront<1, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
-        case 2: ar_wavefront<2, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
-        case 3: ar_wavefront<3, false>(T, GW, GH, G.g.ar_coeffs_y, G.g.ar_coeff_shift, gmin, gmax, nullptr, false, 0, 0); break;
+        case 1: ar_template<1, false>(T, nullptr, GW, GH, cf, sh, gmin, gmax, false, 0, 0); break;
+        case 2: ar_template<2, false>(T, nullptr, GW, GH, cf, sh, gmin, gmax, false, 0, 0); break;
+        case 3: ar_template<3, false>(T, nullptr, GW, GH, cf, sh, gmin, gmax, false, 0, 0); break;
         default: break;
         }
     }
@@ -146,6 +223,7 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
 {
     __shared__ int16_t T[GH * GW];
     __shared__ int16_t L[GH * GW];
+    __builtin_amdgcn_s_setprio(3);
     const Seg sg = segs[blockIdx.x];
     const FrameDesc &D = descs[sg.frame];
     const GrainDesc &G = D.gr;
@@ -159,18 +237,18 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
         L[i] = luma_term ? G.templates[i] : (int16_t)0;
     }
     __syncthreads();
-    gauss_fill(T, (uint16_t)(G.g.seed ^ (pl == 1 ? 0xb524 : 0x49d8)), G.jump_chroma, cw, ch, 12 - bd + G.g.grain_scale_shift);
+    gauss_fill(T, G.row_seed[pl], cw, ch, 12 - bd + G.g.grain_scale_shift);
     __syncthreads();
     const int lag = G.g.ar_coeff_lag;
     if (lag || luma_term) {
-        const int8_t *coef = pl == 1 ? G.g.ar_coeffs_cb : G.g.ar_coeffs_cr;
+        const int8_t *cf = pl == 1 ? G.g.ar_coeffs_cb : G.g.ar_coeffs_cr;
         const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
         const int sh = G.g.ar_coeff_shift;
         switch (lag) {
-        case 0: ar_wavefront<0, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
-        case 1: ar_wavefront<1, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
-        case 2: ar_wavefront<2, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
-        default: ar_wavefront<3, true>(T, cw, ch, coef, sh, gmin, gmax, L, luma_term, ssx, ssy); break;
+        case 0: ar_lag0(T, L, cw, ch, cf[0], sh, gmin, gmax, ssx, ssy); break;
+        case 1: ar_template<1, true>(T, L, cw, ch, cf, sh, gmin, gmax, luma_term, ssx, ssy); break;
+        case 2: ar_template<2, true>(T, L, cw, ch, cf, sh, gmin, gmax, luma_term, ssx, ssy); break;
+        default: ar_template<3, true>(T, L, cw, ch, cf, sh, gmin, gmax, luma_term, ssx, ssy); break;
         }
     }
     __syncthreads();
@@ -210,7 +288,6 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
     const int bd = src.bd;
     const int scaling_shift = g.scaling_shift;
     for (int pl = 0; pl < src.nplanes; pl++) {
-        if (!G.active[pl]) continue;
         const int sx = pl ? src.ssx : 0, sy = pl ? src.ssy : 0;
         const PlaneView &sp = src.p[pl];
         const PlaneView &dp = dst.p[pl];
@@ -219,6 +296,13 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         const int pw = sp.w, ph = sp.h;
         if (px0 >= pw || py0 >= ph) continue;
         const int bwid = min(bw, pw - px0), height = min(bh, ph - py0);
+        if (!G.active[pl]) {   // plane without grain: the display gets the filtered samples
+            for (int i = tid; i < bwid * height; i += 256) {
+                const int y = i / bwid, x = i - y * bwid;
+                row_ptr<T>(dp, py0 + y)[px0 + x] = row_ptr<T>(sp, py0 + y)[px0 + x];
+            }
+            continue;
+        }
         const int ys = (g.overlap && brow > 0) ? min(height, 2 >> sy) : 0;
         const int xs = (g.overlap && bcol > 0) ? min(bwid, 2 >> sx) : 0;
         const int16_t *tmpl = G.templates + pl * GH * GW;
@@ -286,24 +370,10 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__res
     const int bcol = t % sg.nx, brow = t / sg.nx;
     const int tid = threadIdx.x;
     for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = G.luts[i >> 8][i & 255];
-    if (tid == 0) {
-        const av1pf_grain_params &g = G.g;
-        const int nr = g.overlap && brow > 0 ? 2 : 1;
-        for (int i = 0; i < 2; i++) { offs[0][i] = 0; offs[1][i] = 0; }
-        for (int i = 0; i < nr; i++) {
-            const int num = brow - i;
-            uint16_t reg = g.seed;
-            reg ^= (uint16_t)(((num * 37 + 178) & 255) << 8);
-            reg ^= (uint16_t)((num * 173 + 105) & 255);
-            uint8_t prev = 0, cur = 0;
-            for (int c = 0; c <= bcol; c++) {
-                reg = lfsr_step(reg);
-                prev = cur;
-                cur = (uint8_t)(reg >> 8);
-            }
-            offs[0][i] = cur;
-            offs[1][i] = (g.overlap && bcol > 0) ? prev : 0;
-        }
+    if (tid < 4) {   // offsets of this block and its left / top / top-left neighbours
+        const int i = tid & 1, left = tid >> 1;
+        const bool use = (i == 0 || (G.g.overlap && brow > 0)) && (!left || (G.g.overlap && bcol > 0));
+        offs[left][i] = use ? G.offsets[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
     }
     __syncthreads();
     const FrameView &src = D.v[VIEW_A];

@@ -62,9 +62,11 @@ struct LrPlaneDesc {
 struct GrainDesc {
     av1pf_grain_params g;
     int active[3];
-    uint16_t jump_luma[16], jump_chroma[16];   // GF(2) LFSR jumps over one template row
+    uint16_t row_seed[3][73];      // LFSR state at the start of each template row
     uint8_t luts[3][256];
     int16_t *templates;            // 3 x 73 x 82
+    uint8_t *offsets;              // [orows][ocols] per-32x32-block random offsets
+    int orows, ocols;
     int matrix_identity;
 };
 

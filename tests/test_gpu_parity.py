@@ -184,3 +184,19 @@ def test_stage_entry_points(runner, bd):
     want, wdisp = orc.run_chain(fr)
     assert_same(_frames_from(fr, d), want, "stage-level chain")
     assert_same(_frames_from(fr, disp), wdisp, "stage-level display")
+
+
+@pytest.mark.parametrize("lag", [0, 1, 2, 3])
+@pytest.mark.parametrize("ss", [(1, 1), (1, 0), (0, 0)], ids=["420", "422", "444"])
+def test_film_grain_lags(runner, lag, ss):
+    """Every AR lag on every chroma layout (template rows in one and two
+    passes of the wavefront)."""
+    rng = np.random.default_rng(31 * lag + 7 * ss[0] + ss[1])
+    fr = synth.FrameData(150, 94, 10, ss[0], ss[1])
+    synth.fill_blocky(fr, rng)
+    fr.grain = synth.random_grain(rng, 10, lag=lag)
+    fr.stages = abi.STAGE_FILMGRAIN
+    print(f"grain lag={lag} ss={ss}", flush=True)
+    want, wdisp = orc.run_chain(fr)
+    got, gdisp = gpu_run(runner, fr)
+    assert_same(gdisp, wdisp, f"display lag {lag} ss {ss}")
