@@ -21,7 +21,8 @@ _lib = None
 
 EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error", "av1pf_apply",
            "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
-           "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_profile", "av1pf_profile_read"]
+           "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_cdef_search_finish", "av1pf_cdef_lambda",
+           "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search"]
 
 
@@ -56,6 +57,10 @@ def load():
                                     C.POINTER(abi.GrainParams), C.c_void_p]
     lib.av1pf_cdef_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p,
                                       C.c_int, C.c_void_p, C.c_void_p]
+    lib.av1pf_cdef_search_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.POINTER(C.c_int)]
+    lib.av1pf_cdef_lambda.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.av1pf_cdef_lambda.restype = C.c_uint64
     lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
     lib.av1pf_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     _lib = lib
@@ -249,3 +254,25 @@ class PostFilterRunner:
     def cdef_search(self, src, rec, noskip_dev_ptr, damping, mse_dev_ptr, stream=None):
         self._check(self.lib.av1pf_cdef_search(self.h, C.byref(src.abi), C.byref(rec.abi), noskip_dev_ptr,
                                                damping, mse_dev_ptr, self._stream(stream)))
+
+
+def cdef_search_finish(mse, skip_fb, lam):
+    """Host finish of the CDEF search over an mse[2][nfb][64] uint64 table:
+    returns (bits, y_strength[8], uv_strength[8], fb_index[nfb])."""
+    lib = load()
+    m = np.ascontiguousarray(mse, dtype=np.uint64)
+    nfb = m.shape[1]
+    sk = np.ascontiguousarray(skip_fb, dtype=np.uint8)
+    ys = np.zeros(8, np.uint8)
+    uvs = np.zeros(8, np.uint8)
+    idx = np.zeros(max(nfb, 1), np.uint8)
+    bits = C.c_int()
+    rc = lib.av1pf_cdef_search_finish(m.ctypes.data, nfb, sk.ctypes.data, C.c_uint64(int(lam)), ys.ctypes.data,
+                                      uvs.ctypes.data, idx.ctypes.data, C.byref(bits))
+    if rc != abi.OK:
+        raise Av1pfError(rc, "av1pf_cdef_search_finish failed")
+    return bits.value, ys, uvs, idx[:nfb]
+
+
+def cdef_lambda(dc_quant, key_frame=True, gf_update=False, tl_factor=128):
+    return int(load().av1pf_cdef_lambda(int(dc_quant), int(key_frame), int(gf_update), int(tl_factor)))

@@ -394,6 +394,43 @@ def sequence_1080p10(seed=1, nframes=8, w=1920, h=1080, grain=True):
     return frames
 
 
+def search_pair(seed=3, w=1280, h=720, q=160, bd=8, skip_prob=0.15, skip_fb_prob=0.05, src=None):
+    """Encoder-search input: a source frame (the 720p keyframe pixel recipe, or
+    `src` when given) and a reconstruction that carries quantisation-like
+    error (per-8x8 smoothing toward the block mean plus noise that grows with
+    the quantiser), the per-64x64 non-skip bit masks (bit r*8+c = 8x8 block
+    (r, c) has a non-skip MI; some filter blocks have none) and the damping
+    3 + (q >> 6) of the search."""
+    rng = np.random.default_rng(seed)
+    if src is None:
+        src = FrameData(w, h, bd, 1, 1)
+        fill_blocky(src, rng)
+    rec = src.copy_pixels()
+    step = 2 + q / 24.0
+    maxv = (1 << src.bd) - 1
+    for ps, pr in zip(src.planes, rec.planes):
+        x = ps.pix.astype(np.float64)
+        H, W = x.shape
+        hb, wb = (H + 7) // 8, (W + 7) // 8
+        pad = np.pad(x, ((0, hb * 8 - H), (0, wb * 8 - W)), mode="edge")
+        mean = pad.reshape(hb, 8, wb, 8).mean(axis=(1, 3))
+        mean = np.repeat(np.repeat(mean, 8, 0), 8, 1)[:H, :W]
+        y = x + 0.35 * (mean - x) + rng.normal(0, step * 0.6, size=x.shape)
+        pr.pix[:] = np.clip(np.rint(y), 0, maxv).astype(np.uint16)
+    mi_cols, mi_rows = aligned8(src.w) >> 2, aligned8(src.h) >> 2
+    nhfb, nvfb = (mi_cols + 15) // 16, (mi_rows + 15) // 16
+    bits = (rng.random((nvfb * nhfb, 64)) >= skip_prob).astype(np.uint64)
+    bits[rng.random(nvfb * nhfb) < skip_fb_prob] = 0
+    noskip = (bits << np.arange(64, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+    return src, rec, noskip, 3 + (q >> 6)
+
+
+def search_dc_quant(q):
+    """8-bit luma DC quantiser of the search workloads, pinned from AV1's
+    normative dc_qlookup table: q=120 -> 123, q=160 -> 223."""
+    return {120: 123, 160: 223}[q]
+
+
 BATCH_SIZES = [(426, 240), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
 

@@ -215,6 +215,19 @@ int  av1pf_filmgrain(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *
 int  av1pf_cdef_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
                        const uint64_t *noskip, int damping, uint64_t *mse, void *stream);
 
+/* Host finish of the search (finish / jointStrengthSearchDual / searchOneDual,
+ * cdef_search.go:780-945): from mse[2][nfb][64] in HOST memory pick cdef_bits
+ * (0..3), the 1 << bits packed luma / chroma strengths (primary * 4 +
+ * secondary code; unused entries 0) and each filter block's index.  Filter
+ * blocks with skip_fb[fb] != 0 (no non-skip 8x8 block) are left out and get
+ * index 0.  Needs no device.                                                */
+int  av1pf_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda,
+                              uint8_t y_strength[8], uint8_t uv_strength[8], uint8_t *fb_index, int *cdef_bits);
+
+/* Full-search lambda (svtCDEFFullLambda, cdef_search.go:274-299) from the
+ * 8-bit luma DC quantizer of the frame's qindex.  Needs no device.          */
+uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_factor);
+
 /* ---- per-stage timing (no counterpart in the reference, whose closest aid is
  * the FrameWorkPostFilterResult counters, postfilter.go:309-316) ----------
  * While enabled, every stage launch is bracketed by HIP events on its stream

@@ -1,0 +1,167 @@
+"""Encoder CDEF strength search (cdef_search.go:580-945).
+
+CPU tests pin the oracle's MSE table two independent ways (the unfiltered
+trial equals a plain block SSE; every trial equals the SSE of the decoder's
+CDEF at that strength, which the decoder-side tests pin) and check the
+library's host finish against the oracle's finish and against invariants of
+the selection.  GPU tests require the device MSE table to equal the oracle's
+bit for bit and the finish decisions to be identical.
+"""
+import numpy as np
+import pytest
+
+from goav1_amd import abi, synth
+from oracle import orc
+
+# search-level-1 strength list: position -> packed (primary * 4 + secondary code)
+LEVEL1 = [4 * g if g < 16 else 4 * ((g - 16) // 3) + (g - 16) % 3 + 1 for g in range(64)]
+
+
+def test_level1_list_is_every_pair_once():
+    assert sorted(LEVEL1) == list(range(64))
+    assert LEVEL1[:16] == [4 * p for p in range(16)]
+
+
+def block_list(fr, noskip, fb):
+    mi_cols, mi_rows = synth.aligned8(fr.w) >> 2, synth.aligned8(fr.h) >> 2
+    nhfb = (mi_cols + 15) // 16
+    fr_, fc_ = divmod(fb, nhfb)
+    out = []
+    for r in range(8):
+        for c in range(8):
+            if (int(noskip[fb]) >> (r * 8 + c)) & 1 and fr_ * 16 + 2 * r < mi_rows and fc_ * 16 + 2 * c < mi_cols:
+                out.append((fr_ * 8 + r, fc_ * 8 + c))
+    return out
+
+
+def block_sse(a, b, fr, blocks):
+    """SSE over 8x8 luma blocks (8x8 block units) and their 4x4 chroma blocks (U + V)."""
+    sy = sc = 0
+    for by, bx in blocks:
+        d = a.planes[0].pix[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64) - \
+            b.planes[0].pix[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64)
+        sy += int((d * d).sum())
+        for p in (1, 2):
+            d = a.planes[p].pix[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].astype(np.int64) - \
+                b.planes[p].pix[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4].astype(np.int64)
+            sc += int((d * d).sum())
+    return sy, sc
+
+
+@pytest.mark.parametrize("size", [(128, 128), (200, 136)])
+def test_unfiltered_trial_is_block_sse(size):
+    src, rec, noskip, damping = synth.search_pair(seed=11, w=size[0], h=size[1])
+    mse = orc.cdef_search_mse(src, rec, noskip, damping)
+    for fb in range(mse.shape[1]):
+        sy, sc = block_sse(src, rec, src, block_list(src, noskip, fb))
+        assert (int(mse[0, fb, 0]), int(mse[1, fb, 0])) == (sy, sc), f"fb {fb}"
+
+
+@pytest.mark.parametrize("size", [(128, 128), (200, 136)])
+def test_every_trial_equals_decoder_cdef_sse(size):
+    """The search's trial gi is the decoder's CDEF with strength LEVEL1[gi] on
+    every filter block, measured over the non-skip blocks."""
+    src, rec, noskip, damping = synth.search_pair(seed=12, w=size[0], h=size[1])
+    mse = orc.cdef_search_mse(src, rec, noskip, damping)
+    ucols, urows = (rec.w + 63) // 64, (rec.h + 63) // 64
+    assert mse.shape[1] == ucols * urows
+    idx = np.where(noskip == 0, -1, 0).astype(np.int8)
+    for gi in range(64):
+        fr = rec.copy_pixels()
+        fr.deblock = fr.lr = fr.grain = None
+        ys = np.zeros(8, np.uint8)
+        ys[0] = LEVEL1[gi]
+        fr.cdef = dict(damping=damping, strength_count=1, y_strength=ys, uv_strength=ys.copy(),
+                       unit_cols=ucols, unit_rows=urows, index=idx, noskip=np.ascontiguousarray(noskip))
+        out, _ = orc.run_chain(fr, abi.STAGE_CDEF)
+        for fb in range(mse.shape[1]):
+            sy, sc = block_sse(src, out, src, block_list(src, noskip, fb))
+            assert (int(mse[0, fb, gi]), int(mse[1, fb, gi])) == (sy, sc), f"gi {gi} fb {fb}"
+
+
+def random_tables(rng, nfb, hi):
+    mse = rng.integers(0, hi, size=(2, nfb, 64)).astype(np.uint64)
+    skip = (rng.random(nfb) < 0.1).astype(np.uint8)
+    mse[:, skip.astype(bool), :] = 0
+    return mse, skip
+
+
+def test_lambda_value():
+    from goav1_amd.lib import cdef_lambda
+    # q160 key frame: dc 223 -> (3.3 + 0.0015*223) * 223^2 = 180740.1 -> 180740;
+    # * 150 >> 7 = 211804; * 100 >> 7 = 165471
+    assert cdef_lambda(synth.search_dc_quant(160), key_frame=True) == 165471
+    assert cdef_lambda(1, key_frame=False, tl_factor=1) == 1
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_finish_matches_oracle(seed):
+    from goav1_amd.lib import cdef_lambda, cdef_search_finish
+    rng = np.random.default_rng(seed)
+    nfb = int(rng.integers(1, 300))
+    hi = [8, 1000, 1 << 20, 1 << 30, 50, 1 << 12][seed]     # small ranges force ties
+    mse, skip = random_tables(rng, nfb, hi)
+    for lam in (0, cdef_lambda(223), 1 << 30):
+        got = cdef_search_finish(mse, skip, lam)
+        want = orc.cdef_search_finish(mse, skip, lam)
+        assert got[0] == want[0]
+        np.testing.assert_array_equal(got[1], want[1])
+        np.testing.assert_array_equal(got[2], want[2])
+        np.testing.assert_array_equal(got[3], want[3])
+
+
+def test_finish_invariants():
+    from goav1_amd.lib import cdef_search_finish
+    rng = np.random.default_rng(99)
+    mse, skip = random_tables(rng, 120, 1 << 16)
+    inv = {p: g for g, p in enumerate(LEVEL1)}
+    # a large lambda (rate * lambda still inside int64) forces one pair: the
+    # lowest luma and chroma column sums
+    bits, ys, uvs, idx = cdef_search_finish(mse, skip, 1 << 30)
+    act = ~skip.astype(bool)
+    assert bits == 0
+    assert inv[int(ys[0])] == int(np.argmin(mse[0, act].sum(axis=0)))
+    assert inv[int(uvs[0])] == int(np.argmin(mse[1, act].sum(axis=0)))
+    # with lambda 0 every active block takes its cheapest chosen pair
+    bits, ys, uvs, idx = cdef_search_finish(mse, skip, 0)
+    nb = 1 << bits
+    assert all(ys[nb:] == 0) and all(uvs[nb:] == 0)
+    for fb in range(mse.shape[1]):
+        costs = [int(mse[0, fb, inv[int(ys[g])]]) + int(mse[1, fb, inv[int(uvs[g])]]) for g in range(nb)]
+        assert idx[fb] == (0 if skip[fb] else int(np.argmin(costs)))
+
+
+# ---------------------------------------------------------------- GPU parity
+def gpu_search(src, rec, noskip, damping):
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        mse = torch.zeros((2, nfb, 64), dtype=torch.int64, device="cuda")
+        r.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        return mse.cpu().numpy().view(np.uint64)
+    finally:
+        r.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [(1280, 720), (200, 136), (64, 64)])
+def test_gpu_search_parity(size):
+    src, rec, noskip, damping = synth.search_pair(seed=21, w=size[0], h=size[1])
+    print(f"search {size} damping {damping}", flush=True)
+    got = gpu_search(src, rec, noskip, damping)
+    want = orc.cdef_search_mse(src, rec, noskip, damping)
+    np.testing.assert_array_equal(got, want)
+    from goav1_amd.lib import cdef_lambda, cdef_search_finish
+    lam = cdef_lambda(synth.search_dc_quant(160))
+    skip = (noskip == 0).astype(np.uint8)
+    g = cdef_search_finish(got, skip, lam)
+    w = orc.cdef_search_finish(want, skip, lam)
+    assert g[0] == w[0]
+    for a, b in zip(g[1:], w[1:]):
+        np.testing.assert_array_equal(a, b)
