@@ -38,7 +38,23 @@ def parse():
     ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-search", action="store_true", help="skip the encoder-search leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one plain 512 MiB device copy and exit (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """One device-to-device copy of 512 MiB: FETCH_SIZE / WRITE_SIZE of this
+    dispatch against its known 2 x 512 MiB give the counters' scale."""
+    import torch
+    n = 512 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n}), flush=True)
 
 
 def cpu_baseline(frames, seconds, threads):
@@ -64,6 +80,59 @@ def cpu_baseline(frames, seconds, threads):
                       f"oracle/liborc.so, {threads} thread(s), {el:.1f} s"}
 
 
+def search_leg(runner, steps, cpu):
+    """Exhaustive encoder CDEF search (64 luma x 64 chroma strength pairs per
+    64x64 filter block) on the 1280x720 8-bit 4:2:0 search workload at q160:
+    device MSE table + host finish per frame; luma Mpx/s."""
+    import numpy as np
+    import torch
+    from goav1_amd import synth
+    from goav1_amd.lib import DeviceFrame, cdef_lambda, cdef_search_finish
+    src, rec, noskip, damping = synth.search_pair(seed=3, w=1280, h=720, q=160)
+    ds, dr = DeviceFrame(src), DeviceFrame(rec)
+    ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+    nfb = noskip.shape[0]
+    mse = torch.zeros((2, nfb, 64), dtype=torch.int64, device="cuda")
+    lam = cdef_lambda(synth.search_dc_quant(160))
+    skip = (noskip == 0).astype(np.uint8)
+    for _ in range(3):
+        runner.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+    runner.sync()
+    torch.cuda.synchronize()
+    runner.profile(True)
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        runner.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    dev_ms = ev0.elapsed_time(ev1) / steps
+    prof = runner.profile_read()
+    runner.profile(False)
+    t0 = time.perf_counter()
+    table = mse.cpu().numpy().view(np.uint64)
+    decision = cdef_search_finish(table, skip, lam)
+    finish_ms = (time.perf_counter() - t0) * 1e3
+    px = src.w * src.h
+    out = {"workload": "1280x720 8-bit 4:2:0, q160 (damping 5), 64 strength pairs per 64x64 block, "
+                       "device MSE table + host finish",
+           "mpx_s": round(px / ((dev_ms + finish_ms) / 1e3) / 1e6, 2),
+           "device_mpx_s": round(px / (dev_ms / 1e3) / 1e6, 2),
+           "device_ms": round(dev_ms, 4), "finish_ms": round(finish_ms, 3),
+           "kernel_avg_us": round(1e3 * prof["cdef_search"][0] / prof["cdef_search"][1], 2),
+           "cdef_bits": decision[0]}
+    if cpu:
+        from oracle import orc
+        t0 = time.perf_counter()
+        orc.cdef_search_mse(src, rec, noskip, damping)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(px / el / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
+                               "sample": f"1 frame of the search workload, oracle MSE table, 1 thread, {el:.1f} s"}
+    return out
+
+
 def traffic_for(kernel):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -83,6 +152,9 @@ def main():
         print("bench.py measures one GPU: every box of this run has a single MI355X, and the "
               "post-filter path shards by frame with no exchange; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     import torch
     from goav1_amd import abi, synth
     from goav1_amd.lib import DeviceFrame, DeviceSideData, PostFilterRunner
@@ -192,6 +264,8 @@ def main():
                      "avg_launch_us": round(1e3 * ms / launches, 2)},
         "stages": stages,
     }
+    if not args.no_search:
+        out["search"] = search_leg(runner, args.steps, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds / 2, cores)
