@@ -105,3 +105,9 @@ class Request(C.Structure):
 assert C.sizeof(LFMask) == 1280
 assert C.sizeof(LRUnit) == 16
 assert C.sizeof(Frame) == 32 + 24 + 24
+
+
+class CdefDecision(C.Structure):
+    """av1pf_cdef_decision: the device strength selection's result."""
+    _fields_ = [("cdef_bits", C.c_int32), ("y_strength", C.c_uint8 * 8), ("uv_strength", C.c_uint8 * 8),
+                ("pad", C.c_uint8 * 4)]

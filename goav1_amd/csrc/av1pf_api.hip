@@ -51,6 +51,7 @@ struct av1pf_ctx {
     bool tmpl_free_used[2] = {};
     int tpar = 0;
     hipEvent_t uploaded = nullptr;
+    void *finish_state = nullptr;  // device CDEF strength selection state
     int *dev_err = nullptr;       // device-side validation flag
     hipStream_t side = nullptr;   // film-grain template stream
     hipEvent_t join = nullptr;
@@ -75,7 +76,7 @@ struct av1pf_ctx {
 
 namespace {
 
-enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH };
+enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -711,6 +712,7 @@ void av1pf_destroy(av1pf_ctx *c)
         if (c->meta_done[i]) hipEventDestroy(c->meta_done[i]);
     }
     if (c->uploaded) hipEventDestroy(c->uploaded);
+    if (c->finish_state) hipFree(c->finish_state);
     if (c->join) hipEventDestroy(c->join);
     for (int i = 0; i < 2; i++) {
         if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
@@ -874,6 +876,23 @@ int av1pf_cdef_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *r
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH, s, 2 * view_bytes(sv));
     HIPCHK(c, launch_cdef_search(sv, view_of(recon), noskip, damping, mse, nhfb, nvfb, s));
+    return AV1PF_OK;
+}
+
+int av1pf_cdef_search_select(av1pf_ctx *c, const av1pf_frame *recon, const uint64_t *noskip, const uint64_t *mse,
+                             uint64_t lambda, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream)
+{
+    if (!c || !mse || !decision || !fb_index) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_frame(c, recon);
+    if (rc) return rc;
+    const int mi_cols = ((recon->width + 7) >> 3) << 1, mi_rows = ((recon->height + 7) >> 3) << 1;
+    const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
+    if (nhfb * nvfb > kMaxSearchBlocks) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 4096 filter blocks");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->finish_state) HIPCHK(c, hipMalloc(&c->finish_state, finish_state_bytes()));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_SEARCH_SELECT, s, 0);
+    HIPCHK(c, launch_cdef_select(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, c->finish_state, decision, fb_index, s));
     return AV1PF_OK;
 }
 

@@ -113,6 +113,11 @@ hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s);
 
+hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,
+                              uint64_t lambda, void *state, av1pf_cdef_decision *dec, uint8_t *fb_index, hipStream_t s);
+size_t finish_state_bytes();
+constexpr int kMaxSearchBlocks = 4096;   // filter blocks the device selection holds
+
 // tile geometry helpers shared by host and device
 __host__ __device__ inline int lr_stripes(int ph, int sh, int off) { return (ph + off + sh - 1) / sh; }
 

@@ -271,6 +271,235 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     if (tid < 64) m1[tid] = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
+// ---------------------------------------------------------------------------
+// Strength selection on the device (finish / jointStrengthSearchDual /
+// searchOneDual, cdef_search.go:780-945).  The sequence of choose steps is
+// fixed by the algorithm (for cdef_bits 0..3: nb = 1 << bits first choices,
+// then 4 * nb refinements that drop the oldest pair and choose the last slot
+// again), so the host enqueues it as a run of small kernels; only the chosen
+// pairs depend on the data, and they live in FinishState on the device.
+//  - fin_first_kernel: slot 0, luma and chroma chosen separately (lowest
+//    column sums over the active filter blocks);
+//  - fin_gain_kernel: slot > 0; thread = one (luma j, chroma k) pair, summing
+//    over the active blocks how much the pair would lower each block's best
+//    cost; per-workgroup argmax, and the last workgroup to finish (atomic
+//    ticket) picks the global first maximum in (j, k) order;
+//  - both end with the same epilogue: update the level's cost and the best
+//    set at the end of a level, and drop the oldest pair before a refinement;
+//  - fin_assign_kernel: each block's index = its cheapest chosen pair.
+#define FIN_WG 16   // 16 x 256 threads = the 4096 (j, k) pairs
+
+struct FinishState {
+    int ly[8], lc[8];
+    int ys[8], us[8];
+    unsigned long long dist, best_cost;
+    int best_bits;
+    unsigned int ticket;
+    unsigned long long wg_max[FIN_WG];
+    int wg_pos[FIN_WG];
+    int active[4096];      // active filter blocks (at most 4096 = 4096x4096 luma)
+    int nact;
+};
+
+__device__ void fin_epilogue(FinishState *st, int nb, int bits, int end_level, int shift_after, unsigned long long lambda)
+{
+    if (end_level) {
+        const long long rate = (long long)(st->nact * bits + nb * 12) << 9;
+        const unsigned long long cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((st->dist * 16) << 7);
+        if (cost < st->best_cost) {
+            st->best_cost = cost;
+            st->best_bits = bits;
+            for (int i = 0; i < 8; i++) { st->ys[i] = st->ly[i]; st->us[i] = st->lc[i]; }
+        }
+    }
+    if (shift_after)
+        for (int i = 0; i + 1 < nb; i++) { st->ly[i] = st->ly[i + 1]; st->lc[i] = st->lc[i + 1]; }
+}
+
+// Active blocks (a non-skip 8x8 block inside the frame, as the MSE kernel
+// counts them), compacted in ascending order by one wave with ballots, and
+// the selection state.
+__global__ __launch_bounds__(64) void fin_init_kernel(const uint64_t *noskip, int nhfb, int nvfb, int mi_cols,
+                                                      int mi_rows, FinishState *st)
+{
+    const int lane = threadIdx.x;
+    const int nfb = nhfb * nvfb;
+    int n = 0;
+    for (int base = 0; base < nfb; base += 64) {
+        const int fb = base + lane;
+        bool act = false;
+        if (fb < nfb) {
+            const int fbr = fb / nhfb, fbc = fb % nhfb;
+            const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+            uint64_t valid = 0;
+            for (int r = 0; r < nvb; r += 2)
+                for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
+            const uint64_t m = noskip ? noskip[fb] : ~0ull;
+            act = (m & valid) != 0;
+        }
+        const uint64_t vote = __ballot(act);
+        const int before = __popcll(vote & ((1ull << lane) - 1));
+        if (act) st->active[n + before] = fb;
+        n += __popcll(vote);
+    }
+    if (lane == 0) {
+        st->nact = n;
+        st->best_cost = 1ull << 63;
+        st->best_bits = 0;
+        st->ticket = 0;
+        for (int i = 0; i < 8; i++) { st->ly[i] = st->lc[i] = 0; st->ys[i] = st->us[i] = 0; }
+    }
+}
+
+__global__ __launch_bounds__(128) void fin_first_kernel(const uint64_t *mse, int nfb, FinishState *st, int nb, int bits,
+                                                        int end_level, int shift_after, unsigned long long lambda)
+{
+    __shared__ unsigned long long sums[128];
+    const int t = threadIdx.x;
+    const int plane = t >> 6, g = t & 63;
+    unsigned long long s = 0;
+    for (int i = 0; i < st->nact; i++) s += mse[((size_t)plane * nfb + st->active[i]) * 64 + g];
+    sums[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        if (st->nact == 0) {
+            st->ly[0] = st->lc[0] = 0;
+            st->dist = 0;
+        } else {
+            int by = 0, bc = 0;
+            for (int k = 1; k < 64; k++) {
+                if (sums[k] < sums[by]) by = k;
+                if (sums[64 + k] < sums[64 + bc]) bc = k;
+            }
+            st->ly[0] = by;
+            st->lc[0] = bc;
+            st->dist = sums[by] + sums[64 + bc];
+        }
+        fin_epilogue(st, nb, bits, end_level, shift_after, lambda);
+    }
+}
+
+__global__ __launch_bounds__(256) void fin_gain_kernel(const uint64_t *mse, int nfb, FinishState *st, int slot, int nb,
+                                                       int bits, int end_level, int shift_after, unsigned long long lambda)
+{
+    __shared__ unsigned long long best[4096];
+    __shared__ unsigned long long rmax[256];
+    __shared__ int rpos[256];
+    __shared__ bool last;
+    const int t = threadIdx.x;
+    const int nact = st->nact;
+    const uint64_t *L = mse, *Cm = mse + (size_t)nfb * 64;
+    int ly[8], lc[8];
+#pragma unroll
+    for (int g = 0; g < 8; g++) { ly[g] = st->ly[g]; lc[g] = st->lc[g]; }
+    for (int i = t; i < nact; i += 256) {
+        const int fb = st->active[i];
+        unsigned long long b = L[(size_t)fb * 64 + ly[0]] + Cm[(size_t)fb * 64 + lc[0]];
+        for (int g = 1; g < slot; g++) {
+            const unsigned long long c = L[(size_t)fb * 64 + ly[g]] + Cm[(size_t)fb * 64 + lc[g]];
+            if (c < b) b = c;
+        }
+        best[i] = b;
+    }
+    __syncthreads();
+    const int pair = blockIdx.x * 256 + t, j = pair >> 6, k = pair & 63;
+    unsigned long long gain = 0;
+    for (int i = 0; i < nact; i++) {
+        const int fb = st->active[i];
+        const unsigned long long b = best[i], yj = L[(size_t)fb * 64 + j];
+        if (yj < b) {
+            const unsigned long long room = b - yj, ck = Cm[(size_t)fb * 64 + k];
+            if (ck < room) gain += room - ck;
+        }
+    }
+    rmax[t] = gain;
+    rpos[t] = pair;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {   // first maximum in pair order
+        if (t < o) {
+            const unsigned long long a = rmax[t], b = rmax[t + o];
+            if (b > a || (b == a && rpos[t + o] < rpos[t])) { rmax[t] = b; rpos[t] = rpos[t + o]; }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        st->wg_max[blockIdx.x] = rmax[0];
+        st->wg_pos[blockIdx.x] = rpos[0];
+        __threadfence();
+        last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && t == 0) {
+        __threadfence();
+        const volatile unsigned long long *wm = st->wg_max;
+        const volatile int *wp = st->wg_pos;
+        unsigned long long m = wm[0];
+        int p = wp[0];
+        for (int w = 1; w < (int)gridDim.x; w++)
+            if (wm[w] > m) { m = wm[w]; p = wp[w]; }   // workgroups cover ascending pair ranges
+        unsigned long long base = 0;
+        for (int i = 0; i < nact; i++) base += best[i];
+        st->ly[slot] = p >> 6;
+        st->lc[slot] = p & 63;
+        st->dist = base - m;
+        st->ticket = 0;
+        fin_epilogue(st, nb, bits, end_level, shift_after, lambda);
+    }
+}
+
+// One workgroup: blocks are zeroed, then the active ones get their index.
+__global__ __launch_bounds__(256) void fin_assign_kernel(const uint64_t *mse, int nfb, FinishState *st,
+                                                         av1pf_cdef_decision *dec, uint8_t *fb_index)
+{
+    const int nb = 1 << st->best_bits;
+    for (int fb = threadIdx.x; fb < nfb; fb += 256) fb_index[fb] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < st->nact; i += 256) {
+        const int fb = st->active[i];
+        unsigned long long bm = 1ull << 63;
+        int bg = 0;
+        for (int g = 0; g < nb; g++) {
+            const unsigned long long c = mse[(size_t)fb * 64 + st->ys[g]] + mse[((size_t)nfb + fb) * 64 + st->us[g]];
+            if (c < bm) { bm = c; bg = g; }
+        }
+        fb_index[fb] = (uint8_t)bg;
+    }
+    if (threadIdx.x == 0) {
+        dec->cdef_bits = st->best_bits;
+        for (int i = 0; i < 8; i++) {
+            dec->y_strength[i] = i < nb ? kLevel1[st->ys[i]] : 0;
+            dec->uv_strength[i] = i < nb ? kLevel1[st->us[i]] : 0;
+        }
+    }
+}
+
+size_t finish_state_bytes() { return sizeof(FinishState); }
+
+hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,
+                              uint64_t lambda, void *state, av1pf_cdef_decision *dec, uint8_t *fb_index, hipStream_t s)
+{
+    FinishState *st = static_cast<FinishState *>(state);
+    const int nfb = nhfb * nvfb;
+    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(64), 0, s, noskip, nhfb, nvfb, mi_cols, mi_rows, st);
+    for (int bits = 0; bits <= 3; bits++) {
+        const int nb = 1 << bits;
+        const int steps = 5 * nb;   // nb first choices, 4 * nb refinements
+        for (int k = 0; k < steps; k++) {
+            const int slot = k < nb ? k : nb - 1;
+            const int end_level = k == steps - 1;
+            const int shift_after = k + 1 >= nb && k + 1 < steps;   // the next step is a refinement
+            if (slot == 0)
+                hipLaunchKernelGGL(fin_first_kernel, dim3(1), dim3(128), 0, s, mse, nfb, st, nb, bits, end_level,
+                                   shift_after, (unsigned long long)lambda);
+            else
+                hipLaunchKernelGGL(fin_gain_kernel, dim3(FIN_WG), dim3(256), 0, s, mse, nfb, st, slot, nb, bits,
+                                   end_level, shift_after, (unsigned long long)lambda);
+        }
+    }
+    hipLaunchKernelGGL(fin_assign_kernel, dim3(1), dim3(256), 0, s, mse, nfb, st, dec, fb_index);
+    return hipGetLastError();
+}
+
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s)
 {

@@ -22,8 +22,11 @@ _lib = None
 EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error", "av1pf_apply",
            "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
            "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_cdef_search_finish", "av1pf_cdef_lambda",
+           "av1pf_cdef_search_select",
            "av1pf_profile", "av1pf_profile_read"]
-STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search"]
+STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
+               "cdef_search_select"] + [f"reserved{i}" for i in range(9, 16)]
+PROFILE_STAGES = 16
 
 
 class Av1pfError(RuntimeError):
@@ -59,6 +62,8 @@ def load():
                                       C.c_int, C.c_void_p, C.c_void_p]
     lib.av1pf_cdef_search_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.POINTER(C.c_int)]
+    lib.av1pf_cdef_search_select.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.c_void_p, C.c_void_p, C.c_uint64,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_cdef_lambda.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.av1pf_cdef_lambda.restype = C.c_uint64
     lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
@@ -240,16 +245,24 @@ class PostFilterRunner:
     def sync(self, stream=None):
         self._check(self.lib.av1pf_sync(self.h, self._stream(stream)))
 
+    def cdef_search_select(self, rec, noskip_dev_ptr, mse_dev_ptr, lam, decision_dev_ptr, fb_index_dev_ptr,
+                           stream=None):
+        """Device strength selection over the MSE table; decision is a 24-byte
+        av1pf_cdef_decision in device memory."""
+        self._check(self.lib.av1pf_cdef_search_select(self.h, C.byref(rec.abi), noskip_dev_ptr, mse_dev_ptr,
+                                                      C.c_uint64(int(lam)), decision_dev_ptr, fb_index_dev_ptr,
+                                                      self._stream(stream)))
+
     def profile(self, enable=True):
         self._check(self.lib.av1pf_profile(self.h, int(enable)))
 
     def profile_read(self):
         """{stage: (total_ms, launches, algorithmic_bytes)} since profile(True)."""
-        ms = (C.c_double * 8)()
-        n = (C.c_int * 8)()
-        by = (C.c_double * 8)()
+        ms = (C.c_double * PROFILE_STAGES)()
+        n = (C.c_int * PROFILE_STAGES)()
+        by = (C.c_double * PROFILE_STAGES)()
         self._check(self.lib.av1pf_profile_read(self.h, ms, n, by))
-        return {STAGE_NAMES[i]: (ms[i], n[i], by[i]) for i in range(8) if n[i]}
+        return {STAGE_NAMES[i]: (ms[i], n[i], by[i]) for i in range(PROFILE_STAGES) if n[i]}
 
     def cdef_search(self, src, rec, noskip_dev_ptr, damping, mse_dev_ptr, stream=None):
         self._check(self.lib.av1pf_cdef_search(self.h, C.byref(src.abi), C.byref(rec.abi), noskip_dev_ptr,

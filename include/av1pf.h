@@ -224,6 +224,21 @@ int  av1pf_cdef_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame
 int  av1pf_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda,
                               uint8_t y_strength[8], uint8_t uv_strength[8], uint8_t *fb_index, int *cdef_bits);
 
+/* The same selection on the device, after av1pf_cdef_search on the same
+ * stream: mse, noskip, decision and fb_index are DEVICE pointers; recon gives
+ * the frame geometry (the active blocks are those with a non-skip 8x8 block
+ * inside the frame, as the search counts them).  At most 4096 filter blocks
+ * (4096x4096 luma).  Asynchronous.                                          */
+typedef struct av1pf_cdef_decision {
+    int32_t cdef_bits;
+    uint8_t y_strength[8];
+    uint8_t uv_strength[8];
+    uint8_t pad[4];
+} av1pf_cdef_decision;
+
+int  av1pf_cdef_search_select(av1pf_ctx *ctx, const av1pf_frame *recon, const uint64_t *noskip, const uint64_t *mse,
+                              uint64_t lambda, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream);
+
 /* Full-search lambda (svtCDEFFullLambda, cdef_search.go:274-299) from the
  * 8-bit luma DC quantizer of the frame's qindex.  Needs no device.          */
 uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_factor);
@@ -235,8 +250,8 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
  * whole-plane pass) are accumulated.  av1pf_profile_read synchronises the
  * device and returns, per stage id, total ms, launch count and bytes.
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
- * 6 grain_apply, 7 cdef_search.                                              */
-#define AV1PF_PROFILE_STAGES 8
+ * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9..15 reserved.       */
+#define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
                         double bytes[AV1PF_PROFILE_STAGES]);

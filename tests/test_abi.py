@@ -41,7 +41,7 @@ def test_struct_layout_matches_header(tmp_path):
 #define O(s, f) printf(#s "." #f " %zu\n", offsetof(s, f))
 int main(void) {
   P(av1pf_frame); P(av1pf_lf_mask); P(av1pf_deblock_params); P(av1pf_cdef_params);
-  P(av1pf_lr_unit); P(av1pf_lr_params); P(av1pf_grain_params); P(av1pf_request);
+  P(av1pf_lr_unit); P(av1pf_lr_params); P(av1pf_grain_params); P(av1pf_request); P(av1pf_cdef_decision);
   O(av1pf_frame, plane); O(av1pf_frame, stride); O(av1pf_deblock_params, level_cache);
   O(av1pf_cdef_params, index); O(av1pf_lr_params, units); O(av1pf_grain_params, ar_coeffs_cr);
   O(av1pf_grain_params, cr_offset); O(av1pf_request, display);
@@ -52,7 +52,8 @@ int main(void) {
     out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n") if l)
     sizes = {"av1pf_frame": abi.Frame, "av1pf_lf_mask": abi.LFMask, "av1pf_deblock_params": abi.DeblockParams,
              "av1pf_cdef_params": abi.CdefParams, "av1pf_lr_unit": abi.LRUnit, "av1pf_lr_params": abi.LRParams,
-             "av1pf_grain_params": abi.GrainParams, "av1pf_request": abi.Request}
+             "av1pf_grain_params": abi.GrainParams, "av1pf_request": abi.Request,
+             "av1pf_cdef_decision": abi.CdefDecision}
     for name, cls in sizes.items():
         assert int(out[name]) == C.sizeof(cls), name
     offs = {"av1pf_frame.plane": abi.Frame.plane.offset, "av1pf_frame.stride": abi.Frame.stride.offset,

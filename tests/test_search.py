@@ -48,6 +48,11 @@ def block_sse(a, b, fr, blocks):
     return sy, sc
 
 
+def active_blocks(fr, noskip):
+    """1 where the filter block has a non-skip 8x8 block inside the frame."""
+    return np.array([1 if block_list(fr, noskip, fb) else 0 for fb in range(noskip.shape[0])], np.uint8)
+
+
 @pytest.mark.parametrize("size", [(128, 128), (200, 136)])
 def test_unfiltered_trial_is_block_sse(size):
     src, rec, noskip, damping = synth.search_pair(seed=11, w=size[0], h=size[1])
@@ -159,9 +164,42 @@ def test_gpu_search_parity(size):
     np.testing.assert_array_equal(got, want)
     from goav1_amd.lib import cdef_lambda, cdef_search_finish
     lam = cdef_lambda(synth.search_dc_quant(160))
-    skip = (noskip == 0).astype(np.uint8)
+    skip = 1 - active_blocks(src, noskip)
     g = cdef_search_finish(got, skip, lam)
     w = orc.cdef_search_finish(want, skip, lam)
     assert g[0] == w[0]
     for a, b in zip(g[1:], w[1:]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (1920, 1080, 60, 1), (200, 136, 220, 2), (64, 64, 160, 3)])
+def test_gpu_select_matches_host_finish(case):
+    """Device strength selection = host finish, on tables from the device search."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_lambda, cdef_search_finish
+    w, h, q, seed = case
+    src, rec, noskip, damping = synth.search_pair(seed=40 + seed, w=w, h=h, q=q)
+    print(f"select {w}x{h} q{q}", flush=True)
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        mse = torch.zeros((2, nfb, 64), dtype=torch.int64, device="cuda")
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.full((nfb,), 255, dtype=torch.uint8, device="cuda")
+        r.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+        for lam in (cdef_lambda(synth.search_dc_quant(160)), 0, 1 << 30):
+            r.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), lam, dec.data_ptr(), idx.data_ptr())
+            r.sync()
+            torch.cuda.synchronize()
+            table = mse.cpu().numpy().view(np.uint64)
+            bits, ys, uvs, fbi = cdef_search_finish(table, 1 - active_blocks(src, noskip), lam)
+            d = dec.cpu().numpy()
+            assert int(d[:4].view(np.int32)[0]) == bits, f"lambda {lam}"
+            np.testing.assert_array_equal(d[4:12], ys)
+            np.testing.assert_array_equal(d[12:20], uvs)
+            np.testing.assert_array_equal(idx.cpu().numpy(), fbi)
+    finally:
+        r.close()
