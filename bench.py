@@ -98,9 +98,13 @@ def search_leg(runner, steps, cpu):
     idx = torch.zeros(nfb, dtype=torch.uint8, device="cuda")
     lam = cdef_lambda(synth.search_dc_quant(160))
 
+    hz, vt = (1280 + 32) // 64, (720 + 32) // 64
+    lr_units = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
+
     def one():
         runner.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
         runner.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), lam, dec.data_ptr(), idx.data_ptr())
+        runner.lr_search(ds, dr, 64, lr_units.data_ptr())
 
     for _ in range(3):
         one()
@@ -124,11 +128,13 @@ def search_leg(runner, steps, cpu):
     finish_ms = (time.perf_counter() - t0) * 1e3
     px = src.w * src.h
     avg = lambda k: round(1e3 * prof[k][0] / prof[k][1], 2)
-    out = {"workload": "1280x720 8-bit 4:2:0, q160 (damping 5), 64 strength pairs per 64x64 block, "
-                       "device MSE table + device strength selection",
+    out = {"workload": "1280x720 8-bit 4:2:0, q160 (damping 5): CDEF 64 strength pairs per 64x64 block "
+                       "(device MSE table + device strength selection) + curated luma LR search "
+                       "(6 Wiener estimates, exact Wiener re-score, 4 SGR presets per 64x64 unit)",
            "mpx_s": round(px / (dev_ms / 1e3) / 1e6, 2),
            "ms_per_frame": round(dev_ms, 4),
            "mse_table_us": avg("cdef_search"), "select_us": avg("cdef_search_select"),
+           "lr_search_us": avg("lr_search"),
            "host_finish_ms": round(finish_ms, 3),
            "cdef_bits": int(dec.cpu().numpy()[:4].view(np.int32)[0]), "host_finish_bits": host[0]}
     if cpu:
@@ -136,9 +142,10 @@ def search_leg(runner, steps, cpu):
         t0 = time.perf_counter()
         orc.cdef_search_mse(src, rec, noskip, damping)
         orc.cdef_search_finish(table, (noskip == 0).astype(np.uint8), lam)
+        orc.lr_search(src, rec, 64)
         el = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(px / el / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
-                               "sample": f"1 frame of the search workload, oracle MSE table + finish, 1 thread, "
+                               "sample": f"1 frame of the search workload, oracle CDEF table + finish + LR search, 1 thread, "
                                          f"{el:.1f} s"}
     return out
 

@@ -76,7 +76,8 @@ struct av1pf_ctx {
 
 namespace {
 
-enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT };
+enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
+       ST_LR_SEARCH };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -893,6 +894,27 @@ int av1pf_cdef_search_select(av1pf_ctx *c, const av1pf_frame *recon, const uint6
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_SELECT, s, 0);
     HIPCHK(c, launch_cdef_select(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, c->finish_state, decision, fb_index, s));
+    return AV1PF_OK;
+}
+
+int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                    av1pf_lr_unit *units, uint64_t *sse, void *stream)
+{
+    if (!c || !units) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_frame(c, src);
+    if (rc) return rc;
+    if ((rc = validate_frame(c, recon))) return rc;
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != 8 || recon->bit_depth != 8)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "lr search needs two 8-bit frames of one size");
+    if (unit_size != 32 && unit_size != 64 && unit_size != 128 && unit_size != 256)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "lr unit size must be 32, 64, 128 or 256");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int W = recon->width, H = recon->height;
+    const int hz = std::max((W + (unit_size >> 1)) / unit_size, 1), vt = std::max((H + (unit_size >> 1)) / unit_size, 1);
+    const FrameView sv = view_of(src), rv = view_of(recon);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_LR_SEARCH, s, 2.0 * W * H);
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, units, sse, s));
     return AV1PF_OK;
 }
 

@@ -23,28 +23,9 @@
 // LDS once.  The kernel is out of place (post-CDEF frame -> output frame); NONE
 // units and NONE planes copy.
 #include "pf_internal.h"
+#include "lr_filters.h"
 
 namespace pf {
-
-#define LR_TW 70          // 64 + 6
-#define LR_AB 66          // 64 + 2
-
-__device__ __forceinline__ int32_t r2s(int32_t v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
-__device__ __forceinline__ uint32_t r2u(uint32_t v, int n) { return n <= 0 ? v : (v + (1u << (n - 1))) >> n; }
-
-__device__ __constant__ int8_t kSgrR[16][2] = {
-    {2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{2,1},{0,1},{0,1},{0,1},{0,1},{2,0},{2,0}};
-__device__ __constant__ int16_t kSgrS[16][2] = {
-    {140,3236},{112,2158},{93,1618},{80,1438},{70,1295},{58,1177},{47,1079},{37,996},
-    {30,925},{25,863},{-1,2589},{-1,1618},{-1,1177},{-1,925},{56,-1},{22,-1}};
-
-// AV1's x/(x+1) Q8 table from its closed form (exact: no ties occur)
-__device__ __forceinline__ int32_t x_by_xplus1(uint32_t z)
-{
-    if (z == 0) return 1;
-    if (z >= 255) return 256;
-    return (int32_t)((512u * z + (z + 1)) / (2u * (z + 1)));
-}
 
 template <typename T>
 __device__ __forceinline__ int read_px(const PlaneView &pv, int y, int x)
@@ -101,117 +82,10 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     }
     __syncthreads();
     const uint16_t *O = tile + 3 * LR_TW + 3;   // tile origin
-    const int32_t mx = (1 << bd) - 1;
-
-    if (u.type == AV1PF_LR_WIENER) {
-        int round0 = 3, round1 = 11;
-        const int ibr = bd + 7 - round0 + 2;
-        if (ibr > 16) { round0 += ibr - 16; round1 -= ibr - 16; }
-        const int32_t h0 = u.wiener_h[0], h1 = u.wiener_h[1], h2 = u.wiener_h[2];
-        const int32_t h3 = -2 * (h0 + h1 + h2);
-        const int32_t v0 = u.wiener_v[0], v1 = u.wiener_v[1], v2 = u.wiener_v[2];
-        const int32_t v3 = -2 * (v0 + v1 + v2);
-        const int32_t limit = 1 << (bd + 1 + 7 - round0), hoff = 1 << (bd + 6);
-        for (int i = tid; i < w * th; i += 256) {
-            const int r = i / w - 3, c = i - (i / w) * w;
-            const uint16_t *s = O + r * LR_TW + c;
-            int32_t sum = ((int32_t)s[0] << 7) + hoff;
-            sum += h0 * ((int32_t)s[-3] + s[3]) + h1 * ((int32_t)s[-2] + s[2]) + h2 * ((int32_t)s[-1] + s[1]) + h3 * (int32_t)s[0];
-            tmp[(r + 3) * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
-        }
-        __syncthreads();
-        const int32_t voff = 1 << (bd + round1 - 1);
-        for (int i = tid; i < w * h; i += 256) {
-            const int r = i / w, c = i - r * w;
-            const uint16_t *s = tmp + (r + 3) * 64 + c;
-            int32_t sum = ((int32_t)s[0] << 7) - voff;
-            sum += v0 * ((int32_t)s[-3 * 64] + s[3 * 64]) + v1 * ((int32_t)s[-2 * 64] + s[2 * 64]) +
-                   v2 * ((int32_t)s[-64] + s[64]) + v3 * (int32_t)s[0];
-            row_ptr<T>(dp, y0 + r)[x0 + c] = (T)min(max(r2s(sum, round1), 0), mx);
-        }
-        return;
-    }
-
-    // ---- self-guided
-    const int set = u.sgr_set;
-    int xq0, xq1;
-    if (kSgrR[set][0] == 0) { xq0 = 0; xq1 = 128 - u.sgr_xqd[1]; }
-    else if (kSgrR[set][1] == 0) { xq0 = u.sgr_xqd[0]; xq1 = 0; }
-    else { xq0 = u.sgr_xqd[0]; xq1 = 128 - xq0 - u.sgr_xqd[1]; }
-    constexpr int PPT = 16;   // pixels per thread (64x64 / 256)
-    int32_t f0[PPT], f1[PPT];
-#pragma unroll
-    for (int j = 0; j < PPT; j++) { f0[j] = 0; f1[j] = 0; }
-    const int aw = w + 2;
-    for (int pass = 0; pass < 2; pass++) {
-        const int r = kSgrR[set][pass];
-        if (r == 0) continue;
-        const int n = (2 * r + 1) * (2 * r + 1);
-        const uint32_t s = (uint32_t)kSgrS[set][pass];
-        const uint32_t recip = (uint32_t)((8192 + n) / (2 * n));
-        const int step = pass == 0 ? 2 : 1;
-        const int nrows = pass == 0 ? (h + 3) / 2 : h + 2;   // rows -1, 1, 3, ... or -1..h
-        __syncthreads();
-        for (int i = tid; i < nrows * aw; i += 256) {
-            const int ri = i / aw;
-            const int y = -1 + ri * step, x = i - ri * aw - 1;
-            int32_t sum = 0, sq = 0;
-            for (int dy = -r; dy <= r; dy++)
-                for (int dx = -r; dx <= r; dx++) {
-                    const int32_t v = O[(y + dy) * LR_TW + x + dx];
-                    sum += v;
-                    sq += v * v;
-                }
-            const uint32_t a = (uint32_t)r2s(sq, 2 * (bd - 8));
-            const uint32_t b = (uint32_t)r2s(sum, bd - 8);
-            const uint32_t p = a * (uint32_t)n >= b * b ? a * (uint32_t)n - b * b : 0u;
-            uint32_t z = r2u(p * s, 20);
-            z = min(z, 255u);
-            const int32_t aa = x_by_xplus1(z);
-            Ab[(y + 1) * LR_AB + x + 1] = aa;
-            Bb[(y + 1) * LR_AB + x + 1] = (int32_t)r2u((uint32_t)(256 - aa) * (uint32_t)sum * recip, 12);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int i = tid + 256 * j;
-            if (i >= w * h) break;
-            const int y = i / w, x = i - y * w;
-            const int32_t uu = O[y * LR_TW + x];
-            const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
-            const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
-            int32_t a, b;
-            if (pass == 0) {
-                if ((y & 1) == 0) {
-                    a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
-                    b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
-                    f0[j] = r2s(a * uu + b, 9);
-                } else {
-                    a = A[0] * 6 + (A[-1] + A[1]) * 5;
-                    b = B[0] * 6 + (B[-1] + B[1]) * 5;
-                    f0[j] = r2s(a * uu + b, 8);
-                }
-            } else {
-                a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
-                    (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
-                b = (B[0] + B[-1] + B[1] + B[-LR_AB] + B[LR_AB]) * 4 +
-                    (B[-LR_AB - 1] + B[LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB + 1]) * 3;
-                f1[j] = r2s(a * uu + b, 9);
-            }
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        const int i = tid + 256 * j;
-        if (i >= w * h) break;
-        const int y = i / w, x = i - y * w;
-        const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
-        int32_t v = uu << 7;
-        v += xq0 * (f0[j] - uu);
-        v += xq1 * (f1[j] - uu);
-        const int32_t wv = (int16_t)r2s(v, 11);
-        row_ptr<T>(dp, y0 + y)[x0 + x] = (T)min(max(wv, 0), mx);
-    }
+    const int8_t *vt = u.wiener_v, *ht = u.wiener_h;
+    auto put = [&](int y, int x, int v) { row_ptr<T>(dp, y0 + y)[x0 + x] = (T)v; };
+    if (u.type == AV1PF_LR_WIENER) wiener_tile(O, w, h, bd, vt, ht, tmp, put);
+    else sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb, put);
 }
 
 __global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)

@@ -22,10 +22,10 @@ _lib = None
 EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error", "av1pf_apply",
            "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
            "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_cdef_search_finish", "av1pf_cdef_lambda",
-           "av1pf_cdef_search_select",
+           "av1pf_cdef_search_select", "av1pf_lr_search",
            "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
-               "cdef_search_select"] + [f"reserved{i}" for i in range(9, 16)]
+               "cdef_search_select", "lr_search"] + [f"reserved{i}" for i in range(10, 16)]
 PROFILE_STAGES = 16
 
 
@@ -64,6 +64,8 @@ def load():
                                              C.c_void_p, C.POINTER(C.c_int)]
     lib.av1pf_cdef_search_select.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.c_void_p, C.c_void_p, C.c_uint64,
                                              C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.av1pf_lr_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
     lib.av1pf_cdef_lambda.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.av1pf_cdef_lambda.restype = C.c_uint64
     lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
@@ -252,6 +254,11 @@ class PostFilterRunner:
         self._check(self.lib.av1pf_cdef_search_select(self.h, C.byref(rec.abi), noskip_dev_ptr, mse_dev_ptr,
                                                       C.c_uint64(int(lam)), decision_dev_ptr, fb_index_dev_ptr,
                                                       self._stream(stream)))
+
+    def lr_search(self, src, rec, unit_size, units_dev_ptr, sse_dev_ptr=None, stream=None):
+        """Curated encoder LR search on the luma plane; units = hz*vt 16-byte records."""
+        self._check(self.lib.av1pf_lr_search(self.h, C.byref(src.abi), C.byref(rec.abi), int(unit_size), units_dev_ptr,
+                                             sse_dev_ptr, self._stream(stream)))
 
     def profile(self, enable=True):
         self._check(self.lib.av1pf_profile(self.h, int(enable)))

@@ -243,6 +243,21 @@ int  av1pf_cdef_search_select(av1pf_ctx *ctx, const av1pf_frame *recon, const ui
  * 8-bit luma DC quantizer of the frame's qindex.  Needs no device.          */
 uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_factor);
 
+/* ---- encoder loop-restoration search ------------------------------------
+ * The keyframe encoder's curated search on the luma plane of 8-bit frames
+ * (SearchWienerUnitU8 / SearchSGRProjUnitU8, encoder/restoration.go:149-262,
+ * and their driver, encoder/keyframe_lossy.go:2310-2386): per restoration
+ * unit of size unit_size (32..256; units offset 8 rows as the decoder's),
+ * six Wiener tap triples are screened with the reference's H-only estimate,
+ * the winner is re-scored with the exact Wiener, four SGR presets are scored
+ * exactly, and the unit record (NONE / WIENER / SGRPROJ, same layout as the
+ * decoder input) is written to units[hz * vt] with hz = max((W + us/2) / us, 1)
+ * and vt likewise.  sse (optional, [hz * vt][12] uint64): 0 baseline, 1..6
+ * Wiener estimates, 7 exact Wiener, 8..11 SGR presets; UINT64_MAX where the
+ * reference computes nothing.  Device pointers; asynchronous.               */
+int  av1pf_lr_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                     av1pf_lr_unit *units, uint64_t *sse, void *stream);
+
 /* ---- per-stage timing (no counterpart in the reference, whose closest aid is
  * the FrameWorkPostFilterResult counters, postfilter.go:309-316) ----------
  * While enabled, every stage launch is bracketed by HIP events on its stream
@@ -250,7 +265,8 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
  * whole-plane pass) are accumulated.  av1pf_profile_read synchronises the
  * device and returns, per stage id, total ms, launch count and bytes.
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
- * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9..15 reserved.       */
+ * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
+ * 10..15 reserved.                                                          */
 #define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],

@@ -81,6 +81,10 @@ int  orc_cdef_search_mse(const orc_frame *src, const orc_frame *rec, const uint6
 int  orc_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda,
                             uint8_t y_str[8], uint8_t uv_str[8], uint8_t *fb_idx);
 
+/* encoder curated LR search, luma 8-bit (orc_lr_search.c) */
+int orc_lr_search_units(int pw, int ph, int us, int *hz, int *vt);
+int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_unit *out, uint64_t *sse);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,7 @@ def lib():
         _lib.orc_cdef_search_mse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _lib.orc_cdef_search_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64,
                                                 C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_lr_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -146,3 +147,23 @@ def cdef_search_finish(mse, skip_fb, lam):
     bits = lib().orc_cdef_search_finish(m.ctypes.data, nfb, sk.ctypes.data, C.c_uint64(int(lam)),
                                         ys.ctypes.data, uvs.ctypes.data, idx.ctypes.data)
     return bits, ys, uvs, idx
+
+
+def lr_search_units(pw, ph, us):
+    hz = max((pw + (us >> 1)) // us, 1)
+    vt = max((ph + (us >> 1)) // us, 1)
+    return hz, vt
+
+
+def lr_search(src, rec, unit_size=64):
+    """Curated encoder LR search on the luma plane: (units structured array of
+    abi.LRUnit bytes [n, 16], sse [n, 12] uint64)."""
+    hz, vt = lr_search_units(rec.w, rec.h, unit_size)
+    n = hz * vt
+    units = np.zeros((n, 16), np.uint8)
+    sse = np.zeros((n, 12), np.uint64)
+    s, r = orc_frame(src), orc_frame(rec)
+    rc = lib().orc_lr_search(C.byref(s), C.byref(r), unit_size, units.ctypes.data, sse.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_lr_search failed: {rc}")
+    return units, sse
