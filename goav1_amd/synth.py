@@ -265,6 +265,9 @@ def build_side_data(fr, rng, *, lf_levels=None, lf_random=False, sharpness=0,
                       level_cache=np.ascontiguousarray(lvl.reshape(-1)),
                       masks=np.ascontiguousarray(masks.reshape(-1)))
     fr.skip_mi = skip
+    # transform grids the masks were built from (ids and square sizes in 4-sample
+    # units, luma MI grid and chroma 4x4 grid), kept for the edge-list tests
+    fr.tx = dict(luma=(txid, txw), chroma=(ctxid, ctxw))
     # ---- CDEF
     ucols, urows = (fr.w + 63) // 64, (fr.h + 63) // 64
     nsk = ~skip

@@ -200,3 +200,27 @@ def test_film_grain_lags(runner, lag, ss):
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
     assert_same(gdisp, wdisp, f"display lag {lag} ss {ss}")
+
+
+@pytest.mark.parametrize("case", [
+    dict(w=97, h=67, bd=10, ss=1, lag=2, restricted=1), dict(w=72, h=40, bd=10, ss=0, lag=1, overlap=0),
+    dict(w=66, h=66, bd=8, ss=1, lag=0, csfl=1), dict(w=80, h=36, bd=10, ss=1, lag=3, gss=2, shift=8, arshift=9),
+    dict(w=161, h=99, bd=8, ss=0, lag=3, restricted=1, identity=1)], ids=lambda c: "_".join(f"{k}{v}" for k, v in c.items()))
+def test_film_grain_variants(runner, case):
+    """Restricted range (with the identity-matrix chroma limit), no overlap,
+    chroma scaling from luma, grain scale shift and other shifts."""
+    c = dict(restricted=0, overlap=1, csfl=0, gss=0, shift=10, arshift=7, identity=0)
+    c.update(case)
+    rng = np.random.default_rng(900 + c["w"] + c["h"])
+    fr = synth.FrameData(c["w"], c["h"], c["bd"], c["ss"], c["ss"])
+    synth.fill_blocky(fr, rng)
+    fr.matrix_identity = c["identity"]
+    g = synth.random_grain(rng, c["bd"], lag=c["lag"], csfl=bool(c["csfl"]))
+    g.clip_to_restricted_range, g.overlap = c["restricted"], c["overlap"]
+    g.grain_scale_shift, g.scaling_shift, g.ar_coeff_shift = c["gss"], c["shift"], c["arshift"]
+    fr.grain = g
+    fr.stages = abi.STAGE_FILMGRAIN
+    print(f"grain variant {case}", flush=True)
+    want, wdisp = orc.run_chain(fr)
+    got, gdisp = gpu_run(runner, fr)
+    assert_same(gdisp, wdisp, f"display {case}")
