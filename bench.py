@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-search", action="store_true", help="skip the encoder-search leg")
+    ap.add_argument("--batch-frames", type=int, default=64,
+                    help="frames in the mixed-batch leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 512 MiB device copy and exit (HBM counter calibration)")
     return ap.parse_args()
@@ -148,6 +150,66 @@ def search_leg(runner, steps, cpu):
                                "sample": f"1 frame of the search workload, oracle CDEF table + finish + LR search, 1 thread, "
                                          f"{el:.1f} s"}
     return out
+
+
+def batch_leg(runner, steps, nframes):
+    """The mixed batch: `nframes` frames drawn from 426x240 .. 3840x2160,
+    8/10-bit, 4:2:0/4:4:4, random legal deblock/CDEF/LR parameters, through
+    deblock -> CDEF -> LR with one av1pf_apply_batch call per step; luma Mpx/s
+    and samples/s over the whole batch (inputs resident in HBM)."""
+    import torch
+    from goav1_amd import synth
+    from goav1_amd.lib import DeviceFrame, DeviceSideData
+    t0 = time.perf_counter()
+    frames = synth.batch_mixed(seed=2, n=nframes)
+    gen_s = time.perf_counter() - t0
+    base = [DeviceFrame(fr) for fr in frames]
+    sides = [DeviceSideData(fr) for fr in frames]
+    stream = torch.cuda.current_stream()
+
+    def copies():
+        row = []
+        for b in base:
+            c = DeviceFrame.__new__(DeviceFrame)
+            c.meta, c.bd = b.meta, b.bd
+            c.planes = [t.clone() for t in b.planes]
+            c.abi = c._abi()
+            row.append(c)
+        return row
+
+    def run(nsteps, profile=False):
+        rows = [copies() for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        if profile:
+            runner.profile(True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for row in rows:
+            runner.apply_batch(row, sides)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        runner.sync()
+        prof = runner.profile_read() if profile else None
+        if profile:
+            runner.profile(False)
+        return ev0.elapsed_time(ev1) / nsteps, prof
+
+    run(2)
+    ms, _ = run(steps)
+    _, prof = run(2, profile=True)
+    luma = sum(fr.w * fr.h for fr in frames)
+    samples = sum(fr.samples for fr in frames)
+    stage_bytes = sum(2 * fr.samples * (1 if fr.bd == 8 else 2) for fr in frames)
+    stages = {k: {"avg_us": round(1e3 * v[0] / v[1], 1),
+                  "gbs": round((v[2] / v[1]) / ((v[0] / v[1]) / 1e3) / 1e9, 1) if v[2] else None}
+              for k, v in prof.items()}
+    return {"workload": f"{nframes} frames drawn from 426x240..3840x2160, 8/10-bit, 4:2:0/4:4:4, random legal "
+                        "deblock/CDEF/LR parameters, deblock+CDEF+LR, one av1pf_apply_batch per step",
+            "mpx_s": round(luma / (ms / 1e3) / 1e6, 1), "msamples_per_s": round(samples / (ms / 1e3) / 1e6, 1),
+            "ms_per_batch": round(ms, 3), "luma_mpx_per_batch": round(luma / 1e6, 2),
+            "algorithmic_bytes_per_stage_pass": stage_bytes, "stages": stages,
+            "generation_s": round(gen_s, 1)}
 
 
 def traffic_for(kernel):
@@ -283,6 +345,8 @@ def main():
     }
     if not args.no_search:
         out["search"] = search_leg(runner, args.steps, not args.no_cpu_baseline)
+    if args.batch_frames:
+        out["batch"] = batch_leg(runner, max(2, args.steps // 2), args.batch_frames)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds / 2, cores)

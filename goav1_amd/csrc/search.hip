@@ -280,14 +280,13 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
 // pairs depend on the data, and they live in FinishState on the device.
 //  - fin_first_kernel: slot 0, luma and chroma chosen separately (lowest
 //    column sums over the active filter blocks);
-//  - fin_gain_kernel: slot > 0; thread = one (luma j, chroma k) pair, summing
-//    over the active blocks how much the pair would lower each block's best
-//    cost; per-workgroup argmax, and the last workgroup to finish (atomic
-//    ticket) picks the global first maximum in (j, k) order;
+//  - fin_gain_kernel: slot > 0; thread = one (luma j, chroma k) pair over a
+//    group of 32 active blocks, summing how much the pair would lower each
+//    block's best cost into a gain table; the last workgroup to finish
+//    (atomic ticket) picks the first maximum in (j, k) order;
 //  - both end with the same epilogue: update the level's cost and the best
 //    set at the end of a level, and drop the oldest pair before a refinement;
 //  - fin_assign_kernel: each block's index = its cheapest chosen pair.
-#define FIN_WG 16   // 16 x 256 threads = the 4096 (j, k) pairs
 
 struct FinishState {
     int ly[8], lc[8];
@@ -295,8 +294,8 @@ struct FinishState {
     unsigned long long dist, best_cost;
     int best_bits;
     unsigned int ticket;
-    unsigned long long wg_max[FIN_WG];
-    int wg_pos[FIN_WG];
+    unsigned long long gain[4096];   // (j, k) gains of the current step, cleared after each step
+    unsigned long long base;         // sum of the blocks' best costs of the current step
     int active[4096];      // active filter blocks (at most 4096 = 4096x4096 luma)
     int nact;
 };
@@ -347,22 +346,36 @@ __global__ __launch_bounds__(64) void fin_init_kernel(const uint64_t *noskip, in
         st->best_cost = 1ull << 63;
         st->best_bits = 0;
         st->ticket = 0;
+        st->base = 0;
         for (int i = 0; i < 8; i++) { st->ly[i] = st->lc[i] = 0; st->ys[i] = st->us[i] = 0; }
     }
+    for (int i = lane; i < 4096; i += 64) st->gain[i] = 0;
 }
 
-__global__ __launch_bounds__(128) void fin_first_kernel(const uint64_t *mse, int nfb, FinishState *st, int nb, int bits,
-                                                        int end_level, int shift_after, unsigned long long lambda)
+__global__ __launch_bounds__(1024) void fin_first_kernel(const uint64_t *mse, int nfb, FinishState *st, int nb, int bits,
+                                                         int end_level, int shift_after, unsigned long long lambda)
 {
+    // 128 column sums (2 planes x 64 strengths), 8 threads per column over
+    // interleaved blocks, then a reduction in LDS
+    __shared__ unsigned long long part[8][128];
     __shared__ unsigned long long sums[128];
     const int t = threadIdx.x;
-    const int plane = t >> 6, g = t & 63;
+    const int col = t & 127, sub = t >> 7;
+    const int plane = col >> 6, g = col & 63;
+    const int nact = st->nact;
     unsigned long long s = 0;
-    for (int i = 0; i < st->nact; i++) s += mse[((size_t)plane * nfb + st->active[i]) * 64 + g];
-    sums[t] = s;
+#pragma unroll 4
+    for (int i = sub; i < nact; i += 8) s += mse[((size_t)plane * nfb + st->active[i]) * 64 + g];
+    part[sub][col] = s;
+    __syncthreads();
+    if (t < 128) {
+        unsigned long long v = 0;
+        for (int q = 0; q < 8; q++) v += part[q][t];
+        sums[t] = v;
+    }
     __syncthreads();
     if (t == 0) {
-        if (st->nact == 0) {
+        if (nact == 0) {
             st->ly[0] = st->lc[0] = 0;
             st->dist = 0;
         } else {
@@ -379,43 +392,83 @@ __global__ __launch_bounds__(128) void fin_first_kernel(const uint64_t *mse, int
     }
 }
 
+#define FIN_FBS 32   // active blocks per workgroup of the gain step
+
+// Grid (16, ceil(nfb / 32)): workgroup (x, y) scores the 256 pairs of luma
+// strengths 4x..4x+3 over active blocks 32y..32y+31, staged in LDS, and adds
+// its partial gains to the state's gain table; the last workgroup to finish
+// picks the first maximum in (j, k) order and clears the table for the next
+// step.
 __global__ __launch_bounds__(256) void fin_gain_kernel(const uint64_t *mse, int nfb, FinishState *st, int slot, int nb,
                                                        int bits, int end_level, int shift_after, unsigned long long lambda)
 {
-    __shared__ unsigned long long best[4096];
+    __shared__ unsigned long long cch[FIN_FBS][64];
+    __shared__ unsigned long long lch[FIN_FBS][4];
+    __shared__ unsigned long long bch[FIN_FBS];
+    __shared__ int fbs[FIN_FBS];
     __shared__ unsigned long long rmax[256];
     __shared__ int rpos[256];
     __shared__ bool last;
     const int t = threadIdx.x;
     const int nact = st->nact;
     const uint64_t *L = mse, *Cm = mse + (size_t)nfb * 64;
-    int ly[8], lc[8];
-#pragma unroll
-    for (int g = 0; g < 8; g++) { ly[g] = st->ly[g]; lc[g] = st->lc[g]; }
-    for (int i = t; i < nact; i += 256) {
-        const int fb = st->active[i];
-        unsigned long long b = L[(size_t)fb * 64 + ly[0]] + Cm[(size_t)fb * 64 + lc[0]];
-        for (int g = 1; g < slot; g++) {
-            const unsigned long long c = L[(size_t)fb * 64 + ly[g]] + Cm[(size_t)fb * 64 + lc[g]];
-            if (c < b) b = c;
-        }
-        best[i] = b;
-    }
-    __syncthreads();
     const int pair = blockIdx.x * 256 + t, j = pair >> 6, k = pair & 63;
-    unsigned long long gain = 0;
-    for (int i = 0; i < nact; i++) {
-        const int fb = st->active[i];
-        const unsigned long long b = best[i], yj = L[(size_t)fb * 64 + j];
-        if (yj < b) {
-            const unsigned long long room = b - yj, ck = Cm[(size_t)fb * 64 + k];
-            if (ck < room) gain += room - ck;
+    const int j0 = blockIdx.x * 4, jq = j - j0;
+    const int c0 = blockIdx.y * FIN_FBS;
+    const int n = max(0, min(FIN_FBS, nact - c0));
+    if (n > 0) {
+        if (t < n) fbs[t] = st->active[c0 + t];
+        __syncthreads();
+        for (int e = t; e < n * 64; e += 256) {
+            const int i = e >> 6, kk = e & 63;
+            cch[i][kk] = Cm[(size_t)fbs[i] * 64 + kk];
         }
+        if (t < n * 4) {
+            const int i = t >> 2, q = t & 3;
+            lch[i][q] = L[(size_t)fbs[i] * 64 + j0 + q];
+        }
+        if (t < n) {
+            const int fb = fbs[t];
+            unsigned long long b = L[(size_t)fb * 64 + st->ly[0]] + Cm[(size_t)fb * 64 + st->lc[0]];
+            for (int g = 1; g < slot; g++) {
+                const unsigned long long c = L[(size_t)fb * 64 + st->ly[g]] + Cm[(size_t)fb * 64 + st->lc[g]];
+                if (c < b) b = c;
+            }
+            bch[t] = b;
+            if (blockIdx.x == 0) atomicAdd(&st->base, b);
+        }
+        __syncthreads();
+        unsigned long long gain = 0;
+        for (int i = 0; i < n; i++) {
+            const unsigned long long b = bch[i], yj = lch[i][jq];
+            if (yj < b) {
+                const unsigned long long room = b - yj, ck = cch[i][k];
+                if (ck < room) gain += room - ck;
+            }
+        }
+        if (gain) atomicAdd(&st->gain[pair], gain);
     }
-    rmax[t] = gain;
-    rpos[t] = pair;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {   // first maximum in pair order
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&st->ticket, 1u) == gridDim.x * gridDim.y - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // first maximum over the 4096 pairs; the table is read and cleared with
+    // atomic exchanges (the partial sums were added at device scope)
+    unsigned long long *G = st->gain;
+    unsigned long long m = atomicExch(&G[t], 0ull);
+    int p = t;
+    for (int q = 1; q < 16; q++) {
+        const unsigned long long v = atomicExch(&G[t + 256 * q], 0ull);
+        if (v > m) { m = v; p = t + 256 * q; }
+    }
+    rmax[t] = m;
+    rpos[t] = p;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
         if (t < o) {
             const unsigned long long a = rmax[t], b = rmax[t + o];
             if (b > a || (b == a && rpos[t + o] < rpos[t])) { rmax[t] = b; rpos[t] = rpos[t + o]; }
@@ -423,25 +476,10 @@ __global__ __launch_bounds__(256) void fin_gain_kernel(const uint64_t *mse, int 
         __syncthreads();
     }
     if (t == 0) {
-        st->wg_max[blockIdx.x] = rmax[0];
-        st->wg_pos[blockIdx.x] = rpos[0];
-        __threadfence();
-        last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last && t == 0) {
-        __threadfence();
-        const volatile unsigned long long *wm = st->wg_max;
-        const volatile int *wp = st->wg_pos;
-        unsigned long long m = wm[0];
-        int p = wp[0];
-        for (int w = 1; w < (int)gridDim.x; w++)
-            if (wm[w] > m) { m = wm[w]; p = wp[w]; }   // workgroups cover ascending pair ranges
-        unsigned long long base = 0;
-        for (int i = 0; i < nact; i++) base += best[i];
-        st->ly[slot] = p >> 6;
-        st->lc[slot] = p & 63;
-        st->dist = base - m;
+        const int pb = rpos[0];
+        st->ly[slot] = pb >> 6;
+        st->lc[slot] = pb & 63;
+        st->dist = atomicExch(&st->base, 0ull) - rmax[0];
         st->ticket = 0;
         fin_epilogue(st, nb, bits, end_level, shift_after, lambda);
     }
@@ -489,11 +527,11 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
             const int end_level = k == steps - 1;
             const int shift_after = k + 1 >= nb && k + 1 < steps;   // the next step is a refinement
             if (slot == 0)
-                hipLaunchKernelGGL(fin_first_kernel, dim3(1), dim3(128), 0, s, mse, nfb, st, nb, bits, end_level,
+                hipLaunchKernelGGL(fin_first_kernel, dim3(1), dim3(1024), 0, s, mse, nfb, st, nb, bits, end_level,
                                    shift_after, (unsigned long long)lambda);
             else
-                hipLaunchKernelGGL(fin_gain_kernel, dim3(FIN_WG), dim3(256), 0, s, mse, nfb, st, slot, nb, bits,
-                                   end_level, shift_after, (unsigned long long)lambda);
+                hipLaunchKernelGGL(fin_gain_kernel, dim3(16, (nfb + FIN_FBS - 1) / FIN_FBS), dim3(256), 0, s, mse, nfb,
+                                   st, slot, nb, bits, end_level, shift_after, (unsigned long long)lambda);
         }
     }
     hipLaunchKernelGGL(fin_assign_kernel, dim3(1), dim3(256), 0, s, mse, nfb, st, dec, fb_index);

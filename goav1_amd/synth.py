@@ -84,13 +84,15 @@ def fill_blocky(fr, rng, noise_sigma=4.0):
     scale = 1 << (fr.bd - 8)
     for pl in fr.planes:
         H, W = pl.alloc_h, pl.alloc_w
-        dc = rng.integers(16, 236, size=((H + 7) // 8, (W + 7) // 8)).astype(np.float64)
+        dc = rng.integers(16, 236, size=((H + 7) // 8, (W + 7) // 8)).astype(np.float32)
         dc = np.repeat(np.repeat(dc, 8, 0), 8, 1)[:H, :W]
-        g0, g1, g2, g3 = rng.uniform(-40, 40, size=4)
-        yy, xx = np.mgrid[0:H, 0:W]
-        u, v = xx / max(W - 1, 1), yy / max(H - 1, 1)
-        grad = g0 * (1 - u) * (1 - v) + g1 * u * (1 - v) + g2 * (1 - u) * v + g3 * u * v
-        img = (dc * 0.75 + 32 + grad + rng.normal(0, noise_sigma, size=(H, W))) * scale
+        g0, g1, g2, g3 = (float(v) for v in rng.uniform(-40, 40, size=4))
+        u = np.linspace(0, 1, W, dtype=np.float32)[None, :]
+        v = np.linspace(0, 1, H, dtype=np.float32)[:, None]
+        grad = (g0 * (1 - v) + g2 * v) * (1 - u) + (g1 * (1 - v) + g3 * v) * u
+        img = dc * 0.75 + 32 + grad
+        img += rng.standard_normal((H, W), dtype=np.float32) * noise_sigma
+        img *= scale
         pl.pix[:] = np.clip(np.rint(img), 0, (1 << fr.bd) - 1).astype(np.uint16)
 
 
