@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libav1pf.so")
 SOURCES = ["av1pf_api.hip", "deblock.hip", "cdef.hip", "lr.hip", "grain.hip", "search.hip",
-           "lr_search.hip", "search_finish.cpp"]
+           "lr_search.hip", "nn.hip", "search_finish.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

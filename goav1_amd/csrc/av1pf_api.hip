@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 #include "pf_internal.h"
+#include "nn_internal.h"
 
 using namespace pf;
 
@@ -77,7 +78,7 @@ struct av1pf_ctx {
 namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
-       ST_LR_SEARCH };
+       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -915,6 +916,148 @@ int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *rec
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_LR_SEARCH, s, 2.0 * W * H);
     HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, units, sse, s));
+    return AV1PF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposal network (nn.hip): parameters, workspace, passes.
+struct av1pf_nn {
+    av1pf_ctx *ctx;
+    int max_batch;
+    NNShape shape;
+    float *P = nullptr, *m = nullptr, *v = nullptr;   // FP32 master weights, Adam moments
+    uint16_t *Pb = nullptr;                           // BF16 copy read by the GEMMs
+    char *work = nullptr;
+    NNWork ws;
+    NNWeights w;
+    int step = 0;
+};
+
+int64_t av1pf_nn_param_count(void) { return nn_shape().total; }
+
+void av1pf_nn_destroy(av1pf_nn *nn)
+{
+    if (!nn) return;
+    hipSetDevice(nn->ctx->device);
+    hipDeviceSynchronize();
+    hipFree(nn->P);
+    hipFree(nn->m);
+    hipFree(nn->v);
+    hipFree(nn->Pb);
+    hipFree(nn->work);
+    delete nn;
+}
+
+int av1pf_nn_create(av1pf_ctx *c, int max_batch, av1pf_nn **out)
+{
+    if (!c || !out || max_batch < 1 || max_batch > 4096) return AV1PF_ERR_INVALID_ARG;
+    *out = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    av1pf_nn *nn = new av1pf_nn();
+    nn->ctx = c;
+    nn->max_batch = max_batch;
+    nn->shape = nn_shape();
+    const size_t P = (size_t)nn->shape.total;
+    bool ok = hipMalloc(&nn->P, P * 4) == hipSuccess && hipMalloc(&nn->m, P * 4) == hipSuccess &&
+              hipMalloc(&nn->v, P * 4) == hipSuccess && hipMalloc(&nn->Pb, P * 2) == hipSuccess &&
+              hipMalloc(&nn->work, nn_work_bytes(max_batch)) == hipSuccess;
+    if (!ok) {
+        av1pf_nn_destroy(nn);
+        return fail(c, AV1PF_ERR_DEVICE, "out of device memory for the network");
+    }
+    nn_work_carve(nn->work, max_batch, nn->ws);
+    for (int l = 0; l < NN_LAYERS; l++) {
+        nn->w.wb[l] = nn->Pb + nn->shape.woff[l];
+        nn->w.b[l] = nn->P + nn->shape.boff[l];
+        nn->w.woff[l] = nn->shape.woff[l];
+        nn->w.boff[l] = nn->shape.boff[l];
+    }
+    *out = nn;
+    return AV1PF_OK;
+}
+
+int av1pf_nn_set_params(av1pf_nn *nn, const float *params)
+{
+    if (!nn || !params) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t P = (size_t)nn->shape.total;
+    std::vector<uint16_t> b(P);
+    for (size_t i = 0; i < P; i++) {
+        uint32_t u;
+        memcpy(&u, &params[i], 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        b[i] = (uint16_t)(u >> 16);
+    }
+    HIPCHK(c, hipMemcpy(nn->P, params, P * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(nn->Pb, b.data(), P * 2, hipMemcpyHostToDevice));
+    std::vector<float> z(P, 0.f);
+    HIPCHK(c, hipMemcpy(nn->m, z.data(), P * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(nn->v, z.data(), P * 4, hipMemcpyHostToDevice));
+    nn->step = 0;
+    return AV1PF_OK;
+}
+
+int av1pf_nn_get_params(av1pf_nn *nn, float *params)
+{
+    if (!nn || !params) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
+    HIPCHK(c, hipMemcpy(params, nn->P, (size_t)nn->shape.total * 4, hipMemcpyDeviceToHost));
+    return AV1PF_OK;
+}
+
+int av1pf_nn_tiles(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, uint16_t *tiles, void *stream)
+{
+    if (!c || !tiles) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_frame(c, src);
+    if (rc) return rc;
+    if ((rc = validate_frame(c, recon))) return rc;
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != recon->bit_depth)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "source and reconstruction differ in geometry");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64;
+    const FrameView sv = view_of(src), rv = view_of(recon);
+    HIPCHK(c, launch_tiles(sv.p[0], rv.p[0], recon->bit_depth, nhfb, nhfb * nvfb, tiles, static_cast<hipStream_t>(stream)));
+    return AV1PF_OK;
+}
+
+int av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits, void *stream)
+{
+    if (!nn || !tiles || !logits || n < 0) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_NN_FORWARD, s, 0);
+    for (int i = 0; i < n; i += nn->max_batch) {
+        const int b = std::min(nn->max_batch, n - i);
+        HIPCHK(c, nn_forward(nn->w, nn->ws, tiles + (size_t)i * 4096 * 2, b, logits + (size_t)i * NN_OUT, s));
+    }
+    return AV1PF_OK;
+}
+
+int av1pf_nn_backward(av1pf_nn *nn, const uint16_t *tiles, int n, const float *targets, float *grads, float *loss,
+                      void *stream)
+{
+    if (!nn || !tiles || !targets || !grads || !loss || n < 1 || n > nn->max_batch) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_NN_TRAIN, s, 0);
+    HIPCHK(c, nn_zero(loss, 1, s));
+    HIPCHK(c, nn_backward(nn->w, nn->ws, tiles, n, targets, grads, loss, s));
+    return AV1PF_OK;
+}
+
+int av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_decay, void *stream)
+{
+    if (!nn || !grads || !(lr > 0.f)) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    nn->step++;
+    HIPCHK(c, nn_adam(nn->P, nn->Pb, grads, nn->m, nn->v, (int)nn->shape.total, lr, nn->step, weight_decay,
+                      static_cast<hipStream_t>(stream)));
     return AV1PF_OK;
 }
 

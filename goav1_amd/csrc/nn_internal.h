@@ -1,0 +1,70 @@
+// Internal declarations of the learned-proposal network (nn.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+constexpr int NN_OUT = 131;        // 64 luma + 64 chroma strength logits, 3 LR-type logits
+constexpr int NN_LAYERS = 5;       // conv1..3, fc1, heads
+
+// Parameter layout of the flat FP32 vector: per layer W [K][N] row-major then b [N].
+struct NNShape {
+    int K[NN_LAYERS], N[NN_LAYERS];
+    int64_t woff[NN_LAYERS], boff[NN_LAYERS], total;
+};
+
+inline NNShape nn_shape()
+{
+    NNShape s{};
+    const int K[NN_LAYERS] = { 18, 144, 288, 64, 128 }, N[NN_LAYERS] = { 16, 32, 64, 128, NN_OUT };
+    int64_t o = 0;
+    for (int l = 0; l < NN_LAYERS; l++) {
+        s.K[l] = K[l];
+        s.N[l] = N[l];
+        s.woff[l] = o;
+        o += (int64_t)K[l] * N[l];
+        s.boff[l] = o;
+        o += N[l];
+    }
+    s.total = o;
+    return s;
+}
+
+struct NNWeights {
+    const uint16_t *wb[NN_LAYERS];   // BF16 weights (GEMM operands)
+    const float *b[NN_LAYERS];       // FP32 biases
+    int64_t woff[NN_LAYERS], boff[NN_LAYERS];
+};
+
+struct NNWork {
+    uint16_t *cols[3];   // im2col patches per conv layer (kept for the backward pass)
+    uint16_t *act[3];    // conv outputs after ReLU (BF16, NHWC)
+    uint16_t *pool, *fc1;
+    float *logits, *dlog, *dfc1f, *dpoolf, *dcols;
+    uint16_t *dlogb, *dfc1b, *dact[3];
+};
+
+struct GemmArgs {
+    const uint16_t *A; int64_t sam, sak;
+    const uint16_t *B; int64_t sbk, sbn;
+    int M, N, K;
+    const float *bias;
+    int relu;
+    float *C;          // FP32 output (or null)
+    uint16_t *Cb;      // BF16 output (or null)
+    int64_t ldc;
+};
+
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
+hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s);
+hipError_t nn_forward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, float *logits, hipStream_t s);
+hipError_t nn_backward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, const float *target,
+                       float *grads, float *loss, hipStream_t s);
+hipError_t nn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, int n, float lr, int step, float wd,
+                   hipStream_t s);
+hipError_t nn_zero(float *p, int64_t n, hipStream_t s);
+size_t nn_work_bytes(int max_batch);
+void nn_work_carve(char *base, int max_batch, NNWork &ws);
+
+}  // namespace pf

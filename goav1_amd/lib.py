@@ -23,9 +23,12 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_apply_batch", "av1pf_sync", "av1pf_deblock", "av1pf_cdef", "av1pf_lr_save_boundaries",
            "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_cdef_search_finish", "av1pf_cdef_lambda",
            "av1pf_cdef_search_select", "av1pf_lr_search",
+           "av1pf_nn_param_count", "av1pf_nn_create", "av1pf_nn_destroy", "av1pf_nn_set_params",
+           "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_backward", "av1pf_nn_adam",
            "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
-               "cdef_search_select", "lr_search"] + [f"reserved{i}" for i in range(10, 16)]
+               "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned"] + \
+    [f"reserved{i}" for i in range(13, 16)]
 PROFILE_STAGES = 16
 
 
@@ -66,6 +69,16 @@ def load():
                                              C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_lr_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_param_count.restype = C.c_int64
+    lib.av1pf_nn_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+    lib.av1pf_nn_destroy.argtypes = [C.c_void_p]
+    lib.av1pf_nn_destroy.restype = None
+    lib.av1pf_nn_set_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_get_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_tiles.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
     lib.av1pf_cdef_lambda.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.av1pf_cdef_lambda.restype = C.c_uint64
     lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
@@ -296,3 +309,59 @@ def cdef_search_finish(mse, skip_fb, lam):
 
 def cdef_lambda(dc_quant, key_frame=True, gf_update=False, tl_factor=128):
     return int(load().av1pf_cdef_lambda(int(dc_quant), int(key_frame), int(gf_update), int(tl_factor)))
+
+
+class ProposalNet:
+    """The learned-proposal CNN on the device (parameters, workspace, passes)."""
+
+    def __init__(self, runner, max_batch=256):
+        self.runner = runner
+        self.lib = runner.lib
+        h = C.c_void_p()
+        runner._check(self.lib.av1pf_nn_create(runner.h, int(max_batch), C.byref(h)))
+        self.h = h
+        self.max_batch = max_batch
+        self.nparams = int(self.lib.av1pf_nn_param_count())
+
+    def close(self):
+        if self.h:
+            self.lib.av1pf_nn_destroy(self.h)
+            self.h = None
+
+    def set_params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        assert P.size == self.nparams
+        self.runner._check(self.lib.av1pf_nn_set_params(self.h, P.ctypes.data))
+
+    def get_params(self):
+        P = np.zeros(self.nparams, np.float32)
+        self.runner._check(self.lib.av1pf_nn_get_params(self.h, P.ctypes.data))
+        return P
+
+    def tiles(self, src, rec, out=None, stream=None):
+        """BF16 tiles [nfb][64][64][2] (torch int16 tensor) of a DeviceFrame pair."""
+        torch = _torch()
+        nfb = ((rec.meta.w + 63) // 64) * ((rec.meta.h + 63) // 64)
+        if out is None:
+            out = torch.empty((nfb, 64, 64, 2), dtype=torch.int16, device="cuda")
+        self.runner._check(self.lib.av1pf_nn_tiles(self.runner.h, C.byref(src.abi), C.byref(rec.abi), out.data_ptr(),
+                                                   self.runner._stream(stream)))
+        return out
+
+    def forward(self, tiles, n=None, out=None, stream=None):
+        torch = _torch()
+        n = tiles.shape[0] if n is None else n
+        if out is None:
+            out = torch.empty((n, 131), dtype=torch.float32, device="cuda")
+        self.runner._check(self.lib.av1pf_nn_forward(self.h, tiles.data_ptr(), int(n), out.data_ptr(),
+                                                     self.runner._stream(stream)))
+        return out
+
+    def backward(self, tiles, targets, grads, loss, n=None, stream=None):
+        n = tiles.shape[0] if n is None else n
+        self.runner._check(self.lib.av1pf_nn_backward(self.h, tiles.data_ptr(), int(n), targets.data_ptr(),
+                                                      grads.data_ptr(), loss.data_ptr(), self.runner._stream(stream)))
+
+    def adam(self, grads, lr, weight_decay=0.0, stream=None):
+        self.runner._check(self.lib.av1pf_nn_adam(self.h, grads.data_ptr(), float(lr), float(weight_decay),
+                                                  self.runner._stream(stream)))

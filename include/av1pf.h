@@ -258,6 +258,33 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
 int  av1pf_lr_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
                      av1pf_lr_unit *units, uint64_t *sse, void *stream);
 
+/* ---- learned proposal network ---------------------------------------------
+ * No counterpart in the reference: a 48,659-parameter CNN over 64x64 luma
+ * (source, reconstruction) tiles, BF16 on the matrix cores with FP32
+ * accumulation, whose 131 outputs are logits of the best luma CDEF strength
+ * (64, search-level-1 list order), the best chroma strength (64) and the
+ * loop-restoration unit type (3: NONE, WIENER, SGRPROJ).  Trained by the
+ * library's own forward/backward kernels (softmax cross-entropy against
+ * target distributions, Adam on FP32 master weights).
+ * Tiles: BF16, [nfb][64][64][2] (channel 0 source, 1 reconstruction, luma
+ * divided by 2^bd - 1), one per 64x64 filter block in raster order, edge
+ * samples repeated past the frame.  Parameters: FP32, per layer W [K][N]
+ * row-major then b [N], layers conv1 (K 18: 3x3 taps x 2 channels, tap-major),
+ * conv2 (144), conv3 (288), fc1 (64 -> 128), heads (128 -> 131).           */
+typedef struct av1pf_nn av1pf_nn;
+int64_t av1pf_nn_param_count(void);
+int  av1pf_nn_create(av1pf_ctx *ctx, int max_batch, av1pf_nn **out);
+void av1pf_nn_destroy(av1pf_nn *nn);
+int  av1pf_nn_set_params(av1pf_nn *nn, const float *params);     /* host FP32; resets the optimizer */
+int  av1pf_nn_get_params(av1pf_nn *nn, float *params);           /* host FP32 */
+int  av1pf_nn_tiles(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, uint16_t *tiles, void *stream);
+int  av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits, void *stream);
+/* forward + backward of n <= max_batch tiles against targets [n][131] (three
+ * distributions); writes grads [param_count] and the mean loss [1]          */
+int  av1pf_nn_backward(av1pf_nn *nn, const uint16_t *tiles, int n, const float *targets, float *grads, float *loss,
+                       void *stream);
+int  av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_decay, void *stream);
+
 /* ---- per-stage timing (no counterpart in the reference, whose closest aid is
  * the FrameWorkPostFilterResult counters, postfilter.go:309-316) ----------
  * While enabled, every stage launch is bracketed by HIP events on its stream
@@ -266,7 +293,7 @@ int  av1pf_lr_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *
  * device and returns, per stage id, total ms, launch count and bytes.
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
- * 10..15 reserved.                                                          */
+ * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13..15 reserved.     */
 #define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
