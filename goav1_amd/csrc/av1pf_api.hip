@@ -53,6 +53,8 @@ struct av1pf_ctx {
     int tpar = 0;
     hipEvent_t uploaded = nullptr;
     void *finish_state = nullptr;  // device CDEF strength selection state
+    char *learn_buf = nullptr;     // learned search: tiles, logits, shortlist table, LR mask
+    size_t learn_cap = 0;
     int *dev_err = nullptr;       // device-side validation flag
     hipStream_t side = nullptr;   // film-grain template stream
     hipEvent_t join = nullptr;
@@ -715,6 +717,7 @@ void av1pf_destroy(av1pf_ctx *c)
     }
     if (c->uploaded) hipEventDestroy(c->uploaded);
     if (c->finish_state) hipFree(c->finish_state);
+    if (c->learn_buf) hipFree(c->learn_buf);
     if (c->join) hipEventDestroy(c->join);
     for (int i = 0; i < 2; i++) {
         if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
@@ -898,8 +901,8 @@ int av1pf_cdef_search_select(av1pf_ctx *c, const av1pf_frame *recon, const uint6
     return AV1PF_OK;
 }
 
-int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
-                    av1pf_lr_unit *units, uint64_t *sse, void *stream)
+int av1pf_lr_search_masked(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                           const uint8_t *unit_mask, av1pf_lr_unit *units, uint64_t *sse, void *stream)
 {
     if (!c || !units) return AV1PF_ERR_INVALID_ARG;
     int rc = validate_frame(c, src);
@@ -915,7 +918,7 @@ int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *rec
     const FrameView sv = view_of(src), rv = view_of(recon);
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_LR_SEARCH, s, 2.0 * W * H);
-    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, units, sse, s));
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, unit_mask, units, sse, s));
     return AV1PF_OK;
 }
 
@@ -1058,6 +1061,83 @@ int av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_decay
     nn->step++;
     HIPCHK(c, nn_adam(nn->P, nn->Pb, grads, nn->m, nn->v, (int)nn->shape.total, lr, nn->step, weight_decay,
                       static_cast<hipStream_t>(stream)));
+    return AV1PF_OK;
+}
+
+int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                    av1pf_lr_unit *units, uint64_t *sse, void *stream)
+{
+    return av1pf_lr_search_masked(c, src, recon, unit_size, nullptr, units, sse, stream);
+}
+
+namespace {
+int validate_search_pair(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon)
+{
+    int rc = validate_frame(c, src);
+    if (rc) return rc;
+    if ((rc = validate_frame(c, recon))) return rc;
+    if (src->width != recon->width || src->height != recon->height || recon->bit_depth != 8 ||
+        src->bit_depth != 8 || recon->ss_x != 1 || recon->ss_y != 1 || src->ss_x != 1 || src->ss_y != 1 ||
+        recon->mono || (recon->width & 7) || (recon->height & 7))
+        return fail(c, AV1PF_ERR_INVALID_ARG, "search needs matching 8-bit 4:2:0 frames with sizes a multiple of 8");
+    return AV1PF_OK;
+}
+}  // namespace
+
+int av1pf_cdef_search_shortlist(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon,
+                                const uint64_t *noskip, int damping, uint64_t lambda, const int32_t *cand,
+                                uint64_t *mse_sub, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream)
+{
+    if (!c || !cand || !mse_sub || !decision || !fb_index) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_search_pair(c, src, recon);
+    if (rc) return rc;
+    if (damping < 3 || damping > 6) return fail(c, AV1PF_ERR_INVALID_ARG, "damping out of range");
+    const int nfb = ((recon->width + 63) / 64) * ((recon->height + 63) / 64);
+    if (nfb > 2048) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 2048 filter blocks");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
+    HIPCHK(c, launch_search_learned(view_of(src), view_of(recon), noskip, damping, lambda, nullptr,
+                                    const_cast<int32_t *>(cand), nullptr, mse_sub, decision, fb_index, false, s));
+    return AV1PF_OK;
+}
+
+int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *recon,
+                         const uint64_t *noskip, int damping, uint64_t lambda, float lr_none_threshold,
+                         av1pf_cdef_decision *decision, uint8_t *fb_index, av1pf_lr_unit *lr_units, int32_t *cand,
+                         float *conf, void *stream)
+{
+    if (!c || !nn || !decision || !fb_index || !lr_units || !cand) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_search_pair(c, src, recon);
+    if (rc) return rc;
+    if (damping < 3 || damping > 6) return fail(c, AV1PF_ERR_INVALID_ARG, "damping out of range");
+    const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64, nfb = nhfb * nvfb;
+    if (nfb > 2048) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 2048 filter blocks");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t tiles_b = (size_t)nfb * 4096 * 2 * 2, logits_b = (size_t)nfb * NN_OUT * 4;
+    const size_t msub_b = (size_t)2 * nfb * NCAND * 8, mask_b = (size_t)nfb + 256;
+    const size_t need = tiles_b + logits_b + msub_b + mask_b + 4 * 256 + 64;
+    if ((rc = ensure(c, (void **)&c->learn_buf, &c->learn_cap, need))) return rc;
+    char *p = c->learn_buf;
+    uint16_t *tiles = (uint16_t *)p; p += align256(tiles_b);
+    float *logits = (float *)p; p += align256(logits_b);
+    uint64_t *msub = (uint64_t *)p; p += align256(msub_b);
+    uint8_t *mask = (uint8_t *)p; p += align256(mask_b);
+    float *conf_tmp = (float *)p;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
+    const FrameView sv = view_of(src), rv = view_of(recon);
+    HIPCHK(c, launch_tiles(sv.p[0], rv.p[0], 8, nhfb, nfb, tiles, s));
+    for (int i = 0; i < nfb; i += nn->max_batch) {
+        const int b = std::min(nn->max_batch, nfb - i);
+        HIPCHK(c, nn_forward(nn->w, nn->ws, tiles + (size_t)i * 4096 * 2, b, logits + (size_t)i * NN_OUT, s));
+    }
+    HIPCHK(c, launch_search_learned(sv, rv, noskip, damping, lambda, logits, cand, conf ? conf : conf_tmp, msub,
+                                    decision, fb_index, true, s));
+    const int W = recon->width, H = recon->height;
+    const int hz = std::max((W + 32) / 64, 1), vt = std::max((H + 32) / 64, 1);
+    HIPCHK(c, launch_lr_mask(logits, nhfb, nvfb, hz, vt, lr_none_threshold, mask, s));
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], 64, hz, vt, mask, lr_units, nullptr, s));
     return AV1PF_OK;
 }
 

@@ -38,7 +38,7 @@ __device__ __forceinline__ uint64_t wg_sum(uint64_t v, uint64_t *red)
 }
 
 __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
-                                                        av1pf_lr_unit *out, uint64_t *sse_out)
+                                                        const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse_out)
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     __shared__ uint8_t srcT[64 * 64];
@@ -58,6 +58,16 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
     uint64_t e[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) e[i] = ~0ull;
+    if (mask && !mask[u]) {   // unit left out of the search: NONE, nothing scored
+        if (tid == 0) {
+            av1pf_lr_unit none;
+            for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&none)[i] = 0;
+            out[u] = none;
+            if (sse_out)
+                for (int i = 0; i < 12; i++) sse_out[(size_t)u * 12 + i] = e[i];
+        }
+        return;
+    }
 
     // ---- baseline and the H-only Wiener approximations (any unit size)
     uint64_t b = 0, a[6] = { 0, 0, 0, 0, 0, 0 };
@@ -144,10 +154,30 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
     }
 }
 
-hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, av1pf_lr_unit *out,
-                            uint64_t *sse, hipStream_t s)
+hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
+                            av1pf_lr_unit *out, uint64_t *sse, hipStream_t s)
 {
-    hipLaunchKernelGGL(lr_search_kernel, dim3(hz * vt), dim3(256), 0, s, src, rec, us, hz, vt, out, sse);
+    hipLaunchKernelGGL(lr_search_kernel, dim3(hz * vt), dim3(256), 0, s, src, rec, us, hz, vt, mask, out, sse);
+    return hipGetLastError();
+}
+
+// Units the network expects to stay NONE (P(NONE) above the threshold, from
+// the tile of the co-located 64x64 filter block) are left out of the search.
+__global__ void lr_mask_kernel(const float *logits, int nhfb, int nvfb, int hz, int vt, float thr, uint8_t *mask)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= hz * vt) return;
+    const int fb = min(u / hz, nvfb - 1) * nhfb + min(u % hz, nhfb - 1);
+    const float *z = logits + (size_t)fb * 131 + 128;
+    const float mx = fmaxf(z[0], fmaxf(z[1], z[2]));
+    const float e0 = __expf(z[0] - mx), se = e0 + __expf(z[1] - mx) + __expf(z[2] - mx);
+    mask[u] = e0 / se > thr ? 0 : 1;
+}
+
+hipError_t launch_lr_mask(const float *logits, int nhfb, int nvfb, int hz, int vt, float thr, uint8_t *mask,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(lr_mask_kernel, dim3((hz * vt + 255) / 256), dim3(256), 0, s, logits, nhfb, nvfb, hz, vt, thr, mask);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@ namespace pf {
 
 constexpr int NN_OUT = 131;        // 64 luma + 64 chroma strength logits, 3 LR-type logits
 constexpr int NN_LAYERS = 5;       // conv1..3, fc1, heads
+constexpr int NCAND = 8;           // shortlisted strengths per plane in the learned search
 
 // Parameter layout of the flat FP32 vector: per layer W [K][N] row-major then b [N].
 struct NNShape {
@@ -65,6 +66,9 @@ hipError_t nn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, i
                    hipStream_t s);
 hipError_t nn_zero(float *p, int64_t n, hipStream_t s);
 size_t nn_work_bytes(int max_batch);
+hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
+                                 uint64_t lambda, const float *logits, int *cand, float *conf, uint64_t *mse_sub,
+                                 av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose, hipStream_t s);
 void nn_work_carve(char *base, int max_batch, NNWork &ws);
 
 }  // namespace pf

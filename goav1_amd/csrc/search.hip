@@ -21,6 +21,7 @@
 // workgroup with wave shuffles and a 4-way LDS sum.  The work is integer-VALU
 // bound (64 trials x ~70 ops per pixel).
 #include "pf_internal.h"
+#include "nn_internal.h"
 
 namespace pf {
 
@@ -535,6 +536,377 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
         }
     }
     hipLaunchKernelGGL(fin_assign_kernel, dim3(1), dim3(256), 0, s, mse, nfb, st, dec, fb_index);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Learned-proposal search.  The network proposes a frame shortlist of
+// NCAND luma and NCAND chroma strengths; only those are evaluated exactly,
+// and the same greedy selection runs over the NCAND x NCAND pairs.
+
+// Active filter block flags (a non-skip 8x8 block inside the frame).
+__device__ __forceinline__ bool fb_active(const uint64_t *noskip, int fb, int nhfb, int mi_cols, int mi_rows)
+{
+    const int fbr = fb / nhfb, fbc = fb % nhfb;
+    const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+    uint64_t valid = 0;
+    for (int r = 0; r < nvb; r += 2)
+        for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
+    return ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
+}
+
+// Shortlist from the network's logits: per head, the softmax probabilities of
+// the active blocks are summed and the NCAND most likely strengths kept, in
+// ascending strength-list order; conf = mean over active blocks of the
+// largest luma probability.
+__global__ __launch_bounds__(256) void cand_kernel(const float *logits, const uint64_t *noskip, int nhfb, int nvfb,
+                                                   int mi_cols, int mi_rows, int *cand, float *conf)
+{
+    __shared__ float acc[2][64];
+    __shared__ float cmax;
+    __shared__ int nact;
+    const int t = threadIdx.x;
+    if (t < 128) acc[t >> 6][t & 63] = 0.f;
+    if (t == 0) { cmax = 0.f; nact = 0; }
+    __syncthreads();
+    const int nfb = nhfb * nvfb;
+    for (int fb = t; fb < nfb; fb += 256) {
+        if (!fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) continue;
+        atomicAdd(&nact, 1);
+        const float *z = logits + (size_t)fb * NN_OUT;
+        for (int h = 0; h < 2; h++) {
+            float mx = -1e30f;
+            for (int i = 0; i < 64; i++) mx = fmaxf(mx, z[h * 64 + i]);
+            float se = 0.f;
+            for (int i = 0; i < 64; i++) se += __expf(z[h * 64 + i] - mx);
+            const float inv = 1.f / se;
+            float best = 0.f;
+            for (int i = 0; i < 64; i++) {
+                const float p = __expf(z[h * 64 + i] - mx) * inv;
+                atomicAdd(&acc[h][i], p);
+                best = fmaxf(best, p);
+            }
+            if (h == 0) atomicAdd(&cmax, best);
+        }
+    }
+    __syncthreads();
+    if (t < 2) {
+        bool taken[64];
+        for (int i = 0; i < 64; i++) taken[i] = false;
+        for (int k = 0; k < NCAND; k++) {
+            int bi = -1;
+            for (int i = 0; i < 64; i++)
+                if (!taken[i] && (bi < 0 || acc[t][i] > acc[t][bi])) bi = i;
+            taken[bi] = true;
+        }
+        int k = 0;
+        for (int i = 0; i < 64; i++)
+            if (taken[i]) cand[t * NCAND + k++] = i;
+    }
+    if (t == 0) *conf = nact ? cmax / nact : 1.f;
+}
+
+// Exact SSE of the shortlisted strengths: the MSE kernel's decomposition with
+// NCAND trials per plane read from the shortlist.  mse[2][nfb][NCAND].
+template <bool LUMA>
+__device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t *c, int srcv, int dir, int adj_i,
+                                            bool has_var, int damping, const int *pk)
+{
+    const int x = c[0];
+    const int p0 = kSOff[dir + 2][0], p1 = kSOff[dir + 2][1];
+    const int a0 = c[p0], a1 = c[-p0], a2 = c[p1], a3 = c[-p1];
+    int sb[8], sz[8];
+    {
+        const int o[4] = { kSOff[dir + 4][0], kSOff[dir][0], kSOff[dir + 4][1], kSOff[dir][1] };
+#pragma unroll
+        for (int k = 0; k < 4; k++) { sb[2 * k] = c[o[k]]; sb[2 * k + 1] = c[-o[k]]; }
+        const int z[4] = { kSOff[4][0], kSOff[0][0], kSOff[4][1], kSOff[0][1] };
+#pragma unroll
+        for (int k = 0; k < 4; k++) { sz[2 * k] = c[z[k]]; sz[2 * k + 1] = c[-z[k]]; }
+    }
+    const int mk = SVL - 1;
+    int hi = max(max(max(x, a0 & mk), max(a1 & mk, a2 & mk)), a3 & mk);
+    int lo = min(min(min(x, a0), min(a1, a2)), a3);
+#pragma unroll
+    for (int k = 0; k < 8; k++) { hi = max(hi, sb[k] & mk); lo = min(lo, sb[k]); }
+#pragma unroll
+    for (int q = 0; q < NCAND; q++) {
+        const int packed = pk[q];
+        const int pri = packed >> 2;
+        const int sec = (packed & 3) == 3 ? 4 : (packed & 3);
+        int st = pri;
+        if (LUMA && pri) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
+        int out = x;
+        if (st || sec) {
+            const int ssh = sec ? max(damping - s_ilog2((unsigned)sec), 0) : 0;
+            int sum = 0;
+            const int *t = pri ? sb : sz;
+            if (pri && st) {
+                const int psh = max(damping - s_ilog2((unsigned)st), 0);
+                const int t0 = (st & 1) ? 3 : 4, t1 = (st & 1) ? 3 : 2;
+                sum = t0 * (s_constrain(a0 - x, st, psh) + s_constrain(a1 - x, st, psh)) +
+                      t1 * (s_constrain(a2 - x, st, psh) + s_constrain(a3 - x, st, psh));
+            }
+            if (sec)
+                sum += 2 * (s_constrain(t[0] - x, sec, ssh) + s_constrain(t[1] - x, sec, ssh) +
+                            s_constrain(t[2] - x, sec, ssh) + s_constrain(t[3] - x, sec, ssh)) +
+                       s_constrain(t[4] - x, sec, ssh) + s_constrain(t[5] - x, sec, ssh) +
+                       s_constrain(t[6] - x, sec, ssh) + s_constrain(t[7] - x, sec, ssh);
+            out = x + ((8 + sum - (sum < 0)) >> 4);
+            if (pri && st && sec) out = min(max(out, lo), hi);
+        }
+        const int e = srcv - out;
+        acc[q] += (uint32_t)(e * e);
+    }
+}
+
+__global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
+                                                                 int damping, const int *cand, uint64_t *mse, int nhfb,
+                                                                 int nvfb)
+{
+    __shared__ uint16_t tile[68 * SP];
+    __shared__ uint8_t bdir[64];
+    __shared__ int8_t badj[64];
+    __shared__ uint8_t blist[64];
+    __shared__ int nblk_s;
+    __shared__ uint32_t part[4][NCAND];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int fbc = blockIdx.x, fbr = blockIdx.y;
+    const int fb = fbr * nhfb + fbc;
+    const int nfb = nhfb * nvfb;
+    const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
+    const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+    const uint64_t m = noskip ? noskip[fb] : ~0ull;
+    int pk[2][NCAND];
+#pragma unroll
+    for (int q = 0; q < NCAND; q++) {
+        pk[0][q] = kLevel1[cand[q]];
+        pk[1][q] = kLevel1[cand[NCAND + q]];
+    }
+    if (tid == 0) {
+        int n = 0;
+        for (int r = 0; r < nvb; r += 2)
+            for (int c = 0; c < nhb; c += 2) {
+                const int b = (r >> 1) * 8 + (c >> 1);
+                if ((m >> b) & 1) blist[n++] = (uint8_t)b;
+            }
+        nblk_s = n;
+    }
+    __syncthreads();
+    const int nblk = nblk_s;
+    uint64_t *m0 = mse + (size_t)fb * NCAND, *m1 = mse + ((size_t)nfb + fb) * NCAND;
+    if (nblk == 0) {
+        if (tid < NCAND) { m0[tid] = 0; m1[tid] = 0; }
+        return;
+    }
+    uint32_t acc[NCAND];
+    for (int pl = 0; pl < 3; pl++) {
+        const int sh = pl ? 1 : 0;
+        const int x0 = (fbc * 64) >> sh, y0 = (fbr * 64) >> sh, w = (nhb * 4) >> sh, h = (nvb * 4) >> sh;
+        if (pl < 2) {
+#pragma unroll
+            for (int q = 0; q < NCAND; q++) acc[q] = 0;
+        }
+        __syncthreads();
+        s_load(tile, rec.p[pl], x0, y0, w, h, tid);
+        __syncthreads();
+        if (pl == 0) {
+            if (tid < nblk) {
+                const int b = blist[tid];
+                int d;
+                int32_t v;
+                s_find_dir(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
+                bdir[b] = (uint8_t)d;
+                int i = -1;
+                if (v != 0) {
+                    i = 0;
+                    const int32_t vv = v >> 6;
+                    if (vv != 0) i = min(s_ilog2((unsigned)vv), 12);
+                }
+                badj[b] = (int8_t)i;
+            }
+            __syncthreads();
+            for (int p = tid; p < nblk * 64; p += 256) {
+                const int b = blist[p >> 6], q = p & 63;
+                const int yy = (b >> 3) * 8 + (q >> 3), xx = (b & 7) * 8 + (q & 7);
+                s_pixel_sub<true>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[0], y0 + yy)[x0 + xx],
+                                  bdir[b], badj[b], badj[b] >= 0, damping, pk[0]);
+            }
+        } else {
+            for (int p = tid; p < nblk * 16; p += 256) {
+                const int b = blist[p >> 4], q = p & 15;
+                const int yy = (b >> 3) * 4 + (q >> 2), xx = (b & 7) * 4 + (q & 3);
+                s_pixel_sub<false>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[pl], y0 + yy)[x0 + xx],
+                                   bdir[b], 0, true, damping - 1, pk[1]);
+            }
+        }
+        if (pl == 0 || pl == 2) {
+#pragma unroll
+            for (int q = 0; q < NCAND; q++) {
+                uint32_t v = acc[q];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) part[wv][q] = v;
+            }
+            __syncthreads();
+            if (tid < NCAND) {
+                const uint64_t sum = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+                if (pl == 0) m0[tid] = sum; else m1[tid] = sum;
+            }
+        }
+    }
+}
+
+// Greedy selection over the NCAND x NCAND shortlist pairs in one workgroup:
+// the active blocks' shortlist columns (FP exactness: every 8-bit 4:2:0 SSE
+// fits 32 bits) sit in LDS, and all steps run in this one launch.  Thread
+// (pair, sub) handles blocks sub, sub + 4, ... of pair = t / 4.
+#define SMALL_MAX_FB 2048
+__global__ __launch_bounds__(256) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb,
+                                                        int mi_cols, int mi_rows, unsigned long long lambda,
+                                                        const int *cand, av1pf_cdef_decision *dec, uint8_t *fb_index)
+{
+    __shared__ uint32_t Ly[SMALL_MAX_FB][NCAND];
+    __shared__ uint32_t Lc[SMALL_MAX_FB][NCAND];
+    __shared__ uint16_t act[SMALL_MAX_FB];
+    __shared__ unsigned long long red[256];
+    __shared__ int rp[256];
+    __shared__ int ly[8], lc[8], ys[8], us[8];
+    __shared__ int nact_s;
+    __shared__ unsigned long long dist_s;
+    const int t = threadIdx.x;
+    const int nfb = nhfb * nvfb;
+    if (t == 0) {
+        int n = 0;
+        for (int fb = 0; fb < nfb; fb++)
+            if (fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) act[n++] = (uint16_t)fb;
+        nact_s = n;
+    }
+    __syncthreads();
+    const int nact = nact_s;
+    for (int e = t; e < nact * NCAND; e += 256) {
+        const int i = e / NCAND, q = e % NCAND;
+        Ly[i][q] = (uint32_t)mse[(size_t)act[i] * NCAND + q];
+        Lc[i][q] = (uint32_t)mse[((size_t)nfb + act[i]) * NCAND + q];
+    }
+    __syncthreads();
+    const int pair = t >> 2, sub = t & 3, pj = pair / NCAND, pkk = pair % NCAND;
+    unsigned long long best_cost = 1ull << 63;
+    int best_bits = 0;
+    for (int bits = 0; bits <= 3; bits++) {
+        const int nb = 1 << bits;
+        for (int step = 0; step < 5 * nb; step++) {
+            const int slot = step < nb ? step : nb - 1;
+            if (step >= nb) {   // refinement: drop the oldest pair
+                __syncthreads();
+                if (t == 0)
+                    for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
+            }
+            __syncthreads();
+            if (slot == 0) {
+                // lowest column sums, luma and chroma separately (threads 0..2*NCAND-1, 1 per column)
+                unsigned long long sum = 0;
+                if (t < 2 * NCAND) {
+                    const int q = t % NCAND;
+                    for (int i = 0; i < nact; i++) sum += t < NCAND ? Ly[i][q] : Lc[i][q];
+                }
+                red[t] = sum;
+                __syncthreads();
+                if (t == 0) {
+                    int by = 0, bc = 0;
+                    for (int q = 1; q < NCAND; q++) {
+                        if (red[q] < red[by]) by = q;
+                        if (red[NCAND + q] < red[NCAND + bc]) bc = q;
+                    }
+                    ly[0] = by;
+                    lc[0] = bc;
+                    dist_s = nact ? red[by] + red[NCAND + bc] : 0;
+                }
+            } else {
+                unsigned long long gain = 0, base = 0;
+                for (int i = sub; i < nact; i += 4) {
+                    unsigned long long b = (unsigned long long)Ly[i][ly[0]] + Lc[i][lc[0]];
+                    for (int g = 1; g < slot; g++) {
+                        const unsigned long long c = (unsigned long long)Ly[i][ly[g]] + Lc[i][lc[g]];
+                        if (c < b) b = c;
+                    }
+                    base += b;
+                    const unsigned long long yj = Ly[i][pj];
+                    if (yj < b) {
+                        const unsigned long long room = b - yj, ck = Lc[i][pkk];
+                        if (ck < room) gain += room - ck;
+                    }
+                }
+                // sum the four partial gains of a pair, then the first maximum
+                gain += __shfl_xor(gain, 1, 64);
+                gain += __shfl_xor(gain, 2, 64);
+                base += __shfl_xor(base, 1, 64);
+                base += __shfl_xor(base, 2, 64);
+                red[t] = sub == 0 && pair < NCAND * NCAND ? gain : 0;
+                rp[t] = pair;
+                __syncthreads();
+                for (int o = 128; o > 0; o >>= 1) {
+                    if (t < o) {
+                        const unsigned long long a = red[t], b2 = red[t + o];
+                        if (b2 > a || (b2 == a && rp[t + o] < rp[t])) { red[t] = b2; rp[t] = rp[t + o]; }
+                    }
+                    __syncthreads();
+                }
+                if (t == 0) {   // thread 0 is pair 0, sub 0: its base covers every block
+                    ly[slot] = rp[0] / NCAND;
+                    lc[slot] = rp[0] % NCAND;
+                    dist_s = base - red[0];
+                }
+            }
+            __syncthreads();
+            if (t == 0 && step == 5 * nb - 1) {
+                const long long rate = (long long)(nact * bits + nb * 12) << 9;
+                const unsigned long long cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) +
+                                                ((dist_s * 16) << 7);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_bits = bits;
+                    for (int i = 0; i < 8; i++) { ys[i] = ly[i]; us[i] = lc[i]; }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int nb = 1 << best_bits;
+        dec->cdef_bits = best_bits;
+        for (int i = 0; i < 8; i++) {
+            dec->y_strength[i] = i < nb ? kLevel1[cand[ys[i]]] : 0;
+            dec->uv_strength[i] = i < nb ? kLevel1[cand[NCAND + us[i]]] : 0;
+        }
+        ly[0] = nb;
+    }
+    __syncthreads();
+    const int nb = ly[0];
+    for (int fb = t; fb < nfb; fb += 256) fb_index[fb] = 0;
+    __syncthreads();
+    for (int i = t; i < nact; i += 256) {
+        unsigned long long bm = 1ull << 63;
+        int bg = 0;
+        for (int g = 0; g < nb; g++) {
+            const unsigned long long c = (unsigned long long)Ly[i][ys[g]] + Lc[i][us[g]];
+            if (c < bm) { bm = c; bg = g; }
+        }
+        fb_index[act[i]] = (uint8_t)bg;
+    }
+}
+
+hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
+                                 uint64_t lambda, const float *logits, int *cand, float *conf, uint64_t *mse_sub,
+                                 av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose, hipStream_t s)
+{
+    const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
+    const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
+    if (propose)
+        hipLaunchKernelGGL(cand_kernel, dim3(1), dim3(256), 0, s, logits, noskip, nhfb, nvfb, mi_cols, mi_rows, cand, conf);
+    hipLaunchKernelGGL(cdef_search_subset_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, cand,
+                       mse_sub, nhfb, nvfb);
+    hipLaunchKernelGGL(fin_small_kernel, dim3(1), dim3(256), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
+                       (unsigned long long)lambda, cand, dec, fb_index);
     return hipGetLastError();
 }
 

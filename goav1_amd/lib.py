@@ -25,6 +25,7 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_cdef_search_select", "av1pf_lr_search",
            "av1pf_nn_param_count", "av1pf_nn_create", "av1pf_nn_destroy", "av1pf_nn_set_params",
            "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_backward", "av1pf_nn_adam",
+           "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned",
            "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned"] + \
@@ -79,6 +80,14 @@ def load():
     lib.av1pf_nn_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     lib.av1pf_nn_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_nn_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+    lib.av1pf_lr_search_masked.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.av1pf_cdef_search_shortlist.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p,
+                                                C.c_int, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p]
+    lib.av1pf_search_learned.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame),
+                                         C.c_void_p, C.c_int, C.c_uint64, C.c_float, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_cdef_lambda.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.av1pf_cdef_lambda.restype = C.c_uint64
     lib.av1pf_profile.argtypes = [C.c_void_p, C.c_int]
@@ -272,6 +281,22 @@ class PostFilterRunner:
         """Curated encoder LR search on the luma plane; units = hz*vt 16-byte records."""
         self._check(self.lib.av1pf_lr_search(self.h, C.byref(src.abi), C.byref(rec.abi), int(unit_size), units_dev_ptr,
                                              sse_dev_ptr, self._stream(stream)))
+
+    def cdef_search_shortlist(self, src, rec, noskip_ptr, damping, lam, cand_ptr, mse_sub_ptr, dec_ptr, idx_ptr,
+                              stream=None):
+        self._check(self.lib.av1pf_cdef_search_shortlist(self.h, C.byref(src.abi), C.byref(rec.abi), noskip_ptr,
+                                                         int(damping), C.c_uint64(int(lam)), cand_ptr, mse_sub_ptr,
+                                                         dec_ptr, idx_ptr, self._stream(stream)))
+
+    def search_learned(self, net, src, rec, noskip_ptr, damping, lam, lr_thr, dec_ptr, idx_ptr, lr_units_ptr,
+                       cand_ptr, conf_ptr=None, stream=None):
+        self._check(self.lib.av1pf_search_learned(self.h, net.h, C.byref(src.abi), C.byref(rec.abi), noskip_ptr,
+                                                  int(damping), C.c_uint64(int(lam)), float(lr_thr), dec_ptr,
+                                                  idx_ptr, lr_units_ptr, cand_ptr, conf_ptr, self._stream(stream)))
+
+    def lr_search_masked(self, src, rec, unit_size, mask_ptr, units_ptr, sse_ptr=None, stream=None):
+        self._check(self.lib.av1pf_lr_search_masked(self.h, C.byref(src.abi), C.byref(rec.abi), int(unit_size),
+                                                    mask_ptr, units_ptr, sse_ptr, self._stream(stream)))
 
     def profile(self, enable=True):
         self._check(self.lib.av1pf_profile(self.h, int(enable)))

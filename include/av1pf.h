@@ -257,6 +257,10 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
  * reference computes nothing.  Device pointers; asynchronous.               */
 int  av1pf_lr_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
                      av1pf_lr_unit *units, uint64_t *sse, void *stream);
+/* The same with unit_mask[hz * vt] (device): units with mask 0 are written
+ * NONE without being scored (their sse entries are UINT64_MAX).             */
+int  av1pf_lr_search_masked(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                            const uint8_t *unit_mask, av1pf_lr_unit *units, uint64_t *sse, void *stream);
 
 /* ---- learned proposal network ---------------------------------------------
  * No counterpart in the reference: a 48,659-parameter CNN over 64x64 luma
@@ -284,6 +288,29 @@ int  av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits,
 int  av1pf_nn_backward(av1pf_nn *nn, const uint16_t *tiles, int n, const float *targets, float *grads, float *loss,
                        void *stream);
 int  av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_decay, void *stream);
+
+/* ---- learned-proposal search ------------------------------------------------
+ * No counterpart in the reference.  The CDEF search restricted to a frame
+ * shortlist of 8 luma and 8 chroma strengths (cand[16], search-level-1
+ * positions 0..63, ascending within each half): their exact SSE for every
+ * filter block (mse_sub[2][nfb][8], same rules as av1pf_cdef_search), then
+ * the same greedy strength selection over the 64 shortlist pairs, in one
+ * launch.  8-bit 4:2:0 frames up to 2048 filter blocks (4K).  Device
+ * pointers, asynchronous.                                                   */
+int  av1pf_cdef_search_shortlist(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
+                                 const uint64_t *noskip, int damping, uint64_t lambda, const int32_t *cand,
+                                 uint64_t *mse_sub, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream);
+/* Whole learned search of one frame: tiles -> network -> shortlist (the 8
+ * strengths per plane with the largest summed probability over the active
+ * blocks; cand[16] and the mean top luma probability conf[1] are written)
+ * -> av1pf_cdef_search_shortlist -> the curated luma LR search (unit size
+ * 64) on the units whose predicted P(NONE) is at most lr_none_threshold,
+ * the rest NONE.  A caller wanting the exhaustive search as a fallback
+ * checks conf after av1pf_sync.                                             */
+int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *recon,
+                          const uint64_t *noskip, int damping, uint64_t lambda, float lr_none_threshold,
+                          av1pf_cdef_decision *decision, uint8_t *fb_index, av1pf_lr_unit *lr_units, int32_t *cand,
+                          float *conf, void *stream);
 
 /* ---- per-stage timing (no counterpart in the reference, whose closest aid is
  * the FrameWorkPostFilterResult counters, postfilter.go:309-316) ----------

@@ -203,3 +203,100 @@ def test_gpu_select_matches_host_finish(case):
             np.testing.assert_array_equal(idx.cpu().numpy(), fbi)
     finally:
         r.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2)])
+def test_gpu_shortlist_equals_masked_host_finish(case):
+    """The shortlist search's table equals the full table's shortlisted
+    columns, and its one-launch selection equals the host finish over the full
+    table with every other strength priced out (shortlists ascending, with 0)."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_lambda, cdef_search_finish
+    w, h, q, seed = case
+    src, rec, noskip, damping = synth.search_pair(seed=60 + seed, w=w, h=h, q=q)
+    rng = np.random.default_rng(seed)
+    cand = np.concatenate([np.sort(np.r_[0, rng.choice(np.arange(1, 64), 7, replace=False)]) for _ in range(2)])
+    print(f"shortlist {w}x{h} q{q} cand {cand.tolist()}", flush=True)
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        cd = torch.from_numpy(cand.astype(np.int32)).cuda()
+        msub = torch.zeros((2, nfb, 8), dtype=torch.int64, device="cuda")
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.zeros(nfb, dtype=torch.uint8, device="cuda")
+        lam = cdef_lambda(synth.search_dc_quant(160))
+        r.cdef_search_shortlist(ds, dr, ns.data_ptr(), damping, lam, cd.data_ptr(), msub.data_ptr(), dec.data_ptr(),
+                                idx.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        full = orc.cdef_search_mse(src, rec, noskip, damping)
+        sub = msub.cpu().numpy().view(np.uint64)
+        np.testing.assert_array_equal(sub[0], full[0][:, cand[:8]])
+        np.testing.assert_array_equal(sub[1], full[1][:, cand[8:]])
+        masked = np.full_like(full, np.uint64(1 << 40))
+        masked[0][:, cand[:8]] = full[0][:, cand[:8]]
+        masked[1][:, cand[8:]] = full[1][:, cand[8:]]
+        skip = 1 - active_blocks(src, noskip)
+        masked[:, skip.astype(bool), :] = 0
+        bits, ys, uvs, fbi = cdef_search_finish(masked, skip, lam)
+        d = dec.cpu().numpy()
+        assert int(d[:4].view(np.int32)[0]) == bits
+        np.testing.assert_array_equal(d[4:12], ys)
+        np.testing.assert_array_equal(d[12:20], uvs)
+        np.testing.assert_array_equal(idx.cpu().numpy(), fbi)
+    finally:
+        r.close()
+
+
+@pytest.mark.gpu
+def test_gpu_learned_search_is_consistent():
+    """The whole learned search (untrained weights): its shortlist run again
+    through the shortlist entry point gives the same decision, every chosen
+    strength is a legal level-1 strength from the shortlist, and LR units are
+    NONE / WIENER / SGRPROJ."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner, ProposalNet, cdef_lambda
+    from oracle import nn_ref
+    src, rec, noskip, damping = synth.search_pair(seed=70, w=640, h=360, q=140)
+    r = PostFilterRunner()
+    net = ProposalNet(r, max_batch=64)
+    try:
+        net.set_params(nn_ref.init_params(np.random.default_rng(3)))
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        lam = cdef_lambda(synth.search_dc_quant(160))
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.zeros(nfb, dtype=torch.uint8, device="cuda")
+        cand = torch.zeros(16, dtype=torch.int32, device="cuda")
+        conf = torch.zeros(1, dtype=torch.float32, device="cuda")
+        hz, vt = (640 + 32) // 64, (360 + 32) // 64
+        lru = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
+        print("learned search 640x360", flush=True)
+        r.search_learned(net, ds, dr, ns.data_ptr(), damping, lam, 0.9, dec.data_ptr(), idx.data_ptr(),
+                         lru.data_ptr(), cand.data_ptr(), conf.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        c = cand.cpu().numpy()
+        assert all(np.diff(c[:8]) > 0) and all(np.diff(c[8:]) > 0) and c.min() >= 0 and c.max() < 64
+        assert 0 < conf.item() <= 1
+        d1 = dec.cpu().numpy().copy()
+        i1 = idx.cpu().numpy().copy()
+        msub = torch.zeros((2, nfb, 8), dtype=torch.int64, device="cuda")
+        r.cdef_search_shortlist(ds, dr, ns.data_ptr(), damping, lam, cand.data_ptr(), msub.data_ptr(), dec.data_ptr(),
+                                idx.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(dec.cpu().numpy(), d1)
+        np.testing.assert_array_equal(idx.cpu().numpy(), i1)
+        nb = 1 << int(d1[:4].view(np.int32)[0])
+        allowed_y = {LEVEL1[g] for g in c[:8]}
+        allowed_uv = {LEVEL1[g] for g in c[8:]}
+        assert set(d1[4:4 + nb].tolist()) <= allowed_y and set(d1[12:12 + nb].tolist()) <= allowed_uv
+        assert set(lru.cpu().numpy()[:, 0].tolist()) <= {0, 1, 2}
+    finally:
+        net.close()
+        r.close()
