@@ -55,9 +55,11 @@ struct av1pf_ctx {
     void *finish_state = nullptr;  // device CDEF strength selection state
     char *learn_buf = nullptr;     // learned search: tiles, logits, shortlist table, LR mask
     size_t learn_cap = 0;
+    void *small_state = nullptr;   // shortlist selection: per-level results and ticket (zeroed once)
     int *dev_err = nullptr;       // device-side validation flag
     hipStream_t side = nullptr;   // film-grain template stream
     hipEvent_t join = nullptr;
+    hipEvent_t learn_fork = nullptr, learn_join = nullptr;   // learned search: LR search on the side stream
     // descriptor ring
     char *meta_dev[kSlots] = {};
     char *meta_host[kSlots] = {};
@@ -681,7 +683,9 @@ int av1pf_create(av1pf_ctx **out, int device)
               hipEventCreateWithFlags(&c->uploaded, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->tmpl_free[0], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->tmpl_free[1], hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->learn_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->learn_join, hipEventDisableTiming) == hipSuccess;
     // descriptor slots sized for large batches up front, so that no call in
     // steady state allocates
     for (int i = 0; ok && i < kSlots; i++) {
@@ -718,7 +722,10 @@ void av1pf_destroy(av1pf_ctx *c)
     if (c->uploaded) hipEventDestroy(c->uploaded);
     if (c->finish_state) hipFree(c->finish_state);
     if (c->learn_buf) hipFree(c->learn_buf);
+    if (c->small_state) hipFree(c->small_state);
     if (c->join) hipEventDestroy(c->join);
+    if (c->learn_fork) hipEventDestroy(c->learn_fork);
+    if (c->learn_join) hipEventDestroy(c->learn_join);
     for (int i = 0; i < 2; i++) {
         if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
         if (c->templates[i]) hipFree(c->templates[i]);
@@ -930,6 +937,7 @@ struct av1pf_nn {
     NNShape shape;
     float *P = nullptr, *m = nullptr, *v = nullptr;   // FP32 master weights, Adam moments
     uint16_t *Pb = nullptr;                           // BF16 copy read by the GEMMs
+    uint16_t *frag = nullptr;                         // conv weights as MFMA B fragments (fused inference)
     char *work = nullptr;
     NNWork ws;
     NNWeights w;
@@ -947,6 +955,7 @@ void av1pf_nn_destroy(av1pf_nn *nn)
     hipFree(nn->m);
     hipFree(nn->v);
     hipFree(nn->Pb);
+    hipFree(nn->frag);
     hipFree(nn->work);
     delete nn;
 }
@@ -963,6 +972,7 @@ int av1pf_nn_create(av1pf_ctx *c, int max_batch, av1pf_nn **out)
     const size_t P = (size_t)nn->shape.total;
     bool ok = hipMalloc(&nn->P, P * 4) == hipSuccess && hipMalloc(&nn->m, P * 4) == hipSuccess &&
               hipMalloc(&nn->v, P * 4) == hipSuccess && hipMalloc(&nn->Pb, P * 2) == hipSuccess &&
+              hipMalloc(&nn->frag, NN_FRAG_ELEMS * 2) == hipSuccess &&
               hipMalloc(&nn->work, nn_work_bytes(max_batch)) == hipSuccess;
     if (!ok) {
         av1pf_nn_destroy(nn);
@@ -997,6 +1007,8 @@ int av1pf_nn_set_params(av1pf_nn *nn, const float *params)
     std::vector<float> z(P, 0.f);
     HIPCHK(c, hipMemcpy(nn->m, z.data(), P * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(nn->v, z.data(), P * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, nn_pack_frag(nn->w, nn->frag, nullptr));
+    HIPCHK(c, hipDeviceSynchronize());
     nn->step = 0;
     return AV1PF_OK;
 }
@@ -1040,6 +1052,37 @@ int av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits, 
     return AV1PF_OK;
 }
 
+namespace {
+NNInfer infer_args(const av1pf_nn *nn)
+{
+    NNInfer p{};
+    p.frag = nn->frag;
+    for (int l = 0; l < 3; l++) p.bias[l] = nn->w.b[l];
+    p.w3 = nn->w.wb[3];
+    p.w4 = nn->w.wb[4];
+    p.b3 = nn->w.b[3];
+    p.b4 = nn->w.b[4];
+    return p;
+}
+}  // namespace
+
+int av1pf_nn_infer(av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *recon, float *logits, void *stream)
+{
+    if (!nn || !logits) return AV1PF_ERR_INVALID_ARG;
+    av1pf_ctx *c = nn->ctx;
+    int rc = validate_frame(c, src);
+    if (rc) return rc;
+    if ((rc = validate_frame(c, recon))) return rc;
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != 8 || recon->bit_depth != 8)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "fused inference needs 8-bit frames of one geometry");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_NN_FORWARD, s, 0);
+    HIPCHK(c, nn_infer(infer_args(nn), view_of(src).p[0], view_of(recon).p[0], nhfb, nhfb * nvfb, logits, s));
+    return AV1PF_OK;
+}
+
 int av1pf_nn_backward(av1pf_nn *nn, const uint16_t *tiles, int n, const float *targets, float *grads, float *loss,
                       void *stream)
 {
@@ -1061,6 +1104,7 @@ int av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_decay
     nn->step++;
     HIPCHK(c, nn_adam(nn->P, nn->Pb, grads, nn->m, nn->v, (int)nn->shape.total, lr, nn->step, weight_decay,
                       static_cast<hipStream_t>(stream)));
+    HIPCHK(c, nn_pack_frag(nn->w, nn->frag, static_cast<hipStream_t>(stream)));
     return AV1PF_OK;
 }
 
@@ -1071,6 +1115,14 @@ int av1pf_lr_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *rec
 }
 
 namespace {
+int ensure_small_state(av1pf_ctx *c)
+{
+    if (c->small_state) return AV1PF_OK;
+    HIPCHK(c, hipMalloc(&c->small_state, small_state_bytes()));
+    HIPCHK(c, hipMemset(c->small_state, 0, small_state_bytes()));
+    return AV1PF_OK;
+}
+
 int validate_search_pair(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon)
 {
     int rc = validate_frame(c, src);
@@ -1097,8 +1149,10 @@ int av1pf_cdef_search_shortlist(av1pf_ctx *c, const av1pf_frame *src, const av1p
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
-    HIPCHK(c, launch_search_learned(view_of(src), view_of(recon), noskip, damping, lambda, nullptr,
-                                    const_cast<int32_t *>(cand), nullptr, mse_sub, decision, fb_index, false, s));
+    if ((rc = ensure_small_state(c))) return rc;
+    HIPCHK(c, launch_search_learned(view_of(src), view_of(recon), noskip, damping, lambda, nullptr, nullptr,
+                                    const_cast<int32_t *>(cand), nullptr, mse_sub, decision, fb_index, false,
+                                    c->small_state, s));
     return AV1PF_OK;
 }
 
@@ -1114,30 +1168,34 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64, nfb = nhfb * nvfb;
     if (nfb > 2048) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 2048 filter blocks");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t tiles_b = (size_t)nfb * 4096 * 2 * 2, logits_b = (size_t)nfb * NN_OUT * 4;
+    const size_t logits_b = (size_t)nfb * NN_OUT * 4;
     const size_t msub_b = (size_t)2 * nfb * NCAND * 8, mask_b = (size_t)nfb + 256;
-    const size_t need = tiles_b + logits_b + msub_b + mask_b + 4 * 256 + 64;
+    const size_t need = logits_b + msub_b + mask_b + 4 * 256 + 1024;
     if ((rc = ensure(c, (void **)&c->learn_buf, &c->learn_cap, need))) return rc;
     char *p = c->learn_buf;
-    uint16_t *tiles = (uint16_t *)p; p += align256(tiles_b);
     float *logits = (float *)p; p += align256(logits_b);
     uint64_t *msub = (uint64_t *)p; p += align256(msub_b);
     uint8_t *mask = (uint8_t *)p; p += align256(mask_b);
-    float *conf_tmp = (float *)p;
+    float *sums = (float *)p;          // 130 floats of proposal scratch
+    float *conf_tmp = sums + 132;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
     const FrameView sv = view_of(src), rv = view_of(recon);
-    HIPCHK(c, launch_tiles(sv.p[0], rv.p[0], 8, nhfb, nfb, tiles, s));
-    for (int i = 0; i < nfb; i += nn->max_batch) {
-        const int b = std::min(nn->max_batch, nfb - i);
-        HIPCHK(c, nn_forward(nn->w, nn->ws, tiles + (size_t)i * 4096 * 2, b, logits + (size_t)i * NN_OUT, s));
-    }
-    HIPCHK(c, launch_search_learned(sv, rv, noskip, damping, lambda, logits, cand, conf ? conf : conf_tmp, msub,
-                                    decision, fb_index, true, s));
+    if ((rc = ensure_small_state(c))) return rc;
+    HIPCHK(c, nn_zero(sums, 130, s));
+    HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], nhfb, nfb, logits, s));
+    // the LR search depends only on the logits: it runs on the side stream
+    // beside the CDEF shortlist search and selection
+    HIPCHK(c, hipEventRecord(c->learn_fork, s));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->learn_fork, 0));
     const int W = recon->width, H = recon->height;
     const int hz = std::max((W + 32) / 64, 1), vt = std::max((H + 32) / 64, 1);
-    HIPCHK(c, launch_lr_mask(logits, nhfb, nvfb, hz, vt, lr_none_threshold, mask, s));
-    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], 64, hz, vt, mask, lr_units, nullptr, s));
+    HIPCHK(c, launch_lr_mask(logits, nhfb, nvfb, hz, vt, lr_none_threshold, mask, c->side));
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], 64, hz, vt, mask, lr_units, nullptr, c->side));
+    HIPCHK(c, hipEventRecord(c->learn_join, c->side));
+    HIPCHK(c, launch_search_learned(sv, rv, noskip, damping, lambda, logits, sums, cand, conf ? conf : conf_tmp, msub,
+                                    decision, fb_index, true, c->small_state, s));
+    HIPCHK(c, hipStreamWaitEvent(s, c->learn_join, 0));
     return AV1PF_OK;
 }
 

@@ -273,8 +273,9 @@ __global__ void adam_kernel(float *w, uint16_t *wb, const float *g, float *m, fl
 }
 
 // ---- tiles from frames --------------------------------------------------------
-// tiles[fb][0][y][x] = src luma / maxv, tiles[fb][1] = recon luma / maxv, NHWC
-// with C = 2 (so [fb][y][x][2]); samples past the frame repeat the edge.
+// tiles[fb][y][x][0] = recon luma / maxv, tiles[fb][y][x][1] = (source - recon)
+// * 16 / maxv (the coding error, amplified so BF16 keeps its low bits); NHWC
+// with C = 2; samples past the frame repeat the edge.
 template <typename T>
 __device__ __forceinline__ float luma_at(const PlaneView &p, int y, int x)
 {
@@ -293,14 +294,181 @@ __global__ void tiles_kernel(PlaneView S, PlaneView R, int bd, int nhfb, uint16_
         float s, r;
         if (bd == 8) { s = luma_at<uint8_t>(S, fy + y, fx + x); r = luma_at<uint8_t>(R, fy + y, fx + x); }
         else { s = luma_at<uint16_t>(S, fy + y, fx + x); r = luma_at<uint16_t>(R, fy + y, fx + x); }
-        tiles[((int64_t)fb * 4096 + i) * 2 + 0] = f2bf(s * inv);
-        tiles[((int64_t)fb * 4096 + i) * 2 + 1] = f2bf(r * inv);
+        tiles[((int64_t)fb * 4096 + i) * 2 + 0] = f2bf(r * inv);
+        tiles[((int64_t)fb * 4096 + i) * 2 + 1] = f2bf((s - r) * 16.f * inv);
     }
 }
 
 hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s)
 {
     hipLaunchKernelGGL(tiles_kernel, dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, tiles);
+    return hipGetLastError();
+}
+
+// ---- fused inference ----------------------------------------------------------
+// One workgroup per 64x64 filter block runs the whole forward pass from the
+// frame: the tile is built in LDS, the three convolutions are MFMA GEMMs whose
+// A fragments are gathered straight from the LDS activations (the patch
+// matrix is never written), then pooling, fc1 and the heads; only the 131
+// logits leave the CU.  Convolution weights come as pre-packed B fragments
+// (one 8-byte load per lane), fc weights from the BF16 parameter copy.
+// Rounding points are those of the layer-by-layer pass: BF16 tile, BF16
+// activations after bias and ReLU, BF16 pool and fc1, FP32 logits.  LDS:
+// the tile (16 KB) is overwritten by conv2's output, conv1's (32 KB) by
+// conv3's.
+//
+// Fragment layout per conv layer: [k-step][16-column block][lane][4] with
+// lane l holding B[16 ks + 4 (l / 16) + e][16 cb + l % 16], zero past K.
+constexpr int kFragOff[3] = { 0, 512, 5120 };
+
+__global__ void pack_frag_kernel(const uint16_t *Pb, int64_t w0, int64_t w1, int64_t w2, uint16_t *frag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NN_FRAG_ELEMS) return;
+    const int l = i < kFragOff[1] ? 0 : i < kFragOff[2] ? 1 : 2;
+    const int o = i - kFragOff[l];
+    const int K = l == 0 ? 18 : l == 1 ? 144 : 288, N = l == 0 ? 16 : l == 1 ? 32 : 64, CB = N / 16;
+    const int e = o & 3, lane = (o >> 2) & 63, blk = o >> 8, cb = blk % CB, ks = blk / CB;
+    const int k = 16 * ks + 4 * (lane >> 4) + e, n = 16 * cb + (lane & 15);
+    const int64_t woff = l == 0 ? w0 : l == 1 ? w1 : w2;
+    frag[i] = k < K ? Pb[woff + (int64_t)k * N + n] : (uint16_t)0;
+}
+
+hipError_t nn_pack_frag(const NNWeights &w, uint16_t *frag, hipStream_t s)
+{
+    hipLaunchKernelGGL(pack_frag_kernel, dim3((NN_FRAG_ELEMS + 255) / 256), dim3(256), 0, s, w.wb[0], (int64_t)0,
+                       w.woff[1] - w.woff[0], w.woff[2] - w.woff[0], frag);
+    return hipGetLastError();
+}
+
+// Four consecutive BF16 values (8-byte aligned) as an MFMA operand.
+__device__ __forceinline__ v4s ld_frag(const uint16_t *p)
+{
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return (v4s){ (short)(u.x & 0xffff), (short)(u.x >> 16), (short)(u.y & 0xffff), (short)(u.y >> 16) };
+}
+
+__global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R, int nhfb, NNInfer p, float *logits)
+{
+    __shared__ __align__(16) uint16_t bufA[64 * 64 * 2];    // tile [64][64][2], then conv2 out [256][32]
+    __shared__ __align__(16) uint16_t bufB[32 * 32 * 16];   // conv1 out [1024][16], then conv3 out [64][64]
+    __shared__ float pool[64], fc1[128];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, g = lane >> 4, l16 = lane & 15;
+    const int fb = blockIdx.x, fx = (fb % nhfb) * 64, fy = (fb / nhfb) * 64;
+    const float inv = 1.f / 255.f;
+    uint32_t *tile = reinterpret_cast<uint32_t *>(bufA);
+    for (int i = t; i < 4096; i += 256) {
+        const int y = min(fy + (i >> 6), R.h - 1), x = min(fx + (i & 63), R.w - 1);
+        const float sv = row_ptr<uint8_t>(S, y)[x], rv = row_ptr<uint8_t>(R, y)[x];
+        tile[i] = (uint32_t)f2bf(rv * inv) | ((uint32_t)f2bf((sv - rv) * 16.f * inv) << 16);
+    }
+    __syncthreads();
+    // conv1: M 1024 (64 row blocks, one output half-row each), N 16, K 18 in two k-steps
+    {
+        const v4s b0 = ld_frag(p.frag + kFragOff[0] + lane * 4), b1 = ld_frag(p.frag + kFragOff[0] + (64 + lane) * 4);
+        const float bias = p.bias[0][l16];
+        auto tap = [&](int oy, int ox, int tp) -> uint32_t {
+            const int iy = 2 * oy + tp / 3 - 1, ix = 2 * ox + tp % 3 - 1;
+            return (iy >= 0 && iy < 64 && ix >= 0 && ix < 64) ? tile[iy * 64 + ix] : 0u;
+        };
+        for (int rb = w; rb < 64; rb += 4) {
+            const int oy = rb >> 1, ox = (rb & 1) * 16 + l16;
+            const uint32_t t0 = tap(oy, ox, 2 * g), t1 = tap(oy, ox, 2 * g + 1), t8 = g == 0 ? tap(oy, ox, 8) : 0u;
+            const v4s a0 = { (short)(t0 & 0xffff), (short)(t0 >> 16), (short)(t1 & 0xffff), (short)(t1 >> 16) };
+            const v4s a1 = { (short)(t8 & 0xffff), (short)(t8 >> 16), 0, 0 };
+            v4f acc = { 0.f, 0.f, 0.f, 0.f };
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) bufB[(rb * 16 + 4 * g + r) * 16 + l16] = f2bf(fmaxf(acc[r] + bias, 0.f));
+        }
+    }
+    __syncthreads();
+    // conv2: M 256 (16 row blocks = output rows), N 32, K 144 (one tap per k-step); wave w: rows 4w..4w+3
+    {
+        v4s bf[9][2];
+#pragma unroll
+        for (int ks = 0; ks < 9; ks++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) bf[ks][cb] = ld_frag(p.frag + kFragOff[1] + ((ks * 2 + cb) * 64 + lane) * 4);
+        v4f acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++) acc[a][0] = acc[a][1] = (v4f){ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int ks = 0; ks < 9; ks++) {
+            const int ky = ks / 3, kx = ks % 3, ix = 2 * l16 + kx - 1;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int iy = 2 * (4 * w + a) + ky - 1;
+                const v4s av = (iy >= 0 && iy < 32 && ix >= 0) ? ld_frag(&bufB[(iy * 32 + ix) * 16 + 4 * g])
+                                                                : (v4s){ 0, 0, 0, 0 };
+                acc[a][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bf[ks][0], acc[a][0], 0, 0, 0);
+                acc[a][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bf[ks][1], acc[a][1], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const float bias = p.bias[1][cb * 16 + l16];
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    bufA[((4 * w + a) * 16 + 4 * g + r) * 32 + cb * 16 + l16] = f2bf(fmaxf(acc[a][cb][r] + bias, 0.f));
+            }
+    }
+    __syncthreads();
+    // conv3: M 64 (4 row blocks of two output rows), N 64 (wave w: columns 16w..16w+15), K 288
+    {
+        v4s bf[18];
+#pragma unroll
+        for (int ks = 0; ks < 18; ks++) bf[ks] = ld_frag(p.frag + kFragOff[2] + ((ks * 4 + w) * 64 + lane) * 4);
+        v4f acc[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) acc[a] = (v4f){ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int ks = 0; ks < 18; ks++) {
+            const int tp = ks >> 1, h = ks & 1, ky = tp / 3, kx = tp % 3;
+            const int ix = 2 * (l16 & 7) + kx - 1;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int iy = 2 * (2 * a + (l16 >> 3)) + ky - 1;
+                const v4s av = (iy >= 0 && iy < 16 && ix >= 0) ? ld_frag(&bufA[(iy * 16 + ix) * 32 + 16 * h + 4 * g])
+                                                                : (v4s){ 0, 0, 0, 0 };
+                acc[a] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bf[ks], acc[a], 0, 0, 0);
+            }
+        }
+        const float bias = p.bias[2][16 * w + l16];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) bufB[(a * 16 + 4 * g + r) * 64 + 16 * w + l16] = f2bf(fmaxf(acc[a][r] + bias, 0.f));
+    }
+    __syncthreads();
+    if (t < 64) {   // average pool, in pixel order as the layer-by-layer pass
+        float sum = 0.f;
+        for (int q = 0; q < 64; q++) sum += bf2f(bufB[q * 64 + t]);
+        pool[t] = bf2f(f2bf(sum / 64));
+    }
+    __syncthreads();
+    if (t < 128) {
+        float sum = 0.f;
+#pragma unroll 8
+        for (int c = 0; c < 64; c++) sum += pool[c] * bf2f(p.w3[c * 128 + t]);
+        fc1[t] = bf2f(f2bf(fmaxf(sum + p.b3[t], 0.f)));
+    }
+    __syncthreads();
+    if (t < NN_OUT) {
+        float sum = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < 128; j++) sum += fc1[j] * bf2f(p.w4[j * NN_OUT + t]);
+        logits[(int64_t)fb * NN_OUT + t] = sum + p.b4[t];
+    }
+}
+
+hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(nn_infer_kernel, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits);
     return hipGetLastError();
 }
 

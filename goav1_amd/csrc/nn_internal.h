@@ -57,7 +57,20 @@ struct GemmArgs {
     int64_t ldc;
 };
 
+// Fused inference: conv B fragments (packed from the BF16 weights), biases,
+// fc weights.
+constexpr int NN_FRAG_ELEMS = 23552;
+struct NNInfer {
+    const uint16_t *frag;
+    const float *bias[3];
+    const uint16_t *w3, *w4;
+    const float *b3, *b4;
+};
+
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
+hipError_t nn_pack_frag(const NNWeights &w, uint16_t *frag, hipStream_t s);
+hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
+                    hipStream_t s);
 hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s);
 hipError_t nn_forward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, float *logits, hipStream_t s);
 hipError_t nn_backward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, const float *target,
@@ -67,8 +80,10 @@ hipError_t nn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, i
 hipError_t nn_zero(float *p, int64_t n, hipStream_t s);
 size_t nn_work_bytes(int max_batch);
 hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
-                                 uint64_t lambda, const float *logits, int *cand, float *conf, uint64_t *mse_sub,
-                                 av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose, hipStream_t s);
+                                 uint64_t lambda, const float *logits, float *sums, int *cand, float *conf,
+                                 uint64_t *mse_sub, av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose,
+                                 void *small_state, hipStream_t s);
+size_t small_state_bytes();
 void nn_work_carve(char *base, int max_batch, NNWork &ws);
 
 }  // namespace pf

@@ -513,6 +513,7 @@ __global__ __launch_bounds__(256) void fin_assign_kernel(const uint64_t *mse, in
 }
 
 size_t finish_state_bytes() { return sizeof(FinishState); }
+size_t small_state_bytes() { return 1024; }
 
 hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,
                               uint64_t lambda, void *state, av1pf_cdef_decision *dec, uint8_t *fb_index, hipStream_t s)
@@ -555,55 +556,60 @@ __device__ __forceinline__ bool fb_active(const uint64_t *noskip, int fb, int nh
     return ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
 }
 
-// Shortlist from the network's logits: per head, the softmax probabilities of
-// the active blocks are summed and the NCAND most likely strengths kept, in
-// ascending strength-list order; conf = mean over active blocks of the
-// largest luma probability.
-__global__ __launch_bounds__(256) void cand_kernel(const float *logits, const uint64_t *noskip, int nhfb, int nvfb,
-                                                   int mi_cols, int mi_rows, int *cand, float *conf)
+// Shortlist from the network's logits, in two launches:
+//  - prob_sum_kernel: one wave per active filter block, lane i holds luma and
+//    chroma logit i; the softmax probabilities are added to per-strength sums
+//    (sums[0..63] luma, [64..127] chroma, [128] the largest luma probability,
+//    [129] the active block count), device atomics;
+//  - topk_kernel: the NCAND strengths with the largest sums per plane, in
+//    ascending strength-list order, and conf = mean largest luma probability.
+__global__ __launch_bounds__(64) void prob_sum_kernel(const float *logits, const uint64_t *noskip, int nhfb,
+                                                      int mi_cols, int mi_rows, float *sums)
 {
-    __shared__ float acc[2][64];
-    __shared__ float cmax;
-    __shared__ int nact;
-    const int t = threadIdx.x;
-    if (t < 128) acc[t >> 6][t & 63] = 0.f;
-    if (t == 0) { cmax = 0.f; nact = 0; }
-    __syncthreads();
-    const int nfb = nhfb * nvfb;
-    for (int fb = t; fb < nfb; fb += 256) {
-        if (!fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) continue;
-        atomicAdd(&nact, 1);
-        const float *z = logits + (size_t)fb * NN_OUT;
-        for (int h = 0; h < 2; h++) {
-            float mx = -1e30f;
-            for (int i = 0; i < 64; i++) mx = fmaxf(mx, z[h * 64 + i]);
-            float se = 0.f;
-            for (int i = 0; i < 64; i++) se += __expf(z[h * 64 + i] - mx);
-            const float inv = 1.f / se;
-            float best = 0.f;
-            for (int i = 0; i < 64; i++) {
-                const float p = __expf(z[h * 64 + i] - mx) * inv;
-                atomicAdd(&acc[h][i], p);
-                best = fmaxf(best, p);
+    const int fb = blockIdx.x, lane = threadIdx.x;
+    if (!fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) return;
+    const float *z = logits + (size_t)fb * NN_OUT;
+    for (int h = 0; h < 2; h++) {
+        const float v = z[h * 64 + lane];
+        float mx = v;
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        const float e = __expf(v - mx);
+        float se = e;
+        for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+        const float p = e / se;
+        atomicAdd(&sums[h * 64 + lane], p);
+        if (h == 0) {
+            float pm = p;
+            for (int o = 32; o > 0; o >>= 1) pm = fmaxf(pm, __shfl_xor(pm, o, 64));
+            if (lane == 0) {
+                atomicAdd(&sums[128], pm);
+                atomicAdd(&sums[129], 1.f);
             }
-            if (h == 0) atomicAdd(&cmax, best);
         }
     }
+}
+
+__global__ __launch_bounds__(64) void topk_kernel(float *sums, int *cand, float *conf)
+{
+    const int lane = threadIdx.x;
+    for (int h = 0; h < 2; h++) {
+        const float v = sums[h * 64 + lane];
+        // rank = number of strengths strictly ahead (larger sum, or equal and lower index)
+        int rank = 0;
+        for (int j = 0; j < 64; j++) {
+            const float u = sums[h * 64 + j];
+            rank += (u > v) || (u == v && j < lane);
+        }
+        const bool in = rank < NCAND;
+        const uint64_t vote = __ballot(in);
+        if (in) cand[h * NCAND + __popcll(vote & ((1ull << lane) - 1))] = lane;
+    }
+    if (lane == 0) *conf = sums[129] > 0.f ? sums[128] / sums[129] : 1.f;
     __syncthreads();
-    if (t < 2) {
-        bool taken[64];
-        for (int i = 0; i < 64; i++) taken[i] = false;
-        for (int k = 0; k < NCAND; k++) {
-            int bi = -1;
-            for (int i = 0; i < 64; i++)
-                if (!taken[i] && (bi < 0 || acc[t][i] > acc[t][bi])) bi = i;
-            taken[bi] = true;
-        }
-        int k = 0;
-        for (int i = 0; i < 64; i++)
-            if (taken[i]) cand[t * NCAND + k++] = i;
-    }
-    if (t == 0) *conf = nact ? cmax / nact : 1.f;
+    // clear the sums for the next frame
+    sums[lane] = 0.f;
+    sums[64 + lane] = 0.f;
+    if (lane < 2) sums[128 + lane] = 0.f;
 }
 
 // Exact SSE of the shortlisted strengths: the MSE kernel's decomposition with
@@ -756,139 +762,255 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
     }
 }
 
-// Greedy selection over the NCAND x NCAND shortlist pairs in one workgroup:
-// the active blocks' shortlist columns (FP exactness: every 8-bit 4:2:0 SSE
-// fits 32 bits) sit in LDS, and all steps run in this one launch.  Thread
-// (pair, sub) handles blocks sub, sub + 4, ... of pair = t / 4.
+// Greedy selection over the NCAND x NCAND shortlist pairs.  The four levels
+// (cdef_bits 0..3) are independent greedy runs, so workgroup b of 4 runs
+// level b, all of its steps in this one launch.  Thread t keeps the active
+// blocks t, t + 512, ... (NPER of them) with their 16 shortlist SSEs in
+// registers, exact as int32: an 8-bit 64x64 luma block's SSE is at most
+// 4096 * 255^2 and its two 32x32 chroma blocks' 2048 * 255^2, so a pair's
+// cost stays below 2^29.  A step computes each block's best cost among the
+// chosen pairs and its gain max(0, best - y_j - uv_k) for all 64 pairs; a
+// butterfly over the wave (each exchange halves the values a lane holds)
+// leaves lane l with the wave total of pair bitrev6(l), and wave 0 adds the
+// waves' totals and takes the first maximum.  Each level's cost and pairs go
+// to `lv`; the last workgroup to finish (ticket) keeps the cheapest level
+// (the lowest on ties, as a sequential scan with a strict comparison would)
+// and writes the decision.
+struct SmallLevel {
+    unsigned long long cost;
+    int ys[8], us[8];
+};
+struct SmallState {
+    SmallLevel lv[4];
+    unsigned int ticket;
+};
+static_assert(sizeof(SmallState) <= 1024, "small selection state");
 #define SMALL_MAX_FB 2048
-__global__ __launch_bounds__(256) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb,
-                                                        int mi_cols, int mi_rows, unsigned long long lambda,
-                                                        const int *cand, av1pf_cdef_decision *dec, uint8_t *fb_index)
+#define SMALL_T 512
+
+// Lane exchanges: DPP quad permutations for distances 1 and 2, swizzles
+// within 32 lanes for 4..16, a permute for 32.
+template <int X>
+__device__ __forceinline__ uint32_t xchg32(uint32_t v)
 {
-    __shared__ uint32_t Ly[SMALL_MAX_FB][NCAND];
-    __shared__ uint32_t Lc[SMALL_MAX_FB][NCAND];
+    if constexpr (X == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false);
+    else if constexpr (X == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false);
+    else if constexpr (X == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
+    else if constexpr (X == 8) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
+    else if constexpr (X == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+    else return (uint32_t)__shfl_xor((int)v, 32, 64);
+}
+template <int X>
+__device__ __forceinline__ uint64_t xchg64(uint64_t v)
+{
+    return (uint64_t)xchg32<X>((uint32_t)v) | ((uint64_t)xchg32<X>((uint32_t)(v >> 32)) << 32);
+}
+// One butterfly stage: 2N values -> N, keeping the half selected by lane bit
+// log2(X) and adding the partner's copy of that half.
+template <int X, int N, typename T>
+__device__ __forceinline__ void bfly(const T *in, T *out, int lane)
+{
+    const bool hi = lane & X;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const T keep = hi ? in[N + i] : in[i], send = hi ? in[i] : in[N + i];
+        if constexpr (sizeof(T) == 4) out[i] = keep + xchg32<X>(send);
+        else out[i] = keep + xchg64<X>(send);
+    }
+}
+// A block's 8 shortlist SSEs as one register vector: indexing it with a
+// wave-uniform (scalar) index stays in registers.
+typedef int cand8 __attribute__((ext_vector_type(NCAND)));
+__device__ __forceinline__ int pick8(const cand8 &a, int j) { return a[__builtin_amdgcn_readfirstlane(j)]; }
+
+__device__ __forceinline__ void pack_sel(const int *ly, const int *lc, int *selp, int nb)
+{
+    unsigned a = 0, c = 0;
+    for (int i = 0; i < nb; i++) { a |= (unsigned)ly[i] << (4 * i); c |= (unsigned)lc[i] << (4 * i); }
+    selp[0] = (int)a;
+    selp[1] = (int)c;
+}
+
+template <int NPER>
+__global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
+                                                            int nvfb, int mi_cols, int mi_rows,
+                                                            unsigned long long lambda, const int *cand,
+                                                            SmallState *ss, av1pf_cdef_decision *dec,
+                                                            uint8_t *fb_index)
+{
+    constexpr int NW = SMALL_T / 64;
     __shared__ uint16_t act[SMALL_MAX_FB];
-    __shared__ unsigned long long red[256];
-    __shared__ int rp[256];
-    __shared__ int ly[8], lc[8], ys[8], us[8];
-    __shared__ int nact_s;
+    __shared__ uint8_t flag[SMALL_MAX_FB];
+    __shared__ unsigned long long wpart[NW][65];   // per-wave pair gains (or column sums) and base sum
+    __shared__ int ly[8], lc[8];
+    __shared__ int selp[2];
+    __shared__ int nact_s, nb_s;
+    __shared__ bool last;
     __shared__ unsigned long long dist_s;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfb = nhfb * nvfb;
-    if (t == 0) {
+    for (int fb = t; fb < nfb; fb += SMALL_T) flag[fb] = fb_active(noskip, fb, nhfb, mi_cols, mi_rows) ? 1 : 0;
+    __syncthreads();
+    if (t < 64) {   // ascending compaction by one wave
         int n = 0;
-        for (int fb = 0; fb < nfb; fb++)
-            if (fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) act[n++] = (uint16_t)fb;
-        nact_s = n;
+        for (int base = 0; base < nfb; base += 64) {
+            const bool a = base + lane < nfb && flag[base + lane];
+            const uint64_t vote = __ballot(a);
+            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = (uint16_t)(base + lane);
+            n += __popcll(vote);
+        }
+        if (lane == 0) nact_s = n;
     }
     __syncthreads();
     const int nact = nact_s;
-    for (int e = t; e < nact * NCAND; e += 256) {
-        const int i = e / NCAND, q = e % NCAND;
-        Ly[i][q] = (uint32_t)mse[(size_t)act[i] * NCAND + q];
-        Lc[i][q] = (uint32_t)mse[((size_t)nfb + act[i]) * NCAND + q];
+    // blocks past nact hold zeros: best cost 0, no gain, nothing in a column sum
+    cand8 Y[NPER], U[NPER];
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        const size_t fb = i < nact ? act[i] : 0;
+#pragma unroll
+        for (int q = 0; q < NCAND; q++) {
+            Y[r][q] = i < nact ? (int)mse[fb * NCAND + q] : 0;
+            U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
+        }
     }
-    __syncthreads();
-    const int pair = t >> 2, sub = t & 3, pj = pair / NCAND, pkk = pair % NCAND;
-    unsigned long long best_cost = 1ull << 63;
-    int best_bits = 0;
-    for (int bits = 0; bits <= 3; bits++) {
-        const int nb = 1 << bits;
-        for (int step = 0; step < 5 * nb; step++) {
-            const int slot = step < nb ? step : nb - 1;
-            if (step >= nb) {   // refinement: drop the oldest pair
-                __syncthreads();
-                if (t == 0)
-                    for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
+    const int bits = blockIdx.x;
+    const int nb = 1 << bits;
+    for (int step = 0; step < 5 * nb; step++) {
+        const int slot = step < nb ? step : nb - 1;
+        if (slot == 0) {
+            // column sums, one wave reduction per shortlist column
+#pragma unroll
+            for (int q = 0; q < 2 * NCAND; q++) {
+                unsigned long long v = 0;
+#pragma unroll
+                for (int r = 0; r < NPER; r++) v += (unsigned)(q < NCAND ? Y[r][q] : U[r][q - NCAND]);
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) wpart[wv][q] = v;
             }
             __syncthreads();
-            if (slot == 0) {
-                // lowest column sums, luma and chroma separately (threads 0..2*NCAND-1, 1 per column)
-                unsigned long long sum = 0;
-                if (t < 2 * NCAND) {
-                    const int q = t % NCAND;
-                    for (int i = 0; i < nact; i++) sum += t < NCAND ? Ly[i][q] : Lc[i][q];
+            if (t == 0) {
+                unsigned long long cs[2 * NCAND];
+                for (int q = 0; q < 2 * NCAND; q++) {
+                    cs[q] = 0;
+                    for (int w = 0; w < NW; w++) cs[q] += wpart[w][q];
                 }
-                red[t] = sum;
-                __syncthreads();
-                if (t == 0) {
-                    int by = 0, bc = 0;
-                    for (int q = 1; q < NCAND; q++) {
-                        if (red[q] < red[by]) by = q;
-                        if (red[NCAND + q] < red[NCAND + bc]) bc = q;
-                    }
-                    ly[0] = by;
-                    lc[0] = bc;
-                    dist_s = nact ? red[by] + red[NCAND + bc] : 0;
+                int by = 0, bc = 0;
+                for (int q = 1; q < NCAND; q++) {
+                    if (cs[q] < cs[by]) by = q;
+                    if (cs[NCAND + q] < cs[NCAND + bc]) bc = q;
                 }
-            } else {
-                unsigned long long gain = 0, base = 0;
-                for (int i = sub; i < nact; i += 4) {
-                    unsigned long long b = (unsigned long long)Ly[i][ly[0]] + Lc[i][lc[0]];
-                    for (int g = 1; g < slot; g++) {
-                        const unsigned long long c = (unsigned long long)Ly[i][ly[g]] + Lc[i][lc[g]];
-                        if (c < b) b = c;
-                    }
-                    base += b;
-                    const unsigned long long yj = Ly[i][pj];
-                    if (yj < b) {
-                        const unsigned long long room = b - yj, ck = Lc[i][pkk];
-                        if (ck < room) gain += room - ck;
-                    }
-                }
-                // sum the four partial gains of a pair, then the first maximum
-                gain += __shfl_xor(gain, 1, 64);
-                gain += __shfl_xor(gain, 2, 64);
-                base += __shfl_xor(base, 1, 64);
-                base += __shfl_xor(base, 2, 64);
-                red[t] = sub == 0 && pair < NCAND * NCAND ? gain : 0;
-                rp[t] = pair;
-                __syncthreads();
-                for (int o = 128; o > 0; o >>= 1) {
-                    if (t < o) {
-                        const unsigned long long a = red[t], b2 = red[t + o];
-                        if (b2 > a || (b2 == a && rp[t + o] < rp[t])) { red[t] = b2; rp[t] = rp[t + o]; }
-                    }
-                    __syncthreads();
-                }
-                if (t == 0) {   // thread 0 is pair 0, sub 0: its base covers every block
-                    ly[slot] = rp[0] / NCAND;
-                    lc[slot] = rp[0] % NCAND;
-                    dist_s = base - red[0];
-                }
+                ly[0] = by;
+                lc[0] = bc;
+                dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
             }
+        } else {
+            // the chosen pairs, 4 bits each, read once
+            const unsigned sy = (unsigned)__builtin_amdgcn_readfirstlane(selp[0]);
+            const unsigned su = (unsigned)__builtin_amdgcn_readfirstlane(selp[1]);
+            unsigned b[NPER];
+            unsigned long long bsum = 0;
+#pragma unroll
+            for (int r = 0; r < NPER; r++) {
+                b[r] = (unsigned)(Y[r][sy & 7] + U[r][su & 7]);
+                for (int g = 1; g < slot; g++)
+                    b[r] = min(b[r], (unsigned)(Y[r][(sy >> (4 * g)) & 7] + U[r][(su >> (4 * g)) & 7]));
+                bsum += b[r];
+            }
+            // per-thread gains max(0, b - y_j - uv_k) by saturating subtractions:
+            // at most 4 * 2^29 < 2^32, and two lanes' sums fit too
+            uint32_t g64[64];
+#pragma unroll
+            for (int p = 0; p < 64; p++) g64[p] = 0;
+#pragma unroll
+            for (int r = 0; r < NPER; r++)
+#pragma unroll
+                for (int j = 0; j < NCAND; j++) {
+                    const unsigned rj = __builtin_elementwise_sub_sat(b[r], (unsigned)Y[r][j]);
+#pragma unroll
+                    for (int k = 0; k < NCAND; k++)
+                        g64[j * NCAND + k] += __builtin_elementwise_sub_sat(rj, (unsigned)U[r][k]);
+                }
+            uint32_t g32[32];
+            bfly<1, 32>(g64, g32, lane);
+            unsigned long long h32[32], h16[16], h8[8], h4[4], h2[2], h1[1];
+#pragma unroll
+            for (int i = 0; i < 32; i++) h32[i] = g32[i];
+            bfly<2, 16>(h32, h16, lane);
+            bfly<4, 8>(h16, h8, lane);
+            bfly<8, 4>(h8, h4, lane);
+            bfly<16, 2>(h4, h2, lane);
+            bfly<32, 1>(h2, h1, lane);
+            if (step == 5 * nb - 1) {   // the frame distortion is needed at the last step only
+                for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
+                if (lane == 0) wpart[wv][64] = bsum;
+            }
+            wpart[wv][lane] = h1[0];
             __syncthreads();
-            if (t == 0 && step == 5 * nb - 1) {
-                const long long rate = (long long)(nact * bits + nb * 12) << 9;
-                const unsigned long long cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) +
-                                                ((dist_s * 16) << 7);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_bits = bits;
-                    for (int i = 0; i < 8; i++) { ys[i] = ly[i]; us[i] = lc[i]; }
+            if (wv == 0) {   // first maximum over the 64 pairs
+                unsigned long long v = 0, base = 0;
+                for (int w = 0; w < NW; w++) { v += wpart[w][lane]; base += wpart[w][64]; }
+                int p = ((lane & 1) << 5) | ((lane & 2) << 3) | ((lane & 4) << 1) | ((lane & 8) >> 1) |
+                        ((lane & 16) >> 3) | ((lane & 32) >> 5);
+                for (int o = 32; o > 0; o >>= 1) {
+                    const unsigned long long v2 = __shfl_xor(v, o, 64);
+                    const int p2 = __shfl_xor(p, o, 64);
+                    if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+                }
+                if (lane == 0) {
+                    ly[slot] = p / NCAND;
+                    lc[slot] = p % NCAND;
+                    dist_s = base - v;
                 }
             }
         }
+        if (t == 0) {
+            if (step == 5 * nb - 1) {
+                const long long rate = (long long)(nact * bits + nb * 12) << 9;
+                SmallLevel &L = ss->lv[bits];
+                L.cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((dist_s * 16) << 7);
+                for (int i = 0; i < 8; i++) { L.ys[i] = i < nb ? ly[i] : 0; L.us[i] = i < nb ? lc[i] : 0; }
+            } else if (step + 1 >= nb) {   // refinement: the next step drops the oldest pair
+                for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
+            }
+            pack_sel(ly, lc, selp, nb);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&ss->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
+    if (!last) return;
     if (t == 0) {
-        const int nb = 1 << best_bits;
+        __threadfence();
+        volatile SmallLevel *V = ss->lv;
+        int best_bits = 0;
+        unsigned long long best_cost = V[0].cost;
+        for (int b = 1; b < 4; b++)
+            if (V[b].cost < best_cost) { best_cost = V[b].cost; best_bits = b; }
+        const int n = 1 << best_bits;
+        for (int i = 0; i < 8; i++) { ly[i] = V[best_bits].ys[i]; lc[i] = V[best_bits].us[i]; }
+        ss->ticket = 0;
         dec->cdef_bits = best_bits;
         for (int i = 0; i < 8; i++) {
-            dec->y_strength[i] = i < nb ? kLevel1[cand[ys[i]]] : 0;
-            dec->uv_strength[i] = i < nb ? kLevel1[cand[NCAND + us[i]]] : 0;
+            dec->y_strength[i] = i < n ? kLevel1[cand[ly[i]]] : 0;
+            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + lc[i]]] : 0;
         }
-        ly[0] = nb;
+        nb_s = n;
     }
+    for (int fb = t; fb < nfb; fb += SMALL_T) fb_index[fb] = 0;
     __syncthreads();
-    const int nb = ly[0];
-    for (int fb = t; fb < nfb; fb += 256) fb_index[fb] = 0;
-    __syncthreads();
-    for (int i = t; i < nact; i += 256) {
-        unsigned long long bm = 1ull << 63;
-        int bg = 0;
-        for (int g = 0; g < nb; g++) {
-            const unsigned long long c = (unsigned long long)Ly[i][ys[g]] + Lc[i][us[g]];
+    const int n = nb_s;
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        if (i >= nact) continue;
+        int bm = pick8(Y[r], ly[0]) + pick8(U[r], lc[0]), bg = 0;
+        for (int g = 1; g < n; g++) {
+            const int c = pick8(Y[r], ly[g]) + pick8(U[r], lc[g]);
             if (c < bm) { bm = c; bg = g; }
         }
         fb_index[act[i]] = (uint8_t)bg;
@@ -896,17 +1018,23 @@ __global__ __launch_bounds__(256) void fin_small_kernel(const uint64_t *mse, con
 }
 
 hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
-                                 uint64_t lambda, const float *logits, int *cand, float *conf, uint64_t *mse_sub,
-                                 av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose, hipStream_t s)
+                                 uint64_t lambda, const float *logits, float *sums, int *cand, float *conf,
+                                 uint64_t *mse_sub, av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose,
+                                 void *small_state, hipStream_t s)
 {
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
-    if (propose)
-        hipLaunchKernelGGL(cand_kernel, dim3(1), dim3(256), 0, s, logits, noskip, nhfb, nvfb, mi_cols, mi_rows, cand, conf);
+    if (propose) {   // sums: 130 zeroed floats of scratch, cleared again by topk_kernel
+        hipLaunchKernelGGL(prob_sum_kernel, dim3(nhfb * nvfb), dim3(64), 0, s, logits, noskip, nhfb, mi_cols, mi_rows,
+                           sums);
+        hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(64), 0, s, sums, cand, conf);
+    }
     hipLaunchKernelGGL(cdef_search_subset_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, cand,
                        mse_sub, nhfb, nvfb);
-    hipLaunchKernelGGL(fin_small_kernel, dim3(1), dim3(256), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
-                       (unsigned long long)lambda, cand, dec, fb_index);
+    const int nfb = nhfb * nvfb;
+    auto fin = nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2> : fin_small_kernel<4>;
+    hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
+                       (unsigned long long)lambda, cand, static_cast<SmallState *>(small_state), dec, fb_index);
     return hipGetLastError();
 }
 

@@ -24,7 +24,8 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_lr", "av1pf_filmgrain", "av1pf_cdef_search", "av1pf_cdef_search_finish", "av1pf_cdef_lambda",
            "av1pf_cdef_search_select", "av1pf_lr_search",
            "av1pf_nn_param_count", "av1pf_nn_create", "av1pf_nn_destroy", "av1pf_nn_set_params",
-           "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_backward", "av1pf_nn_adam",
+           "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_infer", "av1pf_nn_backward",
+           "av1pf_nn_adam",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned",
            "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
@@ -78,6 +79,7 @@ def load():
     lib.av1pf_nn_get_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.av1pf_nn_tiles.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p, C.c_void_p]
     lib.av1pf_nn_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.av1pf_nn_infer.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p, C.c_void_p]
     lib.av1pf_nn_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_nn_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
     lib.av1pf_lr_search_masked.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_int,
@@ -380,6 +382,16 @@ class ProposalNet:
             out = torch.empty((n, 131), dtype=torch.float32, device="cuda")
         self.runner._check(self.lib.av1pf_nn_forward(self.h, tiles.data_ptr(), int(n), out.data_ptr(),
                                                      self.runner._stream(stream)))
+        return out
+
+    def infer(self, src, rec, out=None, stream=None):
+        """Logits [nfb][131] of a DeviceFrame pair in one fused launch."""
+        torch = _torch()
+        nfb = ((rec.meta.w + 63) // 64) * ((rec.meta.h + 63) // 64)
+        if out is None:
+            out = torch.empty((nfb, 131), dtype=torch.float32, device="cuda")
+        self.runner._check(self.lib.av1pf_nn_infer(self.h, C.byref(src.abi), C.byref(rec.abi), out.data_ptr(),
+                                                   self.runner._stream(stream)))
         return out
 
     def backward(self, tiles, targets, grads, loss, n=None, stream=None):

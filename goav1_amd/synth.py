@@ -399,28 +399,52 @@ def sequence_1080p10(seed=1, nframes=8, w=1920, h=1080, grain=True):
     return frames
 
 
-def search_pair(seed=3, w=1280, h=720, q=160, bd=8, skip_prob=0.15, skip_fb_prob=0.05, src=None):
+def _dct_quantise(x, step):
+    """8x8 orthonormal DCT, dead-zone quantisation (rounding offset 1/3 on AC,
+    1/2 on DC) at `step`, inverse DCT: the blocking, ringing and detail loss of
+    a transform codec."""
+    import scipy.fft
+    H, W = x.shape
+    hb, wb = (H + 7) // 8, (W + 7) // 8
+    pad = np.pad(x, ((0, hb * 8 - H), (0, wb * 8 - W)), mode="edge")
+    blk = pad.reshape(hb, 8, wb, 8).transpose(0, 2, 1, 3)
+    c = scipy.fft.dctn(blk, axes=(2, 3), norm="ortho")
+    off = np.full((8, 8), 1.0 / 3.0)
+    off[0, 0] = 0.5
+    qs = np.full((8, 8), step)
+    qs[0, 0] = step * 0.75
+    c = np.sign(c) * np.floor(np.abs(c) / qs + off) * qs
+    out = scipy.fft.idctn(c, axes=(2, 3), norm="ortho").transpose(0, 2, 1, 3).reshape(hb * 8, wb * 8)
+    return out[:H, :W]
+
+
+def search_pair(seed=3, w=1280, h=720, q=160, bd=8, skip_prob=0.15, skip_fb_prob=0.05, src=None, model="noise"):
     """Encoder-search input: a source frame (the 720p keyframe pixel recipe, or
-    `src` when given) and a reconstruction that carries quantisation-like
-    error (per-8x8 smoothing toward the block mean plus noise that grows with
-    the quantiser), the per-64x64 non-skip bit masks (bit r*8+c = 8x8 block
-    (r, c) has a non-skip MI; some filter blocks have none) and the damping
-    3 + (q >> 6) of the search."""
+    `src` when given), a reconstruction with coding error, the per-64x64
+    non-skip bit masks (bit r*8+c = 8x8 block (r, c) has a non-skip MI; some
+    filter blocks have none) and the damping 3 + (q >> 6) of the search.
+    model "noise": per-8x8 smoothing toward the block mean plus noise growing
+    with the quantiser; model "dct": 8x8 DCT dead-zone quantisation with a
+    step that follows AV1's 8-bit AC quantiser (about 8 at q60, 20 at q120,
+    39 at q160, 100 at q220, in orthonormal-DCT units)."""
     rng = np.random.default_rng(seed)
     if src is None:
         src = FrameData(w, h, bd, 1, 1)
         fill_blocky(src, rng)
     rec = src.copy_pixels()
-    step = 2 + q / 24.0
     maxv = (1 << src.bd) - 1
     for ps, pr in zip(src.planes, rec.planes):
         x = ps.pix.astype(np.float64)
-        H, W = x.shape
-        hb, wb = (H + 7) // 8, (W + 7) // 8
-        pad = np.pad(x, ((0, hb * 8 - H), (0, wb * 8 - W)), mode="edge")
-        mean = pad.reshape(hb, 8, wb, 8).mean(axis=(1, 3))
-        mean = np.repeat(np.repeat(mean, 8, 0), 8, 1)[:H, :W]
-        y = x + 0.35 * (mean - x) + rng.normal(0, step * 0.6, size=x.shape)
+        if model == "dct":
+            y = _dct_quantise(x, 8.0 * np.exp(0.0158 * (q - 60)) * (1 << (src.bd - 8)))
+        else:
+            step = 2 + q / 24.0
+            H, W = x.shape
+            hb, wb = (H + 7) // 8, (W + 7) // 8
+            pad = np.pad(x, ((0, hb * 8 - H), (0, wb * 8 - W)), mode="edge")
+            mean = pad.reshape(hb, 8, wb, 8).mean(axis=(1, 3))
+            mean = np.repeat(np.repeat(mean, 8, 0), 8, 1)[:H, :W]
+            y = x + 0.35 * (mean - x) + rng.normal(0, step * 0.6, size=x.shape)
         pr.pix[:] = np.clip(np.rint(y), 0, maxv).astype(np.uint16)
     mi_cols, mi_rows = aligned8(src.w) >> 2, aligned8(src.h) >> 2
     nhfb, nvfb = (mi_cols + 15) // 16, (mi_rows + 15) // 16

@@ -270,9 +270,10 @@ int  av1pf_lr_search_masked(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_
  * loop-restoration unit type (3: NONE, WIENER, SGRPROJ).  Trained by the
  * library's own forward/backward kernels (softmax cross-entropy against
  * target distributions, Adam on FP32 master weights).
- * Tiles: BF16, [nfb][64][64][2] (channel 0 source, 1 reconstruction, luma
- * divided by 2^bd - 1), one per 64x64 filter block in raster order, edge
- * samples repeated past the frame.  Parameters: FP32, per layer W [K][N]
+ * Tiles: BF16, [nfb][64][64][2] (channel 0 the reconstruction, channel 1
+ * the coding error (source - reconstruction) times 16, both divided by
+ * 2^bd - 1), one per 64x64 filter block in raster order, edge samples
+ * repeated past the frame.  Parameters: FP32, per layer W [K][N]
  * row-major then b [N], layers conv1 (K 18: 3x3 taps x 2 channels, tap-major),
  * conv2 (144), conv3 (288), fc1 (64 -> 128), heads (128 -> 131).           */
 typedef struct av1pf_nn av1pf_nn;
@@ -283,6 +284,11 @@ int  av1pf_nn_set_params(av1pf_nn *nn, const float *params);     /* host FP32; r
 int  av1pf_nn_get_params(av1pf_nn *nn, float *params);           /* host FP32 */
 int  av1pf_nn_tiles(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, uint16_t *tiles, void *stream);
 int  av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits, void *stream);
+/* Inference straight from an 8-bit frame pair: logits [nfb][131] for every
+ * 64x64 filter block, one fused launch (tiles built on chip, the same
+ * rounding points as av1pf_nn_tiles + av1pf_nn_forward, FP32 sums in another
+ * order).  What av1pf_search_learned runs.                                  */
+int  av1pf_nn_infer(av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *recon, float *logits, void *stream);
 /* forward + backward of n <= max_batch tiles against targets [n][131] (three
  * distributions); writes grads [param_count] and the mean loss [1]          */
 int  av1pf_nn_backward(av1pf_nn *nn, const uint16_t *tiles, int n, const float *targets, float *grads, float *loss,

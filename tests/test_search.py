@@ -206,7 +206,8 @@ def test_gpu_select_matches_host_finish(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2)])
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2), (2048, 1152, 120, 3),
+                                  (2304, 1856, 200, 4)])
 def test_gpu_shortlist_equals_masked_host_finish(case):
     """The shortlist search's table equals the full table's shortlisted
     columns, and its one-launch selection equals the host finish over the full
