@@ -1079,7 +1079,7 @@ int av1pf_nn_infer(av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *reco
     const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_NN_FORWARD, s, 0);
-    HIPCHK(c, nn_infer(infer_args(nn), view_of(src).p[0], view_of(recon).p[0], nhfb, nhfb * nvfb, logits, s));
+    HIPCHK(c, nn_infer(infer_args(nn), view_of(src).p[0], view_of(recon).p[0], nhfb, nhfb * nvfb, logits, nullptr, s));
     return AV1PF_OK;
 }
 
@@ -1150,9 +1150,8 @@ int av1pf_cdef_search_shortlist(av1pf_ctx *c, const av1pf_frame *src, const av1p
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
     if ((rc = ensure_small_state(c))) return rc;
-    HIPCHK(c, launch_search_learned(view_of(src), view_of(recon), noskip, damping, lambda, nullptr, nullptr,
-                                    const_cast<int32_t *>(cand), nullptr, mse_sub, decision, fb_index, false,
-                                    c->small_state, s));
+    HIPCHK(c, launch_search_shortlist(view_of(src), view_of(recon), noskip, damping, lambda, cand, mse_sub, decision,
+                                      fb_index, c->small_state, s));
     return AV1PF_OK;
 }
 
@@ -1168,22 +1167,28 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64, nfb = nhfb * nvfb;
     if (nfb > 2048) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 2048 filter blocks");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t logits_b = (size_t)nfb * NN_OUT * 4;
+    const size_t logits_b = (size_t)nfb * NN_OUT * 4, probs_b = (size_t)nfb * NN_PROP_STRIDE * 4;
     const size_t msub_b = (size_t)2 * nfb * NCAND * 8, mask_b = (size_t)nfb + 256;
-    const size_t need = logits_b + msub_b + mask_b + 4 * 256 + 1024;
+    const size_t need = align256(logits_b) + align256(probs_b) + align256(msub_b) + align256(mask_b) + 256;
     if ((rc = ensure(c, (void **)&c->learn_buf, &c->learn_cap, need))) return rc;
     char *p = c->learn_buf;
     float *logits = (float *)p; p += align256(logits_b);
+    float *probs = (float *)p; p += align256(probs_b);
     uint64_t *msub = (uint64_t *)p; p += align256(msub_b);
     uint8_t *mask = (uint8_t *)p; p += align256(mask_b);
-    float *sums = (float *)p;          // 130 floats of proposal scratch
-    float *conf_tmp = sums + 132;
+    float *conf_tmp = (float *)p;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
     const FrameView sv = view_of(src), rv = view_of(recon);
     if ((rc = ensure_small_state(c))) return rc;
-    HIPCHK(c, nn_zero(sums, 130, s));
-    HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], nhfb, nfb, logits, s));
+    NNPropose prop{};
+    prop.noskip = noskip;
+    prop.mi_cols = ((recon->width + 7) >> 3) << 1;
+    prop.mi_rows = ((recon->height + 7) >> 3) << 1;
+    prop.probs = probs;
+    prop.cand = cand;
+    prop.conf = conf ? conf : conf_tmp;
+    HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], nhfb, nfb, logits, &prop, s));
     // the LR search depends only on the logits: it runs on the side stream
     // beside the CDEF shortlist search and selection
     HIPCHK(c, hipEventRecord(c->learn_fork, s));
@@ -1193,8 +1198,8 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     HIPCHK(c, launch_lr_mask(logits, nhfb, nvfb, hz, vt, lr_none_threshold, mask, c->side));
     HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], 64, hz, vt, mask, lr_units, nullptr, c->side));
     HIPCHK(c, hipEventRecord(c->learn_join, c->side));
-    HIPCHK(c, launch_search_learned(sv, rv, noskip, damping, lambda, logits, sums, cand, conf ? conf : conf_tmp, msub,
-                                    decision, fb_index, true, c->small_state, s));
+    HIPCHK(c, launch_search_shortlist(sv, rv, noskip, damping, lambda, cand, msub, decision, fb_index, c->small_state,
+                                      s));
     HIPCHK(c, hipStreamWaitEvent(s, c->learn_join, 0));
     return AV1PF_OK;
 }

@@ -348,7 +348,9 @@ __device__ __forceinline__ v4s ld_frag(const uint16_t *p)
     return (v4s){ (short)(u.x & 0xffff), (short)(u.x >> 16), (short)(u.y & 0xffff), (short)(u.y >> 16) };
 }
 
-__global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R, int nhfb, NNInfer p, float *logits)
+template <bool PROPOSE>
+__global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R, int nhfb, NNInfer p, float *logits,
+                                                       NNPropose pr)
 {
     __shared__ __align__(16) uint16_t bufA[64 * 64 * 2];    // tile [64][64][2], then conv2 out [256][32]
     __shared__ __align__(16) uint16_t bufB[32 * 32 * 16];   // conv1 out [1024][16], then conv3 out [64][64]
@@ -457,18 +459,90 @@ __global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R,
         fc1[t] = bf2f(f2bf(fmaxf(sum + p.b3[t], 0.f)));
     }
     __syncthreads();
+    float lg = 0.f;
     if (t < NN_OUT) {
         float sum = 0.f;
 #pragma unroll 8
         for (int j = 0; j < 128; j++) sum += fc1[j] * bf2f(p.w4[j * NN_OUT + t]);
-        logits[(int64_t)fb * NN_OUT + t] = sum + p.b4[t];
+        lg = sum + p.b4[t];
+        logits[(int64_t)fb * NN_OUT + t] = lg;
+    }
+    if constexpr (PROPOSE) {
+        // waves 0 and 1 hold the luma and chroma strength logits: softmax per wave
+        const bool active = fb_active(pr.noskip, fb, nhfb, pr.mi_cols, pr.mi_rows);
+        float *pb = pr.probs + (size_t)fb * NN_PROP_STRIDE;
+        if (w < 2) {
+            float mx = lg;
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+            const float e = __expf(lg - mx);
+            float se = e;
+            for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+            const float pv = e / se;
+            pb[t] = active ? pv : 0.f;
+            if (w == 0) {
+                float pm = pv;
+                for (int o = 32; o > 0; o >>= 1) pm = fmaxf(pm, __shfl_xor(pm, o, 64));
+                if (t == 0) {
+                    pb[128] = active ? pm : 0.f;
+                    pb[129] = active ? 1.f : 0.f;
+                }
+            }
+        }
     }
 }
 
-hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
-                    hipStream_t s)
+// Shortlist from the per-block probabilities: thread (column c, part k) sums
+// the blocks f = k, k + 8, ...; the parts are added in order, so the sums do
+// not depend on timing.  Then the NCAND strengths with the largest sums per
+// plane (ascending strength order) and the mean largest luma probability.
+__global__ __launch_bounds__(1024) void propose_kernel(const float *probs, int nfb, int *cand, float *conf)
 {
-    hipLaunchKernelGGL(nn_infer_kernel, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits);
+    __shared__ float part[8][NN_PROP_STRIDE];
+    __shared__ float sums[NN_PROP_STRIDE];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    {
+        const int c = t & 127, k = t >> 7;
+        float acc = 0.f;
+#pragma unroll 8
+        for (int f = k; f < nfb; f += 8) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
+        part[k][c] = acc;
+    }
+    if (t < 16) {
+        const int c = 128 + (t & 1), k = t >> 1;
+        float acc = 0.f;
+        for (int f = k; f < nfb; f += 8) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
+        part[k][c] = acc;
+    }
+    __syncthreads();
+    if (t < NN_PROP_STRIDE) {
+        float v = 0.f;
+        for (int k = 0; k < 8; k++) v += part[k][t];
+        sums[t] = v;
+    }
+    __syncthreads();
+    if (w < 2) {
+        // rank = strengths strictly ahead (larger sum, or equal and earlier)
+        const float v = sums[t];
+        int rank = 0;
+        for (int j = 0; j < 64; j++) {
+            const float u = sums[w * 64 + j];
+            rank += (u > v) || (u == v && j < lane);
+        }
+        const bool in = rank < NCAND;
+        const uint64_t vote = __ballot(in);
+        if (in) cand[w * NCAND + __popcll(vote & ((1ull << lane) - 1))] = lane;
+    }
+    if (t == 0) *conf = sums[129] > 0.f ? sums[128] / sums[129] : 1.f;
+}
+
+hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
+                    const NNPropose *prop, hipStream_t s)
+{
+    if (prop) {
+        hipLaunchKernelGGL(nn_infer_kernel<true>, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits, *prop);
+        hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->probs, nfb, prop->cand, prop->conf);
+    } else
+        hipLaunchKernelGGL(nn_infer_kernel<false>, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits, NNPropose{});
     return hipGetLastError();
 }
 

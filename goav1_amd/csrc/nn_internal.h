@@ -57,6 +57,18 @@ struct GemmArgs {
     int64_t ldc;
 };
 
+// A filter block takes part in the CDEF search when one of its 8x8 blocks
+// inside the frame is not skipped (noskip bit (r / 2) * 8 + c / 2).
+__device__ __forceinline__ bool fb_active(const uint64_t *noskip, int fb, int nhfb, int mi_cols, int mi_rows)
+{
+    const int fbr = fb / nhfb, fbc = fb % nhfb;
+    const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+    uint64_t valid = 0;
+    for (int r = 0; r < nvb; r += 2)
+        for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
+    return ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
+}
+
 // Fused inference: conv B fragments (packed from the BF16 weights), biases,
 // fc weights.
 constexpr int NN_FRAG_ELEMS = 23552;
@@ -66,11 +78,23 @@ struct NNInfer {
     const uint16_t *w3, *w4;
     const float *b3, *b4;
 };
+// Shortlist proposal of the learned search: the inference writes, per
+// active filter block, the softmax of the luma and chroma strength logits;
+// a one-workgroup pass sums them over the blocks and keeps the NCAND
+// strengths with the largest sums per plane.
+constexpr int NN_PROP_STRIDE = 130;   // 64 + 64 probabilities, largest luma probability, active flag
+struct NNPropose {
+    const uint64_t *noskip;
+    int mi_cols, mi_rows;
+    float *probs;            // [nfb][NN_PROP_STRIDE] scratch
+    int *cand;               // [2 * NCAND], ascending per plane
+    float *conf;             // mean largest luma probability over the active blocks
+};
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 hipError_t nn_pack_frag(const NNWeights &w, uint16_t *frag, hipStream_t s);
 hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
-                    hipStream_t s);
+                    const NNPropose *prop, hipStream_t s);
 hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s);
 hipError_t nn_forward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, float *logits, hipStream_t s);
 hipError_t nn_backward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, const float *target,
@@ -79,10 +103,13 @@ hipError_t nn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, i
                    hipStream_t s);
 hipError_t nn_zero(float *p, int64_t n, hipStream_t s);
 size_t nn_work_bytes(int max_batch);
-hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
-                                 uint64_t lambda, const float *logits, float *sums, int *cand, float *conf,
-                                 uint64_t *mse_sub, av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose,
-                                 void *small_state, hipStream_t s);
+// Exact SSE of the shortlisted strengths and the one-launch selection over
+// the shortlist pairs.
+hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
+                                   uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
+                                   uint8_t *fb_index, void *small_state, hipStream_t s);
+// Device state of the shortlist selection (zeroed once): per-level results
+// and the ticket.
 size_t small_state_bytes();
 void nn_work_carve(char *base, int max_batch, NNWork &ws);
 

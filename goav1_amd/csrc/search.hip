@@ -545,125 +545,118 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
 // NCAND luma and NCAND chroma strengths; only those are evaluated exactly,
 // and the same greedy selection runs over the NCAND x NCAND pairs.
 
-// Active filter block flags (a non-skip 8x8 block inside the frame).
-__device__ __forceinline__ bool fb_active(const uint64_t *noskip, int fb, int nhfb, int mi_cols, int mi_rows)
+// Exact SSE of the shortlisted strengths.  mse[2][nfb][NCAND].  Per pixel the
+// taps are read once and turned into |d| and a signed weight (primary: +-1,
+// secondary: +-2 for the near and +-1 for the far taps); a strength's
+// contribution is then sum w_k clamp(s - (|d_k| >> shift), 0, |d_k|), the
+// constrain function with its sign folded into the weight.  The secondary
+// sums depend only on the secondary strength, so each one present in the
+// shortlist (`need`: bits 0..2 for strengths 1, 2, 4 along the block
+// direction, bits 3..5 along direction 0 for primary-0 trials) is computed
+// once per pixel and shared by the trials.
+// clamp(v, 0, a) for a >= 0, one v_med3_i32
+__device__ __forceinline__ int s_clamp0(int v, int a)
 {
-    const int fbr = fb / nhfb, fbc = fb % nhfb;
-    const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
-    uint64_t valid = 0;
-    for (int r = 0; r < nvb; r += 2)
-        for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
-    return ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "v"(a));
+    return r;
 }
 
-// Shortlist from the network's logits, in two launches:
-//  - prob_sum_kernel: one wave per active filter block, lane i holds luma and
-//    chroma logit i; the softmax probabilities are added to per-strength sums
-//    (sums[0..63] luma, [64..127] chroma, [128] the largest luma probability,
-//    [129] the active block count), device atomics;
-//  - topk_kernel: the NCAND strengths with the largest sums per plane, in
-//    ascending strength-list order, and conf = mean largest luma probability.
-__global__ __launch_bounds__(64) void prob_sum_kernel(const float *logits, const uint64_t *noskip, int nhfb,
-                                                      int mi_cols, int mi_rows, float *sums)
+template <int N>
+__device__ __forceinline__ int s_taps_sum(const int (&ad)[N], const int (&sw)[N], int st, int sh)
 {
-    const int fb = blockIdx.x, lane = threadIdx.x;
-    if (!fb_active(noskip, fb, nhfb, mi_cols, mi_rows)) return;
-    const float *z = logits + (size_t)fb * NN_OUT;
-    for (int h = 0; h < 2; h++) {
-        const float v = z[h * 64 + lane];
-        float mx = v;
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-        const float e = __expf(v - mx);
-        float se = e;
-        for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
-        const float p = e / se;
-        atomicAdd(&sums[h * 64 + lane], p);
-        if (h == 0) {
-            float pm = p;
-            for (int o = 32; o > 0; o >>= 1) pm = fmaxf(pm, __shfl_xor(pm, o, 64));
-            if (lane == 0) {
-                atomicAdd(&sums[128], pm);
-                atomicAdd(&sums[129], 1.f);
-            }
-        }
-    }
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) s += __mul24(sw[k], s_clamp0(st - (ad[k] >> sh), ad[k]));
+    return s;
 }
 
-__global__ __launch_bounds__(64) void topk_kernel(float *sums, int *cand, float *conf)
-{
-    const int lane = threadIdx.x;
-    for (int h = 0; h < 2; h++) {
-        const float v = sums[h * 64 + lane];
-        // rank = number of strengths strictly ahead (larger sum, or equal and lower index)
-        int rank = 0;
-        for (int j = 0; j < 64; j++) {
-            const float u = sums[h * 64 + j];
-            rank += (u > v) || (u == v && j < lane);
-        }
-        const bool in = rank < NCAND;
-        const uint64_t vote = __ballot(in);
-        if (in) cand[h * NCAND + __popcll(vote & ((1ull << lane) - 1))] = lane;
-    }
-    if (lane == 0) *conf = sums[129] > 0.f ? sums[128] / sums[129] : 1.f;
-    __syncthreads();
-    // clear the sums for the next frame
-    sums[lane] = 0.f;
-    sums[64 + lane] = 0.f;
-    if (lane < 2) sums[128 + lane] = 0.f;
-}
-
-// Exact SSE of the shortlisted strengths: the MSE kernel's decomposition with
-// NCAND trials per plane read from the shortlist.  mse[2][nfb][NCAND].
 template <bool LUMA>
 __device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t *c, int srcv, int dir, int adj_i,
-                                            bool has_var, int damping, const int *pk)
+                                            bool has_var, int damping, const int *pk, unsigned need)
 {
     const int x = c[0];
     const int p0 = kSOff[dir + 2][0], p1 = kSOff[dir + 2][1];
-    const int a0 = c[p0], a1 = c[-p0], a2 = c[p1], a3 = c[-p1];
-    int sb[8], sz[8];
+    const int a[4] = { c[p0], c[-p0], c[p1], c[-p1] };
+    int sb[8];
     {
         const int o[4] = { kSOff[dir + 4][0], kSOff[dir][0], kSOff[dir + 4][1], kSOff[dir][1] };
 #pragma unroll
         for (int k = 0; k < 4; k++) { sb[2 * k] = c[o[k]]; sb[2 * k + 1] = c[-o[k]]; }
-        const int z[4] = { kSOff[4][0], kSOff[0][0], kSOff[4][1], kSOff[0][1] };
-#pragma unroll
-        for (int k = 0; k < 4; k++) { sz[2 * k] = c[z[k]]; sz[2 * k + 1] = c[-z[k]]; }
     }
     const int mk = SVL - 1;
-    int hi = max(max(max(x, a0 & mk), max(a1 & mk, a2 & mk)), a3 & mk);
-    int lo = min(min(min(x, a0), min(a1, a2)), a3);
+    int hi = max(max(max(x, a[0] & mk), max(a[1] & mk, a[2] & mk)), a[3] & mk);
+    int lo = min(min(min(x, a[0]), min(a[1], a[2])), a[3]);
 #pragma unroll
     for (int k = 0; k < 8; k++) { hi = max(hi, sb[k] & mk); lo = min(lo, sb[k]); }
+    int ap[4], wp[4], as[8], ws[8];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int d = a[k] - x;
+        ap[k] = d < 0 ? -d : d;
+        wp[k] = d < 0 ? -1 : 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int d = sb[k] - x, w = k < 4 ? 2 : 1;
+        as[k] = d < 0 ? -d : d;
+        ws[k] = d < 0 ? -w : w;
+    }
+    int S[3] = { 0, 0, 0 }, Z[3] = { 0, 0, 0 };
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if ((need >> i) & 1) S[i] = s_taps_sum(as, ws, 1 << i, max(damping - i, 0));
+    if (need & 0x38) {
+        const int z[4] = { kSOff[4][0], kSOff[0][0], kSOff[4][1], kSOff[0][1] };
+        int az[8], wz[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int d = c[(k & 1) ? -z[k >> 1] : z[k >> 1]] - x, w = k < 4 ? 2 : 1;
+            az[k] = d < 0 ? -d : d;
+            wz[k] = d < 0 ? -w : w;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if ((need >> (3 + i)) & 1) Z[i] = s_taps_sum(az, wz, 1 << i, max(damping - i, 0));
+    }
 #pragma unroll
     for (int q = 0; q < NCAND; q++) {
         const int packed = pk[q];
-        const int pri = packed >> 2;
-        const int sec = (packed & 3) == 3 ? 4 : (packed & 3);
-        int st = pri;
-        if (LUMA && pri) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
-        int out = x;
-        if (st || sec) {
-            const int ssh = sec ? max(damping - s_ilog2((unsigned)sec), 0) : 0;
-            int sum = 0;
-            const int *t = pri ? sb : sz;
-            if (pri && st) {
+        const int pri = packed >> 2, sc = packed & 3;   // secondary code 0..3 = strength 0, 1, 2, 4
+        int out;
+        if (pri) {
+            int st = pri;
+            if (LUMA) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
+            int sum = sc == 0 ? 0 : sc == 1 ? S[0] : sc == 2 ? S[1] : S[2];
+            if (st) {
                 const int psh = max(damping - s_ilog2((unsigned)st), 0);
                 const int t0 = (st & 1) ? 3 : 4, t1 = (st & 1) ? 3 : 2;
-                sum = t0 * (s_constrain(a0 - x, st, psh) + s_constrain(a1 - x, st, psh)) +
-                      t1 * (s_constrain(a2 - x, st, psh) + s_constrain(a3 - x, st, psh));
+                const int m01 = __mul24(wp[0], s_clamp0(st - (ap[0] >> psh), ap[0])) +
+                                __mul24(wp[1], s_clamp0(st - (ap[1] >> psh), ap[1]));
+                const int m23 = __mul24(wp[2], s_clamp0(st - (ap[2] >> psh), ap[2])) +
+                                __mul24(wp[3], s_clamp0(st - (ap[3] >> psh), ap[3]));
+                sum += __mul24(t0, m01) + __mul24(t1, m23);
             }
-            if (sec)
-                sum += 2 * (s_constrain(t[0] - x, sec, ssh) + s_constrain(t[1] - x, sec, ssh) +
-                            s_constrain(t[2] - x, sec, ssh) + s_constrain(t[3] - x, sec, ssh)) +
-                       s_constrain(t[4] - x, sec, ssh) + s_constrain(t[5] - x, sec, ssh) +
-                       s_constrain(t[6] - x, sec, ssh) + s_constrain(t[7] - x, sec, ssh);
             out = x + ((8 + sum - (sum < 0)) >> 4);
-            if (pri && st && sec) out = min(max(out, lo), hi);
+            if (st && sc) out = min(max(out, lo), hi);
+        } else {
+            const int sum = sc == 0 ? 0 : sc == 1 ? Z[0] : sc == 2 ? Z[1] : Z[2];
+            out = x + ((8 + sum - (sum < 0)) >> 4);
         }
         const int e = srcv - out;
-        acc[q] += (uint32_t)(e * e);
+        acc[q] += (uint32_t)__mul24(e, e);
     }
+}
+
+// Which shared secondary sums a shortlist needs (see s_pixel_sub).
+__device__ __forceinline__ unsigned s_need(const int *pk)
+{
+    unsigned need = 0;
+    for (int q = 0; q < NCAND; q++) {
+        const int pri = pk[q] >> 2, sc = pk[q] & 3;
+        if (sc) need |= 1u << ((sc == 3 ? 2 : sc - 1) + (pri ? 0 : 3));
+    }
+    return need;
 }
 
 __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
@@ -689,6 +682,7 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
         pk[0][q] = kLevel1[cand[q]];
         pk[1][q] = kLevel1[cand[NCAND + q]];
     }
+    const unsigned need[2] = { s_need(pk[0]), s_need(pk[1]) };
     if (tid == 0) {
         int n = 0;
         for (int r = 0; r < nvb; r += 2)
@@ -732,18 +726,21 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
                 badj[b] = (int8_t)i;
             }
             __syncthreads();
+            // a wave covers one 8x8 block (64 pixels): its direction and
+            // strength adjustment are wave-uniform
             for (int p = tid; p < nblk * 64; p += 256) {
-                const int b = blist[p >> 6], q = p & 63;
+                const int b = __builtin_amdgcn_readfirstlane(blist[p >> 6]), q = p & 63;
+                const int d = __builtin_amdgcn_readfirstlane(bdir[b]), adj = __builtin_amdgcn_readfirstlane(badj[b]);
                 const int yy = (b >> 3) * 8 + (q >> 3), xx = (b & 7) * 8 + (q & 7);
                 s_pixel_sub<true>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[0], y0 + yy)[x0 + xx],
-                                  bdir[b], badj[b], badj[b] >= 0, damping, pk[0]);
+                                  d, adj, adj >= 0, damping, pk[0], need[0]);
             }
         } else {
             for (int p = tid; p < nblk * 16; p += 256) {
                 const int b = blist[p >> 4], q = p & 15;
                 const int yy = (b >> 3) * 4 + (q >> 2), xx = (b & 7) * 4 + (q & 3);
                 s_pixel_sub<false>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[pl], y0 + yy)[x0 + xx],
-                                   bdir[b], 0, true, damping - 1, pk[1]);
+                                   bdir[b], 0, true, damping - 1, pk[1], need[1]);
             }
         }
         if (pl == 0 || pl == 2) {
@@ -769,10 +766,15 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
 // registers, exact as int32: an 8-bit 64x64 luma block's SSE is at most
 // 4096 * 255^2 and its two 32x32 chroma blocks' 2048 * 255^2, so a pair's
 // cost stays below 2^29.  A step computes each block's best cost among the
-// chosen pairs and its gain max(0, best - y_j - uv_k) for all 64 pairs; a
-// butterfly over the wave (each exchange halves the values a lane holds)
-// leaves lane l with the wave total of pair bitrev6(l), and wave 0 adds the
-// waves' totals and takes the first maximum.  Each level's cost and pairs go
+// chosen pairs and the gains max(0, best - y_j - uv_k) of all 64 pairs, in
+// one of two ways:
+//  - pair lanes (up to 512 blocks): lane p = 8 j + k also holds y_j + uv_k of
+//    the 64 blocks its wave owns; the blocks' best costs go through LDS (a
+//    broadcast read) and the lane sums its own pair's gain;
+//  - butterfly (more blocks): each thread computes its blocks' gains for all
+//    pairs and an exchange over the wave that halves the values a lane holds
+//    at each stage leaves lane l with the wave total of pair bitrev6(l).
+// Wave 0 adds the waves' totals and takes the first maximum.  Each level's cost and pairs go
 // to `lv`; the last workgroup to finish (ticket) keeps the cheapest level
 // (the lowest on ties, as a sequential scan with a strict comparison would)
 // and writes the decision.
@@ -821,17 +823,19 @@ __device__ __forceinline__ void bfly(const T *in, T *out, int lane)
 // A block's 8 shortlist SSEs as one register vector: indexing it with a
 // wave-uniform (scalar) index stays in registers.
 typedef int cand8 __attribute__((ext_vector_type(NCAND)));
-__device__ __forceinline__ int pick8(const cand8 &a, int j) { return a[__builtin_amdgcn_readfirstlane(j)]; }
 
-__device__ __forceinline__ void pack_sel(const int *ly, const int *lc, int *selp, int nb)
+// Largest 64-bit value over the wave (every lane ends with it).
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 {
-    unsigned a = 0, c = 0;
-    for (int i = 0; i < nb; i++) { a |= (unsigned)ly[i] << (4 * i); c |= (unsigned)lc[i] << (4 * i); }
-    selp[0] = (int)a;
-    selp[1] = (int)c;
+    v = max(v, xchg64<1>(v));
+    v = max(v, xchg64<2>(v));
+    v = max(v, xchg64<4>(v));
+    v = max(v, xchg64<8>(v));
+    v = max(v, xchg64<16>(v));
+    return max(v, xchg64<32>(v));
 }
 
-template <int NPER>
+template <int NPER, bool LANE_PAIRS>
 __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
                                                             int nvfb, int mi_cols, int mi_rows,
                                                             unsigned long long lambda, const int *cand,
@@ -842,9 +846,9 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     __shared__ uint16_t act[SMALL_MAX_FB];
     __shared__ uint8_t flag[SMALL_MAX_FB];
     __shared__ unsigned long long wpart[NW][65];   // per-wave pair gains (or column sums) and base sum
-    __shared__ int ly[8], lc[8];
-    __shared__ int selp[2];
-    __shared__ int nact_s, nb_s;
+    __shared__ __align__(16) uint32_t bw[LANE_PAIRS ? NPER * SMALL_T : 1];   // each block's best cost, for the pair lanes
+    __shared__ int pick_s, nact_s, bits_s;
+    __shared__ unsigned sel_s[2];
     __shared__ bool last;
     __shared__ unsigned long long dist_s;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -875,10 +879,27 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
             U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
         }
     }
+    // pair lanes: lane p = 8 j + k of wave w holds y_j + uv_k of the wave's
+    // blocks w * 64 + q (+ 512 r), so its gain needs only the blocks' best costs
+    uint32_t YU[LANE_PAIRS ? NPER : 1][LANE_PAIRS ? 64 : 1];
+    if constexpr (LANE_PAIRS) {
+#pragma unroll
+        for (int r = 0; r < NPER; r++)
+#pragma unroll
+            for (int q = 0; q < 64; q++) {
+                const int i = r * SMALL_T + wv * 64 + q;
+                const size_t fb = i < nact ? act[i] : 0;
+                YU[r][q] = i < nact ? (uint32_t)(mse[fb * NCAND + (lane >> 3)] + mse[((size_t)nfb + fb) * NCAND + (lane & 7)])
+                                    : 0u;
+            }
+    }
     const int bits = blockIdx.x;
     const int nb = 1 << bits;
+    // the chosen pairs, 4 bits per slot, kept by every thread (wave-uniform)
+    unsigned sy = 0, su = 0;
     for (int step = 0; step < 5 * nb; step++) {
         const int slot = step < nb ? step : nb - 1;
+        const bool final_step = step == 5 * nb - 1;
         if (slot == 0) {
             // column sums, one wave reduction per shortlist column
 #pragma unroll
@@ -901,14 +922,10 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
                     if (cs[q] < cs[by]) by = q;
                     if (cs[NCAND + q] < cs[NCAND + bc]) bc = q;
                 }
-                ly[0] = by;
-                lc[0] = bc;
+                pick_s = by * NCAND + bc;
                 dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
             }
         } else {
-            // the chosen pairs, 4 bits each, read once
-            const unsigned sy = (unsigned)__builtin_amdgcn_readfirstlane(selp[0]);
-            const unsigned su = (unsigned)__builtin_amdgcn_readfirstlane(selp[1]);
             unsigned b[NPER];
             unsigned long long bsum = 0;
 #pragma unroll
@@ -918,65 +935,98 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
                     b[r] = min(b[r], (unsigned)(Y[r][(sy >> (4 * g)) & 7] + U[r][(su >> (4 * g)) & 7]));
                 bsum += b[r];
             }
-            // per-thread gains max(0, b - y_j - uv_k) by saturating subtractions:
-            // at most 4 * 2^29 < 2^32, and two lanes' sums fit too
-            uint32_t g64[64];
+            unsigned long long gain;
+            if constexpr (LANE_PAIRS) {
 #pragma unroll
-            for (int p = 0; p < 64; p++) g64[p] = 0;
+                for (int r = 0; r < NPER; r++) bw[r * SMALL_T + t] = b[r];
+                __builtin_amdgcn_wave_barrier();   // the wave reads back only its own blocks' costs
+                gain = 0;
 #pragma unroll
-            for (int r = 0; r < NPER; r++)
+                for (int r = 0; r < NPER; r++)
 #pragma unroll
-                for (int j = 0; j < NCAND; j++) {
-                    const unsigned rj = __builtin_elementwise_sub_sat(b[r], (unsigned)Y[r][j]);
+                    for (int q = 0; q < 64; q += 8) {   // 8 gains of at most 2^29 fit 32 bits
+                        const uint4 b0 = *reinterpret_cast<const uint4 *>(&bw[r * SMALL_T + wv * 64 + q]);
+                        const uint4 b1 = *reinterpret_cast<const uint4 *>(&bw[r * SMALL_T + wv * 64 + q + 4]);
+                        uint32_t g = __builtin_elementwise_sub_sat(b0.x, YU[r][q]) +
+                                     __builtin_elementwise_sub_sat(b0.y, YU[r][q + 1]) +
+                                     __builtin_elementwise_sub_sat(b0.z, YU[r][q + 2]) +
+                                     __builtin_elementwise_sub_sat(b0.w, YU[r][q + 3]);
+                        g += __builtin_elementwise_sub_sat(b1.x, YU[r][q + 4]) +
+                             __builtin_elementwise_sub_sat(b1.y, YU[r][q + 5]) +
+                             __builtin_elementwise_sub_sat(b1.z, YU[r][q + 6]) +
+                             __builtin_elementwise_sub_sat(b1.w, YU[r][q + 7]);
+                        gain += g;
+                    }
+            } else {
+                // per-thread gains max(0, b - y_j - uv_k) by saturating
+                // subtractions: at most 4 * 2^29 < 2^32, and two lanes' sums fit too
+                uint32_t g64[64];
 #pragma unroll
-                    for (int k = 0; k < NCAND; k++)
-                        g64[j * NCAND + k] += __builtin_elementwise_sub_sat(rj, (unsigned)U[r][k]);
-                }
-            uint32_t g32[32];
-            bfly<1, 32>(g64, g32, lane);
-            unsigned long long h32[32], h16[16], h8[8], h4[4], h2[2], h1[1];
+                for (int p = 0; p < 64; p++) g64[p] = 0;
 #pragma unroll
-            for (int i = 0; i < 32; i++) h32[i] = g32[i];
-            bfly<2, 16>(h32, h16, lane);
-            bfly<4, 8>(h16, h8, lane);
-            bfly<8, 4>(h8, h4, lane);
-            bfly<16, 2>(h4, h2, lane);
-            bfly<32, 1>(h2, h1, lane);
-            if (step == 5 * nb - 1) {   // the frame distortion is needed at the last step only
+                for (int r = 0; r < NPER; r++)
+#pragma unroll
+                    for (int j = 0; j < NCAND; j++) {
+                        const unsigned rj = __builtin_elementwise_sub_sat(b[r], (unsigned)Y[r][j]);
+#pragma unroll
+                        for (int k = 0; k < NCAND; k++)
+                            g64[j * NCAND + k] += __builtin_elementwise_sub_sat(rj, (unsigned)U[r][k]);
+                    }
+                uint32_t g32[32];
+                bfly<1, 32>(g64, g32, lane);
+                unsigned long long h32[32], h16[16], h8[8], h4[4], h2[2], h1[1];
+#pragma unroll
+                for (int i = 0; i < 32; i++) h32[i] = g32[i];
+                bfly<2, 16>(h32, h16, lane);
+                bfly<4, 8>(h16, h8, lane);
+                bfly<8, 4>(h8, h4, lane);
+                bfly<16, 2>(h4, h2, lane);
+                bfly<32, 1>(h2, h1, lane);
+                gain = h1[0];
+            }
+            if (final_step) {   // the frame distortion is needed at the last step only
                 for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
                 if (lane == 0) wpart[wv][64] = bsum;
             }
-            wpart[wv][lane] = h1[0];
+            wpart[wv][lane] = gain;
             __syncthreads();
-            if (wv == 0) {   // first maximum over the 64 pairs
-                unsigned long long v = 0, base = 0;
-                for (int w = 0; w < NW; w++) { v += wpart[w][lane]; base += wpart[w][64]; }
-                int p = ((lane & 1) << 5) | ((lane & 2) << 3) | ((lane & 4) << 1) | ((lane & 8) >> 1) |
-                        ((lane & 16) >> 3) | ((lane & 32) >> 5);
-                for (int o = 32; o > 0; o >>= 1) {
-                    const unsigned long long v2 = __shfl_xor(v, o, 64);
-                    const int p2 = __shfl_xor(p, o, 64);
-                    if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
-                }
+            if (wv == 0) {   // first maximum over the 64 pairs: the largest (gain, 63 - pair) key
+                unsigned long long v = 0;
+                for (int w = 0; w < NW; w++) v += wpart[w][lane];
+                const int p = LANE_PAIRS ? lane
+                                         : ((lane & 1) << 5) | ((lane & 2) << 3) | ((lane & 4) << 1) |
+                                               ((lane & 8) >> 1) | ((lane & 16) >> 3) | ((lane & 32) >> 5);
+                const unsigned long long key = wave_max_u64((v << 6) | (unsigned long long)(63 - p));
                 if (lane == 0) {
-                    ly[slot] = p / NCAND;
-                    lc[slot] = p % NCAND;
-                    dist_s = base - v;
+                    pick_s = 63 - (int)(key & 63);
+                    if (final_step) {
+                        unsigned long long base = 0;
+                        for (int w = 0; w < NW; w++) base += wpart[w][64];
+                        dist_s = base - (key >> 6);
+                    }
                 }
             }
         }
-        if (t == 0) {
-            if (step == 5 * nb - 1) {
+        __syncthreads();
+        const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
+        sy = (sy & ~(15u << (4 * slot))) | ((pk / NCAND) << (4 * slot));
+        su = (su & ~(15u << (4 * slot))) | ((pk % NCAND) << (4 * slot));
+        if (final_step) {
+            if (t == 0) {
                 const long long rate = (long long)(nact * bits + nb * 12) << 9;
                 SmallLevel &L = ss->lv[bits];
                 L.cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((dist_s * 16) << 7);
-                for (int i = 0; i < 8; i++) { L.ys[i] = i < nb ? ly[i] : 0; L.us[i] = i < nb ? lc[i] : 0; }
-            } else if (step + 1 >= nb) {   // refinement: the next step drops the oldest pair
-                for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
+                for (int i = 0; i < 8; i++) {
+                    L.ys[i] = i < nb ? (int)((sy >> (4 * i)) & 15) : 0;
+                    L.us[i] = i < nb ? (int)((su >> (4 * i)) & 15) : 0;
+                }
             }
-            pack_sel(ly, lc, selp, nb);
+        } else if (step + 1 >= nb) {   // refinement: the next step drops the oldest pair
+            sy >>= 4;
+            su >>= 4;
         }
-        __syncthreads();
+        // no barrier needed here: wave 0 rewrites pick_s and the partials only
+        // after the next step's first barrier, which every reader of them reaches
     }
     if (t == 0) {
         __threadfence();
@@ -992,47 +1042,49 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
         for (int b = 1; b < 4; b++)
             if (V[b].cost < best_cost) { best_cost = V[b].cost; best_bits = b; }
         const int n = 1 << best_bits;
-        for (int i = 0; i < 8; i++) { ly[i] = V[best_bits].ys[i]; lc[i] = V[best_bits].us[i]; }
+        unsigned a = 0, c = 0;
+        for (int i = 0; i < n; i++) { a |= (unsigned)V[best_bits].ys[i] << (4 * i); c |= (unsigned)V[best_bits].us[i] << (4 * i); }
         ss->ticket = 0;
         dec->cdef_bits = best_bits;
         for (int i = 0; i < 8; i++) {
-            dec->y_strength[i] = i < n ? kLevel1[cand[ly[i]]] : 0;
-            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + lc[i]]] : 0;
+            dec->y_strength[i] = i < n ? kLevel1[cand[V[best_bits].ys[i]]] : 0;
+            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + V[best_bits].us[i]]] : 0;
         }
-        nb_s = n;
+        bits_s = best_bits;
+        sel_s[0] = a;
+        sel_s[1] = c;
     }
     for (int fb = t; fb < nfb; fb += SMALL_T) fb_index[fb] = 0;
     __syncthreads();
-    const int n = nb_s;
+    const int n = 1 << bits_s;
+    const unsigned fy = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[0]), fu = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[1]);
 #pragma unroll
     for (int r = 0; r < NPER; r++) {
         const int i = t + r * SMALL_T;
         if (i >= nact) continue;
-        int bm = pick8(Y[r], ly[0]) + pick8(U[r], lc[0]), bg = 0;
+        int bm = Y[r][fy & 7] + U[r][fu & 7], bg = 0;
         for (int g = 1; g < n; g++) {
-            const int c = pick8(Y[r], ly[g]) + pick8(U[r], lc[g]);
+            const int c = Y[r][(fy >> (4 * g)) & 7] + U[r][(fu >> (4 * g)) & 7];
             if (c < bm) { bm = c; bg = g; }
         }
         fb_index[act[i]] = (uint8_t)bg;
     }
 }
 
-hipError_t launch_search_learned(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
-                                 uint64_t lambda, const float *logits, float *sums, int *cand, float *conf,
-                                 uint64_t *mse_sub, av1pf_cdef_decision *dec, uint8_t *fb_index, bool propose,
-                                 void *small_state, hipStream_t s)
+hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
+                                   uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
+                                   uint8_t *fb_index, void *small_state, hipStream_t s)
 {
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
-    if (propose) {   // sums: 130 zeroed floats of scratch, cleared again by topk_kernel
-        hipLaunchKernelGGL(prob_sum_kernel, dim3(nhfb * nvfb), dim3(64), 0, s, logits, noskip, nhfb, mi_cols, mi_rows,
-                           sums);
-        hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(64), 0, s, sums, cand, conf);
-    }
     hipLaunchKernelGGL(cdef_search_subset_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, cand,
                        mse_sub, nhfb, nvfb);
     const int nfb = nhfb * nvfb;
-    auto fin = nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2> : fin_small_kernel<4>;
+    // up to 512 blocks: pair lanes; beyond, the pair costs no longer fit the
+    // registers and the gains are summed by the butterfly
+    auto fin = nfb <= SMALL_T       ? fin_small_kernel<1, true>
+               : nfb <= 2 * SMALL_T ? fin_small_kernel<2, false>
+                                    : fin_small_kernel<4, false>;
     hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
                        (unsigned long long)lambda, cand, static_cast<SmallState *>(small_state), dec, fb_index);
     return hipGetLastError();
