@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-search", action="store_true", help="skip the encoder-search leg")
+    ap.add_argument("--no-search", action="store_true", help="skip the encoder-search legs")
+    ap.add_argument("--learned-frames", type=int, default=20, help="held-out frames of the learned-search leg")
     ap.add_argument("--batch-frames", type=int, default=64,
                     help="frames in the mixed-batch leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -150,6 +151,42 @@ def search_leg(runner, steps, cpu):
                                "sample": f"1 frame of the search workload, oracle CDEF table + finish + LR search, 1 thread, "
                                          f"{el:.1f} s"}
     return out
+
+
+def learned_leg(runner, nframes, width=1920, height=1080):
+    """Learned-proposal search against the exhaustive search on held-out
+    moving-rectangle frames (8-bit 4:2:0, q 60..220): the network (committed
+    weights) proposes 8 luma + 8 chroma strengths and the LR units worth
+    searching; the shortlist is evaluated exactly on the device.  Both
+    searches run the LR search on a second stream beside the CDEF work.
+    PSNR-Y loss is measured against the exhaustive decision through the full
+    MSE table (CDEF) and the exact LR scores."""
+    import numpy as np
+    from goav1_amd import learned
+    from goav1_amd.lib import ProposalNet, cdef_lambda
+    dc = json.load(open(os.path.join(ROOT, "tests", "golden", "dc_qlookup_8bit.json")))["dc"]
+    meta = json.load(open(os.path.splitext(learned.DEFAULT_WEIGHTS)[0] + ".json"))
+    net = ProposalNet(runner, max_batch=64)
+    try:
+        net.set_params(np.load(learned.DEFAULT_WEIGHTS))
+        frames = learned.held_out(nframes, width, height)
+        res = learned.evaluate(runner, net, frames, lambda q: cdef_lambda(dc[q]),
+                               np.array(meta["static_shortlist"], np.int32))
+    finally:
+        net.close()
+    px = width * height
+    return {"workload": f"{nframes} held-out {width}x{height} 8-bit 4:2:0 frames, q 60..220: CDEF (64 luma x 64 "
+                        "chroma strengths) + curated luma LR search",
+            "exhaustive_mpx_s": round(px / res["ms_exhaustive"] / 1e3, 1),
+            "learned_mpx_s": round(px / res["ms_learned"] / 1e3, 1),
+            "static_shortlist_mpx_s": round(px / res["ms_static"] / 1e3, 1),
+            "ms_exhaustive": round(res["ms_exhaustive"], 4), "ms_learned": round(res["ms_learned"], 4),
+            "speedup": round(res["speedup_learned"], 2),
+            "cdef_psnr_loss_db": round(res["cdef_psnr_loss_learned"], 5),
+            "lr_psnr_loss_db": round(res["lr_psnr_loss_learned"], 5),
+            "cdef_rate_delta_bits": round(res["cdef_rate_delta_learned"], 2),
+            "static_cdef_psnr_loss_db": round(res["cdef_psnr_loss_static"], 5),
+            "static_lr_psnr_loss_db": round(res["lr_psnr_loss_static"], 5)}
 
 
 def batch_leg(runner, steps, nframes):
@@ -345,6 +382,8 @@ def main():
     }
     if not args.no_search:
         out["search"] = search_leg(runner, args.steps, not args.no_cpu_baseline)
+        if args.learned_frames:
+            out["learned_search"] = learned_leg(runner, args.learned_frames)
     if args.batch_frames:
         out["batch"] = batch_leg(runner, max(2, args.steps // 2), args.batch_frames)
     if not args.no_cpu_baseline:

@@ -107,6 +107,10 @@ assert C.sizeof(LRUnit) == 16
 assert C.sizeof(Frame) == 32 + 24 + 24
 
 
+# strengths per plane in the learned search's shortlist (AV1PF_SHORTLIST)
+SHORTLIST = 4
+
+
 class CdefDecision(C.Structure):
     """av1pf_cdef_decision: the device strength selection's result."""
     _fields_ = [("cdef_bits", C.c_int32), ("y_strength", C.c_uint8 * 8), ("uv_strength", C.c_uint8 * 8),

@@ -945,6 +945,7 @@ struct av1pf_nn {
 };
 
 int64_t av1pf_nn_param_count(void) { return nn_shape().total; }
+int av1pf_search_shortlist_size(void) { return AV1PF_SHORTLIST; }
 
 void av1pf_nn_destroy(av1pf_nn *nn)
 {

@@ -7,7 +7,7 @@ namespace pf {
 
 constexpr int NN_OUT = 131;        // 64 luma + 64 chroma strength logits, 3 LR-type logits
 constexpr int NN_LAYERS = 5;       // conv1..3, fc1, heads
-constexpr int NCAND = 8;           // shortlisted strengths per plane in the learned search
+constexpr int NCAND = AV1PF_SHORTLIST;   // shortlisted strengths per plane in the learned search
 
 // Parameter layout of the flat FP32 vector: per layer W [K][N] row-major then b [N].
 struct NNShape {

@@ -762,22 +762,19 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
 // Greedy selection over the NCAND x NCAND shortlist pairs.  The four levels
 // (cdef_bits 0..3) are independent greedy runs, so workgroup b of 4 runs
 // level b, all of its steps in this one launch.  Thread t keeps the active
-// blocks t, t + 512, ... (NPER of them) with their 16 shortlist SSEs in
+// blocks t, t + 512, ... (NPER of them) with their 2 * NCAND shortlist SSEs in
 // registers, exact as int32: an 8-bit 64x64 luma block's SSE is at most
 // 4096 * 255^2 and its two 32x32 chroma blocks' 2048 * 255^2, so a pair's
 // cost stays below 2^29.  A step computes each block's best cost among the
-// chosen pairs and the gains max(0, best - y_j - uv_k) of all 64 pairs, in
-// one of two ways:
-//  - pair lanes (up to 512 blocks): lane p = 8 j + k also holds y_j + uv_k of
-//    the 64 blocks its wave owns; the blocks' best costs go through LDS (a
-//    broadcast read) and the lane sums its own pair's gain;
-//  - butterfly (more blocks): each thread computes its blocks' gains for all
-//    pairs and an exchange over the wave that halves the values a lane holds
-//    at each stage leaves lane l with the wave total of pair bitrev6(l).
-// Wave 0 adds the waves' totals and takes the first maximum.  Each level's cost and pairs go
-// to `lv`; the last workgroup to finish (ticket) keeps the cheapest level
-// (the lowest on ties, as a sequential scan with a strict comparison would)
-// and writes the decision.
+// chosen pairs; the gains max(0, best - y_j - uv_k) of the NP = NCAND^2 pairs
+// are summed by pair lanes: lane l of a wave serves pair l % NP for the
+// (64 / NP)-th part l / NP of the wave's 64 blocks and holds y_j + uv_k of
+// those blocks, the blocks' best costs come through LDS (broadcast reads),
+// and the parts are added across lanes.  Wave 0 adds the waves' totals and
+// takes the first maximum as one 64-bit key (gain, 63 - pair).  Each level's
+// cost and pairs go to `lv`; the last workgroup to finish (ticket) keeps the
+// cheapest level (the lowest on ties, as a sequential scan with a strict
+// comparison would) and writes the decision.
 struct SmallLevel {
     unsigned long long cost;
     int ys[8], us[8];
@@ -790,8 +787,8 @@ static_assert(sizeof(SmallState) <= 1024, "small selection state");
 #define SMALL_MAX_FB 2048
 #define SMALL_T 512
 
-// Lane exchanges: DPP quad permutations for distances 1 and 2, swizzles
-// within 32 lanes for 4..16, a permute for 32.
+// Lane exchanges (lane l reads lane l ^ X): DPP quad permutations for
+// distances 1 and 2, swizzles within 32 lanes for 4..16, a permute for 32.
 template <int X>
 __device__ __forceinline__ uint32_t xchg32(uint32_t v)
 {
@@ -807,21 +804,8 @@ __device__ __forceinline__ uint64_t xchg64(uint64_t v)
 {
     return (uint64_t)xchg32<X>((uint32_t)v) | ((uint64_t)xchg32<X>((uint32_t)(v >> 32)) << 32);
 }
-// One butterfly stage: 2N values -> N, keeping the half selected by lane bit
-// log2(X) and adding the partner's copy of that half.
-template <int X, int N, typename T>
-__device__ __forceinline__ void bfly(const T *in, T *out, int lane)
-{
-    const bool hi = lane & X;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const T keep = hi ? in[N + i] : in[i], send = hi ? in[i] : in[N + i];
-        if constexpr (sizeof(T) == 4) out[i] = keep + xchg32<X>(send);
-        else out[i] = keep + xchg64<X>(send);
-    }
-}
-// A block's 8 shortlist SSEs as one register vector: indexing it with a
-// wave-uniform (scalar) index stays in registers.
+// A block's NCAND shortlist SSEs of one plane as one register vector:
+// indexing it with a wave-uniform (scalar) index stays in registers.
 typedef int cand8 __attribute__((ext_vector_type(NCAND)));
 
 // Largest 64-bit value over the wave (every lane ends with it).
@@ -835,7 +819,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return max(v, xchg64<32>(v));
 }
 
-template <int NPER, bool LANE_PAIRS>
+template <int NPER>
 __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
                                                             int nvfb, int mi_cols, int mi_rows,
                                                             unsigned long long lambda, const int *cand,
@@ -843,10 +827,12 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
                                                             uint8_t *fb_index)
 {
     constexpr int NW = SMALL_T / 64;
+    constexpr int NP = NCAND * NCAND, BPL = NP;   // pairs; blocks per lane = 64 / (64 / NP) lanes per pair
+    static_assert(64 % NP == 0 && BPL % 8 == 0, "pair lanes need NCAND^2 to divide 64 and 8 | NCAND^2");
     __shared__ uint16_t act[SMALL_MAX_FB];
     __shared__ uint8_t flag[SMALL_MAX_FB];
     __shared__ unsigned long long wpart[NW][65];   // per-wave pair gains (or column sums) and base sum
-    __shared__ __align__(16) uint32_t bw[LANE_PAIRS ? NPER * SMALL_T : 1];   // each block's best cost, for the pair lanes
+    __shared__ __align__(16) uint32_t bw[NPER * SMALL_T];   // each block's best cost, for the pair lanes
     __shared__ int pick_s, nact_s, bits_s;
     __shared__ unsigned sel_s[2];
     __shared__ bool last;
@@ -879,20 +865,20 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
             U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
         }
     }
-    // pair lanes: lane p = 8 j + k of wave w holds y_j + uv_k of the wave's
-    // blocks w * 64 + q (+ 512 r), so its gain needs only the blocks' best costs
-    uint32_t YU[LANE_PAIRS ? NPER : 1][LANE_PAIRS ? 64 : 1];
-    if constexpr (LANE_PAIRS) {
+    // pair lanes: lane l of wave w serves pair pr = 8 j + k (j, k < NCAND) for
+    // the blocks w * 64 + part * BPL + m (+ 512 r), m < BPL, and holds their
+    // y_j + uv_k
+    const int pr = lane % NP, part = lane / NP;
+    uint32_t YU[NPER][BPL];
 #pragma unroll
-        for (int r = 0; r < NPER; r++)
+    for (int r = 0; r < NPER; r++)
 #pragma unroll
-            for (int q = 0; q < 64; q++) {
-                const int i = r * SMALL_T + wv * 64 + q;
-                const size_t fb = i < nact ? act[i] : 0;
-                YU[r][q] = i < nact ? (uint32_t)(mse[fb * NCAND + (lane >> 3)] + mse[((size_t)nfb + fb) * NCAND + (lane & 7)])
-                                    : 0u;
-            }
-    }
+        for (int m = 0; m < BPL; m++) {
+            const int i = r * SMALL_T + wv * 64 + part * BPL + m;
+            const size_t fb = i < nact ? act[i] : 0;
+            YU[r][m] = i < nact ? (uint32_t)(mse[fb * NCAND + pr / NCAND] + mse[((size_t)nfb + fb) * NCAND + pr % NCAND])
+                                : 0u;
+        }
     const int bits = blockIdx.x;
     const int nb = 1 << bits;
     // the chosen pairs, 4 bits per slot, kept by every thread (wave-uniform)
@@ -930,73 +916,47 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
             unsigned long long bsum = 0;
 #pragma unroll
             for (int r = 0; r < NPER; r++) {
-                b[r] = (unsigned)(Y[r][sy & 7] + U[r][su & 7]);
+                b[r] = (unsigned)(Y[r][sy & 15] + U[r][su & 15]);
                 for (int g = 1; g < slot; g++)
-                    b[r] = min(b[r], (unsigned)(Y[r][(sy >> (4 * g)) & 7] + U[r][(su >> (4 * g)) & 7]));
+                    b[r] = min(b[r], (unsigned)(Y[r][(sy >> (4 * g)) & 15] + U[r][(su >> (4 * g)) & 15]));
                 bsum += b[r];
             }
-            unsigned long long gain;
-            if constexpr (LANE_PAIRS) {
 #pragma unroll
-                for (int r = 0; r < NPER; r++) bw[r * SMALL_T + t] = b[r];
-                __builtin_amdgcn_wave_barrier();   // the wave reads back only its own blocks' costs
-                gain = 0;
+            for (int r = 0; r < NPER; r++) bw[r * SMALL_T + t] = b[r];
+            __builtin_amdgcn_wave_barrier();   // the wave reads back only its own blocks' costs
+            unsigned long long gain = 0;
 #pragma unroll
-                for (int r = 0; r < NPER; r++)
+            for (int r = 0; r < NPER; r++)
 #pragma unroll
-                    for (int q = 0; q < 64; q += 8) {   // 8 gains of at most 2^29 fit 32 bits
-                        const uint4 b0 = *reinterpret_cast<const uint4 *>(&bw[r * SMALL_T + wv * 64 + q]);
-                        const uint4 b1 = *reinterpret_cast<const uint4 *>(&bw[r * SMALL_T + wv * 64 + q + 4]);
-                        uint32_t g = __builtin_elementwise_sub_sat(b0.x, YU[r][q]) +
-                                     __builtin_elementwise_sub_sat(b0.y, YU[r][q + 1]) +
-                                     __builtin_elementwise_sub_sat(b0.z, YU[r][q + 2]) +
-                                     __builtin_elementwise_sub_sat(b0.w, YU[r][q + 3]);
-                        g += __builtin_elementwise_sub_sat(b1.x, YU[r][q + 4]) +
-                             __builtin_elementwise_sub_sat(b1.y, YU[r][q + 5]) +
-                             __builtin_elementwise_sub_sat(b1.z, YU[r][q + 6]) +
-                             __builtin_elementwise_sub_sat(b1.w, YU[r][q + 7]);
-                        gain += g;
-                    }
-            } else {
-                // per-thread gains max(0, b - y_j - uv_k) by saturating
-                // subtractions: at most 4 * 2^29 < 2^32, and two lanes' sums fit too
-                uint32_t g64[64];
-#pragma unroll
-                for (int p = 0; p < 64; p++) g64[p] = 0;
-#pragma unroll
-                for (int r = 0; r < NPER; r++)
-#pragma unroll
-                    for (int j = 0; j < NCAND; j++) {
-                        const unsigned rj = __builtin_elementwise_sub_sat(b[r], (unsigned)Y[r][j]);
-#pragma unroll
-                        for (int k = 0; k < NCAND; k++)
-                            g64[j * NCAND + k] += __builtin_elementwise_sub_sat(rj, (unsigned)U[r][k]);
-                    }
-                uint32_t g32[32];
-                bfly<1, 32>(g64, g32, lane);
-                unsigned long long h32[32], h16[16], h8[8], h4[4], h2[2], h1[1];
-#pragma unroll
-                for (int i = 0; i < 32; i++) h32[i] = g32[i];
-                bfly<2, 16>(h32, h16, lane);
-                bfly<4, 8>(h16, h8, lane);
-                bfly<8, 4>(h8, h4, lane);
-                bfly<16, 2>(h4, h2, lane);
-                bfly<32, 1>(h2, h1, lane);
-                gain = h1[0];
-            }
+                for (int m = 0; m < BPL; m += 8) {   // 8 gains below 2^29 each fit 32 bits
+                    const uint32_t *bp = &bw[r * SMALL_T + wv * 64 + part * BPL + m];
+                    const uint4 b0 = *reinterpret_cast<const uint4 *>(bp);
+                    const uint4 b1 = *reinterpret_cast<const uint4 *>(bp + 4);
+                    uint32_t g = __builtin_elementwise_sub_sat(b0.x, YU[r][m]) +
+                                 __builtin_elementwise_sub_sat(b0.y, YU[r][m + 1]) +
+                                 __builtin_elementwise_sub_sat(b0.z, YU[r][m + 2]) +
+                                 __builtin_elementwise_sub_sat(b0.w, YU[r][m + 3]);
+                    g += __builtin_elementwise_sub_sat(b1.x, YU[r][m + 4]) +
+                         __builtin_elementwise_sub_sat(b1.y, YU[r][m + 5]) +
+                         __builtin_elementwise_sub_sat(b1.z, YU[r][m + 6]) +
+                         __builtin_elementwise_sub_sat(b1.w, YU[r][m + 7]);
+                    gain += g;
+                }
+            // add the parts: the lanes of a pair are NP apart
+            if constexpr (NP <= 4) gain += xchg64<4>(gain);
+            if constexpr (NP <= 8) gain += xchg64<8>(gain);
+            if constexpr (NP <= 16) gain += xchg64<16>(gain);
+            if constexpr (NP <= 32) gain += xchg64<32>(gain);
             if (final_step) {   // the frame distortion is needed at the last step only
                 for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
                 if (lane == 0) wpart[wv][64] = bsum;
             }
             wpart[wv][lane] = gain;
             __syncthreads();
-            if (wv == 0) {   // first maximum over the 64 pairs: the largest (gain, 63 - pair) key
+            if (wv == 0) {   // first maximum over the pairs: the largest (gain, 63 - pair) key
                 unsigned long long v = 0;
                 for (int w = 0; w < NW; w++) v += wpart[w][lane];
-                const int p = LANE_PAIRS ? lane
-                                         : ((lane & 1) << 5) | ((lane & 2) << 3) | ((lane & 4) << 1) |
-                                               ((lane & 8) >> 1) | ((lane & 16) >> 3) | ((lane & 32) >> 5);
-                const unsigned long long key = wave_max_u64((v << 6) | (unsigned long long)(63 - p));
+                const unsigned long long key = wave_max_u64((v << 6) | (unsigned long long)(63 - pr));
                 if (lane == 0) {
                     pick_s = 63 - (int)(key & 63);
                     if (final_step) {
@@ -1062,9 +1022,9 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     for (int r = 0; r < NPER; r++) {
         const int i = t + r * SMALL_T;
         if (i >= nact) continue;
-        int bm = Y[r][fy & 7] + U[r][fu & 7], bg = 0;
+        int bm = Y[r][fy & 15] + U[r][fu & 15], bg = 0;
         for (int g = 1; g < n; g++) {
-            const int c = Y[r][(fy >> (4 * g)) & 7] + U[r][(fu >> (4 * g)) & 7];
+            const int c = Y[r][(fy >> (4 * g)) & 15] + U[r][(fu >> (4 * g)) & 15];
             if (c < bm) { bm = c; bg = g; }
         }
         fb_index[act[i]] = (uint8_t)bg;
@@ -1080,11 +1040,7 @@ hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, c
     hipLaunchKernelGGL(cdef_search_subset_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, cand,
                        mse_sub, nhfb, nvfb);
     const int nfb = nhfb * nvfb;
-    // up to 512 blocks: pair lanes; beyond, the pair costs no longer fit the
-    // registers and the gains are summed by the butterfly
-    auto fin = nfb <= SMALL_T       ? fin_small_kernel<1, true>
-               : nfb <= 2 * SMALL_T ? fin_small_kernel<2, false>
-                                    : fin_small_kernel<4, false>;
+    auto fin = nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2> : fin_small_kernel<4>;
     hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
                        (unsigned long long)lambda, cand, static_cast<SmallState *>(small_state), dec, fb_index);
     return hipGetLastError();

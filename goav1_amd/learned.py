@@ -1,15 +1,17 @@
 """Learned proposal for the encoder CDEF/LR search: data, training, evaluation.
 
-The network (csrc/nn.hip) reads 64x64 luma (source, reconstruction) tiles and
-predicts, per filter block, a distribution over the 64 luma strengths, one
-over the 64 chroma strengths and the loop-restoration unit type.  Labels come
-from the exhaustive device search itself:
+The network (csrc/nn.hip) reads 64x64 luma tiles (the reconstruction and the
+amplified coding error) and predicts, per filter block, a distribution over
+the 64 luma strengths, one over the 64 chroma strengths and the
+loop-restoration unit type.  Labels come from the exhaustive device search
+itself:
   - strengths: a soft minimum of the block's exhaustive SSE row (probability
-    proportional to exp(-(sse - min) / T), T = max(1, 0.3 % of the minimum));
+    proportional to exp(-(sse - min) / T), T = max(1, temp * minimum));
   - LR type: the curated search's decision for the co-located 64x64 unit.
 Training runs the library's forward/backward/Adam kernels.  The learned
 search (av1pf_search_learned) turns the predictions into a frame shortlist of
-8 + 8 strengths evaluated exactly, and skips LR units predicted NONE.
+K + K strengths (K = AV1PF_SHORTLIST) evaluated exactly, and skips LR units
+predicted NONE.
 
     python -m goav1_amd.learned train --frames 300 --out goav1_amd/models/proposal_net.npy
 
@@ -25,12 +27,13 @@ import time
 import numpy as np
 
 from . import abi, synth
+from .abi import SHORTLIST as K
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "proposal_net.npy")
 LEVEL1 = [4 * g if g < 16 else 4 * ((g - 16) // 3) + (g - 16) % 3 + 1 for g in range(64)]
 LEVEL1_INV = {p: g for g, p in enumerate(LEVEL1)}
-TEMP = 0.003   # label temperature relative to the block's smallest SSE
+TEMP = 0.0001   # label temperature relative to the block's smallest SSE
 
 
 LAYERS = [(18, 16), (144, 32), (288, 64), (64, 128), (128, 131)]   # (K, N) per layer, as the library lays them out
@@ -87,12 +90,12 @@ class Device:
         _, _, nhfb, nvfb, hz, vt = geometry(rec.w, rec.h)
         self.nfb, self.nhfb, self.nvfb, self.hz, self.vt = nhfb * nvfb, nhfb, nvfb, hz, vt
         self.mse = torch.zeros((2, self.nfb, 64), dtype=torch.int64, device="cuda")
-        self.msub = torch.zeros((2, self.nfb, 8), dtype=torch.int64, device="cuda")
+        self.msub = torch.zeros((2, self.nfb, K), dtype=torch.int64, device="cuda")
         self.dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
         self.idx = torch.zeros(self.nfb, dtype=torch.uint8, device="cuda")
         self.lru = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
         self.lse = torch.zeros((hz * vt, 12), dtype=torch.int64, device="cuda")
-        self.cand = torch.zeros(16, dtype=torch.int32, device="cuda")
+        self.cand = torch.zeros(2 * K, dtype=torch.int32, device="cuda")
         self.conf = torch.zeros(1, dtype=torch.float32, device="cuda")
 
 
@@ -134,7 +137,7 @@ def run_shortlist(runner, d, damping, lam, cand_dev):
     cur.wait_stream(side)
 
 
-def targets_for(mse, lr_units, w, h, act):
+def targets_for(mse, lr_units, w, h, act, temp=TEMP):
     """Training targets [n_active][131] of one frame."""
     _, _, nhfb, nvfb, hz, vt = geometry(w, h)
     fbs = np.nonzero(act)[0]
@@ -142,7 +145,7 @@ def targets_for(mse, lr_units, w, h, act):
     for p in range(2):
         m = mse[p][fbs].astype(np.float64)
         mn = m.min(axis=1, keepdims=True)
-        z = -(m - mn) / np.maximum(1.0, TEMP * mn)
+        z = -(m - mn) / np.maximum(1.0, temp * mn)
         e = np.exp(z - z.max(axis=1, keepdims=True))
         T[:, 64 * p:64 * p + 64] = e / e.sum(axis=1, keepdims=True)
     for i, fb in enumerate(fbs):
@@ -152,7 +155,7 @@ def targets_for(mse, lr_units, w, h, act):
     return fbs, T
 
 
-def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None):
+def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None, temp=TEMP):
     """Tiles and targets of n_frames frames from the exhaustive device search."""
     import torch
     tiles, tg, counts = [], [], np.zeros((2, 64), np.int64)
@@ -165,7 +168,7 @@ def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None):
         runner.sync()
         torch.cuda.synchronize()
         act = active_blocks(w, h, noskip)
-        fbs, T = targets_for(d.mse.cpu().numpy().view(np.uint64), d.lru.cpu().numpy(), w, h, act)
+        fbs, T = targets_for(d.mse.cpu().numpy().view(np.uint64), d.lru.cpu().numpy(), w, h, act, temp)
         idx = torch.from_numpy(fbs.astype(np.int64)).cuda()
         tiles.append(t.index_select(0, idx))
         tg.append(torch.from_numpy(T).cuda())
@@ -177,9 +180,9 @@ def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None):
 
 
 def static_shortlist(counts):
-    """The baseline without the network: the 8 strengths per plane that are
+    """The baseline without the network: the K strengths per plane that are
     most often the best in the training labels (ascending)."""
-    return np.concatenate([np.sort(np.argsort(-counts[p], kind="stable")[:8]) for p in range(2)]).astype(np.int32)
+    return np.concatenate([np.sort(np.argsort(-counts[p], kind="stable")[:K]) for p in range(2)]).astype(np.int32)
 
 
 def train(runner, net, tiles, targets, epochs=6, batch=256, lr=2e-3, seed=0, log=None):
@@ -238,7 +241,12 @@ def lr_luma_sse(lse, units, base_sse):
     return s
 
 
-def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=None):
+def topk_gain(lm, order, ks=(1, 2, 4, 8)):
+    """Per-block luma CDEF gain kept by the first k strengths of `order`."""
+    return {k: float((lm[:, 0] - lm[:, np.asarray(order[:k])].min(axis=1)).sum()) for k in ks}
+
+
+def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=None, static_rank=None):
     """PSNR-Y loss, CDEF rate and time of the learned and static-shortlist
     searches against the exhaustive search on held-out frames."""
     import torch
@@ -272,11 +280,36 @@ def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=
         ex_dec, ex_idx, ex_lru = d.dec.cpu().numpy().copy(), d.idx.cpu().numpy().copy(), d.lru.cpu().numpy().copy()
         t_le += timed(lambda: run_learned(runner, net, d, damping, lam, lr_thr))
         le_dec, le_idx, le_lru = d.dec.cpu().numpy().copy(), d.idx.cpu().numpy().copy(), d.lru.cpu().numpy().copy()
+        le_cand = d.cand.cpu().numpy().copy()
         conf = d.conf.item()
         t_st += timed(lambda: run_shortlist(runner, d, damping, lam, cand_static))
         st_dec, st_idx, st_lru = d.dec.cpu().numpy().copy(), d.idx.cpu().numpy().copy(), d.lru.cpu().numpy().copy()
         row = {"q": q, "conf": conf}
         nact = int(act.sum())
+        # how often each luma shortlist holds a block's exhaustive best strength;
+        # "frame_counts" lists this frame's K most frequent best strengths
+        best = full[0][act].argmin(axis=1)
+        counts = np.bincount(best, minlength=64)
+        row["hits"] = {"learned": int(np.isin(best, le_cand[:K]).sum()),
+                       "static": int(np.isin(best, static_cand[:K]).sum()),
+                       "frame_counts": int(np.sort(counts)[-K:].sum()), "blocks": int(best.size)}
+        # the per-block CDEF luma gain (unfiltered SSE - best SSE) a shortlist
+        # keeps when every block may take its best listed strength
+        lm = full[0][act].astype(np.float64)
+        room = (lm[:, 0] - lm.min(axis=1)).sum()
+        keep = lambda c: (lm[:, 0] - lm[:, np.asarray(c[:K])].min(axis=1)).sum()
+        row["gain"] = {"room": room, "learned": keep(le_cand), "static": keep(static_cand),
+                       "frame_counts": keep(np.argsort(-counts, kind="stable")[:K])}
+        # rankings for shorter lists: the network's summed luma probabilities
+        # over the active blocks, the training-label counts, this frame's counts
+        z = net.infer(d.ds, d.dr).cpu().numpy()[act][:, :64].astype(np.float64)
+        pz = np.exp(z - z.max(axis=1, keepdims=True))
+        pz /= pz.sum(axis=1, keepdims=True)
+        row["topk"] = {"learned": topk_gain(lm, np.argsort(-pz.sum(axis=0), kind="stable")),
+                       "frame_counts": topk_gain(lm, np.argsort(-counts, kind="stable"))}
+        if static_rank is not None:
+            row["topk"]["static"] = topk_gain(lm, static_rank)
+        row["unfiltered_psnr"] = psnr(base, npx)
         for name, dec, idx, lru in (("exhaustive", ex_dec, ex_idx, ex_lru), ("learned", le_dec, le_idx, le_lru),
                                     ("static", st_dec, st_idx, st_lru)):
             sse, nb = cdef_luma_sse(full, dec, idx, act, base)
@@ -290,6 +323,15 @@ def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=
     n = len(frames)
     out = {"frames": n, "ms_exhaustive": t_ex / n, "ms_learned": t_le / n, "ms_static": t_st / n,
            "speedup_learned": t_ex / t_le, "rows": rows}
+    tot = lambda k: sum(r["hits"][k] for r in rows)
+    out["luma_best_in_shortlist"] = {k: round(tot(k) / max(tot("blocks"), 1), 4)
+                                     for k in ("learned", "static", "frame_counts")}
+    room = sum(r["gain"]["room"] for r in rows)
+    out["luma_block_gain_kept"] = {k: round(sum(r["gain"][k] for r in rows) / max(room, 1.0), 4)
+                                   for k in ("learned", "static", "frame_counts")}
+    out["topk_gain_kept"] = {name: {k: round(sum(r["topk"][name][k] for r in rows) / max(room, 1.0), 4)
+                                    for k in (1, 2, 4, 8)} for name in rows[0]["topk"]}
+    out["cdef_gain_db_exhaustive"] = float(np.mean([r["exhaustive"]["cdef_psnr"] - r["unfiltered_psnr"] for r in rows]))
     for name in ("learned", "static"):
         out[f"cdef_psnr_loss_{name}"] = float(np.mean([r["exhaustive"]["cdef_psnr"] - r[name]["cdef_psnr"] for r in rows]))
         out[f"lr_psnr_loss_{name}"] = float(np.mean([r["exhaustive"]["lr_psnr"] - r[name]["lr_psnr"] for r in rows]))
@@ -322,9 +364,11 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--temp", type=float, default=TEMP, help="label temperature (relative to the block's best SSE)")
+    ap.add_argument("--lr", type=float, default=3e-3)
     args = ap.parse_args(argv)
     from .lib import PostFilterRunner, ProposalNet
 
@@ -339,8 +383,11 @@ def main(argv=None):
         meta = json.load(open(os.path.splitext(args.weights)[0] + ".json"))
         net.set_params(np.load(args.weights))
         frames = held_out(args.eval_frames, args.eval_width, args.eval_height)
+        rank = None
+        if "label_counts" in meta:
+            rank = np.argsort(-np.asarray(meta["label_counts"][0]), kind="stable")
         res = evaluate(runner, net, frames, lambda q: cdef_lambda(dc[q] if dc else 223),
-                       np.array(meta["static_shortlist"], np.int32), log=log)
+                       np.array(meta["static_shortlist"], np.int32), log=log, static_rank=rank)
         log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
         net.close()
         runner.close()
@@ -348,19 +395,20 @@ def main(argv=None):
     net.set_params(init_params(np.random.default_rng(args.seed)))
     t0 = time.time()
     tiles, targets, counts = build_dataset(runner, net, args.frames, seed=args.seed, w=args.width, h=args.height,
-                                           log=log)
+                                           log=log, temp=args.temp)
     ent = float(-(targets * torch_log(targets)).sum(dim=1).mean())
     log(f"dataset {tiles.shape[0]} tiles in {time.time() - t0:.1f} s; target entropy {ent:.4f} (the loss floor)")
     t0 = time.time()
-    hist = train(runner, net, tiles, targets, epochs=args.epochs, log=log)
+    hist = train(runner, net, tiles, targets, epochs=args.epochs, lr=args.lr, log=log)
     log(f"trained in {time.time() - t0:.1f} s")
     P = net.get_params()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, P)
     meta = {"frames": args.frames, "size": [args.width, args.height], "epochs": args.epochs, "loss": hist,
-            "static_shortlist": static_shortlist(counts).tolist(), "seed": args.seed}
+            "temp": args.temp, "lr": args.lr,
+            "static_shortlist": static_shortlist(counts).tolist(), "label_counts": counts.tolist(), "seed": args.seed}
     json.dump(meta, open(os.path.splitext(args.out)[0] + ".json", "w"))
-    log(json.dumps({k: v for k, v in meta.items() if k != "loss"}))
+    log(json.dumps({k: v for k, v in meta.items() if k not in ("loss", "label_counts")}))
     net.close()
     runner.close()
 

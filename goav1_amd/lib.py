@@ -26,7 +26,7 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_nn_param_count", "av1pf_nn_create", "av1pf_nn_destroy", "av1pf_nn_set_params",
            "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_infer", "av1pf_nn_backward",
            "av1pf_nn_adam",
-           "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned",
+           "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
            "av1pf_profile", "av1pf_profile_read"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned"] + \

@@ -297,18 +297,22 @@ int  av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_deca
 
 /* ---- learned-proposal search ------------------------------------------------
  * No counterpart in the reference.  The CDEF search restricted to a frame
- * shortlist of 8 luma and 8 chroma strengths (cand[16], search-level-1
- * positions 0..63, ascending within each half): their exact SSE for every
- * filter block (mse_sub[2][nfb][8], same rules as av1pf_cdef_search), then
- * the same greedy strength selection over the 64 shortlist pairs, in one
- * launch.  8-bit 4:2:0 frames up to 2048 filter blocks (4K).  Device
- * pointers, asynchronous.                                                   */
+ * shortlist of AV1PF_SHORTLIST luma and AV1PF_SHORTLIST chroma strengths
+ * (cand[2 * AV1PF_SHORTLIST], search-level-1 positions 0..63, ascending
+ * within each half): their exact SSE for every filter block
+ * (mse_sub[2][nfb][AV1PF_SHORTLIST], same rules as av1pf_cdef_search), then
+ * the same greedy strength selection over the AV1PF_SHORTLIST^2 shortlist
+ * pairs, in one launch.  8-bit 4:2:0 frames up to 2048 filter blocks (4K).
+ * Device pointers, asynchronous.                                            */
+#define AV1PF_SHORTLIST 4
+int  av1pf_search_shortlist_size(void);   /* AV1PF_SHORTLIST of the built library */
 int  av1pf_cdef_search_shortlist(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
                                  const uint64_t *noskip, int damping, uint64_t lambda, const int32_t *cand,
                                  uint64_t *mse_sub, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream);
-/* Whole learned search of one frame: tiles -> network -> shortlist (the 8
- * strengths per plane with the largest summed probability over the active
- * blocks; cand[16] and the mean top luma probability conf[1] are written)
+/* Whole learned search of one frame: tiles -> network -> shortlist (the
+ * AV1PF_SHORTLIST strengths per plane with the largest summed probability
+ * over the active blocks; cand[2 * AV1PF_SHORTLIST] and the mean top luma
+ * probability conf[1] are written)
  * -> av1pf_cdef_search_shortlist -> the curated luma LR search (unit size
  * 64) on the units whose predicted P(NONE) is at most lr_none_threshold,
  * the rest NONE.  A caller wanting the exhaustive search as a fallback

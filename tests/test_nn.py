@@ -21,8 +21,10 @@ def soft_targets(rng, n):
 
 
 def test_param_count_matches_library():
+    from goav1_amd import abi
     from goav1_amd.lib import load
     assert int(load().av1pf_nn_param_count()) == nn_ref.param_count() == 48659
+    assert int(load().av1pf_search_shortlist_size()) == abi.SHORTLIST
 
 
 def test_reference_gradients_match_finite_differences():

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from goav1_amd import abi, synth
+from goav1_amd.abi import SHORTLIST as K
 from oracle import orc
 
 # search-level-1 strength list: position -> packed (primary * 4 + secondary code)
@@ -217,7 +218,7 @@ def test_gpu_shortlist_equals_masked_host_finish(case):
     w, h, q, seed = case
     src, rec, noskip, damping = synth.search_pair(seed=60 + seed, w=w, h=h, q=q)
     rng = np.random.default_rng(seed)
-    cand = np.concatenate([np.sort(np.r_[0, rng.choice(np.arange(1, 64), 7, replace=False)]) for _ in range(2)])
+    cand = np.concatenate([np.sort(np.r_[0, rng.choice(np.arange(1, 64), K - 1, replace=False)]) for _ in range(2)])
     print(f"shortlist {w}x{h} q{q} cand {cand.tolist()}", flush=True)
     r = PostFilterRunner()
     try:
@@ -225,7 +226,7 @@ def test_gpu_shortlist_equals_masked_host_finish(case):
         ns = torch.from_numpy(noskip.view(np.int64)).cuda()
         nfb = noskip.shape[0]
         cd = torch.from_numpy(cand.astype(np.int32)).cuda()
-        msub = torch.zeros((2, nfb, 8), dtype=torch.int64, device="cuda")
+        msub = torch.zeros((2, nfb, K), dtype=torch.int64, device="cuda")
         dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
         idx = torch.zeros(nfb, dtype=torch.uint8, device="cuda")
         lam = cdef_lambda(synth.search_dc_quant(160))
@@ -235,11 +236,11 @@ def test_gpu_shortlist_equals_masked_host_finish(case):
         torch.cuda.synchronize()
         full = orc.cdef_search_mse(src, rec, noskip, damping)
         sub = msub.cpu().numpy().view(np.uint64)
-        np.testing.assert_array_equal(sub[0], full[0][:, cand[:8]])
-        np.testing.assert_array_equal(sub[1], full[1][:, cand[8:]])
+        np.testing.assert_array_equal(sub[0], full[0][:, cand[:K]])
+        np.testing.assert_array_equal(sub[1], full[1][:, cand[K:]])
         masked = np.full_like(full, np.uint64(1 << 40))
-        masked[0][:, cand[:8]] = full[0][:, cand[:8]]
-        masked[1][:, cand[8:]] = full[1][:, cand[8:]]
+        masked[0][:, cand[:K]] = full[0][:, cand[:K]]
+        masked[1][:, cand[K:]] = full[1][:, cand[K:]]
         skip = 1 - active_blocks(src, noskip)
         masked[:, skip.astype(bool), :] = 0
         bits, ys, uvs, fbi = cdef_search_finish(masked, skip, lam)
@@ -272,7 +273,7 @@ def test_gpu_learned_search_is_consistent():
         lam = cdef_lambda(synth.search_dc_quant(160))
         dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
         idx = torch.zeros(nfb, dtype=torch.uint8, device="cuda")
-        cand = torch.zeros(16, dtype=torch.int32, device="cuda")
+        cand = torch.zeros(2 * K, dtype=torch.int32, device="cuda")
         conf = torch.zeros(1, dtype=torch.float32, device="cuda")
         hz, vt = (640 + 32) // 64, (360 + 32) // 64
         lru = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
@@ -282,11 +283,11 @@ def test_gpu_learned_search_is_consistent():
         r.sync()
         torch.cuda.synchronize()
         c = cand.cpu().numpy()
-        assert all(np.diff(c[:8]) > 0) and all(np.diff(c[8:]) > 0) and c.min() >= 0 and c.max() < 64
+        assert all(np.diff(c[:K]) > 0) and all(np.diff(c[K:]) > 0) and c.min() >= 0 and c.max() < 64
         assert 0 < conf.item() <= 1
         d1 = dec.cpu().numpy().copy()
         i1 = idx.cpu().numpy().copy()
-        msub = torch.zeros((2, nfb, 8), dtype=torch.int64, device="cuda")
+        msub = torch.zeros((2, nfb, K), dtype=torch.int64, device="cuda")
         r.cdef_search_shortlist(ds, dr, ns.data_ptr(), damping, lam, cand.data_ptr(), msub.data_ptr(), dec.data_ptr(),
                                 idx.data_ptr())
         r.sync()
@@ -294,8 +295,8 @@ def test_gpu_learned_search_is_consistent():
         np.testing.assert_array_equal(dec.cpu().numpy(), d1)
         np.testing.assert_array_equal(idx.cpu().numpy(), i1)
         nb = 1 << int(d1[:4].view(np.int32)[0])
-        allowed_y = {LEVEL1[g] for g in c[:8]}
-        allowed_uv = {LEVEL1[g] for g in c[8:]}
+        allowed_y = {LEVEL1[g] for g in c[:K]}
+        allowed_uv = {LEVEL1[g] for g in c[K:]}
         assert set(d1[4:4 + nb].tolist()) <= allowed_y and set(d1[12:12 + nb].tolist()) <= allowed_uv
         assert set(lru.cpu().numpy()[:, 0].tolist()) <= {0, 1, 2}
     finally:
