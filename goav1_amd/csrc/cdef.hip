@@ -11,18 +11,23 @@
 //   VeryLarge-masked clip (cdef/filter.go:79-188, filter_u8.go:124-134).
 //
 // Layout: the unit plus its 2-row / 8-column halo is staged once per plane in
-// LDS as uint16 (so the VeryLarge sentinel fits).  The direction search runs one
-// lane per 8x8 block with the 8x15 partial sums in registers; the filter runs
-// 16 (luma) or 4..16 (chroma) pixels per lane with all thirteen taps read from
-// LDS.  The kernel is out of place (src -> dst): every sample of the plane's
-// MI-aligned extent is written, filtered or copied, so neighbours always read
-// pre-CDEF samples exactly as the reference's snapshot does.
+// LDS as uint16 (so the VeryLarge sentinel fits; pitch 82 so that the rows of
+// different 8x8 blocks fall in different banks).  The direction search runs on
+// all four waves: wave w evaluates directions w and w + 4 for the block of
+// each lane, the winner is taken per block from the eight costs in LDS.  The
+// per-block filter parameters (tap offsets, adjusted strength, shifts) are
+// then packed once into a 16-byte LDS record, and the filter walks the plane
+// row by row with one lane per column (coalesced global stores), reading the
+// record of its block and the thirteen taps from LDS.  The kernel is out of
+// place (src -> dst): every sample of the plane's MI-aligned extent is
+// written, filtered or copied, so neighbours always read pre-CDEF samples
+// exactly as the reference's snapshot does.
 #include "pf_internal.h"
 
 namespace pf {
 
 #define CDEF_VL 0x4000
-#define TPITCH 80   // 64 + 2*8
+#define TPITCH 82   // 64 + 2*8, plus 2 to spread the block rows over the banks
 #define TROWS 68    // 64 + 2*2
 
 __device__ __constant__ int kDivTable[9] = { 0, 840, 420, 280, 210, 168, 140, 120, 105 };
@@ -55,74 +60,64 @@ __device__ __forceinline__ int adjust_strength(int strength, int32_t var)
     return (strength * (4 + i) + 8) >> 4;
 }
 
-// 8x8 direction search on an LDS tile (cdef/direction.go:37-58, :432-502).
-__device__ void find_dir(const uint16_t *img, int pitch, int cs, int *dir, int32_t *var)
+// Cost of direction D for one 8x8 block (cdef/direction.go:37-58, :432-502):
+// the block's samples v (already (x >> cs) - 128) summed into the direction's
+// lines, then the squared line sums weighted by 840 / line length.
+template <int D>
+__device__ __forceinline__ uint32_t dir_cost(const int32_t (&v)[64])
 {
-    int32_t pa[8][15];
+    int32_t pa[15];
 #pragma unroll
-    for (int d = 0; d < 8; d++)
+    for (int k = 0; k < 15; k++) pa[k] = 0;
 #pragma unroll
-        for (int k = 0; k < 15; k++) pa[d][k] = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < 8; i++)
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int32_t x = (int32_t)(img[i * pitch + j] >> cs) - 128;
-            pa[0][i + j] += x;
-            pa[1][i + (j >> 1)] += x;
-            pa[2][i] += x;
-            pa[3][3 + i - (j >> 1)] += x;
-            pa[4][7 + i - j] += x;
-            pa[5][3 - (i >> 1) + j] += x;
-            pa[6][j] += x;
-            pa[7][(i >> 1) + j] += x;
+            const int32_t x = v[i * 8 + j];
+            if constexpr (D == 0) pa[i + j] += x;
+            if constexpr (D == 1) pa[i + (j >> 1)] += x;
+            if constexpr (D == 2) pa[i] += x;
+            if constexpr (D == 3) pa[3 + i - (j >> 1)] += x;
+            if constexpr (D == 4) pa[7 + i - j] += x;
+            if constexpr (D == 5) pa[3 - (i >> 1) + j] += x;
+            if constexpr (D == 6) pa[j] += x;
+            if constexpr (D == 7) pa[(i >> 1) + j] += x;
         }
-    }
-    uint32_t c[8];
+    uint32_t s = 0;
+    if constexpr (D == 2 || D == 6) {
 #pragma unroll
-    for (int d = 0; d < 8; d += 2) {
-        // d even: 0,4 diagonal; 2,6 straight
-        uint32_t s = 0;
-        if (d == 2 || d == 6) {
+        for (int k = 0; k < 8; k++) s += (uint32_t)(pa[k] * pa[k]);
+        s *= 105u;
+    } else if constexpr (D == 0 || D == 4) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) s += (uint32_t)(pa[d][k] * pa[d][k]);
-            s *= 105u;
-        } else {
+        for (int k = 0; k < 7; k++)
+            s += (uint32_t)(pa[k] * pa[k] + pa[14 - k] * pa[14 - k]) * (uint32_t)kDivTable[k + 1];
+        s += (uint32_t)(pa[7] * pa[7]) * 105u;
+    } else {
 #pragma unroll
-            for (int k = 0; k < 7; k++)
-                s += (uint32_t)(pa[d][k] * pa[d][k] + pa[d][14 - k] * pa[d][14 - k]) * (uint32_t)kDivTable[k + 1];
-            s += (uint32_t)(pa[d][7] * pa[d][7]) * 105u;
-        }
-        c[d] = s;
-    }
-#pragma unroll
-    for (int d = 1; d < 8; d += 2) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int k = 3; k < 8; k++) s += (uint32_t)(pa[d][k] * pa[d][k]);
+        for (int k = 3; k < 8; k++) s += (uint32_t)(pa[k] * pa[k]);
         s *= 105u;
 #pragma unroll
         for (int k = 0; k < 3; k++)
-            s += (uint32_t)(pa[d][k] * pa[d][k] + pa[d][10 - k] * pa[d][10 - k]) * (uint32_t)kDivTable[2 * k + 2];
-        c[d] = s;
+            s += (uint32_t)(pa[k] * pa[k] + pa[10 - k] * pa[10 - k]) * (uint32_t)kDivTable[2 * k + 2];
     }
-    int best = 0;
-    int32_t bc = (int32_t)c[0];
-#pragma unroll
-    for (int d = 1; d < 8; d++)
-        if ((int32_t)c[d] > bc) { bc = (int32_t)c[d]; best = d; }
-    *dir = best;
-    *var = (int32_t)((uint32_t)bc - c[(best + 4) & 7]) >> 10;
+    return s;
+}
+
+// Every sample (x, y) of a uw x uh region, one lane per column: calls f(x, y).
+template <typename F>
+__device__ __forceinline__ void for_each_px(int uw, int uh, int tid, F &&f)
+{
+    const int tpr = uw > 32 ? 64 : uw > 16 ? 32 : uw > 8 ? 16 : 8;   // lanes per row, a power of two >= uw
+    const int x = tid & (tpr - 1);
+    if (x >= uw) return;
+    for (int y = tid / tpr; y < uh; y += 256 / tpr) f(x, y);
 }
 
 template <typename T>
 __device__ __forceinline__ void copy_unit(const PlaneView &s, const PlaneView &d, int ux, int uy, int uw, int uh, int tid)
 {
-    const int n = uw * uh;
-    for (int i = tid; i < n; i += 256) {
-        const int y = i / uw, x = i - y * uw;
-        row_ptr<T>(d, uy + y)[ux + x] = row_ptr<T>(s, uy + y)[ux + x];
-    }
+    for_each_px(uw, uh, tid, [&](int x, int y) { row_ptr<T>(d, uy + y)[ux + x] = row_ptr<T>(s, uy + y)[ux + x]; });
 }
 
 // Stage the unit input (postfilter_cdef.go:1101-1250) into an LDS tile with
@@ -135,25 +130,49 @@ __device__ __forceinline__ void load_tile(uint16_t *tile, int pitch, const Plane
     const bool row_ext = sy1 < uy + uh + 2 && sy1 > 0 && uy + uh < H;
     const bool col_ext = sx1 < ux + uw + 8 && sx1 > 0 && ux + uw < W;
     const int rows = uh + 4, cols = uw + 16;
-    for (int i = tid; i < rows * cols; i += 256) {
-        const int r = i / cols, c = i - r * cols;
-        const int y = uy - 2 + r, x = ux - 8 + c;
-        const bool vy = y >= 0 && (y < sy1 || row_ext);
+    for (int c = tid & 63; c < cols; c += 64) {
+        const int x = ux - 8 + c;
         const bool vx = x >= 0 && (x < sx1 || col_ext);
-        uint16_t v = CDEF_VL;
-        if (vy && vx) v = row_ptr<T>(s, min(y, sy1 - 1))[min(x, sx1 - 1)];
-        tile[r * pitch + c] = v;
+        const int cx = min(x, sx1 - 1);
+        for (int r = tid >> 6; r < rows; r += 4) {
+            const int y = uy - 2 + r;
+            const bool vy = y >= 0 && (y < sy1 || row_ext);
+            uint16_t v = CDEF_VL;
+            if (vy && vx) v = row_ptr<T>(s, min(y, sy1 - 1))[cx];
+            tile[r * pitch + c] = v;
+        }
     }
 }
 
 __device__ __forceinline__ int str_pri(uint8_t p) { return p >> 2; }
 __device__ __forceinline__ int str_sec(uint8_t p) { const int s = p & 3; return s == 3 ? 4 : s; }
 
+// constrain(d, t, sh) = sign(d) min(|d|, max(0, t - (|d| >> sh))), written as
+// the clamp of d to [-c, c] with c = max(0, t - (|d| >> sh)).
+__device__ __forceinline__ int med3i(int a, int b, int c)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ int cst(int a, int x, int t, int sh)
+{
+    const int d = a - x;
+    const unsigned c = __builtin_elementwise_sub_sat((unsigned)t, (unsigned)(d < 0 ? -d : d) >> sh);
+    return med3i(d, -(int)c, (int)c);
+}
+
+// Per-block filter record: primary tap offsets, secondary tap offsets along
+// dir + 2 and dir - 2 (int16 pairs), then strength, primary shift, secondary
+// shift and the filter flag (1: filter, 2: clip to the taps' range).
+struct __align__(16) BlockRec { int o, sa, sb, p; };
+
 template <typename T>
 __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView &dst, const CdefDesc &a, int *err,
-                                          int uc, int ur, uint16_t *tile, uint8_t *bdir, int32_t *bvar, uint8_t *bst)
+                                          int uc, int ur, uint16_t *tile, uint8_t *bdir, int32_t *bvar,
+                                          uint32_t (*dcost)[8], BlockRec *brec)
 {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ui = ur * a.unit_cols + uc;
     int idx = a.index[ui];
     if (idx >= a.strength_count) {
@@ -196,86 +215,109 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         const int pri = str_pri(packed) << cs;
         const int sec = str_sec(packed) << cs;
         const int damping = a.damping + cs - (pl ? 1 : 0);
-        if (tid < 64) {
-            const int by = tid >> 3, bx = tid & 7;
-            const bool listed = (lm >> tid) & 1;
-            if (pl == 0 && need_dir && listed) {
-                int d;
-                int32_t v;
-                find_dir(org + (by << bhl) * TPITCH + (bx << bwl), TPITCH, cs, &d, &v);
-                bdir[tid] = (uint8_t)d;
-                bvar[tid] = v;
+        if (pl == 0 && need_dir) {
+            // wave w: directions w and w + 4 of the block of this lane
+            if ((lm >> lane) & 1) {
+                const uint16_t *img = org + (lane >> 3) * 8 * TPITCH + (lane & 7) * 8;
+                int32_t v[64];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) v[i * 8 + j] = (int32_t)(img[i * TPITCH + j] >> cs) - 128;
+                uint32_t c0, c1;
+                switch (wv) {
+                case 0: c0 = dir_cost<0>(v); c1 = dir_cost<4>(v); break;
+                case 1: c0 = dir_cost<1>(v); c1 = dir_cost<5>(v); break;
+                case 2: c0 = dir_cost<2>(v); c1 = dir_cost<6>(v); break;
+                default: c0 = dir_cost<3>(v); c1 = dir_cost<7>(v); break;
+                }
+                dcost[lane][wv] = c0;
+                dcost[lane][wv + 4] = c1;
+            }
+            __syncthreads();
+            if (tid < 64 && ((lm >> tid) & 1)) {   // first maximum, as a signed comparison
+                int best = 0;
+                int32_t bc = (int32_t)dcost[tid][0];
+#pragma unroll
+                for (int d = 1; d < 8; d++)
+                    if ((int32_t)dcost[tid][d] > bc) { bc = (int32_t)dcost[tid][d]; best = d; }
+                bdir[tid] = (uint8_t)best;
+                bvar[tid] = (int32_t)((uint32_t)bc - dcost[tid][(best + 4) & 7]) >> 10;
             }
         }
         __syncthreads();
         if (tid < 64) {
             const bool listed = (lm >> tid) & 1;
-            int st = 0xff;
+            int dir = 0;
+            if (need_dir && listed) {
+                dir = bdir[tid];
+                if (pl == 1 && xd != yd) {
+                    const uint8_t c422[8] = { 7, 0, 2, 4, 5, 6, 6, 6 };
+                    const uint8_t c440[8] = { 1, 2, 2, 2, 3, 4, 6, 0 };
+                    dir = xd ? c422[dir] : c440[dir];
+                    bdir[tid] = (uint8_t)dir;
+                }
+            }
+            int st = 0;
             if (listed) {
                 const bool luma_adjust = pl == 0 && (need_dir || pri != 0);
                 st = luma_adjust ? adjust_strength(pri, bvar[tid]) : pri;
             }
-            bst[tid] = (uint8_t)min(st, 0xff);
-            if (pl == 1 && need_dir && listed && xd != yd) {
-                const uint8_t c422[8] = { 7, 0, 2, 4, 5, 6, 6, 6 };
-                const uint8_t c440[8] = { 1, 2, 2, 2, 3, 4, 6, 0 };
-                bdir[tid] = xd ? c422[bdir[tid]] : c440[bdir[tid]];
-            }
+            const int fd = pri ? dir : 0;
+            const int psh = st ? max(damping - ilog2_((unsigned)st), 0) : 0;
+            const int ssh = sec ? max(damping - ilog2_((unsigned)sec), 0) : 0;
+            const int flag = (listed && (st != 0 || sec != 0) ? 1 : 0) | (st && sec ? 2 : 0);
+            BlockRec r;
+            auto pack = [](int lo, int hi) { return (int)(((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16)); };
+            r.o = pack(kDirOff[fd + 2][0], kDirOff[fd + 2][1]);
+            r.sa = pack(kDirOff[fd + 4][0], kDirOff[fd + 4][1]);
+            r.sb = pack(kDirOff[fd][0], kDirOff[fd][1]);
+            r.p = min(st, 255) | (psh << 8) | (ssh << 16) | (flag << 24);
+            brec[tid] = r;
         }
         __syncthreads();
         if (dir_only) {
             copy_unit<T>(sp, dp, ux, uy, uw, uh, tid);
             continue;
         }
-        const int psh_base = damping;
-        const int n = uw * uh;
-        for (int i = tid; i < n; i += 256) {
-            const int y = i / uw, x = i - y * uw;
-            const int b = ((y >> bhl) << 3) + (x >> bwl);
+        for_each_px(uw, uh, tid, [&](int x, int y) {
+            const BlockRec r = brec[((y >> bhl) << 3) + (x >> bwl)];
             const uint16_t *c = org + y * TPITCH + x;
             const int xv = c[0];
             int out = xv;
-            const int st = bst[b];
-            if (st != 0xff && (st != 0 || sec != 0)) {
-                const int dir = pri ? bdir[b] : 0;
-                const int psh = st ? max(psh_base - ilog2_((unsigned)st), 0) : 0;
-                const int ssh = sec ? max(psh_base - ilog2_((unsigned)sec), 0) : 0;
-                const int t0 = ((st >> cs) & 1) ? 3 : 4, t1 = ((st >> cs) & 1) ? 3 : 2;
-                const int o0 = kDirOff[dir + 2][0], o1 = kDirOff[dir + 2][1];
-                const int s0a = kDirOff[dir + 4][0], s0b = kDirOff[dir + 4][1];
-                const int s1a = kDirOff[dir][0], s1b = kDirOff[dir][1];
+            const int flag = r.p >> 24;
+            if (flag & 1) {
+                const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
+                const int o0 = (int16_t)(r.o & 0xffff), o1 = r.o >> 16;
+                const int s0a = (int16_t)(r.sa & 0xffff), s0b = r.sa >> 16;
+                const int s1a = (int16_t)(r.sb & 0xffff), s1b = r.sb >> 16;
                 int sum = 0;
-                if (st && sec) {
+                if (pri) {   // st may be 0 for a flat block: its primary terms vanish
+                    const int t0 = ((st >> cs) & 1) ? 3 : 4, t1 = ((st >> cs) & 1) ? 3 : 2;
+                    sum = t0 * (cst(c[o0], xv, st, psh) + cst(c[-o0], xv, st, psh)) +
+                          t1 * (cst(c[o1], xv, st, psh) + cst(c[-o1], xv, st, psh));
+                }
+                if (sec) {
+                    sum += 2 * (cst(c[s0a], xv, sec, ssh) + cst(c[-s0a], xv, sec, ssh) +
+                                cst(c[s1a], xv, sec, ssh) + cst(c[-s1a], xv, sec, ssh)) +
+                           cst(c[s0b], xv, sec, ssh) + cst(c[-s0b], xv, sec, ssh) +
+                           cst(c[s1b], xv, sec, ssh) + cst(c[-s1b], xv, sec, ssh);
+                }
+                out = xv + ((8 + sum - (sum < 0)) >> 4);
+                if (flag & 2) {   // clip to the taps' range, VeryLarge excluded from the maximum
+                    const int mk = CDEF_VL - 1;
                     const int a0 = c[o0], a1 = c[-o0], a2 = c[o1], a3 = c[-o1];
                     const int b0 = c[s0a], b1 = c[-s0a], b2 = c[s1a], b3 = c[-s1a];
                     const int b4 = c[s0b], b5 = c[-s0b], b6 = c[s1b], b7 = c[-s1b];
-                    sum = t0 * (constrain(a0 - xv, st, psh) + constrain(a1 - xv, st, psh)) +
-                          t1 * (constrain(a2 - xv, st, psh) + constrain(a3 - xv, st, psh));
-                    sum += 2 * (constrain(b0 - xv, sec, ssh) + constrain(b1 - xv, sec, ssh) +
-                                constrain(b2 - xv, sec, ssh) + constrain(b3 - xv, sec, ssh));
-                    sum += constrain(b4 - xv, sec, ssh) + constrain(b5 - xv, sec, ssh) +
-                           constrain(b6 - xv, sec, ssh) + constrain(b7 - xv, sec, ssh);
-                    const int mk = CDEF_VL - 1;
                     int hi = max(max(max(xv, a0 & mk), max(a1 & mk, a2 & mk)), max(a3 & mk, b0 & mk));
                     hi = max(max(max(hi, b1 & mk), max(b2 & mk, b3 & mk)), max(max(b4 & mk, b5 & mk), max(b6 & mk, b7 & mk)));
                     int lo = min(min(min(xv, a0), min(a1, a2)), min(a3, b0));
                     lo = min(min(min(lo, b1), min(b2, b3)), min(min(b4, b5), min(b6, b7)));
-                    out = xv + ((8 + sum - (sum < 0)) >> 4);
                     out = min(max(out, lo), hi);
-                } else if (st) {
-                    sum = t0 * (constrain(c[o0] - xv, st, psh) + constrain(c[-o0] - xv, st, psh)) +
-                          t1 * (constrain(c[o1] - xv, st, psh) + constrain(c[-o1] - xv, st, psh));
-                    out = xv + ((8 + sum - (sum < 0)) >> 4);
-                } else {
-                    sum = 2 * (constrain(c[s0a] - xv, sec, ssh) + constrain(c[-s0a] - xv, sec, ssh) +
-                               constrain(c[s1a] - xv, sec, ssh) + constrain(c[-s1a] - xv, sec, ssh)) +
-                          constrain(c[s0b] - xv, sec, ssh) + constrain(c[-s0b] - xv, sec, ssh) +
-                          constrain(c[s1b] - xv, sec, ssh) + constrain(c[-s1b] - xv, sec, ssh);
-                    out = xv + ((8 + sum - (sum < 0)) >> 4);
                 }
             }
             row_ptr<T>(dp, uy + y)[ux + x] = (T)out;
-        }
+        });
     }
 }
 
@@ -285,15 +327,16 @@ __global__ __launch_bounds__(256) void cdef_kernel(const FrameDesc *__restrict__
     __shared__ uint16_t tile[TROWS * TPITCH];
     __shared__ uint8_t bdir[64];
     __shared__ int32_t bvar[64];
-    __shared__ uint8_t bst[64];    // per-block primary strength (after luma adjust), 0xff = not listed
+    __shared__ uint32_t dcost[64][8];   // direction costs per block
+    __shared__ BlockRec brec[64];
     const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const FrameDesc &D = descs[sg.frame];
     const int t = blockIdx.x - sg.base;
     const int uc = t % sg.nx, ur = t / sg.nx;
     if (D.v[VIEW_A].bd == 8)
-        cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, bst);
+        cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
     else
-        cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, bst);
+        cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
 }
 
 hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
