@@ -54,10 +54,9 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
         u = g.units[ur * g.hz + uc];
     }
     if (u.type != AV1PF_LR_WIENER && u.type != AV1PF_LR_SGRPROJ) {
-        for (int i = tid; i < w * h; i += 256) {
-            const int r = i / w, c = i - r * w;
-            row_ptr<T>(dp, y0 + r)[x0 + c] = row_ptr<T>(sp, y0 + r)[x0 + c];
-        }
+        const int lpr = lanes_for(w), c = tid & (lpr - 1);
+        if (c < w)
+            for (int r = tid / lpr; r < h; r += 256 / lpr) row_ptr<T>(dp, y0 + r)[x0 + c] = row_ptr<T>(sp, y0 + r)[x0 + c];
         return;
     }
     // ---- stage the tile with its 3-sample halo
@@ -65,20 +64,23 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
     const int tw = w + 6, th = h + 6;
     const uint16_t *bnd = g.bnd + (int64_t)t * 4 * g.pw;
-    for (int i = tid; i < tw * th; i += 256) {
-        const int r = i / tw - 3, c = i - (i / tw) * tw - 3;
+    for (int cc = tid & 63; cc < tw; cc += 64) {
+        const int c = cc - 3;
         const int x = min(max(x0 + c, 0), g.pw - 1);
-        int v;
-        if (r < 0) {
-            if (copy_above) v = bnd[(r == -1 ? 1 : 0) * g.pw + x];
-            else v = read_px<T>(sp, max(y0 + r, 0), x);
-        } else if (r >= h) {
-            if (copy_below) v = bnd[(2 + min(r - h, 1)) * g.pw + x];
-            else v = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
-        } else {
-            v = read_px<T>(sp, y0 + r, x);
+        for (int rr = tid >> 6; rr < th; rr += 4) {
+            const int r = rr - 3;
+            int v;
+            if (r < 0) {
+                if (copy_above) v = bnd[(r == -1 ? 1 : 0) * g.pw + x];
+                else v = read_px<T>(sp, max(y0 + r, 0), x);
+            } else if (r >= h) {
+                if (copy_below) v = bnd[(2 + min(r - h, 1)) * g.pw + x];
+                else v = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
+            } else {
+                v = read_px<T>(sp, y0 + r, x);
+            }
+            tile[rr * LR_TW + cc] = (uint16_t)v;
         }
-        tile[(r + 3) * LR_TW + c + 3] = (uint16_t)v;
     }
     __syncthreads();
     const uint16_t *O = tile + 3 * LR_TW + 3;   // tile origin
@@ -91,9 +93,12 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
 __global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
-    __shared__ uint16_t tmp[LR_TW * 64];
-    __shared__ int32_t Ab[LR_AB * LR_AB];
-    __shared__ int32_t Bb[LR_AB * LR_AB];
+    // Wiener intermediate and self-guided A/B planes share one buffer
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];
+    int32_t *Bb = reinterpret_cast<int32_t *>(work);
+    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
+    static_assert(LR_TW * 64 * 2 <= LR_AB * LR_AB * 8, "Wiener intermediate fits the shared buffer");
     const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const FrameDesc &D = descs[sg.frame];
     const int t = blockIdx.x - sg.base;

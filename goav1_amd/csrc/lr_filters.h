@@ -10,7 +10,10 @@
 //   - self-guided (restoration/selfguided.go): box sums, uint32 p/z/A/B math,
 //     6/5 (r=2, every other row) and 4/3 (r=1) weighted sums, projection with
 //     an int16 wrap before the clip.
-// Block size 256 threads; the caller synchronises before the call.
+// Block size 256 threads; the caller synchronises before the call.  LDS the
+// caller provides: the tile, a 70 x 64 uint16 Wiener intermediate, and the
+// self-guided A and B planes (int32, LR_AB x LR_AB); the Wiener intermediate
+// may share memory with A and B.
 #pragma once
 #include "pf_internal.h"
 
@@ -36,6 +39,10 @@ __device__ __forceinline__ int32_t x_by_xplus1(uint32_t z)
     return (int32_t)((512u * z + (z + 1)) / (2u * (z + 1)));
 }
 
+// Lanes per row for a region w samples wide (a power of two >= w, at most 64):
+// lane x of a row group handles column x, row groups step by 256 / lanes.
+__device__ __forceinline__ int lanes_for(int w) { return w > 32 ? 64 : w > 16 ? 32 : w > 8 ? 16 : 8; }
+
 template <typename Sink>
 __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int bd, const int8_t *vt, const int8_t *ht,
                                             uint16_t *tmp, Sink sink)
@@ -51,22 +58,86 @@ __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int
     const int32_t v0 = vt[0], v1 = vt[1], v2 = vt[2];
     const int32_t v3 = -2 * (v0 + v1 + v2);
     const int32_t limit = 1 << (bd + 1 + 7 - round0), hoff = 1 << (bd + 6);
-    for (int i = tid; i < w * th; i += 256) {
-        const int r = i / w - 3, c = i - (i / w) * w;
-        const uint16_t *s = O + r * LR_TW + c;
-        int32_t sum = ((int32_t)s[0] << 7) + hoff;
-        sum += h0 * ((int32_t)s[-3] + s[3]) + h1 * ((int32_t)s[-2] + s[2]) + h2 * ((int32_t)s[-1] + s[1]) + h3 * (int32_t)s[0];
-        tmp[(r + 3) * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
-    }
+    const int lpr = lanes_for(w), c = tid & (lpr - 1), rstep = 256 / lpr;
+    if (c < w)
+        for (int rr = tid / lpr; rr < th; rr += rstep) {
+            const int r = rr - 3;
+            const uint16_t *s = O + r * LR_TW + c;
+            int32_t sum = ((int32_t)s[0] << 7) + hoff;
+            sum += h0 * ((int32_t)s[-3] + s[3]) + h1 * ((int32_t)s[-2] + s[2]) + h2 * ((int32_t)s[-1] + s[1]) +
+                   h3 * (int32_t)s[0];
+            tmp[rr * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
+        }
     __syncthreads();
     const int32_t voff = 1 << (bd + round1 - 1);
-    for (int i = tid; i < w * h; i += 256) {
-        const int r = i / w, c = i - r * w;
-        const uint16_t *s = tmp + (r + 3) * 64 + c;
-        int32_t sum = ((int32_t)s[0] << 7) - voff;
-        sum += v0 * ((int32_t)s[-3 * 64] + s[3 * 64]) + v1 * ((int32_t)s[-2 * 64] + s[2 * 64]) +
-               v2 * ((int32_t)s[-64] + s[64]) + v3 * (int32_t)s[0];
-        sink(r, c, min(max(r2s(sum, round1), 0), mx));
+    if (c < w)
+        for (int r = tid / lpr; r < h; r += rstep) {
+            const uint16_t *s = tmp + (r + 3) * 64 + c;
+            int32_t sum = ((int32_t)s[0] << 7) - voff;
+            sum += v0 * ((int32_t)s[-3 * 64] + s[3 * 64]) + v1 * ((int32_t)s[-2 * 64] + s[2 * 64]) +
+                   v2 * ((int32_t)s[-64] + s[64]) + v3 * (int32_t)s[0];
+            sink(r, c, min(max(r2s(sum, round1), 0), mx));
+        }
+}
+
+// Box sum and sum of squares of the (2R+1)-wide row segment centred at s.
+template <int R>
+__device__ __forceinline__ void row_box(const uint16_t *s, int32_t &sum, int32_t &sq)
+{
+    sum = 0;
+    sq = 0;
+#pragma unroll
+    for (int dx = -R; dx <= R; dx++) {
+        const int32_t v = s[dx];
+        sum += v;
+        sq += v * v;
+    }
+}
+
+// A and B of one self-guided pass (radius R, every STEP-th row from -1) for
+// the positions x = -1..w: one lane per column, the rows split among
+// 256 / (w + 2) row groups; the box sums slide down the column, each row's
+// horizontal box sum computed once.
+template <int R, int STEP>
+__device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, uint32_t s, int32_t *Ab, int32_t *Bb)
+{
+    constexpr int N = (2 * R + 1) * (2 * R + 1);
+    const uint32_t recip = (uint32_t)((8192 + N) / (2 * N));
+    const int tid = threadIdx.x;
+    const int ncols = w + 2;                                   // x = -1 .. w
+    const int nrows = STEP == 2 ? (h + 3) / 2 : h + 2;          // y = -1, -1 + STEP, ...
+    const int groups = 256 / ncols;                             // row groups (3 for 64-wide tiles)
+    const int col = tid % ncols, grp = tid / ncols;
+    if (grp >= groups) return;
+    const int per = (nrows + groups - 1) / groups;
+    const int k0 = grp * per, k1 = min(k0 + per, nrows);
+    if (k0 >= k1) return;
+    const int x = col - 1;
+    int32_t S[2 * R + 1], Q[2 * R + 1];
+    {
+        const int y = -1 + k0 * STEP;
+#pragma unroll
+        for (int i = 0; i <= 2 * R; i++) row_box<R>(O + (y - R + i) * LR_TW + x, S[i], Q[i]);
+    }
+    for (int k = k0; k < k1; k++) {
+        const int y = -1 + k * STEP;
+        if (k > k0) {
+#pragma unroll
+            for (int i = 0; i + STEP <= 2 * R; i++) { S[i] = S[i + STEP]; Q[i] = Q[i + STEP]; }
+#pragma unroll
+            for (int i = 2 * R + 1 - STEP; i <= 2 * R; i++) row_box<R>(O + (y - R + i) * LR_TW + x, S[i], Q[i]);
+        }
+        int32_t sum = 0, sq = 0;
+#pragma unroll
+        for (int i = 0; i <= 2 * R; i++) { sum += S[i]; sq += Q[i]; }
+        const uint32_t a = (uint32_t)r2s(sq, 2 * (bd - 8));
+        const uint32_t b = (uint32_t)r2s(sum, bd - 8);
+        const uint32_t p = a * (uint32_t)N >= b * b ? a * (uint32_t)N - b * b : 0u;
+        uint32_t z = r2u(p * s, 20);
+        z = min(z, 255u);
+        const int32_t aa = x_by_xplus1(z);
+        Ab[(y + 1) * LR_AB + x + 1] = aa;
+        Bb[(y + 1) * LR_AB + x + 1] = (int32_t)r2u((uint32_t)(256 - aa) * (uint32_t)sum * recip, 12);
     }
 }
 
@@ -80,79 +151,60 @@ __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd
     if (kSgrR[set][0] == 0) { xq0 = 0; xq1 = 128 - xqd[1]; }
     else if (kSgrR[set][1] == 0) { xq0 = xqd[0]; xq1 = 0; }
     else { xq0 = xqd[0]; xq1 = 128 - xq0 - xqd[1]; }
-    constexpr int PPT = 16;   // pixels per thread (64x64 / 256)
+    constexpr int PPT = 16;   // rows per lane: at most 64 rows / (256 / 64 lanes)
+    const int lpr = lanes_for(w), x = tid & (lpr - 1), rstep = 256 / lpr, ry = tid / lpr;
     int32_t f0[PPT], f1[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) { f0[j] = 0; f1[j] = 0; }
-    const int aw = w + 2;
     for (int pass = 0; pass < 2; pass++) {
         const int r = kSgrR[set][pass];
         if (r == 0) continue;
-        const int n = (2 * r + 1) * (2 * r + 1);
         const uint32_t s = (uint32_t)kSgrS[set][pass];
-        const uint32_t recip = (uint32_t)((8192 + n) / (2 * n));
-        const int step = pass == 0 ? 2 : 1;
-        const int nrows = pass == 0 ? (h + 3) / 2 : h + 2;   // rows -1, 1, 3, ... or -1..h
         __syncthreads();
-        for (int i = tid; i < nrows * aw; i += 256) {
-            const int ri = i / aw;
-            const int y = -1 + ri * step, x = i - ri * aw - 1;
-            int32_t sum = 0, sq = 0;
-            for (int dy = -r; dy <= r; dy++)
-                for (int dx = -r; dx <= r; dx++) {
-                    const int32_t v = O[(y + dy) * LR_TW + x + dx];
-                    sum += v;
-                    sq += v * v;
-                }
-            const uint32_t a = (uint32_t)r2s(sq, 2 * (bd - 8));
-            const uint32_t b = (uint32_t)r2s(sum, bd - 8);
-            const uint32_t p = a * (uint32_t)n >= b * b ? a * (uint32_t)n - b * b : 0u;
-            uint32_t z = r2u(p * s, 20);
-            z = min(z, 255u);
-            const int32_t aa = x_by_xplus1(z);
-            Ab[(y + 1) * LR_AB + x + 1] = aa;
-            Bb[(y + 1) * LR_AB + x + 1] = (int32_t)r2u((uint32_t)(256 - aa) * (uint32_t)sum * recip, 12);
-        }
+        if (pass == 0) sgr_ab<2, 2>(O, w, h, bd, s, Ab, Bb);
+        else sgr_ab<1, 1>(O, w, h, bd, s, Ab, Bb);
         __syncthreads();
+        if (x < w) {
 #pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int i = tid + 256 * j;
-            if (i >= w * h) break;
-            const int y = i / w, x = i - y * w;
-            const int32_t uu = O[y * LR_TW + x];
-            const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
-            const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
-            int32_t a, b;
-            if (pass == 0) {
-                if ((y & 1) == 0) {
-                    a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
-                    b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
-                    f0[j] = r2s(a * uu + b, 9);
+            for (int j = 0; j < PPT; j++) {
+                const int y = ry + rstep * j;
+                if (y >= h) break;
+                const int32_t uu = O[y * LR_TW + x];
+                const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
+                const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
+                int32_t a, b;
+                if (pass == 0) {
+                    if ((y & 1) == 0) {
+                        a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
+                        b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
+                        f0[j] = r2s(a * uu + b, 9);
+                    } else {
+                        a = A[0] * 6 + (A[-1] + A[1]) * 5;
+                        b = B[0] * 6 + (B[-1] + B[1]) * 5;
+                        f0[j] = r2s(a * uu + b, 8);
+                    }
                 } else {
-                    a = A[0] * 6 + (A[-1] + A[1]) * 5;
-                    b = B[0] * 6 + (B[-1] + B[1]) * 5;
-                    f0[j] = r2s(a * uu + b, 8);
+                    a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
+                        (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
+                    b = (B[0] + B[-1] + B[1] + B[-LR_AB] + B[LR_AB]) * 4 +
+                        (B[-LR_AB - 1] + B[LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB + 1]) * 3;
+                    f1[j] = r2s(a * uu + b, 9);
                 }
-            } else {
-                a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
-                    (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
-                b = (B[0] + B[-1] + B[1] + B[-LR_AB] + B[LR_AB]) * 4 +
-                    (B[-LR_AB - 1] + B[LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB + 1]) * 3;
-                f1[j] = r2s(a * uu + b, 9);
             }
         }
     }
+    if (x < w) {
 #pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        const int i = tid + 256 * j;
-        if (i >= w * h) break;
-        const int y = i / w, x = i - y * w;
-        const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
-        int32_t v = uu << 7;
-        v += xq0 * (f0[j] - uu);
-        v += xq1 * (f1[j] - uu);
-        const int32_t wv = (int16_t)r2s(v, 11);
-        sink(y, x, min(max(wv, 0), mx));
+        for (int j = 0; j < PPT; j++) {
+            const int y = ry + rstep * j;
+            if (y >= h) break;
+            const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
+            int32_t v = uu << 7;
+            v += xq0 * (f0[j] - uu);
+            v += xq1 * (f1[j] - uu);
+            const int32_t wv = (int16_t)r2s(v, 11);
+            sink(y, x, min(max(wv, 0), mx));
+        }
     }
 }
 

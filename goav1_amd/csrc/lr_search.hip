@@ -42,10 +42,11 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     __shared__ uint8_t srcT[64 * 64];
-    __shared__ int32_t Ab[LR_AB * LR_AB];
-    __shared__ int32_t Bb[LR_AB * LR_AB];
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
-    uint16_t *tmp = reinterpret_cast<uint16_t *>(Ab);   // Wiener intermediate; SGR runs after it
+    int32_t *Bb = reinterpret_cast<int32_t *>(work);
+    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *tmp = reinterpret_cast<uint16_t *>(work);   // SGR runs after the Wiener re-score
     const int tid = threadIdx.x;
     const int u = blockIdx.x;
     const int col = u % hz, row = u / hz;
