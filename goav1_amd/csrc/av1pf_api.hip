@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 #include "pf_internal.h"
@@ -32,6 +33,31 @@ using namespace pf;
 namespace {
 constexpr int kSlots = 4;
 constexpr size_t kSlotBytes = size_t(1) << 20;   // ~400 frames of descriptors
+// Stages whose work distribution is tuned: CDEF, LR, film-grain apply.
+// Mode = units per workgroup (1, 2, 4) or 0 = persistent workgroups on a queue.
+constexpr int kTunable = 3;
+struct StageModes {
+    int m[kTunable] = { 1, 1, 1 };
+};
+// One captured chain of a shape class: graph, its host image and device tables.
+struct GraphInst {
+    hipGraphExec_t exec = nullptr;
+    char *host = nullptr, *dev = nullptr;
+    hipEvent_t done = nullptr;
+    bool used = false;
+};
+struct GraphClass {
+    GraphInst inst[2];
+    int next = 0;
+};
+void free_graph_inst(GraphInst &I)
+{
+    if (I.exec) hipGraphExecDestroy(I.exec);
+    if (I.host) hipHostFree(I.host);
+    if (I.dev) hipFree(I.dev);
+    if (I.done) hipEventDestroy(I.done);
+    I = GraphInst();
+}
 }
 
 struct av1pf_ctx {
@@ -60,6 +86,21 @@ struct av1pf_ctx {
     hipStream_t side = nullptr;   // film-grain template stream
     hipEvent_t join = nullptr;
     hipEvent_t learn_fork = nullptr, learn_join = nullptr;   // learned search: LR search on the side stream
+    // shape classes: tuned work distributions and captured graphs
+    std::map<std::vector<int>, StageModes> tuned;
+    std::map<std::vector<int>, GraphClass> gclass;
+    bool graphs = false;
+    int forced[kTunable] = { -1, -1, -1 };   // test override of the stage modes
+    hipStream_t cap = nullptr;               // capture stream of the graphs
+    hipEvent_t g_fork = nullptr, g_up = nullptr, g_join = nullptr;
+    void drop_graphs(const std::vector<int> &key)
+    {
+        auto it = gclass.find(key);
+        if (it == gclass.end()) return;
+        hipDeviceSynchronize();
+        for (auto &I : it->second.inst) free_graph_inst(I);
+        gclass.erase(it);
+    }
     // descriptor ring
     char *meta_dev[kSlots] = {};
     char *meta_host[kSlots] = {};
@@ -540,8 +581,9 @@ int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, hipStream_t s,
-              bool skip_lr_save = false)
+              bool skip_lr_save = false, bool tune_modes = false, StageModes *tuned_out = nullptr)
 {
+    StageModes chosen;
     if (n == 0) return AV1PF_OK;
     BatchPlan P;
     std::vector<size_t> scr_off, bnd_off, goff;
@@ -574,7 +616,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             D.gr.offsets = c->grain_offs[par] + goff[i];
         }
     }
-    // ---- pack descriptors + segment tables into one slot
+    // ---- descriptors + segment tables + stage queues, one host image
     SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain };
     const int nl = sizeof(lists) / sizeof(lists[0]);
     size_t off = align256(sizeof(FrameDesc) * n);
@@ -583,6 +625,166 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         offs[k] = off;
         off = align256(off + sizeof(Seg) * (lists[k]->v.size() + 1));
     }
+    const size_t qoff = off;   // work-queue counters of the tunable stages, zero at upload
+    off = align256(off + kTunable * sizeof(unsigned int));
+    auto fill = [&](char *h) {
+        memcpy(h, P.descs.data(), sizeof(FrameDesc) * n);
+        for (int k = 0; k < nl; k++) {
+            Seg sentinel{ 0, 0, 1, lists[k]->total };
+            memcpy(h + offs[k], lists[k]->v.data(), sizeof(Seg) * lists[k]->v.size());
+            memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
+        }
+        memset(h + qoff, 0, kTunable * sizeof(unsigned int));
+    };
+    // shape class: everything a launch's arguments depend on besides data
+    std::vector<int> key;
+    key.push_back(n);
+    key.push_back(side ? 1 : 0);
+    for (int k = 0; k < nl; k++) {
+        key.push_back(lists[k]->total);
+        key.push_back((int)lists[k]->v.size());
+    }
+    StageModes modes;
+    if (auto it = c->tuned.find(key); it != c->tuned.end()) modes = it->second;
+    for (int i = 0; i < kTunable; i++)
+        if (c->forced[i] >= 0) modes.m[i] = c->forced[i];
+
+    // enqueue the chain reading its tables from device memory d (uploaded from h)
+    auto enqueue = [&](char *h, char *d, hipStream_t st, bool capture, StageModes md) -> int {
+        const FrameDesc *dd = reinterpret_cast<const FrameDesc *>(d);
+        auto segp = [&](int k) { return reinterpret_cast<const Seg *>(d + offs[k]); };
+        auto nseg = [&](int k) { return (int)lists[k]->v.size(); };
+        auto launch_of = [&](int i) {
+            StageLaunch L;
+            L.per_wg = md.m[i];
+            L.queue = reinterpret_cast<unsigned int *>(d + qoff) + i;
+            return L;
+        };
+        // tuning: after the stage's real launch, time every work distribution
+        // on the same inputs (the stage is out of place, so the result is the same)
+        auto tune = [&](int i, auto &&run) -> int {
+            if (!tune_modes) return AV1PF_OK;
+            static const int cand[] = { 1, 2, 4, 0 };
+            float best = 1e30f;
+            int pick = md.m[i];
+            hipEvent_t e0 = take_event(c), e1 = take_event(c);
+            for (int m : cand) {
+                StageLaunch L = launch_of(i);
+                L.per_wg = m;
+                for (int rep = 0; rep < 4; rep++) {
+                    if (rep == 1) HIPCHK(c, hipEventRecord(e0, st));
+                    if (m == 0) HIPCHK(c, launch_zero_u32(L.queue, 1, st));
+                    HIPCHK(c, run(L));
+                }
+                HIPCHK(c, hipEventRecord(e1, st));
+                HIPCHK(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; pick = m; }
+            }
+            c->pool.push_back(e0);
+            c->pool.push_back(e1);
+            chosen.m[i] = pick;
+            return AV1PF_OK;
+        };
+        hipEvent_t ev_up = capture ? c->g_up : c->uploaded, ev_join = capture ? c->g_join : c->join;
+        if (side) {
+            // Film-grain templates depend only on the grain parameters: the side
+            // stream uploads the descriptors and builds them without waiting for
+            // the previous call's filtering, only for the grain apply that last
+            // read this template pair (inside a graph the replay is ordered).
+            if (capture) {
+                HIPCHK(c, hipEventRecord(c->g_fork, st));
+                HIPCHK(c, hipStreamWaitEvent(c->side, c->g_fork, 0));
+            } else if (c->tmpl_free_used[par]) {
+                HIPCHK(c, hipStreamWaitEvent(c->side, c->tmpl_free[par], 0));
+            }
+            HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, c->side));
+            HIPCHK(c, hipEventRecord(ev_up, c->side));
+            HIPCHK(c, hipStreamWaitEvent(st, ev_up, 0));
+            {
+                StageTimer t(c, ST_GRAIN_T, c->side, 0);
+                HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
+            }
+            HIPCHK(c, hipEventRecord(ev_join, c->side));
+        } else {
+            HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
+        }
+        if (P.dbv.total || P.dbh.total) {
+            StageTimer t(c, ST_DEBLOCK, st, P.bytes_db);
+            HIPCHK(c, launch_deblock(dd, segp(0), nseg(0), P.dbv.total, segp(1), nseg(1), P.dbh.total, st));
+        }
+        if (P.save.total) {
+            StageTimer t(c, ST_LR_SAVE, st, 0);
+            HIPCHK(c, launch_lr_save(dd, segp(2), nseg(2), P.save.total, st));
+        }
+        int rc2;
+        if (P.cdef.total) {
+            {
+                StageTimer t(c, ST_CDEF, st, P.bytes_cdef);
+                HIPCHK(c, launch_cdef(dd, segp(3), nseg(3), P.cdef.total, launch_of(0), st));
+            }
+            if ((rc2 = tune(0, [&](const StageLaunch &L) { return launch_cdef(dd, segp(3), nseg(3), P.cdef.total, L, st); })))
+                return rc2;
+        }
+        if (P.lr.total) {
+            {
+                StageTimer t(c, ST_LR, st, P.bytes_lr);
+                HIPCHK(c, launch_lr(dd, segp(4), nseg(4), P.lr.total, launch_of(1), st));
+            }
+            if ((rc2 = tune(1, [&](const StageLaunch &L) { return launch_lr(dd, segp(4), nseg(4), P.lr.total, L, st); })))
+                return rc2;
+        }
+        if (P.copy_ba.total || P.copy_ad.total) {
+            StageTimer t(c, ST_COPY, st, P.bytes_copy);
+            HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, st));
+            HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, VIEW_A, VIEW_D, st));
+        }
+        if (side) HIPCHK(c, hipStreamWaitEvent(st, ev_join, 0));
+        if (P.grain.total) {
+            {
+                StageTimer t(c, ST_GRAIN_A, st, P.bytes_grain);
+                HIPCHK(c, launch_grain_apply(dd, segp(9), nseg(9), P.grain.total, launch_of(2), st));
+            }
+            if ((rc2 = tune(2, [&](const StageLaunch &L) {
+                     return launch_grain_apply(dd, segp(9), nseg(9), P.grain.total, L, st);
+                 })))
+                return rc2;
+        }
+        return AV1PF_OK;
+    };
+
+    if (c->graphs && !c->profiling && !tune_modes) {
+        // one graph per shape class, two instances used alternately so that the
+        // host image of one can be written while the other replays
+        GraphClass &G = c->gclass[key];
+        GraphInst &I = G.inst[G.next];
+        G.next ^= 1;
+        if (I.used) HIPCHK(c, hipEventSynchronize(I.done));
+        if (!I.exec) {
+            HIPCHK(c, hipHostMalloc(&I.host, off, hipHostMallocDefault));
+            HIPCHK(c, hipMalloc(&I.dev, off));
+            HIPCHK(c, hipEventCreateWithFlags(&I.done, hipEventDisableTiming));
+            HIPCHK(c, hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(I.host, I.dev, c->cap, true, modes);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(c->cap, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            HIPCHK(c, ec);
+            const hipError_t ei = hipGraphInstantiate(&I.exec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            HIPCHK(c, ei);
+        }
+        fill(I.host);
+        HIPCHK(c, hipGraphLaunch(I.exec, s));
+        HIPCHK(c, hipEventRecord(I.done, s));
+        I.used = true;
+        return AV1PF_OK;
+    }
+
     const int slot = c->slot;
     c->slot = (c->slot + 1) % kSlots;
     if (c->meta_used[slot]) HIPCHK(c, hipEventSynchronize(c->meta_done[slot]));
@@ -594,66 +796,20 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         HIPCHK(c, hipHostMalloc(&c->meta_host[slot], cap, hipHostMallocDefault));
         c->meta_cap[slot] = cap;
     }
-    char *h = c->meta_host[slot];
-    memcpy(h, P.descs.data(), sizeof(FrameDesc) * n);
-    for (int k = 0; k < nl; k++) {
-        Seg sentinel{ 0, 0, 1, lists[k]->total };
-        memcpy(h + offs[k], lists[k]->v.data(), sizeof(Seg) * lists[k]->v.size());
-        memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
-    }
-    char *d = c->meta_dev[slot];
-    const FrameDesc *dd = reinterpret_cast<const FrameDesc *>(d);
-    auto segp = [&](int k) { return reinterpret_cast<const Seg *>(d + offs[k]); };
-    auto nseg = [&](int k) { return (int)lists[k]->v.size(); };
-    if (side) {
-        // Film-grain templates depend only on the grain parameters: the side
-        // stream uploads the descriptors and builds them without waiting for
-        // the previous call's filtering, only for the grain apply that last
-        // read this template pair.
-        if (c->tmpl_free_used[par]) HIPCHK(c, hipStreamWaitEvent(c->side, c->tmpl_free[par], 0));
-        HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, c->side));
-        HIPCHK(c, hipEventRecord(c->uploaded, c->side));
-        HIPCHK(c, hipStreamWaitEvent(s, c->uploaded, 0));
-        {
-            StageTimer t(c, ST_GRAIN_T, c->side, 0);
-            HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
-        }
-        HIPCHK(c, hipEventRecord(c->join, c->side));
-    } else {
-        HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
-    }
-    if (P.dbv.total || P.dbh.total) {
-        StageTimer t(c, ST_DEBLOCK, s, P.bytes_db);
-        HIPCHK(c, launch_deblock(dd, segp(0), nseg(0), P.dbv.total, segp(1), nseg(1), P.dbh.total, s));
-    }
-    if (P.save.total) {
-        StageTimer t(c, ST_LR_SAVE, s, 0);
-        HIPCHK(c, launch_lr_save(dd, segp(2), nseg(2), P.save.total, s));
-    }
-    if (P.cdef.total) {
-        StageTimer t(c, ST_CDEF, s, P.bytes_cdef);
-        HIPCHK(c, launch_cdef(dd, segp(3), nseg(3), P.cdef.total, s));
-    }
-    if (P.lr.total) {
-        StageTimer t(c, ST_LR, s, P.bytes_lr);
-        HIPCHK(c, launch_lr(dd, segp(4), nseg(4), P.lr.total, s));
-    }
-    if (P.copy_ba.total || P.copy_ad.total) {
-        StageTimer t(c, ST_COPY, s, P.bytes_copy);
-        HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, s));
-        HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, VIEW_A, VIEW_D, s));
-    }
-    if (side) HIPCHK(c, hipStreamWaitEvent(s, c->join, 0));
-    if (P.grain.total) {
-        StageTimer t(c, ST_GRAIN_A, s, P.bytes_grain);
-        HIPCHK(c, launch_grain_apply(dd, segp(9), nseg(9), P.grain.total, s));
-    }
+    fill(c->meta_host[slot]);
+    chosen = modes;
+    if ((rc = enqueue(c->meta_host[slot], c->meta_dev[slot], s, false, modes))) return rc;
     if (side) {
         HIPCHK(c, hipEventRecord(c->tmpl_free[par], s));
         c->tmpl_free_used[par] = true;
     }
     HIPCHK(c, hipEventRecord(c->meta_done[slot], s));
     c->meta_used[slot] = true;
+    if (tune_modes) {
+        c->tuned[key] = chosen;
+        c->drop_graphs(key);   // graphs captured with the previous modes
+        if (tuned_out) *tuned_out = chosen;
+    }
     return AV1PF_OK;
 }
 
@@ -685,7 +841,11 @@ int av1pf_create(av1pf_ctx **out, int device)
               hipEventCreateWithFlags(&c->tmpl_free[1], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->learn_fork, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&c->learn_join, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&c->learn_join, hipEventDisableTiming) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->g_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->g_up, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->g_join, hipEventDisableTiming) == hipSuccess;
     // descriptor slots sized for large batches up front, so that no call in
     // steady state allocates
     for (int i = 0; ok && i < kSlots; i++) {
@@ -726,6 +886,11 @@ void av1pf_destroy(av1pf_ctx *c)
     if (c->join) hipEventDestroy(c->join);
     if (c->learn_fork) hipEventDestroy(c->learn_fork);
     if (c->learn_join) hipEventDestroy(c->learn_join);
+    for (auto &kv : c->gclass)
+        for (auto &I : kv.second.inst) free_graph_inst(I);
+    if (c->cap) hipStreamDestroy(c->cap);
+    for (hipEvent_t e : { c->g_fork, c->g_up, c->g_join })
+        if (e) hipEventDestroy(e);
     for (int i = 0; i < 2; i++) {
         if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
         if (c->templates[i]) hipFree(c->templates[i]);
@@ -758,6 +923,47 @@ int av1pf_apply_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1p
     for (int i = 0; i < n; i++)
         if (reqs[i].stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;
     return run_batch(c, n, frames, reqs, static_cast<hipStream_t>(stream));
+}
+
+int av1pf_set_graphs(av1pf_ctx *c, int enable)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!enable) {
+        HIPCHK(c, hipDeviceSynchronize());
+        for (auto &kv : c->gclass)
+            for (auto &I : kv.second.inst) free_graph_inst(I);
+        c->gclass.clear();
+    }
+    c->graphs = enable != 0;
+    return AV1PF_OK;
+}
+
+int av1pf_autotune(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, int32_t *modes,
+                   void *stream)
+{
+    if (!c || n < 0 || (n && (!frames || !reqs))) return AV1PF_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    StageModes chosen;
+    const int rc = run_batch(c, n, frames, reqs, static_cast<hipStream_t>(stream), false, true, &chosen);
+    if (rc) return rc;
+    if (modes)
+        for (int i = 0; i < kTunable; i++) modes[i] = chosen.m[i];
+    return AV1PF_OK;
+}
+
+int av1pf_set_stage_mode(av1pf_ctx *c, uint32_t stage, int mode)
+{
+    if (!c || mode < -1 || mode > 8) return AV1PF_ERR_INVALID_ARG;
+    const int i = stage == AV1PF_STAGE_CDEF ? 0 : stage == AV1PF_STAGE_LR ? 1 : stage == AV1PF_STAGE_FILMGRAIN ? 2 : -1;
+    if (i < 0) return fail(c, AV1PF_ERR_INVALID_ARG, "stage mode applies to CDEF, LR and film grain only");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
+    c->forced[i] = mode;
+    for (auto &kv : c->gclass)   // graphs captured with the previous modes
+        for (auto &I : kv.second.inst) free_graph_inst(I);
+    c->gclass.clear();
+    return AV1PF_OK;
 }
 
 int av1pf_sync(av1pf_ctx *c, void *stream)

@@ -322,26 +322,30 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
 }
 
 __global__ __launch_bounds__(256) void cdef_kernel(const FrameDesc *__restrict__ descs,
-                                                   const Seg *__restrict__ segs, int nseg)
+                                                   const Seg *__restrict__ segs, int nseg, int total, StageLaunch L)
 {
     __shared__ uint16_t tile[TROWS * TPITCH];
     __shared__ uint8_t bdir[64];
     __shared__ int32_t bvar[64];
     __shared__ uint32_t dcost[64][8];   // direction costs per block
     __shared__ BlockRec brec[64];
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const int t = blockIdx.x - sg.base;
-    const int uc = t % sg.nx, ur = t / sg.nx;
-    if (D.v[VIEW_A].bd == 8)
-        cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
-    else
-        cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
+    for_each_unit(total, L, [&](int u) {
+        const Seg sg = segs[find_seg(segs, nseg, u)];
+        const FrameDesc &D = descs[sg.frame];
+        const int t = u - sg.base;
+        const int uc = t % sg.nx, ur = t / sg.nx;
+        if (D.v[VIEW_A].bd == 8)
+            cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
+        else
+            cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
+    });
 }
 
-hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
+hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s)
 {
-    if (total) hipLaunchKernelGGL(cdef_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
+    if (total)
+        hipLaunchKernelGGL(cdef_kernel, dim3(stage_grid((const void *)cdef_kernel, total, L.per_wg)), dim3(256), 0, s,
+                           d, segs, nseg, total, L);
     return hipGetLastError();
 }
 

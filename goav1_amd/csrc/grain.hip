@@ -359,32 +359,38 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
     }
 }
 
-__global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+__global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
+                                                          int nseg, int total, StageLaunch L)
 {
     __shared__ uint8_t lut[3][256];
     __shared__ uint8_t offs[2][2];
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const GrainDesc &G = D.gr;
-    const int t = blockIdx.x - sg.base;
-    const int bcol = t % sg.nx, brow = t / sg.nx;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = G.luts[i >> 8][i & 255];
-    if (tid < 4) {   // offsets of this block and its left / top / top-left neighbours
-        const int i = tid & 1, left = tid >> 1;
-        const bool use = (i == 0 || (G.g.overlap && brow > 0)) && (!left || (G.g.overlap && bcol > 0));
-        offs[left][i] = use ? G.offsets[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
-    }
-    __syncthreads();
-    const FrameView &src = D.v[VIEW_A];
-    const FrameView &dst = D.v[VIEW_D];
-    if (src.bd == 8) grain_block<uint8_t>(src, dst, G, bcol, brow, lut, offs);
-    else grain_block<uint16_t>(src, dst, G, bcol, brow, lut, offs);
+    for_each_unit(total, L, [&](int u) {
+        const Seg sg = segs[find_seg(segs, nseg, u)];
+        const FrameDesc &D = descs[sg.frame];
+        const GrainDesc &G = D.gr;
+        const int t = u - sg.base;
+        const int bcol = t % sg.nx, brow = t / sg.nx;
+        const int tid = threadIdx.x;
+        for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = G.luts[i >> 8][i & 255];
+        if (tid < 4) {   // offsets of this block and its left / top / top-left neighbours
+            const int i = tid & 1, left = tid >> 1;
+            const bool use = (i == 0 || (G.g.overlap && brow > 0)) && (!left || (G.g.overlap && bcol > 0));
+            offs[left][i] = use ? G.offsets[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
+        }
+        __syncthreads();
+        const FrameView &src = D.v[VIEW_A];
+        const FrameView &dst = D.v[VIEW_D];
+        if (src.bd == 8) grain_block<uint8_t>(src, dst, G, bcol, brow, lut, offs);
+        else grain_block<uint16_t>(src, dst, G, bcol, brow, lut, offs);
+    });
 }
 
-hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
+hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L,
+                              hipStream_t s)
 {
-    if (total) hipLaunchKernelGGL(grain_apply_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
+    if (total)
+        hipLaunchKernelGGL(grain_apply_kernel, dim3(stage_grid((const void *)grain_apply_kernel, total, L.per_wg)),
+                           dim3(256), 0, s, d, segs, nseg, total, L);
     return hipGetLastError();
 }
 

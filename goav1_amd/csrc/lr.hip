@@ -90,7 +90,8 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     else sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb, put);
 }
 
-__global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+__global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
+                                                 int total, StageLaunch L)
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     // Wiener intermediate and self-guided A/B planes share one buffer
@@ -99,23 +100,21 @@ __global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ d
     int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
     static_assert(LR_TW * 64 * 2 <= LR_AB * LR_AB * 8, "Wiener intermediate fits the shared buffer");
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const int t = blockIdx.x - sg.base;
-    const int k = t % sg.nx, st = t / sg.nx;
-    const FrameView &src = D.v[D.lr_src];
-    const FrameView &dst = D.v[D.lr_dst];
-    const int pl = sg.plane;
-    if (src.bd == 8)
-        lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab, Bb);
-    else
-        lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab, Bb);
+    for_each_unit(total, L, [&](int u) {
+        const Seg sg = segs[find_seg(segs, nseg, u)];
+        const FrameDesc &D = descs[sg.frame];
+        const int t = u - sg.base;
+        const int k = t % sg.nx, st = t / sg.nx;
+        const FrameView &src = D.v[D.lr_src];
+        const FrameView &dst = D.v[D.lr_dst];
+        const int pl = sg.plane;
+        if (src.bd == 8)
+            lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab, Bb);
+        else
+            lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab, Bb);
+    });
 }
 
-// Save the deblocked context rows of every stripe: bnd[t][0..1] = rows y0-2,
-// y0-1 (t > 0), bnd[t][2..3] = rows y1, y1+1 (duplicated when y1+1 is past
-// the plane), for stripes with an interior lower seam.  Workgroup = 256
-// columns of one stripe.
 template <typename T>
 __device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneDesc &g, int x, int t)
 {
@@ -150,10 +149,36 @@ hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int tot
     return hipGetLastError();
 }
 
-hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
+hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s)
 {
-    if (total) hipLaunchKernelGGL(lr_kernel, dim3(total), dim3(256), 0, s, d, segs, nseg);
+    if (total)
+        hipLaunchKernelGGL(lr_kernel, dim3(stage_grid((const void *)lr_kernel, total, L.per_wg)), dim3(256), 0, s, d, segs,
+                           nseg, total, L);
     return hipGetLastError();
+}
+
+__global__ void zero_u32_kernel(unsigned int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+int stage_grid(const void *kernel, int total, int per_wg)
+{
+    if (per_wg > 0) return (total + per_wg - 1) / per_wg;
+    static int cus = 0;
+    int dev = 0, per_cu = 0;
+    if (!cus && (hipGetDevice(&dev) != hipSuccess ||
+                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return std::min(total, cus * per_cu);
 }
 
 // Plain copy of one plane's aligned extent: workgroup = 256 samples of a row.

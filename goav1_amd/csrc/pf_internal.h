@@ -99,15 +99,56 @@ __device__ __forceinline__ int find_seg(const Seg *__restrict__ segs, int n, int
     return lo;
 }
 
+// ---- work distribution of a stage launch ---------------------------------
+// per_wg > 0: workgroup b filters the units (tiles) b * per_wg .. b * per_wg +
+// per_wg - 1 (grid = ceil(total / per_wg)); per_wg == 0: a persistent grid of
+// resident workgroups takes units from a device queue (a counter that is zero
+// before the launch) until it runs dry.  Every workgroup leaves the loop on
+// the same shared value, so every wave reaches the exit.
+struct StageLaunch {
+    int per_wg;
+    unsigned int *queue;
+};
+
+template <typename Body>
+__device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, Body &&body)
+{
+    __shared__ int next_unit;
+    if (L.per_wg > 0) {
+        const int u0 = blockIdx.x * L.per_wg;
+        for (int k = 0; k < L.per_wg; k++) {
+            const int u = u0 + k;
+            if (u >= total) return;
+            if (k) __syncthreads();   // the previous unit is done with the LDS
+            body(u);
+        }
+    } else {
+        for (;;) {
+            __syncthreads();           // the previous unit is done with the LDS and next_unit
+            if (threadIdx.x == 0) next_unit = (int)atomicAdd(L.queue, 1u);
+            __syncthreads();
+            const int u = next_unit;
+            if (u >= total) return;
+            body(u);
+        }
+    }
+}
+
+// Grid of a stage launch: ceil(total / per_wg), or the resident workgroups of
+// the kernel (occupancy x compute units, at most total) when persistent.
+int stage_grid(const void *kernel, int total, int per_wg);
+
 // ---- launches (all batched) ------------------------------------------------
 hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,
                           hipStream_t s);
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
-hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
-hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
+hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
 hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s);
-hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L,
+                              hipStream_t s);
+hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t s);
 
 // ---- encoder CDEF search (single frame) ----------------------------------
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,

@@ -27,7 +27,7 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_infer", "av1pf_nn_backward",
            "av1pf_nn_adam",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
-           "av1pf_profile", "av1pf_profile_read"]
+           "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned"] + \
     [f"reserved{i}" for i in range(13, 16)]
@@ -57,6 +57,10 @@ def load():
     lib.av1pf_apply.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
     lib.av1pf_apply_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
     lib.av1pf_sync.argtypes = [C.c_void_p, C.c_void_p]
+    lib.av1pf_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    lib.av1pf_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request),
+                                   C.POINTER(C.c_int32), C.c_void_p]
+    lib.av1pf_set_stage_mode.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
     lib.av1pf_deblock.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.DeblockParams), C.c_void_p]
     lib.av1pf_cdef.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.CdefParams), C.c_void_p]
     lib.av1pf_lr_save_boundaries.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_int, C.c_void_p]
@@ -249,6 +253,25 @@ class PostFilterRunner:
         fa = (abi.Frame * n)(*[f.abi for f in frames])
         ra = (abi.Request * n)(*[s.req for s in sides])
         self._check(self.lib.av1pf_apply_batch(self.h, n, fa, ra, self._stream(stream)))
+
+    def set_graphs(self, enable):
+        """Capture each batch shape class as a HIP graph and replay it."""
+        self._check(self.lib.av1pf_set_graphs(self.h, int(bool(enable))))
+
+    def autotune(self, frames, sides, stream=None):
+        """apply_batch that also picks each tunable stage's work distribution
+        for this shape class; returns {stage: tiles per workgroup, 0 = queue}."""
+        n = len(frames)
+        fa = (abi.Frame * n)(*[f.abi for f in frames])
+        ra = (abi.Request * n)(*[s.req for s in sides])
+        modes = (C.c_int32 * 3)()
+        self._check(self.lib.av1pf_autotune(self.h, n, fa, ra, modes, self._stream(stream)))
+        return {"cdef": modes[0], "lr": modes[1], "grain_apply": modes[2]}
+
+    def set_stage_mode(self, stage, mode):
+        """Force a stage's work distribution (abi.STAGE_CDEF / _LR / _FILMGRAIN;
+        1, 2, 4 tiles per workgroup, 0 = persistent queue, -1 = automatic)."""
+        self._check(self.lib.av1pf_set_stage_mode(self.h, int(stage), int(mode)))
 
     # stage-level entry points (the stage functions the chain composes)
     def deblock(self, frame, side, stream=None):

@@ -190,6 +190,27 @@ int  av1pf_apply(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *
  * runs frames one at a time).  Equivalent to n av1pf_apply calls.           */
 int  av1pf_apply_batch(av1pf_ctx *ctx, int n, const av1pf_frame *frames, const av1pf_request *reqs, void *stream);
 
+/* ---- shape classes (no counterpart in the reference) -----------------------
+ * A shape class is everything a batch's launches depend on besides data:
+ * the frame count and, per stage, the number of tiles and of segments.
+ * av1pf_set_graphs(ctx, 1): each class is captured once as a HIP graph
+ * (descriptor upload, side-stream grain templates, every stage launch) and
+ * replayed on later calls of the class with the new descriptors; two graph
+ * instances per class alternate, so a call waits only for the replay two
+ * calls back.  Off while per-stage profiling is on.
+ * av1pf_autotune: av1pf_apply_batch (same results) that also times, for
+ * CDEF, LR and film-grain apply, each work distribution of the stage's tiles
+ * (1, 2 or 4 tiles per workgroup, or persistent workgroups taking tiles from
+ * a device queue) on the stage's own inputs and keeps the fastest for the
+ * batch's class; modes[3] (may be NULL) receives the choice (0 = queue).
+ * Synchronous.  Later calls of the class, captured graphs included, use it.
+ * av1pf_set_stage_mode forces one stage's mode for every class (-1 clears);
+ * stage is AV1PF_STAGE_CDEF, _LR or _FILMGRAIN.                              */
+int  av1pf_set_graphs(av1pf_ctx *ctx, int enable);
+int  av1pf_autotune(av1pf_ctx *ctx, int n, const av1pf_frame *frames, const av1pf_request *reqs, int32_t *modes,
+                    void *stream);
+int  av1pf_set_stage_mode(av1pf_ctx *ctx, uint32_t stage, int mode);
+
 /* Wait for all work the handle has queued on `stream`.                      */
 int  av1pf_sync(av1pf_ctx *ctx, void *stream);
 

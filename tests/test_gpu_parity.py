@@ -129,15 +129,12 @@ def _frames_from(fr, devs):
     return out
 
 
-def test_mixed_batch(runner):
-    """One apply_batch over frames of mixed size, depth, chroma format and
-    stage set must equal the oracle frame by frame."""
-    from goav1_amd.lib import DeviceFrame, DeviceSideData
-    import torch
+def mixed_frames():
+    """14 seeded frames of mixed size, depth, chroma format and stage set."""
     rng = np.random.default_rng(4242)
     sizes = [(200, 136), (136, 72), (66, 66), (426, 240), (90, 50), (320, 180)]
     stage_sets = STAGE_SETS + [abi.STAGES_ALL, abi.STAGE_FILMGRAIN, abi.STAGE_CDEF | abi.STAGE_FILMGRAIN]
-    frames, devs, disps, sides = [], [], [], []
+    frames = []
     for i in range(14):
         w, h = sizes[i % len(sizes)]
         bd = 8 if i % 3 else 10
@@ -145,21 +142,99 @@ def test_mixed_batch(runner):
         fr = small_frame(9000 + i, w, h, bd, ss, lr_unit=int(rng.choice([64, 128])))
         fr.grain = synth.random_grain(rng, bd, lag=int(rng.integers(0, 4)))
         fr.stages = stage_sets[i % len(stage_sets)]
+        frames.append(fr)
+    return frames
+
+
+def device_batch(frames):
+    from goav1_amd.lib import DeviceFrame, DeviceSideData
+    devs, disps, sides = [], [], []
+    for fr in frames:
         d = DeviceFrame(fr)
         disp = DeviceFrame.empty_like(fr) if fr.stages & abi.STAGE_FILMGRAIN else None
-        frames.append(fr)
         devs.append(d)
         disps.append(disp)
         sides.append(DeviceSideData(fr, display=disp, stages=fr.stages))
+    return devs, disps, sides
+
+
+def check_batch(frames, devs, disps, wants, what):
+    for i, fr in enumerate(frames):
+        want, wdisp = wants[i]
+        assert_same(_frames_from(fr, devs[i]), want, f"{what} frame {i} stages {fr.stages:#x}")
+        if disps[i] is not None:
+            assert_same(_frames_from(fr, disps[i]), wdisp, f"{what} frame {i} display")
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    frames = mixed_frames()
+    return frames, [orc.run_chain(fr, fr.stages) for fr in frames]
+
+
+def test_mixed_batch(runner, mixed):
+    """One apply_batch over frames of mixed size, depth, chroma format and
+    stage set must equal the oracle frame by frame."""
+    import torch
+    frames, wants = mixed
+    devs, disps, sides = device_batch(frames)
     print(f"mixed batch of {len(frames)} frames", flush=True)
     runner.apply_batch(devs, sides)
     runner.sync()
     torch.cuda.synchronize()
-    for i, fr in enumerate(frames):
-        want, wdisp = orc.run_chain(fr, fr.stages)
-        assert_same(_frames_from(fr, devs[i]), want, f"batch frame {i} stages {fr.stages:#x}")
-        if disps[i] is not None:
-            assert_same(_frames_from(fr, disps[i]), wdisp, f"batch frame {i} display")
+    check_batch(frames, devs, disps, wants, "batch")
+
+
+@pytest.mark.parametrize("mode", [2, 4, 0])
+def test_mixed_batch_work_distribution(runner, mixed, mode):
+    """CDEF, LR and film-grain apply with several tiles per workgroup or
+    persistent workgroups on the device queue give the same frames."""
+    import torch
+    frames, wants = mixed
+    for st in (abi.STAGE_CDEF, abi.STAGE_LR, abi.STAGE_FILMGRAIN):
+        runner.set_stage_mode(st, mode)
+    try:
+        for rep in range(2):
+            devs, disps, sides = device_batch(frames)
+            print(f"mixed batch, stage mode {mode}, run {rep}", flush=True)
+            runner.apply_batch(devs, sides)
+            runner.sync()
+            torch.cuda.synchronize()
+            check_batch(frames, devs, disps, wants, f"mode {mode} run {rep}")
+    finally:
+        for st in (abi.STAGE_CDEF, abi.STAGE_LR, abi.STAGE_FILMGRAIN):
+            runner.set_stage_mode(st, -1)
+
+
+def test_mixed_batch_autotune_and_graph_replays(runner, mixed):
+    """autotune returns the oracle's frames and picks a mode per stage; then
+    the class is captured as a graph and replayed three times on fresh copies,
+    with other device work and a single-frame call (another class) between
+    replays; every replay equals the oracle."""
+    import torch
+    frames, wants = mixed
+    devs, disps, sides = device_batch(frames)
+    print("mixed batch autotune", flush=True)
+    modes = runner.autotune(devs, sides)
+    runner.sync()
+    torch.cuda.synchronize()
+    assert set(modes.values()) <= {0, 1, 2, 4}, modes
+    check_batch(frames, devs, disps, wants, "autotune")
+    runner.set_graphs(True)
+    try:
+        for rep in range(3):
+            devs, disps, sides = device_batch(frames)
+            print(f"mixed batch graph replay {rep}", flush=True)
+            runner.apply_batch(devs, sides)
+            runner.sync()
+            torch.cuda.synchronize()
+            check_batch(frames, devs, disps, wants, f"graph replay {rep}")
+            junk = torch.arange(1 << 16, device="cuda").float().sum()   # eager work between replays
+            out, _ = gpu_run(runner, frames[0])                          # another shape class
+            assert_same(out, wants[0][0], "single-frame call between replays")
+            assert junk.item() > 0
+    finally:
+        runner.set_graphs(False)
 
 
 @pytest.mark.parametrize("bd", [8, 10])
