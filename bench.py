@@ -3,8 +3,9 @@
 Workload (one step): the 1920x1080 10-bit 4:2:0 moving-rectangle sequence, 8
 frames, each through deblock -> CDEF -> loop restoration (Wiener + SGR units)
 -> film grain into a display frame, via the C ABI (libav1pf.so), one
-av1pf_apply_batch call per step.  Inputs are resident in HBM before the timed
-region: every timed step filters its own pristine copy of the 8 pre-filter
+av1pf_apply_batch call per step, with the work distribution the autotuner
+picked for this batch (av1pf_autotune, run once before timing).  Inputs are
+resident in HBM before the timed region: every timed step filters its own pristine copy of the 8 pre-filter
 frames (the chain works in place).  The same steps run once more with one
 av1pf_apply per frame (per_frame_apply_mpx_s).
 
@@ -214,7 +215,7 @@ def batch_leg(runner, steps, nframes):
             row.append(c)
         return row
 
-    def run(nsteps, profile=False):
+    def run(nsteps, profile=False, per_frame=False):
         rows = [copies() for _ in range(nsteps)]
         torch.cuda.synchronize()
         if profile:
@@ -223,7 +224,11 @@ def batch_leg(runner, steps, nframes):
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for row in rows:
-            runner.apply_batch(row, sides)
+            if per_frame:
+                for f, sd in zip(row, sides):
+                    runner.apply(f, sd)
+            else:
+                runner.apply_batch(row, sides)
         ev1.record(stream)
         torch.cuda.synchronize()
         runner.sync()
@@ -235,6 +240,19 @@ def batch_leg(runner, steps, nframes):
     run(2)
     ms, _ = run(steps)
     _, prof = run(2, profile=True)
+    # the autotuner's work distribution for this class, then graph replay
+    modes = runner.autotune(copies(), sides)
+    runner.sync()
+    run(1)
+    ms_tuned, _ = run(steps)
+    runner.set_graphs(True)
+    run(2)
+    ms_graph, _ = run(steps)
+    run(1, per_frame=True)
+    ms_frames_graph, _ = run(max(1, steps // 2), per_frame=True)
+    runner.set_graphs(False)
+    run(1, per_frame=True)
+    ms_frames, _ = run(max(1, steps // 2), per_frame=True)
     luma = sum(fr.w * fr.h for fr in frames)
     samples = sum(fr.samples for fr in frames)
     stage_bytes = sum(2 * fr.samples * (1 if fr.bd == 8 else 2) for fr in frames)
@@ -245,6 +263,10 @@ def batch_leg(runner, steps, nframes):
                         "deblock/CDEF/LR parameters, deblock+CDEF+LR, one av1pf_apply_batch per step",
             "mpx_s": round(luma / (ms / 1e3) / 1e6, 1), "msamples_per_s": round(samples / (ms / 1e3) / 1e6, 1),
             "ms_per_batch": round(ms, 3), "luma_mpx_per_batch": round(luma / 1e6, 2),
+            "autotuned_modes": modes, "tuned_mpx_s": round(luma / (ms_tuned / 1e3) / 1e6, 1),
+            "graph_tuned_mpx_s": round(luma / (ms_graph / 1e3) / 1e6, 1),
+            "one_call_per_frame_mpx_s": round(luma / (ms_frames / 1e3) / 1e6, 1),
+            "one_call_per_frame_graph_mpx_s": round(luma / (ms_frames_graph / 1e3) / 1e6, 1),
             "algorithmic_bytes_per_stage_pass": stage_bytes, "stages": stages,
             "generation_s": round(gen_s, 1)}
 
@@ -319,6 +341,10 @@ def main():
     # warm-up, then the timed steps: one apply_batch per step covers the 8
     # frames.  The K timed steps run as one region, three times over fresh
     # copies; the median region is reported.
+    # the autotuner picks the work distribution of CDEF, LR and grain apply for
+    # this batch's shape class (on a pristine copy, outside the timed region)
+    tuned_modes = runner.autotune(pristine(1)[0], sides)
+    runner.sync()
     timed(pristine(args.warmup), True, False)
     regions = []
     for _ in range(3):
@@ -371,6 +397,7 @@ def main():
                    "chroma": "4:2:0"},
         "msamples_per_s": round(samples * args.steps / (gpu_ms / 1e3) / 1e6, 2),
         "per_frame_apply_mpx_s": round(luma_px * args.steps / (frame_ms / 1e3) / 1e6, 2),
+        "autotuned_modes": tuned_modes,
         "profiled_pass_ms_per_step": round(prof_ms / args.steps, 4),
         "timed_regions_ms": [round(r[0], 3) for r in regions],
         "wall_s_timed": round(wall, 4),
