@@ -123,7 +123,7 @@ struct av1pf_ctx {
 namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
-       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED };
+       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -1132,6 +1132,26 @@ int av1pf_lr_search_masked(av1pf_ctx *c, const av1pf_frame *src, const av1pf_fra
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_LR_SEARCH, s, 2.0 * W * H);
     HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, unit_mask, units, sse, s));
+    return AV1PF_OK;
+}
+
+int av1pf_lr_search_exhaustive(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                               av1pf_lr_unit *units, uint64_t *sse, int8_t *fit, void *stream)
+{
+    if (!c || !units || !sse || !fit) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_frame(c, src);
+    if (rc) return rc;
+    if ((rc = validate_frame(c, recon))) return rc;
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != 8 || recon->bit_depth != 8)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "lr search needs two 8-bit frames of one size");
+    if (unit_size != 32 && unit_size != 64 && unit_size != 128 && unit_size != 256)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "lr unit size must be 32, 64, 128 or 256");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int W = recon->width, H = recon->height;
+    const int hz = std::max((W + (unit_size >> 1)) / unit_size, 1), vt = std::max((H + (unit_size >> 1)) / unit_size, 1);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_LR_SEARCH_EXH, s, 2.0 * W * H);
+    HIPCHK(c, launch_lr_search_exhaustive(view_of(src).p[0], view_of(recon).p[0], unit_size, hz, vt, units, sse, fit, s));
     return AV1PF_OK;
 }
 

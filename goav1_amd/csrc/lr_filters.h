@@ -141,21 +141,26 @@ __device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, 
     }
 }
 
-template <typename Sink>
-__device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd, int set, const int8_t *xqd,
-                                         int32_t *Ab, int32_t *Bb, Sink sink)
+// Lane layout of the per-pixel filter stages: lane x of row group ry handles
+// column x of rows ry, ry + rstep, ... (at most SGR_PPT rows).
+#define SGR_PPT 16
+struct LaneRows {
+    int x, ry, rstep;
+};
+__device__ __forceinline__ LaneRows lane_rows(int w)
 {
-    const int tid = threadIdx.x;
-    const int32_t mx = (1 << bd) - 1;
-    int xq0, xq1;
-    if (kSgrR[set][0] == 0) { xq0 = 0; xq1 = 128 - xqd[1]; }
-    else if (kSgrR[set][1] == 0) { xq0 = xqd[0]; xq1 = 0; }
-    else { xq0 = xqd[0]; xq1 = 128 - xq0 - xqd[1]; }
-    constexpr int PPT = 16;   // rows per lane: at most 64 rows / (256 / 64 lanes)
-    const int lpr = lanes_for(w), x = tid & (lpr - 1), rstep = 256 / lpr, ry = tid / lpr;
-    int32_t f0[PPT], f1[PPT];
+    const int lpr = lanes_for(w);
+    return { (int)threadIdx.x & (lpr - 1), (int)threadIdx.x / lpr, 256 / lpr };
+}
+
+// The two self-guided outputs of set `set` for this lane's pixels (Q4; zero
+// for a pass of radius 0): f0 from the r = 2 pass, f1 from the r = 1 pass.
+__device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
+                                            int32_t (&f0)[SGR_PPT], int32_t (&f1)[SGR_PPT])
+{
+    const LaneRows L = lane_rows(w);
 #pragma unroll
-    for (int j = 0; j < PPT; j++) { f0[j] = 0; f1[j] = 0; }
+    for (int j = 0; j < SGR_PPT; j++) { f0[j] = 0; f1[j] = 0; }
     for (int pass = 0; pass < 2; pass++) {
         const int r = kSgrR[set][pass];
         if (r == 0) continue;
@@ -164,10 +169,10 @@ __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd
         if (pass == 0) sgr_ab<2, 2>(O, w, h, bd, s, Ab, Bb);
         else sgr_ab<1, 1>(O, w, h, bd, s, Ab, Bb);
         __syncthreads();
-        if (x < w) {
+        if (L.x < w) {
 #pragma unroll
-            for (int j = 0; j < PPT; j++) {
-                const int y = ry + rstep * j;
+            for (int j = 0; j < SGR_PPT; j++) {
+                const int y = L.ry + L.rstep * j, x = L.x;
                 if (y >= h) break;
                 const int32_t uu = O[y * LR_TW + x];
                 const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
@@ -193,17 +198,44 @@ __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd
             }
         }
     }
-    if (x < w) {
+}
+
+// xq of a set from its coded xqd (restoration/selfguided.go:111-126).
+__device__ __forceinline__ void sgr_xq(int set, const int8_t *xqd, int &xq0, int &xq1)
+{
+    if (kSgrR[set][0] == 0) { xq0 = 0; xq1 = 128 - xqd[1]; }
+    else if (kSgrR[set][1] == 0) { xq0 = xqd[0]; xq1 = 0; }
+    else { xq0 = xqd[0]; xq1 = 128 - xq0 - xqd[1]; }
+}
+
+// The projection of one pixel: u + xq0 (f0 - u) + xq1 (f1 - u) in Q11, the
+// int16 wrap, then the clip.
+__device__ __forceinline__ int sgr_project(int u, int32_t f0, int32_t f1, int xq0, int xq1, int32_t mx)
+{
+    const int32_t uu = (int32_t)u << 4;
+    int32_t v = uu << 7;
+    v += xq0 * (f0 - uu);
+    v += xq1 * (f1 - uu);
+    const int32_t wv = (int16_t)r2s(v, 11);
+    return min(max(wv, 0), mx);
+}
+
+template <typename Sink>
+__device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd, int set, const int8_t *xqd,
+                                         int32_t *Ab, int32_t *Bb, Sink sink)
+{
+    int xq0, xq1;
+    sgr_xq(set, xqd, xq0, xq1);
+    int32_t f0[SGR_PPT], f1[SGR_PPT];
+    sgr_filters(O, w, h, bd, set, Ab, Bb, f0, f1);
+    const LaneRows L = lane_rows(w);
+    const int32_t mx = (1 << bd) - 1;
+    if (L.x < w) {
 #pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int y = ry + rstep * j;
+        for (int j = 0; j < SGR_PPT; j++) {
+            const int y = L.ry + L.rstep * j;
             if (y >= h) break;
-            const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
-            int32_t v = uu << 7;
-            v += xq0 * (f0[j] - uu);
-            v += xq1 * (f1[j] - uu);
-            const int32_t wv = (int16_t)r2s(v, 11);
-            sink(y, x, min(max(wv, 0), mx));
+            sink(y, L.x, sgr_project(O[y * LR_TW + L.x], f0[j], f1[j], xq0, xq1, mx));
         }
     }
 }

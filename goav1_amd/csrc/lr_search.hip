@@ -155,6 +155,261 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
     }
 }
 
+// ---- exhaustive superset mode ------------------------------------------------
+// No counterpart in the reference (its search is the curated one above).  Per
+// unit of at most 64x64: the baseline; the six curated Wiener triples scored
+// exactly; a least-squares Wiener fit (symmetric Q7 taps summing to 128,
+// alternating: horizontal taps given the vertical ones, then vertical given
+// horizontal, twice from the identity; each 3x3 system from int64 statistics
+// of the 7x7 windows, solved by Cramer's rule in double, rounded
+// floor(x + 0.5) and clamped to [-5,10], [-23,8], [-17,46]); the four curated
+// SGR presets; and all 16 SGR sets with a least-squares projection (the xq
+// minimising sum (2^7 (16 s - 16 u) - xq0 (f0 - 16 u) - xq1 (f1 - 16 u))^2,
+// coded as xqd in [-96,31] x [-32,95] as libaom codes it), every candidate
+// scored exactly with the decoder's filters.  Candidates in that order replace
+// the running best when sse + w*h < best, so the decision is never worse than
+// the curated one.  sse[unit][28]; fit[unit][38] = Wiener v[3], h[3], then the
+// fitted xqd[2] of each set.
+__device__ __constant__ int8_t kTapMin[3] = { -5, -23, -17 };
+__device__ __constant__ int8_t kTapMax[3] = { 10, 8, 46 };
+
+__device__ __forceinline__ int round_clamp(double x, int lo, int hi)
+{
+    const int v = (int)floor(x + 0.5);
+    return min(max(v, lo), hi);
+}
+
+// 3x3 symmetric system (m00 m01 m02 m11 m12 m22) by Cramer's rule; the same
+// operations in the same order as the CPU restatement, without contraction.
+__device__ void solve_taps(const int64_t *M, const int64_t *R, int8_t *out)
+{
+#pragma clang fp contract(off)
+    const double a = (double)M[0], b = (double)M[1], c = (double)M[2], d = (double)M[3], e = (double)M[4],
+                 f = (double)M[5];
+    const double r0 = (double)R[0], r1 = (double)R[1], r2 = (double)R[2];
+    const double c00 = d * f - e * e, c01 = b * f - e * c, c02 = b * e - d * c;
+    const double det = a * c00 - b * c01 + c * c02;
+    if (det == 0.0) {
+        out[0] = out[1] = out[2] = 0;
+        return;
+    }
+    const double x0 = (r0 * c00 - b * (r1 * f - e * r2) + c * (r1 * e - d * r2)) / det;
+    const double x1 = (a * (r1 * f - e * r2) - r0 * c01 + c * (b * r2 - r1 * c)) / det;
+    const double x2 = (a * (d * r2 - r1 * e) - b * (b * r2 - r1 * c) + r0 * c02) / det;
+    out[0] = (int8_t)round_clamp(x0, kTapMin[0], kTapMax[0]);
+    out[1] = (int8_t)round_clamp(x1, kTapMin[1], kTapMax[1]);
+    out[2] = (int8_t)round_clamp(x2, kTapMin[2], kTapMax[2]);
+}
+
+__device__ void sgr_fit(int set, const int64_t *st, int8_t *xqd)
+{
+#pragma clang fp contract(off)
+    const int r0 = kSgrR[set][0], r1 = kSgrR[set][1];
+    double xq0 = 0.0, xq1 = 0.0;
+    if (r0 && r1) {
+        const double a = (double)st[0], b = (double)st[1], d = (double)st[2], c0 = (double)st[3], c1 = (double)st[4];
+        const double det = a * d - b * b;
+        if (det != 0.0) {
+            xq0 = (c0 * d - c1 * b) / det;
+            xq1 = (c1 * a - c0 * b) / det;
+        }
+    } else if (r0) {
+        if (st[0]) xq0 = (double)st[3] / (double)st[0];
+    } else {
+        if (st[2]) xq1 = (double)st[4] / (double)st[2];
+    }
+    if (r0 && r1) {
+        xqd[0] = (int8_t)round_clamp(xq0, -96, 31);
+        xqd[1] = (int8_t)round_clamp(128.0 - (double)xqd[0] - floor(xq1 + 0.5), -32, 95);
+    } else if (r0) {
+        xqd[0] = (int8_t)round_clamp(xq0, -96, 31);
+        xqd[1] = (int8_t)min(max(128 - xqd[0], -32), 95);
+    } else {
+        xqd[0] = 0;
+        xqd[1] = (int8_t)round_clamp(128.0 - floor(xq1 + 0.5), -32, 95);
+    }
+}
+
+__global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
+                                                            av1pf_lr_unit *out, uint64_t *sse_out, int8_t *fit_out)
+{
+    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ uint8_t srcT[64 * 64];
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];   // Wiener intermediate, then SGR A/B
+    __shared__ uint64_t red[4];
+    __shared__ int8_t taps_s[2][3];
+    __shared__ int8_t xqd_s[2];
+    int32_t *Bb = reinterpret_cast<int32_t *>(work);
+    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
+    const int tid = threadIdx.x;
+    const int u = blockIdx.x;
+    const int col = u % hz, row = u / hz;
+    const int W = R.w, H = R.h;
+    const int x0 = col * us, x1 = col == hz - 1 ? W : x0 + us;
+    int y0 = row * us, y1 = row == vt - 1 ? H : y0 + us;
+    if (y0 > 0) y0 -= 8;
+    if (y1 < H) y1 -= 8;
+    const int w = x1 - x0, h = y1 - y0;
+    uint64_t *e = sse_out + (size_t)u * 28;
+    int8_t *ft = fit_out + (size_t)u * 38;
+    for (int i = tid; i < 28; i += 256) e[i] = ~0ull;
+    for (int i = tid; i < 38; i += 256) ft[i] = 0;
+    uint64_t b = 0;
+    for (int i = tid; i < w * h; i += 256) {
+        const int y = y0 + i / w, x = x0 + i % w;
+        const int d = (int)row_ptr<uint8_t>(S, y)[x] - (int)row_ptr<uint8_t>(R, y)[x];
+        b += (uint64_t)(d * d);
+    }
+    const uint64_t base = wg_sum(b, red);
+    av1pf_lr_unit res;
+    for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
+    res.type = AV1PF_LR_NONE;
+    if (base != 0 && w <= 64 && h <= 64) {
+        const int tw = w + 6, th = h + 6;
+        for (int i = tid; i < tw * th; i += 256) {
+            const int r = i / tw, c = i - r * tw;
+            const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
+            tile[r * LR_TW + c] = row_ptr<uint8_t>(R, y)[x];
+        }
+        for (int i = tid; i < w * h; i += 256) {
+            const int r = i / w, c = i - r * w;
+            srcT[r * 64 + c] = row_ptr<uint8_t>(S, y0 + r)[x0 + c];
+        }
+        __syncthreads();
+        const uint16_t *O = tile + 3 * LR_TW + 3;
+        const uint64_t wh = (uint64_t)(w * h);
+        uint64_t best = base;
+        const LaneRows L = lane_rows(w);
+        // ---- Wiener fit: alternating least squares over this lane's pixels
+        int8_t fv[3] = { 0, 0, 0 }, fh[3] = { 0, 0, 0 };
+        for (int half = 0; half < 4; half++) {
+            const bool horiz = (half & 1) == 0;   // solve H given V, then V given H
+            const int8_t *fx = horiz ? fv : fh;
+            const int ftp[7] = { fx[0], fx[1], fx[2], 128 - 2 * (fx[0] + fx[1] + fx[2]), fx[2], fx[1], fx[0] };
+            int64_t st[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+            if (L.x < w) {
+                for (int j = 0; j < SGR_PPT; j++) {
+                    const int y = L.ry + L.rstep * j, x = L.x;
+                    if (y >= h) break;
+                    int64_t Y[7];
+#pragma unroll
+                    for (int k = 0; k < 7; k++) {
+                        int64_t acc = 0;
+#pragma unroll
+                        for (int q = 0; q < 7; q++) {
+                            const int yy = horiz ? y + q - 3 : y + k - 3, xx = horiz ? x + k - 3 : x + q - 3;
+                            acc += (int64_t)ftp[q] * O[yy * LR_TW + xx];
+                        }
+                        Y[k] = acc;
+                    }
+                    const int64_t z0 = Y[0] + Y[6] - 2 * Y[3], z1 = Y[1] + Y[5] - 2 * Y[3], z2 = Y[2] + Y[4] - 2 * Y[3];
+                    const int64_t t = ((int64_t)srcT[y * 64 + x] << 14) - 128 * Y[3];
+                    st[0] += z0 * z0; st[1] += z0 * z1; st[2] += z0 * z2;
+                    st[3] += z1 * z1; st[4] += z1 * z2; st[5] += z2 * z2;
+                    st[6] += z0 * t; st[7] += z1 * t; st[8] += z2 * t;
+                }
+            }
+            int64_t tot[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) tot[k] = (int64_t)wg_sum((uint64_t)st[k], red);
+            if (tid == 0) solve_taps(tot, tot + 6, taps_s[horiz ? 1 : 0]);
+            __syncthreads();
+            int8_t *dstt = horiz ? fh : fv;
+            for (int k = 0; k < 3; k++) dstt[k] = taps_s[horiz ? 1 : 0][k];
+            __syncthreads();
+        }
+        if (tid == 0)
+            for (int k = 0; k < 3; k++) { ft[k] = fv[k]; ft[3 + k] = fh[k]; }
+        // ---- Wiener: six curated triples, then the fit, all exact
+        uint64_t acc = 0;
+        auto err = [&](int y, int x, int v) {
+            const int d = (int)srcT[y * 64 + x] - v;
+            acc += (uint64_t)(d * d);
+        };
+        for (int c = 0; c < 7; c++) {
+            const int8_t *vt = c < 6 ? kWienerCand[c] : fv;
+            const int8_t *ht = c < 6 ? kWienerCand[c] : fh;
+            acc = 0;
+            __syncthreads();
+            wiener_tile(O, w, h, 8, vt, ht, tmp, err);
+            const uint64_t sc = wg_sum(acc, red);
+            if (tid == 0) e[1 + c] = sc;
+            if (sc + wh < best) {
+                best = sc;
+                for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
+                res.type = AV1PF_LR_WIENER;
+                for (int k = 0; k < 3; k++) { res.wiener_v[k] = vt[k]; res.wiener_h[k] = ht[k]; }
+            }
+        }
+        // ---- SGR: per set the curated preset (when it has one), then the fit
+        for (int set = 0; set < 16; set++) {
+            int32_t f0[SGR_PPT], f1[SGR_PPT];
+            __syncthreads();
+            sgr_filters(O, w, h, 8, set, Ab, Bb, f0, f1);
+            const int r0 = kSgrR[set][0], r1 = kSgrR[set][1];
+            int64_t st[5] = { 0, 0, 0, 0, 0 };
+            if (L.x < w) {
+#pragma unroll
+                for (int j = 0; j < SGR_PPT; j++) {
+                    const int y = L.ry + L.rstep * j, x = L.x;
+                    if (y >= h) break;
+                    const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
+                    const int64_t d0 = r0 ? f0[j] - uu : 0, d1 = r1 ? f1[j] - uu : 0;
+                    const int64_t ee = (((int64_t)srcT[y * 64 + x] << 4) - uu) << 7;
+                    st[0] += d0 * d0; st[1] += d0 * d1; st[2] += d1 * d1; st[3] += d0 * ee; st[4] += d1 * ee;
+                }
+            }
+            int64_t tot[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) tot[k] = (int64_t)wg_sum((uint64_t)st[k], red);
+            if (tid == 0) sgr_fit(set, tot, xqd_s);
+            __syncthreads();
+            int8_t fx[2] = { xqd_s[0], xqd_s[1] };
+            if (tid == 0) { ft[6 + 2 * set] = fx[0]; ft[7 + 2 * set] = fx[1]; }
+            int p = -1;
+            for (int q = 0; q < 4; q++)
+                if (kSgrPreset[q] == set) p = q;
+            for (int k = p >= 0 ? 0 : 1; k < 2; k++) {
+                const int8_t *xq = k == 0 ? kSgrXqd[p] : fx;
+                int xq0, xq1;
+                sgr_xq(set, xq, xq0, xq1);
+                uint64_t a2 = 0;
+                if (L.x < w) {
+#pragma unroll
+                    for (int j = 0; j < SGR_PPT; j++) {
+                        const int y = L.ry + L.rstep * j, x = L.x;
+                        if (y >= h) break;
+                        const int d = (int)srcT[y * 64 + x] - sgr_project(O[y * LR_TW + x], f0[j], f1[j], xq0, xq1, 255);
+                        a2 += (uint64_t)(d * d);
+                    }
+                }
+                const uint64_t sc = wg_sum(a2, red);
+                if (tid == 0) e[k == 0 ? 8 + p : 12 + set] = sc;
+                if (sc + wh < best) {
+                    best = sc;
+                    for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
+                    res.type = AV1PF_LR_SGRPROJ;
+                    res.sgr_set = (uint8_t)set;
+                    res.sgr_xqd[0] = xq[0];
+                    res.sgr_xqd[1] = xq[1];
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        out[u] = res;
+        e[0] = base;
+    }
+}
+
+hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt,
+                                       av1pf_lr_unit *out, uint64_t *sse, int8_t *fit, hipStream_t s)
+{
+    hipLaunchKernelGGL(lr_search_exh_kernel, dim3(hz * vt), dim3(256), 0, s, src, rec, us, hz, vt, out, sse, fit);
+    return hipGetLastError();
+}
+
 hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
                             av1pf_lr_unit *out, uint64_t *sse, hipStream_t s)
 {

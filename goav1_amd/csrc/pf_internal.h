@@ -159,6 +159,8 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
 size_t finish_state_bytes();
 hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
                             av1pf_lr_unit *out, uint64_t *sse, hipStream_t s);
+hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt,
+                                       av1pf_lr_unit *out, uint64_t *sse, int8_t *fit, hipStream_t s);
 hipError_t launch_lr_mask(const float *logits, int nhfb, int nvfb, int hz, int vt, float thr, uint8_t *mask,
                           hipStream_t s);
 constexpr int kMaxSearchBlocks = 4096;   // filter blocks the device selection holds

@@ -27,10 +27,11 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_nn_get_params", "av1pf_nn_tiles", "av1pf_nn_forward", "av1pf_nn_infer", "av1pf_nn_backward",
            "av1pf_nn_adam",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
-           "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode"]
+           "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode",
+           "av1pf_lr_search_exhaustive"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
-               "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned"] + \
-    [f"reserved{i}" for i in range(13, 16)]
+               "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
+               "lr_search_exhaustive"] + [f"reserved{i}" for i in range(14, 16)]
 PROFILE_STAGES = 16
 
 
@@ -58,6 +59,8 @@ def load():
     lib.av1pf_apply_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
     lib.av1pf_sync.argtypes = [C.c_void_p, C.c_void_p]
     lib.av1pf_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    lib.av1pf_lr_search_exhaustive.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_int,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.av1pf_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request),
                                    C.POINTER(C.c_int32), C.c_void_p]
     lib.av1pf_set_stage_mode.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
@@ -318,6 +321,11 @@ class PostFilterRunner:
         self._check(self.lib.av1pf_search_learned(self.h, net.h, C.byref(src.abi), C.byref(rec.abi), noskip_ptr,
                                                   int(damping), C.c_uint64(int(lam)), float(lr_thr), dec_ptr,
                                                   idx_ptr, lr_units_ptr, cand_ptr, conf_ptr, self._stream(stream)))
+
+    def lr_search_exhaustive(self, src, rec, unit_size, units_ptr, sse_ptr, fit_ptr, stream=None):
+        """Exhaustive superset LR search: unit records, sse[n][28], fit[n][38] int8."""
+        self._check(self.lib.av1pf_lr_search_exhaustive(self.h, C.byref(src.abi), C.byref(rec.abi), int(unit_size),
+                                                        units_ptr, sse_ptr, fit_ptr, self._stream(stream)))
 
     def lr_search_masked(self, src, rec, unit_size, mask_ptr, units_ptr, sse_ptr=None, stream=None):
         self._check(self.lib.av1pf_lr_search_masked(self.h, C.byref(src.abi), C.byref(rec.abi), int(unit_size),

@@ -278,6 +278,19 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
  * reference computes nothing.  Device pointers; asynchronous.               */
 int  av1pf_lr_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
                      av1pf_lr_unit *units, uint64_t *sse, void *stream);
+/* The exhaustive superset mode (no counterpart in the reference): per unit
+ * of at most 64x64, the six curated Wiener triples scored exactly, a
+ * least-squares Wiener fit (symmetric Q7 taps, alternating H/V solves from
+ * int64 window statistics, rounded and clamped to the tap ranges), the four
+ * curated SGR presets and all 16 SGR sets with a least-squares projection
+ * (coded xqd clamped to [-96,31] x [-32,95]); every candidate scored with the
+ * decoder's filters and taken, in that order, when sse + w*h beats the
+ * running best, so the decision is never worse than av1pf_lr_search's.
+ * sse[unit][28]: 0 baseline, 1..6 curated Wiener, 7 fitted Wiener, 8..11
+ * curated SGR presets, 12..27 SGR sets 0..15 fitted (UINT64_MAX when not
+ * scored); fit[unit][38]: fitted Wiener v[3], h[3], then xqd[2] per set.   */
+int  av1pf_lr_search_exhaustive(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
+                                av1pf_lr_unit *units, uint64_t *sse, int8_t *fit, void *stream);
 /* The same with unit_mask[hz * vt] (device): units with mask 0 are written
  * NONE without being scored (their sse entries are UINT64_MAX).             */
 int  av1pf_lr_search_masked(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, int unit_size,
@@ -351,7 +364,8 @@ int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, 
  * device and returns, per stage id, total ms, launch count and bytes.
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
- * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13..15 reserved.     */
+ * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13 lr_search_exhaustive,
+ * 14..15 reserved.                                                          */
 #define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],

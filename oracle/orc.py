@@ -35,6 +35,7 @@ def lib():
         _lib.orc_cdef_search_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64,
                                                 C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.orc_lr_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.orc_lr_search_exhaustive.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -153,6 +154,22 @@ def lr_search_units(pw, ph, us):
     hz = max((pw + (us >> 1)) // us, 1)
     vt = max((ph + (us >> 1)) // us, 1)
     return hz, vt
+
+
+def lr_search_exhaustive(src, rec, unit_size=64):
+    """Exhaustive superset LR search: (units [n, 16] uint8, sse [n, 28] uint64,
+    fit [n, 38] int8)."""
+    hz, vt = lr_search_units(rec.w, rec.h, unit_size)
+    n = hz * vt
+    units = np.zeros((n, 16), np.uint8)
+    sse = np.zeros((n, 28), np.uint64)
+    fit = np.zeros((n, 38), np.int8)
+    s, r = orc_frame(src), orc_frame(rec)
+    rc = lib().orc_lr_search_exhaustive(C.byref(s), C.byref(r), unit_size, units.ctypes.data, sse.ctypes.data,
+                                        fit.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_lr_search_exhaustive failed: {rc}")
+    return units, sse, fit
 
 
 def lr_search(src, rec, unit_size=64):

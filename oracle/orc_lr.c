@@ -123,13 +123,14 @@ static void sgr_coeffs(const uint16_t *src, int ss, int w, int h, int bd, int r,
     }
 }
 
-void orc_sgr_unit(const uint16_t *src, int ss, uint16_t *dst, int ds,
-                  int w, int h, int set, const int8_t xqd[2], int bd)
+/* The two self-guided filter outputs f0 (r = 2 pass) and f1 (r = 1 pass) of
+ * set `set`, [h][w] in Q4, zero for a pass of radius 0. */
+void orc_sgr_filters(const uint16_t *src, int ss, int w, int h, int set, int bd, int32_t *f0, int32_t *f1)
 {
     int32_t *A = calloc((size_t)(w + 2) * (h + 2), sizeof(int32_t));
     int32_t *B = calloc((size_t)(w + 2) * (h + 2), sizeof(int32_t));
-    int32_t *f0 = calloc((size_t)w * h, sizeof(int32_t));
-    int32_t *f1 = calloc((size_t)w * h, sizeof(int32_t));
+    memset(f0, 0, sizeof(int32_t) * (size_t)w * h);
+    memset(f1, 0, sizeof(int32_t) * (size_t)w * h);
     const int W2 = w + 2;
 #define AA(y, x) A[((y) + 1) * W2 + (x) + 1]
 #define BB(y, x) B[((y) + 1) * W2 + (x) + 1]
@@ -163,6 +164,15 @@ void orc_sgr_unit(const uint16_t *src, int ss, uint16_t *dst, int ds,
     }
 #undef AA
 #undef BB
+    free(A); free(B);
+}
+
+void orc_sgr_unit(const uint16_t *src, int ss, uint16_t *dst, int ds,
+                  int w, int h, int set, const int8_t xqd[2], int bd)
+{
+    int32_t *f0 = calloc((size_t)w * h, sizeof(int32_t));
+    int32_t *f1 = calloc((size_t)w * h, sizeof(int32_t));
+    orc_sgr_filters(src, ss, w, h, set, bd, f0, f1);
     int xq[2];
     orc_sgr_decode_xq(set, xqd, xq);
     const int32_t mx = (1 << bd) - 1;
@@ -176,7 +186,7 @@ void orc_sgr_unit(const uint16_t *src, int ss, uint16_t *dst, int ds,
             int32_t wv = (int16_t)rr;
             dst[(ptrdiff_t)y * ds + x] = (uint16_t)clamp32(wv, 0, mx);
         }
-    free(A); free(B); free(f0); free(f1);
+    free(f0); free(f1);
 }
 
 /* ---- geometry ----------------------------------------------------------- */

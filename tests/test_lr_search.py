@@ -201,3 +201,190 @@ def test_gpu_lr_search_parity(case):
     wu, we = orc.lr_search(src, rec, us)
     np.testing.assert_array_equal(ge, we)
     np.testing.assert_array_equal(gu, wu)
+
+
+# ---------------------------------------------------------------- exhaustive superset mode
+def decoder_lr_units(rec, recs):
+    """Oracle decoder LR on the luma plane with per-unit 16-byte records
+    (64-unit grid); the frame is its own deblocked and CDEF output."""
+    fr = rec.copy_pixels()
+    fr.deblock = fr.cdef = fr.grain = None
+    fr.lr = dict(frame_type=[abi.LR_SWITCHABLE, abi.LR_NONE, abi.LR_NONE], unit_size=[64, 32, 32],
+                 units=[np.ascontiguousarray(recs, np.uint8).reshape(-1, 16), None, None])
+    out, _ = orc.run_chain(fr, abi.STAGE_LR)
+    return out.planes[0].pix
+
+
+def wiener_record(v, h):
+    b = bytearray(16)
+    b[0] = abi.LR_WIENER
+    b[4:7] = np.array(v, np.int8).tobytes()
+    b[7:10] = np.array(h, np.int8).tobytes()
+    return np.frombuffer(bytes(b), np.uint8)
+
+
+def sgr_record(s, xqd):
+    b = bytearray(16)
+    b[0] = abi.LR_SGRPROJ
+    b[1] = s
+    b[2:4] = np.array(xqd, np.int8).tobytes()
+    return np.frombuffer(bytes(b), np.uint8)
+
+
+def test_exhaustive_scores_equal_decoder_lr():
+    """The fitted Wiener filter and the fitted projection of several SGR sets,
+    put on every unit, give the decoder LR SSE the search reports."""
+    src, rec, _, _ = synth.search_pair(seed=71, w=256, h=192, model="dct", q=150)
+    seam_safe(rec)
+    units, e, fit = orc.lr_search_exhaustive(src, rec, 64)
+    S = src.planes[0].pix
+    rects = unit_rects(rec.w, rec.h)
+    cases = [("wiener", None)] + [("sgr", s) for s in (0, 5, 11, 14)]
+    for kind, s in cases:
+        recs = np.zeros((len(rects), 16), np.uint8)
+        for i in range(len(rects)):
+            recs[i] = wiener_record(fit[i, 0:3], fit[i, 3:6]) if kind == "wiener" else \
+                sgr_record(s, fit[i, 6 + 2 * s:8 + 2 * s])
+        out = decoder_lr_units(rec, recs)
+        for i, (x0, x1, y0, y1) in enumerate(rects):
+            if x1 - x0 > 64 or y1 - y0 > 64 or e[i, 0] == 0:
+                continue
+            k = 7 if kind == "wiener" else 12 + s
+            assert int(e[i, k]) == sse(S[y0:y1, x0:x1], out[y0:y1, x0:x1]), f"{kind} {s} unit {i}"
+
+
+def test_exhaustive_contains_curated():
+    """The curated candidates keep their exact scores, and the exhaustive
+    decision's SSE is never above the curated decision's."""
+    for seed, model in ((72, "noise"), (73, "dct")):
+        src, rec, _, _ = synth.search_pair(seed=seed, w=320, h=256, model=model, q=140)
+        cu, ce = orc.lr_search(src, rec, 64)
+        xu, xe, _ = orc.lr_search_exhaustive(src, rec, 64)
+        np.testing.assert_array_equal(xe[:, 0], ce[:, 0])
+        np.testing.assert_array_equal(xe[:, 8:12], ce[:, 8:12])
+        for i in range(len(ce)):
+            if ce[i, 7] != NONE_SSE:   # the curated exact Wiener re-score is one of the six
+                assert ce[i, 7] in set(int(v) for v in xe[i, 1:7])
+
+        def chosen(units, e, i, ex):
+            t = units[i, 0]
+            if t == abi.LR_NONE:
+                return int(e[i, 0])
+            if t == abi.LR_WIENER:
+                return min(int(v) for v in (e[i, 1:8] if ex else e[i, 7:8]))
+            return min(int(v) for v in (e[i, 8:28] if ex else e[i, 8:12]))
+        for i in range(len(ce)):
+            assert chosen(xu, xe, i, True) <= chosen(cu, ce, i, False), i
+
+
+def bordered_unit(rec, x0, x1, y0, y1):
+    R = rec.planes[0].pix[:rec.h, :rec.w].astype(np.int64)
+    yy = np.clip(np.arange(y0 - 3, y1 + 3), 0, rec.h - 1)
+    xx = np.clip(np.arange(x0 - 3, x1 + 3), 0, rec.w - 1)
+    return R[yy][:, xx]
+
+
+def test_exhaustive_fits_match_numpy_least_squares():
+    """The fitted SGR projections equal numpy's least-squares solution of the
+    same objective (rounded), and the fitted Wiener taps equal an independent
+    numpy run of the alternating solves; both up to clamping and exact .5
+    ties."""
+    import ctypes as C
+    src, rec, _, _ = synth.search_pair(seed=74, w=192, h=128, model="dct", q=120)
+    units, e, fit = orc.lr_search_exhaustive(src, rec, 64)
+    S = src.planes[0].pix[:rec.h, :rec.w].astype(np.int64)
+    lib = orc.lib()
+    lib.orc_sgr_filters.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    radius = [(2, 1)] * 10 + [(0, 1)] * 4 + [(2, 0)] * 2
+    tap_lo, tap_hi = (-5, -23, -17), (10, 8, 46)
+    checked = 0
+    for i, (x0, x1, y0, y1) in enumerate(unit_rects(rec.w, rec.h)):
+        w, h = x1 - x0, y1 - y0
+        if w > 64 or h > 64 or e[i, 0] == 0:
+            continue
+        B = bordered_unit(rec, x0, x1, y0, y1)
+        U = B[3:3 + h, 3:3 + w]
+        Su = S[y0:y1, x0:x1]
+        bu = np.ascontiguousarray(B, np.uint16)
+        for s in range(16):
+            f0 = np.zeros(w * h, np.int32)
+            f1 = np.zeros(w * h, np.int32)
+            lib.orc_sgr_filters(bu.ctypes.data + 2 * (3 * bu.shape[1] + 3), bu.shape[1], w, h, s, 8,
+                                f0.ctypes.data, f1.ctypes.data)
+            uu = (U << 4).ravel()
+            cols = [c for c, r in zip((f0 - uu, f1 - uu), radius[s]) if r]
+            A = np.stack(cols, axis=1).astype(np.float64)
+            t = (((Su << 4).ravel() - uu) << 7).astype(np.float64)
+            sol = np.linalg.lstsq(A, t, rcond=None)[0]
+            r0, r1 = radius[s]
+            xq = [0.0, 0.0]
+            if r0 and r1:
+                xq = list(sol)
+            elif r0:
+                xq[0] = sol[0]
+            else:
+                xq[1] = sol[0]
+            if any(abs((v % 1.0) - 0.5) < 1e-6 for v in xq):
+                continue
+            q0, q1 = int(np.floor(xq[0] + 0.5)), int(np.floor(xq[1] + 0.5))
+            if r0 and r1:
+                want0 = min(max(q0, -96), 31)
+                want = (want0, min(max(128 - want0 - q1, -32), 95))
+            elif r0:
+                want0 = min(max(q0, -96), 31)
+                want = (want0, min(max(128 - want0, -32), 95))
+            else:
+                want = (0, min(max(128 - q1, -32), 95))
+            assert tuple(int(v) for v in fit[i, 6 + 2 * s:8 + 2 * s]) == want, (i, s, xq)
+            checked += 1
+        # Wiener: the alternating solves in numpy
+        def solve_dir(fixed, horiz):
+            ft = np.array([fixed[0], fixed[1], fixed[2], 128 - 2 * sum(fixed), fixed[2], fixed[1], fixed[0]], np.int64)
+            Y = np.zeros((7, h, w), np.int64)
+            for k in range(7):
+                for q in range(7):
+                    if horiz:
+                        Y[k] += ft[q] * B[q:q + h, k:k + w]
+                    else:
+                        Y[k] += ft[q] * B[k:k + h, q:q + w]
+            z = np.stack([Y[0] + Y[6] - 2 * Y[3], Y[1] + Y[5] - 2 * Y[3], Y[2] + Y[4] - 2 * Y[3]]).reshape(3, -1)
+            tt = ((Su << 14) - 128 * Y[3]).ravel()
+            M = (z.astype(np.float64) @ z.T.astype(np.float64))
+            r = z.astype(np.float64) @ tt.astype(np.float64)
+            x = np.linalg.solve(M, r)
+            return [int(min(max(np.floor(x[k] + 0.5), tap_lo[k]), tap_hi[k])) for k in range(3)]
+        v, hh = [0, 0, 0], [0, 0, 0]
+        for _ in range(2):
+            hh = solve_dir(v, True)
+            v = solve_dir(hh, False)
+        assert list(fit[i, 0:3]) == v and list(fit[i, 3:6]) == hh, (i, v, hh, fit[i, :6])
+        checked += 1
+    assert checked > 50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(1280, 720, 64, "noise"), (200, 136, 64, "dct"), (256, 192, 32, "dct"),
+                                  (640, 360, 128, "dct")])
+def test_gpu_lr_search_exhaustive_parity(case):
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner
+    w, h, us, model = case
+    src, rec, _, _ = synth.search_pair(seed=51, w=w, h=h, model=model, q=130)
+    hz, vt = orc.lr_search_units(rec.w, rec.h, us)
+    print(f"exhaustive lr search {case}", flush=True)
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        units = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
+        e = torch.zeros((hz * vt, 28), dtype=torch.int64, device="cuda")
+        f = torch.zeros((hz * vt, 38), dtype=torch.int8, device="cuda")
+        r.lr_search_exhaustive(ds, dr, us, units.data_ptr(), e.data_ptr(), f.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        gu, ge, gf = units.cpu().numpy(), e.cpu().numpy().view(np.uint64), f.cpu().numpy()
+    finally:
+        r.close()
+    wu, we, wf = orc.lr_search_exhaustive(src, rec, us)
+    np.testing.assert_array_equal(gf, wf)
+    np.testing.assert_array_equal(ge, we)
+    np.testing.assert_array_equal(gu, wu)
