@@ -110,6 +110,20 @@ def search_leg(runner, steps, cpu):
         runner.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), lam, dec.data_ptr(), idx.data_ptr())
         runner.lr_search(ds, dr, 64, lr_units.data_ptr())
 
+    # the exhaustive LR mode (Wiener fit + 16 SGR sets), timed on its own
+    ex_sse = torch.zeros((hz * vt, 28), dtype=torch.int64, device="cuda")
+    ex_fit = torch.zeros((hz * vt, 38), dtype=torch.int8, device="cuda")
+    for _ in range(2):
+        runner.lr_search_exhaustive(ds, dr, 64, lr_units.data_ptr(), ex_sse.data_ptr(), ex_fit.data_ptr())
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(torch.cuda.current_stream())
+    for _ in range(steps):
+        runner.lr_search_exhaustive(ds, dr, 64, lr_units.data_ptr(), ex_sse.data_ptr(), ex_fit.data_ptr())
+    e1.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    lr_exh_ms = e0.elapsed_time(e1) / steps
+
     for _ in range(3):
         one()
     runner.sync()
@@ -139,6 +153,7 @@ def search_leg(runner, steps, cpu):
            "ms_per_frame": round(dev_ms, 4),
            "mse_table_us": avg("cdef_search"), "select_us": avg("cdef_search_select"),
            "lr_search_us": avg("lr_search"),
+           "lr_search_exhaustive_us": round(lr_exh_ms * 1e3, 1),
            "host_finish_ms": round(finish_ms, 3),
            "cdef_bits": int(dec.cpu().numpy()[:4].view(np.int32)[0]), "host_finish_bits": host[0]}
     if cpu:
