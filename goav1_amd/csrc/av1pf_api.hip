@@ -625,8 +625,8 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         offs[k] = off;
         off = align256(off + sizeof(Seg) * (lists[k]->v.size() + 1));
     }
-    const size_t qoff = off;   // work-queue counters of the tunable stages, zero at upload
-    off = align256(off + kTunable * sizeof(unsigned int));
+    const size_t qoff = off;   // work-queue counters of the tunable stages (one per XCD), zero at upload
+    off = align256(off + kTunable * kXcds * sizeof(unsigned int));
     auto fill = [&](char *h) {
         memcpy(h, P.descs.data(), sizeof(FrameDesc) * n);
         for (int k = 0; k < nl; k++) {
@@ -634,7 +634,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             memcpy(h + offs[k], lists[k]->v.data(), sizeof(Seg) * lists[k]->v.size());
             memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
         }
-        memset(h + qoff, 0, kTunable * sizeof(unsigned int));
+        memset(h + qoff, 0, kTunable * kXcds * sizeof(unsigned int));
     };
     // shape class: everything a launch's arguments depend on besides data
     std::vector<int> key;
@@ -657,7 +657,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         auto launch_of = [&](int i) {
             StageLaunch L;
             L.per_wg = md.m[i];
-            L.queue = reinterpret_cast<unsigned int *>(d + qoff) + i;
+            L.queue = reinterpret_cast<unsigned int *>(d + qoff) + i * kXcds;
             return L;
         };
         // tuning: after the stage's real launch, time every work distribution
@@ -673,7 +673,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
                 L.per_wg = m;
                 for (int rep = 0; rep < 4; rep++) {
                     if (rep == 1) HIPCHK(c, hipEventRecord(e0, st));
-                    if (m == 0) HIPCHK(c, launch_zero_u32(L.queue, 1, st));
+                    if (m == 0) HIPCHK(c, launch_zero_u32(L.queue, kXcds, st));
                     HIPCHK(c, run(L));
                 }
                 HIPCHK(c, hipEventRecord(e1, st));

@@ -171,7 +171,10 @@ hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t s)
 
 int stage_grid(const void *kernel, int total, int per_wg)
 {
-    if (per_wg > 0) return (total + per_wg - 1) / per_wg;
+    if (per_wg > 0) {
+        const int g = (total + per_wg - 1) / per_wg, step = kXcds * kChunk;
+        return (g + step - 1) / step * step;
+    }
     static int cus = 0;
     int dev = 0, per_cu = 0;
     if (!cus && (hipGetDevice(&dev) != hipSuccess ||

@@ -100,22 +100,35 @@ __device__ __forceinline__ int find_seg(const Seg *__restrict__ segs, int n, int
 }
 
 // ---- work distribution of a stage launch ---------------------------------
-// per_wg > 0: workgroup b filters the units (tiles) b * per_wg .. b * per_wg +
-// per_wg - 1 (grid = ceil(total / per_wg)); per_wg == 0: a persistent grid of
-// resident workgroups takes units from a device queue (a counter that is zero
-// before the launch) until it runs dry.  Every workgroup leaves the loop on
-// the same shared value, so every wave reaches the exit.
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  Neighbouring tiles share halo
+// rows and columns, so neighbours should run on one XCD:
+//   per_wg > 0: the tiles are cut into chunks of kChunk workgroups' worth of
+//     consecutive tiles, chunk c going to XCD c % 8; the i-th workgroup of
+//     XCD x takes workgroup slot i % kChunk of chunk (i / kChunk) * 8 + x
+//     (grid = ceil(total / per_wg) rounded up to 8 * kChunk; the extra
+//     workgroups find no tile);
+//   per_wg == 0: a persistent grid of resident workgroups; a workgroup takes
+//     tiles one at a time from its XCD's queue (8 counters, zero before the
+//     launch, over 8 contiguous runs of the tiles) and then from the others.
+// Every workgroup leaves the loop on the same shared value, so every wave
+// reaches the exit.
+constexpr int kXcds = 8;
+constexpr int kChunk = 8;
 struct StageLaunch {
     int per_wg;
-    unsigned int *queue;
+    unsigned int *queue;   // kXcds counters (persistent mode)
 };
 
 template <typename Body>
 __device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, Body &&body)
 {
     __shared__ int next_unit;
+    const int b = blockIdx.x, x = b % kXcds;
     if (L.per_wg > 0) {
-        const int u0 = blockIdx.x * L.per_wg;
+        const int i = b / kXcds;
+        const int lw = ((i / kChunk) * kXcds + x) * kChunk + i % kChunk;   // logical workgroup
+        const int u0 = lw * L.per_wg;
         for (int k = 0; k < L.per_wg; k++) {
             const int u = u0 + k;
             if (u >= total) return;
@@ -123,19 +136,24 @@ __device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, B
             body(u);
         }
     } else {
-        for (;;) {
-            __syncthreads();           // the previous unit is done with the LDS and next_unit
-            if (threadIdx.x == 0) next_unit = (int)atomicAdd(L.queue, 1u);
-            __syncthreads();
-            const int u = next_unit;
-            if (u >= total) return;
-            body(u);
+        for (int q = 0; q < kXcds; q++) {
+            const int qq = (x + q) % kXcds;
+            const int lo = (int)((int64_t)total * qq / kXcds), hi = (int)((int64_t)total * (qq + 1) / kXcds);
+            for (;;) {
+                __syncthreads();           // the previous unit is done with the LDS and next_unit
+                if (threadIdx.x == 0) next_unit = lo + (int)atomicAdd(L.queue + qq, 1u);
+                __syncthreads();
+                const int u = next_unit;
+                if (u >= hi) break;
+                body(u);
+            }
         }
     }
 }
 
-// Grid of a stage launch: ceil(total / per_wg), or the resident workgroups of
-// the kernel (occupancy x compute units, at most total) when persistent.
+// Grid of a stage launch: ceil(total / per_wg) rounded up to 8 * kChunk, or
+// the resident workgroups of the kernel (occupancy x compute units, at most
+// total) when persistent.
 int stage_grid(const void *kernel, int total, int per_wg);
 
 // ---- launches (all batched) ------------------------------------------------
