@@ -448,8 +448,8 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
                 const int w4 = ((((f->width + sx) >> sx) + 3) & ~3) >> 2;
                 const int h4 = ((((f->height + sy) >> sy) + 3) & ~3) >> 2;
-                P.dbv.add(i, pl, (w4 + 63) / 64, h4);
-                P.dbh.add(i, pl, (w4 * 4 + 63) / 64, (h4 + 3) / 4);
+                P.dbv.add(i, pl, (w4 + 63) / 64, (h4 + 3) / 4);
+                P.dbh.add(i, pl, (w4 + 63) / 64, (h4 + 3) / 4);
             }
             P.bytes_db += 2 * view_bytes(A);
         }

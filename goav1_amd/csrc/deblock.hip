@@ -108,6 +108,7 @@ __device__ __forceinline__ void lpf_line(T *p, int64_t st, int bd, int E, int I,
 struct DbPlaneK {
     int w4, h4;       // MI(4)-aligned position extent, 4x4 units
     int rcw, rch;     // region size in 4x4 units (32 >> ss)
+    int lrcw, lrch;   // their log2
     int lane_v, lane_h;
     int comp_v, comp_h;
     bool luma;
@@ -122,6 +123,8 @@ __device__ __forceinline__ DbPlaneK db_plane(const FrameView &f, int pl)
     k.h4 = ((ph + 3) & ~3) >> 2;
     k.rcw = 32 >> sx;
     k.rch = 32 >> sy;
+    k.lrcw = 5 - sx;
+    k.lrch = 5 - sy;
     k.luma = pl == 0;
     k.lane_v = pl ? (16 >> sy) : 16;
     k.lane_h = pl ? (16 >> sx) : 16;
@@ -144,7 +147,9 @@ __device__ __forceinline__ uint32_t mask_bit(const uint16_t *halves, int n, int 
     return n < lane ? (halves[0] >> n) & 1u : (halves[1] >> (n - lane)) & 1u;
 }
 
-// Vertical edges: a 64 x 4 workgroup covers 64 edge columns (x4) by 4 pixel rows.
+// Vertical edges: a 64 x 4 workgroup covers 64 edge columns (x4) by 4 rows of
+// 4x4 units (y4); a thread looks up its unit's mask and level once and filters
+// the unit's 4 pixel rows.
 __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restrict__ descs,
                                                         const Seg *__restrict__ segs, int nseg)
 {
@@ -156,11 +161,10 @@ __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restr
     const int t = blockIdx.x - sg.base;
     const int tx = t % sg.nx, ty = t / sg.nx;
     const int x4 = tx * 64 + threadIdx.x;
-    const int y = ty * 4 + threadIdx.y;
-    const int y4 = y >> 2;
+    const int y4 = ty * 4 + threadIdx.y;
     if (x4 == 0 || x4 >= P.w4 || y4 >= P.h4) return;
-    const int rr = y4 / P.rch, rc = x4 / P.rcw;
-    const int pos = x4 - rc * P.rcw, n = y4 - rr * P.rch;
+    const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
+    const int pos = x4 & (P.rcw - 1), n = y4 & (P.rch - 1);
     const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
     int wd;
     if (P.luma)
@@ -174,13 +178,17 @@ __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restr
     if (L == 0) L = lc[4 * (li - 1) + P.comp_v];
     if (L == 0) return;
     const PlaneView &pv = f.p[pl];
-    if (f.bd == 8)
-        lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y) + x4 * 4, 1, 8, D.db.E[L], D.db.I[L], L >> 4, wd);
-    else
-        lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y) + x4 * 4, 1, f.bd, D.db.E[L], D.db.I[L], L >> 4, wd);
+    const int E = D.db.E[L], I = D.db.I[L], Hh = L >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        if (f.bd == 8) lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4 + r) + x4 * 4, 1, 8, E, I, Hh, wd);
+        else lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4 + r) + x4 * 4, 1, f.bd, E, I, Hh, wd);
+    }
 }
 
-// Horizontal edges: a 64 x 4 workgroup covers 64 pixel columns by 4 edge rows (y4).
+// Horizontal edges: a 64 x 4 workgroup covers 64 columns of 4x4 units (x4) by 4
+// edge rows (y4); a thread looks up its unit's mask and level once and filters
+// the unit's 4 pixel columns.
 __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restrict__ descs,
                                                         const Seg *__restrict__ segs, int nseg)
 {
@@ -191,12 +199,11 @@ __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restr
     const DbPlaneK P = db_plane(f, pl);
     const int t = blockIdx.x - sg.base;
     const int tx = t % sg.nx, ty = t / sg.nx;
-    const int x = tx * 64 + threadIdx.x;
+    const int x4 = tx * 64 + threadIdx.x;
     const int y4 = ty * 4 + threadIdx.y;
-    const int x4 = x >> 2;
     if (y4 == 0 || y4 >= P.h4 || x4 >= P.w4) return;
-    const int rr = y4 / P.rch, rc = x4 / P.rcw;
-    const int pos = y4 - rr * P.rch, n = x4 - rc * P.rcw;
+    const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
+    const int pos = y4 & (P.rch - 1), n = x4 & (P.rcw - 1);
     const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
     int wd;
     if (P.luma)
@@ -210,10 +217,12 @@ __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restr
     if (L == 0) L = lc[4 * (li - D.db.mi_cols) + P.comp_h];
     if (L == 0) return;
     const PlaneView &pv = f.p[pl];
-    if (f.bd == 8)
-        lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4) + x, pv.stride, 8, D.db.E[L], D.db.I[L], L >> 4, wd);
-    else
-        lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4) + x, pv.stride / 2, f.bd, D.db.E[L], D.db.I[L], L >> 4, wd);
+    const int E = D.db.E[L], I = D.db.I[L], Hh = L >> 4;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (f.bd == 8) lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4 + c, pv.stride, 8, E, I, Hh, wd);
+        else lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4 + c, pv.stride / 2, f.bd, E, I, Hh, wd);
+    }
 }
 
 hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,

@@ -296,11 +296,13 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         const int pw = sp.w, ph = sp.h;
         if (px0 >= pw || py0 >= ph) continue;
         const int bwid = min(bw, pw - px0), height = min(bh, ph - py0);
+        // one lane per column (a power of two >= the block width), rows stepping
+        const int lpr = bwid > 16 ? 32 : bwid > 8 ? 16 : bwid > 4 ? 8 : 4;
+        const int x = tid & (lpr - 1), ystep = 256 / lpr;
+        if (x >= bwid) continue;
         if (!G.active[pl]) {   // plane without grain: the display gets the filtered samples
-            for (int i = tid; i < bwid * height; i += 256) {
-                const int y = i / bwid, x = i - y * bwid;
+            for (int y = tid / lpr; y < height; y += ystep)
                 row_ptr<T>(dp, py0 + y)[px0 + x] = row_ptr<T>(sp, py0 + y)[px0 + x];
-            }
             continue;
         }
         const int ys = (g.overlap && brow > 0) ? min(height, 2 >> sy) : 0;
@@ -321,8 +323,7 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         const int ox1 = 3 + (2 >> sx) * (3 + (offs[1][0] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1][0] & 15));
         const int ox2 = 3 + (2 >> sx) * (3 + (offs[0][1] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[0][1] & 15));
         const int ox3 = 3 + (2 >> sx) * (3 + (offs[1][1] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[1][1] & 15));
-        for (int i = tid; i < bwid * height; i += 256) {
-            const int y = i / bwid, x = i - y * bwid;
+        for (int y = tid / lpr; y < height; y += ystep) {
             int gv = tmpl[(oy0 + y) * GW + ox0 + x];
             if (x < xs) {
                 const int left = tmpl[(oy1 + y) * GW + ox1 + x + bw];
