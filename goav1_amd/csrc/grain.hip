@@ -323,7 +323,17 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         const int ox1 = 3 + (2 >> sx) * (3 + (offs[1][0] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1][0] & 15));
         const int ox2 = 3 + (2 >> sx) * (3 + (offs[0][1] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[0][1] & 15));
         const int ox3 = 3 + (2 >> sx) * (3 + (offs[1][1] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[1][1] & 15));
-        for (int y = tid / lpr; y < height; y += ystep) {
+        // at most 4 rows per lane: a block is at most 32 rows and a row group
+        // step is at least 8 (32 lanes per row), so MR * ystep >= height.
+        // Every load of the lane's rows is issued before the first store, so
+        // the loads do not wait behind the stores.
+        constexpr int MR = 4;
+        const int yb = tid / lpr;
+        int sv[MR], gvv[MR], idxv[MR];
+#pragma unroll
+        for (int k = 0; k < MR; k++) {
+            const int y = yb + k * ystep;
+            if (y >= height) break;
             int gv = tmpl[(oy0 + y) * GW + ox0 + x];
             if (x < xs) {
                 const int left = tmpl[(oy1 + y) * GW + ox1 + x + bw];
@@ -337,25 +347,31 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
                 }
                 gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
             }
+            gvv[k] = gv;
             const int px = px0 + x, py = py0 + y;
-            const int s = row_ptr<T>(sp, py)[px];
-            int idx;
+            const int sa = row_ptr<T>(sp, py)[px];
+            sv[k] = sa;
             if (pl == 0) {
-                idx = s;
+                idxv[k] = sa;
             } else {
                 const PlaneView &lp = src.p[0];
                 const int lx = px << sx, ly = py << sy;
                 int avg = row_ptr<T>(lp, ly)[lx];
                 if (sx && !(odd_luma && px == pw - 1)) avg = (avg + row_ptr<T>(lp, ly)[lx + 1] + 1) >> 1;
-                if (g.chroma_scaling_from_luma) idx = avg;
+                if (g.chroma_scaling_from_luma) idxv[k] = avg;
                 else {
-                    const int combined = avg * (lmult - 128) + s * (mult - 128);
-                    idx = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
+                    const int combined = avg * (lmult - 128) + sa * (mult - 128);
+                    idxv[k] = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
                 }
             }
-            const int scale = scale_lut(lut[pl], idx, bd);
-            const int noise = r2i(scale * gv, scaling_shift);
-            row_ptr<T>(dp, py)[px] = (T)clipi(s + noise, minv, maxv);
+        }
+#pragma unroll
+        for (int k = 0; k < MR; k++) {
+            const int y = yb + k * ystep;
+            if (y >= height) break;
+            const int scale = scale_lut(lut[pl], idxv[k], bd);
+            const int noise = r2i(scale * gvv[k], scaling_shift);
+            row_ptr<T>(dp, py0 + y)[px0 + x] = (T)clipi(sv[k] + noise, minv, maxv);
         }
     }
 }
