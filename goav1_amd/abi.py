@@ -98,7 +98,7 @@ class Request(C.Structure):
         ("stages", C.c_uint32), ("pad", C.c_uint32),
         ("deblock", C.POINTER(DeblockParams)), ("cdef", C.POINTER(CdefParams)),
         ("lr", C.POINTER(LRParams)), ("grain", C.POINTER(GrainParams)),
-        ("display", C.POINTER(Frame)),
+        ("display", C.POINTER(Frame)), ("upscaled", C.POINTER(Frame)),
     ]
 
 

@@ -123,7 +123,7 @@ struct av1pf_ctx {
 namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
-       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH };
+       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH, ST_SUPERRES };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -270,9 +270,12 @@ int lr_geometry(const av1pf_frame *f, const av1pf_lr_params *p, int pl, int *pw,
     const int sx = pl ? f->ss_x : 0, sy = pl ? f->ss_y : 0;
     *pw = (f->width + sx) >> sx;
     *ph = (f->height + sy) >> sy;
+    // luma units are 64..256; chroma units are the luma size, halved only for
+    // 4:2:0 (parser/restoration.go:70-105), so a unit is never shorter than a
+    // 64-luma-row stripe of its plane
     const int us = p->unit_size[pl];
     if (us != 32 && us != 64 && us != 128 && us != 256) return AV1PF_ERR_INVALID_ARG;
-    if (us < (64 >> sx)) return AV1PF_ERR_INVALID_ARG;
+    if (us < ((pl && sx && sy) ? 32 : 64)) return AV1PF_ERR_INVALID_ARG;
     *hz = std::max((*pw + (us >> 1)) / us, 1);
     *vt = std::max((*ph + (us >> 1)) / us, 1);
     return AV1PF_OK;
@@ -365,20 +368,74 @@ uint16_t gf2_apply(const uint16_t J[16], uint16_t s)
     return o;
 }
 
+// The upscaled frame of a super-resolution request: same format and height as
+// the coded frame, wider by a factor in (1, 2] (denominators 9..16 over 8,
+// postfilter_superres.go:112-117), planes not overlapping the coded frame's.
+int validate_superres(av1pf_ctx *c, const av1pf_frame *f, const av1pf_frame *u)
+{
+    if (!u) return fail(c, AV1PF_ERR_INVALID_ARG, "superres needs an upscaled frame");
+    int rc = validate_frame(c, u);
+    if (rc) return rc;
+    if (u->bit_depth != f->bit_depth || u->ss_x != f->ss_x || u->ss_y != f->ss_y || u->mono != f->mono ||
+        u->height != f->height)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "upscaled frame format or height differs from the coded frame");
+    if (u->width <= f->width || u->width > 2 * f->width)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "upscaled width must be above the coded width and at most twice it");
+    const int np = f->mono ? 1 : 3;
+    for (int i = 0; i < np; i++)
+        for (int j = 0; j < np; j++) {
+            const int sy = j ? f->ss_y : 0;
+            const char *fa = static_cast<const char *>(f->plane[j]);
+            const char *fb = fa + f->stride[j] * (int64_t)((((f->height + 7) & ~7) >> sy));
+            const char *ua = static_cast<const char *>(u->plane[i]);
+            const char *ub = ua + u->stride[i] * (int64_t)((((u->height + 7) & ~7) >> sy));
+            if (ua < fb && fa < ub) return fail(c, AV1PF_ERR_INVALID_ARG, "upscaled frame overlaps the coded frame");
+        }
+    return AV1PF_OK;
+}
+
+// Stages run on the coded frame up to CDEF; with super-resolution, loop
+// restoration and film grain run on the upscaled frame
+// (ApplyCallerPostFilters, postfilter.go:947-979).
 int validate_request(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req)
 {
     if (!req) return fail(c, AV1PF_ERR_INVALID_ARG, "null request");
     int rc = validate_frame(c, frame);
     if (rc) return rc;
     const uint32_t st = req->stages;
-    if (st & AV1PF_STAGE_SUPERRES) return fail(c, AV1PF_ERR_UNSUPPORTED, "superres stage is not supported");
-    if (st & ~(AV1PF_STAGE_DEBLOCK | AV1PF_STAGE_CDEF | AV1PF_STAGE_LR | AV1PF_STAGE_FILMGRAIN))
+    if (st & ~(AV1PF_STAGE_DEBLOCK | AV1PF_STAGE_CDEF | AV1PF_STAGE_SUPERRES | AV1PF_STAGE_LR | AV1PF_STAGE_FILMGRAIN))
         return fail(c, AV1PF_ERR_INVALID_ARG, "unknown stage bits");
+    const av1pf_frame *post = frame;
+    if (st & AV1PF_STAGE_SUPERRES) {
+        if ((rc = validate_superres(c, frame, req->upscaled))) return rc;
+        post = req->upscaled;
+    }
     if ((st & AV1PF_STAGE_DEBLOCK) && (rc = validate_deblock(c, frame, req->deblock))) return rc;
     if ((st & AV1PF_STAGE_CDEF) && (rc = validate_cdef(c, frame, req->cdef))) return rc;
-    if ((st & AV1PF_STAGE_LR) && (rc = validate_lr(c, frame, req->lr))) return rc;
-    if ((st & AV1PF_STAGE_FILMGRAIN) && (rc = validate_grain(c, frame, req->grain, req->display))) return rc;
+    if ((st & AV1PF_STAGE_LR) && (rc = validate_lr(c, post, req->lr))) return rc;
+    if ((st & AV1PF_STAGE_FILMGRAIN) && (rc = validate_grain(c, post, req->grain, req->display))) return rc;
     return AV1PF_OK;
+}
+
+// Upscale geometry of plane pl (superres.go:38-48): step and initial phase
+// from the coded and upscaled plane widths, source span the MI-aligned coded
+// width (postfilter_superres.go:241-252).
+SrPlane sr_plane(const av1pf_frame *f, const av1pf_frame *u, int pl)
+{
+    const int sx = pl ? f->ss_x : 0, sy = pl ? f->ss_y : 0;
+    SrPlane g{};
+    g.codedw = (f->width + sx) >> sx;
+    g.dstw = (u->width + sx) >> sx;
+    g.srcw = ((f->width + 7) & ~7) >> sx;
+    g.rows = (f->height + sy) >> sy;
+    const int64_t cw = g.codedw, uw = g.dstw;
+    const int64_t step = ((cw << 14) + uw / 2) / uw;
+    const int64_t err = uw * step - (cw << 14);
+    int64_t x0 = (-((uw - cw) << 13) + uw / 2) / uw;
+    x0 += (1 << 7) - err / 2;
+    g.step = (int)step;
+    g.x0 = (int)(x0 & 0x3fff);
+    return g;
 }
 
 struct SegList {
@@ -394,8 +451,8 @@ struct SegList {
 
 struct BatchPlan {
     std::vector<FrameDesc> descs;
-    SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain;
-    double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0;
+    SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain, sr;
+    double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0, bytes_sr = 0;
     size_t scratch = 0, bnd = 0, goffs = 0;
     int ngrain = 0;
 };
@@ -404,11 +461,12 @@ struct BatchPlan {
 // once the arena bases are known (offsets kept in the descriptors meanwhile).
 int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, BatchPlan &P,
                std::vector<size_t> &scr_off, std::vector<size_t> &bnd_off, std::vector<int> &grain_idx,
-               std::vector<size_t> &goff, bool skip_lr_save)
+               std::vector<size_t> &goff, bool skip_lr_save, std::vector<size_t> *sscr_off = nullptr)
 {
     goff.assign(n, 0);
     P.descs.assign(n, FrameDesc{});
     scr_off.assign(n, 0);
+    if (sscr_off) sscr_off->assign(n, SIZE_MAX);
     bnd_off.assign(n * 3, 0);
     grain_idx.assign(n, -1);
     for (int i = 0; i < n; i++) {
@@ -422,9 +480,17 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
         P.scratch += scratch_bytes(A, strides);
         D.v[VIEW_B] = A;
         for (int p = 0; p < A.nplanes; p++) D.v[VIEW_B].p[p].stride = strides[p];   // base patched later
-        D.v[VIEW_D] = r->display ? view_of(r->display) : A;
-        D.err = c->dev_err;
         const uint32_t st = r->stages;
+        const bool sr = (st & AV1PF_STAGE_SUPERRES) != 0;
+        const av1pf_frame *pf = sr ? r->upscaled : f;   // the frame LR and film grain see
+        D.v[VIEW_U] = sr ? view_of(pf) : A;
+        D.v[VIEW_S] = D.v[VIEW_U];
+        D.v[VIEW_D] = r->display ? view_of(r->display) : D.v[VIEW_U];
+        D.fin = sr ? VIEW_U : VIEW_A;
+        if (sr)
+            for (int p = 0; p < A.nplanes; p++) D.sr[p] = sr_plane(f, pf, p);
+        const FrameView &U = D.v[VIEW_U];
+        D.err = c->dev_err;
         int cur = VIEW_A;
         // ---- deblock
         if (st & AV1PF_STAGE_DEBLOCK) {
@@ -461,8 +527,9 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 LrPlaneDesc &g = D.lr[pl];
                 const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
                 g.type = p->frame_type[pl];
-                g.pw = A.p[pl].w;
-                g.ph = A.p[pl].h;
+                g.sr = sr;
+                g.pw = U.p[pl].w;
+                g.ph = U.p[pl].h;
                 g.procw = 64 >> sx;
                 g.sh = 64 >> sy;
                 g.off = 8 >> sy;
@@ -470,7 +537,7 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 g.stripes = lr_stripes(g.ph, g.sh, g.off);
                 if (g.type == AV1PF_LR_NONE) continue;
                 int pw, ph;
-                lr_geometry(f, p, pl, &pw, &ph, &g.hz, &g.vt);
+                lr_geometry(pf, p, pl, &pw, &ph, &g.hz, &g.vt);
                 g.us = p->unit_size[pl];
                 g.units = p->units[pl];
                 bnd_off[i * 3 + pl] = P.bnd;
@@ -503,12 +570,36 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 cur = VIEW_B;
             }
         }
+        // ---- super-resolution: coded picture -> upscaled frame (or its scratch
+        // when LR follows); the coded frame keeps the CDEF output
+        if (sr) {
+            D.sr_src = cur;
+            D.sr_dst = lr_any ? VIEW_S : VIEW_U;
+            double samples = 0;
+            for (int pl = 0; pl < A.nplanes; pl++) {
+                const SrPlane &g = D.sr[pl];
+                P.sr.add(i, pl, (g.dstw + 255) / 256, (g.rows + SR_ROWS - 1) / SR_ROWS);
+                samples += (double)(g.codedw + g.dstw) * g.rows;
+            }
+            P.bytes_sr += samples * bps_of(A.bd);
+            if (cur == VIEW_B) {
+                for (int pl = 0; pl < A.nplanes; pl++) P.copy_ba.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+                P.bytes_copy += 2 * view_bytes(A);
+            }
+            if (lr_any) {
+                int64_t strides[3];
+                if (sscr_off) (*sscr_off)[i] = P.scratch;
+                P.scratch += scratch_bytes(U, strides);
+                for (int p = 0; p < U.nplanes; p++) D.v[VIEW_S].p[p].stride = strides[p];   // base patched later
+            }
+            cur = D.sr_dst;
+        }
         // ---- LR
         if (lr_any) {
             D.lr_src = cur;
-            D.lr_dst = cur == VIEW_A ? VIEW_B : VIEW_A;
+            D.lr_dst = sr ? VIEW_U : cur == VIEW_A ? VIEW_B : VIEW_A;
             for (int pl = 0; pl < A.nplanes; pl++) P.lr.add(i, pl, D.lr[pl].xtiles, D.lr[pl].stripes);
-            P.bytes_lr += 2 * view_bytes(A);
+            P.bytes_lr += 2 * view_bytes(U);
             cur = D.lr_dst;
         }
         if (cur == VIEW_B) {
@@ -519,14 +610,14 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
         if (st & AV1PF_STAGE_FILMGRAIN) {
             const av1pf_grain_params *g = r->grain;
             if (grain_noop(g)) {
-                for (int pl = 0; pl < A.nplanes; pl++) P.copy_ad.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
-                P.bytes_copy += 2 * view_bytes(A);
+                for (int pl = 0; pl < U.nplanes; pl++) P.copy_ad.add(i, pl, (U.p[pl].aw + 255) / 256, U.p[pl].ah);
+                P.bytes_copy += 2 * view_bytes(U);
             } else {
                 GrainDesc &G = D.gr;
                 G.g = *g;
                 G.active[0] = g->num_y_points != 0;
-                G.active[1] = !f->mono && (g->num_cb_points != 0 || g->chroma_scaling_from_luma);
-                G.active[2] = !f->mono && (g->num_cr_points != 0 || g->chroma_scaling_from_luma);
+                G.active[1] = !pf->mono && (g->num_cb_points != 0 || g->chroma_scaling_from_luma);
+                G.active[2] = !pf->mono && (g->num_cr_points != 0 || g->chroma_scaling_from_luma);
                 build_scaling_lut(G.luts[0], (const uint8_t(*)[2])g->y_points, g->num_y_points);
                 if (g->chroma_scaling_from_luma) {
                     memcpy(G.luts[1], G.luts[0], 256);
@@ -537,7 +628,7 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 }
                 uint16_t jl[16], jc[16];
                 lfsr_jump(jl, 82);
-                lfsr_jump(jc, f->ss_x ? 44 : 82);
+                lfsr_jump(jc, pf->ss_x ? 44 : 82);
                 const uint16_t seeds[3] = { g->seed, (uint16_t)(g->seed ^ 0xb524),
                                             (uint16_t)(g->seed ^ 0x49d8) };
                 for (int p = 0; p < 3; p++) {
@@ -547,19 +638,19 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                         st = gf2_apply(p ? jc : jl, st);
                     }
                 }
-                G.matrix_identity = f->matrix_identity;
-                G.orows = (f->height + 31) / 32;
-                G.ocols = (f->width + 31) / 32;
+                G.matrix_identity = pf->matrix_identity;
+                G.orows = (pf->height + 31) / 32;
+                G.ocols = (pf->width + 31) / 32;
                 goff[i] = P.goffs;
                 P.goffs += (size_t)G.orows * G.ocols;
                 grain_idx[i] = P.ngrain++;
                 P.tl.add(i, 0, 1, 1);
-                if (!f->mono) {
+                if (!pf->mono) {
                     P.tc.add(i, 1, 1, 1);
                     P.tc.add(i, 2, 1, 1);
                 }
-                P.grain.add(i, -1, (f->width + 31) / 32, (f->height + 31) / 32);
-                P.bytes_grain += 2 * view_bytes(A);
+                P.grain.add(i, -1, (pf->width + 31) / 32, (pf->height + 31) / 32);
+                P.bytes_grain += 2 * view_bytes(U);
             }
         }
     }
@@ -586,9 +677,9 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     StageModes chosen;
     if (n == 0) return AV1PF_OK;
     BatchPlan P;
-    std::vector<size_t> scr_off, bnd_off, goff;
+    std::vector<size_t> scr_off, bnd_off, goff, sscr_off;
     std::vector<int> grain_idx;
-    int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, goff, skip_lr_save);
+    int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, goff, skip_lr_save, &sscr_off);
     if (rc) return rc;
     if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
     if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
@@ -609,6 +700,13 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             D.v[VIEW_B].p[p].base = base;
             base += (size_t)D.v[VIEW_B].p[p].stride * D.v[VIEW_B].p[p].ah;
         }
+        if (sscr_off[i] != SIZE_MAX) {
+            base = c->scratch + sscr_off[i];
+            for (int p = 0; p < D.v[VIEW_S].nplanes; p++) {
+                D.v[VIEW_S].p[p].base = base;
+                base += (size_t)D.v[VIEW_S].p[p].stride * D.v[VIEW_S].p[p].ah;
+            }
+        }
         for (int p = 0; p < 3; p++)
             if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = c->lr_bnd + bnd_off[i * 3 + p];
         if (grain_idx[i] >= 0) {
@@ -617,7 +715,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         }
     }
     // ---- descriptors + segment tables + stage queues, one host image
-    SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain };
+    SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain, &P.sr };
     const int nl = sizeof(lists) / sizeof(lists[0]);
     size_t off = align256(sizeof(FrameDesc) * n);
     size_t offs[nl];
@@ -727,6 +825,10 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             if ((rc2 = tune(0, [&](const StageLaunch &L) { return launch_cdef(dd, segp(3), nseg(3), P.cdef.total, L, st); })))
                 return rc2;
         }
+        if (P.sr.total) {
+            StageTimer t(c, ST_SUPERRES, st, P.bytes_sr);
+            HIPCHK(c, launch_superres(dd, segp(10), nseg(10), P.sr.total, st));
+        }
         if (P.lr.total) {
             {
                 StageTimer t(c, ST_LR, st, P.bytes_lr);
@@ -738,7 +840,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         if (P.copy_ba.total || P.copy_ad.total) {
             StageTimer t(c, ST_COPY, st, P.bytes_copy);
             HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, st));
-            HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, VIEW_A, VIEW_D, st));
+            HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, -1, VIEW_D, st));
         }
         if (side) HIPCHK(c, hipStreamWaitEvent(st, ev_join, 0));
         if (P.grain.total) {
@@ -1064,6 +1166,14 @@ int av1pf_lr(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_lr_params *p, v
     const bool saved = c->bnd_frame == frame->plane[0];
     c->bnd_frame = nullptr;
     return run_batch(c, 1, frame, &r, static_cast<hipStream_t>(stream), saved);
+}
+
+int av1pf_superres(av1pf_ctx *c, const av1pf_frame *coded, const av1pf_frame *upscaled, void *stream)
+{
+    av1pf_request r{};
+    r.stages = AV1PF_STAGE_SUPERRES;
+    r.upscaled = upscaled;
+    return av1pf_apply(c, coded, &r, stream);
 }
 
 int av1pf_filmgrain(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *dst, const av1pf_grain_params *p, void *stream)

@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__res
             offs[left][i] = use ? G.offsets[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
         }
         __syncthreads();
-        const FrameView &src = D.v[VIEW_A];
+        const FrameView &src = D.v[D.fin];
         const FrameView &dst = D.v[VIEW_D];
         if (src.bd == 8) grain_block<uint8_t>(src, dst, G, bcol, brow, lut, offs);
         else grain_block<uint16_t>(src, dst, G, bcol, brow, lut, offs);

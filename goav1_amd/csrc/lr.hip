@@ -24,6 +24,7 @@
 // units and NONE planes copy.
 #include "pf_internal.h"
 #include "lr_filters.h"
+#include "superres_taps.h"
 
 namespace pf {
 
@@ -115,20 +116,25 @@ __global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ d
     });
 }
 
+// Deblocked context rows of stripe t, column x.  With super-resolution the
+// rows are read from the coded frame and upscaled to the LR plane width
+// (restoration_boundary.go:257-345): sr is the plane's upscale geometry.
 template <typename T>
-__device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneDesc &g, int x, int t)
+__device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneDesc &g, const SrPlane *sr, int maxv,
+                                             int x, int t)
 {
     if (x >= g.pw || t >= g.stripes) return;
     const int y0 = max(0, t * g.sh - g.off);
     const int y1 = min((t + 1) * g.sh - g.off, g.ph);
     uint16_t *b = g.bnd + (int64_t)t * 4 * g.pw;
+    auto px = [&](int y) { return sr ? sr_sample<T>(row_ptr<T>(pv, y), *sr, x, maxv) : (int)row_ptr<T>(pv, y)[x]; };
     if (t > 0) {
-        b[x] = row_ptr<T>(pv, y0 - 2)[x];
-        b[g.pw + x] = row_ptr<T>(pv, y0 - 1)[x];
+        b[x] = px(y0 - 2);
+        b[g.pw + x] = px(y0 - 1);
     }
     if (y1 < g.ph) {
-        b[2 * g.pw + x] = row_ptr<T>(pv, y1)[x];
-        b[3 * g.pw + x] = row_ptr<T>(pv, y1 + 1 < g.ph ? y1 + 1 : y1)[x];
+        b[2 * g.pw + x] = px(y1);
+        b[3 * g.pw + x] = px(y1 + 1 < g.ph ? y1 + 1 : y1);
     }
 }
 
@@ -139,8 +145,11 @@ __global__ __launch_bounds__(256) void lr_save_kernel(const FrameDesc *__restric
     const int t = blockIdx.x - sg.base;
     const int x = (t % sg.nx) * 256 + threadIdx.x, st = t / sg.nx;
     const FrameView &f = D.v[VIEW_A];
-    if (f.bd == 8) lr_save_cols<uint8_t>(f.p[sg.plane], D.lr[sg.plane], x, st);
-    else lr_save_cols<uint16_t>(f.p[sg.plane], D.lr[sg.plane], x, st);
+    const LrPlaneDesc &g = D.lr[sg.plane];
+    const SrPlane *sr = g.sr ? &D.sr[sg.plane] : nullptr;
+    const int maxv = (1 << f.bd) - 1;
+    if (f.bd == 8) lr_save_cols<uint8_t>(f.p[sg.plane], g, sr, maxv, x, st);
+    else lr_save_cols<uint16_t>(f.p[sg.plane], g, sr, maxv, x, st);
 }
 
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)
@@ -199,7 +208,7 @@ __global__ __launch_bounds__(256) void copy_kernel(const FrameDesc *__restrict__
     const FrameDesc &D = descs[sg.frame];
     const int t = blockIdx.x - sg.base;
     const int x = (t % sg.nx) * 256 + threadIdx.x, y = t / sg.nx;
-    const FrameView &S = D.v[src];
+    const FrameView &S = D.v[src < 0 ? D.fin : src];
     if (S.bd == 8) copy_cols<uint8_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);
     else copy_cols<uint16_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);
 }

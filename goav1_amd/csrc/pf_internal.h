@@ -51,8 +51,19 @@ struct CdefDesc {
     uint8_t y_strength[8], uv_strength[8];
 };
 
+// Super-resolution geometry of one plane (the position arithmetic is in
+// superres_taps.h).
+struct SrPlane {
+    int srcw;      // MI-aligned coded width: source samples the filter may read
+    int codedw;    // coded plane width (sets step and phase)
+    int dstw;      // upscaled plane width
+    int rows;      // rows upscaled (the plane's visible height)
+    int step, x0;  // position step and initial subpel position, 1/16384 sample
+};
+
 struct LrPlaneDesc {
     int type;                      // frame type, 0 = none (plane copied)
+    int sr;                        // boundary rows are upscaled from the coded frame (SrPlane of the plane)
     int us, hz, vt;                // unit size and grid
     const av1pf_lr_unit *units;
     uint16_t *bnd;                 // [stripes][4][pw]: rows y0-2, y0-1, y1, y1+1 of the deblocked frame
@@ -70,11 +81,17 @@ struct GrainDesc {
     int matrix_identity;
 };
 
-enum { VIEW_A = 0, VIEW_B = 1, VIEW_D = 2 };
+// Views of a frame: A the frame (coded size), B scratch of A's shape, D the
+// film-grain display frame; with super-resolution U is the caller's upscaled
+// frame and S scratch of U's shape.
+enum { VIEW_A = 0, VIEW_B = 1, VIEW_D = 2, VIEW_U = 3, VIEW_S = 4, NVIEWS = 5 };
 
 struct FrameDesc {
-    FrameView v[3];                // frame (A), scratch (B), display (D)
+    FrameView v[NVIEWS];
     int lr_src, lr_dst;            // views the LR stage reads / writes
+    int sr_src, sr_dst;            // views the super-resolution stage reads / writes
+    int fin;                       // view holding the filtered picture (film grain reads it)
+    SrPlane sr[3];
     DbDesc db;
     CdefDesc cd;
     LrPlaneDesc lr[3];
@@ -162,7 +179,10 @@ hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv,
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
 hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
 hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
+// src < 0: each frame's fin view
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
+hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+constexpr int SR_ROWS = 8;         // rows per super-resolution tile (tile = 256 output columns)
 hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s);
 hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L,
                               hipStream_t s);

@@ -28,10 +28,10 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_nn_adam",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
            "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode",
-           "av1pf_lr_search_exhaustive"]
+           "av1pf_lr_search_exhaustive", "av1pf_superres"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
-               "lr_search_exhaustive"] + [f"reserved{i}" for i in range(14, 16)]
+               "lr_search_exhaustive", "superres", "reserved15"]
 PROFILE_STAGES = 16
 
 
@@ -68,6 +68,7 @@ def load():
     lib.av1pf_cdef.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.CdefParams), C.c_void_p]
     lib.av1pf_lr_save_boundaries.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_int, C.c_void_p]
     lib.av1pf_lr.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_void_p]
+    lib.av1pf_superres.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p]
     lib.av1pf_filmgrain.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame),
                                     C.POINTER(abi.GrainParams), C.c_void_p]
     lib.av1pf_cdef_search.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p,
@@ -160,7 +161,7 @@ class DeviceFrame:
 class DeviceSideData:
     """Per-frame side data uploaded to the device, plus the host request struct."""
 
-    def __init__(self, fr, display=None, device="cuda", stages=None):
+    def __init__(self, fr, display=None, device="cuda", stages=None, upscaled=None):
         torch = _torch()
         self.keep = []
         req = abi.Request()
@@ -210,6 +211,9 @@ class DeviceSideData:
         if display is not None:
             self.keep.append(display.abi)
             req.display = C.pointer(display.abi)
+        if upscaled is not None:
+            self.keep.append(upscaled.abi)
+            req.upscaled = C.pointer(upscaled.abi)
         self.req = req
 
 
@@ -293,6 +297,10 @@ class PostFilterRunner:
     def filmgrain(self, src, dst, side, stream=None):
         self._check(self.lib.av1pf_filmgrain(self.h, C.byref(src.abi), C.byref(dst.abi), side.req.grain,
                                              self._stream(stream)))
+
+    def superres(self, coded, upscaled, stream=None):
+        """Upscale every plane of `coded` into `upscaled` (DeviceFrames)."""
+        self._check(self.lib.av1pf_superres(self.h, C.byref(coded.abi), C.byref(upscaled.abi), self._stream(stream)))
 
     def sync(self, stream=None):
         self._check(self.lib.av1pf_sync(self.h, self._stream(stream)))

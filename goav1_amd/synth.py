@@ -56,6 +56,7 @@ class FrameData:
         self.cdef = None
         self.lr = None
         self.grain = None
+        self.upscaled = None   # FrameData of the upscaled geometry (super-resolution requests)
         self.stages = 0
 
     @property
@@ -294,7 +295,9 @@ def build_side_data(fr, rng, *, lf_levels=None, lf_random=False, sharpness=0,
     # ---- loop restoration
     if lr_types is not None:
         types = list(lr_types)
-        sizes = [lr_unit, max(lr_unit >> ssx, 32), max(lr_unit >> ssx, 32)]
+        # chroma units halve only for 4:2:0 (parser/restoration.go:70-105)
+        cu = max(lr_unit >> (ssx & ssy), 32)
+        sizes = [lr_unit, cu, cu]
         units = []
         for pi in range(3):
             if pi >= len(fr.planes):
@@ -461,6 +464,36 @@ def search_dc_quant(q):
 
 
 BATCH_SIZES = [(426, 240), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def superres_coded_width(up_w, denom):
+    """Coded width of a super-resolution frame: (UpscaledWidth * 8 + denom / 2)
+    / denom, at least min(16, UpscaledWidth) (AV1 superres_params)."""
+    return max((up_w * 8 + denom // 2) // denom, min(16, up_w))
+
+
+def superres_frame(seed=0, up_w=1920, h=1080, denom=12, bd=10, ssx=1, ssy=1, mono=False, grain=False,
+                   lr_unit=64, stages=None):
+    """A super-resolution frame: blocky coded picture with deblock and CDEF
+    side data at the coded size, LR units on the upscaled grid, an empty
+    upscaled frame, optional film grain (on the upscaled frame)."""
+    rng = np.random.default_rng(seed)
+    fr = FrameData(superres_coded_width(up_w, denom), h, bd, ssx, ssy, mono)
+    fill_blocky(fr, rng)
+    build_side_data(fr, rng, lr_types=None)
+    up = FrameData(up_w, h, bd, ssx, ssy, mono)
+    grid = FrameData(up_w, h, bd, ssx, ssy, mono)
+    build_side_data(grid, rng, cdef=False, lr_types=(abi.LR_SWITCHABLE,) * 3, lr_unit=lr_unit)
+    fr.lr = grid.lr
+    fr.upscaled = up
+    if grain:
+        fr.grain = random_grain(rng, bd, mono=bool(mono))
+    if stages is None:
+        stages = abi.STAGES_ALL | abi.STAGE_SUPERRES
+        if not grain:
+            stages &= ~abi.STAGE_FILMGRAIN
+    fr.stages = stages
+    return fr
 
 
 def batch_mixed(seed=2, n=64, sizes=BATCH_SIZES):

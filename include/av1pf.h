@@ -46,7 +46,7 @@ extern "C" {
 /* ---- status codes ------------------------------------------------------ */
 #define AV1PF_OK                 0
 #define AV1PF_ERR_INVALID_ARG   -1  /* threading.ErrInvalidBatch / frame.ErrInvalidFormat */
-#define AV1PF_ERR_UNSUPPORTED   -2  /* decoder.ErrUnsupportedPostFilter (e.g. superres)   */
+#define AV1PF_ERR_UNSUPPORTED   -2  /* decoder.ErrUnsupportedPostFilter                   */
 #define AV1PF_ERR_DEVICE        -3  /* HIP runtime failure                                */
 #define AV1PF_ERR_SHORT_BUFFER  -4  /* frame.ErrShortBuffer                               */
 #define AV1PF_ERR_NO_DEVICE     -5  /* no usable gfx950 device                            */
@@ -54,7 +54,7 @@ extern "C" {
 /* ---- stage bits (FrameWorkPostFilter* enum, postfilter.go:13-21) -------- */
 #define AV1PF_STAGE_DEBLOCK    0x01u
 #define AV1PF_STAGE_CDEF       0x02u
-#define AV1PF_STAGE_SUPERRES   0x04u  /* not supported: returns AV1PF_ERR_UNSUPPORTED */
+#define AV1PF_STAGE_SUPERRES   0x04u  /* needs av1pf_request.upscaled */
 #define AV1PF_STAGE_LR         0x08u
 #define AV1PF_STAGE_FILMGRAIN  0x10u
 
@@ -168,6 +168,15 @@ typedef struct av1pf_request {
     const av1pf_lr_params      *lr;
     const av1pf_grain_params   *grain;
     const av1pf_frame          *display;    /* film-grain output frame (separate buffer) */
+    /* AV1PF_STAGE_SUPERRES: the upscaled frame (FrameWorkSuperResPostFilterRequest
+     * output, postfilter_superres.go:85-95).  Deblocking and CDEF run on the
+     * coded frame, which keeps their result; the upscale writes this frame's
+     * visible samples; loop restoration (with its deblocked context rows
+     * upscaled from the coded frame) and film grain run on it, so lr and
+     * display are validated against this frame.  Same bit depth, format and
+     * height as the coded frame, width in (coded width, 2 * coded width],
+     * no overlap with the coded frame's planes.                            */
+    const av1pf_frame          *upscaled;
 } av1pf_request;
 
 typedef struct av1pf_ctx av1pf_ctx;
@@ -181,7 +190,9 @@ const char *av1pf_last_error(const av1pf_ctx *ctx);
 
 /* Run the remaining post-filter stages of one frame in place on `frame`
  * (film grain goes to req->display).  Replaces FrameWorkPostFilterRunner.Apply
- * (decoder/work.go:153-155, decoder_postfilter_bind.go:414).               */
+ * (decoder/work.go:153-155, decoder_postfilter_bind.go:414); with
+ * AV1PF_STAGE_SUPERRES it is the caller chain ApplyCallerPostFilters
+ * (postfilter.go:947-979), which moves LR and film grain to req->upscaled.  */
 int  av1pf_apply(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *req, void *stream);
 
 /* Run many frames of mixed size, bit depth and chroma format.  One launch per
@@ -220,12 +231,21 @@ int  av1pf_sync(av1pf_ctx *ctx, void *stream);
  *   lr       -> ApplyLoopRestorationPostFilter (postfilter.go:1277); needs the two
  *               boundary saves, made by av1pf_lr_save_boundaries
  *   grain    -> ApplyFilmGrainPostFilter       (postfilter_filmgrain.go:516)
- * The chain av1pf_apply composes them; these exist for stage-level callers.  */
+ *   superres -> ApplySuperResPostFilter        (postfilter_superres.go:198)
+ * The chain av1pf_apply composes them; these exist for stage-level callers.
+ * LR after a super-resolution upscale needs its deblocked context rows
+ * upscaled from the coded frame: run it through the chain.                  */
 int  av1pf_deblock(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_deblock_params *p, void *stream);
 int  av1pf_cdef(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_cdef_params *p, void *stream);
 int  av1pf_lr_save_boundaries(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, int after_cdef, void *stream);
 int  av1pf_lr(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, void *stream);
 int  av1pf_filmgrain(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *dst, const av1pf_grain_params *p, void *stream);
+/* Super-resolution upscale of every plane of `coded` into `upscaled`
+ * (ApplySuperResPostFilter, decoder/postfilter_superres.go:198-264, one
+ * superres.UpscalePlaneCoded per plane, superres/superres.go:38-60): each
+ * visible row is resampled with AV1's 8-tap normative filter from the coded
+ * plane width to the upscaled one, reading the MI-aligned coded span.      */
+int  av1pf_superres(av1pf_ctx *ctx, const av1pf_frame *coded, const av1pf_frame *upscaled, void *stream);
 
 /* ---- encoder CDEF strength search (cdef_search.go:580-749) ---------------
  * For every 64x64 filter block that has a non-skip 8x8 block, evaluate the 64
@@ -365,7 +385,7 @@ int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, 
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
  * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13 lr_search_exhaustive,
- * 14..15 reserved.                                                          */
+ * 14 superres, 15 reserved.                                                 */
 #define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],

@@ -72,8 +72,15 @@ int  orc_fg_chroma_grain(int16_t *grain, const int16_t *luma, const av1pf_grain_
                          int bd, int plane, int ssx, int ssy);
 int  orc_filmgrain(const orc_frame *src, orc_frame *dst, const av1pf_grain_params *g);
 
-/* ---------------- chain (decoder/postfilter.go:589-845) */
+/* ---------------- super-resolution (superres/superres.go, decoder/postfilter_superres.go) */
+const int16_t *orc_superres_filter(int phase);
+int  orc_superres_row(const uint16_t *src, int srcw, int codedw, uint16_t *dst, int dstw, int bd);
+int  orc_superres(const orc_frame *coded, orc_frame *up);
+int  orc_lr_save_superres(const orc_frame *coded, const orc_frame *up, const av1pf_lr_params *p, orc_lr_bounds b[3]);
+
+/* ---------------- chain (decoder/postfilter.go:589-845, 947-979) */
 int  orc_chain(orc_frame *f, const av1pf_request *req, orc_frame *display);
+int  orc_chain_sr(orc_frame *f, const av1pf_request *req, orc_frame *up, orc_frame *display);
 
 /* ---------------- encoder CDEF search (encoder/cdef_search.go) */
 int  orc_cdef_search_mse(const orc_frame *src, const orc_frame *rec, const uint64_t *noskip,

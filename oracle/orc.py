@@ -36,6 +36,11 @@ def lib():
                                                 C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.orc_lr_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         _lib.orc_lr_search_exhaustive.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_chain_sr.argtypes = [C.c_void_p, C.POINTER(abi.Request), C.c_void_p, C.c_void_p]
+        _lib.orc_superres_row.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]
+        _lib.orc_superres.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.orc_superres_filter.argtypes = [C.c_int]
+        _lib.orc_superres_filter.restype = C.POINTER(C.c_int16)
     return _lib
 
 
@@ -99,20 +104,56 @@ class HostRequest:
             self.req.grain = C.pointer(fr.grain)
 
 
-def run_chain(fr, stages=None):
-    """Run the oracle chain on a copy of fr; returns (filtered frame, display frame or None)."""
+def run_chain(fr, stages=None, with_upscaled=False):
+    """Run the oracle chain on a copy of fr; returns (filtered frame, display
+    frame or None), plus the upscaled frame (or None) when with_upscaled.  A
+    super-resolution request upscales into a copy of fr.upscaled."""
     out = fr.copy_pixels()
     hr = HostRequest(fr, stages)
     of = orc_frame(out)
+    up = uof = None
+    if hr.req.stages & abi.STAGE_SUPERRES:
+        up = fr.upscaled.copy_pixels()
+        uof = orc_frame(up)
     disp = None
     dof = None
     if hr.req.stages & abi.STAGE_FILMGRAIN:
-        disp = fr.copy_pixels()
+        disp = (up if up is not None else fr).copy_pixels()
         dof = orc_frame(disp)
-    rc = lib().orc_chain(C.byref(of), C.byref(hr.req), C.byref(dof) if dof is not None else None)
+    rc = lib().orc_chain_sr(C.byref(of), C.byref(hr.req), C.byref(uof) if uof is not None else None,
+                            C.byref(dof) if dof is not None else None)
     if rc != 0:
         raise RuntimeError(f"orc_chain failed: {rc}")
-    return out, disp
+    return (out, disp, up) if with_upscaled else (out, disp)
+
+
+def superres_row(src, coded_w, dst_w, bd):
+    """One row upscaled from the first coded_w of len(src) source samples."""
+    s = np.ascontiguousarray(src, np.uint16)
+    d = np.zeros(dst_w, np.uint16)
+    rc = lib().orc_superres_row(s.ctypes.data, len(s), int(coded_w), d.ctypes.data, int(dst_w), int(bd))
+    if rc != 0:
+        raise ValueError(f"orc_superres_row rejected the geometry: {rc}")
+    return d
+
+
+def superres_filter():
+    """The 64 x 8 upscale filter table the oracle uses."""
+    t = np.zeros((64, 8), np.int16)
+    for ph in range(64):
+        p = lib().orc_superres_filter(ph)
+        t[ph] = [p[k] for k in range(8)]
+    return t
+
+
+def superres(coded, up):
+    """Upscale every plane of `coded` into a copy of `up` (synth.FrameData)."""
+    out = up.copy_pixels()
+    of, uf = orc_frame(coded), orc_frame(out)
+    rc = lib().orc_superres(C.byref(of), C.byref(uf))
+    if rc != 0:
+        raise RuntimeError(f"orc_superres failed: {rc}")
+    return out
 
 
 def plane_md5(fr):

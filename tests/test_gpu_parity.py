@@ -299,3 +299,18 @@ def test_film_grain_variants(runner, case):
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
     assert_same(gdisp, wdisp, f"display {case}")
+
+
+def test_lr_chroma_unit_below_stripe_rejected(runner):
+    """Chroma LR units halve only for 4:2:0: a 32-sample unit on a 4:2:2 plane
+    (64-row stripes) is not a legal request and is refused before launch."""
+    from goav1_amd.lib import Av1pfError, DeviceFrame, DeviceSideData
+    rng = np.random.default_rng(422)
+    fr = synth.FrameData(128, 96, 8, 1, 0)
+    synth.fill_blocky(fr, rng)
+    synth.build_side_data(fr, rng, lr_unit=64)
+    assert fr.lr["unit_size"] == [64, 64, 64]
+    fr.lr["unit_size"] = [64, 32, 32]
+    fr.stages = abi.STAGE_LR
+    with pytest.raises(Av1pfError):
+        runner.apply(DeviceFrame(fr), DeviceSideData(fr))
