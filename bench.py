@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-search", action="store_true", help="skip the encoder-search legs")
+    ap.add_argument("--no-superres", action="store_true", help="skip the super-resolution leg")
     ap.add_argument("--learned-frames", type=int, default=20, help="held-out frames of the learned-search leg")
     ap.add_argument("--batch-frames", type=int, default=64,
                     help="frames in the mixed-batch leg (0 skips it)")
@@ -286,6 +287,68 @@ def batch_leg(runner, steps, nframes):
             "generation_s": round(gen_s, 1)}
 
 
+def superres_leg(runner, steps, nframes=8):
+    """Super-resolution frames: 1280 -> 1920 x 1080 10-bit 4:2:0 (denominator
+    12), deblock + CDEF on the coded frame, the upscale, LR and film grain on
+    the upscaled frame, `nframes` frames per av1pf_apply_batch step (four
+    distinct frames, each twice).  Mpx/s counts upscaled luma pixels; the
+    superres kernel's HBM rate counts its coded reads and upscaled writes."""
+    import torch
+    from goav1_amd import synth
+    from goav1_amd.lib import DeviceFrame, DeviceSideData
+    srcs = [synth.superres_frame(seed=700 + i, up_w=1920, h=1080, denom=12, bd=10, grain=True)
+            for i in range(min(4, nframes))]
+    frames = [srcs[i % len(srcs)] for i in range(nframes)]
+    base = [DeviceFrame(fr) for fr in frames]
+    ups = [DeviceFrame.empty_like(fr.upscaled) for fr in frames]
+    disps = [DeviceFrame.empty_like(fr.upscaled) for fr in frames]
+    sides = [DeviceSideData(fr, display=d, upscaled=u) for fr, d, u in zip(frames, disps, ups)]
+    stream = torch.cuda.current_stream()
+
+    def run(nsteps, profile=False):
+        rows = []
+        for _ in range(nsteps):
+            row = []
+            for b in base:
+                c = DeviceFrame.__new__(DeviceFrame)
+                c.meta, c.bd = b.meta, b.bd
+                c.planes = [t.clone() for t in b.planes]
+                c.abi = c._abi()
+                row.append(c)
+            rows.append(row)
+        torch.cuda.synchronize()
+        if profile:
+            runner.profile(True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for row in rows:
+            runner.apply_batch(row, sides)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        runner.sync()
+        prof = runner.profile_read() if profile else None
+        if profile:
+            runner.profile(False)
+        return ev0.elapsed_time(ev1) / nsteps, prof
+
+    runner.autotune([DeviceFrame(fr) for fr in frames], sides)
+    runner.sync()
+    run(2)
+    ms, _ = run(steps)
+    _, prof = run(2, profile=True)
+    luma = sum(fr.upscaled.w * fr.upscaled.h for fr in frames)
+    stages = {k: {"avg_us": round(1e3 * v[0] / v[1], 1),
+                  "gbs": round((v[2] / v[1]) / ((v[0] / v[1]) / 1e3) / 1e9, 1) if v[2] else None}
+              for k, v in prof.items()}
+    sr = stages.get("superres", {})
+    if sr.get("gbs"):
+        sr["hbm_frac"] = round(sr["gbs"] / HBM_PEAK_GBS, 4)
+    return {"workload": f"{nframes} x 1280->1920x1080 10-bit 4:2:0 (denominator 12): deblock+CDEF coded, "
+                        "superres, LR+film grain upscaled, one av1pf_apply_batch per step",
+            "mpx_s": round(luma / (ms / 1e3) / 1e6, 1), "ms_per_step": round(ms, 3), "stages": stages}
+
+
 def traffic_for(kernel):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -428,6 +491,8 @@ def main():
             out["learned_search"] = learned_leg(runner, args.learned_frames)
     if args.batch_frames:
         out["batch"] = batch_leg(runner, max(2, args.steps // 2), args.batch_frames)
+    if not args.no_superres:
+        out["superres"] = superres_leg(runner, max(2, args.steps // 2))
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds / 2, cores)
