@@ -578,7 +578,7 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             double samples = 0;
             for (int pl = 0; pl < A.nplanes; pl++) {
                 const SrPlane &g = D.sr[pl];
-                P.sr.add(i, pl, (g.dstw + 255) / 256, (g.rows + SR_ROWS - 1) / SR_ROWS);
+                P.sr.add(i, pl, (g.dstw + SR_COLS - 1) / SR_COLS, (g.rows + SR_ROWS * SR_CHUNKS - 1) / (SR_ROWS * SR_CHUNKS));
                 samples += (double)(g.codedw + g.dstw) * g.rows;
             }
             P.bytes_sr += samples * bps_of(A.bd);

@@ -182,7 +182,9 @@ hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, c
 // src < 0: each frame's fin view
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
 hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
-constexpr int SR_ROWS = 8;         // rows per super-resolution tile (tile = 256 output columns)
+constexpr int SR_COLS = 512;       // output columns per super-resolution tile
+constexpr int SR_ROWS = 8;         // rows per super-resolution chunk
+constexpr int SR_CHUNKS = 1;       // chunks per super-resolution tile
 hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s);
 hipError_t launch_grain_apply(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L,
                               hipStream_t s);

@@ -74,7 +74,7 @@ def small_frame(seed, w, h, bd, ss, lr_unit=64, grain=False, lr_types=(abi.LR_SW
 STAGE_SETS = [abi.STAGE_DEBLOCK, abi.STAGE_CDEF, abi.STAGE_LR,
               abi.STAGE_DEBLOCK | abi.STAGE_CDEF | abi.STAGE_LR]
 GEOMS = [(200, 136, 8, 1), (200, 136, 10, 1), (136, 72, 8, 0), (136, 72, 10, 0),
-         (66, 66, 8, 1), (426, 240, 10, 1), (90, 50, 8, 1)]
+         (66, 66, 8, 1), (426, 240, 10, 1), (90, 50, 8, 1), (200, 136, 12, 1), (136, 72, 12, 0)]
 
 
 @pytest.mark.parametrize("geom", GEOMS, ids=lambda g: f"{g[0]}x{g[1]}_{g[2]}b_ss{g[3]}")
@@ -88,7 +88,7 @@ def test_stage_parity(runner, geom, stages):
     assert_same(got, want, f"{w}x{h} bd{bd} ss{ss} stages {stages:#x}")
 
 
-@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("bd", [8, 10, 12])
 @pytest.mark.parametrize("ss", [0, 1])
 def test_film_grain_parity(runner, bd, ss):
     fr = small_frame(77 + bd + ss, 232, 150, bd, ss, grain=True)
@@ -97,6 +97,22 @@ def test_film_grain_parity(runner, bd, ss):
     got, gdisp = gpu_run(runner, fr)
     assert_same(got, want, "filtered frame")
     assert_same(gdisp, wdisp, "display frame")
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_chain_422(runner, bd):
+    """4:2:2 through every stage (chroma LR units as large as luma units)."""
+    rng = np.random.default_rng(4220 + bd)
+    fr = synth.FrameData(232, 150, bd, 1, 0)
+    synth.fill_blocky(fr, rng)
+    synth.build_side_data(fr, rng, lf_random=True, sharpness=int(rng.integers(0, 8)), lr_unit=64)
+    fr.grain = synth.random_grain(rng, bd)
+    fr.stages = abi.STAGES_ALL
+    print(f"4:2:2 chain bd={bd}", flush=True)
+    want, wdisp = orc.run_chain(fr)
+    got, gdisp = gpu_run(runner, fr)
+    assert_same(got, want, "4:2:2 filtered frame")
+    assert_same(gdisp, wdisp, "4:2:2 display frame")
 
 
 @pytest.mark.parametrize("unit", [64, 128, 256])
