@@ -93,6 +93,18 @@ class GrainParams(C.Structure):
     ]
 
 
+class LFBuildParams(C.Structure):
+    _fields_ = [("mi_cols", C.c_int32), ("mi_rows", C.c_int32), ("ss_x", C.c_int32), ("ss_y", C.c_int32),
+                ("mono", C.c_int32), ("level_y", C.c_uint8 * 2), ("level_u", C.c_uint8), ("level_v", C.c_uint8),
+                ("mode_ref_deltas", C.c_uint8), ("delta_lf_multi", C.c_uint8), ("seg_enabled", C.c_uint8),
+                ("pad", C.c_uint8), ("ref_delta", C.c_int8 * 8), ("mode_delta", C.c_int8 * 2),
+                ("pad2", C.c_int8 * 2), ("seg_delta", (C.c_int16 * 4) * 8)]
+
+
+LF_VALID, LF_SKIP, LF_INTRA, LF_HAS_UV = 0x01, 0x02, 0x04, 0x08
+LF_RECORD_BYTES = 20
+
+
 class Request(C.Structure):
     _fields_ = [
         ("stages", C.c_uint32), ("pad", C.c_uint32),

@@ -123,7 +123,8 @@ struct av1pf_ctx {
 namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
-       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH, ST_SUPERRES };
+       ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH, ST_SUPERRES,
+       ST_DEBLOCK_BUILD };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -1166,6 +1167,28 @@ int av1pf_lr(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_lr_params *p, v
     const bool saved = c->bnd_frame == frame->plane[0];
     c->bnd_frame = nullptr;
     return run_batch(c, 1, frame, &r, static_cast<hipStream_t>(stream), saved);
+}
+
+int av1pf_deblock_build(av1pf_ctx *c, const av1pf_lf_record *records, const av1pf_lf_build_params *p,
+                        uint8_t *level_cache, av1pf_lf_mask *masks, void *stream)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    if (!records || !p || !level_cache || !masks) return fail(c, AV1PF_ERR_INVALID_ARG, "deblock build: null argument");
+    if (reinterpret_cast<uintptr_t>(level_cache) & 3)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "deblock build: level cache must be 4-byte aligned");
+    // the MI grid of a frame is ((W+7)>>3)<<1 by ((H+7)>>3)<<1: even, at most 16384 pixels
+    if (p->mi_cols <= 0 || p->mi_rows <= 0 || (p->mi_cols & 1) || (p->mi_rows & 1) || p->mi_cols > 4096 ||
+        p->mi_rows > 4096)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "deblock build: MI grid must be even and at most 4096 cells a side");
+    if (p->ss_x < 0 || p->ss_x > 1 || p->ss_y < 0 || p->ss_y > 1 || p->ss_y > p->ss_x)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "deblock build: unsupported chroma subsampling");
+    if (p->level_y[0] > 63 || p->level_y[1] > 63 || p->level_u > 63 || p->level_v > 63)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "deblock build: level out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StageTimer t(c, ST_DEBLOCK_BUILD, s, 0);
+    HIPCHK(c, launch_deblock_build(records, *p, level_cache, masks, s));
+    return AV1PF_OK;
 }
 
 int av1pf_superres(av1pf_ctx *c, const av1pf_frame *coded, const av1pf_frame *upscaled, void *stream)

@@ -182,6 +182,8 @@ hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, c
 // src < 0: each frame's fin view
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
 hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+hipError_t launch_deblock_build(const av1pf_lf_record *rec, const av1pf_lf_build_params &p, uint8_t *level_cache,
+                                av1pf_lf_mask *masks, hipStream_t s);
 constexpr int SR_COLS = 512;       // output columns per super-resolution tile
 constexpr int SR_ROWS = 8;         // rows per super-resolution chunk
 constexpr int SR_CHUNKS = 1;       // chunks per super-resolution tile

@@ -466,6 +466,129 @@ def search_dc_quant(q):
 BATCH_SIZES = [(426, 240), (640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
 
+# ---------------------------------------------------------------- loop-filter record maps
+LF_RECORD = np.dtype([('mi_col', '<u2'), ('mi_row', '<u2'), ('size', 'u1'), ('tx_y', 'u1'), ('tx_uv', 'u1'),
+                      ('flags', 'u1'), ('split', '<u2', (2,)), ('segment', 'u1'), ('ref', 'u1'), ('mode', 'u1'),
+                      ('pad', 'u1'), ('delta_lf', 'i1', (4,))])
+assert LF_RECORD.itemsize == 20
+# tile.BlockSize (dav1d order) as (width, height) in 4x4 units
+BLOCK_W4H4 = [(32, 32), (32, 16), (16, 32), (16, 16), (16, 8), (16, 4), (8, 16), (8, 8), (8, 4), (8, 2), (4, 16),
+              (4, 8), (4, 4), (4, 2), (4, 1), (2, 8), (2, 4), (2, 2), (2, 1), (1, 4), (1, 2), (1, 1)]
+# tile.TransformSize as (log2 width, log2 height) in 4x4 units, and its one-split size
+TX_L2 = [(0, 0), (1, 1), (2, 2), (3, 3), (4, 4), (0, 1), (1, 0), (1, 2), (2, 1), (2, 3), (3, 2), (3, 4), (4, 3),
+         (0, 2), (2, 0), (1, 3), (3, 1), (2, 4), (4, 2)]
+TX_SUB = [0, 0, 1, 2, 3, 0, 0, 1, 1, 2, 2, 3, 3, 5, 6, 7, 8, 9, 10]
+_BS_INDEX = {d: i for i, d in enumerate(BLOCK_W4H4)}
+_TX_INDEX = {d: i for i, d in enumerate(TX_L2)}
+
+
+def _largest_tx(w4, h4, cap_l2):
+    lw, lh = min(int(w4).bit_length() - 1, cap_l2), min(int(h4).bit_length() - 1, cap_l2)
+    lw, lh = min(lw, lh + 2), min(lh, lw + 2)   # transforms are at most 4:1
+    return _TX_INDEX[(lw, lh)]
+
+
+def record_map(rng, w, h, ssx=1, ssy=1, mono=False, intra_prob=0.4, skip_prob=0.3, delta_lf=False,
+               split_prob=0.5):
+    """A loop-filter record map for a w x h frame: a random AV1 partition of
+    every 128x128 superblock (NONE / HORZ / VERT / SPLIT / HORZ_4 / VERT_4),
+    intra blocks with one transform size (the largest or one or two splits
+    smaller), inter blocks with the largest transform and random split words,
+    random skip, segment, reference, mode class and delta_lf state.  Returns
+    (records [mi_rows, mi_cols] of LF_RECORD, list of block origins)."""
+    cols, rows = ((w + 7) >> 3) << 1, ((h + 7) >> 3) << 1
+    recs = np.zeros((rows, cols), LF_RECORD)
+    blocks = []
+
+    def emit(x, y, w4, h4):
+        if x >= cols or y >= rows:
+            return
+        r = np.zeros((), LF_RECORD)
+        bs = _BS_INDEX[(w4, h4)]
+        intra = rng.random() < intra_prob
+        skip = rng.random() < skip_prob
+        tmax = _largest_tx(w4, h4, 4)
+        if intra:
+            tx = tmax
+            for _ in range(int(rng.integers(0, 3))):
+                tx = TX_SUB[tx]
+            r['tx_y'] = tx
+        else:
+            r['tx_y'] = tmax
+            if rng.random() < split_prob:
+                r['split'][0] = int(rng.integers(0, 1 << 16))
+                r['split'][1] = int(rng.integers(0, 1 << 16))
+        cw, ch = max(w4 >> ssx, 1), max(h4 >> ssy, 1)
+        r['tx_uv'] = _largest_tx(cw, ch, 3)
+        has_uv = (not mono) and (w4 > ssx or x & 1) and (h4 > ssy or y & 1)
+        r['flags'] = 1 | (2 if skip else 0) | (4 if intra else 0) | (8 if has_uv else 0)
+        r['mi_col'], r['mi_row'], r['size'] = x, y, bs
+        r['segment'] = int(rng.integers(0, 8))
+        r['ref'] = 0 if intra else int(rng.integers(1, 8))
+        r['mode'] = 0 if intra else int(rng.integers(0, 2))
+        if delta_lf:
+            r['delta_lf'] = rng.integers(-63, 64, size=4)
+        recs[y:min(y + h4, rows), x:min(x + w4, cols)] = r
+        blocks.append((x, y))
+
+    def part(x, y, n):
+        if x >= cols or y >= rows:
+            return
+        opts = ['none', 'horz', 'vert', 'split']
+        if n in (4, 8, 16):
+            opts += ['horz4', 'vert4']
+        weights = np.array([1.0, 1.0, 1.0, 3.0 if n > 4 else 1.5] + [0.5] * (len(opts) - 4))
+        if n == 32:
+            weights[0] = 0.2
+        kind = opts[int(rng.choice(len(opts), p=weights / weights.sum()))]
+        if kind == 'none':
+            emit(x, y, n, n)
+        elif kind == 'horz':
+            emit(x, y, n, n // 2)
+            emit(x, y + n // 2, n, n // 2)
+        elif kind == 'vert':
+            emit(x, y, n // 2, n)
+            emit(x + n // 2, y, n // 2, n)
+        elif kind == 'horz4':
+            for k in range(4):
+                emit(x, y + k * n // 4, n, n // 4)
+        elif kind == 'vert4':
+            for k in range(4):
+                emit(x + k * n // 4, y, n // 4, n)
+        elif n == 2:
+            for k in range(4):
+                emit(x + (k & 1), y + (k >> 1), 1, 1)
+        else:
+            for k in range(4):
+                part(x + (k & 1) * n // 2, y + (k >> 1) * n // 2, n // 2)
+
+    for sy in range(0, rows, 32):
+        for sx in range(0, cols, 32):
+            part(sx, sy, 32)
+    return recs, blocks
+
+
+def lf_build_params(rng, w, h, ssx=1, ssy=1, mono=False, seg=True, deltas=True, multi=False):
+    """Random loop-filter header fields for av1pf_lf_build_params."""
+    from . import abi
+    p = abi.LFBuildParams()
+    p.mi_cols, p.mi_rows = ((w + 7) >> 3) << 1, ((h + 7) >> 3) << 1
+    p.ss_x, p.ss_y, p.mono = ssx, ssy, int(mono)
+    lv = lambda: int(rng.choice([0, int(rng.integers(28, 36)), int(rng.integers(0, 64)), int(rng.integers(0, 64))]))
+    p.level_y[0], p.level_y[1] = max(lv(), 1), lv()
+    p.level_u, p.level_v = lv(), lv()
+    p.mode_ref_deltas = int(deltas)
+    p.delta_lf_multi = int(multi)
+    p.seg_enabled = int(seg)
+    for i in range(8):
+        p.ref_delta[i] = int(rng.integers(-63, 64))
+        for k in range(4):
+            p.seg_delta[i][k] = int(rng.integers(-63, 64)) if rng.random() < 0.7 else 0
+    for i in range(2):
+        p.mode_delta[i] = int(rng.integers(-63, 64))
+    return p
+
+
 def superres_coded_width(up_w, denom):
     """Coded width of a super-resolution frame: (UpscaledWidth * 8 + denom / 2)
     / denom, at least min(16, UpscaledWidth) (AV1 superres_params)."""

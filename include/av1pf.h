@@ -159,6 +159,49 @@ typedef struct av1pf_grain_params {
     uint16_t cr_offset;
 } av1pf_grain_params;
 
+/* ---- deblocking side data built on the device ------------------------------
+ * One record per MI (4x4 luma) cell of the frame's loop-filter record map
+ * (threading.FrameWorkLoopFilterBlockRecord, loop_filter_map.go:46-60): every
+ * cell a block covers holds that block's record, cells covered by no block are
+ * not valid.  Enumerations are goav1's (dav1d's): size is tile.BlockSize
+ * (0 = 128x128 ... 21 = 4x4, partition.go:43-66), tx_y / tx_uv are
+ * tile.TransformSize (0 = 4x4 ... 18 = 64x16, tx_size.go:20-40).           */
+#define AV1PF_LF_VALID   0x01u
+#define AV1PF_LF_SKIP    0x02u   /* SkipTransform */
+#define AV1PF_LF_INTRA   0x04u
+#define AV1PF_LF_HAS_UV  0x08u   /* TransformTree.HasUV */
+
+typedef struct av1pf_lf_record {
+    uint16_t mi_col, mi_row;  /* Block.MICol / MIRow: the block's top-left cell */
+    uint8_t  size;            /* Block.Size */
+    uint8_t  tx_y, tx_uv;     /* TransformTree.Y (largest luma transform) / .UV */
+    uint8_t  flags;           /* AV1PF_LF_* */
+    uint16_t split[2];        /* TransformTree.Split: inter transform split bits per depth */
+    uint8_t  segment;         /* SegmentID */
+    uint8_t  ref;             /* RefFrame: 0 intra, 1..7 LAST..ALTREF */
+    uint8_t  mode;            /* loopfilter.ModeDeltaClass: 0 GLOBALMV, 1 any other */
+    uint8_t  pad;
+    int8_t   delta_lf[4];     /* the block's delta_lf state: DeltaLF[0..3] with
+                                 delta_lf_multi, else {DeltaLFFromBase, ..}; zero
+                                 when delta_lf_present is off */
+} av1pf_lf_record;
+
+/* Frame-header fields of the level computation (lfmask.LFValueParams,
+ * values.go:15-26) and the MI grid.                                         */
+typedef struct av1pf_lf_build_params {
+    int32_t mi_cols, mi_rows;  /* ((W+7)>>3)<<1, ((H+7)>>3)<<1 */
+    int32_t ss_x, ss_y, mono;
+    uint8_t level_y[2], level_u, level_v;  /* loop_filter_level[0..3] */
+    uint8_t mode_ref_deltas;   /* loop_filter_delta_enabled */
+    uint8_t delta_lf_multi;
+    uint8_t seg_enabled;
+    uint8_t pad;
+    int8_t  ref_delta[8];      /* loop_filter_ref_deltas */
+    int8_t  mode_delta[2];     /* loop_filter_mode_deltas */
+    int8_t  pad2[2];
+    int16_t seg_delta[8][4];   /* per segment: delta_lf_y_v, _y_h, _u, _v */
+} av1pf_lf_build_params;
+
 /* ---- one frame's request (FrameWorkPostFilterRequest, postfilter.go:278-284) */
 typedef struct av1pf_request {
     uint32_t stages;                        /* AV1PF_STAGE_* still to run */
@@ -240,6 +283,18 @@ int  av1pf_cdef(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_cdef_param
 int  av1pf_lr_save_boundaries(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, int after_cdef, void *stream);
 int  av1pf_lr(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_lr_params *p, void *stream);
 int  av1pf_filmgrain(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *dst, const av1pf_grain_params *p, void *stream);
+/* Build the deblocking side data on the device from the record map (device
+ * array of mi_rows * mi_cols records): the edge masks as
+ * FrameWorkLoopFilterMasks.BuildFromMap derives them from neighbouring
+ * records (threading/loop_filter_masks.go:452-546: per block the lfmask
+ * builders CreateIntra / CreateInter, lfmask/build.go:59,197, mask.go:76-528)
+ * and the level cache as the decode-time build fills it (build.go:232,
+ * levels from loop_filter_masks.go:88-123 and values.go:57-119).  masks
+ * receives sb128_w * sb128_h records, every word written; level_cache
+ * receives 4 * mi_cols * mi_rows bytes, cells no valid block covers (and
+ * chroma bytes of cells no chroma block covers) left as they were.       */
+int  av1pf_deblock_build(av1pf_ctx *ctx, const av1pf_lf_record *records, const av1pf_lf_build_params *p,
+                         uint8_t *level_cache, av1pf_lf_mask *masks, void *stream);
 /* Super-resolution upscale of every plane of `coded` into `upscaled`
  * (ApplySuperResPostFilter, decoder/postfilter_superres.go:198-264, one
  * superres.UpscalePlaneCoded per plane, superres/superres.go:38-60): each
@@ -385,7 +440,7 @@ int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, 
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
  * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13 lr_search_exhaustive,
- * 14 superres, 15 reserved.                                                 */
+ * 14 superres, 15 deblock_build.                                            */
 #define AV1PF_PROFILE_STAGES 16
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],

@@ -72,6 +72,13 @@ int  orc_fg_chroma_grain(int16_t *grain, const int16_t *luma, const av1pf_grain_
                          int bd, int plane, int ssx, int ssy);
 int  orc_filmgrain(const orc_frame *src, orc_frame *dst, const av1pf_grain_params *g);
 
+/* ---------------- deblocking side data from the record map
+ * (threading/loop_filter_masks.go BuildFromMap, lfmask/) */
+int  orc_deblock_build(const av1pf_lf_record *map, const av1pf_lf_build_params *p, uint8_t *level_cache,
+                       av1pf_lf_mask *masks);
+void orc_lf_block_levels(const av1pf_lf_build_params *p, const av1pf_lf_record *r, uint8_t out[4]);
+const uint8_t *orc_lf_tables(int which);
+
 /* ---------------- super-resolution (superres/superres.go, decoder/postfilter_superres.go) */
 const int16_t *orc_superres_filter(int phase);
 int  orc_superres_row(const uint16_t *src, int srcw, int codedw, uint16_t *dst, int dstw, int bd);

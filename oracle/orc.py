@@ -41,6 +41,8 @@ def lib():
         _lib.orc_superres.argtypes = [C.c_void_p, C.c_void_p]
         _lib.orc_superres_filter.argtypes = [C.c_int]
         _lib.orc_superres_filter.restype = C.POINTER(C.c_int16)
+        _lib.orc_deblock_build.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_lf_block_levels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -135,6 +137,20 @@ def superres_row(src, coded_w, dst_w, bd):
     if rc != 0:
         raise ValueError(f"orc_superres_row rejected the geometry: {rc}")
     return d
+
+
+def deblock_build(records, params, level_cache=None):
+    """Masks (uint16 array [sbh, sbw, 640]) and level cache (uint8 [rows*cols*4])
+    from a record map (synth.LF_RECORD [rows, cols]) and abi.LFBuildParams."""
+    rows, cols = records.shape
+    rec = np.ascontiguousarray(records)
+    lc = np.zeros(rows * cols * 4, np.uint8) if level_cache is None else level_cache.copy()
+    sbw, sbh = (cols + 31) >> 5, (rows + 31) >> 5
+    masks = np.zeros((sbh, sbw, 640), np.uint16)
+    rc = lib().orc_deblock_build(rec.ctypes.data, C.byref(params), lc.ctypes.data, masks.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_deblock_build failed: {rc}")
+    return masks, lc
 
 
 def superres_filter():

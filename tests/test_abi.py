@@ -42,6 +42,9 @@ def test_struct_layout_matches_header(tmp_path):
 int main(void) {
   P(av1pf_frame); P(av1pf_lf_mask); P(av1pf_deblock_params); P(av1pf_cdef_params);
   P(av1pf_lr_unit); P(av1pf_lr_params); P(av1pf_grain_params); P(av1pf_request); P(av1pf_cdef_decision);
+  P(av1pf_lf_record); P(av1pf_lf_build_params);
+  O(av1pf_lf_record, split); O(av1pf_lf_record, delta_lf); O(av1pf_lf_build_params, ref_delta);
+  O(av1pf_lf_build_params, seg_delta); O(av1pf_request, upscaled);
   O(av1pf_frame, plane); O(av1pf_frame, stride); O(av1pf_deblock_params, level_cache);
   O(av1pf_cdef_params, index); O(av1pf_lr_params, units); O(av1pf_grain_params, ar_coeffs_cr);
   O(av1pf_grain_params, cr_offset); O(av1pf_request, display);
@@ -53,7 +56,11 @@ int main(void) {
     sizes = {"av1pf_frame": abi.Frame, "av1pf_lf_mask": abi.LFMask, "av1pf_deblock_params": abi.DeblockParams,
              "av1pf_cdef_params": abi.CdefParams, "av1pf_lr_unit": abi.LRUnit, "av1pf_lr_params": abi.LRParams,
              "av1pf_grain_params": abi.GrainParams, "av1pf_request": abi.Request,
-             "av1pf_cdef_decision": abi.CdefDecision}
+             "av1pf_cdef_decision": abi.CdefDecision, "av1pf_lf_build_params": abi.LFBuildParams}
+    from goav1_amd import synth
+    assert int(out["av1pf_lf_record"]) == synth.LF_RECORD.itemsize == abi.LF_RECORD_BYTES
+    for f in ("split", "delta_lf"):
+        assert int(out[f"av1pf_lf_record.{f}"]) == synth.LF_RECORD.fields[f][1], f
     for name, cls in sizes.items():
         assert int(out[name]) == C.sizeof(cls), name
     offs = {"av1pf_frame.plane": abi.Frame.plane.offset, "av1pf_frame.stride": abi.Frame.stride.offset,
@@ -62,7 +69,10 @@ int main(void) {
             "av1pf_lr_params.units": abi.LRParams.units.offset,
             "av1pf_grain_params.ar_coeffs_cr": abi.GrainParams.ar_coeffs_cr.offset,
             "av1pf_grain_params.cr_offset": abi.GrainParams.cr_offset.offset,
-            "av1pf_request.display": abi.Request.display.offset}
+            "av1pf_request.display": abi.Request.display.offset,
+            "av1pf_request.upscaled": abi.Request.upscaled.offset,
+            "av1pf_lf_build_params.ref_delta": abi.LFBuildParams.ref_delta.offset,
+            "av1pf_lf_build_params.seg_delta": abi.LFBuildParams.seg_delta.offset}
     for name, off in offs.items():
         assert int(out[name]) == off, name
 
