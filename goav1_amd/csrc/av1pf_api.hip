@@ -110,6 +110,9 @@ struct av1pf_ctx {
     int slot = 0;
     // stage-level LR: boundaries saved by av1pf_lr_save_boundaries for this frame
     const void *bnd_frame = nullptr;
+    // encoder CDEF apply: the device-side decision the next single-frame chain uses
+    const av1pf_cdef_decision *apply_dec = nullptr;
+    const uint8_t *apply_idx = nullptr;
     // per-stage timing
     bool profiling = false;
     struct Rec { int stage; hipEvent_t a, b; double bytes; };
@@ -556,6 +559,15 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             for (int k = 0; k < p->strength_count && k < 8; k++) {
                 if (p->y_strength[k]) cd.yfilt = 1;
                 if (!f->mono && p->uv_strength[k]) cd.uvfilt = 1;
+            }
+            if (i == 0 && c->apply_dec) {
+                // strengths and unit indices of a device-side search decision:
+                // known only on the device, so the stage always runs
+                cd.dec = c->apply_dec;
+                cd.fb_index = c->apply_idx;
+                cd.rows8 = (f->height + 7) >> 3;
+                cd.cols8 = (f->width + 7) >> 3;
+                cd.yfilt = cd.uvfilt = 1;
             }
             if (cd.yfilt || cd.uvfilt) {
                 cd.index = p->index;
@@ -1167,6 +1179,33 @@ int av1pf_lr(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_lr_params *p, v
     const bool saved = c->bnd_frame == frame->plane[0];
     c->bnd_frame = nullptr;
     return run_batch(c, 1, frame, &r, static_cast<hipStream_t>(stream), saved);
+}
+
+int av1pf_cdef_search_apply(av1pf_ctx *c, const av1pf_frame *recon, const uint64_t *noskip, int damping,
+                            const av1pf_cdef_decision *decision, const uint8_t *fb_index, void *stream)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    if (!decision || !fb_index) return fail(c, AV1PF_ERR_INVALID_ARG, "cdef apply: decision and indices needed");
+    int rc = validate_frame(c, recon);
+    if (rc) return rc;
+    av1pf_cdef_params p{};
+    p.damping = (uint8_t)damping;
+    p.strength_count = 8;   // the decision's count is read on the device
+    p.unit_cols = (recon->width + 63) / 64;
+    p.unit_rows = (recon->height + 63) / 64;
+    p.index = reinterpret_cast<const int8_t *>(fb_index);
+    p.noskip = noskip;
+    av1pf_request r{};
+    r.stages = AV1PF_STAGE_CDEF;
+    r.cdef = &p;
+    if ((rc = validate_request(c, recon, &r))) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    c->apply_dec = decision;
+    c->apply_idx = fb_index;
+    rc = run_batch(c, 1, recon, &r, static_cast<hipStream_t>(stream));
+    c->apply_dec = nullptr;
+    c->apply_idx = nullptr;
+    return rc;
 }
 
 int av1pf_deblock_build(av1pf_ctx *c, const av1pf_lf_record *records, const av1pf_lf_build_params *p,

@@ -174,14 +174,38 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ui = ur * a.unit_cols + uc;
-    int idx = a.index[ui];
-    if (idx >= a.strength_count) {
+    const uint64_t m = a.noskip ? a.noskip[ui] : ~0ull;
+    int idx, count, yfilt, uvfilt;
+    const uint8_t *ystr, *uvstr;
+    if (a.dec) {
+        // encoder apply (cdef_apply.go:202-228, the noskip path): a unit is
+        // read when one of its 8x8 blocks inside the frame has residual
+        count = 1 << a.dec->cdef_bits;
+        ystr = a.dec->y_strength;
+        uvstr = a.dec->uv_strength;
+        yfilt = uvfilt = 0;
+        for (int k = 0; k < count; k++) {
+            yfilt |= ystr[k] != 0;
+            uvfilt |= src.nplanes > 1 && uvstr[k] != 0;
+        }
+        const int rb = min(8, a.rows8 - ur * 8), cb = min(8, a.cols8 - uc * 8);
+        uint64_t in = 0;
+        for (int by = 0; by < rb; by++) in |= ((1ull << cb) - 1) << (by * 8);
+        idx = (m & in) ? a.fb_index[ui] : -1;
+    } else {
+        count = a.strength_count;
+        ystr = a.y_strength;
+        uvstr = a.uv_strength;
+        yfilt = a.yfilt;
+        uvfilt = a.uvfilt;
+        idx = a.index[ui];
+    }
+    if (idx >= count) {
         if (tid == 0) atomicOr(err, 1);
         idx = -1;
     }
-    const uint64_t m = a.noskip ? a.noskip[ui] : ~0ull;
     const int cs = src.bd - 8;
-    const bool chroma_filtering = a.uvfilt != 0;
+    const bool chroma_filtering = uvfilt != 0;
 
     for (int pl = 0; pl < src.nplanes; pl++) {
         const int xd = pl ? src.ssx : 0, yd = pl ? src.ssy : 0;
@@ -190,13 +214,13 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         const int ux = (uc * 64) >> xd, uy = (ur * 64) >> yd;
         const int uw = min(64 >> xd, sp.aw - ux), uh = min(64 >> yd, sp.ah - uy);
         if (uw <= 0 || uh <= 0) continue;
-        const bool process = pl == 0 ? (a.yfilt || chroma_filtering) : a.uvfilt;
+        const bool process = pl == 0 ? (yfilt || chroma_filtering) : uvfilt;
         bool active = false, dir_only = false, need_dir = false;
         uint8_t packed = 0;
         if (process && idx >= 0) {
-            packed = pl ? a.uv_strength[idx] : a.y_strength[idx];
-            const int uvpri = str_pri(a.uv_strength[idx]);
-            need_dir = pl == 0 ? (str_pri(a.y_strength[idx]) != 0 || (chroma_filtering && uvpri != 0)) : uvpri != 0;
+            packed = pl ? uvstr[idx] : ystr[idx];
+            const int uvpri = str_pri(uvstr[idx]);
+            need_dir = pl == 0 ? (str_pri(ystr[idx]) != 0 || (chroma_filtering && uvpri != 0)) : uvpri != 0;
             dir_only = pl == 0 && chroma_filtering && packed == 0 && uvpri != 0;
             active = packed != 0 || dir_only;
         }

@@ -22,9 +22,8 @@
 //     bottom-right luma row (column), 1 at the frame edge; inner edges lie on
 //     multiples of the chroma transform size;
 //   - horizontal edges likewise with heights and the cell above.
-// One workgroup per 128x128 region: 128 threads own the luma words
-// (direction, position, half) and 128 the chroma words; each thread walks the
-// 16 cells of its half-lane and writes its 3 (luma) or 2 (chroma) class words.
+// Four workgroups per 128x128 region (luma and chroma, each direction): a
+// thread classifies 4 cells and ORs them into the region's words in LDS.
 #include "pf_internal.h"
 
 namespace pf {
@@ -170,39 +169,55 @@ __device__ int chroma_edge(const LfMap &M, int dir, int cx, int cy)
     return own;
 }
 
+// One workgroup per (region, kind): kind 0/1 luma vertical / horizontal
+// edges, 2/3 chroma.  Each thread classifies 4 cells of the region's 32 x 32
+// (luma) grid and ORs the bits into per-(position, class) words in LDS; then
+// the words are stored, every word of the kind written.
 __global__ __launch_bounds__(256) void lf_mask_kernel(LfMap M, av1pf_lf_mask *masks, int sbw)
 {
-    const int rx = blockIdx.x % sbw, ry = blockIdx.x / sbw;
-    av1pf_lf_mask &m = masks[blockIdx.x];
+    __shared__ uint32_t w[32][3];
+    const int region = blockIdx.x >> 2, kind = blockIdx.x & 3;
+    const int rx = region % sbw, ry = region / sbw;
+    av1pf_lf_mask &m = masks[region];
     const int t = threadIdx.x;
-    if (t < 128) {
-        const int dir = t >> 6, pos = (t >> 1) & 31, half = t & 1;
-        uint32_t w[3] = { 0, 0, 0 };
-        for (int i = 0; i < 16; i++) {
-            const int along = half * 16 + i;
+    if (t < 96) (&w[0][0])[t] = 0;
+    __syncthreads();
+    const int dir = kind & 1;
+    const int pos = t & 31;
+    if (kind < 2) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int along = (t >> 5) * 4 + k;
             const int x = rx * 32 + (dir == 0 ? pos : along), y = ry * 32 + (dir == 0 ? along : pos);
             const int c = luma_edge(M, dir, x, y);
-            if (c >= 0) w[c] |= 1u << i;
+            if (c >= 0) atomicOr(&w[pos][c], 1u << along);
         }
-#pragma unroll
-        for (int c = 0; c < 3; c++) m.y[dir][pos][c][half] = (uint16_t)w[c];
+        __syncthreads();
+        if (t < 96) {
+            const int p = t / 3, c = t % 3;
+            m.y[dir][p][c][0] = (uint16_t)(w[p][c] & 0xffffu);
+            m.y[dir][p][c][1] = (uint16_t)(w[p][c] >> 16);
+        }
     } else {
-        const int u = t - 128;
-        const int dir = u >> 6, pos = (u >> 1) & 31, half = u & 1;
-        uint32_t w[2] = { 0, 0 };
         const int npos = 32 >> (dir == 0 ? M.ssx : M.ssy);
-        const int hn = 16 >> (dir == 0 ? M.ssy : M.ssx);   // cells per half along the lane
+        const int nalong = 32 >> (dir == 0 ? M.ssy : M.ssx), hn = nalong >> 1;
         if (M.chroma && pos < npos) {
             const int cx0 = (rx * 32) >> M.ssx, cy0 = (ry * 32) >> M.ssy;
-            for (int i = 0; i < hn; i++) {
-                const int along = half * hn + i;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int along = (t >> 5) * 4 + k;
+                if (along >= nalong) break;
                 const int cx = cx0 + (dir == 0 ? pos : along), cy = cy0 + (dir == 0 ? along : pos);
                 const int c = chroma_edge(M, dir, cx, cy);
-                if (c >= 0) w[c] |= 1u << i;
+                if (c >= 0) atomicOr(&w[pos][c], 1u << along);
             }
         }
-        m.uv[dir][pos][0][half] = (uint16_t)w[0];
-        m.uv[dir][pos][1][half] = (uint16_t)w[1];
+        __syncthreads();
+        if (t < 64) {
+            const int p = t >> 1, c = t & 1;
+            m.uv[dir][p][c][0] = (uint16_t)(w[p][c] & ((1u << hn) - 1));
+            m.uv[dir][p][c][1] = (uint16_t)(w[p][c] >> hn);
+        }
     }
 }
 
@@ -278,7 +293,7 @@ hipError_t launch_deblock_build(const av1pf_lf_record *rec, const av1pf_lf_build
     L.mode_delta[0] = p.mode_delta[0];
     L.mode_delta[1] = p.mode_delta[1];
     const int sbw = (p.mi_cols + 31) >> 5, sbh = (p.mi_rows + 31) >> 5;
-    hipLaunchKernelGGL(lf_mask_kernel, dim3(sbw * sbh), dim3(256), 0, s, M, masks, sbw);
+    hipLaunchKernelGGL(lf_mask_kernel, dim3(sbw * sbh * 4), dim3(256), 0, s, M, masks, sbw);
     const int n = p.mi_cols * p.mi_rows;
     hipLaunchKernelGGL(lf_level_kernel, dim3((n + 255) / 256), dim3(256), 0, s, M, L, reinterpret_cast<uint32_t *>(level_cache));
     return hipGetLastError();

@@ -49,6 +49,12 @@ struct CdefDesc {
     int damping, strength_count;
     int yfilt, uvfilt;
     uint8_t y_strength[8], uv_strength[8];
+    // encoder apply: strengths from a device-side search decision and the
+    // unit index from its per-block indices (then index and the strengths
+    // above are unused)
+    const av1pf_cdef_decision *dec;
+    const uint8_t *fb_index;
+    int rows8, cols8;              // 8x8 blocks inside the frame
 };
 
 // Super-resolution geometry of one plane (the position arithmetic is in

@@ -28,7 +28,8 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_nn_adam",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
            "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode",
-           "av1pf_lr_search_exhaustive", "av1pf_superres", "av1pf_deblock_build"]
+           "av1pf_lr_search_exhaustive", "av1pf_superres", "av1pf_deblock_build",
+           "av1pf_cdef_search_apply"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
                "lr_search_exhaustive", "superres", "deblock_build"]
@@ -69,6 +70,8 @@ def load():
     lib.av1pf_lr_save_boundaries.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_int, C.c_void_p]
     lib.av1pf_lr.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_void_p]
     lib.av1pf_superres.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame), C.c_void_p]
+    lib.av1pf_cdef_search_apply.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
     lib.av1pf_deblock_build.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.LFBuildParams), C.c_void_p, C.c_void_p,
                                         C.c_void_p]
     lib.av1pf_filmgrain.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Frame),
@@ -303,6 +306,13 @@ class PostFilterRunner:
     def superres(self, coded, upscaled, stream=None):
         """Upscale every plane of `coded` into `upscaled` (DeviceFrames)."""
         self._check(self.lib.av1pf_superres(self.h, C.byref(coded.abi), C.byref(upscaled.abi), self._stream(stream)))
+
+    def cdef_search_apply(self, recon, noskip_ptr, damping, decision_ptr, fb_index_ptr, stream=None):
+        """CDEF on `recon` (DeviceFrame, in place) with a device-side search
+        decision and per-block indices."""
+        self._check(self.lib.av1pf_cdef_search_apply(self.h, C.byref(recon.abi), C.c_void_p(noskip_ptr), int(damping),
+                                                     C.c_void_p(decision_ptr), C.c_void_p(fb_index_ptr),
+                                                     self._stream(stream)))
 
     def deblock_build(self, records_ptr, params, level_cache_ptr, masks_ptr, stream=None):
         """Build the deblock masks and level cache on the device from a record

@@ -335,6 +335,16 @@ typedef struct av1pf_cdef_decision {
 int  av1pf_cdef_search_select(av1pf_ctx *ctx, const av1pf_frame *recon, const uint64_t *noskip, const uint64_t *mse,
                               uint64_t lambda, av1pf_cdef_decision *decision, uint8_t *fb_index, void *stream);
 
+/* Apply the searched CDEF to the reconstruction on the device, in place, as
+ * the encoder's cdefApplier.apply does (encoder/cdef_apply.go:326-371: the
+ * decoder's CDEF with the frame's chosen parameters; the unit index map of
+ * markFromLoopFilterMap's noskip path, :202-228, reads a unit when one of its
+ * 8x8 blocks inside the frame has residual).  decision and fb_index are the
+ * DEVICE outputs of av1pf_cdef_search_select (no host round trip); noskip
+ * as for the search; damping = 3 + (qindex >> 6).  Asynchronous.           */
+int  av1pf_cdef_search_apply(av1pf_ctx *ctx, const av1pf_frame *recon, const uint64_t *noskip, int damping,
+                             const av1pf_cdef_decision *decision, const uint8_t *fb_index, void *stream);
+
 /* Full-search lambda (svtCDEFFullLambda, cdef_search.go:274-299) from the
  * 8-bit luma DC quantizer of the frame's qindex.  Needs no device.          */
 uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_factor);

@@ -302,3 +302,60 @@ def test_gpu_learned_search_is_consistent():
     finally:
         net.close()
         r.close()
+
+
+def in_frame_word(fr, fb):
+    """Bits of the 8x8 blocks of filter block fb that lie inside the frame."""
+    nhfb = (fr.w + 63) // 64
+    fr_, fc_ = divmod(fb, nhfb)
+    rows8, cols8 = (fr.h + 7) >> 3, (fr.w + 7) >> 3
+    word = 0
+    for by in range(min(8, rows8 - fr_ * 8)):
+        for bx in range(min(8, cols8 - fc_ * 8)):
+            word |= 1 << (by * 8 + bx)
+    return word
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2), (200, 136, 160, 3)])
+def test_gpu_search_apply_equals_decoder_cdef(case):
+    """The encoder's CDEF apply from the device decision (no host round trip)
+    equals the decoder CDEF run with that decision's parameters and the index
+    map of cdef_apply.go:202-228 (a unit is read when an in-frame 8x8 block has
+    residual)."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_lambda
+    w, h, q, seed = case
+    src, rec, noskip, damping = synth.search_pair(seed=70 + seed, w=w, h=h, q=q)
+    lam = cdef_lambda(synth.search_dc_quant(160)) if seed != 3 else 1 << 40
+    print(f"search apply {w}x{h} q{q} lambda {lam}", flush=True)
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        mse = torch.zeros((2, nfb, 64), dtype=torch.int64, device="cuda")
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.zeros((nfb,), dtype=torch.uint8, device="cuda")
+        r.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+        r.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), lam, dec.data_ptr(), idx.data_ptr())
+        r.cdef_search_apply(dr, ns.data_ptr(), damping, dec.data_ptr(), idx.data_ptr())
+        r.sync()
+        torch.cuda.synchronize()
+        got = rec.copy_pixels()
+        for pl, a in zip(got.planes, dr.to_numpy()):
+            pl.pix[:] = a
+        d = dec.cpu().numpy()
+        fbi = idx.cpu().numpy()
+    finally:
+        r.close()
+    bits = int(d[:4].view(np.int32)[0])
+    index = np.array([int(fbi[u]) if int(noskip[u]) & in_frame_word(rec, u) else -1 for u in range(nfb)], np.int8)
+    ref = rec.copy_pixels()
+    ref.cdef = dict(damping=damping, strength_count=1 << bits, y_strength=d[4:12].copy(), uv_strength=d[12:20].copy(),
+                    unit_cols=(rec.w + 63) // 64, unit_rows=(rec.h + 63) // 64, index=index,
+                    noskip=np.ascontiguousarray(noskip))
+    want, _ = orc.run_chain(ref, stages=abi.STAGE_CDEF)
+    assert orc.plane_md5(got) == orc.plane_md5(want)
+    if seed != 3:
+        assert orc.plane_md5(want) != orc.plane_md5(rec)   # something was filtered
