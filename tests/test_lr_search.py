@@ -388,3 +388,61 @@ def test_gpu_lr_search_exhaustive_parity(case):
     np.testing.assert_array_equal(gf, wf)
     np.testing.assert_array_equal(ge, we)
     np.testing.assert_array_equal(gu, wu)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 120, 1)])
+def test_gpu_encoder_post_search_flow(case):
+    """The encoder's filters after its searches, all on the device: save the
+    deblocked LR context rows, apply the searched CDEF from the device decision
+    (av1pf_cdef_search_apply), search LR on the CDEF output and apply the
+    device unit records (av1pf_lr) -- equal to the decoder chain (CDEF then LR)
+    run by the oracle with the same decisions."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, DeviceSideData, PostFilterRunner, cdef_lambda
+    from test_search import in_frame_word
+    w, h, q, seed = case
+    src, rec, noskip, damping = synth.search_pair(seed=90 + seed, w=w, h=h, q=q)
+    hz, vt = orc.lr_search_units(w, h, 64)
+    print(f"encoder flow {w}x{h} q{q}", flush=True)
+    r = PostFilterRunner()
+    try:
+        ds, dr = DeviceFrame(src), DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        nfb = noskip.shape[0]
+        mse = torch.zeros((2, nfb, 64), dtype=torch.int64, device="cuda")
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.zeros((nfb,), dtype=torch.uint8, device="cuda")
+        units = torch.zeros((hz * vt, 16), dtype=torch.uint8, device="cuda")
+        lrf = rec.copy_pixels()
+        lrf.deblock = lrf.cdef = lrf.grain = None
+        lrf.lr = dict(frame_type=[abi.LR_SWITCHABLE, abi.LR_NONE, abi.LR_NONE], unit_size=[64, 32, 32],
+                      units=[np.zeros((hz * vt, 16), np.uint8), None, None])
+        side = DeviceSideData(lrf, stages=abi.STAGE_LR)
+        side.req.lr.contents.units[0] = units.data_ptr()
+        r.cdef_search(ds, dr, ns.data_ptr(), damping, mse.data_ptr())
+        r.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), cdef_lambda(synth.search_dc_quant(q)), dec.data_ptr(),
+                             idx.data_ptr())
+        r.lr_save_boundaries(dr, side, False)
+        r.cdef_search_apply(dr, ns.data_ptr(), damping, dec.data_ptr(), idx.data_ptr())
+        r.lr_search(ds, dr, 64, units.data_ptr())
+        r.lr(dr, side)
+        r.sync()
+        torch.cuda.synchronize()
+        got = rec.copy_pixels()
+        for pl, a in zip(got.planes, dr.to_numpy()):
+            pl.pix[:] = a
+        d, fbi, u = dec.cpu().numpy(), idx.cpu().numpy(), units.cpu().numpy()
+    finally:
+        r.close()
+    bits = int(d[:4].view(np.int32)[0])
+    index = np.array([int(fbi[k]) if int(noskip[k]) & in_frame_word(rec, k) else -1 for k in range(nfb)], np.int8)
+    ref = rec.copy_pixels()
+    ref.deblock = ref.grain = None
+    ref.cdef = dict(damping=damping, strength_count=1 << bits, y_strength=d[4:12].copy(), uv_strength=d[12:20].copy(),
+                    unit_cols=(w + 63) // 64, unit_rows=(h + 63) // 64, index=index, noskip=np.ascontiguousarray(noskip))
+    ref.lr = dict(frame_type=[abi.LR_SWITCHABLE, abi.LR_NONE, abi.LR_NONE], unit_size=[64, 32, 32],
+                  units=[np.ascontiguousarray(u), None, None])
+    want, _ = orc.run_chain(ref, stages=abi.STAGE_CDEF | abi.STAGE_LR)
+    assert orc.plane_md5(got) == orc.plane_md5(want)
+    assert (u[:, 0] != abi.LR_NONE).any()   # the LR stage had units to apply
