@@ -349,6 +349,42 @@ def superres_leg(runner, steps, nframes=8):
             "mpx_s": round(luma / (ms / 1e3) / 1e6, 1), "ms_per_step": round(ms, 3), "stages": stages}
 
 
+def deblock_build_leg(runner, reps=50):
+    """Deblocking side data built on the device (av1pf_deblock_build) from a
+    random 1920x1080 4:2:0 record map: edge masks + level cache per frame.
+    Algorithmic bytes: the records (20 B per MI cell), the level cache read and
+    written (8 B per cell) and the masks written (1280 B per 128x128 region)."""
+    import numpy as np
+    import torch
+    from goav1_amd import synth
+    rng = np.random.default_rng(5)
+    w, h = 1920, 1080
+    recs, _ = synth.record_map(rng, w, h, 1, 1, delta_lf=True)
+    p = synth.lf_build_params(rng, w, h, 1, 1)
+    rows, cols = recs.shape
+    sbw, sbh = (cols + 31) >> 5, (rows + 31) >> 5
+    d_rec = torch.from_numpy(recs.view(np.uint8).reshape(-1).copy()).cuda()
+    d_lc = torch.zeros(rows * cols * 4, dtype=torch.uint8, device="cuda")
+    d_m = torch.zeros(sbw * sbh * 640, dtype=torch.int16, device="cuda")
+    for _ in range(3):
+        runner.deblock_build(d_rec.data_ptr(), p, d_lc.data_ptr(), d_m.data_ptr())
+    runner.sync()
+    runner.profile(True)
+    for _ in range(reps):
+        runner.deblock_build(d_rec.data_ptr(), p, d_lc.data_ptr(), d_m.data_ptr())
+    runner.sync()
+    prof = runner.profile_read()
+    runner.profile(False)
+    ms, n, _ = prof["deblock_build"]
+    us = 1e3 * ms / n
+    nbytes = rows * cols * (20 + 8) + sbw * sbh * 1280
+    return {"workload": "1920x1080 4:2:0 random record map (AV1 partitions, intra/inter, split transforms, "
+                        "skip, segments, delta_lf): edge masks + level cache",
+            "us_per_frame": round(us, 2), "mi_cells": rows * cols,
+            "gcells_per_s": round(rows * cols / (us * 1e-6) / 1e9, 2),
+            "algorithmic_bytes": nbytes, "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)}
+
+
 def traffic_for(kernel):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -493,6 +529,7 @@ def main():
         out["batch"] = batch_leg(runner, max(2, args.steps // 2), args.batch_frames)
     if not args.no_superres:
         out["superres"] = superres_leg(runner, max(2, args.steps // 2))
+    out["deblock_build"] = deblock_build_leg(runner)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds / 2, cores)
