@@ -174,15 +174,16 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ui = ur * a.unit_cols + uc;
-    const uint64_t m = a.noskip ? a.noskip[ui] : ~0ull;
+    const uint64_t m = a.noskip ? gmem(a.noskip)[ui] : ~0ull;
     int idx, count, yfilt, uvfilt;
-    const uint8_t *ystr, *uvstr;
+    gptr<const uint8_t> ystr, uvstr;
     if (a.dec) {
         // encoder apply (cdef_apply.go:202-228, the noskip path): a unit is
         // read when one of its 8x8 blocks inside the frame has residual
-        count = 1 << a.dec->cdef_bits;
-        ystr = a.dec->y_strength;
-        uvstr = a.dec->uv_strength;
+        const auto dec = gmem(a.dec);
+        count = 1 << dec->cdef_bits;
+        ystr = dec->y_strength;
+        uvstr = dec->uv_strength;
         yfilt = uvfilt = 0;
         for (int k = 0; k < count; k++) {
             yfilt |= ystr[k] != 0;
@@ -191,14 +192,14 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         const int rb = min(8, a.rows8 - ur * 8), cb = min(8, a.cols8 - uc * 8);
         uint64_t in = 0;
         for (int by = 0; by < rb; by++) in |= ((1ull << cb) - 1) << (by * 8);
-        idx = (m & in) ? a.fb_index[ui] : -1;
+        idx = (m & in) ? gmem(a.fb_index)[ui] : -1;
     } else {
         count = a.strength_count;
-        ystr = a.y_strength;
-        uvstr = a.uv_strength;
+        ystr = gmem(a.y_strength);
+        uvstr = gmem(a.uv_strength);
         yfilt = a.yfilt;
         uvfilt = a.uvfilt;
-        idx = a.index[ui];
+        idx = gmem(a.index)[ui];
     }
     if (idx >= count) {
         if (tid == 0) atomicOr(err, 1);

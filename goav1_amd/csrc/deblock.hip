@@ -142,7 +142,7 @@ __device__ __forceinline__ int unit_wd(uint32_t m0, uint32_t m1, uint32_t m2, bo
     return 0;
 }
 
-__device__ __forceinline__ uint32_t mask_bit(const uint16_t *halves, int n, int lane)
+__device__ __forceinline__ uint32_t mask_bit(gptr<const uint16_t> halves, int n, int lane)
 {
     return n < lane ? (halves[0] >> n) & 1u : (halves[1] >> (n - lane)) & 1u;
 }
@@ -165,14 +165,14 @@ __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restr
     if (x4 == 0 || x4 >= P.w4 || y4 >= P.h4) return;
     const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
     const int pos = x4 & (P.rcw - 1), n = y4 & (P.rch - 1);
-    const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
+    const auto &m = gmem(D.db.masks)[rr * D.db.sbw + rc];
     int wd;
     if (P.luma)
         wd = unit_wd(mask_bit(m.y[0][pos][0], n, 16), mask_bit(m.y[0][pos][1], n, 16), mask_bit(m.y[0][pos][2], n, 16), true);
     else
         wd = unit_wd(mask_bit(m.uv[0][pos][0], n, P.lane_v), mask_bit(m.uv[0][pos][1], n, P.lane_v), 0, false);
     if (!wd) return;
-    const uint8_t *lc = D.db.lc;
+    const auto lc = gmem(D.db.lc);
     const int li = y4 * D.db.mi_cols + x4;
     int L = lc[4 * li + P.comp_v];
     if (L == 0) L = lc[4 * (li - 1) + P.comp_v];
@@ -204,14 +204,14 @@ __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restr
     if (y4 == 0 || y4 >= P.h4 || x4 >= P.w4) return;
     const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
     const int pos = y4 & (P.rch - 1), n = x4 & (P.rcw - 1);
-    const av1pf_lf_mask &m = D.db.masks[rr * D.db.sbw + rc];
+    const auto &m = gmem(D.db.masks)[rr * D.db.sbw + rc];
     int wd;
     if (P.luma)
         wd = unit_wd(mask_bit(m.y[1][pos][0], n, 16), mask_bit(m.y[1][pos][1], n, 16), mask_bit(m.y[1][pos][2], n, 16), true);
     else
         wd = unit_wd(mask_bit(m.uv[1][pos][0], n, P.lane_h), mask_bit(m.uv[1][pos][1], n, P.lane_h), 0, false);
     if (!wd) return;
-    const uint8_t *lc = D.db.lc;
+    const auto lc = gmem(D.db.lc);
     const int li = y4 * D.db.mi_cols + x4;
     int L = lc[4 * li + P.comp_h];
     if (L == 0) L = lc[4 * (li - D.db.mi_cols) + P.comp_h];

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc
         uint16_t reg = G.g.seed;
         reg ^= (uint16_t)(((r * 37 + 178) & 255) << 8);
         reg ^= (uint16_t)((r * 173 + 105) & 255);
-        uint8_t *o = G.offsets + (size_t)r * G.ocols;
+        const auto o = gmem(G.offsets) + (size_t)r * G.ocols;
         for (int c = 0; c < G.ocols; c++) {
             reg = lfsr_step(reg);
             o[c] = (uint8_t)(reg >> 8);
@@ -216,7 +216,8 @@ __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < GH * GW; i += 64) G.templates[i] = T[i];
+    const auto tout = gmem(G.templates);
+    for (int i = threadIdx.x; i < GH * GW; i += 64) tout[i] = T[i];
 }
 
 __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
     const bool luma_term = G.g.num_y_points != 0;
     for (int i = threadIdx.x; i < GH * GW; i += 64) {
         T[i] = 0;
-        L[i] = luma_term ? G.templates[i] : (int16_t)0;
+        L[i] = luma_term ? gmem(G.templates)[i] : (int16_t)0;
     }
     __syncthreads();
     gauss_fill(T, G.row_seed[pl], cw, ch, 12 - bd + G.g.grain_scale_shift);
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
         }
     }
     __syncthreads();
-    int16_t *out = G.templates + pl * GH * GW;
+    const auto out = gmem(G.templates) + pl * GH * GW;
     for (int i = threadIdx.x; i < GH * GW; i += 64) out[i] = T[i];
 }
 
@@ -263,6 +264,8 @@ hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, c
     return hipGetLastError();
 }
 
+// Products below are of values under 2^12 and weights under 2^8: 24-bit
+// multiplies (full rate) give the same results.
 __device__ __forceinline__ int scale_lut(const uint8_t *lut, int index, int bd)
 {
     index = clipi(index, 0, (1 << bd) - 1);
@@ -271,12 +274,12 @@ __device__ __forceinline__ int scale_lut(const uint8_t *lut, int index, int bd)
     if (sh == 0 || x == 255) return lut[x];
     const int rem = index - (x << sh);
     const int start = lut[x], end = lut[x + 1];
-    return (uint8_t)(start + r2i((end - start) * rem, sh));
+    return (uint8_t)(start + r2i(__mul24(end - start, rem), sh));
 }
 
 __device__ __forceinline__ int blendw(int prev, int cur, int wp, int wc, int bd)
 {
-    return clipi(r2i(prev * wp + cur * wc, 5), -(1 << (bd - 1)), (1 << (bd - 1)) - 1);
+    return clipi(r2i(__mul24(prev, wp) + __mul24(cur, wc), 5), -(1 << (bd - 1)), (1 << (bd - 1)) - 1);
 }
 
 template <typename T>
@@ -297,17 +300,18 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         if (px0 >= pw || py0 >= ph) continue;
         const int bwid = min(bw, pw - px0), height = min(bh, ph - py0);
         // one lane per column (a power of two >= the block width), rows stepping
-        const int lpr = bwid > 16 ? 32 : bwid > 8 ? 16 : bwid > 4 ? 8 : 4;
-        const int x = tid & (lpr - 1), ystep = 256 / lpr;
+        const int lpr_log2 = bwid > 16 ? 5 : bwid > 8 ? 4 : bwid > 4 ? 3 : 2;
+        const int lpr = 1 << lpr_log2;
+        const int x = tid & (lpr - 1), ystep = 256 >> lpr_log2;
         if (x >= bwid) continue;
         if (!G.active[pl]) {   // plane without grain: the display gets the filtered samples
-            for (int y = tid / lpr; y < height; y += ystep)
+            for (int y = tid >> lpr_log2; y < height; y += ystep)
                 row_ptr<T>(dp, py0 + y)[px0 + x] = row_ptr<T>(sp, py0 + y)[px0 + x];
             continue;
         }
         const int ys = (g.overlap && brow > 0) ? min(height, 2 >> sy) : 0;
         const int xs = (g.overlap && bcol > 0) ? min(bwid, 2 >> sx) : 0;
-        const int16_t *tmpl = G.templates + pl * GH * GW;
+        const auto tmpl = gmem(G.templates) + pl * GH * GW;
         int minv = 0, maxv = (256 << (bd - 8)) - 1;
         if (g.clip_to_restricted_range) {
             minv = 16 << (bd - 8);
@@ -328,13 +332,22 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         // Every load of the lane's rows is issued before the first store, so
         // the loads do not wait behind the stores.
         constexpr int MR = 4;
-        const int yb = tid / lpr;
+        const int yb = tid >> lpr_log2;
+        // row pointers stepped by ystep rows (one 64-bit multiply per plane)
+        const int px = px0 + x;
+        const auto srow = gmem(sp.base) + (int64_t)(py0 + yb) * sp.stride + px * (int)sizeof(T);
+        const int64_t sstep = (int64_t)ystep * sp.stride;
+        const PlaneView &lp = src.p[0];
+        const auto lrow = gmem(lp.base) + (int64_t)((py0 + yb) << sy) * lp.stride + (px << sx) * (int)sizeof(T);
+        const int64_t lstep = (int64_t)(ystep << sy) * lp.stride;
+        const bool pair = sx && !(odd_luma && px == pw - 1);
+        const auto t0 = tmpl + (oy0 + yb) * GW + ox0 + x;
         int sv[MR], gvv[MR], idxv[MR];
 #pragma unroll
         for (int k = 0; k < MR; k++) {
             const int y = yb + k * ystep;
             if (y >= height) break;
-            int gv = tmpl[(oy0 + y) * GW + ox0 + x];
+            int gv = t0[k * ystep * GW];
             if (x < xs) {
                 const int left = tmpl[(oy1 + y) * GW + ox1 + x + bw];
                 gv = csub_x ? blendw(left, gv, 23, 22, bd) : blendw(left, gv, x == 1 ? 17 : 27, x == 1 ? 27 : 17, bd);
@@ -348,30 +361,30 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
                 gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
             }
             gvv[k] = gv;
-            const int px = px0 + x, py = py0 + y;
-            const int sa = row_ptr<T>(sp, py)[px];
+            const int sa = *(gptr<const T>)(srow + k * sstep);
             sv[k] = sa;
             if (pl == 0) {
                 idxv[k] = sa;
             } else {
-                const PlaneView &lp = src.p[0];
-                const int lx = px << sx, ly = py << sy;
-                int avg = row_ptr<T>(lp, ly)[lx];
-                if (sx && !(odd_luma && px == pw - 1)) avg = (avg + row_ptr<T>(lp, ly)[lx + 1] + 1) >> 1;
+                const auto l = (gptr<const T>)(lrow + k * lstep);
+                int avg = l[0];
+                if (pair) avg = (avg + l[1] + 1) >> 1;
                 if (g.chroma_scaling_from_luma) idxv[k] = avg;
                 else {
-                    const int combined = avg * (lmult - 128) + sa * (mult - 128);
+                    const int combined = __mul24(avg, lmult - 128) + __mul24(sa, mult - 128);
                     idxv[k] = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
                 }
             }
         }
+        const auto drow = gmem(dp.base) + (int64_t)(py0 + yb) * dp.stride + px * (int)sizeof(T);
+        const int64_t dstep = (int64_t)ystep * dp.stride;
 #pragma unroll
         for (int k = 0; k < MR; k++) {
             const int y = yb + k * ystep;
             if (y >= height) break;
             const int scale = scale_lut(lut[pl], idxv[k], bd);
-            const int noise = r2i(scale * gvv[k], scaling_shift);
-            row_ptr<T>(dp, py0 + y)[px0 + x] = (T)clipi(sv[k] + noise, minv, maxv);
+            const int noise = r2i(__mul24(scale, gvv[k]), scaling_shift);
+            *(gptr<T>)(drow + k * dstep) = (T)clipi(sv[k] + noise, minv, maxv);
         }
     }
 }
@@ -392,7 +405,7 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__res
         if (tid < 4) {   // offsets of this block and its left / top / top-left neighbours
             const int i = tid & 1, left = tid >> 1;
             const bool use = (i == 0 || (G.g.overlap && brow > 0)) && (!left || (G.g.overlap && bcol > 0));
-            offs[left][i] = use ? G.offsets[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
+            offs[left][i] = use ? gmem(G.offsets)[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
         }
         __syncthreads();
         const FrameView &src = D.v[D.fin];

@@ -64,7 +64,7 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     const int nominal = g.sh - (y0 == 0 ? g.off : 0);
     const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
     const int tw = w + 6, th = h + 6;
-    const uint16_t *bnd = g.bnd + (int64_t)t * 4 * g.pw;
+    const auto bnd = gmem(g.bnd) + (int64_t)t * 4 * g.pw;
     for (int cc = tid & 63; cc < tw; cc += 64) {
         const int c = cc - 3;
         const int x = min(max(x0 + c, 0), g.pw - 1);
@@ -126,7 +126,7 @@ __device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneD
     if (x >= g.pw || t >= g.stripes) return;
     const int y0 = max(0, t * g.sh - g.off);
     const int y1 = min((t + 1) * g.sh - g.off, g.ph);
-    uint16_t *b = g.bnd + (int64_t)t * 4 * g.pw;
+    const auto b = gmem(g.bnd) + (int64_t)t * 4 * g.pw;
     auto px = [&](int y) { return sr ? sr_sample<T>(row_ptr<T>(pv, y), *sr, x, maxv) : (int)row_ptr<T>(pv, y)[x]; };
     if (t > 0) {
         b[x] = px(y0 - 2);

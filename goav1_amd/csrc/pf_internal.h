@@ -30,9 +30,24 @@ struct FrameView {
     int w, h;          // luma visible size
 };
 
+// A pointer into device global memory, as the compiler's global address space
+// (the cast lets it use global rather than flat accesses; flat accesses also
+// count as LDS traffic, so every LDS wait would wait for them too).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(T *p)
+{
+    return (gptr<T>)p;
+}
+
+// Plane memory is device global memory: the cast lets the compiler use global
+// (not flat) loads and stores for everything addressed from a row pointer, so
+// waits on LDS traffic do not also wait on frame accesses.
 template <typename T>
 __device__ __forceinline__ T *row_ptr(const PlaneView &pv, int y) {
-    return reinterpret_cast<T *>(pv.base + (int64_t)y * pv.stride);
+    auto g = (__attribute__((address_space(1))) char *)pv.base;
+    return (T *)(g + (int64_t)y * pv.stride);
 }
 
 struct DbDesc {
