@@ -114,10 +114,21 @@ __device__ __forceinline__ void for_each_px(int uw, int uh, int tid, F &&f)
     for (int y = tid / tpr; y < uh; y += 256 / tpr) f(x, y);
 }
 
+// A lane's loads of up to 8 rows are in flight before its stores.
 template <typename T>
 __device__ __forceinline__ void copy_unit(const PlaneView &s, const PlaneView &d, int ux, int uy, int uw, int uh, int tid)
 {
-    for_each_px(uw, uh, tid, [&](int x, int y) { row_ptr<T>(d, uy + y)[ux + x] = row_ptr<T>(s, uy + y)[ux + x]; });
+    const int tpr = uw > 32 ? 64 : uw > 16 ? 32 : uw > 8 ? 16 : 8;
+    const int x = tid & (tpr - 1), step = 256 / tpr;
+    if (x >= uw) return;
+    for (int y0 = tid / tpr; y0 < uh; y0 += 8 * step) {
+        T v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = row_ptr<T>(s, uy + min(y0 + j * step, uh - 1))[ux + x];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (y0 + j * step < uh) row_ptr<T>(d, uy + y0 + j * step)[ux + x] = v[j];
+    }
 }
 
 // Stage the unit input (postfilter_cdef.go:1101-1250) into an LDS tile with
@@ -134,12 +145,23 @@ __device__ __forceinline__ void load_tile(uint16_t *tile, int pitch, const Plane
         const int x = ux - 8 + c;
         const bool vx = x >= 0 && (x < sx1 || col_ext);
         const int cx = min(x, sx1 - 1);
-        for (int r = tid >> 6; r < rows; r += 4) {
-            const int y = uy - 2 + r;
-            const bool vy = y >= 0 && (y < sy1 || row_ext);
-            uint16_t v = CDEF_VL;
-            if (vy && vx) v = row_ptr<T>(s, min(y, sy1 - 1))[cx];
-            tile[r * pitch + c] = v;
+        // a lane's loads of 6 rows are in flight before its LDS writes
+        for (int r0 = tid >> 6; r0 < rows; r0 += 24) {
+            // loads from clamped (always valid) positions, unconditionally,
+            // so that none waits at a branch join; VeryLarge chosen after
+            uint16_t v[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const int y = uy - 2 + r0 + 4 * j;
+                v[j] = row_ptr<T>(s, min(max(y, 0), sy1 - 1))[min(max(cx, 0), sx1 - 1)];
+            }
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const int r = r0 + 4 * j;
+                const int y = uy - 2 + r;
+                const bool vy = y >= 0 && (y < sy1 || row_ext);
+                if (r < rows) tile[r * pitch + c] = (vy && vx) ? v[j] : (uint16_t)CDEF_VL;
+            }
         }
     }
 }

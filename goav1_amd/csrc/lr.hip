@@ -55,12 +55,22 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
         u = g.units[ur * g.hz + uc];
     }
     if (u.type != AV1PF_LR_WIENER && u.type != AV1PF_LR_SGRPROJ) {
-        const int lpr = lanes_for(w), c = tid & (lpr - 1);
+        // copy: every load of the lane's rows in flight before its stores
+        const int lpr = lanes_for(w), c = tid & (lpr - 1), step = 256 / lpr;
         if (c < w)
-            for (int r = tid / lpr; r < h; r += 256 / lpr) row_ptr<T>(dp, y0 + r)[x0 + c] = row_ptr<T>(sp, y0 + r)[x0 + c];
+            for (int r0 = tid / lpr; r0 < h; r0 += 8 * step) {
+                T v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (r0 + j * step < h) v[j] = row_ptr<T>(sp, y0 + r0 + j * step)[x0 + c];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (r0 + j * step < h) row_ptr<T>(dp, y0 + r0 + j * step)[x0 + c] = v[j];
+            }
         return;
     }
-    // ---- stage the tile with its 3-sample halo
+    // ---- stage the tile with its 3-sample halo; a wave's rows are uniform,
+    // and a lane issues the loads of 6 rows before writing them to LDS
     const int nominal = g.sh - (y0 == 0 ? g.off : 0);
     const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
     const int tw = w + 6, th = h + 6;
@@ -68,19 +78,29 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     for (int cc = tid & 63; cc < tw; cc += 64) {
         const int c = cc - 3;
         const int x = min(max(x0 + c, 0), g.pw - 1);
-        for (int rr = tid >> 6; rr < th; rr += 4) {
-            const int r = rr - 3;
-            int v;
-            if (r < 0) {
-                if (copy_above) v = bnd[(r == -1 ? 1 : 0) * g.pw + x];
-                else v = read_px<T>(sp, max(y0 + r, 0), x);
-            } else if (r >= h) {
-                if (copy_below) v = bnd[(2 + min(r - h, 1)) * g.pw + x];
-                else v = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
-            } else {
-                v = read_px<T>(sp, y0 + r, x);
+        for (int rr0 = tid >> 6; rr0 < th; rr0 += 24) {
+            int v[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const int rr = rr0 + 4 * j;
+                if (rr >= th) break;
+                const int r = rr - 3;
+                if (r < 0) {
+                    if (copy_above) v[j] = bnd[(r == -1 ? 1 : 0) * g.pw + x];
+                    else v[j] = read_px<T>(sp, max(y0 + r, 0), x);
+                } else if (r >= h) {
+                    if (copy_below) v[j] = bnd[(2 + min(r - h, 1)) * g.pw + x];
+                    else v[j] = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
+                } else {
+                    v[j] = read_px<T>(sp, y0 + r, x);
+                }
             }
-            tile[rr * LR_TW + cc] = (uint16_t)v;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const int rr = rr0 + 4 * j;
+                if (rr >= th) break;
+                tile[rr * LR_TW + cc] = (uint16_t)v[j];
+            }
         }
     }
     __syncthreads();
