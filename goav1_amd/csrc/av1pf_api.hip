@@ -654,6 +654,21 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 G.matrix_identity = pf->matrix_identity;
                 G.orows = (pf->height + 31) / 32;
                 G.ocols = (pf->width + 31) / 32;
+                G.k[GK_OVERLAP] = g->overlap;
+                G.k[GK_CSFL] = g->chroma_scaling_from_luma;
+                G.k[GK_CLIP] = g->clip_to_restricted_range;
+                G.k[GK_SHIFT] = g->scaling_shift;
+                G.k[GK_MID] = pf->matrix_identity;
+                G.k[GK_CB_MULT] = g->cb_mult;
+                G.k[GK_CB_LMULT] = g->cb_luma_mult;
+                G.k[GK_CB_OFF] = g->cb_offset;
+                G.k[GK_CR_MULT] = g->cr_mult;
+                G.k[GK_CR_LMULT] = g->cr_luma_mult;
+                G.k[GK_CR_OFF] = g->cr_offset;
+                G.k[GK_ACT0] = G.active[0];
+                G.k[GK_ACT1] = G.active[1];
+                G.k[GK_ACT2] = G.active[2];
+                G.k[GK_OCOLS] = G.ocols;
                 goff[i] = P.goffs;
                 P.goffs += (size_t)G.orows * G.ocols;
                 grain_idx[i] = P.ngrain++;

@@ -282,14 +282,21 @@ __device__ __forceinline__ int blendw(int prev, int cur, int wp, int wc, int bd)
     return clipi(r2i(__mul24(prev, wp) + __mul24(cur, wc), 5), -(1 << (bd - 1)), (1 << (bd - 1)) - 1);
 }
 
+// One 32x32 luma block and its chroma blocks.  Per plane, a lane owns one
+// column and up to 4 rows; every load of the lane (templates of the block and
+// of its overlap neighbours, the sample, the luma pair of a chroma sample) is
+// issued before any of them is used -- positions the lane does not need are
+// replaced by valid ones, the values selected afterwards -- so each plane
+// waits for memory once.
 template <typename T>
 __device__ __forceinline__ void grain_block(const FrameView &src, const FrameView &dst, const GrainDesc &G,
-                                            int bcol, int brow, const uint8_t (*lut)[256], const uint8_t (*offs)[2])
+                                            const int32_t *K, int bcol, int brow, const uint8_t (*lut)[256],
+                                            const uint8_t (*offs)[2])
 {
     const int tid = threadIdx.x;
-    const av1pf_grain_params &g = G.g;
     const int bd = src.bd;
-    const int scaling_shift = g.scaling_shift;
+    const int scaling_shift = K[GK_SHIFT];
+    const auto tmpl0 = gmem(G.templates);
     for (int pl = 0; pl < src.nplanes; pl++) {
         const int sx = pl ? src.ssx : 0, sy = pl ? src.ssy : 0;
         const PlaneView &sp = src.p[pl];
@@ -304,87 +311,91 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
         const int lpr = 1 << lpr_log2;
         const int x = tid & (lpr - 1), ystep = 256 >> lpr_log2;
         if (x >= bwid) continue;
-        if (!G.active[pl]) {   // plane without grain: the display gets the filtered samples
-            for (int y = tid >> lpr_log2; y < height; y += ystep)
-                row_ptr<T>(dp, py0 + y)[px0 + x] = row_ptr<T>(sp, py0 + y)[px0 + x];
+        constexpr int MR = 4;   // rows per lane: a block has at most 32 rows, ystep >= 8
+        const int yb = tid >> lpr_log2;
+        if (yb >= height) continue;
+        const int px = px0 + x;
+        const auto srow = gmem(sp.base) + (int64_t)(py0 + yb) * sp.stride + px * (int)sizeof(T);
+        const int64_t sstep = (int64_t)ystep * sp.stride;
+        const auto drow = gmem(dp.base) + (int64_t)(py0 + yb) * dp.stride + px * (int)sizeof(T);
+        const int64_t dstep = (int64_t)ystep * dp.stride;
+        const int ylast = height - 1 - yb;   // rows k * ystep <= ylast exist (ylast >= 0)
+        if (!K[GK_ACT0 + pl]) {   // plane without grain: the display gets the filtered samples
+            T v[MR];
+#pragma unroll
+            for (int k = 0; k < MR; k++) v[k] = *(gptr<const T>)(srow + min(k * ystep, ylast) / ystep * sstep);
+#pragma unroll
+            for (int k = 0; k < MR; k++)
+                if (k * ystep <= ylast) *(gptr<T>)(drow + k * dstep) = v[k];
             continue;
         }
-        const int ys = (g.overlap && brow > 0) ? min(height, 2 >> sy) : 0;
-        const int xs = (g.overlap && bcol > 0) ? min(bwid, 2 >> sx) : 0;
-        const auto tmpl = gmem(G.templates) + pl * GH * GW;
+        const int ys = (K[GK_OVERLAP] && brow > 0) ? min(height, 2 >> sy) : 0;
+        const int xs = (K[GK_OVERLAP] && bcol > 0) ? min(bwid, 2 >> sx) : 0;
+        const auto tmpl = tmpl0 + pl * GH * GW;
         int minv = 0, maxv = (256 << (bd - 8)) - 1;
-        if (g.clip_to_restricted_range) {
+        if (K[GK_CLIP]) {
             minv = 16 << (bd - 8);
-            maxv = (pl == 0 ? 235 : (G.matrix_identity ? 235 : 240)) << (bd - 8);
+            maxv = (pl == 0 ? 235 : (K[GK_MID] ? 235 : 240)) << (bd - 8);
         }
         const bool odd_luma = pl && sx && (src.p[0].w & 1);
-        const int mult = pl == 1 ? g.cb_mult : g.cr_mult;
-        const int lmult = pl == 1 ? g.cb_luma_mult : g.cr_luma_mult;
-        const int coff = pl == 1 ? g.cb_offset : g.cr_offset;
+        const int mult = pl == 1 ? K[GK_CB_MULT] : K[GK_CR_MULT];
+        const int lmult = pl == 1 ? K[GK_CB_LMULT] : K[GK_CR_LMULT];
+        const int coff = pl == 1 ? K[GK_CB_OFF] : K[GK_CR_OFF];
         const bool csub_x = pl && sx, csub_y = pl && sy;
         // template offsets of the four candidate blocks (sample.go:9-77)
         const int ox0 = 3 + (2 >> sx) * (3 + (offs[0][0] >> 4)), oy0 = 3 + (2 >> sy) * (3 + (offs[0][0] & 15));
         const int ox1 = 3 + (2 >> sx) * (3 + (offs[1][0] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1][0] & 15));
         const int ox2 = 3 + (2 >> sx) * (3 + (offs[0][1] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[0][1] & 15));
         const int ox3 = 3 + (2 >> sx) * (3 + (offs[1][1] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[1][1] & 15));
-        // at most 4 rows per lane: a block is at most 32 rows and a row group
-        // step is at least 8 (32 lanes per row), so MR * ystep >= height.
-        // Every load of the lane's rows is issued before the first store, so
-        // the loads do not wait behind the stores.
-        constexpr int MR = 4;
-        const int yb = tid >> lpr_log2;
-        // row pointers stepped by ystep rows (one 64-bit multiply per plane)
-        const int px = px0 + x;
-        const auto srow = gmem(sp.base) + (int64_t)(py0 + yb) * sp.stride + px * (int)sizeof(T);
-        const int64_t sstep = (int64_t)ystep * sp.stride;
         const PlaneView &lp = src.p[0];
         const auto lrow = gmem(lp.base) + (int64_t)((py0 + yb) << sy) * lp.stride + (px << sx) * (int)sizeof(T);
         const int64_t lstep = (int64_t)(ystep << sy) * lp.stride;
-        const bool pair = sx && !(odd_luma && px == pw - 1);
-        const auto t0 = tmpl + (oy0 + yb) * GW + ox0 + x;
-        int sv[MR], gvv[MR], idxv[MR];
+        const int l1 = (sx && !(odd_luma && px == pw - 1)) ? (int)sizeof(T) : 0;   // second luma sample
+        const bool left = x < xs;
+        // loads
+        int gm[MR], gl[MR], gt[MR], gtl[MR], sa[MR], la[MR], lb[MR];
 #pragma unroll
         for (int k = 0; k < MR; k++) {
-            const int y = yb + k * ystep;
-            if (y >= height) break;
-            int gv = t0[k * ystep * GW];
-            if (x < xs) {
-                const int left = tmpl[(oy1 + y) * GW + ox1 + x + bw];
-                gv = csub_x ? blendw(left, gv, 23, 22, bd) : blendw(left, gv, x == 1 ? 17 : 27, x == 1 ? 27 : 17, bd);
-            }
-            if (y < ys) {
-                int top = tmpl[(oy2 + y + bh) * GW + ox2 + x];
-                if (x < xs) {
-                    const int tl = tmpl[(oy3 + y + bh) * GW + ox3 + x + bw];
-                    top = csub_x ? blendw(tl, top, 23, 22, bd) : blendw(tl, top, x == 1 ? 17 : 27, x == 1 ? 27 : 17, bd);
-                }
-                gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
-            }
-            gvv[k] = gv;
-            const int sa = *(gptr<const T>)(srow + k * sstep);
-            sv[k] = sa;
-            if (pl == 0) {
-                idxv[k] = sa;
-            } else {
-                const auto l = (gptr<const T>)(lrow + k * lstep);
-                int avg = l[0];
-                if (pair) avg = (avg + l[1] + 1) >> 1;
-                if (g.chroma_scaling_from_luma) idxv[k] = avg;
-                else {
-                    const int combined = __mul24(avg, lmult - 128) + __mul24(sa, mult - 128);
-                    idxv[k] = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
-                }
+            const int kk = min(k * ystep, ylast) / ystep;   // a row that exists
+            const int y = yb + kk * ystep;
+            const int im = (oy0 + y) * GW + ox0 + x;
+            const bool top = y < ys;
+            gm[k] = tmpl[im];
+            gl[k] = tmpl[left ? (oy1 + y) * GW + ox1 + x + bw : im];
+            gt[k] = tmpl[top ? (oy2 + y + bh) * GW + ox2 + x : im];
+            gtl[k] = tmpl[(top && left) ? (oy3 + y + bh) * GW + ox3 + x + bw : im];
+            sa[k] = *(gptr<const T>)(srow + kk * sstep);
+            if (pl) {
+                const auto l = lrow + kk * lstep;
+                la[k] = *(gptr<const T>)l;
+                lb[k] = *(gptr<const T>)(l + l1);
             }
         }
-        const auto drow = gmem(dp.base) + (int64_t)(py0 + yb) * dp.stride + px * (int)sizeof(T);
-        const int64_t dstep = (int64_t)ystep * dp.stride;
+        // grain, scaling, stores
 #pragma unroll
         for (int k = 0; k < MR; k++) {
+            if (k * ystep > ylast) break;
             const int y = yb + k * ystep;
-            if (y >= height) break;
-            const int scale = scale_lut(lut[pl], idxv[k], bd);
-            const int noise = r2i(__mul24(scale, gvv[k]), scaling_shift);
-            *(gptr<T>)(drow + k * dstep) = (T)clipi(sv[k] + noise, minv, maxv);
+            int gv = gm[k];
+            const int wl = csub_x ? 23 : (x == 1 ? 17 : 27), wc = csub_x ? 22 : (x == 1 ? 27 : 17);
+            if (left) gv = blendw(gl[k], gv, wl, wc, bd);
+            if (y < ys) {
+                int top = gt[k];
+                if (left) top = blendw(gtl[k], top, wl, wc, bd);
+                gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
+            }
+            int idx = sa[k];
+            if (pl) {
+                const int avg = l1 ? (la[k] + lb[k] + 1) >> 1 : la[k];
+                if (K[GK_CSFL]) idx = avg;
+                else {
+                    const int combined = __mul24(avg, lmult - 128) + __mul24(sa[k], mult - 128);
+                    idx = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
+                }
+            }
+            const int scale = scale_lut(lut[pl], idx, bd);
+            const int noise = r2i(__mul24(scale, gv), scaling_shift);
+            *(gptr<T>)(drow + k * dstep) = (T)clipi(sa[k] + noise, minv, maxv);
         }
     }
 }
@@ -392,8 +403,9 @@ __device__ __forceinline__ void grain_block(const FrameView &src, const FrameVie
 __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
                                                           int nseg, int total, StageLaunch L)
 {
-    __shared__ uint8_t lut[3][256];
+    __shared__ uint32_t lut32[3 * 64];
     __shared__ uint8_t offs[2][2];
+    const uint8_t (*lut)[256] = reinterpret_cast<const uint8_t (*)[256]>(lut32);
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
         const FrameDesc &D = descs[sg.frame];
@@ -401,17 +413,24 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__res
         const int t = u - sg.base;
         const int bcol = t % sg.nx, brow = t / sg.nx;
         const int tid = threadIdx.x;
-        for (int i = tid; i < 3 * 256; i += 256) (&lut[0][0])[i] = G.luts[i >> 8][i & 255];
-        if (tid < 4) {   // offsets of this block and its left / top / top-left neighbours
-            const int i = tid & 1, left = tid >> 1;
-            const bool use = (i == 0 || (G.g.overlap && brow > 0)) && (!left || (G.g.overlap && bcol > 0));
-            offs[left][i] = use ? gmem(G.offsets)[(size_t)(brow - i) * G.ocols + bcol - left] : 0;
+        int32_t K[GK_N];
+#pragma unroll
+        for (int i = 0; i < GK_N; i++) K[i] = G.k[i];
+        // the three scaling tables as 192 words, and the block offsets of this
+        // block and its left / top / top-left neighbours, in one round trip
+        if (tid < 3 * 64) lut32[tid] = gmem(reinterpret_cast<const uint32_t *>(&G.luts[0][0]))[tid];
+        else if (tid < 3 * 64 + 4) {
+            const int j = tid - 3 * 64, i = j & 1, lf = j >> 1;
+            const bool use = (i == 0 || (K[GK_OVERLAP] && brow > 0)) && (!lf || (K[GK_OVERLAP] && bcol > 0));
+            const int br = use ? brow - i : brow, bc = use ? bcol - lf : bcol;
+            const uint8_t o = gmem(G.offsets)[(size_t)br * K[GK_OCOLS] + bc];
+            offs[lf][i] = use ? o : 0;
         }
         __syncthreads();
         const FrameView &src = D.v[D.fin];
         const FrameView &dst = D.v[VIEW_D];
-        if (src.bd == 8) grain_block<uint8_t>(src, dst, G, bcol, brow, lut, offs);
-        else grain_block<uint16_t>(src, dst, G, bcol, brow, lut, offs);
+        if (src.bd == 8) grain_block<uint8_t>(src, dst, G, K, bcol, brow, lut, offs);
+        else grain_block<uint16_t>(src, dst, G, K, bcol, brow, lut, offs);
     });
 }
 

@@ -91,11 +91,16 @@ struct LrPlaneDesc {
     int pw, ph, procw, sh, off, xtiles, stripes;
 };
 
+// 32-bit copies of the grain parameters the apply reads (one scalar load)
+enum { GK_OVERLAP, GK_CSFL, GK_CLIP, GK_SHIFT, GK_MID, GK_CB_MULT, GK_CB_LMULT, GK_CB_OFF, GK_CR_MULT,
+       GK_CR_LMULT, GK_CR_OFF, GK_ACT0, GK_ACT1, GK_ACT2, GK_OCOLS, GK_N };
+
 struct GrainDesc {
+    int32_t k[16];                 // GK_* (GK_MID: matrix identity)
     av1pf_grain_params g;
     int active[3];
     uint16_t row_seed[3][73];      // LFSR state at the start of each template row
-    uint8_t luts[3][256];
+    alignas(16) uint8_t luts[3][256];
     int16_t *templates;            // 3 x 73 x 82
     uint8_t *offsets;              // [orows][ocols] per-32x32-block random offsets
     int orows, ocols;
