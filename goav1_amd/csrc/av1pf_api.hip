@@ -247,6 +247,8 @@ size_t scratch_bytes(const FrameView &f, int64_t strides[3])
 int validate_deblock(av1pf_ctx *c, const av1pf_frame *f, const av1pf_deblock_params *p)
 {
     if (!p || !p->level_cache || !p->masks) return fail(c, AV1PF_ERR_INVALID_ARG, "deblock params missing");
+    if (reinterpret_cast<uintptr_t>(p->masks) & 3)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "deblock masks must be 4-byte aligned");
     const int mi_cols = ((f->width + 7) >> 3) << 1, mi_rows = ((f->height + 7) >> 3) << 1;
     if (p->mi_cols != mi_cols || p->mi_rows != mi_rows || p->sb128_w != (mi_cols + 31) >> 5 ||
         p->sb128_h != (mi_rows + 31) >> 5)
@@ -504,6 +506,7 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             db.masks = p->masks;
             db.mi_cols = p->mi_cols;
             db.sbw = p->sb128_w;
+            db.sharp = p->sharpness;
             for (int l = 0; l < 64; l++) {
                 int shift = (p->sharpness > 0) + (p->sharpness > 4);
                 int lim = l >> shift;

@@ -22,10 +22,12 @@ namespace pf {
 __device__ __forceinline__ int iabs_(int v) { return v < 0 ? -v : v; }
 __device__ __forceinline__ int iclip_(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// One line across the edge: dav1d loop_filter() body (loopfilter/sb.go:348-446).
-// p points at q0; st steps across the edge (elements).
-template <typename T>
-__device__ __forceinline__ void lpf_line(T *p, int64_t st, int bd, int E, int I, int H, int wd)
+// One line across the edge: dav1d loop_filter() body (loopfilter/sb.go:348-446)
+// on values already in registers: a[0..7] = p3 p2 p1 p0 q0 q1 q2 q3, w[0..5] =
+// p6 p5 p4 q4 q5 q6 (read only for wd 16).  The new values go back into a and
+// w; the return value says which samples changed: 0 none, 2 p0 q0, 4 p1..q1,
+// 6 p2..q2, 12 p5..q5.
+__device__ __forceinline__ int lpf_vals(int *a, int *w, int bd, int E, int I, int H, int wd)
 {
     const int s = bd - 8;
     const int F = 1 << s;
@@ -33,18 +35,13 @@ __device__ __forceinline__ void lpf_line(T *p, int64_t st, int bd, int E, int I,
     const unsigned twoI = 2u * (unsigned)I, twoF = 2u * (unsigned)F, twoH = 2u * (unsigned)H;
     const int maxPix = (1 << bd) - 1;
     const int dmin = -(128 << s), dmax = (128 << s) - 1;
-    int p1 = p[-2 * st], p0 = p[-st], q0 = p[0], q1 = p[st];
+    const int p3 = a[0], p2 = a[1], p1 = a[2], p0 = a[3], q0 = a[4], q1 = a[5], q2 = a[6], q3 = a[7];
     if ((unsigned)(p1 - p0 + I) > twoI || (unsigned)(q1 - q0 + I) > twoI ||
         iabs_(p0 - q0) * 2 + (iabs_(p1 - q1) >> 1) > E)
-        return;
-    int p2 = 0, q2 = 0, p3 = 0, q3 = 0;
+        return 0;
     if (wd > 4) {
-        p2 = p[-3 * st]; q2 = p[2 * st];
-        if ((unsigned)(p2 - p1 + I) > twoI || (unsigned)(q2 - q1 + I) > twoI) return;
-        if (wd > 6) {
-            p3 = p[-4 * st]; q3 = p[3 * st];
-            if ((unsigned)(p3 - p2 + I) > twoI || (unsigned)(q3 - q2 + I) > twoI) return;
-        }
+        if ((unsigned)(p2 - p1 + I) > twoI || (unsigned)(q2 - q1 + I) > twoI) return 0;
+        if (wd > 6 && ((unsigned)(p3 - p2 + I) > twoI || (unsigned)(q3 - q2 + I) > twoI)) return 0;
     }
     bool flat8in = false;
     if (wd >= 6) {
@@ -55,53 +52,127 @@ __device__ __forceinline__ void lpf_line(T *p, int64_t st, int bd, int E, int I,
     }
     if (flat8in) {
         if (wd >= 16) {
-            int p6 = p[-7 * st], p5 = p[-6 * st], p4 = p[-5 * st];
-            int q4 = p[4 * st], q5 = p[5 * st], q6 = p[6 * st];
+            const int p6 = w[0], p5 = w[1], p4 = w[2], q4 = w[3], q5 = w[4], q6 = w[5];
             if ((unsigned)(p6 - p0 + F) <= twoF && (unsigned)(p5 - p0 + F) <= twoF &&
                 (unsigned)(p4 - p0 + F) <= twoF && (unsigned)(q4 - q0 + F) <= twoF &&
                 (unsigned)(q5 - q0 + F) <= twoF && (unsigned)(q6 - q0 + F) <= twoF) {
-                p[-6 * st] = (T)((p6 * 7 + p5 * 2 + p4 * 2 + p3 + p2 + p1 + p0 + q0 + 8) >> 4);
-                p[-5 * st] = (T)((p6 * 5 + p5 * 2 + p4 * 2 + p3 * 2 + p2 + p1 + p0 + q0 + q1 + 8) >> 4);
-                p[-4 * st] = (T)((p6 * 4 + p5 + p4 * 2 + p3 * 2 + p2 * 2 + p1 + p0 + q0 + q1 + q2 + 8) >> 4);
-                p[-3 * st] = (T)((p6 * 3 + p5 + p4 + p3 * 2 + p2 * 2 + p1 * 2 + p0 + q0 + q1 + q2 + q3 + 8) >> 4);
-                p[-2 * st] = (T)((p6 * 2 + p5 + p4 + p3 + p2 * 2 + p1 * 2 + p0 * 2 + q0 + q1 + q2 + q3 + q4 + 8) >> 4);
-                p[-st] = (T)((p6 + p5 + p4 + p3 + p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + q2 + q3 + q4 + q5 + 8) >> 4);
-                p[0] = (T)((p5 + p4 + p3 + p2 + p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + q3 + q4 + q5 + q6 + 8) >> 4);
-                p[st] = (T)((p4 + p3 + p2 + p1 + p0 + q0 * 2 + q1 * 2 + q2 * 2 + q3 + q4 + q5 + q6 * 2 + 8) >> 4);
-                p[2 * st] = (T)((p3 + p2 + p1 + p0 + q0 + q1 * 2 + q2 * 2 + q3 * 2 + q4 + q5 + q6 * 3 + 8) >> 4);
-                p[3 * st] = (T)((p2 + p1 + p0 + q0 + q1 + q2 * 2 + q3 * 2 + q4 * 2 + q5 + q6 * 4 + 8) >> 4);
-                p[4 * st] = (T)((p1 + p0 + q0 + q1 + q2 + q3 * 2 + q4 * 2 + q5 * 2 + q6 * 5 + 8) >> 4);
-                p[5 * st] = (T)((p0 + q0 + q1 + q2 + q3 + q4 * 2 + q5 * 2 + q6 * 7 + 8) >> 4);
-                return;
+                w[1] = (p6 * 7 + p5 * 2 + p4 * 2 + p3 + p2 + p1 + p0 + q0 + 8) >> 4;
+                w[2] = (p6 * 5 + p5 * 2 + p4 * 2 + p3 * 2 + p2 + p1 + p0 + q0 + q1 + 8) >> 4;
+                a[0] = (p6 * 4 + p5 + p4 * 2 + p3 * 2 + p2 * 2 + p1 + p0 + q0 + q1 + q2 + 8) >> 4;
+                a[1] = (p6 * 3 + p5 + p4 + p3 * 2 + p2 * 2 + p1 * 2 + p0 + q0 + q1 + q2 + q3 + 8) >> 4;
+                a[2] = (p6 * 2 + p5 + p4 + p3 + p2 * 2 + p1 * 2 + p0 * 2 + q0 + q1 + q2 + q3 + q4 + 8) >> 4;
+                a[3] = (p6 + p5 + p4 + p3 + p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + q2 + q3 + q4 + q5 + 8) >> 4;
+                a[4] = (p5 + p4 + p3 + p2 + p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + q3 + q4 + q5 + q6 + 8) >> 4;
+                a[5] = (p4 + p3 + p2 + p1 + p0 + q0 * 2 + q1 * 2 + q2 * 2 + q3 + q4 + q5 + q6 * 2 + 8) >> 4;
+                a[6] = (p3 + p2 + p1 + p0 + q0 + q1 * 2 + q2 * 2 + q3 * 2 + q4 + q5 + q6 * 3 + 8) >> 4;
+                a[7] = (p2 + p1 + p0 + q0 + q1 + q2 * 2 + q3 * 2 + q4 * 2 + q5 + q6 * 4 + 8) >> 4;
+                w[3] = (p1 + p0 + q0 + q1 + q2 + q3 * 2 + q4 * 2 + q5 * 2 + q6 * 5 + 8) >> 4;
+                w[4] = (p0 + q0 + q1 + q2 + q3 + q4 * 2 + q5 * 2 + q6 * 7 + 8) >> 4;
+                return 12;
             }
         }
         if (wd >= 8) {
-            p[-3 * st] = (T)((p3 * 3 + p2 * 2 + p1 + p0 + q0 + 4) >> 3);
-            p[-2 * st] = (T)((p3 * 2 + p2 + p1 * 2 + p0 + q0 + q1 + 4) >> 3);
-            p[-st] = (T)((p3 + p2 + p1 + p0 * 2 + q0 + q1 + q2 + 4) >> 3);
-            p[0] = (T)((p2 + p1 + p0 + q0 * 2 + q1 + q2 + q3 + 4) >> 3);
-            p[st] = (T)((p1 + p0 + q0 + q1 * 2 + q2 + q3 * 2 + 4) >> 3);
-            p[2 * st] = (T)((p0 + q0 + q1 + q2 * 2 + q3 * 3 + 4) >> 3);
-            return;
+            a[1] = (p3 * 3 + p2 * 2 + p1 + p0 + q0 + 4) >> 3;
+            a[2] = (p3 * 2 + p2 + p1 * 2 + p0 + q0 + q1 + 4) >> 3;
+            a[3] = (p3 + p2 + p1 + p0 * 2 + q0 + q1 + q2 + 4) >> 3;
+            a[4] = (p2 + p1 + p0 + q0 * 2 + q1 + q2 + q3 + 4) >> 3;
+            a[5] = (p1 + p0 + q0 + q1 * 2 + q2 + q3 * 2 + 4) >> 3;
+            a[6] = (p0 + q0 + q1 + q2 * 2 + q3 * 3 + 4) >> 3;
+            return 6;
         }
-        p[-2 * st] = (T)((p2 * 3 + p1 * 2 + p0 * 2 + q0 + 4) >> 3);
-        p[-st] = (T)((p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + 4) >> 3);
-        p[0] = (T)((p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + 4) >> 3);
-        p[st] = (T)((p0 + q0 * 2 + q1 * 2 + q2 * 3 + 4) >> 3);
-        return;
+        a[2] = (p2 * 3 + p1 * 2 + p0 * 2 + q0 + 4) >> 3;
+        a[3] = (p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + 4) >> 3;
+        a[4] = (p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + 4) >> 3;
+        a[5] = (p0 + q0 * 2 + q1 * 2 + q2 * 3 + 4) >> 3;
+        return 4;
     }
     const bool hev = (unsigned)(p1 - p0 + H) > twoH || (unsigned)(q1 - q0 + H) > twoH;
     int f = hev ? iclip_(iclip_(p1 - q1, dmin, dmax) + 3 * (q0 - p0), dmin, dmax)
                 : iclip_(3 * (q0 - p0), dmin, dmax);
     const int f1 = min(f + 4, dmax) >> 3;
     const int f2 = min(f + 3, dmax) >> 3;
-    p[-st] = (T)iclip_(p0 + f2, 0, maxPix);
-    p[0] = (T)iclip_(q0 - f1, 0, maxPix);
-    if (!hev) {
-        f = (f1 + 1) >> 1;
-        p[-2 * st] = (T)iclip_(p1 + f, 0, maxPix);
-        p[st] = (T)iclip_(q1 - f, 0, maxPix);
+    a[3] = iclip_(p0 + f2, 0, maxPix);
+    a[4] = iclip_(q0 - f1, 0, maxPix);
+    if (hev) return 2;
+    f = (f1 + 1) >> 1;
+    a[2] = iclip_(p1 + f, 0, maxPix);
+    a[5] = iclip_(q1 - f, 0, maxPix);
+    return 4;
+}
+
+// Write back samples [LO, 7 - LO] of a (and p5 p4 q4 q5 from w when LO is 0)
+// for all 4 lines, unconditionally and in address order (the 4 lines of a
+// sample position first when the lines are adjacent, else a line's samples
+// first), so neighbouring samples merge into wide stores.
+template <int LO, bool ACROSS, typename T>
+__device__ __forceinline__ void lpf_store(gptr<T> p, int64_t lstep, int64_t st, const int (*a)[8], const int (*w)[6])
+{
+    auto put = [&](int i, int k, int v) { p[i * lstep + k * st] = (T)v; };
+    if constexpr (ACROSS) {
+        if constexpr (LO == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) put(i, -6, w[i][1]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) put(i, -5, w[i][2]);
+        }
+#pragma unroll
+        for (int j = LO; j <= 7 - LO; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) put(i, j - 4, a[i][j]);
+        if constexpr (LO == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) put(i, 4, w[i][3]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) put(i, 5, w[i][4]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if constexpr (LO == 0) {
+                put(i, -6, w[i][1]);
+                put(i, -5, w[i][2]);
+            }
+#pragma unroll
+            for (int j = LO; j <= 7 - LO; j++) put(i, j - 4, a[i][j]);
+            if constexpr (LO == 0) {
+                put(i, 4, w[i][3]);
+                put(i, 5, w[i][4]);
+            }
+        }
     }
+}
+
+// The 4 lines of one edge unit: q0 of line i at p + i * lstep, samples across
+// the edge st apart (elements).  Every line's p3..q3 is loaded before any is
+// used, and p6..p4 / q4..q6 of every line together when the edge is 16 wide.
+// The unit writes back, for all 4 lines, the widest span any of its lines
+// changed: a line that changed less rewrites values it read, and no other edge
+// of the same direction writes a sample this edge reads (transform spacing), so
+// the rewrite is safe in parallel.
+template <bool ACROSS, typename T>
+__device__ __forceinline__ void lpf_unit(gptr<T> p, int64_t lstep, int64_t st, int bd, int E, int I, int H, int wd)
+{
+    int a[4][8], w[4][6];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[i][j] = p[i * lstep + (j - 4) * st];
+    if (wd >= 16) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                w[i][j] = p[i * lstep + (j - 7) * st];
+                w[i][3 + j] = p[i * lstep + (j + 4) * st];
+            }
+    }
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) n = max(n, lpf_vals(a[i], w[i], bd, E, I, H, wd));
+    if (n == 12) lpf_store<0, ACROSS, T>(p, lstep, st, a, w);
+    else if (n == 6) lpf_store<1, ACROSS, T>(p, lstep, st, a, w);
+    else if (n == 4) lpf_store<2, ACROSS, T>(p, lstep, st, a, w);
+    else if (n == 2) lpf_store<3, ACROSS, T>(p, lstep, st, a, w);
 }
 
 // Per-plane constants of one deblock segment.
@@ -142,9 +213,46 @@ __device__ __forceinline__ int unit_wd(uint32_t m0, uint32_t m1, uint32_t m2, bo
     return 0;
 }
 
-__device__ __forceinline__ uint32_t mask_bit(gptr<const uint16_t> halves, int n, int lane)
+// Bit n of a lane whose two 16-bit halves (the second starting at bit
+// `lane`) are one 32-bit word.
+__device__ __forceinline__ uint32_t mask_bit(uint32_t halves, int n, int lane)
 {
-    return n < lane ? (halves[0] >> n) & 1u : (halves[1] >> (n - lane)) & 1u;
+    return n < lane ? (halves >> n) & 1u : (halves >> (16 + n - lane)) & 1u;
+}
+
+// Limits of a level (loopfilter thresholds: the interior limit shifted by the
+// sharpness and capped, the edge limit 2 * (level + 2) + interior).
+__device__ __forceinline__ void db_limits(int L, int sharp, int &E, int &I)
+{
+    int lim = L >> ((sharp > 0) + (sharp > 4));
+    if (sharp > 0 && lim > 9 - sharp) lim = 9 - sharp;
+    I = max(lim, 1);
+    E = 2 * (L + 2) + I;
+}
+
+// Width class and level of an edge unit: the unit's mask words and both level
+// candidates (the unit's cell, then the previous cell across the edge) are
+// loaded before either is used.
+__device__ __forceinline__ int unit_edge(const DbDesc &db, const DbPlaneK &P, int dir, int rr, int rc, int pos,
+                                         int n, int li, int prev, int comp, int &L)
+{
+    const auto m = gmem(db.masks) + rr * db.sbw + rc;
+    const int lane = dir == 0 ? P.lane_v : P.lane_h;
+    uint32_t w0, w1, w2 = 0;
+    if (P.luma) {
+        const auto q = (gptr<const uint32_t>)&m->y[dir][pos][0][0];
+        w0 = q[0];
+        w1 = q[1];
+        w2 = q[2];
+    } else {
+        const auto q = (gptr<const uint32_t>)&m->uv[dir][pos][0][0];
+        w0 = q[0];
+        w1 = q[1];
+    }
+    const auto lc = gmem(db.lc);
+    const int l0 = lc[4 * li + comp], l1 = lc[4 * (li - prev) + comp];
+    L = l0 ? l0 : l1;
+    return unit_wd(mask_bit(w0, n, lane), mask_bit(w1, n, lane), P.luma ? mask_bit(w2, n, lane) : 0, P.luma);
 }
 
 // Vertical edges: a 64 x 4 workgroup covers 64 edge columns (x4) by 4 rows of
@@ -163,27 +271,17 @@ __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restr
     const int x4 = tx * 64 + threadIdx.x;
     const int y4 = ty * 4 + threadIdx.y;
     if (x4 == 0 || x4 >= P.w4 || y4 >= P.h4) return;
-    const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
-    const int pos = x4 & (P.rcw - 1), n = y4 & (P.rch - 1);
-    const auto &m = gmem(D.db.masks)[rr * D.db.sbw + rc];
-    int wd;
-    if (P.luma)
-        wd = unit_wd(mask_bit(m.y[0][pos][0], n, 16), mask_bit(m.y[0][pos][1], n, 16), mask_bit(m.y[0][pos][2], n, 16), true);
-    else
-        wd = unit_wd(mask_bit(m.uv[0][pos][0], n, P.lane_v), mask_bit(m.uv[0][pos][1], n, P.lane_v), 0, false);
-    if (!wd) return;
-    const auto lc = gmem(D.db.lc);
-    const int li = y4 * D.db.mi_cols + x4;
-    int L = lc[4 * li + P.comp_v];
-    if (L == 0) L = lc[4 * (li - 1) + P.comp_v];
-    if (L == 0) return;
+    int L;
+    const int wd = unit_edge(D.db, P, 0, y4 >> P.lrch, x4 >> P.lrcw, x4 & (P.rcw - 1), y4 & (P.rch - 1),
+                             y4 * D.db.mi_cols + x4, 1, P.comp_v, L);
+    if (!wd || L == 0) return;
+    int E, I;
+    db_limits(L, D.db.sharp, E, I);
     const PlaneView &pv = f.p[pl];
-    const int E = D.db.E[L], I = D.db.I[L], Hh = L >> 4;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        if (f.bd == 8) lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4 + r) + x4 * 4, 1, 8, E, I, Hh, wd);
-        else lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4 + r) + x4 * 4, 1, f.bd, E, I, Hh, wd);
-    }
+    if (f.bd == 8)
+        lpf_unit<false, uint8_t>(gmem(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4), pv.stride, 1, 8, E, I, L >> 4, wd);
+    else
+        lpf_unit<false, uint16_t>(gmem(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4), pv.stride / 2, 1, f.bd, E, I, L >> 4, wd);
 }
 
 // Horizontal edges: a 64 x 4 workgroup covers 64 columns of 4x4 units (x4) by 4
@@ -202,27 +300,17 @@ __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restr
     const int x4 = tx * 64 + threadIdx.x;
     const int y4 = ty * 4 + threadIdx.y;
     if (y4 == 0 || y4 >= P.h4 || x4 >= P.w4) return;
-    const int rr = y4 >> P.lrch, rc = x4 >> P.lrcw;
-    const int pos = y4 & (P.rch - 1), n = x4 & (P.rcw - 1);
-    const auto &m = gmem(D.db.masks)[rr * D.db.sbw + rc];
-    int wd;
-    if (P.luma)
-        wd = unit_wd(mask_bit(m.y[1][pos][0], n, 16), mask_bit(m.y[1][pos][1], n, 16), mask_bit(m.y[1][pos][2], n, 16), true);
-    else
-        wd = unit_wd(mask_bit(m.uv[1][pos][0], n, P.lane_h), mask_bit(m.uv[1][pos][1], n, P.lane_h), 0, false);
-    if (!wd) return;
-    const auto lc = gmem(D.db.lc);
-    const int li = y4 * D.db.mi_cols + x4;
-    int L = lc[4 * li + P.comp_h];
-    if (L == 0) L = lc[4 * (li - D.db.mi_cols) + P.comp_h];
-    if (L == 0) return;
+    int L;
+    const int wd = unit_edge(D.db, P, 1, y4 >> P.lrch, x4 >> P.lrcw, y4 & (P.rch - 1), x4 & (P.rcw - 1),
+                             y4 * D.db.mi_cols + x4, D.db.mi_cols, P.comp_h, L);
+    if (!wd || L == 0) return;
+    int E, I;
+    db_limits(L, D.db.sharp, E, I);
     const PlaneView &pv = f.p[pl];
-    const int E = D.db.E[L], I = D.db.I[L], Hh = L >> 4;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        if (f.bd == 8) lpf_line<uint8_t>(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4 + c, pv.stride, 8, E, I, Hh, wd);
-        else lpf_line<uint16_t>(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4 + c, pv.stride / 2, f.bd, E, I, Hh, wd);
-    }
+    if (f.bd == 8)
+        lpf_unit<true, uint8_t>(gmem(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4), 1, pv.stride, 8, E, I, L >> 4, wd);
+    else
+        lpf_unit<true, uint16_t>(gmem(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4), 1, pv.stride / 2, f.bd, E, I, L >> 4, wd);
 }
 
 hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,

@@ -54,6 +54,7 @@ struct DbDesc {
     const uint8_t *lc;             // level cache, 4 bytes per MI
     const av1pf_lf_mask *masks;
     int mi_cols, sbw;
+    int sharp;                     // sharpness 0..7
     uint8_t E[64], I[64];          // per-level limits for the frame sharpness
 };
 
