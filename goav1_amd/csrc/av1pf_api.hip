@@ -714,7 +714,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     if (rc) return rc;
     if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
     if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
-    const size_t tsz = (size_t)3 * 73 * 82 * sizeof(int16_t);
+    const size_t tsz = GR_FRAME_BYTES;
     const bool side = P.tl.total > 0;
     const int par = c->tpar;
     if (side) {
@@ -741,7 +741,9 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         for (int p = 0; p < 3; p++)
             if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = c->lr_bnd + bnd_off[i * 3 + p];
         if (grain_idx[i] >= 0) {
-            D.gr.templates = c->templates[par] + (size_t)grain_idx[i] * 3 * 73 * 82;
+            char *gw = reinterpret_cast<char *>(c->templates[par]) + (size_t)grain_idx[i] * GR_FRAME_BYTES;
+            D.gr.templates = reinterpret_cast<int16_t *>(gw);
+            D.gr.scale = reinterpret_cast<uint8_t *>(gw + GR_TMPL_BYTES);
             D.gr.offsets = c->grain_offs[par] + goff[i];
         }
     }

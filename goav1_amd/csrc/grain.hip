@@ -179,6 +179,8 @@ __device__ void ar_template(int16_t *T, const int16_t *L, int w, int h, const in
     }
 }
 
+__device__ void expand_scaling(const GrainDesc &G, int bd);
+
 __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
 {
     __shared__ int16_t T[GH * GW];
@@ -201,6 +203,7 @@ __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc
             o[c] = (uint8_t)(reg >> 8);
         }
     }
+    expand_scaling(G, bd);
     __syncthreads();
     if (G.g.num_y_points) {
         gauss_fill(T, G.row_seed[0], GW, GH, 12 - bd + G.g.grain_scale_shift);
@@ -282,155 +285,243 @@ __device__ __forceinline__ int blendw(int prev, int cur, int wp, int wc, int bd)
     return clipi(r2i(__mul24(prev, wp) + __mul24(cur, wc), 5), -(1 << (bd - 1)), (1 << (bd - 1)) - 1);
 }
 
-// One 32x32 luma block and its chroma blocks.  Per plane, a lane owns one
-// column and up to 4 rows; every load of the lane (templates of the block and
-// of its overlap neighbours, the sample, the luma pair of a chroma sample) is
-// issued before any of them is used -- positions the lane does not need are
-// replaced by valid ones, the values selected afterwards -- so each plane
-// waits for memory once.
-template <typename T>
-__device__ __forceinline__ void grain_block(const FrameView &src, const FrameView &dst, const GrainDesc &G,
-                                            const int32_t *K, int bcol, int brow, const uint8_t (*lut)[256],
-                                            const uint8_t (*offs)[2])
+// Scaling function of every sample value (1 << bd entries per plane), from the
+// 256-entry piecewise-linear table with the 10/12-bit interpolation: the apply
+// then looks a sample's scale up with one LDS byte read.
+__device__ void expand_scaling(const GrainDesc &G, int bd)
 {
-    const int tid = threadIdx.x;
+    const auto out = gmem(G.scale);
+    for (int pl = 0; pl < 3; pl++)
+        for (int i = threadIdx.x; i < (1 << bd); i += 64) out[pl * GR_SCALE + i] = (uint8_t)scale_lut(G.luts[pl], i, bd);
+}
+
+// One task of the apply: 16 rows of one plane of a block, lane (r, g) owning
+// row r and the C adjacent columns from g * C (C = block width / 4).  The
+// lane's samples, template values and (chroma) luma samples are read with
+// one vector load each; the overlap neighbours' template values only by the
+// lanes in the 2-sample overlap band.  A group reaching past the plane's
+// right edge reads sample by sample with clamped positions.  The plane's
+// expanded scaling table is loaded by the whole wave together with the task's
+// own loads and written to the wave's LDS slice before the first lookup, so a
+// task waits for memory once.
+template <typename T, int C, bool CHROMA>
+__device__ __forceinline__ void grain_task(const FrameView &src, const FrameView &dst, const GrainDesc &G,
+                                           const int32_t *K, int pl, int bcol, int brow, int rg, uint8_t *lut,
+                                           const uint8_t *offs, int lane)
+{
     const int bd = src.bd;
-    const int scaling_shift = K[GK_SHIFT];
-    const auto tmpl0 = gmem(G.templates);
-    for (int pl = 0; pl < src.nplanes; pl++) {
-        const int sx = pl ? src.ssx : 0, sy = pl ? src.ssy : 0;
-        const PlaneView &sp = src.p[pl];
-        const PlaneView &dp = dst.p[pl];
-        const int bw = 32 >> sx, bh = 32 >> sy;
-        const int px0 = bcol * bw, py0 = brow * bh;
-        const int pw = sp.w, ph = sp.h;
-        if (px0 >= pw || py0 >= ph) continue;
-        const int bwid = min(bw, pw - px0), height = min(bh, ph - py0);
-        // one lane per column (a power of two >= the block width), rows stepping
-        const int lpr_log2 = bwid > 16 ? 5 : bwid > 8 ? 4 : bwid > 4 ? 3 : 2;
-        const int lpr = 1 << lpr_log2;
-        const int x = tid & (lpr - 1), ystep = 256 >> lpr_log2;
-        if (x >= bwid) continue;
-        constexpr int MR = 4;   // rows per lane: a block has at most 32 rows, ystep >= 8
-        const int yb = tid >> lpr_log2;
-        if (yb >= height) continue;
-        const int px = px0 + x;
-        const auto srow = gmem(sp.base) + (int64_t)(py0 + yb) * sp.stride + px * (int)sizeof(T);
-        const int64_t sstep = (int64_t)ystep * sp.stride;
-        const auto drow = gmem(dp.base) + (int64_t)(py0 + yb) * dp.stride + px * (int)sizeof(T);
-        const int64_t dstep = (int64_t)ystep * dp.stride;
-        const int ylast = height - 1 - yb;   // rows k * ystep <= ylast exist (ylast >= 0)
-        if (!K[GK_ACT0 + pl]) {   // plane without grain: the display gets the filtered samples
-            T v[MR];
+    const int sx = CHROMA ? src.ssx : 0, sy = CHROMA ? src.ssy : 0;
+    const PlaneView &sp = src.p[pl];
+    const PlaneView &dp = dst.p[pl];
+    const int bw = 32 >> sx, bh = 32 >> sy;
+    const int px0 = bcol * bw, py0 = brow * bh;
+    const int pw = sp.w, ph = sp.h;
+    const int height = min(bh, ph - py0), bwid = min(bw, pw - px0);
+    const int y = rg * 16 + (lane >> 2), x = (lane & 3) * C;
+    const bool act = y < height && x < bwid;
+    const int px = px0 + x, py = py0 + y;
+    const bool full = px + C <= pw;
+    const auto srow = gmem(reinterpret_cast<const T *>(sp.base + (int64_t)py * sp.stride));
+    const auto drow = gmem(reinterpret_cast<T *>(dp.base + (int64_t)py * dp.stride));
+    if (!K[GK_ACT0 + pl]) {   // plane without grain: the display gets the filtered samples
+        if (!act) return;
+        int sa[C];
+        if (full) {
 #pragma unroll
-            for (int k = 0; k < MR; k++) v[k] = *(gptr<const T>)(srow + min(k * ystep, ylast) / ystep * sstep);
+            for (int j = 0; j < C; j++) sa[j] = srow[px + j];
 #pragma unroll
-            for (int k = 0; k < MR; k++)
-                if (k * ystep <= ylast) *(gptr<T>)(drow + k * dstep) = v[k];
-            continue;
+            for (int j = 0; j < C; j++) drow[px + j] = (T)sa[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; j++) sa[j] = srow[min(px + j, pw - 1)];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                if (px + j < pw) drow[px + j] = (T)sa[j];
         }
-        const int ys = (K[GK_OVERLAP] && brow > 0) ? min(height, 2 >> sy) : 0;
-        const int xs = (K[GK_OVERLAP] && bcol > 0) ? min(bwid, 2 >> sx) : 0;
-        const auto tmpl = tmpl0 + pl * GH * GW;
-        int minv = 0, maxv = (256 << (bd - 8)) - 1;
-        if (K[GK_CLIP]) {
-            minv = 16 << (bd - 8);
-            maxv = (pl == 0 ? 235 : (K[GK_MID] ? 235 : 240)) << (bd - 8);
-        }
-        const bool odd_luma = pl && sx && (src.p[0].w & 1);
-        const int mult = pl == 1 ? K[GK_CB_MULT] : K[GK_CR_MULT];
-        const int lmult = pl == 1 ? K[GK_CB_LMULT] : K[GK_CR_LMULT];
-        const int coff = pl == 1 ? K[GK_CB_OFF] : K[GK_CR_OFF];
-        const bool csub_x = pl && sx, csub_y = pl && sy;
-        // template offsets of the four candidate blocks (sample.go:9-77)
-        const int ox0 = 3 + (2 >> sx) * (3 + (offs[0][0] >> 4)), oy0 = 3 + (2 >> sy) * (3 + (offs[0][0] & 15));
-        const int ox1 = 3 + (2 >> sx) * (3 + (offs[1][0] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1][0] & 15));
-        const int ox2 = 3 + (2 >> sx) * (3 + (offs[0][1] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[0][1] & 15));
-        const int ox3 = 3 + (2 >> sx) * (3 + (offs[1][1] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[1][1] & 15));
-        const PlaneView &lp = src.p[0];
-        const auto lrow = gmem(lp.base) + (int64_t)((py0 + yb) << sy) * lp.stride + (px << sx) * (int)sizeof(T);
-        const int64_t lstep = (int64_t)(ystep << sy) * lp.stride;
-        const int l1 = (sx && !(odd_luma && px == pw - 1)) ? (int)sizeof(T) : 0;   // second luma sample
-        const bool left = x < xs;
-        // loads
-        int gm[MR], gl[MR], gt[MR], gtl[MR], sa[MR], la[MR], lb[MR];
+        return;
+    }
+    // the first 1024 bytes of the scaling table (all of it below 12 bits)
+    const auto sc = reinterpret_cast<gptr<const uint32_t>>(gmem(G.scale) + pl * GR_SCALE);
+    const int n16 = (1 << bd) >> 4;
+    uint4 s0 = make_uint4(0, 0, 0, 0);
+    if (lane < n16) {
+        const auto q = sc + 4 * lane;
+        s0 = make_uint4(q[0], q[1], q[2], q[3]);
+    }
+    const int ys = (K[GK_OVERLAP] && brow > 0) ? min(height, 2 >> sy) : 0;
+    const int xs = (K[GK_OVERLAP] && bcol > 0) ? min(bwid, 2 >> sx) : 0;
+    const bool left = act && x < xs, top = act && y < ys;
+    constexpr int NL = CHROMA ? 2 * C : 1;   // luma samples under a chroma group: 2C or C
+    int sa[C], gv[C], la[NL], tp[C];
+    int gl0 = 0, gl1 = 0, gu0 = 0, gu1 = 0;
+    int lw = 0;
+    const auto tmpl = gmem(G.templates) + pl * GH * GW;
+    if (act) {
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < MR; k++) {
-            const int kk = min(k * ystep, ylast) / ystep;   // a row that exists
-            const int y = yb + kk * ystep;
-            const int im = (oy0 + y) * GW + ox0 + x;
-            const bool top = y < ys;
-            gm[k] = tmpl[im];
-            gl[k] = tmpl[left ? (oy1 + y) * GW + ox1 + x + bw : im];
-            gt[k] = tmpl[top ? (oy2 + y + bh) * GW + ox2 + x : im];
-            gtl[k] = tmpl[(top && left) ? (oy3 + y + bh) * GW + ox3 + x + bw : im];
-            sa[k] = *(gptr<const T>)(srow + kk * sstep);
-            if (pl) {
-                const auto l = lrow + kk * lstep;
-                la[k] = *(gptr<const T>)l;
-                lb[k] = *(gptr<const T>)(l + l1);
+            for (int j = 0; j < C; j++) sa[j] = srow[px + j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; j++) sa[j] = srow[min(px + j, pw - 1)];
+        }
+        if constexpr (CHROMA) {
+            const PlaneView &lp = src.p[0];
+            lw = lp.w;
+            const auto lrow = gmem(reinterpret_cast<const T *>(lp.base + (int64_t)(py << sy) * lp.stride));
+            const int lx = px << sx, nl = C << sx;
+            if (lx + nl <= lw) {
+#pragma unroll
+                for (int j = 0; j < NL; j++)
+                    if (j < nl) la[j] = lrow[lx + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < NL; j++)
+                    if (j < nl) la[j] = lrow[min(lx + j, lw - 1)];
             }
         }
-        // grain, scaling, stores
+        // template values (sample.go:9-77 offsets)
+        const int ox0 = 3 + (2 >> sx) * (3 + (offs[0] >> 4)), oy0 = 3 + (2 >> sy) * (3 + (offs[0] & 15));
+        const auto t = tmpl + (oy0 + y) * GW + ox0 + x;
 #pragma unroll
-        for (int k = 0; k < MR; k++) {
-            if (k * ystep > ylast) break;
-            const int y = yb + k * ystep;
-            int gv = gm[k];
-            const int wl = csub_x ? 23 : (x == 1 ? 17 : 27), wc = csub_x ? 22 : (x == 1 ? 27 : 17);
-            if (left) gv = blendw(gl[k], gv, wl, wc, bd);
-            if (y < ys) {
-                int top = gt[k];
-                if (left) top = blendw(gtl[k], top, wl, wc, bd);
-                gv = csub_y ? blendw(top, gv, 23, 22, bd) : blendw(top, gv, y == 1 ? 17 : 27, y == 1 ? 27 : 17, bd);
-            }
-            int idx = sa[k];
-            if (pl) {
-                const int avg = l1 ? (la[k] + lb[k] + 1) >> 1 : la[k];
-                if (K[GK_CSFL]) idx = avg;
-                else {
-                    const int combined = __mul24(avg, lmult - 128) + __mul24(sa[k], mult - 128);
-                    idx = clipi((combined >> 6) + ((coff << (bd - 8)) - (1 << bd)), 0, (1 << bd) - 1);
-                }
-            }
-            const int scale = scale_lut(lut[pl], idx, bd);
-            const int noise = r2i(__mul24(scale, gv), scaling_shift);
-            *(gptr<T>)(drow + k * dstep) = (T)clipi(sa[k] + noise, minv, maxv);
+        for (int j = 0; j < C; j++) gv[j] = t[j];
+        if (left) {
+            const int ox1 = 3 + (2 >> sx) * (3 + (offs[1] >> 4)), oy1 = 3 + (2 >> sy) * (3 + (offs[1] & 15));
+            const auto u = tmpl + (oy1 + y) * GW + ox1 + bw;
+            gl0 = u[0];
+            gl1 = u[1];
         }
+        if (top) {
+            const int ox2 = 3 + (2 >> sx) * (3 + (offs[2] >> 4)), oy2 = 3 + (2 >> sy) * (3 + (offs[2] & 15));
+            const auto u = tmpl + (oy2 + y + bh) * GW + ox2 + x;
+#pragma unroll
+            for (int j = 0; j < C; j++) tp[j] = u[j];
+            if (left) {
+                const int ox3 = 3 + (2 >> sx) * (3 + (offs[3] >> 4)), oy3 = 3 + (2 >> sy) * (3 + (offs[3] & 15));
+                const auto v = tmpl + (oy3 + y + bh) * GW + ox3 + bw;
+                gu0 = v[0];
+                gu1 = v[1];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();   // the previous task of this wave is done with the table
+    if (lane < n16) reinterpret_cast<uint4 *>(lut)[lane] = s0;
+    for (int k = 1; k < GR_SCALE / 16 / 64; k++)   // 12-bit: the other 3 KB
+        if (lane + 64 * k < n16) {
+            const auto q = sc + 4 * (lane + 64 * k);
+            reinterpret_cast<uint4 *>(lut)[lane + 64 * k] = make_uint4(q[0], q[1], q[2], q[3]);
+        }
+    __builtin_amdgcn_wave_barrier();   // LDS operations of a wave complete in order
+    if (!act) return;
+    // horizontal overlap weights of columns 0 and 1 (blend.go)
+    const int wl0 = sx ? 23 : 27, wc0 = sx ? 22 : 17;
+    if (left) {
+        gv[0] = blendw(gl0, gv[0], wl0, wc0, bd);
+        if (xs > 1) gv[1] = blendw(gl1, gv[1], 17, 27, bd);
+    }
+    if (top) {
+        if (left) {
+            tp[0] = blendw(gu0, tp[0], wl0, wc0, bd);
+            if (xs > 1) tp[1] = blendw(gu1, tp[1], 17, 27, bd);
+        }
+        const int wt = sy ? 23 : (y == 1 ? 17 : 27), wb = sy ? 22 : (y == 1 ? 27 : 17);
+#pragma unroll
+        for (int j = 0; j < C; j++) gv[j] = blendw(tp[j], gv[j], wt, wb, bd);
+    }
+    int minv = 0, maxv = (256 << (bd - 8)) - 1;
+    if (K[GK_CLIP]) {
+        minv = 16 << (bd - 8);
+        maxv = (!CHROMA ? 235 : (K[GK_MID] ? 235 : 240)) << (bd - 8);
+    }
+    const int imax = (1 << bd) - 1;
+    const int shift = K[GK_SHIFT], rnd = 1 << (shift - 1);
+    int mult = 0, lmult = 0, cbase = 0;
+    if constexpr (CHROMA) {
+        mult = (pl == 1 ? K[GK_CB_MULT] : K[GK_CR_MULT]) - 128;
+        lmult = (pl == 1 ? K[GK_CB_LMULT] : K[GK_CR_LMULT]) - 128;
+        cbase = ((pl == 1 ? K[GK_CB_OFF] : K[GK_CR_OFF]) << (bd - 8)) - (1 << bd);
+    }
+    int out[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        int idx = sa[j];
+        if constexpr (CHROMA) {
+            int avg = la[j];
+            if (sx) {
+                // the last chroma column of an odd-width luma plane has one luma sample
+                avg = (((px + j) << 1) + 1 < lw) ? (la[2 * j] + la[2 * j + 1] + 1) >> 1 : la[2 * j];
+            }
+            if (K[GK_CSFL]) idx = avg;
+            else idx = clipi(((__mul24(avg, lmult) + __mul24(sa[j], mult)) >> 6) + cbase, 0, imax);
+        }
+        const int scale = lut[min((unsigned)idx, (unsigned)imax)];
+        const int noise = (__mul24(scale, gv[j]) + rnd) >> shift;
+        out[j] = clipi(sa[j] + noise, minv, maxv);
+    }
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < C; j++) drow[px + j] = (T)out[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            if (px + j < pw) drow[px + j] = (T)out[j];
     }
 }
 
+// A unit is one 32x32 luma block position with its chroma blocks.  Its tasks
+// (16-row bands of a plane: 2 of luma, 2 >> ssy of each chroma plane) go to
+// the workgroup's 4 waves in turn; each wave stages its plane's expanded
+// scaling table into its own LDS slice (LDS operations of one wave complete in
+// order, so no barrier is needed) and reads the block offsets of the block and
+// of its left / top / top-left neighbours itself.
 __global__ __launch_bounds__(256) void grain_apply_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
                                                           int nseg, int total, StageLaunch L)
 {
-    __shared__ uint32_t lut32[3 * 64];
-    __shared__ uint8_t offs[2][2];
-    const uint8_t (*lut)[256] = reinterpret_cast<const uint8_t (*)[256]>(lut32);
+    __shared__ __align__(16) uint8_t slut[4][GR_SCALE];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint8_t *lut = slut[wave];
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
         const FrameDesc &D = descs[sg.frame];
         const GrainDesc &G = D.gr;
         const int t = u - sg.base;
         const int bcol = t % sg.nx, brow = t / sg.nx;
-        const int tid = threadIdx.x;
         int32_t K[GK_N];
 #pragma unroll
         for (int i = 0; i < GK_N; i++) K[i] = G.k[i];
-        // the three scaling tables as 192 words, and the block offsets of this
-        // block and its left / top / top-left neighbours, in one round trip
-        if (tid < 3 * 64) lut32[tid] = gmem(reinterpret_cast<const uint32_t *>(&G.luts[0][0]))[tid];
-        else if (tid < 3 * 64 + 4) {
-            const int j = tid - 3 * 64, i = j & 1, lf = j >> 1;
-            const bool use = (i == 0 || (K[GK_OVERLAP] && brow > 0)) && (!lf || (K[GK_OVERLAP] && bcol > 0));
-            const int br = use ? brow - i : brow, bc = use ? bcol - lf : bcol;
-            const uint8_t o = gmem(G.offsets)[(size_t)br * K[GK_OCOLS] + bc];
-            offs[lf][i] = use ? o : 0;
-        }
-        __syncthreads();
         const FrameView &src = D.v[D.fin];
         const FrameView &dst = D.v[VIEW_D];
-        if (src.bd == 8) grain_block<uint8_t>(src, dst, G, K, bcol, brow, lut, offs);
-        else grain_block<uint16_t>(src, dst, G, K, bcol, brow, lut, offs);
+        const int ncb = src.nplanes > 1 ? 2 >> src.ssy : 0;   // chroma tasks per plane
+        const int ntask = 2 + 2 * ncb;
+        // offsets of this block and its left / top / top-left neighbours
+        // (those outside the frame or without overlap are not read: 0)
+        uint8_t offs[4];
+        {
+            const auto o = gmem(G.offsets) + (size_t)brow * K[GK_OCOLS] + bcol;
+            const bool ov = K[GK_OVERLAP];
+            const bool hl = ov && bcol > 0, ht = ov && brow > 0;
+            const uint8_t o0 = o[0];
+            const uint8_t o1 = o[hl ? -1 : 0];
+            const uint8_t o2 = o[ht ? -K[GK_OCOLS] : 0];
+            const uint8_t o3 = o[(hl && ht) ? -K[GK_OCOLS] - 1 : 0];
+            offs[0] = o0;
+            offs[1] = hl ? o1 : 0;
+            offs[2] = ht ? o2 : 0;
+            offs[3] = (hl && ht) ? o3 : 0;
+        }
+        const int bd = src.bd;
+        for (int task = wave; task < ntask; task += 4) {
+            const int pl = task < 2 ? 0 : 1 + (task - 2) / ncb;
+            const int rg = task < 2 ? task : (task - 2) % ncb;
+            if (bd == 8) {
+                if (pl == 0) grain_task<uint8_t, 8, false>(src, dst, G, K, 0, bcol, brow, rg, lut, offs, lane);
+                else if (src.ssx) grain_task<uint8_t, 4, true>(src, dst, G, K, pl, bcol, brow, rg, lut, offs, lane);
+                else grain_task<uint8_t, 8, true>(src, dst, G, K, pl, bcol, brow, rg, lut, offs, lane);
+            } else {
+                if (pl == 0) grain_task<uint16_t, 8, false>(src, dst, G, K, 0, bcol, brow, rg, lut, offs, lane);
+                else if (src.ssx) grain_task<uint16_t, 4, true>(src, dst, G, K, pl, bcol, brow, rg, lut, offs, lane);
+                else grain_task<uint16_t, 8, true>(src, dst, G, K, pl, bcol, brow, rg, lut, offs, lane);
+            }
+        }
     });
 }
 

@@ -96,6 +96,11 @@ struct LrPlaneDesc {
 enum { GK_OVERLAP, GK_CSFL, GK_CLIP, GK_SHIFT, GK_MID, GK_CB_MULT, GK_CB_LMULT, GK_CB_OFF, GK_CR_MULT,
        GK_CR_LMULT, GK_CR_OFF, GK_ACT0, GK_ACT1, GK_ACT2, GK_OCOLS, GK_N };
 
+constexpr int GR_SCALE = 4096;     // expanded scaling table bytes per plane (12-bit samples)
+// film-grain working memory per frame: templates, then the expanded scaling tables
+constexpr size_t GR_TMPL_BYTES = ((size_t)3 * 73 * 82 * 2 + 255) / 256 * 256;
+constexpr size_t GR_FRAME_BYTES = GR_TMPL_BYTES + 3 * GR_SCALE;
+
 struct GrainDesc {
     int32_t k[16];                 // GK_* (GK_MID: matrix identity)
     av1pf_grain_params g;
@@ -103,6 +108,7 @@ struct GrainDesc {
     uint16_t row_seed[3][73];      // LFSR state at the start of each template row
     alignas(16) uint8_t luts[3][256];
     int16_t *templates;            // 3 x 73 x 82
+    uint8_t *scale;                // 3 x GR_SCALE: the scaling function of every sample value
     uint8_t *offsets;              // [orows][ocols] per-32x32-block random offsets
     int orows, ocols;
     int matrix_identity;
