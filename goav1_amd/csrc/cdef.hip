@@ -17,8 +17,9 @@
 // each lane, the winner is taken per block from the eight costs in LDS.  The
 // per-block filter parameters (tap offsets, adjusted strength, shifts) are
 // then packed once into a 16-byte LDS record, and the filter walks the plane
-// row by row with one lane per column (coalesced global stores), reading the
-// record of its block and the thirteen taps from LDS.  The kernel is out of
+// row by row with one lane per pair of columns (coalesced global stores),
+// reading the record of its block and the thirteen taps of both samples from
+// LDS, the arithmetic packed two samples to a register.  The kernel is out of
 // place (src -> dst): every sample of the plane's MI-aligned extent is
 // written, filtered or copied, so neighbours always read pre-CDEF samples
 // exactly as the reference's snapshot does.
@@ -104,16 +105,6 @@ __device__ __forceinline__ uint32_t dir_cost(const int32_t (&v)[64])
     return s;
 }
 
-// Every sample (x, y) of a uw x uh region, one lane per column: calls f(x, y).
-template <typename F>
-__device__ __forceinline__ void for_each_px(int uw, int uh, int tid, F &&f)
-{
-    const int tpr = uw > 32 ? 64 : uw > 16 ? 32 : uw > 8 ? 16 : 8;   // lanes per row, a power of two >= uw
-    const int x = tid & (tpr - 1);
-    if (x >= uw) return;
-    for (int y = tid / tpr; y < uh; y += 256 / tpr) f(x, y);
-}
-
 // A lane's loads of up to 8 rows are in flight before its stores.
 template <typename T>
 __device__ __forceinline__ void copy_unit(const PlaneView &s, const PlaneView &d, int ux, int uy, int uw, int uh, int tid)
@@ -169,19 +160,31 @@ __device__ __forceinline__ void load_tile(uint16_t *tile, int pitch, const Plane
 __device__ __forceinline__ int str_pri(uint8_t p) { return p >> 2; }
 __device__ __forceinline__ int str_sec(uint8_t p) { const int s = p & 3; return s == 3 ? 4 : s; }
 
-// constrain(d, t, sh) = sign(d) min(|d|, max(0, t - (|d| >> sh))), written as
-// the clamp of d to [-c, c] with c = max(0, t - (|d| >> sh)).
-__device__ __forceinline__ int med3i(int a, int b, int c)
+// Two samples in the halves of a register: packed 16-bit arithmetic.
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2s splat2(int v) { return (v2s){ (short)v, (short)v }; }
+__device__ __forceinline__ v2u splatu2(int v) { return (v2u){ (unsigned short)v, (unsigned short)v }; }
+__device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
+// Two adjacent tile samples, read as two 16-bit LDS loads (the second into
+// the high half): the pair is 4-byte aligned only for even offsets, and a
+// merged misaligned 32-bit LDS read stalls.  The empty asm hides the second
+// address from the load merger.
+__device__ __forceinline__ v2s ld2(const uint16_t *p)
 {
-    int r;
-    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
+    const uint16_t *q = p + 1;
+    asm volatile("" : "+v"(q));
+    return (v2s){ (short)p[0], (short)*q };
 }
-__device__ __forceinline__ int cst(int a, int x, int t, int sh)
+// constrain(d, t, sh) = sign(d) min(|d|, max(0, t - (|d| >> sh))) of two
+// samples, written as the clamp of d to [-c, c] with c = max(0, t - (|d| >> sh)).
+__device__ __forceinline__ v2s cst2(v2s a, v2s x, v2u t, v2u sh)
 {
-    const int d = a - x;
-    const unsigned c = __builtin_elementwise_sub_sat((unsigned)t, (unsigned)(d < 0 ? -d : d) >> sh);
-    return med3i(d, -(int)c, (int)c);
+    const v2s d = a - x;
+    const v2u ad = __builtin_bit_cast(v2u, vmax(d, -d));
+    const v2s c = __builtin_bit_cast(v2s, __builtin_elementwise_sub_sat(t, ad >> sh));
+    return vmin(vmax(d, -c), c);
 }
 
 // Per-block filter record: primary tap offsets, secondary tap offsets along
@@ -327,44 +330,57 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
             copy_unit<T>(sp, dp, ux, uy, uw, uh, tid);
             continue;
         }
-        for_each_px(uw, uh, tid, [&](int x, int y) {
-            const BlockRec r = brec[((y >> bhl) << 3) + (x >> bwl)];
-            const uint16_t *c = org + y * TPITCH + x;
-            const int xv = c[0];
-            int out = xv;
-            const int flag = r.p >> 24;
-            if (flag & 1) {
-                const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
-                const int o0 = (int16_t)(r.o & 0xffff), o1 = r.o >> 16;
-                const int s0a = (int16_t)(r.sa & 0xffff), s0b = r.sa >> 16;
-                const int s1a = (int16_t)(r.sb & 0xffff), s1b = r.sb >> 16;
-                int sum = 0;
-                if (pri) {   // st may be 0 for a flat block: its primary terms vanish
-                    const int t0 = ((st >> cs) & 1) ? 3 : 4, t1 = ((st >> cs) & 1) ? 3 : 2;
-                    sum = t0 * (cst(c[o0], xv, st, psh) + cst(c[-o0], xv, st, psh)) +
-                          t1 * (cst(c[o1], xv, st, psh) + cst(c[-o1], xv, st, psh));
+        // the filter walks horizontal sample pairs, one pair per lane: both
+        // samples of a pair lie in one block (blocks are at least 4 wide), so
+        // they share the record, and every tap is one 32-bit LDS read and
+        // packed 16-bit arithmetic (values stay below 2^15: samples < 2^12,
+        // the VeryLarge sentinel 2^14, the weighted sum below 2^12)
+        const int np = (uw + 1) >> 1;
+        const int tpr = np > 16 ? 32 : np > 8 ? 16 : np > 4 ? 8 : 4;   // lanes per row, a power of two >= np
+        const int x = 2 * (tid & (tpr - 1));
+        if (x < uw) {
+            for (int y = tid / tpr; y < uh; y += 256 / tpr) {
+                const BlockRec r = brec[((y >> bhl) << 3) + (x >> bwl)];
+                const uint16_t *c = org + y * TPITCH + x;
+                const v2s xv = ld2(c);
+                v2s out = xv;
+                const int flag = r.p >> 24;
+                if (flag & 1) {
+                    const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
+                    const int o0 = (int16_t)(r.o & 0xffff), o1 = r.o >> 16;
+                    const int s0a = (int16_t)(r.sa & 0xffff), s0b = r.sa >> 16;
+                    const int s1a = (int16_t)(r.sb & 0xffff), s1b = r.sb >> 16;
+                    const v2s a0 = ld2(c + o0), a1 = ld2(c - o0), a2 = ld2(c + o1), a3 = ld2(c - o1);
+                    const v2s b0 = ld2(c + s0a), b1 = ld2(c - s0a), b2 = ld2(c + s1a), b3 = ld2(c - s1a);
+                    const v2s b4 = ld2(c + s0b), b5 = ld2(c - s0b), b6 = ld2(c + s1b), b7 = ld2(c - s1b);
+                    v2s sum = splat2(0);
+                    if (pri) {   // st may be 0 for a flat block: its primary terms vanish
+                        const v2u t = splatu2(st), sh = splatu2(psh);
+                        const v2s t0 = splat2(((st >> cs) & 1) ? 3 : 4), t1 = splat2(((st >> cs) & 1) ? 3 : 2);
+                        sum = t0 * (cst2(a0, xv, t, sh) + cst2(a1, xv, t, sh)) + t1 * (cst2(a2, xv, t, sh) + cst2(a3, xv, t, sh));
+                    }
+                    if (sec) {
+                        const v2u t = splatu2(sec), sh = splatu2(ssh);
+                        sum += splat2(2) * (cst2(b0, xv, t, sh) + cst2(b1, xv, t, sh) + cst2(b2, xv, t, sh) +
+                                            cst2(b3, xv, t, sh)) +
+                               cst2(b4, xv, t, sh) + cst2(b5, xv, t, sh) + cst2(b6, xv, t, sh) + cst2(b7, xv, t, sh);
+                    }
+                    // (8 + sum - (sum < 0)) >> 4; a true vector comparison is -1
+                    out = xv + ((splat2(8) + sum + (v2s)(sum < splat2(0))) >> splat2(4));
+                    if (flag & 2) {   // clip to the taps' range, VeryLarge excluded from the maximum
+                        const v2s mk = splat2(CDEF_VL - 1);
+                        v2s hi = vmax(vmax(vmax(xv, a0 & mk), vmax(a1 & mk, a2 & mk)), vmax(a3 & mk, b0 & mk));
+                        hi = vmax(vmax(vmax(hi, b1 & mk), vmax(b2 & mk, b3 & mk)), vmax(vmax(b4 & mk, b5 & mk), vmax(b6 & mk, b7 & mk)));
+                        v2s lo = vmin(vmin(vmin(xv, a0), vmin(a1, a2)), vmin(a3, b0));
+                        lo = vmin(vmin(vmin(lo, b1), vmin(b2, b3)), vmin(vmin(b4, b5), vmin(b6, b7)));
+                        out = vmin(vmax(out, lo), hi);
+                    }
                 }
-                if (sec) {
-                    sum += 2 * (cst(c[s0a], xv, sec, ssh) + cst(c[-s0a], xv, sec, ssh) +
-                                cst(c[s1a], xv, sec, ssh) + cst(c[-s1a], xv, sec, ssh)) +
-                           cst(c[s0b], xv, sec, ssh) + cst(c[-s0b], xv, sec, ssh) +
-                           cst(c[s1b], xv, sec, ssh) + cst(c[-s1b], xv, sec, ssh);
-                }
-                out = xv + ((8 + sum - (sum < 0)) >> 4);
-                if (flag & 2) {   // clip to the taps' range, VeryLarge excluded from the maximum
-                    const int mk = CDEF_VL - 1;
-                    const int a0 = c[o0], a1 = c[-o0], a2 = c[o1], a3 = c[-o1];
-                    const int b0 = c[s0a], b1 = c[-s0a], b2 = c[s1a], b3 = c[-s1a];
-                    const int b4 = c[s0b], b5 = c[-s0b], b6 = c[s1b], b7 = c[-s1b];
-                    int hi = max(max(max(xv, a0 & mk), max(a1 & mk, a2 & mk)), max(a3 & mk, b0 & mk));
-                    hi = max(max(max(hi, b1 & mk), max(b2 & mk, b3 & mk)), max(max(b4 & mk, b5 & mk), max(b6 & mk, b7 & mk)));
-                    int lo = min(min(min(xv, a0), min(a1, a2)), min(a3, b0));
-                    lo = min(min(min(lo, b1), min(b2, b3)), min(min(b4, b5), min(b6, b7)));
-                    out = min(max(out, lo), hi);
-                }
+                const auto o = gmem(row_ptr<T>(dp, uy + y) + ux + x);
+                o[0] = (T)(uint16_t)out.x;
+                if (x + 1 < uw) o[1] = (T)(uint16_t)out.y;
             }
-            row_ptr<T>(dp, uy + y)[ux + x] = (T)out;
-        });
+        }
     }
 }
 

@@ -31,12 +31,15 @@ static __device__ __constant__ int16_t kSgrS[16][2] = {
     {140,3236},{112,2158},{93,1618},{80,1438},{70,1295},{58,1177},{47,1079},{37,996},
     {30,925},{25,863},{-1,2589},{-1,1618},{-1,1177},{-1,925},{56,-1},{22,-1}};
 
-// AV1's x/(x+1) Q8 table from its closed form (exact: no ties occur)
+// AV1's x/(x+1) Q8 table from its closed form floor((513 z + 1) / (2 z + 2)).
+// The quotient is never an integer for 1 <= z <= 254 (2z + 2 would have to
+// divide 511 - z, i.e. 2m + 1 would divide 512), so it lies at least 1/510
+// from one; a float product with the hardware reciprocal (relative error
+// under 2^-22, absolute under 1e-4 here) floors to the exact value.
 __device__ __forceinline__ int32_t x_by_xplus1(uint32_t z)
 {
-    if (z == 0) return 1;
-    if (z >= 255) return 256;
-    return (int32_t)((512u * z + (z + 1)) / (2u * (z + 1)));
+    const float q = (float)(513u * z + 1u) * __builtin_amdgcn_rcpf((float)(2u * z + 2u));
+    return z == 0 ? 1 : (z >= 255 ? 256 : (int32_t)q);
 }
 
 // Lanes per row for a region w samples wide (a power of two >= w, at most 64):
