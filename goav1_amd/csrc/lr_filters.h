@@ -42,6 +42,18 @@ __device__ __forceinline__ int32_t x_by_xplus1(uint32_t z)
     return z == 0 ? 1 : (z >= 255 ? 256 : (int32_t)q);
 }
 
+// Samples s[K0..K1] of an LDS row as separate 16-bit loads: the odd ones go
+// through an opaque copy of the base, so no two loads are adjacent to the load
+// merger (a merged 32-bit read is misaligned for half the lanes and stalls).
+template <int K0, int K1>
+__device__ __forceinline__ void lds_run(const uint16_t *s, int32_t (&v)[K1 - K0 + 1])
+{
+    const uint16_t *so = s;
+    asm volatile("" : "+v"(so));
+#pragma unroll
+    for (int k = K0; k <= K1; k++) v[k - K0] = (k & 1) ? so[k] : s[k];
+}
+
 // Lanes per row for a region w samples wide (a power of two >= w, at most 64):
 // lane x of a row group handles column x, row groups step by 256 / lanes.
 __device__ __forceinline__ int lanes_for(int w) { return w > 32 ? 64 : w > 16 ? 32 : w > 8 ? 16 : 8; }
@@ -65,10 +77,10 @@ __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int
     if (c < w)
         for (int rr = tid / lpr; rr < th; rr += rstep) {
             const int r = rr - 3;
-            const uint16_t *s = O + r * LR_TW + c;
-            int32_t sum = ((int32_t)s[0] << 7) + hoff;
-            sum += h0 * ((int32_t)s[-3] + s[3]) + h1 * ((int32_t)s[-2] + s[2]) + h2 * ((int32_t)s[-1] + s[1]) +
-                   h3 * (int32_t)s[0];
+            int32_t t[7];
+            lds_run<-3, 3>(O + r * LR_TW + c, t);
+            int32_t sum = (t[3] << 7) + hoff;
+            sum += h0 * (t[0] + t[6]) + h1 * (t[1] + t[5]) + h2 * (t[2] + t[4]) + h3 * t[3];
             tmp[rr * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
         }
     __syncthreads();
@@ -87,13 +99,14 @@ __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int
 template <int R>
 __device__ __forceinline__ void row_box(const uint16_t *s, int32_t &sum, int32_t &sq)
 {
+    int32_t t[2 * R + 1];
+    lds_run<-R, R>(s, t);
     sum = 0;
     sq = 0;
 #pragma unroll
-    for (int dx = -R; dx <= R; dx++) {
-        const int32_t v = s[dx];
-        sum += v;
-        sq += v * v;
+    for (int i = 0; i <= 2 * R; i++) {
+        sum += t[i];
+        sq += t[i] * t[i];
     }
 }
 
