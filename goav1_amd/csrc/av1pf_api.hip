@@ -507,22 +507,14 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             db.mi_cols = p->mi_cols;
             db.sbw = p->sb128_w;
             db.sharp = p->sharpness;
-            for (int l = 0; l < 64; l++) {
-                int shift = (p->sharpness > 0) + (p->sharpness > 4);
-                int lim = l >> shift;
-                if (p->sharpness > 0 && lim > 9 - p->sharpness) lim = 9 - p->sharpness;
-                if (lim < 1) lim = 1;
-                db.I[l] = (uint8_t)lim;
-                db.E[l] = (uint8_t)(2 * (l + 2) + lim);
-            }
             const bool luma_zero = p->level_y[0] == 0 && p->level_y[1] == 0;
             for (int pl = 0; pl < A.nplanes; pl++) {
                 if (pl && (luma_zero || (pl == 1 ? p->level_u : p->level_v) == 0)) continue;
                 const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
                 const int w4 = ((((f->width + sx) >> sx) + 3) & ~3) >> 2;
                 const int h4 = ((((f->height + sy) >> sy) + 3) & ~3) >> 2;
-                P.dbv.add(i, pl, (w4 + 63) / 64, (h4 + 3) / 4);
-                P.dbh.add(i, pl, (w4 + 63) / 64, (h4 + 3) / 4);
+                P.dbv.add(i, pl, (w4 + 63) / 64, (h4 + DB_ROWS - 1) / DB_ROWS);
+                P.dbh.add(i, pl, (w4 + 63) / 64, (h4 + DB_ROWS - 1) / DB_ROWS);
             }
             P.bytes_db += 2 * view_bytes(A);
         }

@@ -10,11 +10,12 @@
 // Parallel decomposition.  The reference walks 128-px region rows, vertical
 // edges of every plane then horizontal edges.  Because decoder masks obey AV1's
 // transform-spacing rule, the edges of one direction never read a sample another
-// edge of that direction writes, so the whole vertical pass is one launch with a
-// thread per filtered line (across all planes), then the horizontal pass.  Each
-// line is independent inside dav1d's loop_filter(), so a thread per line is the
-// reference's per-unit loop split four ways.  The pass is HBM bound: a thread
-// touches at most 14 samples of its own row (vertical) or column (horizontal).
+// edge of that direction writes, so the whole vertical pass is one launch
+// (across all planes and frames), then the horizontal pass.  A workgroup takes
+// 64 x DB_ROWS 4x4 units of a plane: it gathers the units with an edge into one
+// LDS list per filter width, then filters the lists with a thread per unit
+// (its 4 lines; every line is independent inside dav1d's loop_filter()), so no
+// lane idles on a unit without an edge and a wave's lanes share the width.
 #include "pf_internal.h"
 
 namespace pf {
@@ -255,69 +256,116 @@ __device__ __forceinline__ int unit_edge(const DbDesc &db, const DbPlaneK &P, in
     return unit_wd(mask_bit(w0, n, lane), mask_bit(w1, n, lane), P.luma ? mask_bit(w2, n, lane) : 0, P.luma);
 }
 
-// Vertical edges: a 64 x 4 workgroup covers 64 edge columns (x4) by 4 rows of
-// 4x4 units (y4); a thread looks up its unit's mask and level once and filters
-// the unit's 4 pixel rows.
+// Append one entry per lane with p set to an LDS list: one LDS atomic per
+// wave, each lane's slot from the count of set lanes below it.  Returns the
+// lane's slot (-1 when p is false).
+__device__ __forceinline__ int wave_append(int *cnt, bool p)
+{
+    const uint64_t b = __ballot(p);
+    if (!b) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(b));
+    base = __shfl(base, leader);
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+    return p ? base + below : -1;
+}
+
+// One direction of a plane's edges (DIR 0: vertical edges, filtered along
+// rows; 1: horizontal edges, filtered along columns).  A workgroup covers 64
+// columns by DB_ROWS rows of 4x4 units.  First every unit's mask bits and level
+// are read (a thread per unit column, DB_ROWS / 4 rows per thread, all loads
+// in flight together) and the units with an edge to filter are appended to an
+// LDS list per filter width; then the lists are filtered width by width, a
+// thread per unit, so every lane has an edge and the lanes of a wave share the
+// width (same loads, same filter branches).
+template <int DIR>
+__device__ __forceinline__ void deblock_dir(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+{
+    __shared__ uint32_t list[3][64 * DB_ROWS];
+    __shared__ int cnt[3];
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const FrameDesc &D = descs[sg.frame];
+    const FrameView &f = D.v[VIEW_A];
+    const int pl = sg.plane;
+    const DbPlaneK P = db_plane(f, pl);
+    const int t = blockIdx.x - sg.base;
+    const int tx = t % sg.nx, ty = t / sg.nx;
+    const int tid = threadIdx.x;
+    if (tid < 3) cnt[tid] = 0;
+    __syncthreads();
+    constexpr int K = DB_ROWS / 4;
+    int wd[K], L[K];
+    const int lx = tid & 63;
+    const int x4 = tx * 64 + lx;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int ly = (tid >> 6) + 4 * k;
+        const int y4 = ty * DB_ROWS + ly;
+        wd[k] = 0;
+        L[k] = 0;
+        const bool in = x4 < P.w4 && y4 < P.h4 && (DIR == 0 ? x4 > 0 : y4 > 0);
+        if (in) {
+            if (DIR == 0)
+                wd[k] = unit_edge(D.db, P, 0, y4 >> P.lrch, x4 >> P.lrcw, x4 & (P.rcw - 1), y4 & (P.rch - 1),
+                                  y4 * D.db.mi_cols + x4, 1, P.comp_v, L[k]);
+            else
+                wd[k] = unit_edge(D.db, P, 1, y4 >> P.lrch, x4 >> P.lrcw, y4 & (P.rch - 1), x4 & (P.rcw - 1),
+                                  y4 * D.db.mi_cols + x4, D.db.mi_cols, P.comp_h, L[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int ly = (tid >> 6) + 4 * k;
+        const bool on = wd[k] && L[k];
+        const int cls = wd[k] >= 16 ? 0 : wd[k] >= 6 ? 1 : 2;
+        const uint32_t e = (uint32_t)lx | ((uint32_t)ly << 6) | ((uint32_t)L[k] << 10);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int slot = wave_append(&cnt[c], on && cls == c);
+            if (slot >= 0) list[c][slot] = e;
+        }
+    }
+    __syncthreads();
+    const PlaneView &pv = f.p[pl];
+    for (int c = 0; c < 3; c++) {
+        const int w = c == 0 ? 16 : c == 1 ? (P.luma ? 8 : 6) : 4;
+        for (int i = tid; i < cnt[c]; i += 256) {
+            const uint32_t e = list[c][i];
+            const int ex = tx * 64 + (int)(e & 63), ey = ty * DB_ROWS + (int)((e >> 6) & 15), el = (int)(e >> 10);
+            int E, I;
+            db_limits(el, D.db.sharp, E, I);
+            if (f.bd == 8) {
+                const auto q = gmem(row_ptr<uint8_t>(pv, ey * 4) + ex * 4);
+                if (DIR == 0) lpf_unit<false, uint8_t>(q, pv.stride, 1, 8, E, I, el >> 4, w);
+                else lpf_unit<true, uint8_t>(q, 1, pv.stride, 8, E, I, el >> 4, w);
+            } else {
+                const auto q = gmem(row_ptr<uint16_t>(pv, ey * 4) + ex * 4);
+                if (DIR == 0) lpf_unit<false, uint16_t>(q, pv.stride / 2, 1, f.bd, E, I, el >> 4, w);
+                else lpf_unit<true, uint16_t>(q, 1, pv.stride / 2, f.bd, E, I, el >> 4, w);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restrict__ descs,
                                                         const Seg *__restrict__ segs, int nseg)
 {
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const FrameView &f = D.v[VIEW_A];
-    const int pl = sg.plane;
-    const DbPlaneK P = db_plane(f, pl);
-    const int t = blockIdx.x - sg.base;
-    const int tx = t % sg.nx, ty = t / sg.nx;
-    const int x4 = tx * 64 + threadIdx.x;
-    const int y4 = ty * 4 + threadIdx.y;
-    if (x4 == 0 || x4 >= P.w4 || y4 >= P.h4) return;
-    int L;
-    const int wd = unit_edge(D.db, P, 0, y4 >> P.lrch, x4 >> P.lrcw, x4 & (P.rcw - 1), y4 & (P.rch - 1),
-                             y4 * D.db.mi_cols + x4, 1, P.comp_v, L);
-    if (!wd || L == 0) return;
-    int E, I;
-    db_limits(L, D.db.sharp, E, I);
-    const PlaneView &pv = f.p[pl];
-    if (f.bd == 8)
-        lpf_unit<false, uint8_t>(gmem(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4), pv.stride, 1, 8, E, I, L >> 4, wd);
-    else
-        lpf_unit<false, uint16_t>(gmem(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4), pv.stride / 2, 1, f.bd, E, I, L >> 4, wd);
+    deblock_dir<0>(descs, segs, nseg);
 }
 
-// Horizontal edges: a 64 x 4 workgroup covers 64 columns of 4x4 units (x4) by 4
-// edge rows (y4); a thread looks up its unit's mask and level once and filters
-// the unit's 4 pixel columns.
 __global__ __launch_bounds__(256) void deblock_h_kernel(const FrameDesc *__restrict__ descs,
                                                         const Seg *__restrict__ segs, int nseg)
 {
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const FrameView &f = D.v[VIEW_A];
-    const int pl = sg.plane;
-    const DbPlaneK P = db_plane(f, pl);
-    const int t = blockIdx.x - sg.base;
-    const int tx = t % sg.nx, ty = t / sg.nx;
-    const int x4 = tx * 64 + threadIdx.x;
-    const int y4 = ty * 4 + threadIdx.y;
-    if (y4 == 0 || y4 >= P.h4 || x4 >= P.w4) return;
-    int L;
-    const int wd = unit_edge(D.db, P, 1, y4 >> P.lrch, x4 >> P.lrcw, y4 & (P.rch - 1), x4 & (P.rcw - 1),
-                             y4 * D.db.mi_cols + x4, D.db.mi_cols, P.comp_h, L);
-    if (!wd || L == 0) return;
-    int E, I;
-    db_limits(L, D.db.sharp, E, I);
-    const PlaneView &pv = f.p[pl];
-    if (f.bd == 8)
-        lpf_unit<true, uint8_t>(gmem(row_ptr<uint8_t>(pv, y4 * 4) + x4 * 4), 1, pv.stride, 8, E, I, L >> 4, wd);
-    else
-        lpf_unit<true, uint16_t>(gmem(row_ptr<uint16_t>(pv, y4 * 4) + x4 * 4), 1, pv.stride / 2, f.bd, E, I, L >> 4, wd);
+    deblock_dir<1>(descs, segs, nseg);
 }
 
 hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,
                           hipStream_t s)
 {
-    if (totv) hipLaunchKernelGGL(deblock_v_kernel, dim3(totv), dim3(64, 4), 0, s, d, segv, nv);
-    if (toth) hipLaunchKernelGGL(deblock_h_kernel, dim3(toth), dim3(64, 4), 0, s, d, segh, nh);
+    if (totv) hipLaunchKernelGGL(deblock_v_kernel, dim3(totv), dim3(256), 0, s, d, segv, nv);
+    if (toth) hipLaunchKernelGGL(deblock_h_kernel, dim3(toth), dim3(256), 0, s, d, segh, nh);
     return hipGetLastError();
 }
 

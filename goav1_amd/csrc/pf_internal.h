@@ -54,8 +54,7 @@ struct DbDesc {
     const uint8_t *lc;             // level cache, 4 bytes per MI
     const av1pf_lf_mask *masks;
     int mi_cols, sbw;
-    int sharp;                     // sharpness 0..7
-    uint8_t E[64], I[64];          // per-level limits for the frame sharpness
+    int sharp;                     // sharpness 0..7 (the kernels derive the limits of a level from it)
 };
 
 struct CdefDesc {
@@ -207,6 +206,7 @@ __device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, B
 int stage_grid(const void *kernel, int total, int per_wg);
 
 // ---- launches (all batched) ------------------------------------------------
+constexpr int DB_ROWS = 8;         // rows of 4x4 units per deblock workgroup (64 units wide)
 hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv, const Seg *segh, int nh, int toth,
                           hipStream_t s);
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
