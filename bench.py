@@ -46,6 +46,9 @@ def parse():
                     help="frames in the mixed-batch leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 512 MiB device copy and exit (HBM counter calibration)")
+    ap.add_argument("--modes", default=None,
+                    help="CDEF,LR,grain work distributions of the main leg (tiles per workgroup, 0 = queue) "
+                         "instead of autotuning, e.g. to count HBM traffic at the distribution a timed run chose")
     return ap.parse_args()
 
 
@@ -457,7 +460,14 @@ def main():
     # copies; the median region is reported.
     # the autotuner picks the work distribution of CDEF, LR and grain apply for
     # this batch's shape class (on a pristine copy, outside the timed region)
-    tuned_modes = runner.autotune(pristine(1)[0], sides)
+    if args.modes:
+        m = [int(v) for v in args.modes.split(",")]
+        for stage, mode in zip((abi.STAGE_CDEF, abi.STAGE_LR, abi.STAGE_FILMGRAIN), m):
+            runner.set_stage_mode(stage, mode)
+        tuned_modes = {"cdef": m[0], "lr": m[1], "grain_apply": m[2]}
+        timed(pristine(1), True, False)
+    else:
+        tuned_modes = runner.autotune(pristine(1)[0], sides)
     runner.sync()
     timed(pristine(args.warmup), True, False)
     regions = []

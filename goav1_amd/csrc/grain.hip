@@ -25,8 +25,11 @@
 // side stream overlapped with deblock / CDEF / LR.
 // The luma template kernel also draws the per-block random offsets (one LFSR
 // walk per 32-line stripe) into a small table.
+// The luma template kernel also expands each plane's scaling function to all
+// 1 << bd sample values (one byte each) next to the templates.
 // Apply: one workgroup per 32x32 luma block and its chroma blocks (the offsets
-// of block (row, col) are shared by all planes); scaling LUTs in LDS.  Planes
+// of block (row, col) are shared by all planes); a wave per 16-row band of a
+// plane, the plane's expanded scaling table in the wave's LDS slice.  Planes
 // without grain are copied, so the display frame is complete.
 #include "pf_internal.h"
 #include "av1pf_gaussian_table.h"
