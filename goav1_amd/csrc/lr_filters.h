@@ -157,26 +157,37 @@ __device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, 
     }
 }
 
-// Lane layout of the per-pixel filter stages: lane x of row group ry handles
-// column x of rows ry, ry + rstep, ... (at most SGR_PPT rows).
+// Lane layout of the per-pixel filter stages: lane x of row group g handles
+// column x of the consecutive rows [ry, end) (at most SGR_PPT; ry even, so a
+// row's parity is fixed by its step j).
 #define SGR_PPT 16
 struct LaneRows {
-    int x, ry, rstep;
+    int x, ry, end;
 };
-__device__ __forceinline__ LaneRows lane_rows(int w)
+__device__ __forceinline__ LaneRows lane_rows(int w, int h)
 {
-    const int lpr = lanes_for(w);
-    return { (int)threadIdx.x & (lpr - 1), (int)threadIdx.x / lpr, 256 / lpr };
+    const int lpr = lanes_for(w), groups = 256 / lpr;
+    const int rpg = (((h + groups - 1) / groups) + 1) & ~1;   // rows per group, even, <= SGR_PPT for h <= 64
+    const int ry = ((int)threadIdx.x / lpr) * rpg;
+    return { (int)threadIdx.x & (lpr - 1), ry, min(ry + rpg, h) };
 }
 
-// The two self-guided outputs of set `set` for this lane's pixels (Q4; zero
-// for a pass of radius 0): f0 from the r = 2 pass, f1 from the r = 1 pass.
-__device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
-                                            int32_t (&f0)[SGR_PPT], int32_t (&f1)[SGR_PPT])
+// The self-guided filter outputs of set `set` for this lane's pixels (Q4),
+// pass by pass (pass 0: r = 2, pass 1: r = 1; a pass of radius 0 is skipped):
+// use(pass, j, u, f) gets row ry + j's output f and its centre sample u.
+// A lane walks its rows downwards, so the weighted 3x3 sums of A and B
+// (selfguided.go) are built from per-row sums kept across steps: with H the
+// sum of a row's three values around x and C the centre one,
+//   r = 2 (A, B on rows -1, 1, 3, ...): P = 6 C + 5 (left + right) = 5 H + C,
+//     an even row takes P(y - 1) + P(y + 1) (shift 9), an odd row P(y) (shift 8);
+//   r = 1 (every row): 4 (cross) + 3 (diagonals) = 4 H(y) + Q(y - 1) + Q(y + 1),
+//     Q = 3 H + C;
+// the same integer sums as the stencils, regrouped.
+template <typename Use>
+__device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
+                                           Use use)
 {
-    const LaneRows L = lane_rows(w);
-#pragma unroll
-    for (int j = 0; j < SGR_PPT; j++) { f0[j] = 0; f1[j] = 0; }
+    const LaneRows L = lane_rows(w, h);
     for (int pass = 0; pass < 2; pass++) {
         const int r = kSgrR[set][pass];
         if (r == 0) continue;
@@ -185,35 +196,72 @@ __device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int
         if (pass == 0) sgr_ab<2, 2>(O, w, h, bd, s, Ab, Bb);
         else sgr_ab<1, 1>(O, w, h, bd, s, Ab, Bb);
         __syncthreads();
-        if (L.x < w) {
+        if (L.x >= w || L.ry >= L.end) continue;
+        const int x = L.x;
+        // H and C of A and B on row y (the tables are offset by one row and column)
+        auto row = [&](int y, int32_t &ha, int32_t &ca, int32_t &hb, int32_t &cb) {
+            const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
+            const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
+            ca = A[0];
+            ha = A[-1] + A[0] + A[1];
+            cb = B[0];
+            hb = B[-1] + B[0] + B[1];
+        };
+        if (pass == 0) {
+            int32_t ha, ca, hb, cb;
+            row(L.ry - 1, ha, ca, hb, cb);
+            int32_t pa = 5 * ha + ca, pb = 5 * hb + cb;
 #pragma unroll
             for (int j = 0; j < SGR_PPT; j++) {
-                const int y = L.ry + L.rstep * j, x = L.x;
-                if (y >= h) break;
+                const int y = L.ry + j;
+                if (y >= L.end) break;
                 const int32_t uu = O[y * LR_TW + x];
-                const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
-                const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
-                int32_t a, b;
-                if (pass == 0) {
-                    if ((y & 1) == 0) {
-                        a = (A[-LR_AB] + A[LR_AB]) * 6 + (A[-LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB - 1] + A[LR_AB + 1]) * 5;
-                        b = (B[-LR_AB] + B[LR_AB]) * 6 + (B[-LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB - 1] + B[LR_AB + 1]) * 5;
-                        f0[j] = r2s(a * uu + b, 9);
-                    } else {
-                        a = A[0] * 6 + (A[-1] + A[1]) * 5;
-                        b = B[0] * 6 + (B[-1] + B[1]) * 5;
-                        f0[j] = r2s(a * uu + b, 8);
-                    }
+                if ((j & 1) == 0) {
+                    row(y + 1, ha, ca, hb, cb);
+                    const int32_t na = 5 * ha + ca, nb = 5 * hb + cb;
+                    use(0, j, uu, r2s((pa + na) * uu + (pb + nb), 9));
+                    pa = na;
+                    pb = nb;
                 } else {
-                    a = (A[0] + A[-1] + A[1] + A[-LR_AB] + A[LR_AB]) * 4 +
-                        (A[-LR_AB - 1] + A[LR_AB - 1] + A[-LR_AB + 1] + A[LR_AB + 1]) * 3;
-                    b = (B[0] + B[-1] + B[1] + B[-LR_AB] + B[LR_AB]) * 4 +
-                        (B[-LR_AB - 1] + B[LR_AB - 1] + B[-LR_AB + 1] + B[LR_AB + 1]) * 3;
-                    f1[j] = r2s(a * uu + b, 9);
+                    use(0, j, uu, r2s(pa * uu + pb, 8));
                 }
+            }
+        } else {
+            int32_t ha0, ca0, hb0, cb0, ha1, ca1, hb1, cb1;
+            row(L.ry - 1, ha0, ca0, hb0, cb0);
+            row(L.ry, ha1, ca1, hb1, cb1);
+            int32_t qa0 = 3 * ha0 + ca0, qb0 = 3 * hb0 + cb0;
+#pragma unroll
+            for (int j = 0; j < SGR_PPT; j++) {
+                const int y = L.ry + j;
+                if (y >= L.end) break;
+                int32_t ha2, ca2, hb2, cb2;
+                row(y + 1, ha2, ca2, hb2, cb2);
+                const int32_t qa2 = 3 * ha2 + ca2, qb2 = 3 * hb2 + cb2;
+                const int32_t uu = O[y * LR_TW + x];
+                use(1, j, uu, r2s((4 * ha1 + qa0 + qa2) * uu + (4 * hb1 + qb0 + qb2), 9));
+                qa0 = 3 * ha1 + ca1;
+                qb0 = 3 * hb1 + cb1;
+                ha1 = ha2;
+                ca1 = ca2;
+                hb1 = hb2;
+                cb1 = cb2;
             }
         }
     }
+}
+
+// The two self-guided outputs of set `set` for this lane's pixels (Q4; zero
+// for a pass of radius 0): f0 from the r = 2 pass, f1 from the r = 1 pass.
+__device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
+                                            int32_t (&f0)[SGR_PPT], int32_t (&f1)[SGR_PPT])
+{
+#pragma unroll
+    for (int j = 0; j < SGR_PPT; j++) { f0[j] = 0; f1[j] = 0; }
+    sgr_passes(O, w, h, bd, set, Ab, Bb, [&](int pass, int j, int32_t, int32_t f) {
+        if (pass == 0) f0[j] = f;
+        else f1[j] = f;
+    });
 }
 
 // xq of a set from its coded xqd (restoration/selfguided.go:111-126).
@@ -244,13 +292,13 @@ __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd
     sgr_xq(set, xqd, xq0, xq1);
     int32_t f0[SGR_PPT], f1[SGR_PPT];
     sgr_filters(O, w, h, bd, set, Ab, Bb, f0, f1);
-    const LaneRows L = lane_rows(w);
+    const LaneRows L = lane_rows(w, h);
     const int32_t mx = (1 << bd) - 1;
     if (L.x < w) {
 #pragma unroll
         for (int j = 0; j < SGR_PPT; j++) {
-            const int y = L.ry + L.rstep * j;
-            if (y >= h) break;
+            const int y = L.ry + j;
+            if (y >= L.end) break;
             sink(y, L.x, sgr_project(O[y * LR_TW + L.x], f0[j], f1[j], xq0, xq1, mx));
         }
     }

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
         const uint16_t *O = tile + 3 * LR_TW + 3;
         const uint64_t wh = (uint64_t)(w * h);
         uint64_t best = base;
-        const LaneRows L = lane_rows(w);
+        const LaneRows L = lane_rows(w, h);
         // ---- Wiener fit: alternating least squares over this lane's pixels
         int8_t fv[3] = { 0, 0, 0 }, fh[3] = { 0, 0, 0 };
         for (int half = 0; half < 4; half++) {
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             int64_t st[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
             if (L.x < w) {
                 for (int j = 0; j < SGR_PPT; j++) {
-                    const int y = L.ry + L.rstep * j, x = L.x;
-                    if (y >= h) break;
+                    const int y = L.ry + j, x = L.x;
+                    if (y >= L.end) break;
                     int64_t Y[7];
 #pragma unroll
                     for (int k = 0; k < 7; k++) {
@@ -352,8 +352,8 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             if (L.x < w) {
 #pragma unroll
                 for (int j = 0; j < SGR_PPT; j++) {
-                    const int y = L.ry + L.rstep * j, x = L.x;
-                    if (y >= h) break;
+                    const int y = L.ry + j, x = L.x;
+                    if (y >= L.end) break;
                     const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
                     const int64_t d0 = r0 ? f0[j] - uu : 0, d1 = r1 ? f1[j] - uu : 0;
                     const int64_t ee = (((int64_t)srcT[y * 64 + x] << 4) - uu) << 7;
@@ -378,8 +378,8 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
                 if (L.x < w) {
 #pragma unroll
                     for (int j = 0; j < SGR_PPT; j++) {
-                        const int y = L.ry + L.rstep * j, x = L.x;
-                        if (y >= h) break;
+                        const int y = L.ry + j, x = L.x;
+                        if (y >= L.end) break;
                         const int d = (int)srcT[y * 64 + x] - sgr_project(O[y * LR_TW + x], f0[j], f1[j], xq0, xq1, 255);
                         a2 += (uint64_t)(d * d);
                     }
