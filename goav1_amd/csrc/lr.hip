@@ -36,7 +36,7 @@ __device__ __forceinline__ int read_px(const PlaneView &pv, int y, int x)
 
 template <typename T>
 __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp, const LrPlaneDesc &g, int bd,
-                                        int k, int t, uint16_t *tile, uint16_t *tmp, int32_t *Ab, int32_t *Bb)
+                                        int k, int t, uint16_t *tile, uint16_t *tmp, uint16_t *Ab, int32_t *Bb)
 {
     if (k >= g.xtiles || t >= g.stripes) return;
     const int x0 = k * g.procw;
@@ -111,16 +111,17 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     else sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb, put);
 }
 
-__global__ __launch_bounds__(256) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
+// 4 workgroups per CU: 35.9 KB of LDS each, and at most 128 VGPRs
+__global__ __launch_bounds__(256, 4) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
                                                  int total, StageLaunch L)
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     // Wiener intermediate and self-guided A/B planes share one buffer
-    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 6];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
-    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
-    static_assert(LR_TW * 64 * 2 <= LR_AB * LR_AB * 8, "Wiener intermediate fits the shared buffer");
+    static_assert(LR_TW * 64 * 2 <= LR_AB * LR_AB * 6, "Wiener intermediate fits the shared buffer");
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
         const FrameDesc &D = descs[sg.frame];

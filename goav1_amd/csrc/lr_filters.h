@@ -12,8 +12,8 @@
 //     an int16 wrap before the clip.
 // Block size 256 threads; the caller synchronises before the call.  LDS the
 // caller provides: the tile, a 70 x 64 uint16 Wiener intermediate, and the
-// self-guided A and B planes (int32, LR_AB x LR_AB); the Wiener intermediate
-// may share memory with A and B.
+// self-guided A (uint16: at most 256) and B (int32) planes, LR_AB x LR_AB; the
+// Wiener intermediate may share memory with A and B.
 #pragma once
 #include "pf_internal.h"
 
@@ -115,7 +115,7 @@ __device__ __forceinline__ void row_box(const uint16_t *s, int32_t &sum, int32_t
 // 256 / (w + 2) row groups; the box sums slide down the column, each row's
 // horizontal box sum computed once.
 template <int R, int STEP>
-__device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, uint32_t s, int32_t *Ab, int32_t *Bb)
+__device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, uint32_t s, uint16_t *Ab, int32_t *Bb)
 {
     constexpr int N = (2 * R + 1) * (2 * R + 1);
     const uint32_t recip = (uint32_t)((8192 + N) / (2 * N));
@@ -152,7 +152,7 @@ __device__ __forceinline__ void sgr_ab(const uint16_t *O, int w, int h, int bd, 
         uint32_t z = r2u(p * s, 20);
         z = min(z, 255u);
         const int32_t aa = x_by_xplus1(z);
-        Ab[(y + 1) * LR_AB + x + 1] = aa;
+        Ab[(y + 1) * LR_AB + x + 1] = (uint16_t)aa;
         Bb[(y + 1) * LR_AB + x + 1] = (int32_t)r2u((uint32_t)(256 - aa) * (uint32_t)sum * recip, 12);
     }
 }
@@ -184,7 +184,7 @@ __device__ __forceinline__ LaneRows lane_rows(int w, int h)
 //     Q = 3 H + C;
 // the same integer sums as the stencils, regrouped.
 template <typename Use>
-__device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
+__device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int bd, int set, uint16_t *Ab, int32_t *Bb,
                                            Use use)
 {
     const LaneRows L = lane_rows(w, h);
@@ -198,12 +198,20 @@ __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int 
         __syncthreads();
         if (L.x >= w || L.ry >= L.end) continue;
         const int x = L.x;
+        // the centre A value is read through a second base the load merger
+        // cannot relate to the first (an opaque zero offset), so the three
+        // 16-bit reads of a row stay separate: merged, they would be
+        // misaligned for half the lanes
+        int oz = 0;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+        const uint16_t *Az = Ab + oz;
         // H and C of A and B on row y (the tables are offset by one row and column)
         auto row = [&](int y, int32_t &ha, int32_t &ca, int32_t &hb, int32_t &cb) {
-            const int32_t *A = Ab + (y + 1) * LR_AB + x + 1;
-            const int32_t *B = Bb + (y + 1) * LR_AB + x + 1;
-            ca = A[0];
-            ha = A[-1] + A[0] + A[1];
+            const int i = (y + 1) * LR_AB + x + 1;
+            const uint16_t *A = Ab + i;
+            const int32_t *B = Bb + i;
+            ca = Az[i];
+            ha = (int32_t)A[-1] + ca + A[1];
             cb = B[0];
             hb = B[-1] + B[0] + B[1];
         };
@@ -253,7 +261,7 @@ __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int 
 
 // The two self-guided outputs of set `set` for this lane's pixels (Q4; zero
 // for a pass of radius 0): f0 from the r = 2 pass, f1 from the r = 1 pass.
-__device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, int32_t *Ab, int32_t *Bb,
+__device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, uint16_t *Ab, int32_t *Bb,
                                             int32_t (&f0)[SGR_PPT], int32_t (&f1)[SGR_PPT])
 {
 #pragma unroll
@@ -286,7 +294,7 @@ __device__ __forceinline__ int sgr_project(int u, int32_t f0, int32_t f1, int xq
 
 template <typename Sink>
 __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd, int set, const int8_t *xqd,
-                                         int32_t *Ab, int32_t *Bb, Sink sink)
+                                         uint16_t *Ab, int32_t *Bb, Sink sink)
 {
     int xq0, xq1;
     sgr_xq(set, xqd, xq0, xq1);

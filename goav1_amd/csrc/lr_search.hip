@@ -42,10 +42,10 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     __shared__ uint8_t srcT[64 * 64];
-    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];   // Wiener intermediate, then SGR A/B
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
-    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);   // SGR runs after the Wiener re-score
     const int tid = threadIdx.x;
     const int u = blockIdx.x;
@@ -235,12 +235,12 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
 {
     __shared__ uint16_t tile[LR_TW * LR_TW];
     __shared__ uint8_t srcT[64 * 64];
-    __shared__ __align__(16) char work[LR_AB * LR_AB * 8];   // Wiener intermediate, then SGR A/B
+    __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
     __shared__ int8_t taps_s[2][3];
     __shared__ int8_t xqd_s[2];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
-    int32_t *Ab = reinterpret_cast<int32_t *>(work + LR_AB * LR_AB * 4);
+    uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
     const int tid = threadIdx.x;
     const int u = blockIdx.x;
