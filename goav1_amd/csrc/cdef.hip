@@ -61,29 +61,18 @@ __device__ __forceinline__ int adjust_strength(int strength, int32_t var)
     return (strength * (4 + i) + 8) >> 4;
 }
 
-// Cost of direction D for one 8x8 block (cdef/direction.go:37-58, :432-502):
-// the block's samples v (already (x >> cs) - 128) summed into the direction's
-// lines, then the squared line sums weighted by 840 / line length.
+// Line of sample (i, j) of an 8x8 block in direction D, and the cost of the
+// direction from its line sums: squared sums weighted by 840 / line length
+// (cdef/direction.go:37-58, :432-502).
 template <int D>
-__device__ __forceinline__ uint32_t dir_cost(const int32_t (&v)[64])
+__device__ __forceinline__ constexpr int dir_line(int i, int j)
 {
-    int32_t pa[15];
-#pragma unroll
-    for (int k = 0; k < 15; k++) pa[k] = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int32_t x = v[i * 8 + j];
-            if constexpr (D == 0) pa[i + j] += x;
-            if constexpr (D == 1) pa[i + (j >> 1)] += x;
-            if constexpr (D == 2) pa[i] += x;
-            if constexpr (D == 3) pa[3 + i - (j >> 1)] += x;
-            if constexpr (D == 4) pa[7 + i - j] += x;
-            if constexpr (D == 5) pa[3 - (i >> 1) + j] += x;
-            if constexpr (D == 6) pa[j] += x;
-            if constexpr (D == 7) pa[(i >> 1) + j] += x;
-        }
+    return D == 0 ? i + j : D == 1 ? i + (j >> 1) : D == 2 ? i : D == 3 ? 3 + i - (j >> 1)
+         : D == 4 ? 7 + i - j : D == 5 ? 3 - (i >> 1) + j : D == 6 ? j : (i >> 1) + j;
+}
+template <int D>
+__device__ __forceinline__ uint32_t dir_finish(const int32_t (&pa)[15])
+{
     uint32_t s = 0;
     if constexpr (D == 2 || D == 6) {
 #pragma unroll
@@ -103,6 +92,29 @@ __device__ __forceinline__ uint32_t dir_cost(const int32_t (&v)[64])
             s += (uint32_t)(pa[k] * pa[k] + pa[10 - k] * pa[10 - k]) * (uint32_t)kDivTable[2 * k + 2];
     }
     return s;
+}
+
+// Costs of directions D0 and D1 for the 8x8 block at img: the block is read
+// row by row (samples (x >> cs) - 128) into both directions' line sums, so
+// only a row of samples and the two sets of sums are live.
+template <int D0, int D1>
+__device__ __forceinline__ void dir_cost2(const uint16_t *img, int cs, uint32_t &c0, uint32_t &c1)
+{
+    int32_t p0[15], p1[15];
+#pragma unroll
+    for (int k = 0; k < 15; k++) { p0[k] = 0; p1[k] = 0; }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int32_t x = (int32_t)(img[i * TPITCH + j] >> cs) - 128;
+            p0[dir_line<D0>(i, j)] += x;
+            p1[dir_line<D1>(i, j)] += x;
+        }
+        __builtin_amdgcn_sched_barrier(0);   // one row's reads in flight, not the block's 64
+    }
+    c0 = dir_finish<D0>(p0);
+    c1 = dir_finish<D1>(p1);
 }
 
 // A lane's loads of up to 8 rows are in flight before its stores.
@@ -269,17 +281,12 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
             // wave w: directions w and w + 4 of the block of this lane
             if ((lm >> lane) & 1) {
                 const uint16_t *img = org + (lane >> 3) * 8 * TPITCH + (lane & 7) * 8;
-                int32_t v[64];
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-#pragma unroll
-                    for (int j = 0; j < 8; j++) v[i * 8 + j] = (int32_t)(img[i * TPITCH + j] >> cs) - 128;
                 uint32_t c0, c1;
                 switch (wv) {
-                case 0: c0 = dir_cost<0>(v); c1 = dir_cost<4>(v); break;
-                case 1: c0 = dir_cost<1>(v); c1 = dir_cost<5>(v); break;
-                case 2: c0 = dir_cost<2>(v); c1 = dir_cost<6>(v); break;
-                default: c0 = dir_cost<3>(v); c1 = dir_cost<7>(v); break;
+                case 0: dir_cost2<0, 4>(img, cs, c0, c1); break;
+                case 1: dir_cost2<1, 5>(img, cs, c0, c1); break;
+                case 2: dir_cost2<2, 6>(img, cs, c0, c1); break;
+                default: dir_cost2<3, 7>(img, cs, c0, c1); break;
                 }
                 dcost[lane][wv] = c0;
                 dcost[lane][wv + 4] = c1;
