@@ -105,30 +105,30 @@ __device__ __forceinline__ int lpf_vals(int *a, int *w, int bd, int E, int I, in
 // for all 4 lines, unconditionally and in address order (the 4 lines of a
 // sample position first when the lines are adjacent, else a line's samples
 // first), so neighbouring samples merge into wide stores.
-template <int LO, bool ACROSS, typename T>
+template <int LO, bool ACROSS, int NL, typename T>
 __device__ __forceinline__ void lpf_store(gptr<T> p, int64_t lstep, int64_t st, const int (*a)[8], const int (*w)[6])
 {
     auto put = [&](int i, int k, int v) { p[i * lstep + k * st] = (T)v; };
     if constexpr (ACROSS) {
         if constexpr (LO == 0) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) put(i, -6, w[i][1]);
+            for (int i = 0; i < NL; i++) put(i, -6, w[i][1]);
 #pragma unroll
-            for (int i = 0; i < 4; i++) put(i, -5, w[i][2]);
+            for (int i = 0; i < NL; i++) put(i, -5, w[i][2]);
         }
 #pragma unroll
         for (int j = LO; j <= 7 - LO; j++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) put(i, j - 4, a[i][j]);
+            for (int i = 0; i < NL; i++) put(i, j - 4, a[i][j]);
         if constexpr (LO == 0) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) put(i, 4, w[i][3]);
+            for (int i = 0; i < NL; i++) put(i, 4, w[i][3]);
 #pragma unroll
-            for (int i = 0; i < 4; i++) put(i, 5, w[i][4]);
+            for (int i = 0; i < NL; i++) put(i, 5, w[i][4]);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < NL; i++) {
             if constexpr (LO == 0) {
                 put(i, -6, w[i][1]);
                 put(i, -5, w[i][2]);
@@ -143,24 +143,23 @@ __device__ __forceinline__ void lpf_store(gptr<T> p, int64_t lstep, int64_t st, 
     }
 }
 
-// The 4 lines of one edge unit: q0 of line i at p + i * lstep, samples across
+// NL lines of one edge unit: q0 of line i at p + i * lstep, samples across
 // the edge st apart (elements).  Every line's p3..q3 is loaded before any is
 // used, and p6..p4 / q4..q6 of every line together when the edge is 16 wide.
-// The unit writes back, for all 4 lines, the widest span any of its lines
-// changed: a line that changed less rewrites values it read, and no other edge
+// The lines write back the widest span any of them changed: a line that changed less rewrites values it read, and no other edge
 // of the same direction writes a sample this edge reads (transform spacing), so
 // the rewrite is safe in parallel.
-template <bool ACROSS, typename T>
-__device__ __forceinline__ void lpf_unit(gptr<T> p, int64_t lstep, int64_t st, int bd, int E, int I, int H, int wd)
+template <bool ACROSS, int NL, typename T>
+__device__ __forceinline__ void lpf_lines(gptr<T> p, int64_t lstep, int64_t st, int bd, int E, int I, int H, int wd)
 {
-    int a[4][8], w[4][6];
+    int a[NL][8], w[NL][6];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NL; i++)
 #pragma unroll
         for (int j = 0; j < 8; j++) a[i][j] = p[i * lstep + (j - 4) * st];
     if (wd >= 16) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NL; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 w[i][j] = p[i * lstep + (j - 7) * st];
@@ -169,11 +168,24 @@ __device__ __forceinline__ void lpf_unit(gptr<T> p, int64_t lstep, int64_t st, i
     }
     int n = 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) n = max(n, lpf_vals(a[i], w[i], bd, E, I, H, wd));
-    if (n == 12) lpf_store<0, ACROSS, T>(p, lstep, st, a, w);
-    else if (n == 6) lpf_store<1, ACROSS, T>(p, lstep, st, a, w);
-    else if (n == 4) lpf_store<2, ACROSS, T>(p, lstep, st, a, w);
-    else if (n == 2) lpf_store<3, ACROSS, T>(p, lstep, st, a, w);
+    for (int i = 0; i < NL; i++) n = max(n, lpf_vals(a[i], w[i], bd, E, I, H, wd));
+    if (n == 12) lpf_store<0, ACROSS, NL, T>(p, lstep, st, a, w);
+    else if (n == 6) lpf_store<1, ACROSS, NL, T>(p, lstep, st, a, w);
+    else if (n == 4) lpf_store<2, ACROSS, NL, T>(p, lstep, st, a, w);
+    else if (n == 2) lpf_store<3, ACROSS, NL, T>(p, lstep, st, a, w);
+}
+
+// A 16-wide unit reads 14 samples a line: its lines go in two pairs, so the
+// samples in flight stay within the registers of the narrower widths.
+template <bool ACROSS, typename T>
+__device__ __forceinline__ void lpf_unit(gptr<T> p, int64_t lstep, int64_t st, int bd, int E, int I, int H, int wd)
+{
+    if (wd >= 16) {
+        lpf_lines<ACROSS, 2, T>(p, lstep, st, bd, E, I, H, wd);
+        lpf_lines<ACROSS, 2, T>(p + 2 * lstep, lstep, st, bd, E, I, H, wd);
+    } else {
+        lpf_lines<ACROSS, 4, T>(p, lstep, st, bd, E, I, H, wd);
+    }
 }
 
 // Per-plane constants of one deblock segment.
