@@ -349,6 +349,24 @@ def test_scaling_lut(L):
 
 
 # ------------------------------------------------------------------ loop restoration
+def test_x_by_xplus1_float_form(L):
+    """The LR kernels form x/(x+1) as floor((513 z + 1) * rcp(2 z + 2)) in
+    float.  For 1 <= z <= 254 the quotient lies at least 1/510 from an integer
+    (2z + 2 never divides 511 - z), far above the product's error (relative
+    2^-22 with a 1-ulp reciprocal), so the floor is the table value: checked
+    exactly, and with the product perturbed by +-2^-22."""
+    from fractions import Fraction
+    for z in range(1, 255):
+        q = Fraction(513 * z + 1, 2 * z + 2)
+        fl = q.numerator // q.denominator
+        assert L.orc_x_by_xplus1(z) == fl
+        frac = q - fl
+        assert min(frac, 1 - frac) >= Fraction(1, 510), z
+        for eps in (-2.0 ** -22, 2.0 ** -22):
+            assert int(float(513 * z + 1) * (1.0 / (2 * z + 2)) * (1.0 + eps)) == fl, (z, eps)
+    assert L.orc_x_by_xplus1(0) == 1 and L.orc_x_by_xplus1(255) == 256
+
+
 def test_sgr_tables_and_xq(L):
     k = KAT["sgr_tables"]
     for z, want in k["x_by_xplus1"].items():
