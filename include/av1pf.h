@@ -96,7 +96,7 @@ typedef struct av1pf_deblock_params {
     int32_t  mi_cols, mi_rows;/* ((W+7)>>3)<<1, ((H+7)>>3)<<1 */
     int32_t  sb128_w, sb128_h;/* (mi_cols+31)>>5, (mi_rows+31)>>5 */
     const uint8_t       *level_cache; /* 4*mi_cols*mi_rows bytes */
-    const av1pf_lf_mask *masks;       /* sb128_w*sb128_h records */
+    const av1pf_lf_mask *masks;       /* sb128_w*sb128_h records, 4-byte aligned (read as 32-bit words) */
 } av1pf_deblock_params;
 
 /* ---- CDEF side data (parser/cdef.go:14-21, threading/cdef_index.go:13-18) */
