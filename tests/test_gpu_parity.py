@@ -317,6 +317,21 @@ def test_film_grain_variants(runner, case):
     assert_same(gdisp, wdisp, f"display {case}")
 
 
+def test_deblock_misaligned_masks_rejected(runner):
+    """The deblock kernels read the mask records as 32-bit words: a mask array
+    that is not 4-byte aligned is refused by the host before any launch."""
+    from goav1_amd.lib import Av1pfError, DeviceFrame, DeviceSideData
+    rng = np.random.default_rng(4)
+    fr = synth.FrameData(128, 96, 8, 1, 1)
+    synth.fill_blocky(fr, rng)
+    synth.build_side_data(fr, rng, lf_random=True)
+    fr.stages = abi.STAGE_DEBLOCK
+    side = DeviceSideData(fr)
+    side.req.deblock.contents.masks += 2
+    with pytest.raises(Av1pfError):
+        runner.apply(DeviceFrame(fr), side)
+
+
 def test_lr_chroma_unit_below_stripe_rejected(runner):
     """Chroma LR units halve only for 4:2:0: a 32-sample unit on a 4:2:2 plane
     (64-row stripes) is not a legal request and is refused before launch."""
