@@ -30,6 +30,7 @@ namespace pf {
 #define CDEF_VL 0x4000
 #define TPITCH 82   // 64 + 2*8, plus 2 to spread the block rows over the banks
 #define TROWS 68    // 64 + 2*2
+#define TSHIFT (TROWS * TPITCH + 1)   // odd offset of the tile's second copy (samples)
 
 __device__ __constant__ int kDivTable[9] = { 0, 840, 420, 280, 210, 168, 140, 120, 105 };
 
@@ -163,7 +164,11 @@ __device__ __forceinline__ void load_tile(uint16_t *tile, int pitch, const Plane
                 const int r = r0 + 4 * j;
                 const int y = uy - 2 + r;
                 const bool vy = y >= 0 && (y < sy1 || row_ext);
-                if (r < rows) tile[r * pitch + c] = (vy && vx) ? v[j] : (uint16_t)CDEF_VL;
+                if (r < rows) {
+                    const uint16_t val = (vy && vx) ? v[j] : (uint16_t)CDEF_VL;
+                    tile[r * pitch + c] = val;
+                    tile[TSHIFT + r * pitch + c] = val;   // the copy at an odd offset
+                }
             }
         }
     }
@@ -179,16 +184,7 @@ __device__ __forceinline__ v2s splat2(int v) { return (v2s){ (short)v, (short)v 
 __device__ __forceinline__ v2u splatu2(int v) { return (v2u){ (unsigned short)v, (unsigned short)v }; }
 __device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
-// Two adjacent tile samples, read as two 16-bit LDS loads (the second into
-// the high half): the pair is 4-byte aligned only for even offsets, and a
-// merged misaligned 32-bit LDS read stalls.  The empty asm hides the second
-// address from the load merger.
-__device__ __forceinline__ v2s ld2(const uint16_t *p)
-{
-    const uint16_t *q = p + 1;
-    asm volatile("" : "+v"(q));
-    return (v2s){ (short)p[0], (short)*q };
-}
+__device__ __forceinline__ v2s as2(uint32_t v) { return __builtin_bit_cast(v2s, v); }
 // constrain(d, t, sh) = sign(d) min(|d|, max(0, t - (|d| >> sh))) of two
 // samples, written as the clamp of d to [-c, c] with c = max(0, t - (|d| >> sh)).
 __device__ __forceinline__ v2s cst2(v2s a, v2s x, v2u t, v2u sh)
@@ -199,10 +195,16 @@ __device__ __forceinline__ v2s cst2(v2s a, v2s x, v2u t, v2u sh)
     return vmin(vmax(d, -c), c);
 }
 
-// Per-block filter record: primary tap offsets, secondary tap offsets along
-// dir + 2 and dir - 2 (int16 pairs), then strength, primary shift, secondary
-// shift and the filter flag (1: filter, 2: clip to the taps' range).
-struct __align__(16) BlockRec { int o, sa, sb, p; };
+// Per-block filter record: for the 12 taps (+-primary 0, +-primary 1, +-dir+2
+// secondary 0, +-dir-2 secondary 0, +-dir+2 secondary 1, +-dir-2 secondary 1)
+// the 32-bit-word offset of the tap pair from the pair at the filtered
+// samples, then strength, primary shift, secondary shift and the filter flag
+// (1: filter, 2: clip to the taps' range).  The filtered pair starts at an even
+// sample; a tap at an even sample offset o reads the tile at o, one at an odd
+// offset reads the copy TSHIFT (odd) samples further on, so both are 4-byte
+// aligned single reads.
+struct __align__(16) BlockRec { int16_t e[12]; int p; int pad[1]; };
+__device__ __forceinline__ int tap_word(int o) { return ((o & 1) ? TSHIFT + o : o) >> 1; }
 
 template <typename T>
 __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView &dst, const CdefDesc &a, int *err,
@@ -325,11 +327,15 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
             const int ssh = sec ? max(damping - ilog2_((unsigned)sec), 0) : 0;
             const int flag = (listed && (st != 0 || sec != 0) ? 1 : 0) | (st && sec ? 2 : 0);
             BlockRec r;
-            auto pack = [](int lo, int hi) { return (int)(((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16)); };
-            r.o = pack(kDirOff[fd + 2][0], kDirOff[fd + 2][1]);
-            r.sa = pack(kDirOff[fd + 4][0], kDirOff[fd + 4][1]);
-            r.sb = pack(kDirOff[fd][0], kDirOff[fd][1]);
+            const int tap[6] = { kDirOff[fd + 2][0], kDirOff[fd + 2][1], kDirOff[fd + 4][0], kDirOff[fd][0],
+                                 kDirOff[fd + 4][1], kDirOff[fd][1] };
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                r.e[2 * k] = (int16_t)tap_word(tap[k]);
+                r.e[2 * k + 1] = (int16_t)tap_word(-tap[k]);
+            }
             r.p = min(st, 255) | (psh << 8) | (ssh << 16) | (flag << 24);
+            r.pad[0] = 0;
             brec[tid] = r;
         }
         __syncthreads();
@@ -348,18 +354,15 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         if (x < uw) {
             for (int y = tid / tpr; y < uh; y += 256 / tpr) {
                 const BlockRec r = brec[((y >> bhl) << 3) + (x >> bwl)];
-                const uint16_t *c = org + y * TPITCH + x;
-                const v2s xv = ld2(c);
+                const uint32_t *W = reinterpret_cast<const uint32_t *>(tile) + ((org - tile + y * TPITCH + x) >> 1);
+                const v2s xv = as2(W[0]);
                 v2s out = xv;
                 const int flag = r.p >> 24;
                 if (flag & 1) {
                     const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
-                    const int o0 = (int16_t)(r.o & 0xffff), o1 = r.o >> 16;
-                    const int s0a = (int16_t)(r.sa & 0xffff), s0b = r.sa >> 16;
-                    const int s1a = (int16_t)(r.sb & 0xffff), s1b = r.sb >> 16;
-                    const v2s a0 = ld2(c + o0), a1 = ld2(c - o0), a2 = ld2(c + o1), a3 = ld2(c - o1);
-                    const v2s b0 = ld2(c + s0a), b1 = ld2(c - s0a), b2 = ld2(c + s1a), b3 = ld2(c - s1a);
-                    const v2s b4 = ld2(c + s0b), b5 = ld2(c - s0b), b6 = ld2(c + s1b), b7 = ld2(c - s1b);
+                    const v2s a0 = as2(W[r.e[0]]), a1 = as2(W[r.e[1]]), a2 = as2(W[r.e[2]]), a3 = as2(W[r.e[3]]);
+                    const v2s b0 = as2(W[r.e[4]]), b1 = as2(W[r.e[5]]), b2 = as2(W[r.e[6]]), b3 = as2(W[r.e[7]]);
+                    const v2s b4 = as2(W[r.e[8]]), b5 = as2(W[r.e[9]]), b6 = as2(W[r.e[10]]), b7 = as2(W[r.e[11]]);
                     v2s sum = splat2(0);
                     if (pri) {   // st may be 0 for a flat block: its primary terms vanish
                         const v2u t = splatu2(st), sh = splatu2(psh);
@@ -391,10 +394,11 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
     }
 }
 
-__global__ __launch_bounds__(256) void cdef_kernel(const FrameDesc *__restrict__ descs,
+// 6 workgroups per CU: 26.7 KB of LDS each, and at most 80 VGPRs
+__global__ __launch_bounds__(256, 6) void cdef_kernel(const FrameDesc *__restrict__ descs,
                                                    const Seg *__restrict__ segs, int nseg, int total, StageLaunch L)
 {
-    __shared__ uint16_t tile[TROWS * TPITCH];
+    __shared__ __align__(16) uint16_t tile[TSHIFT + TROWS * TPITCH + 1];   // the tile, then its copy at TSHIFT
     __shared__ uint8_t bdir[64];
     __shared__ int32_t bvar[64];
     __shared__ uint32_t dcost[64][8];   // direction costs per block
