@@ -88,63 +88,67 @@ __device__ void ar_rows(int16_t *T, const int16_t *L, int w, int y0, int nrows, 
     const int lane = threadIdx.x;
     const int y = y0 + lane;
     const bool live = lane < nrows;
+    // The windows are circular: steps go in blocks of WIN, and at phase u of a
+    // block (t = t0 + u) column x - LAG + j of row y - 1 - d sits in
+    // A[d][(u + j) % WIN] and this row's result of step t' in O[phase of t'],
+    // so a step only writes its new column over the oldest one (all indices
+    // compile-time constants: registers, no shifting moves).
     int A[LAG][WIN];
-    int own[LAG];
+    int O[WIN];
 #pragma unroll
-    for (int d = 0; d < LAG; d++) {
+    for (int j = 0; j < WIN; j++) {
+        O[j] = 0;
 #pragma unroll
-        for (int j = 0; j < WIN; j++) A[d][j] = 0;
-        own[d] = 0;
+        for (int d = 0; d < LAG; d++) A[d][j] = 0;
     }
     const int t_end = (w - 4) + K * (nrows - 1);
-    for (int t = -K; t <= t_end; t++) {
-        const int x = t - K * lane;
-        int v = (live && x >= 0 && x < w) ? T[y * GW + x] : 0;
-        if (live && x >= 3 && x < w - 3) {
-            int sum = 0, pos = 0;
+    for (int t0 = -K; t0 <= t_end; t0 += WIN) {
 #pragma unroll
-            for (int dr = LAG; dr >= 1; dr--) {
+        for (int u = 0; u < WIN; u++) {
+            const int t = t0 + u;
+            if (t > t_end) break;
+            const int x = t - K * lane;
+            int v = (live && x >= 0 && x < w) ? T[y * GW + x] : 0;
+            if (live && x >= 3 && x < w - 3) {
+                int sum = 0, pos = 0;
 #pragma unroll
-                for (int j = 0; j < WIN; j++) sum += __mul24(c[pos++], A[dr - 1][j]);
-            }
+                for (int dr = LAG; dr >= 1; dr--) {
 #pragma unroll
-            for (int j = 0; j < LAG; j++) sum += __mul24(c[pos++], own[j]);
-            if (CHROMA && luma_term) {
-                const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
-                int a = L[ly * GW + lx];
-                if (ssx) a += L[ly * GW + lx + 1];
-                if (ssy) {
-                    a += L[(ly + 1) * GW + lx];
-                    if (ssx) a += L[(ly + 1) * GW + lx + 1];
+                    for (int j = 0; j < WIN; j++) sum += __mul24(c[pos++], A[dr - 1][(u + j) % WIN]);
                 }
-                sum += r2i(a, ssx + ssy) * c[pos];
+#pragma unroll
+                for (int j = 0; j < LAG; j++) sum += __mul24(c[pos++], O[(u - LAG + j + WIN) % WIN]);
+                if (CHROMA && luma_term) {
+                    const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
+                    int a = L[ly * GW + lx];
+                    if (ssx) a += L[ly * GW + lx + 1];
+                    if (ssy) {
+                        a += L[(ly + 1) * GW + lx];
+                        if (ssx) a += L[(ly + 1) * GW + lx + 1];
+                    }
+                    sum += r2i(a, ssx + ssy) * c[pos];
+                }
+                v = clipi(v + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
+                T[y * GW + x] = (int16_t)v;
             }
-            v = clipi(v + ((sum + (1 << (shift - 1))) >> shift), gmin, gmax);
-            T[y * GW + x] = (int16_t)v;
+            // right-hand column (x + 1 + LAG) of the windows for the next step:
+            // the lane above's newest result and the centres of its windows
+            int nv = from_prev_lane(v);
+            int na[LAG > 1 ? LAG - 1 : 1];
+#pragma unroll
+            for (int d = 0; d + 1 < LAG; d++) na[d] = from_prev_lane(A[d][(u + LAG) % WIN]);
+            if (lane == 0) {
+                const int xx = x + 1 + LAG;
+                const bool ok = xx >= 0 && xx < w;
+                nv = ok ? T[(y - 1) * GW + xx] : 0;
+#pragma unroll
+                for (int d = 0; d + 1 < LAG; d++) na[d] = ok ? T[(y - 2 - d) * GW + xx] : 0;
+            }
+            A[0][u] = nv;
+#pragma unroll
+            for (int d = 1; d < LAG; d++) A[d][u] = na[d - 1];
+            O[u] = v;
         }
-        // right-hand column (x + 1 + LAG) of the windows for the next step
-        int nv = from_prev_lane(v);
-        int na[LAG > 1 ? LAG - 1 : 1];
-#pragma unroll
-        for (int d = 0; d + 1 < LAG; d++) na[d] = from_prev_lane(A[d][LAG]);
-        if (lane == 0) {
-            const int xx = x + 1 + LAG;
-            const bool ok = xx >= 0 && xx < w;
-            nv = ok ? T[(y - 1) * GW + xx] : 0;
-#pragma unroll
-            for (int d = 0; d + 1 < LAG; d++) na[d] = ok ? T[(y - 2 - d) * GW + xx] : 0;
-        }
-#pragma unroll
-        for (int d = 0; d < LAG; d++) {
-#pragma unroll
-            for (int j = 0; j + 1 < WIN; j++) A[d][j] = A[d][j + 1];
-        }
-        if (LAG > 0) A[0][WIN - 1] = nv;
-#pragma unroll
-        for (int d = 1; d < LAG; d++) A[d][WIN - 1] = na[d - 1];
-#pragma unroll
-        for (int j = 0; j + 1 < LAG; j++) own[j] = own[j + 1];
-        if (LAG > 0) own[LAG - 1] = v;
     }
 }
 
