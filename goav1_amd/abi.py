@@ -29,6 +29,8 @@ STAGE_LR = 0x08
 STAGE_FILMGRAIN = 0x10
 STAGES_ALL = STAGE_DEBLOCK | STAGE_CDEF | STAGE_LR | STAGE_FILMGRAIN
 
+REQ_REFRESH = 0x01
+
 LR_NONE, LR_WIENER, LR_SGRPROJ, LR_SWITCHABLE = 0, 1, 2, 3
 
 
@@ -36,7 +38,7 @@ class Frame(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("bit_depth", C.c_int32),
         ("ss_x", C.c_int32), ("ss_y", C.c_int32), ("mono", C.c_int32),
-        ("matrix_identity", C.c_int32), ("reserved", C.c_int32),
+        ("matrix_identity", C.c_int32), ("alloc_height", C.c_int32),
         ("plane", C.c_void_p * 3), ("stride", C.c_int64 * 3),
     ]
 
@@ -107,7 +109,7 @@ LF_RECORD_BYTES = 20
 
 class Request(C.Structure):
     _fields_ = [
-        ("stages", C.c_uint32), ("pad", C.c_uint32),
+        ("stages", C.c_uint32), ("flags", C.c_uint32),
         ("deblock", C.POINTER(DeblockParams)), ("cdef", C.POINTER(CdefParams)),
         ("lr", C.POINTER(LRParams)), ("grain", C.POINTER(GrainParams)),
         ("display", C.POINTER(Frame)), ("upscaled", C.POINTER(Frame)),

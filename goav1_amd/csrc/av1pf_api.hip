@@ -30,6 +30,7 @@
 
 using namespace pf;
 
+
 namespace {
 constexpr int kSlots = 4;
 constexpr size_t kSlotBytes = size_t(1) << 20;   // ~400 frames of descriptors
@@ -127,7 +128,7 @@ namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
        ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH, ST_SUPERRES,
-       ST_DEBLOCK_BUILD };
+       ST_DEBLOCK_BUILD, ST_EXTEND };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -202,6 +203,8 @@ int validate_frame(av1pf_ctx *c, const av1pf_frame *f)
         return fail(c, AV1PF_ERR_INVALID_ARG, "unsupported chroma subsampling");
     const int np = f->mono ? 1 : 3;
     const int aw = (f->width + 7) & ~7;
+    if (f->alloc_height != 0 && (f->alloc_height < ((f->height + 7) & ~7) || f->alloc_height > 16384 + 128))
+        return fail(c, AV1PF_ERR_INVALID_ARG, "allocated height below the MI-aligned height or out of range");
     for (int i = 0; i < np; i++) {
         const int sx = i ? f->ss_x : 0;
         if (!f->plane[i]) return fail(c, AV1PF_ERR_INVALID_ARG, "null plane");
@@ -229,6 +232,8 @@ FrameView view_of(const av1pf_frame *f)
         v.p[i].h = (f->height + sy) >> sy;
         v.p[i].aw = aw >> sx;
         v.p[i].ah = ah >> sy;
+        v.p[i].cw = (int)(f->stride[i] / bps_of(f->bit_depth));
+        v.p[i].ch = (f->alloc_height ? f->alloc_height : ah) >> sy;
     }
     return v;
 }
@@ -411,6 +416,7 @@ int validate_request(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request
     const uint32_t st = req->stages;
     if (st & ~(AV1PF_STAGE_DEBLOCK | AV1PF_STAGE_CDEF | AV1PF_STAGE_SUPERRES | AV1PF_STAGE_LR | AV1PF_STAGE_FILMGRAIN))
         return fail(c, AV1PF_ERR_INVALID_ARG, "unknown stage bits");
+    if (req->flags & ~AV1PF_REQ_REFRESH) return fail(c, AV1PF_ERR_INVALID_ARG, "unknown request flags");
     const av1pf_frame *post = frame;
     if (st & AV1PF_STAGE_SUPERRES) {
         if ((rc = validate_superres(c, frame, req->upscaled))) return rc;
@@ -457,8 +463,8 @@ struct SegList {
 
 struct BatchPlan {
     std::vector<FrameDesc> descs;
-    SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain, sr;
-    double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0, bytes_sr = 0;
+    SegList dbv, dbh, save, cdef, lr, copy_ba, copy_ad, tl, tc, grain, sr, ext;
+    double bytes_db = 0, bytes_cdef = 0, bytes_lr = 0, bytes_copy = 0, bytes_grain = 0, bytes_sr = 0, bytes_ext = 0;
     size_t scratch = 0, bnd = 0, goffs = 0;
     int ngrain = 0;
 };
@@ -527,6 +533,7 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 const int sx = pl ? A.ssx : 0, sy = pl ? A.ssy : 0;
                 g.type = p->frame_type[pl];
                 g.sr = sr;
+                g.copy_pad = !sr;
                 g.pw = U.p[pl].w;
                 g.ph = U.p[pl].h;
                 g.procw = 64 >> sx;
@@ -614,6 +621,19 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             for (int pl = 0; pl < A.nplanes; pl++) P.copy_ba.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
             P.bytes_copy += 2 * view_bytes(A);
         }
+        // ---- border extension of a refreshing frame (postfilter.go:760-771):
+        // the frame LR and film grain see, once it holds the filtered picture
+        D.ext = D.fin;
+        if (r->flags & AV1PF_REQ_REFRESH) {
+            const FrameView &E = D.v[D.ext];
+            for (int pl = 0; pl < E.nplanes; pl++) {
+                const PlaneView &pv = E.p[pl];
+                const int padw = pv.cw - pv.w, padh = pv.ch - pv.h;
+                if (padw > 0) P.ext.add(i, pl, (padw + EXT_COLS - 1) / EXT_COLS, (pv.h + EXT_ROWS - 1) / EXT_ROWS);
+                if (padh > 0) P.ext.add(i, 3 + pl, (pv.cw + EXT_COLS - 1) / EXT_COLS, (padh + EXT_ROWS - 1) / EXT_ROWS);
+                P.bytes_ext += ((double)std::max(padw, 0) * pv.h + (double)std::max(padh, 0) * pv.cw) * 2 * bps_of(E.bd);
+            }
+        }
         // ---- film grain
         if (st & AV1PF_STAGE_FILMGRAIN) {
             const av1pf_grain_params *g = r->grain;
@@ -694,8 +714,17 @@ int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// A caller-provided device region that replaces every arena of one call:
+// descriptor image, second frame buffers,
+// LR context rows, film-grain working memory.
+struct FixedArena {
+    char *base;
+    size_t cap;
+};
+
 int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, hipStream_t s,
-              bool skip_lr_save = false, bool tune_modes = false, StageModes *tuned_out = nullptr)
+              bool skip_lr_save = false, bool tune_modes = false, StageModes *tuned_out = nullptr,
+              const FixedArena *fixed = nullptr)
 {
     StageModes chosen;
     if (n == 0) return AV1PF_OK;
@@ -704,43 +733,8 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     std::vector<int> grain_idx;
     int rc = plan_batch(c, n, frames, reqs, P, scr_off, bnd_off, grain_idx, goff, skip_lr_save, &sscr_off);
     if (rc) return rc;
-    if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
-    if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
-    const size_t tsz = GR_FRAME_BYTES;
-    const bool side = P.tl.total > 0;
-    const int par = c->tpar;
-    if (side) {
-        c->tpar ^= 1;
-        if ((rc = ensure(c, (void **)&c->templates[par], &c->templates_cap[par], std::max<size_t>(P.ngrain * tsz, 256))))
-            return rc;
-        if ((rc = ensure(c, (void **)&c->grain_offs[par], &c->grain_offs_cap[par], std::max<size_t>(P.goffs, 256))))
-            return rc;
-    }
-    for (int i = 0; i < n; i++) {
-        FrameDesc &D = P.descs[i];
-        char *base = c->scratch + scr_off[i];
-        for (int p = 0; p < D.v[VIEW_B].nplanes; p++) {
-            D.v[VIEW_B].p[p].base = base;
-            base += (size_t)D.v[VIEW_B].p[p].stride * D.v[VIEW_B].p[p].ah;
-        }
-        if (sscr_off[i] != SIZE_MAX) {
-            base = c->scratch + sscr_off[i];
-            for (int p = 0; p < D.v[VIEW_S].nplanes; p++) {
-                D.v[VIEW_S].p[p].base = base;
-                base += (size_t)D.v[VIEW_S].p[p].stride * D.v[VIEW_S].p[p].ah;
-            }
-        }
-        for (int p = 0; p < 3; p++)
-            if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = c->lr_bnd + bnd_off[i * 3 + p];
-        if (grain_idx[i] >= 0) {
-            char *gw = reinterpret_cast<char *>(c->templates[par]) + (size_t)grain_idx[i] * GR_FRAME_BYTES;
-            D.gr.templates = reinterpret_cast<int16_t *>(gw);
-            D.gr.scale = reinterpret_cast<uint8_t *>(gw + GR_TMPL_BYTES);
-            D.gr.offsets = c->grain_offs[par] + goff[i];
-        }
-    }
-    // ---- descriptors + segment tables + stage queues, one host image
-    SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain, &P.sr };
+    SegList *lists[] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.lr, &P.copy_ba, &P.copy_ad, &P.tl, &P.tc, &P.grain, &P.sr,
+                         &P.ext };
     const int nl = sizeof(lists) / sizeof(lists[0]);
     size_t off = align256(sizeof(FrameDesc) * n);
     size_t offs[nl];
@@ -750,6 +744,72 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     }
     const size_t qoff = off;   // work-queue counters of the tunable stages (one per XCD), zero at upload
     off = align256(off + kTunable * kXcds * sizeof(unsigned int));
+
+    const size_t tsz = GR_FRAME_BYTES;
+    const bool side = P.tl.total > 0;
+    const int par = c->tpar;
+    char *scratch_base, *fixed_meta = nullptr;
+    uint16_t *bnd_base;
+    int16_t *tmpl_base = nullptr;
+    uint8_t *goffs_base = nullptr;
+    if (fixed) {
+        // every arena carved from the caller's region: nothing is allocated
+        size_t at = off;   // the descriptor image comes first
+        const size_t scr_at = align256(at);
+        at = scr_at + align256(P.scratch);
+        const size_t bnd_at = at;
+        at += align256(P.bnd * 2);
+        const size_t tmpl_at = at;
+        at += align256(P.ngrain * tsz);
+        const size_t goff_at = at;
+        at += align256(P.goffs);
+        if (at > fixed->cap || off > c->meta_cap[c->slot])
+            return fail(c, AV1PF_ERR_SHORT_BUFFER, "request needs more scratch than the allocation-free entry point holds");
+        fixed_meta = fixed->base;
+        scratch_base = fixed->base + scr_at;
+        bnd_base = reinterpret_cast<uint16_t *>(fixed->base + bnd_at);
+        tmpl_base = reinterpret_cast<int16_t *>(fixed->base + tmpl_at);
+        goffs_base = reinterpret_cast<uint8_t *>(fixed->base + goff_at);
+    } else {
+        if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
+        if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
+        if (side) {
+            c->tpar ^= 1;
+            if ((rc = ensure(c, (void **)&c->templates[par], &c->templates_cap[par],
+                             std::max<size_t>(P.ngrain * tsz, 256))))
+                return rc;
+            if ((rc = ensure(c, (void **)&c->grain_offs[par], &c->grain_offs_cap[par], std::max<size_t>(P.goffs, 256))))
+                return rc;
+            tmpl_base = c->templates[par];
+            goffs_base = c->grain_offs[par];
+        }
+        scratch_base = c->scratch;
+        bnd_base = c->lr_bnd;
+    }
+    for (int i = 0; i < n; i++) {
+        FrameDesc &D = P.descs[i];
+        char *base = scratch_base + scr_off[i];
+        for (int p = 0; p < D.v[VIEW_B].nplanes; p++) {
+            D.v[VIEW_B].p[p].base = base;
+            base += (size_t)D.v[VIEW_B].p[p].stride * D.v[VIEW_B].p[p].ah;
+        }
+        if (sscr_off[i] != SIZE_MAX) {
+            base = scratch_base + sscr_off[i];
+            for (int p = 0; p < D.v[VIEW_S].nplanes; p++) {
+                D.v[VIEW_S].p[p].base = base;
+                base += (size_t)D.v[VIEW_S].p[p].stride * D.v[VIEW_S].p[p].ah;
+            }
+        }
+        for (int p = 0; p < 3; p++)
+            if (D.lr[p].type != AV1PF_LR_NONE) D.lr[p].bnd = bnd_base + bnd_off[i * 3 + p];
+        if (grain_idx[i] >= 0) {
+            char *gw = reinterpret_cast<char *>(tmpl_base) + (size_t)grain_idx[i] * GR_FRAME_BYTES;
+            D.gr.templates = reinterpret_cast<int16_t *>(gw);
+            D.gr.scale = reinterpret_cast<uint8_t *>(gw + GR_TMPL_BYTES);
+            D.gr.offsets = goffs_base + goff[i];
+        }
+    }
+    // ---- descriptors + segment tables + stage queues, one host image
     auto fill = [&](char *h) {
         memcpy(h, P.descs.data(), sizeof(FrameDesc) * n);
         for (int k = 0; k < nl; k++) {
@@ -811,7 +871,19 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             return AV1PF_OK;
         };
         hipEvent_t ev_up = capture ? c->g_up : c->uploaded, ev_join = capture ? c->g_join : c->join;
-        if (side) {
+        if (side && fixed) {
+            // one template region shared by every call: the upload and the
+            // templates follow the stream's previous work (the last call's
+            // grain apply included)
+            HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
+            HIPCHK(c, hipEventRecord(ev_up, st));
+            HIPCHK(c, hipStreamWaitEvent(c->side, ev_up, 0));
+            {
+                StageTimer t(c, ST_GRAIN_T, c->side, 0);
+                HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
+            }
+            HIPCHK(c, hipEventRecord(ev_join, c->side));
+        } else if (side) {
             // Film-grain templates depend only on the grain parameters: the side
             // stream uploads the descriptors and builds them without waiting for
             // the previous call's filtering, only for the grain apply that last
@@ -862,9 +934,16 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             if ((rc2 = tune(1, [&](const StageLaunch &L) { return launch_lr(dd, segp(4), nseg(4), P.lr.total, L, st); })))
                 return rc2;
         }
-        if (P.copy_ba.total || P.copy_ad.total) {
+        if (P.copy_ba.total) {
             StageTimer t(c, ST_COPY, st, P.bytes_copy);
             HIPCHK(c, launch_copy(dd, segp(5), nseg(5), P.copy_ba.total, VIEW_B, VIEW_A, st));
+        }
+        if (P.ext.total) {
+            StageTimer t(c, ST_EXTEND, st, P.bytes_ext);
+            HIPCHK(c, launch_extend(dd, segp(11), nseg(11), P.ext.total, st));
+        }
+        if (P.copy_ad.total) {
+            StageTimer t(c, ST_COPY, st, 0);
             HIPCHK(c, launch_copy(dd, segp(6), nseg(6), P.copy_ad.total, -1, VIEW_D, st));
         }
         if (side) HIPCHK(c, hipStreamWaitEvent(st, ev_join, 0));
@@ -881,7 +960,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         return AV1PF_OK;
     };
 
-    if (c->graphs && !c->profiling && !tune_modes) {
+    if (c->graphs && !c->profiling && !tune_modes && !fixed) {
         // one graph per shape class, two instances used alternately so that the
         // host image of one can be written while the other replays
         GraphClass &G = c->gclass[key];
@@ -909,6 +988,12 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         HIPCHK(c, hipGraphLaunch(I.exec, s));
         HIPCHK(c, hipEventRecord(I.done, s));
         I.used = true;
+        if (side) {
+            // the replay's grain apply read template pair par: a later call
+            // outside the graph waits for it before rewriting the pair
+            HIPCHK(c, hipEventRecord(c->tmpl_free[par], s));
+            c->tmpl_free_used[par] = true;
+        }
         return AV1PF_OK;
     }
 
@@ -925,8 +1010,8 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     }
     fill(c->meta_host[slot]);
     chosen = modes;
-    if ((rc = enqueue(c->meta_host[slot], c->meta_dev[slot], s, false, modes))) return rc;
-    if (side) {
+    if ((rc = enqueue(c->meta_host[slot], fixed ? fixed_meta : c->meta_dev[slot], s, false, modes))) return rc;
+    if (side && !fixed) {
         HIPCHK(c, hipEventRecord(c->tmpl_free[par], s));
         c->tmpl_free_used[par] = true;
     }
@@ -985,6 +1070,7 @@ int av1pf_create(av1pf_ctx **out, int device)
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, nullptr, c->dev_err);
         ok = hipDeviceSynchronize() == hipSuccess;
     }
+
     if (!ok) {
         av1pf_destroy(c);
         return AV1PF_ERR_DEVICE;
@@ -1070,7 +1156,13 @@ int av1pf_autotune(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_r
                    void *stream)
 {
     if (!c || n < 0 || (n && (!frames || !reqs))) return AV1PF_ERR_INVALID_ARG;
+    for (int i = 0; i < n; i++) {
+        int rc = validate_request(c, &frames[i], &reqs[i]);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipSetDevice(c->device));
+    for (int i = 0; i < n; i++)
+        if (reqs[i].stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;
     StageModes chosen;
     const int rc = run_batch(c, n, frames, reqs, static_cast<hipStream_t>(stream), false, true, &chosen);
     if (rc) return rc;

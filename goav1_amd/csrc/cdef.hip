@@ -220,7 +220,12 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         // encoder apply (cdef_apply.go:202-228, the noskip path): a unit is
         // read when one of its 8x8 blocks inside the frame has residual
         const auto dec = gmem(a.dec);
-        count = 1 << dec->cdef_bits;
+        const int bits = dec->cdef_bits;
+        // cdef_bits is 0..3 (8 strengths); anything else is flagged and read as 3
+        if (bits < 0 || bits > 3) {
+            if (tid == 0) atomicOr(err, 1);
+        }
+        count = 1 << min(max(bits, 0), 3);
         ystr = dec->y_strength;
         uvstr = dec->uv_strength;
         yfilt = uvfilt = 0;

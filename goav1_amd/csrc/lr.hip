@@ -46,6 +46,22 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     if (w <= 0 || y0 >= y1) return;
     const int h = y1 - y0;
     const int tid = threadIdx.x;
+    if (g.copy_pad) {
+        // the reference restores in place and writes only the visible plane:
+        // the MI-aligned padding keeps what the input holds there
+        const int padw = sp.aw - g.pw;
+        if (k == g.xtiles - 1 && padw > 0)
+            for (int i = tid; i < padw * h; i += 256) {
+                const int y = y0 + i / padw, x = g.pw + i % padw;
+                row_ptr<T>(dp, y)[x] = row_ptr<T>(sp, y)[x];
+            }
+        const int padh = sp.ah - g.ph, cw = (k == g.xtiles - 1 ? sp.aw : x0 + w) - x0;
+        if (t == g.stripes - 1 && padh > 0)
+            for (int i = tid; i < padh * cw; i += 256) {
+                const int y = g.ph + i / cw, x = x0 + i % cw;
+                row_ptr<T>(dp, y)[x] = row_ptr<T>(sp, y)[x];
+            }
+    }
 
     av1pf_lr_unit u;
     u.type = AV1PF_LR_NONE;

@@ -20,6 +20,7 @@ struct PlaneView {
     int64_t stride;    // bytes
     int     w, h;      // visible size
     int     aw, ah;    // MI-aligned extent (what deblock / CDEF read and write)
+    int     cw, ch;    // allocation: stride / bytes per sample columns, allocated rows
 };
 
 struct FrameView {
@@ -85,6 +86,8 @@ struct SrPlane {
 struct LrPlaneDesc {
     int type;                      // frame type, 0 = none (plane copied)
     int sr;                        // boundary rows are upscaled from the coded frame (SrPlane of the plane)
+    int copy_pad;                  // copy the MI-aligned padding from the LR input (input and output are views
+                                   // of one frame; not with super-resolution, whose output keeps its own)
     int us, hz, vt;                // unit size and grid
     const av1pf_lr_unit *units;
     uint16_t *bnd;                 // [stripes][4][pw]: rows y0-2, y0-1, y1, y1+1 of the deblocked frame
@@ -123,6 +126,7 @@ struct FrameDesc {
     int lr_src, lr_dst;            // views the LR stage reads / writes
     int sr_src, sr_dst;            // views the super-resolution stage reads / writes
     int fin;                       // view holding the filtered picture (film grain reads it)
+    int ext;                       // view whose borders are edge-replicated (refreshing frames)
     SrPlane sr[3];
     DbDesc db;
     CdefDesc cd;
@@ -215,6 +219,12 @@ hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, c
 // src < 0: each frame's fin view
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
 hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
+// Border extension of each frame's ext view.  Segment plane p < 3: the right
+// padding of plane p's visible rows; p >= 3: the rows of plane p - 3 below the
+// visible height.  Tiles of EXT_COLS columns x EXT_ROWS rows.
+constexpr int EXT_COLS = 64;
+constexpr int EXT_ROWS = 4;
+hipError_t launch_extend(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
 hipError_t launch_deblock_build(const av1pf_lf_record *rec, const av1pf_lf_build_params &p, uint8_t *level_cache,
                                 av1pf_lf_mask *masks, hipStream_t s);
 constexpr int SR_COLS = 512;       // output columns per super-resolution tile

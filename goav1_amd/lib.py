@@ -32,8 +32,8 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_cdef_search_apply"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
-               "lr_search_exhaustive", "superres", "deblock_build"]
-PROFILE_STAGES = 16
+               "lr_search_exhaustive", "superres", "deblock_build", "extend"]
+PROFILE_STAGES = 17
 
 
 class Av1pfError(RuntimeError):
@@ -149,6 +149,7 @@ class DeviceFrame:
         f = abi.Frame()
         f.width, f.height, f.bit_depth = fr.w, fr.h, fr.bd
         f.ss_x, f.ss_y, f.mono, f.matrix_identity = fr.ssx, fr.ssy, fr.mono, fr.matrix_identity
+        f.alloc_height = fr.planes[0].alloc_h
         for i, t in enumerate(self.planes):
             f.plane[i] = t.data_ptr()
             f.stride[i] = t.stride(0) * t.element_size()
@@ -171,6 +172,7 @@ class DeviceSideData:
         self.keep = []
         req = abi.Request()
         req.stages = fr.stages if stages is None else stages
+        req.flags = abi.REQ_REFRESH if getattr(fr, "refresh", False) else 0
 
         def dev(arr):
             t = torch.from_numpy(np.ascontiguousarray(arr)).to(device)

@@ -32,7 +32,8 @@ def aligned8(v):
 
 
 class Plane:
-    """One plane held as uint16 (all bit depths), allocated at the MI-aligned extent."""
+    """One plane held as uint16 (all bit depths): visible w x h inside an
+    allocation of alloc_w (the row pitch in samples) x alloc_h rows."""
 
     def __init__(self, w, h, alloc_w, alloc_h):
         self.w, self.h = w, h
@@ -41,12 +42,24 @@ class Plane:
 
 
 class FrameData:
-    """A frame: geometry plus planes and the per-frame post-filter side data."""
+    """A frame: geometry plus planes and the per-frame post-filter side data.
 
-    def __init__(self, w, h, bd, ssx, ssy, mono=False):
+    Planes are allocated at the MI-aligned extent, or with sb > 0 at the
+    superblock-aligned extent goav1's frame.RequiredSize allocates
+    (frame/buffer.go:141-236); pad_cols adds row-pitch columns past it (a
+    stride alignment).  refresh marks a frame that refreshes a reference slot
+    (its borders are edge-replicated after the filters)."""
+
+    def __init__(self, w, h, bd, ssx, ssy, mono=False, sb=0, pad_cols=0):
         self.w, self.h, self.bd, self.ssx, self.ssy, self.mono = w, h, bd, ssx, ssy, int(mono)
         self.matrix_identity = 0
-        aw, ah = aligned8(w), aligned8(h)
+        self.refresh = False
+        if sb:
+            aw, ah = (w + sb - 1) // sb * sb, (h + sb - 1) // sb * sb
+        else:
+            aw, ah = aligned8(w), aligned8(h)
+        assert pad_cols % 2 == 0
+        aw += pad_cols
         self.planes = [Plane(w, h, aw, ah)]
         if not mono:
             cw, ch = (w + ssx) >> ssx, (h + ssy) >> ssy
@@ -373,9 +386,11 @@ def filter_level_from_qindex(q):
 
 def keyframe_720p(seed=0, w=1280, h=720, bd=8, ssx=1, ssy=1, stages=abi.STAGES_ALL & ~abi.STAGE_FILMGRAIN):
     """720p 8-bit 4:2:0 keyframe: blocky pixels, q120 deblock level, random
-    CDEF strengths (damping 3 + (120 >> 6) = 4) and switchable 64-px LR units."""
+    CDEF strengths (damping 3 + (120 >> 6) = 4) and switchable 64-px LR units;
+    allocated at the 64x64-superblock extent and refreshing a reference slot."""
     rng = np.random.default_rng(seed)
-    fr = FrameData(w, h, bd, ssx, ssy)
+    fr = FrameData(w, h, bd, ssx, ssy, sb=64)
+    fr.refresh = True
     fill_blocky(fr, rng)
     lv = filter_level_from_qindex(120)
     build_side_data(fr, rng, lf_levels=(lv, lv, lv, lv), cdef_damping=3 + (120 >> 6),
@@ -386,12 +401,14 @@ def keyframe_720p(seed=0, w=1280, h=720, bd=8, ssx=1, ssy=1, stages=abi.STAGES_A
 
 def sequence_1080p10(seed=1, nframes=8, w=1920, h=1080, grain=True):
     """1080p 10-bit 4:2:0 moving-rectangle sequence with Wiener + SGR units and
-    film grain (lag 3, 8 luma points, scaling shift 10)."""
+    film grain (lag 3, 8 luma points, scaling shift 10); every frame allocated
+    at the 64x64-superblock extent (1920x1088) and refreshing a reference slot."""
     rng = np.random.default_rng(seed)
     rects = make_rects(rng, w, h)
     frames = []
     for t in range(nframes):
-        fr = FrameData(w, h, 10, 1, 1)
+        fr = FrameData(w, h, 10, 1, 1, sb=64)
+        fr.refresh = True
         fill_rectangles(fr, rng, rects, t)
         build_side_data(fr, rng, lf_random=True, sharpness=int(rng.integers(0, 8)),
                         lr_types=(abi.LR_SWITCHABLE,) * 3, lr_unit=int(rng.choice([64, 128, 256])))
@@ -604,7 +621,8 @@ def superres_frame(seed=0, up_w=1920, h=1080, denom=12, bd=10, ssx=1, ssy=1, mon
     fr = FrameData(superres_coded_width(up_w, denom), h, bd, ssx, ssy, mono)
     fill_blocky(fr, rng)
     build_side_data(fr, rng, lr_types=None)
-    up = FrameData(up_w, h, bd, ssx, ssy, mono)
+    up = FrameData(up_w, h, bd, ssx, ssy, mono, sb=64)
+    fr.refresh = True
     grid = FrameData(up_w, h, bd, ssx, ssy, mono)
     build_side_data(grid, rng, cdef=False, lr_types=(abi.LR_SWITCHABLE,) * 3, lr_unit=lr_unit)
     fr.lr = grid.lr
@@ -627,7 +645,8 @@ def batch_mixed(seed=2, n=64, sizes=BATCH_SIZES):
         w, h = sizes[int(rng.integers(0, len(sizes)))]
         bd = int(rng.choice([8, 10]))
         sub = int(rng.integers(0, 2))
-        fr = FrameData(w, h, bd, sub, sub)
+        fr = FrameData(w, h, bd, sub, sub, sb=64)
+        fr.refresh = True
         fill_blocky(fr, rng)
         lrt = [int(rng.integers(0, 4)) for _ in range(3)]
         build_side_data(fr, rng, lf_random=True, sharpness=int(rng.integers(0, 8)),

@@ -66,7 +66,13 @@ typedef struct av1pf_frame {
     int32_t ss_x, ss_y;       /* chroma subsampling shifts (0 or 1) */
     int32_t mono;             /* 1 = luma only */
     int32_t matrix_identity;  /* ColorConfig.MatrixCoefficients == identity (film grain clip) */
-    int32_t reserved;
+    /* Luma rows allocated (len(Y.Pix) / Y.Stride, frame.RequiredSize,
+     * frame/buffer.go:141-236: the superblock-aligned height); chroma planes
+     * hold alloc_height >> ss_y rows.  0 = the MI-aligned height (H+7)&~7.
+     * Every plane's allocated width is stride / bytes per sample.  Only the
+     * border extension of a refreshing frame (AV1PF_REQ_REFRESH) writes past
+     * the MI-aligned extent.                                                */
+    int32_t alloc_height;
     void   *plane[3];         /* Y, U, V sample pointers */
     int64_t stride[3];        /* row pitch in BYTES */
 } av1pf_frame;
@@ -203,9 +209,21 @@ typedef struct av1pf_lf_build_params {
 } av1pf_lf_build_params;
 
 /* ---- one frame's request (FrameWorkPostFilterRequest, postfilter.go:278-284) */
+/* Request flags.
+ * AV1PF_REQ_REFRESH: the frame refreshes a reference slot
+ * (FrameSize.RefreshFrameFlags != 0).  After loop restoration (and the
+ * upscale) every plane of the output frame -- the upscaled frame with
+ * AV1PF_STAGE_SUPERRES -- is edge-replicated into its whole allocation before
+ * film grain, as goav1 does before a frame becomes a reference
+ * (postfilter.go:760-771, Frame.ExtendBordersFrom, frame/buffer.go:293-409):
+ * columns [plane width, stride / bytes per sample) of each visible row take
+ * the row's last visible sample, and rows [plane height, allocated rows)
+ * repeat the extended last visible row.  Runs even when no stage is left.  */
+#define AV1PF_REQ_REFRESH 0x01u
+
 typedef struct av1pf_request {
     uint32_t stages;                        /* AV1PF_STAGE_* still to run */
-    uint32_t pad;
+    uint32_t flags;                         /* AV1PF_REQ_* */
     const av1pf_deblock_params *deblock;    /* host struct, device arrays */
     const av1pf_cdef_params    *cdef;
     const av1pf_lr_params      *lr;
@@ -225,7 +243,11 @@ typedef struct av1pf_request {
 typedef struct av1pf_ctx av1pf_ctx;
 
 /* Library handle: one per decoder (the reference runner is not reentrant,
- * postfilter_parallel.go:143-146).  device < 0 picks the current device.   */
+ * postfilter_parallel.go:143-146).  device < 0 picks the current device.
+ * A handle's scratch arenas (second frame buffers, LR context rows, the
+ * device selection state) are ordered by the stream the calls are issued on:
+ * issue every call of one handle on one stream, or order the streams (an
+ * event) between calls; a decoder thread drives its handle from one stream. */
 int  av1pf_create(av1pf_ctx **out, int device);
 void av1pf_destroy(av1pf_ctx *ctx);
 const char *av1pf_version(void);
@@ -450,8 +472,8 @@ int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, 
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
  * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13 lr_search_exhaustive,
- * 14 superres, 15 deblock_build.                                            */
-#define AV1PF_PROFILE_STAGES 16
+ * 14 superres, 15 deblock_build, 16 border extension.                      */
+#define AV1PF_PROFILE_STAGES 17
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
                         double bytes[AV1PF_PROFILE_STAGES]);

@@ -85,6 +85,9 @@ int  orc_superres_row(const uint16_t *src, int srcw, int codedw, uint16_t *dst, 
 int  orc_superres(const orc_frame *coded, orc_frame *up);
 int  orc_lr_save_superres(const orc_frame *coded, const orc_frame *up, const av1pf_lr_params *p, orc_lr_bounds b[3]);
 
+/* ---------------- border extension of a refreshing frame (frame/buffer.go:293-409) */
+int  orc_extend_borders(orc_frame *f);
+
 /* ---------------- chain (decoder/postfilter.go:589-845, 947-979) */
 int  orc_chain(orc_frame *f, const av1pf_request *req, orc_frame *display);
 int  orc_chain_sr(orc_frame *f, const av1pf_request *req, orc_frame *up, orc_frame *display);

@@ -72,6 +72,7 @@ class HostRequest:
         self.keep = []
         self.req = abi.Request()
         self.req.stages = fr.stages if stages is None else stages
+        self.req.flags = abi.REQ_REFRESH if getattr(fr, "refresh", False) else 0
         if fr.deblock is not None:
             d = fr.deblock
             p = abi.DeblockParams()

@@ -10,6 +10,9 @@
  * (postfilter.go:947-979): deblock and CDEF on the coded frame, the deblocked
  * LR rows upscaled on save (postfilter.go:879-941), the upscale into the
  * upscaled frame, then the after-CDEF save, LR and film grain on it.
+ * A refreshing frame (AV1PF_REQ_REFRESH: RefreshFrameFlags != 0) has its
+ * borders edge-replicated after LR, before film grain (postfilter.go:760-771),
+ * on the frame the tail ran on (the upscaled frame with super-resolution).
  */
 #include "orc.h"
 
@@ -43,6 +46,8 @@ int orc_chain_sr(orc_frame *f, const av1pf_request *req, orc_frame *up, orc_fram
         orc_lr_bounds_free(b);
         if (rc != AV1PF_OK) return rc;
     }
+    if (req->flags & AV1PF_REQ_REFRESH)
+        if ((rc = orc_extend_borders(post)) != AV1PF_OK) return rc;
     if (st & AV1PF_STAGE_FILMGRAIN)
         if ((rc = orc_filmgrain(post, display, req->grain)) != AV1PF_OK) return rc;
     return AV1PF_OK;

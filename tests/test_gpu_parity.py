@@ -45,16 +45,22 @@ def gpu_run(runner, fr, stages=None):
     return out, dout
 
 
-def assert_same(a, b, what):
+def assert_same(a, b, what, full=True):
+    """Per-plane MD5 of the visible samples, and with full=True every sample
+    of the allocation (MI-aligned padding written by deblock and CDEF, the
+    edge-replicated borders of a refreshing frame, untouched padding)."""
     ma, mb = orc.plane_md5(a), orc.plane_md5(b)
-    if ma != mb:
-        msgs = []
-        for i, (pa, pb) in enumerate(zip(a.planes, b.planes)):
-            va, vb = pa.pix[:pa.h, :pa.w], pb.pix[:pb.h, :pb.w]
-            diff = np.argwhere(va != vb)
-            if len(diff):
-                y, x = diff[0]
-                msgs.append(f"plane {i}: {len(diff)} diffs, first at y={y} x={x}: gpu={va[y, x]} oracle={vb[y, x]}")
+    msgs = []
+    for i, (pa, pb) in enumerate(zip(a.planes, b.planes)):
+        va, vb = (pa.pix, pb.pix) if full else (pa.pix[:pa.h, :pa.w], pb.pix[:pb.h, :pb.w])
+        assert va.shape == vb.shape, f"{what}: plane {i} shapes {va.shape} {vb.shape}"
+        diff = np.argwhere(va != vb)
+        if len(diff):
+            y, x = diff[0]
+            where = "visible" if y < pa.h and x < pa.w else "padding"
+            msgs.append(f"plane {i}: {len(diff)} diffs, first at y={y} x={x} ({where}): "
+                        f"gpu={va[y, x]} oracle={vb[y, x]}")
+    if msgs or ma != mb:
         raise AssertionError(f"{what}: " + "; ".join(msgs))
 
 
@@ -96,7 +102,7 @@ def test_film_grain_parity(runner, bd, ss):
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
     assert_same(got, want, "filtered frame")
-    assert_same(gdisp, wdisp, "display frame")
+    assert_same(gdisp, wdisp, "display frame", full=False)
 
 
 @pytest.mark.parametrize("bd", [8, 10, 12])
@@ -112,7 +118,7 @@ def test_chain_422(runner, bd):
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
     assert_same(got, want, "4:2:2 filtered frame")
-    assert_same(gdisp, wdisp, "4:2:2 display frame")
+    assert_same(gdisp, wdisp, "4:2:2 display frame", full=False)
 
 
 @pytest.mark.parametrize("unit", [64, 128, 256])
@@ -135,7 +141,7 @@ def test_sequence_1080p10_first_frames(runner):
         want, wdisp = orc.run_chain(fr)
         got, gdisp = gpu_run(runner, fr)
         assert_same(got, want, "1080p10 filtered")
-        assert_same(gdisp, wdisp, "1080p10 display")
+        assert_same(gdisp, wdisp, "1080p10 display", full=False)
 
 
 def _frames_from(fr, devs):
@@ -179,7 +185,7 @@ def check_batch(frames, devs, disps, wants, what):
         want, wdisp = wants[i]
         assert_same(_frames_from(fr, devs[i]), want, f"{what} frame {i} stages {fr.stages:#x}")
         if disps[i] is not None:
-            assert_same(_frames_from(fr, disps[i]), wdisp, f"{what} frame {i} display")
+            assert_same(_frames_from(fr, disps[i]), wdisp, f"{what} frame {i} display", full=False)
 
 
 @pytest.fixture(scope="module")
@@ -274,7 +280,7 @@ def test_stage_entry_points(runner, bd):
     torch.cuda.synchronize()
     want, wdisp = orc.run_chain(fr)
     assert_same(_frames_from(fr, d), want, "stage-level chain")
-    assert_same(_frames_from(fr, disp), wdisp, "stage-level display")
+    assert_same(_frames_from(fr, disp), wdisp, "stage-level display", full=False)
 
 
 @pytest.mark.parametrize("lag", [0, 1, 2, 3])
@@ -290,7 +296,7 @@ def test_film_grain_lags(runner, lag, ss):
     print(f"grain lag={lag} ss={ss}", flush=True)
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
-    assert_same(gdisp, wdisp, f"display lag {lag} ss {ss}")
+    assert_same(gdisp, wdisp, f"display lag {lag} ss {ss}", full=False)
 
 
 @pytest.mark.parametrize("case", [
@@ -314,7 +320,7 @@ def test_film_grain_variants(runner, case):
     print(f"grain variant {case}", flush=True)
     want, wdisp = orc.run_chain(fr)
     got, gdisp = gpu_run(runner, fr)
-    assert_same(gdisp, wdisp, f"display {case}")
+    assert_same(gdisp, wdisp, f"display {case}", full=False)
 
 
 def test_deblock_misaligned_masks_rejected(runner):
@@ -345,3 +351,53 @@ def test_lr_chroma_unit_below_stripe_rejected(runner):
     fr.stages = abi.STAGE_LR
     with pytest.raises(Av1pfError):
         runner.apply(DeviceFrame(fr), DeviceSideData(fr))
+
+
+def test_autotune_validates_requests(runner):
+    """autotune validates every request before planning, as apply_batch does:
+    a missing CDEF parameter block or a mask grid that does not match the frame
+    is refused with INVALID_ARG and nothing is launched."""
+    from goav1_amd.lib import Av1pfError, DeviceFrame, DeviceSideData
+    rng = np.random.default_rng(8)
+    fr = synth.FrameData(128, 96, 8, 1, 1)
+    synth.fill_blocky(fr, rng)
+    synth.build_side_data(fr, rng, lf_random=True)
+    fr.stages = abi.STAGE_DEBLOCK | abi.STAGE_CDEF
+    side = DeviceSideData(fr)
+    side.req.cdef = None
+    with pytest.raises(Av1pfError) as e:
+        runner.autotune([DeviceFrame(fr)], [side])
+    assert e.value.code == abi.ERR_INVALID_ARG
+    side = DeviceSideData(fr)
+    side.req.deblock.contents.mi_cols += 2
+    with pytest.raises(Av1pfError) as e:
+        runner.autotune([DeviceFrame(fr)], [side])
+    assert e.value.code == abi.ERR_INVALID_ARG
+
+
+def test_graph_replay_then_autotune_without_sync(runner, mixed):
+    """A graph replay followed at once by autotune (which starts building film
+    grain templates on the side stream) and a second replay: the template
+    pair the replay reads is not rewritten under it, every result equals the
+    oracle."""
+    import torch
+    frames, wants = mixed
+    runner.set_graphs(True)
+    try:
+        devs, disps, sides = device_batch(frames)
+        runner.apply_batch(devs, sides)               # capture
+        runner.sync()
+        devs, disps, sides = device_batch(frames)
+        print("graph replay then autotune", flush=True)
+        runner.apply_batch(devs, sides)               # replay, not synchronised
+        devs2, disps2, sides2 = device_batch(frames)
+        runner.autotune(devs2, sides2)
+        devs3, disps3, sides3 = device_batch(frames)
+        runner.apply_batch(devs3, sides3)             # capture again (autotune dropped the class)
+        runner.sync()
+        torch.cuda.synchronize()
+        check_batch(frames, devs, disps, wants, "replay before autotune")
+        check_batch(frames, devs2, disps2, wants, "autotune after replay")
+        check_batch(frames, devs3, disps3, wants, "replay after autotune")
+    finally:
+        runner.set_graphs(False)
