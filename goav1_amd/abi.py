@@ -30,6 +30,7 @@ STAGE_FILMGRAIN = 0x10
 STAGES_ALL = STAGE_DEBLOCK | STAGE_CDEF | STAGE_LR | STAGE_FILMGRAIN
 
 REQ_REFRESH = 0x01
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 LR_NONE, LR_WIENER, LR_SGRPROJ, LR_SWITCHABLE = 0, 1, 2, 3
 

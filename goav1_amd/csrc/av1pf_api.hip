@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "pf_internal.h"
@@ -30,6 +31,40 @@
 
 using namespace pf;
 
+// Scratch of the allocation-free entry point av1pf_apply_noalloc: every arena
+// one call needs (descriptor image, second frame buffers, LR context rows,
+// film-grain working memory), AV1PF_NOALLOC_SCRATCH_BYTES per device, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel or a copy: each device's copy is reached through the address
+// hipGetSymbolAddress returns for it (noalloc_scratch below).
+static __device__ __align__(256) char scratch[AV1PF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device `device`, looked up once
+// per device; nullptr when the lookup fails.
+void *noalloc_scratch(int device, std::string *err)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        *err = "device index past the allocation-free scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace {
 constexpr int kSlots = 4;
@@ -111,6 +146,9 @@ struct av1pf_ctx {
     int slot = 0;
     // stage-level LR: boundaries saved by av1pf_lr_save_boundaries for this frame
     const void *bnd_frame = nullptr;
+    // device address of the allocation-free entry point's scratch (nullptr: unavailable)
+    void *noalloc = nullptr;
+    std::string noalloc_err;
     // encoder CDEF apply: the device-side decision the next single-frame chain uses
     const av1pf_cdef_decision *apply_dec = nullptr;
     const uint8_t *apply_idx = nullptr;
@@ -714,8 +752,8 @@ int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// A caller-provided device region that replaces every arena of one call:
-// descriptor image, second frame buffers,
+// A caller-provided device region that replaces every arena of one call
+// (the allocation-free entry point): descriptor image, second frame buffers,
 // LR context rows, film-grain working memory.
 struct FixedArena {
     char *base;
@@ -1070,7 +1108,9 @@ int av1pf_create(av1pf_ctx **out, int device)
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, nullptr, c->dev_err);
         ok = hipDeviceSynchronize() == hipSuccess;
     }
-
+    // the allocation-free entry point's scratch on this device (current here);
+    // a failed lookup disables only that entry point
+    if (ok) c->noalloc = noalloc_scratch(device, &c->noalloc_err);
     if (!ok) {
         av1pf_destroy(c);
         return AV1PF_ERR_DEVICE;
@@ -1123,6 +1163,18 @@ int av1pf_apply(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req
     HIPCHK(c, hipSetDevice(c->device));
     if (req->stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;   // the chain's save overwrites saved rows
     return run_batch(c, 1, frame, req, static_cast<hipStream_t>(stream));
+}
+
+int av1pf_apply_noalloc(av1pf_ctx *c, const av1pf_frame *frame, const av1pf_request *req, void *stream)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    int rc = validate_request(c, frame, req);
+    if (rc) return rc;
+    if (!c->noalloc) return fail(c, AV1PF_ERR_DEVICE, ("allocation-free scratch unavailable: " + c->noalloc_err).c_str());
+    HIPCHK(c, hipSetDevice(c->device));
+    if (req->stages & AV1PF_STAGE_LR) c->bnd_frame = nullptr;
+    const FixedArena arena{ static_cast<char *>(c->noalloc), (size_t)AV1PF_NOALLOC_SCRATCH_BYTES };
+    return run_batch(c, 1, frame, req, static_cast<hipStream_t>(stream), false, false, nullptr, &arena);
 }
 
 int av1pf_apply_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_request *reqs, void *stream)

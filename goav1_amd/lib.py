@@ -29,7 +29,7 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
            "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode",
            "av1pf_lr_search_exhaustive", "av1pf_superres", "av1pf_deblock_build",
-           "av1pf_cdef_search_apply"]
+           "av1pf_cdef_search_apply", "av1pf_apply_noalloc"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
                "lr_search_exhaustive", "superres", "deblock_build", "extend"]
@@ -57,6 +57,7 @@ def load():
     lib.av1pf_last_error.argtypes = [C.c_void_p]
     lib.av1pf_last_error.restype = C.c_char_p
     lib.av1pf_apply.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
+    lib.av1pf_apply_noalloc.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
     lib.av1pf_apply_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request), C.c_void_p]
     lib.av1pf_sync.argtypes = [C.c_void_p, C.c_void_p]
     lib.av1pf_set_graphs.argtypes = [C.c_void_p, C.c_int]
@@ -261,6 +262,11 @@ class PostFilterRunner:
     def apply(self, frame, side, stream=None):
         """Run the request's stages on `frame` (DeviceFrame), asynchronously."""
         self._check(self.lib.av1pf_apply(self.h, C.byref(frame.abi), C.byref(side.req), self._stream(stream)))
+
+    def apply_noalloc(self, frame, side, stream=None):
+        """apply() through the allocation-free entry point (the library's
+        static per-device scratch; requests up to NOALLOC_SCRATCH_BYTES)."""
+        self._check(self.lib.av1pf_apply_noalloc(self.h, C.byref(frame.abi), C.byref(side.req), self._stream(stream)))
 
     def apply_batch(self, frames, sides, stream=None):
         n = len(frames)

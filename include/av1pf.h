@@ -260,6 +260,21 @@ const char *av1pf_last_error(const av1pf_ctx *ctx);
  * (postfilter.go:947-979), which moves LR and film grain to req->upscaled.  */
 int  av1pf_apply(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *req, void *stream);
 
+/* av1pf_apply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same chain, same results as av1pf_apply; every arena the call
+ * needs (descriptor image, second frame buffers, LR context rows, film-grain
+ * templates) is carved from one static device array of
+ * AV1PF_NOALLOC_SCRATCH_BYTES per device, enough for one 3840x2160 10-bit
+ * 4:2:0 frame without super-resolution; a larger request fails with
+ * AV1PF_ERR_SHORT_BUFFER before anything is launched.  The array is shared by
+ * every handle on a device and keeps the last call's contents: calls issued
+ * on one stream are ordered, calls that overlap on one GPU (other streams,
+ * other handles) race.  No graph replay or autotuning on this path.         */
+#define AV1PF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int  av1pf_apply_noalloc(av1pf_ctx *ctx, const av1pf_frame *frame, const av1pf_request *req, void *stream);
+
 /* Run many frames of mixed size, bit depth and chroma format.  One launch per
  * stage covers every frame of the batch: each workgroup finds its frame,
  * plane and tile in a segment table (no counterpart in the reference, which
