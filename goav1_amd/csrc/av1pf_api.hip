@@ -272,6 +272,8 @@ FrameView view_of(const av1pf_frame *f)
         v.p[i].ah = ah >> sy;
         v.p[i].cw = (int)(f->stride[i] / bps_of(f->bit_depth));
         v.p[i].ch = (f->alloc_height ? f->alloc_height : ah) >> sy;
+        const uint64_t bits = reinterpret_cast<uintptr_t>(f->plane[i]) | (uint64_t)f->stride[i] | 16u;
+        v.p[i].al = (int)(bits & (~bits + 1));
     }
     return v;
 }
@@ -529,7 +531,13 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
         scr_off[i] = P.scratch;
         P.scratch += scratch_bytes(A, strides);
         D.v[VIEW_B] = A;
-        for (int p = 0; p < A.nplanes; p++) D.v[VIEW_B].p[p].stride = strides[p];   // base patched later
+        for (int p = 0; p < A.nplanes; p++) {   // base patched later (256-byte aligned planes)
+            PlaneView &b = D.v[VIEW_B].p[p];
+            b.stride = strides[p];
+            b.cw = (int)(strides[p] / bps_of(A.bd));
+            b.ch = b.ah;
+            b.al = 16;
+        }
         const uint32_t st = r->stages;
         const bool sr = (st & AV1PF_STAGE_SUPERRES) != 0;
         const av1pf_frame *pf = sr ? r->upscaled : f;   // the frame LR and film grain see
@@ -584,8 +592,9 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
                 lr_geometry(pf, p, pl, &pw, &ph, &g.hz, &g.vt);
                 g.us = p->unit_size[pl];
                 g.units = p->units[pl];
+                g.bp = (g.pw + 7) & ~7;
                 bnd_off[i * 3 + pl] = P.bnd;
-                P.bnd += (size_t)g.stripes * 4 * g.pw;
+                P.bnd += (size_t)g.stripes * 4 * g.bp;
                 lr_any = true;
             }
             if (lr_any && !skip_lr_save)
@@ -636,14 +645,21 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             }
             P.bytes_sr += samples * bps_of(A.bd);
             if (cur == VIEW_B) {
-                for (int pl = 0; pl < A.nplanes; pl++) P.copy_ba.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+                for (int pl = 0; pl < A.nplanes; pl++)
+                    P.copy_ba.add(i, pl, (A.p[pl].aw + COPY_COLS - 1) / COPY_COLS, (A.p[pl].ah + COPY_ROWS - 1) / COPY_ROWS);
                 P.bytes_copy += 2 * view_bytes(A);
             }
             if (lr_any) {
                 int64_t strides[3];
                 if (sscr_off) (*sscr_off)[i] = P.scratch;
                 P.scratch += scratch_bytes(U, strides);
-                for (int p = 0; p < U.nplanes; p++) D.v[VIEW_S].p[p].stride = strides[p];   // base patched later
+                for (int p = 0; p < U.nplanes; p++) {   // base patched later (256-byte aligned planes)
+                    PlaneView &v = D.v[VIEW_S].p[p];
+                    v.stride = strides[p];
+                    v.cw = (int)(strides[p] / bps_of(U.bd));
+                    v.ch = v.ah;
+                    v.al = 16;
+                }
             }
             cur = D.sr_dst;
         }
@@ -656,7 +672,8 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
             cur = D.lr_dst;
         }
         if (cur == VIEW_B) {
-            for (int pl = 0; pl < A.nplanes; pl++) P.copy_ba.add(i, pl, (A.p[pl].aw + 255) / 256, A.p[pl].ah);
+            for (int pl = 0; pl < A.nplanes; pl++)
+                P.copy_ba.add(i, pl, (A.p[pl].aw + COPY_COLS - 1) / COPY_COLS, (A.p[pl].ah + COPY_ROWS - 1) / COPY_ROWS);
             P.bytes_copy += 2 * view_bytes(A);
         }
         // ---- border extension of a refreshing frame (postfilter.go:760-771):
@@ -676,7 +693,8 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
         if (st & AV1PF_STAGE_FILMGRAIN) {
             const av1pf_grain_params *g = r->grain;
             if (grain_noop(g)) {
-                for (int pl = 0; pl < U.nplanes; pl++) P.copy_ad.add(i, pl, (U.p[pl].aw + 255) / 256, U.p[pl].ah);
+                for (int pl = 0; pl < U.nplanes; pl++)
+                    P.copy_ad.add(i, pl, (U.p[pl].aw + COPY_COLS - 1) / COPY_COLS, (U.p[pl].ah + COPY_ROWS - 1) / COPY_ROWS);
                 P.bytes_copy += 2 * view_bytes(U);
             } else {
                 GrainDesc &G = D.gr;

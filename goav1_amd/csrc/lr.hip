@@ -22,16 +22,88 @@
 // record is looked up per tile.  The tile and its 3-sample halo are staged in
 // LDS once.  The kernel is out of place (post-CDEF frame -> output frame); NONE
 // units and NONE planes copy.
+#include <type_traits>
 #include "pf_internal.h"
 #include "lr_filters.h"
 #include "superres_taps.h"
 
 namespace pf {
 
+// Four samples of a row starting at column x (a multiple of 4) as packed
+// 16-bit pairs: one vector load when the four lie inside [0, pw) and the row
+// allows al-byte accesses, else edge-clamped sample loads.
 template <typename T>
-__device__ __forceinline__ int read_px(const PlaneView &pv, int y, int x)
+__device__ __forceinline__ uint2 load4(const T *row, int x, int pw, bool vec)
 {
-    return row_ptr<T>(pv, y)[x];
+    if (vec && x >= 0 && x + 4 <= pw) {
+        if constexpr (sizeof(T) == 2) {
+            return *(const uint2 *)(row + x);
+        } else {
+            const uint32_t v = *(const uint32_t *)(row + x);
+            return make_uint2((v & 0xffu) | ((v & 0xff00u) << 8), ((v >> 16) & 0xffu) | ((v >> 8) & 0xff0000u));
+        }
+    }
+    uint32_t s[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = row[min(max(x + k, 0), pw - 1)];
+    return make_uint2(s[0] | (s[1] << 16), s[2] | (s[3] << 16));
+}
+
+// Four output samples at column x of a row (a multiple of 4), n of them valid.
+template <typename T>
+__device__ __forceinline__ void store4(T *row, int x, const int *v, int n, bool vec)
+{
+    if (vec && n == 4) {
+        if constexpr (sizeof(T) == 2)
+            *reinterpret_cast<uint2 *>(row + x) = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                                                             (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        else
+            *reinterpret_cast<uint32_t *>(row + x) =
+                (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+        return;
+    }
+    for (int k = 0; k < n; k++) row[x + k] = (T)v[k];
+}
+
+// Copy of the tile's w x h samples from sp to dp (a NONE unit): eight
+// samples per lane and row when both planes allow it.
+template <typename T>
+__device__ __forceinline__ void copy_tile(const PlaneView &sp, const PlaneView &dp, int x0, int y0, int w, int h)
+{
+    const int tid = threadIdx.x;
+    constexpr int VB = 8 * (int)sizeof(T);   // bytes of eight samples
+    if (sp.al >= VB && dp.al >= VB) {
+        using V = typename std::conditional<sizeof(T) == 2, uint4, uint2>::type;
+        const int full = w >> 3, tail = w & 7, per = full + (tail ? 1 : 0);
+        for (int i = tid; i < per * h; i += 256) {
+            const int r = i / per, k = i - r * per;
+            const T *src = row_ptr<T>(sp, y0 + r) + x0 + 8 * k;
+            T *dst = row_ptr<T>(dp, y0 + r) + x0 + 8 * k;
+            if (k < full) {
+                *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(src);
+            } else {
+                for (int j = 0; j < tail; j++) dst[j] = src[j];
+            }
+        }
+        return;
+    }
+    const int lpr = lanes_for(w), c = tid & (lpr - 1), step = 256 / lpr;
+    if (c < w)
+        for (int r = tid / lpr; r < h; r += step) row_ptr<T>(dp, y0 + r)[x0 + c] = row_ptr<T>(sp, y0 + r)[x0 + c];
+}
+
+// The unit record of tile (k, t) of a plane (NONE for a NONE plane).
+__device__ __forceinline__ av1pf_lr_unit lr_tile_unit(const LrPlaneDesc &g, int k, int t)
+{
+    av1pf_lr_unit u;
+    u.type = AV1PF_LR_NONE;
+    if (g.type != AV1PF_LR_NONE) {
+        const int x0 = k * g.procw, y0 = max(0, t * g.sh - g.off);
+        const int uc = min(x0 / g.us, g.hz - 1);
+        const int ur = min((y0 + g.off) / g.us, g.vt - 1);
+        u = g.units[ur * g.hz + uc];
+    }
+    return u;
 }
 
 template <typename T>
@@ -63,81 +135,67 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
             }
     }
 
-    av1pf_lr_unit u;
-    u.type = AV1PF_LR_NONE;
-    if (g.type != AV1PF_LR_NONE) {
-        const int uc = min(x0 / g.us, g.hz - 1);
-        const int ur = min((y0 + g.off) / g.us, g.vt - 1);
-        u = g.units[ur * g.hz + uc];
-    }
+    const av1pf_lr_unit u = lr_tile_unit(g, k, t);
     if (u.type != AV1PF_LR_WIENER && u.type != AV1PF_LR_SGRPROJ) {
-        // copy: every load of the lane's rows in flight before its stores
-        const int lpr = lanes_for(w), c = tid & (lpr - 1), step = 256 / lpr;
-        if (c < w)
-            for (int r0 = tid / lpr; r0 < h; r0 += 8 * step) {
-                T v[8];
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (r0 + j * step < h) v[j] = row_ptr<T>(sp, y0 + r0 + j * step)[x0 + c];
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (r0 + j * step < h) row_ptr<T>(dp, y0 + r0 + j * step)[x0 + c] = v[j];
-            }
+        copy_tile<T>(sp, dp, x0, y0, w, h);
         return;
     }
-    // ---- stage the tile with its 3-sample halo; a wave's rows are uniform,
-    // and a lane issues the loads of 6 rows before writing them to LDS
+    // ---- stage the tile with its 3-sample halo: four samples per load, the
+    // loads of a lane all in flight before its LDS writes.  Tile column
+    // LR_OX + 4i - 4 holds frame column x0 - 4 + 4i.
     const int nominal = g.sh - (y0 == 0 ? g.off : 0);
     const bool copy_above = y0 != 0, copy_below = y0 + nominal < g.ph;
-    const int tw = w + 6, th = h + 6;
-    const auto bnd = gmem(g.bnd) + (int64_t)t * 4 * g.pw;
-    for (int cc = tid & 63; cc < tw; cc += 64) {
-        const int c = cc - 3;
-        const int x = min(max(x0 + c, 0), g.pw - 1);
-        for (int rr0 = tid >> 6; rr0 < th; rr0 += 24) {
-            int v[6];
+    const int th = h + 6, nch = (w + 6) / 4 + 1, items = th * nch;
+    // the saved rows are global memory: reached through a global pointer as row_ptr does
+    const uint16_t *bnd = (const uint16_t *)(gmem(g.bnd) + (int64_t)t * 4 * g.bp);
+    const bool fvec = sp.al >= 4 * (int)sizeof(T);
+    constexpr int KMAX = (LR_TH * 18 + 255) / 256;
+    uint2 v[KMAX];
 #pragma unroll
-            for (int j = 0; j < 6; j++) {
-                const int rr = rr0 + 4 * j;
-                if (rr >= th) break;
-                const int r = rr - 3;
-                if (r < 0) {
-                    if (copy_above) v[j] = bnd[(r == -1 ? 1 : 0) * g.pw + x];
-                    else v[j] = read_px<T>(sp, max(y0 + r, 0), x);
-                } else if (r >= h) {
-                    if (copy_below) v[j] = bnd[(2 + min(r - h, 1)) * g.pw + x];
-                    else v[j] = read_px<T>(sp, min(y0 + r, g.ph - 1), x);
-                } else {
-                    v[j] = read_px<T>(sp, y0 + r, x);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 6; j++) {
-                const int rr = rr0 + 4 * j;
-                if (rr >= th) break;
-                tile[rr * LR_TW + cc] = (uint16_t)v[j];
-            }
+    for (int j = 0; j < KMAX; j++) {
+        const int it = tid + j * 256;
+        if (it >= items) break;
+        const int rr = it / nch, ci = it - rr * nch;
+        const int r = rr - 3, x = x0 - 4 + 4 * ci;
+        if (r < 0 && copy_above) {
+            v[j] = load4<uint16_t>(bnd + (r == -1 ? 1 : 0) * g.bp, x, g.pw, true);
+        } else if (r >= h && copy_below) {
+            v[j] = load4<uint16_t>(bnd + (2 + min(r - h, 1)) * g.bp, x, g.pw, true);
+        } else {
+            v[j] = load4<T>(row_ptr<T>(sp, min(max(y0 + r, 0), g.ph - 1)), x, g.pw, fvec);
         }
     }
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        const int it = tid + j * 256;
+        if (it >= items) break;
+        const int rr = it / nch, ci = it - rr * nch;
+        *reinterpret_cast<uint2 *>(tile + rr * LR_TW + 4 * ci) = v[j];
+    }
     __syncthreads();
-    const uint16_t *O = tile + 3 * LR_TW + 3;   // tile origin
-    const int8_t *vt = u.wiener_v, *ht = u.wiener_h;
-    auto put = [&](int y, int x, int v) { row_ptr<T>(dp, y0 + y)[x0 + x] = (T)v; };
-    if (u.type == AV1PF_LR_WIENER) wiener_tile(O, w, h, bd, vt, ht, tmp, put);
-    else sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb, put);
+    const uint16_t *O = tile + 3 * LR_TW + LR_OX;   // tile origin
+    if (u.type == AV1PF_LR_WIENER) {
+        const bool dvec = dp.al >= 4 * (int)sizeof(T);
+        wiener_tile(O, w, h, bd, u.wiener_v, u.wiener_h, tmp, [&](int y, int x, const int *o, int n) {
+            store4<T>(row_ptr<T>(dp, y0 + y), x0 + x, o, n, dvec);
+        });
+    } else {
+        sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb,
+                 [&](int y, int x, int o) { row_ptr<T>(dp, y0 + y)[x0 + x] = (T)o; });
+    }
 }
 
-// 4 workgroups per CU: 35.9 KB of LDS each, and at most 128 VGPRs
+// 4 workgroups per CU: 36.2 KB of LDS each, and at most 128 VGPRs
 __global__ __launch_bounds__(256, 4) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
                                                  int total, StageLaunch L)
 {
-    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
     // Wiener intermediate and self-guided A/B planes share one buffer
     __shared__ __align__(16) char work[LR_AB * LR_AB * 6];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
     uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
-    static_assert(LR_TW * 64 * 2 <= LR_AB * LR_AB * 6, "Wiener intermediate fits the shared buffer");
+    static_assert(LR_TH * 64 * 2 <= LR_AB * LR_AB * 6, "Wiener intermediate fits the shared buffer");
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
         const FrameDesc &D = descs[sg.frame];
@@ -163,15 +221,15 @@ __device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneD
     if (x >= g.pw || t >= g.stripes) return;
     const int y0 = max(0, t * g.sh - g.off);
     const int y1 = min((t + 1) * g.sh - g.off, g.ph);
-    const auto b = gmem(g.bnd) + (int64_t)t * 4 * g.pw;
+    const auto b = gmem(g.bnd) + (int64_t)t * 4 * g.bp;
     auto px = [&](int y) { return sr ? sr_sample<T>(row_ptr<T>(pv, y), *sr, x, maxv) : (int)row_ptr<T>(pv, y)[x]; };
     if (t > 0) {
         b[x] = px(y0 - 2);
-        b[g.pw + x] = px(y0 - 1);
+        b[g.bp + x] = px(y0 - 1);
     }
     if (y1 < g.ph) {
-        b[2 * g.pw + x] = px(y1);
-        b[3 * g.pw + x] = px(y1 + 1 < g.ph ? y1 + 1 : y1);
+        b[2 * g.bp + x] = px(y1);
+        b[3 * g.bp + x] = px(y1 + 1 < g.ph ? y1 + 1 : y1);
     }
 }
 
@@ -230,12 +288,22 @@ int stage_grid(const void *kernel, int total, int per_wg)
     return std::min(total, cus * per_cu);
 }
 
-// Plain copy of one plane's aligned extent: workgroup = 256 samples of a row.
+// Plain copy of one plane's aligned extent.  A workgroup copies COPY_COLS
+// samples of COPY_ROWS rows; a lane moves eight adjacent samples with one
+// 16-byte (8-bit: 8-byte) load and store when both planes allow it.
 template <typename T>
 __device__ __forceinline__ void copy_cols(const PlaneView &s, const PlaneView &d, int x, int y)
 {
     if (x >= s.aw || y >= s.ah) return;
-    row_ptr<T>(d, y)[x] = row_ptr<T>(s, y)[x];
+    const T *src = row_ptr<T>(s, y) + x;
+    T *dst = row_ptr<T>(d, y) + x;
+    constexpr int VB = 8 * (int)sizeof(T);
+    if (x + 8 <= s.aw && s.al >= VB && d.al >= VB) {
+        using V = typename std::conditional<sizeof(T) == 2, uint4, uint2>::type;
+        *(V *)dst = *(const V *)src;
+        return;
+    }
+    for (int k = 0; k < 8 && x + k < s.aw; k++) dst[k] = src[k];
 }
 
 __global__ __launch_bounds__(256) void copy_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
@@ -244,7 +312,8 @@ __global__ __launch_bounds__(256) void copy_kernel(const FrameDesc *__restrict__
     const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const FrameDesc &D = descs[sg.frame];
     const int t = blockIdx.x - sg.base;
-    const int x = (t % sg.nx) * 256 + threadIdx.x, y = t / sg.nx;
+    const int x = (t % sg.nx) * COPY_COLS + (threadIdx.x % (COPY_COLS / 8)) * 8;
+    const int y = (t / sg.nx) * COPY_ROWS + threadIdx.x / (COPY_COLS / 8);
     const FrameView &S = D.v[src < 0 ? D.fin : src];
     if (S.bd == 8) copy_cols<uint8_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);
     else copy_cols<uint16_t>(S.p[sg.plane], D.v[dst].p[sg.plane], x, y);

@@ -2,9 +2,11 @@
 // (lr.hip) and the encoder search (lr_search.hip).
 //
 // The tile holds a w x h block (w, h <= 64) plus a 3-sample halo on every
-// side at pitch LR_TW; O points at sample (0, 0).  Each filter hands every
-// output sample to sink(y, x, value), so the decoder writes the frame and the
-// search accumulates squared errors from the same arithmetic:
+// side, LR_TH rows at pitch LR_TW, sample (0, 0) at row 3, column LR_OX
+// (O points there): column LR_OX = 4 puts every fourth sample column at an
+// 8-byte boundary, so four adjacent samples are one aligned 64-bit LDS read.
+// Each filter hands its outputs to a sink, so the decoder writes the frame and
+// the search accumulates squared errors from the same arithmetic:
 //   - Wiener (restoration/wiener.go:132-216): H pass with offset 2^(bd+6),
 //     round0, clamp to 2^(bd+1+7-round0)-1; V pass with round1;
 //   - self-guided (restoration/selfguided.go): box sums, uint32 p/z/A/B math,
@@ -19,7 +21,9 @@
 
 namespace pf {
 
-#define LR_TW 70          // 64 + 6
+#define LR_TW 72          // tile pitch: 4 + 64 + 3, rounded to 8 bytes
+#define LR_TH 70          // tile rows: 64 + 6
+#define LR_OX 4           // tile column of sample x = 0
 #define LR_AB 66          // 64 + 2
 
 __device__ __forceinline__ int32_t r2s(int32_t v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
@@ -58,9 +62,24 @@ __device__ __forceinline__ void lds_run(const uint16_t *s, int32_t (&v)[K1 - K0 
 // lane x of a row group handles column x, row groups step by 256 / lanes.
 __device__ __forceinline__ int lanes_for(int w) { return w > 32 ? 64 : w > 16 ? 32 : w > 8 ? 16 : 8; }
 
-template <typename Sink>
+// Four samples from one aligned 64-bit LDS word.
+__device__ __forceinline__ void unpack4(uint2 v, int32_t *t)
+{
+    t[0] = (int32_t)(v.x & 0xffffu);
+    t[1] = (int32_t)(v.x >> 16);
+    t[2] = (int32_t)(v.y & 0xffffu);
+    t[3] = (int32_t)(v.y >> 16);
+}
+
+// Wiener over the tile.  A lane computes four adjacent columns c..c+3 (c a
+// multiple of 4): the H pass reads the twelve samples c-4..c+7 of a row as
+// three aligned 64-bit words and writes four intermediates as one word of
+// tmp (pitch 64, rows -3..h+2); the V pass reads seven words of tmp.
+// sink(y, x, v[4], n) receives the outputs of columns x..x+n-1 (n < 4 only
+// at the tile's right edge).
+template <typename Sink4>
 __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int bd, const int8_t *vt, const int8_t *ht,
-                                            uint16_t *tmp, Sink sink)
+                                            uint16_t *tmp, Sink4 sink)
 {
     const int tid = threadIdx.x;
     const int th = h + 6;
@@ -73,25 +92,39 @@ __device__ __forceinline__ void wiener_tile(const uint16_t *O, int w, int h, int
     const int32_t v0 = vt[0], v1 = vt[1], v2 = vt[2];
     const int32_t v3 = -2 * (v0 + v1 + v2);
     const int32_t limit = 1 << (bd + 1 + 7 - round0), hoff = 1 << (bd + 6);
-    const int lpr = lanes_for(w), c = tid & (lpr - 1), rstep = 256 / lpr;
+    const int gpr = lanes_for(w) >> 2, c = (tid & (gpr - 1)) * 4, rstep = 256 / gpr;
     if (c < w)
-        for (int rr = tid / lpr; rr < th; rr += rstep) {
-            const int r = rr - 3;
-            int32_t t[7];
-            lds_run<-3, 3>(O + r * LR_TW + c, t);
-            int32_t sum = (t[3] << 7) + hoff;
-            sum += h0 * (t[0] + t[6]) + h1 * (t[1] + t[5]) + h2 * (t[2] + t[4]) + h3 * t[3];
-            tmp[rr * 64 + c] = (uint16_t)min(max(r2s(sum, round0), 0), limit - 1);
+        for (int rr = tid / gpr; rr < th; rr += rstep) {
+            const uint2 *sw = reinterpret_cast<const uint2 *>(O + (rr - 3) * LR_TW + c - 4);
+            int32_t t[12];
+            unpack4(sw[0], t);
+            unpack4(sw[1], t + 4);
+            unpack4(sw[2], t + 8);
+            uint32_t o[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int32_t sum = (t[k + 4] << 7) + hoff;
+                sum += h0 * (t[k + 1] + t[k + 7]) + h1 * (t[k + 2] + t[k + 6]) + h2 * (t[k + 3] + t[k + 5]) +
+                       h3 * t[k + 4];
+                o[k] = (uint32_t)min(max(r2s(sum, round0), 0), limit - 1);
+            }
+            *reinterpret_cast<uint2 *>(tmp + rr * 64 + c) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
         }
     __syncthreads();
     const int32_t voff = 1 << (bd + round1 - 1);
     if (c < w)
-        for (int r = tid / lpr; r < h; r += rstep) {
-            const uint16_t *s = tmp + (r + 3) * 64 + c;
-            int32_t sum = ((int32_t)s[0] << 7) - voff;
-            sum += v0 * ((int32_t)s[-3 * 64] + s[3 * 64]) + v1 * ((int32_t)s[-2 * 64] + s[2 * 64]) +
-                   v2 * ((int32_t)s[-64] + s[64]) + v3 * (int32_t)s[0];
-            sink(r, c, min(max(r2s(sum, round1), 0), mx));
+        for (int r = tid / gpr; r < h; r += rstep) {
+            int32_t s[7][4];
+#pragma unroll
+            for (int k = 0; k < 7; k++) unpack4(*reinterpret_cast<const uint2 *>(tmp + (r + k) * 64 + c), s[k]);
+            int v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int32_t sum = (s[3][j] << 7) - voff;
+                sum += v0 * (s[0][j] + s[6][j]) + v1 * (s[1][j] + s[5][j]) + v2 * (s[2][j] + s[4][j]) + v3 * s[3][j];
+                v[j] = min(max(r2s(sum, round1), 0), mx);
+            }
+            sink(r, c, v, min(4, w - c));
         }
 }
 

@@ -40,7 +40,7 @@ __device__ __forceinline__ uint64_t wg_sum(uint64_t v, uint64_t *red)
 __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
                                                         const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse_out)
 {
-    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
     __shared__ uint8_t srcT[64 * 64];
     __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
@@ -109,22 +109,25 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
             for (int i = tid; i < tw * th; i += 256) {
                 const int r = i / tw, c = i - r * tw;
                 const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
-                tile[r * LR_TW + c] = row_ptr<uint8_t>(R, y)[x];
+                tile[r * LR_TW + c + LR_OX - 3] = row_ptr<uint8_t>(R, y)[x];
             }
             for (int i = tid; i < w * h; i += 256) {
                 const int r = i / w, c = i - r * w;
                 srcT[r * 64 + c] = row_ptr<uint8_t>(S, y0 + r)[x0 + c];
             }
             __syncthreads();
-            const uint16_t *O = tile + 3 * LR_TW + 3;
+            const uint16_t *O = tile + 3 * LR_TW + LR_OX;
             uint64_t acc = 0;
             auto err = [&](int y, int x, int v) {
                 const int d = (int)srcT[y * 64 + x] - v;
                 acc += (uint64_t)(d * d);
             };
+            auto err4 = [&](int y, int x, const int *v, int n) {
+                for (int k = 0; k < n; k++) err(y, x + k, v[k]);
+            };
             uint64_t best_sse = ~0ull;
             if (wbest >= 0) {
-                wiener_tile(O, w, h, 8, kWienerCand[wbest], kWienerCand[wbest], tmp, err);
+                wiener_tile(O, w, h, 8, kWienerCand[wbest], kWienerCand[wbest], tmp, err4);
                 e[7] = best_sse = wg_sum(acc, red);
                 res.type = AV1PF_LR_WIENER;
                 for (int k = 0; k < 3; k++) res.wiener_v[k] = res.wiener_h[k] = kWienerCand[wbest][k];
@@ -233,7 +236,7 @@ __device__ void sgr_fit(int set, const int64_t *st, int8_t *xqd)
 __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
                                                             av1pf_lr_unit *out, uint64_t *sse_out, int8_t *fit_out)
 {
-    __shared__ uint16_t tile[LR_TW * LR_TW];
+    __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
     __shared__ uint8_t srcT[64 * 64];
     __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
@@ -270,14 +273,14 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
         for (int i = tid; i < tw * th; i += 256) {
             const int r = i / tw, c = i - r * tw;
             const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
-            tile[r * LR_TW + c] = row_ptr<uint8_t>(R, y)[x];
+            tile[r * LR_TW + c + LR_OX - 3] = row_ptr<uint8_t>(R, y)[x];
         }
         for (int i = tid; i < w * h; i += 256) {
             const int r = i / w, c = i - r * w;
             srcT[r * 64 + c] = row_ptr<uint8_t>(S, y0 + r)[x0 + c];
         }
         __syncthreads();
-        const uint16_t *O = tile + 3 * LR_TW + 3;
+        const uint16_t *O = tile + 3 * LR_TW + LR_OX;
         const uint64_t wh = (uint64_t)(w * h);
         uint64_t best = base;
         const LaneRows L = lane_rows(w, h);
@@ -327,12 +330,15 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             const int d = (int)srcT[y * 64 + x] - v;
             acc += (uint64_t)(d * d);
         };
+        auto err4 = [&](int y, int x, const int *v, int n) {
+            for (int k = 0; k < n; k++) err(y, x + k, v[k]);
+        };
         for (int c = 0; c < 7; c++) {
             const int8_t *vt = c < 6 ? kWienerCand[c] : fv;
             const int8_t *ht = c < 6 ? kWienerCand[c] : fh;
             acc = 0;
             __syncthreads();
-            wiener_tile(O, w, h, 8, vt, ht, tmp, err);
+            wiener_tile(O, w, h, 8, vt, ht, tmp, err4);
             const uint64_t sc = wg_sum(acc, red);
             if (tid == 0) e[1 + c] = sc;
             if (sc + wh < best) {

@@ -21,6 +21,8 @@ struct PlaneView {
     int     w, h;      // visible size
     int     aw, ah;    // MI-aligned extent (what deblock / CDEF read and write)
     int     cw, ch;    // allocation: stride / bytes per sample columns, allocated rows
+    int     al;        // alignment in bytes (power of two, at most 16) of base and stride: every
+                       // row start is al-aligned, so vector accesses of al bytes at aligned columns are legal
 };
 
 struct FrameView {
@@ -90,7 +92,8 @@ struct LrPlaneDesc {
                                    // of one frame; not with super-resolution, whose output keeps its own)
     int us, hz, vt;                // unit size and grid
     const av1pf_lr_unit *units;
-    uint16_t *bnd;                 // [stripes][4][pw]: rows y0-2, y0-1, y1, y1+1 of the deblocked frame
+    uint16_t *bnd;                 // [stripes][4][bp]: rows y0-2, y0-1, y1, y1+1 of the deblocked frame
+    int bp;                        // row pitch of bnd in samples (pw rounded up to 8: 16-byte rows)
     int pw, ph, procw, sh, off, xtiles, stripes;
 };
 
@@ -216,7 +219,10 @@ hipError_t launch_deblock(const FrameDesc *d, const Seg *segv, int nv, int totv,
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
 hipError_t launch_cdef(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
 hipError_t launch_lr(const FrameDesc *d, const Seg *segs, int nseg, int total, const StageLaunch &L, hipStream_t s);
-// src < 0: each frame's fin view
+// src < 0: each frame's fin view.  Segments tile the aligned extent in
+// COPY_COLS x COPY_ROWS blocks.
+constexpr int COPY_COLS = 512;
+constexpr int COPY_ROWS = 4;
 hipError_t launch_copy(const FrameDesc *d, const Seg *segs, int nseg, int total, int src, int dst, hipStream_t s);
 hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s);
 // Border extension of each frame's ext view.  Segment plane p < 3: the right
