@@ -101,6 +101,8 @@ int  orc_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb
 /* encoder curated LR search, luma 8-bit (orc_lr_search.c) */
 int orc_lr_search_units(int pw, int ph, int us, int *hz, int *vt);
 int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_unit *out, uint64_t *sse);
+/* the unit decision of the curated search from its 12 scores */
+void orc_lr_decide(const uint64_t e[12], int w, int h, av1pf_lr_unit *out);
 /* exhaustive superset: curated candidates scored exactly, a least-squares
  * Wiener fit and all 16 SGR sets with a least-squares projection; sse[unit][28]
  * and the fitted parameters fit[unit][38] (Wiener v[3], h[3], then xqd[2] per set) */

@@ -32,6 +32,56 @@ static const int8_t kWienerCand[6][3] = { { 0, -3, 12 }, { 0, -5, 18 }, { 0, -2,
 static const int kSgrPreset[4] = { 0, 2, 10, 14 };
 static const int8_t kSgrXqd[4][2] = { { 8, 24 }, { 16, 40 }, { 0, 48 }, { 24, 72 } };
 
+/* The Wiener screening (restoration.go:170-207): candidates in order, each
+ * taken when its H-only estimate plus w*h is below the running best, which
+ * starts at the baseline; -1 when none is taken. */
+static int wiener_pick(const uint64_t e[12], uint64_t wh)
+{
+    uint64_t best = e[0];
+    int pick = -1;
+    for (int c = 0; c < 6; c++)
+        if (e[1 + c] != UINT64_MAX && e[1 + c] + wh < best) { best = e[1 + c]; pick = c; }
+    return pick;
+}
+
+/* The unit decision from its scores (restoration.go:149-262 and the
+ * keyframe driver keyframe_lossy.go:2338-2386):
+ *   - a zero baseline is left NONE (both searches return not found);
+ *   - the Wiener screening picks a triple; the unit becomes WIENER with the
+ *     exact SSE e[7] as the best so far, whatever e[7] is (the exact score is
+ *     not compared with the baseline);
+ *   - units wider or taller than 64 samples get neither filter (the exact
+ *     filters' scratch refuses them), so they stay NONE;
+ *   - the SGR presets run in order from the baseline with the same
+ *     "sse + w*h < running best" rule; the found preset replaces the Wiener
+ *     choice only when its SSE is strictly below the best so far. */
+void orc_lr_decide(const uint64_t e[12], int w, int h, av1pf_lr_unit *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->type = AV1PF_LR_NONE;
+    const uint64_t base = e[0], wh = (uint64_t)w * (uint64_t)h;
+    if (base == 0 || w <= 0 || h <= 0 || w > 64 || h > 64) return;
+    uint64_t best_sse = UINT64_MAX;
+    const int wbest = wiener_pick(e, wh);
+    if (wbest >= 0) {
+        best_sse = e[7];
+        out->type = AV1PF_LR_WIENER;
+        memcpy(out->wiener_v, kWienerCand[wbest], 3);
+        memcpy(out->wiener_h, kWienerCand[wbest], 3);
+    }
+    uint64_t sbest = base;
+    int pbest = -1;
+    for (int p = 0; p < 4; p++)
+        if (e[8 + p] != UINT64_MAX && e[8 + p] + wh < sbest) { sbest = e[8 + p]; pbest = p; }
+    if (pbest >= 0 && sbest < best_sse) {
+        memset(out, 0, sizeof(*out));
+        out->type = AV1PF_LR_SGRPROJ;
+        out->sgr_set = (uint8_t)kSgrPreset[pbest];
+        out->sgr_xqd[0] = kSgrXqd[pbest][0];
+        out->sgr_xqd[1] = kSgrXqd[pbest][1];
+    }
+}
+
 int orc_lr_search_units(int pw, int ph, int us, int *hz, int *vt)
 {
     *hz = (pw + (us >> 1)) / us;
@@ -84,8 +134,6 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
             e[0] = base;
             if (base == 0) continue;
             /* Wiener: H-only approximation over the unit */
-            uint64_t best = base;
-            int wbest = -1;
             for (int c = 0; c < 6; c++) {
                 const int t0 = kWienerCand[c][0], t1 = kWienerCand[c][1], t2 = kWienerCand[c][2];
                 const int f[7] = { t0, t1, t2, -2 * (t0 + t1 + t2), t2, t1, t0 };
@@ -104,11 +152,10 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
                         s += (uint64_t)(d * d);
                     }
                 e[1 + c] = s;
-                if (s + (uint64_t)(w * h) < best) { best = s; wbest = c; }
             }
+            const int wbest = wiener_pick(e, (uint64_t)(w * h));
             const int valid = w > 0 && h > 0 && w <= 64 && h <= 64;
             const uint16_t *org = bord + (size_t)(y0 + pad) * bs + x0 + pad;
-            uint64_t best_sse = UINT64_MAX;
             if (wbest >= 0 && valid) {
                 orc_wiener_unit(org, bs, tmp, 64, w, h, kWienerCand[wbest], kWienerCand[wbest], 8);
                 uint64_t s = 0;
@@ -118,14 +165,8 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
                         s += (uint64_t)(d * d);
                     }
                 e[7] = s;
-                best_sse = s;
-                out[u].type = AV1PF_LR_WIENER;
-                memcpy(out[u].wiener_v, kWienerCand[wbest], 3);
-                memcpy(out[u].wiener_h, kWienerCand[wbest], 3);
             }
             if (!valid) continue;
-            uint64_t sbest = base;
-            int pbest = -1;
             for (int p = 0; p < 4; p++) {
                 orc_sgr_unit(org, bs, tmp, 64, w, h, kSgrPreset[p], kSgrXqd[p], 8);
                 uint64_t s = 0;
@@ -135,15 +176,8 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
                         s += (uint64_t)(d * d);
                     }
                 e[8 + p] = s;
-                if (s + (uint64_t)(w * h) < sbest) { sbest = s; pbest = p; }
             }
-            if (pbest >= 0 && sbest < best_sse) {
-                memset(&out[u], 0, sizeof(out[u]));
-                out[u].type = AV1PF_LR_SGRPROJ;
-                out[u].sgr_set = (uint8_t)kSgrPreset[pbest];
-                out[u].sgr_xqd[0] = kSgrXqd[pbest][0];
-                out[u].sgr_xqd[1] = kSgrXqd[pbest][1];
-            }
+            orc_lr_decide(e, w, h, &out[u]);
         }
     }
     free(bord);

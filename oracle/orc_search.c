@@ -158,9 +158,45 @@ static uint64_t search_one_dual(const uint64_t *mse, int nfb, const int *fbs, in
     return sum_best - bs;
 }
 
+/* The finish with an optional trace of every searchOneDual step: per step
+ * (75 of them: levels 0..3, nb greedy steps then 4 nb refinement steps
+ * each) the level, the slot filled, the chosen pair (search-level-1
+ * positions) and the returned total; per level the RD cost.  Test
+ * infrastructure (the steps are checked against brute-force optima). */
+typedef struct orc_finish_trace {
+    int32_t step[75][4];   /* level, slot, lev0, lev1 */
+    uint64_t total[75];
+    uint64_t cost[4];
+    int32_t nsteps;
+} orc_finish_trace;
+
+static int finish_impl(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda, uint8_t y_str[8],
+                       uint8_t uv_str[8], uint8_t *fb_idx, orc_finish_trace *tr);
+
+int orc_cdef_search_finish_trace(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda,
+                                 uint8_t y_str[8], uint8_t uv_str[8], uint8_t *fb_idx, orc_finish_trace *tr)
+{
+    memset(tr, 0, sizeof(*tr));
+    return finish_impl(mse, nfb, skip_fb, lambda, y_str, uv_str, fb_idx, tr);
+}
+
 /* Returns bits; writes y/uv packed strengths (8 each) and per-FB index. */
 int orc_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda,
                            uint8_t y_str[8], uint8_t uv_str[8], uint8_t *fb_idx)
+{
+    return finish_impl(mse, nfb, skip_fb, lambda, y_str, uv_str, fb_idx, NULL);
+}
+
+static void trace_step(orc_finish_trace *tr, int level, int slot, const int *lev0, const int *lev1, uint64_t t)
+{
+    if (!tr || tr->nsteps >= 75) return;
+    int32_t *e = tr->step[tr->nsteps];
+    e[0] = level; e[1] = slot; e[2] = lev0[slot]; e[3] = lev1[slot];
+    tr->total[tr->nsteps++] = t;
+}
+
+static int finish_impl(const uint64_t *mse, int nfb, const uint8_t *skip_fb, uint64_t lambda, uint8_t y_str[8],
+                       uint8_t uv_str[8], uint8_t *fb_idx, orc_finish_trace *tr)
 {
     int *fbs = malloc(sizeof(int) * (nfb > 0 ? nfb : 1));
     uint64_t (*tot)[64] = malloc(sizeof(uint64_t) * 64 * 64);
@@ -174,14 +210,19 @@ int orc_cdef_search_finish(const uint64_t *mse, int nfb, const uint8_t *skip_fb,
         int lev0[8] = { 0 }, lev1[8] = { 0 };
         int nb = 1 << i;
         uint64_t t = 1ull << 63;
-        for (int k = 0; k < nb; k++) t = search_one_dual(mse, nfb, fbs, nsb, lev0, lev1, k, endGI, tot);
+        for (int k = 0; k < nb; k++) {
+            t = search_one_dual(mse, nfb, fbs, nsb, lev0, lev1, k, endGI, tot);
+            trace_step(tr, i, k, lev0, lev1, t);
+        }
         for (int k = 0; k < 4 * nb; k++) {
             for (int j = 0; j < nb - 1; j++) { lev0[j] = lev0[j + 1]; lev1[j] = lev1[j + 1]; }
             t = search_one_dual(mse, nfb, fbs, nsb, lev0, lev1, nb - 1, endGI, tot);
+            trace_step(tr, i, nb - 1, lev0, lev1, t);
         }
         int total_bits = nsb * i + nb * 6 * 2;
         int64_t rate = (int64_t)total_bits << 9;
         uint64_t cost = (uint64_t)((rate * (int64_t)lambda + (1 << 8)) >> 9) + ((t * 16) << 7);
+        if (tr) tr->cost[i] = cost;
         if (cost < best_tot) {
             best_tot = cost; bits = i;
             for (int j = 0; j < nb; j++) { ys[j] = lev0[j]; us[j] = lev1[j]; }
