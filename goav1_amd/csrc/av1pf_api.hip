@@ -114,7 +114,8 @@ struct av1pf_ctx {
     bool tmpl_free_used[2] = {};
     int tpar = 0;
     hipEvent_t uploaded = nullptr;
-    void *finish_state = nullptr;  // device CDEF strength selection state
+    void *finish_state = nullptr;  // device CDEF strength selection state (zeroed once)
+    unsigned int select_epoch = 0; // selection launches on finish_state
     char *learn_buf = nullptr;     // learned search: tiles, logits, shortlist table, LR mask
     size_t learn_cap = 0;
     void *small_state = nullptr;   // shortlist selection: per-level results and ticket (zeroed once)
@@ -1266,6 +1267,12 @@ int av1pf_sync(av1pf_ctx *c, void *stream)
     if (flag) {
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), c->dev_err);
         HIPCHK(c, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+        if (flag & 2) {
+            // an aborted selection leaves its barrier counters behind: start over
+            HIPCHK(c, hipMemset(c->finish_state, 0, finish_state_bytes()));
+            c->select_epoch = 0;
+            return fail(c, AV1PF_ERR_DEVICE, "device strength selection: a barrier did not complete");
+        }
         return fail(c, AV1PF_ERR_INVALID_ARG, "cdef strength index past strength_count");
     }
     return AV1PF_OK;
@@ -1453,10 +1460,15 @@ int av1pf_cdef_search_select(av1pf_ctx *c, const av1pf_frame *recon, const uint6
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
     if (nhfb * nvfb > kMaxSearchBlocks) return fail(c, AV1PF_ERR_INVALID_ARG, "more than 4096 filter blocks");
     HIPCHK(c, hipSetDevice(c->device));
-    if (!c->finish_state) HIPCHK(c, hipMalloc(&c->finish_state, finish_state_bytes()));
+    if (!c->finish_state) {
+        HIPCHK(c, hipMalloc(&c->finish_state, finish_state_bytes()));
+        HIPCHK(c, hipMemset(c->finish_state, 0, finish_state_bytes()));
+        c->select_epoch = 0;
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_SELECT, s, 0);
-    HIPCHK(c, launch_cdef_select(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, c->finish_state, decision, fb_index, s));
+    HIPCHK(c, launch_cdef_select(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, c->finish_state, ++c->select_epoch,
+                                 decision, fb_index, c->dev_err, s));
     return AV1PF_OK;
 }
 

@@ -245,8 +245,12 @@ hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t s);
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s);
 
+// One launch (cooperative: every workgroup resident); epoch counts the
+// launches on this state from 1 (the state is zero before the first); bit 1
+// of err is set if a barrier did not complete.
 hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,
-                              uint64_t lambda, void *state, av1pf_cdef_decision *dec, uint8_t *fb_index, hipStream_t s);
+                              uint64_t lambda, void *state, unsigned int epoch, av1pf_cdef_decision *dec,
+                              uint8_t *fb_index, int *err, hipStream_t s);
 size_t finish_state_bytes();
 hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
                             av1pf_lr_unit *out, uint64_t *sse, hipStream_t s);

@@ -273,241 +273,348 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
 }
 
 // ---------------------------------------------------------------------------
-// Strength selection on the device (finish / jointStrengthSearchDual /
-// searchOneDual, cdef_search.go:780-945).  The sequence of choose steps is
-// fixed by the algorithm (for cdef_bits 0..3: nb = 1 << bits first choices,
-// then 4 * nb refinements that drop the oldest pair and choose the last slot
-// again), so the host enqueues it as a run of small kernels; only the chosen
-// pairs depend on the data, and they live in FinishState on the device.
-//  - fin_first_kernel: slot 0, luma and chroma chosen separately (lowest
-//    column sums over the active filter blocks);
-//  - fin_gain_kernel: slot > 0; thread = one (luma j, chroma k) pair over a
-//    group of 32 active blocks, summing how much the pair would lower each
-//    block's best cost into a gain table; the last workgroup to finish
-//    (atomic ticket) picks the first maximum in (j, k) order;
-//  - both end with the same epilogue: update the level's cost and the best
-//    set at the end of a level, and drop the oldest pair before a refinement;
-//  - fin_assign_kernel: each block's index = its cheapest chosen pair.
+// Strength selection on the device in one launch (finish /
+// jointStrengthSearchDual / searchOneDual, cdef_search.go:780-945).
+//
+// The four cdef_bits levels are independent searches, so they run
+// concurrently: level 0 on workgroup 0, level L >= 1 on SEL_G workgroups,
+// one per luma strength j.  A level's step sequence is fixed (nb =
+// 1 << L greedy steps filling slots 0..nb-1, then 4 nb refinements that drop
+// the oldest pair and fill the last slot again); only the chosen pairs depend
+// on the data.
+//  - A step with no prior pair (slot 0) chooses luma and chroma separately
+//    (first minimum of the column sums over the active blocks); every
+//    workgroup of the level computes it, so it needs no exchange.
+//  - Any other step: every workgroup computes the active blocks' best cost
+//    over the prior pairs (best_i); a wave sums, for the pairs (j, k = lane),
+//    the savings max(0, best_i - a_ij - b_ik) over an eighth of the blocks
+//    (a block with a_ij >= best_i saves nothing for any k, as the reference
+//    skips it; with the rows in LDS and every best_i below 2^32 the savings
+//    are saturating 32-bit differences, summed in 64 bits); the workgroup's best pair
+//    becomes a key (gain << 12 | 4095 - (64 j + k): the largest gain, then the
+//    first pair in (j, k) order) that it publishes in its own slot of the
+//    step, and every workgroup reads the level's SEL_G slots and keeps the
+//    largest key, so all of them hold the same lists.
+//  - The last level to finish (a ticket) takes the cheapest level by the RD
+//    cost and writes the decision and each block's index.
+// The MSE rows of the active blocks are staged in LDS as 32-bit values when
+// they fit (TABLES = 2: luma and chroma, 1: chroma; values of 2^32 or more
+// switch the workgroup -- every workgroup alike -- to reading device memory).
+// The slots are written and read only by device-scope atomic operations,
+// which are performed where every XCD sees them, so the exchange needs no
+// cache write-back or invalidation; the keys are the only data exchanged.
+// The slots alternate between two tables by launch parity, each launch
+// clearing the table the next one uses; the ticket counter only grows (4
+// arrivals per launch).  A slot that stays empty past a bounded spin marks
+// the launch aborted (every workgroup then leaves) and sets bit 1 of the
+// error flag.
+constexpr int SEL_G = 64;                  // workgroups per level >= 1 (one per luma strength)
+static_assert(SEL_G == 64, "one wave reads a level's slots, one lane each");
+constexpr int SEL_T = 512;                 // threads per workgroup (8 waves)
+constexpr int SEL_GRID = 1 + 3 * SEL_G;    // level 0 on workgroup 0
+constexpr int SEL_STEPS = 39;              // exchanged steps of level 3 (8 greedy + 32 refinements, less the first)
+constexpr int SEL_T2_MAX = 280;            // filter blocks for both tables in LDS
+constexpr int SEL_T1_MAX = 560;            // filter blocks for the chroma table in LDS
 
 struct FinishState {
-    int ly[8], lc[8];
-    int ys[8], us[8];
-    unsigned long long dist, best_cost;
-    int best_bits;
-    unsigned int ticket;
-    unsigned long long gain[4096];   // (j, k) gains of the current step, cleared after each step
-    unsigned long long base;         // sum of the blocks' best costs of the current step
-    int active[4096];      // active filter blocks (at most 4096 = 4096x4096 luma)
-    int nact;
+    unsigned int done;                     // levels finished (monotonic, 4 per launch)
+    unsigned int abort;                    // epoch of an aborted launch
+    unsigned long long slot[2][3][SEL_STEPS][SEL_G];   // published keys, bit 63 = present
+    int ly[4][8], lc[4][8];                // each level's final pairs
+    unsigned long long cost[4];
 };
 
-__device__ void fin_epilogue(FinishState *st, int nb, int bits, int end_level, int shift_after, unsigned long long lambda)
+__host__ __device__ inline size_t sel_lds_bytes(int nfb, int tables)
 {
-    if (end_level) {
-        const long long rate = (long long)(st->nact * bits + nb * 12) << 9;
-        const unsigned long long cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((st->dist * 16) << 7);
-        if (cost < st->best_cost) {
-            st->best_cost = cost;
-            st->best_bits = bits;
-            for (int i = 0; i < 8; i++) { st->ys[i] = st->ly[i]; st->us[i] = st->lc[i]; }
-        }
-    }
-    if (shift_after)
-        for (int i = 0; i + 1 < nb; i++) { st->ly[i] = st->ly[i + 1]; st->lc[i] = st->lc[i + 1]; }
+    const size_t base = ((size_t)nfb * 4 + 15) / 16 * 16 + (size_t)nfb * 8;
+    return base + (size_t)nfb * 64 * 4 * (size_t)tables;
 }
 
-// Active blocks (a non-skip 8x8 block inside the frame, as the MSE kernel
-// counts them), compacted in ascending order by one wave with ballots, and
-// the selection state.
-__global__ __launch_bounds__(64) void fin_init_kernel(const uint64_t *noskip, int nhfb, int nvfb, int mi_cols,
-                                                      int mi_rows, FinishState *st)
+template <int TABLES>
+__global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__restrict__ mse, const uint64_t *noskip,
+                                                            int nhfb, int nvfb, int mi_cols, int mi_rows,
+                                                            unsigned long long lambda, FinishState *st,
+                                                            unsigned int epoch, av1pf_cdef_decision *dec,
+                                                            uint8_t *fb_index, int *err)
 {
-    const int lane = threadIdx.x;
-    const int nfb = nhfb * nvfb;
-    int n = 0;
-    for (int base = 0; base < nfb; base += 64) {
-        const int fb = base + lane;
-        bool act = false;
-        if (fb < nfb) {
-            const int fbr = fb / nhfb, fbc = fb % nhfb;
-            const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
-            uint64_t valid = 0;
-            for (int r = 0; r < nvb; r += 2)
-                for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
-            const uint64_t m = noskip ? noskip[fb] : ~0ull;
-            act = (m & valid) != 0;
-        }
-        const uint64_t vote = __ballot(act);
-        const int before = __popcll(vote & ((1ull << lane) - 1));
-        if (act) st->active[n + before] = fb;
-        n += __popcll(vote);
-    }
-    if (lane == 0) {
-        st->nact = n;
-        st->best_cost = 1ull << 63;
-        st->best_bits = 0;
-        st->ticket = 0;
-        st->base = 0;
-        for (int i = 0; i < 8; i++) { st->ly[i] = st->lc[i] = 0; st->ys[i] = st->us[i] = 0; }
-    }
-    for (int i = lane; i < 4096; i += 64) st->gain[i] = 0;
-}
-
-__global__ __launch_bounds__(1024) void fin_first_kernel(const uint64_t *mse, int nfb, FinishState *st, int nb, int bits,
-                                                         int end_level, int shift_after, unsigned long long lambda)
-{
-    // 128 column sums (2 planes x 64 strengths), 8 threads per column over
-    // interleaved blocks, then a reduction in LDS
-    __shared__ unsigned long long part[8][128];
+    extern __shared__ __align__(16) char smem[];
+    __shared__ unsigned long long part[8][64];
     __shared__ unsigned long long sums[128];
-    const int t = threadIdx.x;
-    const int col = t & 127, sub = t >> 7;
-    const int plane = col >> 6, g = col & 63;
-    const int nact = st->nact;
-    unsigned long long s = 0;
-#pragma unroll 4
-    for (int i = sub; i < nact; i += 8) s += mse[((size_t)plane * nfb + st->active[i]) * 64 + g];
-    part[sub][col] = s;
+    __shared__ unsigned long long s_key;
+    __shared__ int ly[8], lc[8];
+    __shared__ int s_nact, s_bad, s_abort, s_wide;
+    __shared__ unsigned long long s_dist, s_sum;
+    __shared__ bool s_last;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nfb = nhfb * nvfb;
+    const int b = blockIdx.x;
+    const int level = b == 0 ? 0 : 1 + (b - 1) / SEL_G;
+    const int gw = b == 0 ? 0 : (b - 1) % SEL_G;
+    int *act = reinterpret_cast<int *>(smem);
+    unsigned long long *best = reinterpret_cast<unsigned long long *>(smem + ((size_t)nfb * 4 + 15) / 16 * 16);
+    uint32_t *ct = reinterpret_cast<uint32_t *>(best + nfb);    // chroma rows of the active blocks
+    uint32_t *lt = ct + (size_t)nfb * 64;                       // luma rows (TABLES == 2)
+    const uint64_t *Lm = mse, *Cm = mse + (size_t)nfb * 64;
+    // ---- active blocks (a non-skip 8x8 block inside the frame), ascending
+    if (t < 64) {
+        int n = 0;
+        for (int base = 0; base < nfb; base += 64) {
+            const int fb = base + lane;
+            bool a = false;
+            if (fb < nfb) {
+                const int fbr = fb / nhfb, fbc = fb % nhfb;
+                const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+                uint64_t valid = 0;
+                for (int r = 0; r < nvb; r += 2)
+                    for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
+                a = ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
+            }
+            const uint64_t vote = __ballot(a);
+            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = fb;
+            n += __popcll(vote);
+        }
+        if (lane == 0) {
+            s_nact = n;
+            s_bad = 0;
+            s_abort = 0;
+        }
+    }
+    // the next launch's slots start empty
+    if (b == 0)
+        for (int i = t; i < 3 * SEL_STEPS * SEL_G; i += SEL_T) (&st->slot[(epoch + 1) & 1][0][0][0])[i] = 0ull;
     __syncthreads();
-    if (t < 128) {
-        unsigned long long v = 0;
-        for (int q = 0; q < 8; q++) v += part[q][t];
-        sums[t] = v;
+    const int nact = s_nact;
+    // ---- the MSE rows of the active blocks in LDS
+    if (TABLES >= 1 && b != 0) {
+        for (int e = t; e < nact * 64; e += SEL_T) {
+            const int fb = act[e >> 6], g = e & 63;
+            const uint64_t c = Cm[(size_t)fb * 64 + g];
+            ct[e] = (uint32_t)c;
+            uint64_t l = 0;
+            if (TABLES == 2) {
+                l = Lm[(size_t)fb * 64 + g];
+                lt[e] = (uint32_t)l;
+            }
+            if ((c | l) >> 32) s_bad = 1;
+        }
     }
     __syncthreads();
-    if (t == 0) {
-        if (nact == 0) {
-            st->ly[0] = st->lc[0] = 0;
-            st->dist = 0;
-        } else {
-            int by = 0, bc = 0;
-            for (int k = 1; k < 64; k++) {
-                if (sums[k] < sums[by]) by = k;
-                if (sums[64 + k] < sums[64 + bc]) bc = k;
-            }
-            st->ly[0] = by;
-            st->lc[0] = bc;
-            st->dist = sums[by] + sums[64 + bc];
-        }
-        fin_epilogue(st, nb, bits, end_level, shift_after, lambda);
-    }
-}
-
-#define FIN_FBS 32   // active blocks per workgroup of the gain step
-
-// Grid (16, ceil(nfb / 32)): workgroup (x, y) scores the 256 pairs of luma
-// strengths 4x..4x+3 over active blocks 32y..32y+31, staged in LDS, and adds
-// its partial gains to the state's gain table; the last workgroup to finish
-// picks the first maximum in (j, k) order and clears the table for the next
-// step.
-__global__ __launch_bounds__(256) void fin_gain_kernel(const uint64_t *mse, int nfb, FinishState *st, int slot, int nb,
-                                                       int bits, int end_level, int shift_after, unsigned long long lambda)
-{
-    __shared__ unsigned long long cch[FIN_FBS][64];
-    __shared__ unsigned long long lch[FIN_FBS][4];
-    __shared__ unsigned long long bch[FIN_FBS];
-    __shared__ int fbs[FIN_FBS];
-    __shared__ unsigned long long rmax[256];
-    __shared__ int rpos[256];
-    __shared__ bool last;
-    const int t = threadIdx.x;
-    const int nact = st->nact;
-    const uint64_t *L = mse, *Cm = mse + (size_t)nfb * 64;
-    const int pair = blockIdx.x * 256 + t, j = pair >> 6, k = pair & 63;
-    const int j0 = blockIdx.x * 4, jq = j - j0;
-    const int c0 = blockIdx.y * FIN_FBS;
-    const int n = max(0, min(FIN_FBS, nact - c0));
-    if (n > 0) {
-        if (t < n) fbs[t] = st->active[c0 + t];
+    const bool ctab = TABLES >= 1 && b != 0 && !s_bad, ltab = TABLES == 2 && b != 0 && !s_bad;
+    auto luma = [&](int i, int g) -> unsigned long long {
+        return ltab ? (unsigned long long)lt[i * 64 + g] : Lm[(size_t)act[i] * 64 + g];
+    };
+    auto chroma = [&](int i, int k) -> unsigned long long {
+        return ctab ? (unsigned long long)ct[i * 64 + k] : Cm[(size_t)act[i] * 64 + k];
+    };
+    const int nb = 1 << level;
+    const int steps = level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: compute it once
+    const int j = gw;
+    int gstep = 0;
+    for (int s = 0; s < steps; s++) {
+        const int slot = s < nb ? s : nb - 1;
+        if (s >= nb && t == 0)   // refinement: drop the oldest pair
+            for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
         __syncthreads();
-        for (int e = t; e < n * 64; e += 256) {
-            const int i = e >> 6, kk = e & 63;
-            cch[i][kk] = Cm[(size_t)fbs[i] * 64 + kk];
-        }
-        if (t < n * 4) {
-            const int i = t >> 2, q = t & 3;
-            lch[i][q] = L[(size_t)fbs[i] * 64 + j0 + q];
-        }
-        if (t < n) {
-            const int fb = fbs[t];
-            unsigned long long b = L[(size_t)fb * 64 + st->ly[0]] + Cm[(size_t)fb * 64 + st->lc[0]];
-            for (int g = 1; g < slot; g++) {
-                const unsigned long long c = L[(size_t)fb * 64 + st->ly[g]] + Cm[(size_t)fb * 64 + st->lc[g]];
-                if (c < b) b = c;
+        if (slot == 0) {
+            // column sums: 4 threads per column over interleaved blocks
+            const int col = t & 127, sub = t >> 7;
+            unsigned long long cs = 0;
+            for (int i = sub; i < nact; i += 4) cs += col < 64 ? luma(i, col) : chroma(i, col - 64);
+            reinterpret_cast<unsigned long long *>(part)[sub * 128 + col] = cs;   // part as [4][128]
+            __syncthreads();
+            if (t < 128) {
+                const unsigned long long *pp = reinterpret_cast<const unsigned long long *>(part);
+                sums[t] = pp[t] + pp[128 + t] + pp[256 + t] + pp[384 + t];
             }
-            bch[t] = b;
-            if (blockIdx.x == 0) atomicAdd(&st->base, b);
+            __syncthreads();
+            if (t == 0) {
+                int by = 0, bc = 0;
+                for (int k = 1; k < 64; k++) {
+                    if (sums[k] < sums[by]) by = k;
+                    if (sums[64 + k] < sums[64 + bc]) bc = k;
+                }
+                ly[0] = nact ? by : 0;
+                lc[0] = nact ? bc : 0;
+                s_dist = nact ? sums[by] + sums[64 + bc] : 0ull;
+            }
+            __syncthreads();
+            continue;
         }
+        // best cost of every active block over the prior pairs, and their
+        // sum; the pair lists are copied to registers first so that all of a
+        // block's loads are in flight together
+        int py[8], pc[8];
+#pragma unroll
+        for (int g = 0; g < 8; g++) { py[g] = ly[g]; pc[g] = lc[g]; }
+        unsigned long long psum = 0;
+        if (t == 0) s_wide = 0;
         __syncthreads();
+        bool wide = false;
+        for (int i = t; i < nact; i += SEL_T) {
+            unsigned long long v[8];
+            if (ltab) {   // both rows in LDS: every lookup issued before any sum
+                uint32_t a[8], c[8];
+#pragma unroll
+                for (int g = 0; g < 8; g++) {
+                    a[g] = lt[i * 64 + py[g]];
+                    c[g] = ct[i * 64 + pc[g]];
+                }
+#pragma unroll
+                for (int g = 0; g < 8; g++) v[g] = g < slot ? (unsigned long long)a[g] + c[g] : ~0ull;
+            } else {
+#pragma unroll
+                for (int g = 0; g < 8; g++) v[g] = g < slot ? luma(i, py[g]) + chroma(i, pc[g]) : ~0ull;
+            }
+            unsigned long long m = v[0];
+#pragma unroll
+            for (int g = 1; g < 8; g++) m = v[g] < m ? v[g] : m;
+            best[i] = m;
+            psum += m;
+            wide |= (m >> 32) != 0;
+        }
+        if (wide) s_wide = 1;
+        for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
+        if (lane == 0) part[wv][0] = psum;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long v = 0;
+            for (int w = 0; w < 8; w++) v += part[w][0];
+            s_sum = v;
+        }
+        // savings of the pairs (j, lane) over this wave's eighth of the
+        // blocks, eight blocks' loads in flight at a time (a block whose
+        // a_ij is not below best_i has no room and saves nothing)
         unsigned long long gain = 0;
-        for (int i = 0; i < n; i++) {
-            const unsigned long long b = bch[i], yj = lch[i][jq];
-            if (yj < b) {
-                const unsigned long long room = b - yj, ck = cch[i][k];
-                if (ck < room) gain += room - ck;
+        if (ltab && !s_wide) {
+            int i = wv;
+            for (; i + 56 < nact; i += 64) {
+                uint32_t bi[8], a[8], c[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    bi[u] = (uint32_t)best[i + 8 * u];
+                    a[u] = lt[(i + 8 * u) * 64 + j];
+                    c[u] = ct[(i + 8 * u) * 64 + lane];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    gain += __builtin_elementwise_sub_sat(__builtin_elementwise_sub_sat(bi[u], a[u]), c[u]);
             }
-        }
-        if (gain) atomicAdd(&st->gain[pair], gain);
-    }
-    __syncthreads();
-    if (t == 0) {
-        __threadfence();
-        last = atomicAdd(&st->ticket, 1u) == gridDim.x * gridDim.y - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // first maximum over the 4096 pairs; the table is read and cleared with
-    // atomic exchanges (the partial sums were added at device scope)
-    unsigned long long *G = st->gain;
-    unsigned long long m = atomicExch(&G[t], 0ull);
-    int p = t;
-    for (int q = 1; q < 16; q++) {
-        const unsigned long long v = atomicExch(&G[t + 256 * q], 0ull);
-        if (v > m) { m = v; p = t + 256 * q; }
-    }
-    rmax[t] = m;
-    rpos[t] = p;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) {
-            const unsigned long long a = rmax[t], b = rmax[t + o];
-            if (b > a || (b == a && rpos[t + o] < rpos[t])) { rmax[t] = b; rpos[t] = rpos[t + o]; }
+            for (; i < nact; i += 8)
+                gain += __builtin_elementwise_sub_sat(
+                    __builtin_elementwise_sub_sat((uint32_t)best[i], lt[i * 64 + j]), ct[i * 64 + lane]);
+        } else {
+            auto save = [&](unsigned long long bi, unsigned long long a, unsigned long long c) {
+                const unsigned long long room = a < bi ? bi - a : 0ull;
+                return c < room ? room - c : 0ull;
+            };
+            int i = wv;
+            for (; i + 56 < nact; i += 64) {
+                unsigned long long bi[8], a[8], c[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    bi[u] = best[i + 8 * u];
+                    a[u] = luma(i + 8 * u, j);
+                    c[u] = chroma(i + 8 * u, lane);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) gain += save(bi[u], a[u], c[u]);
+            }
+            for (; i < nact; i += 8) gain += save(best[i], luma(i, j), chroma(i, lane));
         }
         __syncthreads();
+        part[wv][lane] = gain;
+        __syncthreads();
+        if (wv == 0) {
+            unsigned long long g = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) g += part[w][lane];
+            unsigned long long key = (g << 12) | (unsigned long long)(4095 - (j * 64 + lane));
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long other = __shfl_xor(key, o);
+                key = other > key ? other : key;
+            }
+            if (lane == 0) atomicExch(&st->slot[epoch & 1][level - 1][gstep][gw], key | (1ull << 63));
+        }
+        // every workgroup of the level reads the level's slots of this step
+        if (t < SEL_G) {
+            unsigned long long v;
+            for (int spin = 0;; spin++) {
+                v = atomicOr(&st->slot[epoch & 1][level - 1][gstep][t], 0ull);
+                if (v >> 63) break;
+                if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch ||
+                    spin > (1 << 21)) {
+                    atomicExch(&st->abort, epoch);
+                    atomicOr(err, 2);
+                    s_abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            unsigned long long key = v & ~(1ull << 63);
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long other = __shfl_xor(key, o);
+                key = other > key ? other : key;
+            }
+            if (t == 0) s_key = key;
+        }
+        __syncthreads();
+        if (s_abort) break;
+        if (t == 0) {
+            const unsigned long long key = s_key;
+            const int pair = 4095 - (int)(key & 4095ull);
+            ly[slot] = pair >> 6;
+            lc[slot] = pair & 63;
+            s_dist = s_sum - (key >> 12);
+        }
+        gstep++;
+        __syncthreads();
     }
+    // ---- level result, then the last level to finish decides
+    if (b != 0 && gw != 0) return;
     if (t == 0) {
-        const int pb = rpos[0];
-        st->ly[slot] = pb >> 6;
-        st->lc[slot] = pb & 63;
-        st->dist = atomicExch(&st->base, 0ull) - rmax[0];
-        st->ticket = 0;
-        fin_epilogue(st, nb, bits, end_level, shift_after, lambda);
+        if (!s_abort) {
+            const long long rate = (long long)(nact * level + nb * 12) << 9;
+            st->cost[level] =
+                (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((s_dist * 16) << 7);
+            for (int i = 0; i < 8; i++) {
+                st->ly[level][i] = i < nb ? ly[i] : 0;
+                st->lc[level][i] = i < nb ? lc[i] : 0;
+            }
+        }
+        __threadfence();
+        s_last = atomicAdd(&st->done, 1u) == 4u * epoch - 1u;
     }
-}
-
-// One workgroup: blocks are zeroed, then the active ones get their index.
-__global__ __launch_bounds__(256) void fin_assign_kernel(const uint64_t *mse, int nfb, FinishState *st,
-                                                         av1pf_cdef_decision *dec, uint8_t *fb_index)
-{
-    const int nb = 1 << st->best_bits;
-    for (int fb = threadIdx.x; fb < nfb; fb += 256) fb_index[fb] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < st->nact; i += 256) {
-        const int fb = st->active[i];
+    if (!s_last) return;
+    __threadfence();
+    if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
+    __shared__ int ys[8], us[8], bits;
+    if (t == 0) {
+        int bb = 0;
+        for (int l = 1; l < 4; l++)
+            if (st->cost[l] < st->cost[bb]) bb = l;
+        bits = bb;
+        for (int i = 0; i < 8; i++) { ys[i] = st->ly[bb][i]; us[i] = st->lc[bb][i]; }
+    }
+    __syncthreads();
+    const int nbb = 1 << bits;
+    for (int fb = t; fb < nfb; fb += SEL_T) fb_index[fb] = 0;
+    __syncthreads();
+    for (int i = t; i < nact; i += SEL_T) {
+        const int fb = act[i];
         unsigned long long bm = 1ull << 63;
         int bg = 0;
-        for (int g = 0; g < nb; g++) {
-            const unsigned long long c = mse[(size_t)fb * 64 + st->ys[g]] + mse[((size_t)nfb + fb) * 64 + st->us[g]];
+        for (int g = 0; g < nbb; g++) {
+            const unsigned long long c = Lm[(size_t)fb * 64 + ys[g]] + Cm[(size_t)fb * 64 + us[g]];
             if (c < bm) { bm = c; bg = g; }
         }
         fb_index[fb] = (uint8_t)bg;
     }
-    if (threadIdx.x == 0) {
-        dec->cdef_bits = st->best_bits;
+    if (t == 0) {
+        dec->cdef_bits = bits;
         for (int i = 0; i < 8; i++) {
-            dec->y_strength[i] = i < nb ? kLevel1[st->ys[i]] : 0;
-            dec->uv_strength[i] = i < nb ? kLevel1[st->us[i]] : 0;
+            dec->y_strength[i] = i < nbb ? kLevel1[ys[i]] : 0;
+            dec->uv_strength[i] = i < nbb ? kLevel1[us[i]] : 0;
         }
     }
 }
@@ -516,28 +623,24 @@ size_t finish_state_bytes() { return sizeof(FinishState); }
 size_t small_state_bytes() { return 1024; }
 
 hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,
-                              uint64_t lambda, void *state, av1pf_cdef_decision *dec, uint8_t *fb_index, hipStream_t s)
+                              uint64_t lambda, void *state, unsigned int epoch, av1pf_cdef_decision *dec,
+                              uint8_t *fb_index, int *err, hipStream_t s)
 {
     FinishState *st = static_cast<FinishState *>(state);
+    unsigned long long lam = lambda;
     const int nfb = nhfb * nvfb;
-    hipLaunchKernelGGL(fin_init_kernel, dim3(1), dim3(64), 0, s, noskip, nhfb, nvfb, mi_cols, mi_rows, st);
-    for (int bits = 0; bits <= 3; bits++) {
-        const int nb = 1 << bits;
-        const int steps = 5 * nb;   // nb first choices, 4 * nb refinements
-        for (int k = 0; k < steps; k++) {
-            const int slot = k < nb ? k : nb - 1;
-            const int end_level = k == steps - 1;
-            const int shift_after = k + 1 >= nb && k + 1 < steps;   // the next step is a refinement
-            if (slot == 0)
-                hipLaunchKernelGGL(fin_first_kernel, dim3(1), dim3(1024), 0, s, mse, nfb, st, nb, bits, end_level,
-                                   shift_after, (unsigned long long)lambda);
-            else
-                hipLaunchKernelGGL(fin_gain_kernel, dim3(16, (nfb + FIN_FBS - 1) / FIN_FBS), dim3(256), 0, s, mse, nfb,
-                                   st, slot, nb, bits, end_level, shift_after, (unsigned long long)lambda);
-        }
-    }
-    hipLaunchKernelGGL(fin_assign_kernel, dim3(1), dim3(256), 0, s, mse, nfb, st, dec, fb_index);
-    return hipGetLastError();
+    const int tables = nfb <= SEL_T2_MAX ? 2 : nfb <= SEL_T1_MAX ? 1 : 0;
+    const void *fn = tables == 2 ? (const void *)cdef_select_kernel<2>
+                   : tables == 1 ? (const void *)cdef_select_kernel<1>
+                                 : (const void *)cdef_select_kernel<0>;
+    const size_t lds = sel_lds_bytes(nfb, tables);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    void *args[] = { (void *)&mse, (void *)&noskip, (void *)&nhfb, (void *)&nvfb, (void *)&mi_cols, (void *)&mi_rows,
+                     (void *)&lam, (void *)&st, (void *)&epoch, (void *)&dec, (void *)&fb_index, (void *)&err };
+    // a cooperative launch: every workgroup resident at once (a workgroup
+    // waits for the slots of the other workgroups of its level)
+    return hipLaunchCooperativeKernel(fn, dim3(SEL_GRID), dim3(SEL_T), args, (unsigned int)lds, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -207,6 +207,46 @@ def test_gpu_select_matches_host_finish(case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", [(64, 64, 3, 0.0), (640, 360, 5, 0.2), (1920, 1080, 40, 0.1), (3840, 2160, 2, 0.0),
+                                  (4096, 4096, 1000, 0.3)], ids=lambda c: f"{c[0]}x{c[1]}_v{c[2]}_skip{c[3]}")
+def test_gpu_select_random_tables_with_ties(case):
+    """Device selection = host finish on random tables over a small value
+    range (many equal gains and sums: the first-maximum / first-minimum
+    tie rules decide), up to the 4096-block limit, several calls in a row on
+    one handle (the selection state carries over between launches)."""
+    import torch
+    from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_search_finish
+    w, h, vmax, pskip = case
+    rng = np.random.default_rng(w + vmax)
+    rec = synth.FrameData(w, h, 8, 1, 1)
+    nhfb, nvfb = (w + 63) // 64, (h + 63) // 64
+    nfb = nhfb * nvfb
+    noskip = np.where(rng.random(nfb) < pskip, 0, ~0).astype(np.int64).view(np.uint64)
+    act = active_blocks(rec, noskip)
+    print(f"select random tables {w}x{h} ({nfb} blocks)", flush=True)
+    r = PostFilterRunner()
+    try:
+        dr = DeviceFrame(rec)
+        ns = torch.from_numpy(noskip.view(np.int64)).cuda()
+        dec = torch.zeros(24, dtype=torch.uint8, device="cuda")
+        idx = torch.full((nfb,), 255, dtype=torch.uint8, device="cuda")
+        for rep, lam in enumerate((0, 3000, 1 << 24)):
+            table = rng.integers(0, vmax + 1, size=(2, nfb, 64)).astype(np.uint64)
+            mse = torch.from_numpy(table.view(np.int64)).cuda()
+            r.cdef_search_select(dr, ns.data_ptr(), mse.data_ptr(), lam, dec.data_ptr(), idx.data_ptr())
+            r.sync()
+            torch.cuda.synchronize()
+            bits, ys, uvs, fbi = cdef_search_finish(table, 1 - act, lam)
+            d = dec.cpu().numpy()
+            assert int(d[:4].view(np.int32)[0]) == bits, f"call {rep} lambda {lam}"
+            np.testing.assert_array_equal(d[4:12], ys)
+            np.testing.assert_array_equal(d[12:20], uvs)
+            np.testing.assert_array_equal(idx.cpu().numpy(), fbi)
+    finally:
+        r.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2), (2048, 1152, 120, 3),
                                   (2304, 1856, 200, 4)])
 def test_gpu_shortlist_equals_masked_host_finish(case):
