@@ -68,6 +68,12 @@ void *noalloc_scratch(int device, std::string *err)
 
 namespace {
 constexpr int kSlots = 4;
+// Film-grain template sets (and side streams) the calls rotate through: one
+// frame's templates (a serial chain of about 220 us for 1080p: luma, then
+// chroma) build while the previous frame filters.  Two: the process has four
+// hardware queues (GPU_MAX_HW_QUEUES), and streams beyond them share queues,
+// which serialises their work with the main stream's.
+constexpr int kTmplSets = 2;
 constexpr size_t kSlotBytes = size_t(1) << 20;   // ~400 frames of descriptors
 // Stages whose work distribution is tuned: CDEF, LR, film-grain apply.
 // Mode = units per workgroup (1, 2, 4) or 0 = persistent workgroups on a queue.
@@ -103,15 +109,15 @@ struct av1pf_ctx {
     size_t scratch_cap = 0;
     uint16_t *lr_bnd = nullptr;   // LR stripe context rows
     size_t lr_bnd_cap = 0;
-    // film-grain templates (3 x 73 x 82 per frame) and block offsets, double
-    // buffered: the side stream fills one pair for call N+1 while the grain
-    // apply of call N reads the other
-    int16_t *templates[2] = {};
-    size_t templates_cap[2] = {};
-    uint8_t *grain_offs[2] = {};
-    size_t grain_offs_cap[2] = {};
-    hipEvent_t tmpl_free[2] = {};  // grain apply done with pair p
-    bool tmpl_free_used[2] = {};
+    // film-grain templates (3 x 73 x 82 per frame) and block offsets in
+    // kTmplSets rotating sets: the side streams fill the sets of the next
+    // calls while the grain apply of this call reads its own
+    int16_t *templates[kTmplSets] = {};
+    size_t templates_cap[kTmplSets] = {};
+    uint8_t *grain_offs[kTmplSets] = {};
+    size_t grain_offs_cap[kTmplSets] = {};
+    hipEvent_t tmpl_free[kTmplSets] = {};  // grain apply done with set p
+    bool tmpl_free_used[kTmplSets] = {};
     int tpar = 0;
     hipEvent_t uploaded = nullptr;
     void *finish_state = nullptr;  // device CDEF strength selection state (zeroed once)
@@ -120,7 +126,8 @@ struct av1pf_ctx {
     size_t learn_cap = 0;
     void *small_state = nullptr;   // shortlist selection: per-level results and ticket (zeroed once)
     int *dev_err = nullptr;       // device-side validation flag
-    hipStream_t side = nullptr;   // film-grain template stream
+    hipStream_t sides[kTmplSets] = {};    // film-grain template stream of each template set
+    hipStream_t side = nullptr;   // sides[0]: graphs, allocation-free calls, the learned search's LR
     hipEvent_t join = nullptr;
     hipEvent_t learn_fork = nullptr, learn_join = nullptr;   // learned search: LR search on the side stream
     // shape classes: tuned work distributions and captured graphs
@@ -831,7 +838,7 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         if ((rc = ensure(c, (void **)&c->scratch, &c->scratch_cap, std::max<size_t>(P.scratch, 256)))) return rc;
         if ((rc = ensure(c, (void **)&c->lr_bnd, &c->lr_bnd_cap, std::max<size_t>(P.bnd * 2, 256)))) return rc;
         if (side) {
-            c->tpar ^= 1;
+            c->tpar = (c->tpar + 1) % kTmplSets;
             if ((rc = ensure(c, (void **)&c->templates[par], &c->templates_cap[par],
                              std::max<size_t>(P.ngrain * tsz, 256))))
                 return rc;
@@ -941,24 +948,27 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             }
             HIPCHK(c, hipEventRecord(ev_join, c->side));
         } else if (side) {
-            // Film-grain templates depend only on the grain parameters: the side
+            // Film-grain templates depend only on the grain parameters: a side
             // stream uploads the descriptors and builds them without waiting for
             // the previous call's filtering, only for the grain apply that last
             // read this template pair (inside a graph the replay is ordered).
+            // Each template pair has its own side stream, so the templates of
+            // consecutive calls build concurrently.
+            hipStream_t ss = capture ? c->side : c->sides[par];
             if (capture) {
                 HIPCHK(c, hipEventRecord(c->g_fork, st));
-                HIPCHK(c, hipStreamWaitEvent(c->side, c->g_fork, 0));
+                HIPCHK(c, hipStreamWaitEvent(ss, c->g_fork, 0));
             } else if (c->tmpl_free_used[par]) {
-                HIPCHK(c, hipStreamWaitEvent(c->side, c->tmpl_free[par], 0));
+                HIPCHK(c, hipStreamWaitEvent(ss, c->tmpl_free[par], 0));
             }
-            HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, c->side));
-            HIPCHK(c, hipEventRecord(ev_up, c->side));
+            HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, ss));
+            HIPCHK(c, hipEventRecord(ev_up, ss));
             HIPCHK(c, hipStreamWaitEvent(st, ev_up, 0));
             {
-                StageTimer t(c, ST_GRAIN_T, c->side, 0);
-                HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), c->side));
+                StageTimer t(c, ST_GRAIN_T, ss, 0);
+                HIPCHK(c, launch_grain_templates(dd, segp(7), nseg(7), segp(8), nseg(8), ss));
             }
-            HIPCHK(c, hipEventRecord(ev_join, c->side));
+            HIPCHK(c, hipEventRecord(ev_join, ss));
         } else {
             HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
         }
@@ -1104,10 +1114,7 @@ int av1pf_create(av1pf_ctx **out, int device)
     av1pf_ctx *c = new av1pf_ctx();
     c->device = device;
     bool ok = hipMalloc(&c->dev_err, sizeof(int)) == hipSuccess &&
-              hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&c->uploaded, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&c->tmpl_free[0], hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&c->tmpl_free[1], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->learn_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->learn_join, hipEventDisableTiming) == hipSuccess &&
@@ -1115,6 +1122,10 @@ int av1pf_create(av1pf_ctx **out, int device)
               hipEventCreateWithFlags(&c->g_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->g_up, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&c->g_join, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kTmplSets; i++)
+        ok = hipStreamCreateWithFlags(&c->sides[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->tmpl_free[i], hipEventDisableTiming) == hipSuccess;
+    if (ok) c->side = c->sides[0];
     // descriptor slots sized for large batches up front, so that no call in
     // steady state allocates
     for (int i = 0; ok && i < kSlots; i++) {
@@ -1163,12 +1174,13 @@ void av1pf_destroy(av1pf_ctx *c)
     if (c->cap) hipStreamDestroy(c->cap);
     for (hipEvent_t e : { c->g_fork, c->g_up, c->g_join })
         if (e) hipEventDestroy(e);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < kTmplSets; i++) {
         if (c->tmpl_free[i]) hipEventDestroy(c->tmpl_free[i]);
         if (c->templates[i]) hipFree(c->templates[i]);
         if (c->grain_offs[i]) hipFree(c->grain_offs[i]);
     }
-    if (c->side) hipStreamDestroy(c->side);
+    for (hipStream_t ss : c->sides)
+        if (ss) hipStreamDestroy(ss);
     for (auto &r : c->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : c->pool) hipEventDestroy(e);
     delete c;
@@ -1261,7 +1273,7 @@ int av1pf_sync(av1pf_ctx *c, void *stream)
     if (!c) return AV1PF_ERR_INVALID_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-    HIPCHK(c, hipStreamSynchronize(c->side));
+    for (hipStream_t ss : c->sides) HIPCHK(c, hipStreamSynchronize(ss));
     int flag = 0;
     HIPCHK(c, hipMemcpy(&flag, c->dev_err, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) {

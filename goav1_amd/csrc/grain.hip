@@ -50,17 +50,52 @@ __device__ __forceinline__ uint16_t lfsr_step(uint16_t r)
     return (uint16_t)((r >> 1) | (bit << 15));
 }
 
-// Gaussian fill of one template (rows [0,h), cols [0,w)) by a 64-lane wave;
-// lane r starts from the LFSR state at the start of row r (host-computed).
-__device__ __forceinline__ void gauss_fill(int16_t *T, const uint16_t *row_seed, int w, int h, int shift)
+// Gaussian fill of one template (rows [0,h), cols [0,w)) by the workgroup;
+// thread r starts from the LFSR state at the start of row r (host-computed)
+// and reads the Gaussian sequence from its copy in LDS (gz), eight lookups
+// in flight at a time.
+__device__ __forceinline__ void gauss_fill(int16_t *T, const int16_t *gz, const uint16_t *row_seed, int w, int h,
+                                           int shift)
 {
-    for (int row = threadIdx.x; row < h; row += 64) {
+    for (int row = threadIdx.x; row < h; row += blockDim.x) {
         uint16_t s = row_seed[row];
-        for (int x = 0; x < w; x++) {
+        int x = 0;
+        for (; x + 8 <= w; x += 8) {
+            int v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                s = lfsr_step(s);
+                v[k] = gz[s >> 5];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) T[row * GW + x + k] = (int16_t)r2i(v[k], shift);
+        }
+        for (; x < w; x++) {
             s = lfsr_step(s);
-            T[row * GW + x] = (int16_t)r2i(kGauss[s >> 5], shift);
+            T[row * GW + x] = (int16_t)r2i(gz[s >> 5], shift);
         }
     }
+}
+
+// Template kernels run a workgroup of GT_THREADS: every thread shares the
+// fills and copies, and the first wave runs the autoregressive wavefront (the
+// other waves walk the same loop with no live rows, so that the passes'
+// barriers are reached by all of them).
+constexpr int GT_THREADS = 256;
+
+// The Gaussian sequence and (luma kernel) the three scaling tables in LDS.
+__device__ __forceinline__ void stage_tables(int16_t *gz)
+{
+    for (int i = threadIdx.x; i < 2048; i += GT_THREADS) gz[i] = kGauss[i];
+}
+
+// A template (GH x GW int16) between LDS and device memory, two samples per
+// 32-bit access (both ends are 4-byte aligned).
+__device__ __forceinline__ void tmpl_store(gptr<int16_t> out, const int16_t *T)
+{
+    const auto o = (gptr<uint32_t>)out;
+    const uint32_t *t = reinterpret_cast<const uint32_t *>(T);
+    for (int i = threadIdx.x; i < GH * GW / 2; i += GT_THREADS) o[i] = t[i];
 }
 
 // Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 is overwritten by
@@ -157,7 +192,7 @@ __device__ void ar_lag0(int16_t *T, const int16_t *L, int w, int h, int c0, int 
                         int ssy)
 {
     const int cols = w - 6;
-    for (int i = threadIdx.x; i < (h - 3) * cols; i += 64) {
+    for (int i = threadIdx.x; i < (h - 3) * cols; i += blockDim.x) {
         const int y = 3 + i / cols, x = 3 + i % cols;
         const int lx = ((x - 3) << ssx) + 3, ly = ((y - 3) << ssy) + 3;
         int a = L[ly * GW + lx];
@@ -186,21 +221,26 @@ __device__ void ar_template(int16_t *T, const int16_t *L, int w, int h, const in
     }
 }
 
-__device__ void expand_scaling(const GrainDesc &G, int bd);
+__device__ void expand_scaling(const GrainDesc &G, const uint8_t (*luts)[256], int bd);
 
-__global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
+__global__ __launch_bounds__(GT_THREADS) void grain_luma_template_kernel(const FrameDesc *__restrict__ descs,
+                                                                         const Seg *__restrict__ segs)
 {
-    __shared__ int16_t T[GH * GW];
-    // one wave on a serial dependency chain shares its SIMD with the streaming
-    // kernels of the main stream: let it issue first
+    __shared__ __align__(16) int16_t T[GH * GW];
+    __shared__ int16_t gz[2048];
+    __shared__ uint8_t luts[3][256];
+    // a serial dependency chain shares its SIMD with the streaming kernels of
+    // the main stream: let it issue first
     __builtin_amdgcn_s_setprio(3);
     const FrameDesc &D = descs[segs[blockIdx.x].frame];
     const GrainDesc &G = D.gr;
     const int bd = D.v[VIEW_A].bd;
-    for (int i = threadIdx.x; i < GH * GW; i += 64) T[i] = 0;
+    stage_tables(gz);
+    for (int i = threadIdx.x; i < 3 * 256; i += GT_THREADS) luts[i >> 8][i & 255] = G.luts[i >> 8][i & 255];
+    for (int i = threadIdx.x; i < GH * GW; i += GT_THREADS) T[i] = 0;
     // per-block random offsets: one LFSR per 32-line stripe, one draw per
     // 32-column block (apply.go:43-70, random.go:26-48)
-    for (int r = threadIdx.x; r < G.orows; r += 64) {
+    for (int r = threadIdx.x; r < G.orows; r += GT_THREADS) {
         uint16_t reg = G.g.seed;
         reg ^= (uint16_t)(((r * 37 + 178) & 255) << 8);
         reg ^= (uint16_t)((r * 173 + 105) & 255);
@@ -210,10 +250,10 @@ __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc
             o[c] = (uint8_t)(reg >> 8);
         }
     }
-    expand_scaling(G, bd);
     __syncthreads();
+    expand_scaling(G, luts, bd);
     if (G.g.num_y_points) {
-        gauss_fill(T, G.row_seed[0], GW, GH, 12 - bd + G.g.grain_scale_shift);
+        gauss_fill(T, gz, G.row_seed[0], GW, GH, 12 - bd + G.g.grain_scale_shift);
         __syncthreads();
         const int gmin = -(1 << (bd - 1)), gmax = (1 << (bd - 1)) - 1;
         const int8_t *cf = G.g.ar_coeffs_y;
@@ -226,14 +266,15 @@ __global__ __launch_bounds__(64) void grain_luma_template_kernel(const FrameDesc
         }
     }
     __syncthreads();
-    const auto tout = gmem(G.templates);
-    for (int i = threadIdx.x; i < GH * GW; i += 64) tout[i] = T[i];
+    tmpl_store(gmem(G.templates), T);
 }
 
-__global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs)
+__global__ __launch_bounds__(GT_THREADS) void grain_chroma_template_kernel(const FrameDesc *__restrict__ descs,
+                                                                           const Seg *__restrict__ segs)
 {
-    __shared__ int16_t T[GH * GW];
-    __shared__ int16_t L[GH * GW];
+    __shared__ __align__(16) int16_t T[GH * GW];
+    __shared__ __align__(16) int16_t L[GH * GW];
+    __shared__ int16_t gz[2048];
     __builtin_amdgcn_s_setprio(3);
     const Seg sg = segs[blockIdx.x];
     const FrameDesc &D = descs[sg.frame];
@@ -243,12 +284,17 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
     const int pl = sg.plane;
     const int cw = ssx ? 44 : GW, ch = ssy ? 38 : GH;
     const bool luma_term = G.g.num_y_points != 0;
-    for (int i = threadIdx.x; i < GH * GW; i += 64) {
-        T[i] = 0;
-        L[i] = luma_term ? gmem(G.templates)[i] : (int16_t)0;
+    stage_tables(gz);
+    {
+        const auto lin = (gptr<const uint32_t>)gmem(G.templates);
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(L), *t32 = reinterpret_cast<uint32_t *>(T);
+        for (int i = threadIdx.x; i < GH * GW / 2; i += GT_THREADS) {
+            t32[i] = 0u;
+            l32[i] = luma_term ? lin[i] : 0u;
+        }
     }
     __syncthreads();
-    gauss_fill(T, G.row_seed[pl], cw, ch, 12 - bd + G.g.grain_scale_shift);
+    gauss_fill(T, gz, G.row_seed[pl], cw, ch, 12 - bd + G.g.grain_scale_shift);
     __syncthreads();
     const int lag = G.g.ar_coeff_lag;
     if (lag || luma_term) {
@@ -263,14 +309,13 @@ __global__ __launch_bounds__(64) void grain_chroma_template_kernel(const FrameDe
         }
     }
     __syncthreads();
-    const auto out = gmem(G.templates) + pl * GH * GW;
-    for (int i = threadIdx.x; i < GH * GW; i += 64) out[i] = T[i];
+    tmpl_store(gmem(G.templates) + pl * GH * GW, T);
 }
 
 hipError_t launch_grain_templates(const FrameDesc *d, const Seg *luma, int nl, const Seg *chroma, int nc, hipStream_t s)
 {
-    if (nl) hipLaunchKernelGGL(grain_luma_template_kernel, dim3(nl), dim3(64), 0, s, d, luma);
-    if (nc) hipLaunchKernelGGL(grain_chroma_template_kernel, dim3(nc), dim3(64), 0, s, d, chroma);
+    if (nl) hipLaunchKernelGGL(grain_luma_template_kernel, dim3(nl), dim3(GT_THREADS), 0, s, d, luma);
+    if (nc) hipLaunchKernelGGL(grain_chroma_template_kernel, dim3(nc), dim3(GT_THREADS), 0, s, d, chroma);
     return hipGetLastError();
 }
 
@@ -295,11 +340,13 @@ __device__ __forceinline__ int blendw(int prev, int cur, int wp, int wc, int bd)
 // Scaling function of every sample value (1 << bd entries per plane), from the
 // 256-entry piecewise-linear table with the 10/12-bit interpolation: the apply
 // then looks a sample's scale up with one LDS byte read.
-__device__ void expand_scaling(const GrainDesc &G, int bd)
+__device__ void expand_scaling(const GrainDesc &G, const uint8_t (*luts)[256], int bd)
 {
     const auto out = gmem(G.scale);
-    for (int pl = 0; pl < 3; pl++)
-        for (int i = threadIdx.x; i < (1 << bd); i += 64) out[pl * GR_SCALE + i] = (uint8_t)scale_lut(G.luts[pl], i, bd);
+    for (int i = threadIdx.x; i < 3 << bd; i += GT_THREADS) {
+        const int pl = i >> bd, v = i & ((1 << bd) - 1);
+        out[pl * GR_SCALE + v] = (uint8_t)scale_lut(luts[pl], v, bd);
+    }
 }
 
 // One task of the apply: 16 rows of one plane of a block, lane (r, g) owning
