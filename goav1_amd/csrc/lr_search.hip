@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
                     if (y >= L.end) break;
                     const int32_t uu = (int32_t)O[y * LR_TW + x] << 4;
                     const int64_t d0 = r0 ? f0[j] - uu : 0, d1 = r1 ? f1[j] - uu : 0;
-                    const int64_t ee = (((int64_t)srcT[y * 64 + x] << 4) - uu) << 7;
+                    const int64_t ee = (((int64_t)srcT[y * 64 + x] << 4) - uu) * 128;
                     st[0] += d0 * d0; st[1] += d0 * d1; st[2] += d1 * d1; st[3] += d0 * ee; st[4] += d1 * ee;
                 }
             }

@@ -14,7 +14,8 @@ import numpy as np
 from goav1_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liborc.so")
+# ORC_LIB selects another build of the same sources (the sanitizer build)
+_LIB = os.environ.get("ORC_LIB") or os.path.join(_HERE, "liborc.so")
 _lib = None
 
 

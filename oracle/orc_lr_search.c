@@ -413,7 +413,7 @@ int orc_lr_search_exhaustive(const orc_frame *src, const orc_frame *rec, int us,
                             const int32_t uu = (int32_t)org[(size_t)y * bs + x] << 4;
                             const int64_t d0 = sgr_radius_of(set, 0) ? f0[y * w + x] - uu : 0;
                             const int64_t d1 = sgr_radius_of(set, 1) ? f1[y * w + x] - uu : 0;
-                            const int64_t ee = (((int64_t)S[(size_t)y * ss + x] << 4) - uu) << 7;
+                            const int64_t ee = (((int64_t)S[(size_t)y * ss + x] << 4) - uu) * 128;
                             H00 += d0 * d0; H01 += d0 * d1; H11 += d1 * d1; C0 += d0 * ee; C1 += d1 * ee;
                         }
                     sgr_fit(set, H00, H01, H11, C0, C1, cxqd[nc]);
