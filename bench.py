@@ -227,11 +227,7 @@ def batch_leg(runner, steps, nframes):
     def copies():
         row = []
         for b in base:
-            c = DeviceFrame.__new__(DeviceFrame)
-            c.meta, c.bd = b.meta, b.bd
-            c.planes = [t.clone() for t in b.planes]
-            c.abi = c._abi()
-            row.append(c)
+            row.append(b.clone())
         return row
 
     def run(nsteps, profile=False, per_frame=False):
@@ -313,11 +309,7 @@ def superres_leg(runner, steps, nframes=8):
         for _ in range(nsteps):
             row = []
             for b in base:
-                c = DeviceFrame.__new__(DeviceFrame)
-                c.meta, c.bd = b.meta, b.bd
-                c.planes = [t.clone() for t in b.planes]
-                c.abi = c._abi()
-                row.append(c)
+                row.append(b.clone())
             rows.append(row)
         torch.cuda.synchronize()
         if profile:
@@ -425,11 +417,7 @@ def main():
         for _ in range(n):
             row = []
             for b in base:
-                c = DeviceFrame.__new__(DeviceFrame)
-                c.meta, c.bd = b.meta, b.bd
-                c.planes = [t.clone() for t in b.planes]
-                c.abi = c._abi()
-                row.append(c)
+                row.append(b.clone())
             rows.append(row)
         torch.cuda.synchronize()
         return rows

@@ -154,6 +154,8 @@ struct av1pf_ctx {
     int slot = 0;
     // stage-level LR: boundaries saved by av1pf_lr_save_boundaries for this frame
     const void *bnd_frame = nullptr;
+    // arenas replaced by larger ones while queued work may still read them
+    std::vector<void *> retired;
     // device address of the allocation-free entry point's scratch (nullptr: unavailable)
     void *noalloc = nullptr;
     std::string noalloc_err;
@@ -767,9 +769,12 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
 int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
 {
     if (need <= *cap) return AV1PF_OK;
-    HIPCHK(c, hipDeviceSynchronize());   // previous work may still use the old arena
-    if (*buf) HIPCHK(c, hipFree(*buf));
+    // work this handle queued earlier may still read the old arena: it is
+    // retired, not freed, and released once the handle's streams are idle
+    // (av1pf_sync, av1pf_destroy), so growth waits on no stream at all
+    if (*buf) c->retired.push_back(*buf);
     *buf = nullptr;
+    *cap = 0;
     need = need + need / 4;
     HIPCHK(c, hipMalloc(buf, need));
     *cap = need;
@@ -1154,6 +1159,7 @@ void av1pf_destroy(av1pf_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     hipDeviceSynchronize();
+    for (void *p : c->retired) hipFree(p);
     if (c->scratch) hipFree(c->scratch);
     if (c->lr_bnd) hipFree(c->lr_bnd);
     if (c->dev_err) hipFree(c->dev_err);
@@ -1274,6 +1280,8 @@ int av1pf_sync(av1pf_ctx *c, void *stream)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     for (hipStream_t ss : c->sides) HIPCHK(c, hipStreamSynchronize(ss));
+    for (void *p : c->retired) HIPCHK(c, hipFree(p));   // nothing of this handle is in flight
+    c->retired.clear();
     int flag = 0;
     HIPCHK(c, hipMemcpy(&flag, c->dev_err, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) {

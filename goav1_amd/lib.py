@@ -120,30 +120,71 @@ def _torch():
 
 
 class DeviceFrame:
-    """A frame resident in device memory in the ABI's sample layout."""
+    """A frame resident in device memory in the ABI's sample layout.
+
+    Each plane lives inside a larger device buffer: guard bands of at least
+    4 KiB and two rows (a multiple of 256 bytes, so the plane keeps the
+    buffer's alignment) sit before and after it, filled with a canary value.
+    No stage may write outside the planes it is given; check_canaries()
+    reports any sample of a band that changed, and to_numpy() calls it, so
+    every comparison of a device frame also checks its footprint.
+    """
+
+    CANARY8 = 0xA5
+    CANARY16 = 0x5AA5  # above every 12-bit sample value
 
     def __init__(self, fr, device="cuda"):
-        torch = _torch()
         self.meta = fr
         self.bd = fr.bd
-        self.planes = []
-        for pl in fr.planes:
-            if fr.bd == 8:
-                t = torch.from_numpy(pl.pix.astype(np.uint8)).to(device)
-            else:
-                t = torch.from_numpy(pl.pix.view(np.int16)).to(device)
-            self.planes.append(t)
+        self._guarded([pl.pix for pl in fr.planes], device)
         self.abi = self._abi()
 
     @classmethod
     def empty_like(cls, fr, device="cuda"):
-        torch = _torch()
         obj = cls.__new__(cls)
         obj.meta, obj.bd = fr, fr.bd
-        dt = torch.uint8 if fr.bd == 8 else torch.int16
-        obj.planes = [torch.zeros(pl.pix.shape, dtype=dt, device=device) for pl in fr.planes]
+        obj._guarded([np.zeros(pl.pix.shape, np.uint16) for pl in fr.planes], device)
         obj.abi = obj._abi()
         return obj
+
+    def _guarded(self, pixes, device):
+        torch = _torch()
+        esz = 1 if self.bd == 8 else 2
+        canary = self.CANARY8 if self.bd == 8 else self.CANARY16 - 0x10000 * (self.CANARY16 >= 0x8000)
+        self.planes, self._bufs, self._guard = [], [], []
+        for pix in pixes:
+            rows, cols = pix.shape
+            g = max(4096, 2 * cols * esz)
+            g = (g + 255) // 256 * 256 // esz  # guard length in samples
+            host = pix.astype(np.uint8) if self.bd == 8 else pix.view(np.int16)
+            buf = torch.full((2 * g + rows * cols,), canary, dtype=torch.uint8 if self.bd == 8 else torch.int16,
+                             device=device)
+            buf[g:g + rows * cols].copy_(torch.from_numpy(np.ascontiguousarray(host).reshape(-1)))
+            self._bufs.append(buf)
+            self._guard.append(g)
+            self.planes.append(buf[g:g + rows * cols].view(rows, cols))
+
+    def clone(self):
+        """A device copy of this frame, guard bands included."""
+        c = DeviceFrame.__new__(DeviceFrame)
+        c.meta, c.bd = self.meta, self.bd
+        c._bufs = [b.clone() for b in self._bufs]
+        c._guard = list(self._guard)
+        c.planes = [b[g:g + t.numel()].view(t.shape) for b, g, t in zip(c._bufs, c._guard, self.planes)]
+        c.abi = c._abi()
+        return c
+
+    def check_canaries(self):
+        canary = self.CANARY8 if self.bd == 8 else self.CANARY16 - 0x10000 * (self.CANARY16 >= 0x8000)
+        for i, (buf, g) in enumerate(zip(self._bufs, self._guard)):
+            n = buf.numel()
+            for name, band, base in (("before", buf[:g], -g), ("after", buf[n - g:], n - 2 * g)):
+                bad = (band != canary).nonzero()
+                if bad.numel():
+                    k = int(bad[0, 0])
+                    raise AssertionError(
+                        f"plane {i}: {bad.numel()} guard samples {name} the plane were written, first at "
+                        f"sample offset {base + k} from the plane start")
 
     def _abi(self):
         fr = self.meta
@@ -157,7 +198,9 @@ class DeviceFrame:
         return f
 
     def to_numpy(self):
-        """Planes back on the host as uint16 arrays (synth.FrameData layout)."""
+        """Planes back on the host as uint16 arrays (synth.FrameData layout),
+        after checking that no guard band around them was written."""
+        self.check_canaries()
         out = []
         for t in self.planes:
             a = t.cpu().numpy()

@@ -3,7 +3,9 @@
 Each case builds a seeded synthetic frame (pre-filter reconstruction plus legal
 per-superblock side data), runs the requested stages through the C ABI on the
 device and through the oracle on the CPU, and requires identical per-plane MD5
-of the visible samples (bit-exact integer work).
+of the visible samples (bit-exact integer work).  Every device plane sits
+between canary-filled guard bands (DeviceFrame), checked each time a frame
+is read back: a write outside a stage's planes fails the case.
 """
 import numpy as np
 import pytest
@@ -401,3 +403,21 @@ def test_graph_replay_then_autotune_without_sync(runner, mixed):
         check_batch(frames, devs3, disps3, wants, "replay after autotune")
     finally:
         runner.set_graphs(False)
+
+
+def test_guard_bands_catch_writes_outside_the_planes(runner):
+    """The footprint check itself: host writes into either guard band of a
+    plane are reported, untouched bands pass after a full chain."""
+    from goav1_amd.lib import DeviceFrame
+    fr = small_frame(9, 136, 72, 10, 1, grain=True)
+    got, _ = gpu_run(runner, fr)  # reads back (and checks) frame and display
+    want, _ = orc.run_chain(fr)
+    assert_same(got, want, "guarded chain")
+    for band in ("before", "after"):
+        d = DeviceFrame(fr)
+        d.check_canaries()
+        buf, g = d._bufs[1], d._guard[1]
+        k = g - 1 if band == "before" else buf.numel() - g
+        buf[k] = 0
+        with pytest.raises(AssertionError, match=band):
+            d.to_numpy()
