@@ -16,7 +16,11 @@ Prints one JSON line.  `value` is luma pixels per second (Mpx/s);
 dominant kernel (the HBM-streaming stage with the most GPU time), timed with
 HIP events around each of its launches on the launch stream, in a second pass
 over the same steps (the events add small gaps between kernels, so `value`
-comes from the first, event-free pass).  The CPU baseline times the oracle on the host at up to 16 threads
+comes from the first, event-free pass).  Its `valu_frac` divides the kernel's
+VALU issue floor (the committed rocprofv3 SQ_INSTS_VALU count per launch at
+one wave64 instruction per CU per cycle) by the same live launch time, and
+`bound` names the ceiling (HBM bytes or VALU issue) with the longer floor.
+The CPU baseline times the oracle on the host at up to 16 threads
 (cpu_baseline) and at 1 thread (cpu_baseline_1thread).
 """
 import argparse
@@ -176,7 +180,8 @@ def search_leg(runner, steps, cpu):
 def learned_leg(runner, nframes, width=1920, height=1080):
     """Learned-proposal search against the exhaustive search on held-out
     moving-rectangle frames (8-bit 4:2:0, q 60..220): the network (committed
-    weights) proposes 8 luma + 8 chroma strengths and the LR units worth
+    weights) proposes a shortlist of luma and of chroma strengths (the
+    library's av1pf_search_shortlist_size() each) and the LR units worth
     searching; the shortlist is evaluated exactly on the device.  Both
     searches run the LR search on a second stream beside the CDEF work.
     PSNR-Y loss is measured against the exhaustive decision through the full
@@ -380,16 +385,42 @@ def deblock_build_leg(runner, reps=50):
             "algorithmic_bytes": nbytes, "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)}
 
 
-def traffic_for(kernel):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+def counters_for(kernel):
+    """The committed rocprofv3 counter summary of a kernel (HBM bytes and
+    VALU instructions per launch of the 8-frame batch), or {}."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(p):
-        return None
+        return {}
     try:
-        d = json.load(open(p))
-        return d.get(kernel, {}).get("bytes_per_launch")
+        return json.load(open(p)).get(kernel, {})
     except Exception:
-        return None
+        return {}
+
+
+# one wave64 VALU instruction per CU per cycle: 256 CUs x 2.4 GHz
+VALU_WAVE_INSTS_PER_US = 256 * 2400
+
+
+def roofline(name, ms, launches, nbytes):
+    """Roofline of the dominant kernel: HBM fraction from the algorithmic
+    bytes and the live launch time, VALU fraction from the committed
+    instruction count per launch over the same live time; `bound` names the
+    ceiling the kernel is closer to."""
+    avg_us = 1e3 * ms / launches
+    achieved = (nbytes / launches) / (avg_us * 1e-6) / 1e9
+    c = counters_for(name)
+    hbm_floor_us = (nbytes / launches) / (HBM_PEAK_GBS * 1e3)
+    valu = c.get("valu_insts_per_launch")
+    valu_floor_us = valu / VALU_WAVE_INSTS_PER_US if valu else None
+    r = {"bound": "valu" if valu_floor_us and valu_floor_us > hbm_floor_us else "hbm", "kernel": name,
+         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": c.get("bytes_per_launch"),
+         "algorithmic_bytes_per_launch": int(nbytes / launches), "avg_launch_us": round(avg_us, 2),
+         "hbm_floor_us": round(hbm_floor_us, 2)}
+    if valu_floor_us:
+        r.update({"valu_insts_per_launch": valu, "valu_floor_us": round(valu_floor_us, 2),
+                  "valu_frac": round(valu_floor_us / avg_us, 4)})
+    return r
 
 
 def main():
@@ -485,7 +516,6 @@ def main():
     # dominant kernel: the HBM-streaming stage with the most GPU time
     dom = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][0])
     name, (ms, launches, nbytes) = dom
-    achieved = (nbytes / launches) / ((ms / launches) / 1e3) / 1e9
     stages = {k: {"ms_total": round(v[0], 3), "launches": v[1],
                   "avg_us": round(1e3 * v[0] / v[1], 2),
                   "gbs": round((v[2] / v[1]) / ((v[0] / v[1]) / 1e3) / 1e9, 1) if v[2] else None}
@@ -513,10 +543,7 @@ def main():
         "profiled_pass_ms_per_step": round(prof_ms / args.steps, 4),
         "timed_regions_ms": [round(r[0], 3) for r in regions],
         "wall_s_timed": round(wall, 4),
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(name),
-                     "algorithmic_bytes_per_launch": int(nbytes / launches),
-                     "avg_launch_us": round(1e3 * ms / launches, 2)},
+        "roofline": roofline(name, ms, launches, nbytes),
         "stages": stages,
     }
     if not args.no_search:
