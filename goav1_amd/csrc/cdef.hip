@@ -27,7 +27,13 @@
 
 namespace pf {
 
-#define CDEF_VL 0x4000
+// Outside-the-frame sentinel of the LDS tile.  goav1 uses VeryLarge = 0x4000
+// (cdef/constants.go:17); here 0xC000 stands in for it: a tap at the
+// sentinel still differs from every sample by more than 2^(damping + 1), so
+// its constrained term is 0 as with 0x4000, and it is the smallest int16 and
+// the largest uint16, so it drops out of the signed maximum and the unsigned
+// minimum of the clip without masking.
+#define CDEF_VL 0xC000
 #define TPITCH 82   // 64 + 2*8, plus 2 to spread the block rows over the banks
 #define TROWS 68    // 64 + 2*2
 #define TSHIFT (TROWS * TPITCH + 1)   // odd offset of the tile's second copy (samples)
@@ -136,44 +142,139 @@ __device__ __forceinline__ void copy_unit(const PlaneView &s, const PlaneView &d
 }
 
 // Stage the unit input (postfilter_cdef.go:1101-1250) into an LDS tile with
-// origin at tile[2*pitch + 8].
-template <typename T>
-__device__ __forceinline__ void load_tile(uint16_t *tile, int pitch, const PlaneView &s, int ux, int uy, int uw, int uh, int tid)
+// origin at tile[2*TPITCH + 8], in two steps so that the global loads of a
+// plane can be issued early and all be in flight at once: tile_fetch loads a
+// lane's share -- up to TCH chunks of 4 horizontally adjacent samples, one
+// 4-sample vector load per chunk that lies inside the plane (clamped single
+// loads for chunks at the plane's edges) -- into registers; tile_store
+// replaces the samples outside the frame by the sentinel (or replicates the
+// last row / column for a unit that is not the last one), and writes the
+// tile and its copy at the odd offset TSHIFT.
+constexpr int TCH = 6;   // chunks per lane: (64 + 4) rows x (64 + 16) / 4 chunks <= 6 x 256
+struct TileGeo {
+    int rows, nch, sy1, sx1, ux, uy;
+    bool row_ext, col_ext;
+};
+__device__ __forceinline__ TileGeo tile_geo(const PlaneView &s, int ux, int uy, int uw, int uh)
 {
+    TileGeo g;
     const int W = s.aw, H = s.ah;
-    const int sy1 = min(uy + uh + 2, H), sx1 = min(ux + uw + 8, W);
-    const bool row_ext = sy1 < uy + uh + 2 && sy1 > 0 && uy + uh < H;
-    const bool col_ext = sx1 < ux + uw + 8 && sx1 > 0 && ux + uw < W;
-    const int rows = uh + 4, cols = uw + 16;
-    for (int c = tid & 63; c < cols; c += 64) {
-        const int x = ux - 8 + c;
-        const bool vx = x >= 0 && (x < sx1 || col_ext);
-        const int cx = min(x, sx1 - 1);
-        // a lane's loads of 6 rows are in flight before its LDS writes
-        for (int r0 = tid >> 6; r0 < rows; r0 += 24) {
-            // loads from clamped (always valid) positions, unconditionally,
-            // so that none waits at a branch join; VeryLarge chosen after
-            uint16_t v[6];
+    g.sy1 = min(uy + uh + 2, H);
+    g.sx1 = min(ux + uw + 8, W);
+    g.row_ext = g.sy1 < uy + uh + 2 && g.sy1 > 0 && uy + uh < H;
+    g.col_ext = g.sx1 < ux + uw + 8 && g.sx1 > 0 && ux + uw < W;
+    g.rows = uh + 4;
+    g.nch = (uw + 16) >> 2;   // uw is a multiple of 4 (MI-aligned extent)
+    g.ux = ux;
+    g.uy = uy;
+    return g;
+}
+template <typename T>
+struct TileRegs {
+    uint32_t v[TCH][sizeof(T)];   // 4 samples: 2 words (16-bit) or 1 word (8-bit)
+};
+template <typename T>
+__device__ __forceinline__ void tile_fetch(TileRegs<T> &R, const PlaneView &s, const TileGeo &g, int tid)
+{
+    const bool vec = s.al >= 4 * (int)sizeof(T);
+    // chunk i = tid + 256 j is (row r, chunk k) of the tile: one division, then steps
+    int r = tid / g.nch, k = tid - r * g.nch;
+    const int dr = 256 / g.nch, dk = 256 - dr * g.nch;
 #pragma unroll
-            for (int j = 0; j < 6; j++) {
-                const int y = uy - 2 + r0 + 4 * j;
-                v[j] = row_ptr<T>(s, min(max(y, 0), sy1 - 1))[min(max(cx, 0), sx1 - 1)];
-            }
+    for (int j = 0; j < TCH; j++) {
+        if (j) {
+            r += dr;
+            k += dk;
+            if (k >= g.nch) { k -= g.nch; r++; }
+        }
+        if (r < g.rows) {
+            const int y = min(max(g.uy - 2 + r, 0), g.sy1 - 1);
+            const int x = g.ux - 8 + 4 * k;
+            const T *row = row_ptr<T>(s, y);
+            if (vec && x >= 0 && x + 3 < g.sx1) {
+                if (sizeof(T) == 2) {
+                    const uint2 w = *(const uint2 *)(row + x);
+                    R.v[j][0] = w.x;
+                    R.v[j][sizeof(T) - 1] = w.y;
+                } else {
+                    R.v[j][0] = *(const uint32_t *)(row + x);
+                }
+            } else {
+                uint32_t e[4];
 #pragma unroll
-            for (int j = 0; j < 6; j++) {
-                const int r = r0 + 4 * j;
-                const int y = uy - 2 + r;
-                const bool vy = y >= 0 && (y < sy1 || row_ext);
-                if (r < rows) {
-                    const uint16_t val = (vy && vx) ? v[j] : (uint16_t)CDEF_VL;
-                    tile[r * pitch + c] = val;
-                    tile[TSHIFT + r * pitch + c] = val;   // the copy at an odd offset
+                for (int q = 0; q < 4; q++) e[q] = row[min(max(x + q, 0), g.sx1 - 1)];
+                if (sizeof(T) == 2) {
+                    R.v[j][0] = e[0] | (e[1] << 16);
+                    R.v[j][sizeof(T) - 1] = e[2] | (e[3] << 16);
+                } else {
+                    R.v[j][0] = e[0] | (e[1] << 8) | (e[2] << 16) | (e[3] << 24);
                 }
             }
         }
     }
 }
+template <typename T>
+__device__ __forceinline__ void tile_store(uint16_t *tile, const TileRegs<T> &R, const TileGeo &g, int tid)
+{
+    // a unit whose whole input lies inside the plane stores its samples as
+    // loaded: no sentinel, no replication
+    const bool inner = g.ux >= 8 && g.uy >= 2 && g.sx1 == g.ux + 4 * g.nch - 8 && g.sy1 == g.uy + g.rows - 2;
+    int r = tid / g.nch, k = tid - r * g.nch;
+    const int dr = 256 / g.nch, dk = 256 - dr * g.nch;
+#pragma unroll
+    for (int j = 0; j < TCH; j++) {
+        if (j) {
+            r += dr;
+            k += dk;
+            if (k >= g.nch) { k -= g.nch; r++; }
+        }
+        if (r < g.rows) {
+            uint32_t *t = reinterpret_cast<uint32_t *>(tile + r * TPITCH + 4 * k);
+            uint16_t *c = tile + TSHIFT + r * TPITCH + 4 * k;
+            if (inner) {
+                uint32_t w0, w1;
+                if (sizeof(T) == 2) {
+                    w0 = R.v[j][0];
+                    w1 = R.v[j][sizeof(T) - 1];
+                } else {   // bytes 0..3 to half words
+                    w0 = __builtin_amdgcn_perm(0u, R.v[j][0], 0x0c010c00u);
+                    w1 = __builtin_amdgcn_perm(0u, R.v[j][0], 0x0c030c02u);
+                }
+                t[0] = w0;
+                t[1] = w1;
+                c[0] = (uint16_t)w0;
+                *reinterpret_cast<uint32_t *>(c + 1) = __builtin_amdgcn_alignbit(w1, w0, 16);
+                c[3] = (uint16_t)(w1 >> 16);
+                continue;
+            }
+            const int y = g.uy - 2 + r;
+            const int x = g.ux - 8 + 4 * k;
+            const bool vy = y >= 0 && (y < g.sy1 || g.row_ext);
+            uint32_t e[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t v = sizeof(T) == 2 ? (R.v[j][q >> 1] >> (16 * (q & 1))) & 0xffffu
+                                                  : (R.v[j][0] >> (8 * q)) & 0xffu;
+                const bool vx = x + q >= 0 && (x + q < g.sx1 || g.col_ext);
+                e[q] = (vy && vx) ? v : (uint32_t)CDEF_VL;
+            }
+            // the tile row starts 4-byte aligned (even pitch), the chunk at an even column
+            t[0] = e[0] | (e[1] << 16);
+            t[1] = e[2] | (e[3] << 16);
+            // the copy starts at an odd sample: one half word, a word, one half word
+            c[0] = (uint16_t)e[0];
+            *reinterpret_cast<uint32_t *>(c + 1) = e[1] | (e[2] << 16);
+            c[3] = (uint16_t)e[3];
+        }
+    }
+}
 
+// The eight packed strengths of a table (4-byte aligned) as one value, byte i = strength i.
+__device__ __forceinline__ uint64_t str8(gptr<const uint8_t> p)
+{
+    const auto w = (gptr<const uint32_t>)p;
+    return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+}
 __device__ __forceinline__ int str_pri(uint8_t p) { return p >> 2; }
 __device__ __forceinline__ int str_sec(uint8_t p) { const int s = p & 3; return s == 3 ? 4 : s; }
 
@@ -185,6 +286,8 @@ __device__ __forceinline__ v2u splatu2(int v) { return (v2u){ (unsigned short)v,
 __device__ __forceinline__ v2s vmax(v2s a, v2s b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ v2s vmin(v2s a, v2s b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ v2s as2(uint32_t v) { return __builtin_bit_cast(v2s, v); }
+__device__ __forceinline__ v2u asu(v2s v) { return __builtin_bit_cast(v2u, v); }
+__device__ __forceinline__ v2u vminu(v2u a, v2u b) { return __builtin_elementwise_min(a, b); }
 // constrain(d, t, sh) = sign(d) min(|d|, max(0, t - (|d| >> sh))) of two
 // samples, written as the clamp of d to [-c, c] with c = max(0, t - (|d| >> sh)).
 __device__ __forceinline__ v2s cst2(v2s a, v2s x, v2u t, v2u sh)
@@ -213,6 +316,12 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ui = ur * a.unit_cols + uc;
+    // plane 0's tile loads go out first: they depend only on the unit's
+    // position, so they overlap the loads of the unit's parameters below
+    TileRegs<T> R;
+    const int uw0 = min(64, src.p[0].aw - uc * 64), uh0 = min(64, src.p[0].ah - ur * 64);
+    const TileGeo g0 = tile_geo(src.p[0], uc * 64, ur * 64, uw0, uh0);
+    if (uw0 > 0 && uh0 > 0) tile_fetch<T>(R, src.p[0], g0, tid);
     const uint64_t m = a.noskip ? gmem(a.noskip)[ui] : ~0ull;
     int idx, count, yfilt, uvfilt;
     gptr<const uint8_t> ystr, uvstr;
@@ -228,11 +337,10 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         count = 1 << min(max(bits, 0), 3);
         ystr = dec->y_strength;
         uvstr = dec->uv_strength;
-        yfilt = uvfilt = 0;
-        for (int k = 0; k < count; k++) {
-            yfilt |= ystr[k] != 0;
-            uvfilt |= src.nplanes > 1 && uvstr[k] != 0;
-        }
+        const uint64_t ys = str8(ystr), uvs = str8(uvstr);
+        const uint64_t used = count == 8 ? ~0ull : (1ull << (8 * count)) - 1;
+        yfilt = (ys & used) != 0;
+        uvfilt = src.nplanes > 1 && (uvs & used) != 0;
         const int rb = min(8, a.rows8 - ur * 8), cb = min(8, a.cols8 - uc * 8);
         uint64_t in = 0;
         for (int by = 0; by < rb; by++) in |= ((1ull << cb) - 1) << (by * 8);
@@ -251,6 +359,11 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
     }
     const int cs = src.bd - 8;
     const bool chroma_filtering = uvfilt != 0;
+    // all eight strengths of each kind as two words each, read without
+    // waiting for the index, then picked by it
+    const uint64_t ysw = str8(ystr), uvsw = str8(uvstr);
+    const int ypk = idx >= 0 ? (int)((ysw >> (8 * idx)) & 0xff) : 0;
+    const int uvpk = idx >= 0 ? (int)((uvsw >> (8 * idx)) & 0xff) : 0;
 
     for (int pl = 0; pl < src.nplanes; pl++) {
         const int xd = pl ? src.ssx : 0, yd = pl ? src.ssy : 0;
@@ -263,9 +376,9 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         bool active = false, dir_only = false, need_dir = false;
         uint8_t packed = 0;
         if (process && idx >= 0) {
-            packed = pl ? uvstr[idx] : ystr[idx];
-            const int uvpri = str_pri(uvstr[idx]);
-            need_dir = pl == 0 ? (str_pri(ystr[idx]) != 0 || (chroma_filtering && uvpri != 0)) : uvpri != 0;
+            packed = (uint8_t)(pl ? uvpk : ypk);
+            const int uvpri = str_pri((uint8_t)uvpk);
+            need_dir = pl == 0 ? (str_pri((uint8_t)ypk) != 0 || (chroma_filtering && uvpri != 0)) : uvpri != 0;
             dir_only = pl == 0 && chroma_filtering && packed == 0 && uvpri != 0;
             active = packed != 0 || dir_only;
         }
@@ -277,8 +390,10 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
             copy_unit<T>(sp, dp, ux, uy, uw, uh, tid);
             continue;
         }
+        const TileGeo g = pl == 0 ? g0 : tile_geo(sp, ux, uy, uw, uh);
+        if (pl > 0) tile_fetch<T>(R, sp, g, tid);   // in flight across the barrier
         __syncthreads();   // previous plane finished with the tile / block tables
-        load_tile<T>(tile, TPITCH, sp, ux, uy, uw, uh, tid);
+        tile_store<T>(tile, R, g, tid);
         __syncthreads();
         const uint16_t *org = tile + 2 * TPITCH + 8;
         const int pri = str_pri(packed) << cs;
@@ -348,52 +463,84 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
             copy_unit<T>(sp, dp, ux, uy, uw, uh, tid);
             continue;
         }
-        // the filter walks horizontal sample pairs, one pair per lane: both
-        // samples of a pair lie in one block (blocks are at least 4 wide), so
-        // they share the record, and every tap is one 32-bit LDS read and
-        // packed 16-bit arithmetic (values stay below 2^15: samples < 2^12,
-        // the VeryLarge sentinel 2^14, the weighted sum below 2^12)
-        const int np = (uw + 1) >> 1;
-        const int tpr = np > 16 ? 32 : np > 8 ? 16 : np > 4 ? 8 : 4;   // lanes per row, a power of two >= np
-        const int x = 2 * (tid & (tpr - 1));
-        if (x < uw) {
-            for (int y = tid / tpr; y < uh; y += 256 / tpr) {
-                const BlockRec r = brec[((y >> bhl) << 3) + (x >> bwl)];
-                const uint32_t *W = reinterpret_cast<const uint32_t *>(tile) + ((org - tile + y * TPITCH + x) >> 1);
-                const v2s xv = as2(W[0]);
-                v2s out = xv;
+        // one lane per (block, pair of columns): both samples of a pair lie
+        // in one block (blocks are at least 4 wide) and share its record, so
+        // the lane reads the record and forms the 12 tap addresses once, then
+        // walks the block's rows, each tap one 32-bit LDS read at an
+        // immediate row offset.  Packed 16-bit arithmetic (values stay below
+        // 2^15 in magnitude: samples < 2^12, the sentinel -2^14, the weighted
+        // sum below 2^12).  Lanes of a wave cover whole rows of a block row,
+        // so the stores coalesce.
+        const int P = bw >> 1;               // pairs per block row: 2 or 4
+        const int per_brow = 8 * P;          // items per block row (8 block columns)
+        if (tid < brows * per_brow) {
+            const int br = tid / per_brow, c = tid - br * per_brow;
+            const int bc = c / P;
+            const int x = bc * bw + 2 * (c - bc * P);
+            const int y0 = br * bh;
+            const int rows = min(bh, uh - y0);
+            if (x < uw) {
+                const BlockRec r = brec[br * 8 + bc];
+                const uint32_t *Wt = reinterpret_cast<const uint32_t *>(tile);
+                const int w0 = (int)((org - tile + y0 * TPITCH + x) >> 1);   // word of the pair in row y0
                 const int flag = r.p >> 24;
-                if (flag & 1) {
-                    const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
-                    const v2s a0 = as2(W[r.e[0]]), a1 = as2(W[r.e[1]]), a2 = as2(W[r.e[2]]), a3 = as2(W[r.e[3]]);
-                    const v2s b0 = as2(W[r.e[4]]), b1 = as2(W[r.e[5]]), b2 = as2(W[r.e[6]]), b3 = as2(W[r.e[7]]);
-                    const v2s b4 = as2(W[r.e[8]]), b5 = as2(W[r.e[9]]), b6 = as2(W[r.e[10]]), b7 = as2(W[r.e[11]]);
-                    v2s sum = splat2(0);
-                    if (pri) {   // st may be 0 for a flat block: its primary terms vanish
-                        const v2u t = splatu2(st), sh = splatu2(psh);
-                        const v2s t0 = splat2(((st >> cs) & 1) ? 3 : 4), t1 = splat2(((st >> cs) & 1) ? 3 : 2);
-                        sum = t0 * (cst2(a0, xv, t, sh) + cst2(a1, xv, t, sh)) + t1 * (cst2(a2, xv, t, sh) + cst2(a3, xv, t, sh));
-                    }
-                    if (sec) {
-                        const v2u t = splatu2(sec), sh = splatu2(ssh);
-                        sum += splat2(2) * (cst2(b0, xv, t, sh) + cst2(b1, xv, t, sh) + cst2(b2, xv, t, sh) +
-                                            cst2(b3, xv, t, sh)) +
-                               cst2(b4, xv, t, sh) + cst2(b5, xv, t, sh) + cst2(b6, xv, t, sh) + cst2(b7, xv, t, sh);
-                    }
-                    // (8 + sum - (sum < 0)) >> 4; a true vector comparison is -1
-                    out = xv + ((splat2(8) + sum + (v2s)(sum < splat2(0))) >> splat2(4));
-                    if (flag & 2) {   // clip to the taps' range, VeryLarge excluded from the maximum
-                        const v2s mk = splat2(CDEF_VL - 1);
-                        v2s hi = vmax(vmax(vmax(xv, a0 & mk), vmax(a1 & mk, a2 & mk)), vmax(a3 & mk, b0 & mk));
-                        hi = vmax(vmax(vmax(hi, b1 & mk), vmax(b2 & mk, b3 & mk)), vmax(vmax(b4 & mk, b5 & mk), vmax(b6 & mk, b7 & mk)));
-                        v2s lo = vmin(vmin(vmin(xv, a0), vmin(a1, a2)), vmin(a3, b0));
-                        lo = vmin(vmin(vmin(lo, b1), vmin(b2, b3)), vmin(vmin(b4, b5), vmin(b6, b7)));
-                        out = vmin(vmax(out, lo), hi);
+                const int st = r.p & 255, psh = (r.p >> 8) & 255, ssh = (r.p >> 16) & 255;
+                const v2u tp = splatu2(st), shp = splatu2(psh), ts = splatu2(sec), shs = splatu2(ssh);
+                const v2s t0 = splat2(((st >> cs) & 1) ? 3 : 4), t1 = splat2(((st >> cs) & 1) ? 3 : 2);
+                int A[12];   // word index of each tap pair in row y0
+#pragma unroll
+                for (int k = 0; k < 12; k++) A[k] = w0 + r.e[k];
+                const bool pair = x + 1 < uw;
+                const bool wide = pair && dp.al >= (int)(2 * sizeof(T));   // one store for the pair
+                auto row_out = gmem(row_ptr<T>(dp, uy + y0) + ux + x);
+#pragma unroll
+                for (int yy = 0; yy < 8; yy++) {
+                    if (yy < rows) {
+                        constexpr int RW = TPITCH / 2;   // words per tile row
+                        const v2s xv = as2(Wt[w0 + yy * RW]);
+                        v2s out = xv;
+                        if (flag & 1) {
+                            const v2s a0 = as2(Wt[A[0] + yy * RW]), a1 = as2(Wt[A[1] + yy * RW]);
+                            const v2s a2 = as2(Wt[A[2] + yy * RW]), a3 = as2(Wt[A[3] + yy * RW]);
+                            const v2s b0 = as2(Wt[A[4] + yy * RW]), b1 = as2(Wt[A[5] + yy * RW]);
+                            const v2s b2 = as2(Wt[A[6] + yy * RW]), b3 = as2(Wt[A[7] + yy * RW]);
+                            const v2s b4 = as2(Wt[A[8] + yy * RW]), b5 = as2(Wt[A[9] + yy * RW]);
+                            const v2s b6 = as2(Wt[A[10] + yy * RW]), b7 = as2(Wt[A[11] + yy * RW]);
+                            v2s sum = splat2(0);
+                            if (pri)   // st may be 0 for a flat block: its primary terms vanish
+                                sum = t0 * (cst2(a0, xv, tp, shp) + cst2(a1, xv, tp, shp)) +
+                                      t1 * (cst2(a2, xv, tp, shp) + cst2(a3, xv, tp, shp));
+                            if (sec)
+                                sum += splat2(2) * (cst2(b0, xv, ts, shs) + cst2(b1, xv, ts, shs) + cst2(b2, xv, ts, shs) +
+                                                    cst2(b3, xv, ts, shs)) +
+                                       cst2(b4, xv, ts, shs) + cst2(b5, xv, ts, shs) + cst2(b6, xv, ts, shs) +
+                                       cst2(b7, xv, ts, shs);
+                            // (8 + sum - (sum < 0)) >> 4, with sum >> 15 the -1 of a negative sum
+                            out = xv + ((splat2(8) + sum + (sum >> splat2(15))) >> splat2(4));
+                            if (flag & 2) {
+                                // clip to the taps' range: the sentinel (0xC000) is the
+                                // smallest signed value, so it never sets the maximum,
+                                // and the largest unsigned one, so it never sets the minimum
+                                v2s hi = vmax(vmax(vmax(xv, a0), vmax(a1, a2)), vmax(a3, b0));
+                                hi = vmax(vmax(vmax(hi, b1), vmax(b2, b3)), vmax(vmax(b4, b5), vmax(b6, b7)));
+                                const v2u ux0 = __builtin_bit_cast(v2u, xv);
+                                v2u lo = vminu(vminu(vminu(ux0, asu(a0)), vminu(asu(a1), asu(a2))), vminu(asu(a3), asu(b0)));
+                                lo = vminu(vminu(vminu(lo, asu(b1)), vminu(asu(b2), asu(b3))),
+                                           vminu(vminu(asu(b4), asu(b5)), vminu(asu(b6), asu(b7))));
+                                out = vmin(vmax(out, __builtin_bit_cast(v2s, lo)), hi);
+                            }
+                        }
+                        const auto o = row_out + (int64_t)yy * (dp.stride / (int64_t)sizeof(T));
+                        if (wide && sizeof(T) == 2) {
+                            *(gptr<uint32_t>)o = __builtin_bit_cast(uint32_t, out);
+                        } else if (wide) {
+                            *(gptr<uint16_t>)o = (uint16_t)((uint16_t)out.x | ((uint16_t)out.y << 8));
+                        } else {
+                            o[0] = (T)(uint16_t)out.x;
+                            if (pair) o[1] = (T)(uint16_t)out.y;
+                        }
                     }
                 }
-                const auto o = gmem(row_ptr<T>(dp, uy + y) + ux + x);
-                o[0] = (T)(uint16_t)out.x;
-                if (x + 1 < uw) o[1] = (T)(uint16_t)out.y;
             }
         }
     }
