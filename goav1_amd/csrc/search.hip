@@ -315,6 +315,8 @@ constexpr int SEL_GRID = 1 + 3 * SEL_G;    // level 0 on workgroup 0
 constexpr int SEL_STEPS = 39;              // exchanged steps of level 3 (8 greedy + 32 refinements, less the first)
 constexpr int SEL_T2_MAX = 280;            // filter blocks for both tables in LDS
 constexpr int SEL_T1_MAX = 560;            // filter blocks for the chroma table in LDS
+constexpr int SEL_TP = 65;                 // LDS row pitch of the tables (words): lanes reading one
+                                           // column of consecutive rows hit distinct banks
 
 struct FinishState {
     unsigned int done;                     // levels finished (monotonic, 4 per launch)
@@ -324,10 +326,12 @@ struct FinishState {
     unsigned long long cost[4];
 };
 
+// LDS: active block list, each block's best cost, the workgroup's own luma
+// column (when the luma table is not in LDS), then the chroma and luma tables.
 __host__ __device__ inline size_t sel_lds_bytes(int nfb, int tables)
 {
-    const size_t base = ((size_t)nfb * 4 + 15) / 16 * 16 + (size_t)nfb * 8;
-    return base + (size_t)nfb * 64 * 4 * (size_t)tables;
+    const size_t base = ((size_t)nfb * 4 + 15) / 16 * 16 + (size_t)nfb * 8 + (tables < 2 ? (size_t)nfb * 8 : 0);
+    return base + (size_t)nfb * SEL_TP * 4 * (size_t)tables;
 }
 
 template <int TABLES>
@@ -352,8 +356,9 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
     const int gw = b == 0 ? 0 : (b - 1) % SEL_G;
     int *act = reinterpret_cast<int *>(smem);
     unsigned long long *best = reinterpret_cast<unsigned long long *>(smem + ((size_t)nfb * 4 + 15) / 16 * 16);
-    uint32_t *ct = reinterpret_cast<uint32_t *>(best + nfb);    // chroma rows of the active blocks
-    uint32_t *lt = ct + (size_t)nfb * 64;                       // luma rows (TABLES == 2)
+    unsigned long long *lcol = best + nfb;                      // a_ij of this workgroup's j (TABLES < 2)
+    uint32_t *ct = reinterpret_cast<uint32_t *>(best + nfb + (TABLES < 2 ? nfb : 0));   // chroma rows
+    uint32_t *lt = ct + (size_t)nfb * SEL_TP;                   // luma rows (TABLES == 2)
     const uint64_t *Lm = mse, *Cm = mse + (size_t)nfb * 64;
     // ---- active blocks (a non-skip 8x8 block inside the frame), ascending
     if (t < 64) {
@@ -386,29 +391,70 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
     const int nact = s_nact;
     // ---- the MSE rows of the active blocks in LDS
     if (TABLES >= 1 && b != 0) {
-        for (int e = t; e < nact * 64; e += SEL_T) {
-            const int fb = act[e >> 6], g = e & 63;
-            const uint64_t c = Cm[(size_t)fb * 64 + g];
-            ct[e] = (uint32_t)c;
-            uint64_t l = 0;
-            if (TABLES == 2) {
-                l = Lm[(size_t)fb * 64 + g];
-                lt[e] = (uint32_t)l;
+        // eight rows' loads in flight per thread before any store
+        bool bad = false;
+        for (int e0 = t; e0 < nact * 64; e0 += 8 * SEL_T) {
+            uint64_t c[8], l[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = e0 + u * SEL_T;
+                const int fb = act[min(e, nact * 64 - 1) >> 6], g = e & 63;
+                c[u] = Cm[(size_t)fb * 64 + g];
+                l[u] = TABLES == 2 ? Lm[(size_t)fb * 64 + g] : 0;
             }
-            if ((c | l) >> 32) s_bad = 1;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = e0 + u * SEL_T;
+                if (e < nact * 64) {
+                    ct[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)c[u];
+                    if (TABLES == 2) lt[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)l[u];
+                    bad |= ((c[u] | l[u]) >> 32) != 0;
+                }
+            }
         }
+        if (bad) s_bad = 1;
     }
     __syncthreads();
     const bool ctab = TABLES >= 1 && b != 0 && !s_bad, ltab = TABLES == 2 && b != 0 && !s_bad;
     auto luma = [&](int i, int g) -> unsigned long long {
-        return ltab ? (unsigned long long)lt[i * 64 + g] : Lm[(size_t)act[i] * 64 + g];
+        return ltab ? (unsigned long long)lt[i * SEL_TP + g] : Lm[(size_t)act[i] * 64 + g];
     };
     auto chroma = [&](int i, int k) -> unsigned long long {
-        return ctab ? (unsigned long long)ct[i * 64 + k] : Cm[(size_t)act[i] * 64 + k];
+        return ctab ? (unsigned long long)ct[i * SEL_TP + k] : Cm[(size_t)act[i] * 64 + k];
     };
     const int nb = 1 << level;
     const int steps = level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: compute it once
     const int j = gw;
+    // a_ij of this workgroup's luma strength for every active block
+    if (TABLES < 2 && b != 0) {
+        for (int i0 = t; i0 < nact; i0 += 8 * SEL_T) {
+            unsigned long long v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = Lm[(size_t)act[min(i0 + u * SEL_T, nact - 1)] * 64 + j];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (i0 + u * SEL_T < nact) lcol[i0 + u * SEL_T] = v[u];
+        }
+    }
+    auto lj = [&](int i) -> unsigned long long { return ltab ? (unsigned long long)lt[i * SEL_TP + j] : lcol[i]; };
+    // best_i, the cheapest prior pair of every active block, is kept across
+    // steps: a new pair only lowers it (min with the pair's cost), and a
+    // refinement's dropped pair is handled by recomputing, while the step's
+    // exchange is in flight, the minimum over the pairs the next step keeps
+    // but the new one (then the new pair's cost joins it)
+    auto fold_new = [&](int py_new, int pc_new, bool fresh) {
+        unsigned long long ps = 0;
+        bool wide = false;
+        for (int i = t; i < nact; i += SEL_T) {
+            const unsigned long long c = luma(i, py_new) + chroma(i, pc_new);
+            const unsigned long long m = fresh ? c : (c < best[i] ? c : best[i]);
+            best[i] = m;
+            ps += m;
+            wide |= (m >> 32) != 0;
+        }
+        struct { unsigned long long sum; bool wide; } r = { ps, wide };
+        return r;
+    };
     int gstep = 0;
     for (int s = 0; s < steps; s++) {
         const int slot = s < nb ? s : nb - 1;
@@ -423,63 +469,40 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
             reinterpret_cast<unsigned long long *>(part)[sub * 128 + col] = cs;   // part as [4][128]
             __syncthreads();
             if (t < 128) {
+                // wave 0 holds the luma sums, wave 1 the chroma sums, lane k
+                // strength k: the first minimum by a shuffle reduction
                 const unsigned long long *pp = reinterpret_cast<const unsigned long long *>(part);
-                sums[t] = pp[t] + pp[128 + t] + pp[256 + t] + pp[384 + t];
-            }
-            __syncthreads();
-            if (t == 0) {
-                int by = 0, bc = 0;
-                for (int k = 1; k < 64; k++) {
-                    if (sums[k] < sums[by]) by = k;
-                    if (sums[64 + k] < sums[64 + bc]) bc = k;
+                unsigned long long v = pp[t] + pp[128 + t] + pp[256 + t] + pp[384 + t];
+                int k = lane;
+                for (int o = 32; o > 0; o >>= 1) {
+                    const unsigned long long ov = __shfl_xor(v, o);
+                    const int ok = __shfl_xor(k, o);
+                    if (ov < v || (ov == v && ok < k)) { v = ov; k = ok; }
                 }
-                ly[0] = nact ? by : 0;
-                lc[0] = nact ? bc : 0;
-                s_dist = nact ? sums[by] + sums[64 + bc] : 0ull;
+                if (lane == 0) {
+                    sums[wv] = v;
+                    if (wv == 0) ly[0] = nact ? k : 0;
+                    else lc[0] = nact ? k : 0;
+                }
             }
             __syncthreads();
+            if (t == 0) s_dist = nact ? sums[0] + sums[1] : 0ull;
+            if (s + 1 < steps) {   // the blocks' best costs for the next step: pair 0
+                if (t == 0) s_wide = 0;
+                __syncthreads();
+                const auto r = fold_new(ly[0], lc[0], true);
+                unsigned long long psum = r.sum;
+                if (r.wide) s_wide = 1;
+                for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
+                if (lane == 0) part[wv][0] = psum;
+                __syncthreads();
+                if (t == 0) {
+                    unsigned long long v = 0;
+                    for (int w = 0; w < 8; w++) v += part[w][0];
+                    s_sum = v;
+                }
+            }
             continue;
-        }
-        // best cost of every active block over the prior pairs, and their
-        // sum; the pair lists are copied to registers first so that all of a
-        // block's loads are in flight together
-        int py[8], pc[8];
-#pragma unroll
-        for (int g = 0; g < 8; g++) { py[g] = ly[g]; pc[g] = lc[g]; }
-        unsigned long long psum = 0;
-        if (t == 0) s_wide = 0;
-        __syncthreads();
-        bool wide = false;
-        for (int i = t; i < nact; i += SEL_T) {
-            unsigned long long v[8];
-            if (ltab) {   // both rows in LDS: every lookup issued before any sum
-                uint32_t a[8], c[8];
-#pragma unroll
-                for (int g = 0; g < 8; g++) {
-                    a[g] = lt[i * 64 + py[g]];
-                    c[g] = ct[i * 64 + pc[g]];
-                }
-#pragma unroll
-                for (int g = 0; g < 8; g++) v[g] = g < slot ? (unsigned long long)a[g] + c[g] : ~0ull;
-            } else {
-#pragma unroll
-                for (int g = 0; g < 8; g++) v[g] = g < slot ? luma(i, py[g]) + chroma(i, pc[g]) : ~0ull;
-            }
-            unsigned long long m = v[0];
-#pragma unroll
-            for (int g = 1; g < 8; g++) m = v[g] < m ? v[g] : m;
-            best[i] = m;
-            psum += m;
-            wide |= (m >> 32) != 0;
-        }
-        if (wide) s_wide = 1;
-        for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
-        if (lane == 0) part[wv][0] = psum;
-        __syncthreads();
-        if (t == 0) {
-            unsigned long long v = 0;
-            for (int w = 0; w < 8; w++) v += part[w][0];
-            s_sum = v;
         }
         // savings of the pairs (j, lane) over this wave's eighth of the
         // blocks, eight blocks' loads in flight at a time (a block whose
@@ -492,8 +515,8 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     bi[u] = (uint32_t)best[i + 8 * u];
-                    a[u] = lt[(i + 8 * u) * 64 + j];
-                    c[u] = ct[(i + 8 * u) * 64 + lane];
+                    a[u] = lt[(i + 8 * u) * SEL_TP + j];
+                    c[u] = ct[(i + 8 * u) * SEL_TP + lane];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; u++)
@@ -501,7 +524,7 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
             }
             for (; i < nact; i += 8)
                 gain += __builtin_elementwise_sub_sat(
-                    __builtin_elementwise_sub_sat((uint32_t)best[i], lt[i * 64 + j]), ct[i * 64 + lane]);
+                    __builtin_elementwise_sub_sat((uint32_t)best[i], lt[i * SEL_TP + j]), ct[i * SEL_TP + lane]);
         } else {
             auto save = [&](unsigned long long bi, unsigned long long a, unsigned long long c) {
                 const unsigned long long room = a < bi ? bi - a : 0ull;
@@ -513,13 +536,13 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     bi[u] = best[i + 8 * u];
-                    a[u] = luma(i + 8 * u, j);
+                    a[u] = lj(i + 8 * u);
                     c[u] = chroma(i + 8 * u, lane);
                 }
 #pragma unroll
                 for (int u = 0; u < 8; u++) gain += save(bi[u], a[u], c[u]);
             }
-            for (; i < nact; i += 8) gain += save(best[i], luma(i, j), chroma(i, lane));
+            for (; i < nact; i += 8) gain += save(best[i], lj(i), chroma(i, lane));
         }
         __syncthreads();
         part[wv][lane] = gain;
@@ -533,7 +556,45 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
                 const unsigned long long other = __shfl_xor(key, o);
                 key = other > key ? other : key;
             }
-            if (lane == 0) atomicExch(&st->slot[epoch & 1][level - 1][gstep][gw], key | (1ull << 63));
+            // a store that is not waited for: the workgroup goes on to poll
+            if (lane == 0)
+                __hip_atomic_store(&st->slot[epoch & 1][level - 1][gstep][gw], key | (1ull << 63), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // while the exchange is in flight: the next step's best costs over
+        // the pairs it keeps, less the one this step adds.  A greedy next
+        // step keeps every pair (best_i stays); a refinement drops slot 0,
+        // keeping slots 1 .. slot-1 of the current lists.
+        const bool next_refines = s + 1 < steps && s + 1 >= nb;
+        if (next_refines && wv > 0) {
+            int py[8], pc[8];
+#pragma unroll
+            for (int g = 0; g < 8; g++) { py[g] = ly[g]; pc[g] = lc[g]; }
+            for (int i = t - 64; i < nact; i += SEL_T - 64) {
+                unsigned long long m = ~0ull;
+                if (ltab) {
+                    uint32_t a[8], c[8];
+#pragma unroll
+                    for (int g = 1; g < 8; g++) {
+                        a[g] = lt[i * SEL_TP + py[g]];
+                        c[g] = ct[i * SEL_TP + pc[g]];
+                    }
+#pragma unroll
+                    for (int g = 1; g < 8; g++)
+                        if (g < slot) {
+                            const unsigned long long v = (unsigned long long)a[g] + c[g];
+                            m = v < m ? v : m;
+                        }
+                } else {
+#pragma unroll
+                    for (int g = 1; g < 8; g++)
+                        if (g < slot) {
+                            const unsigned long long v = luma(i, py[g]) + chroma(i, pc[g]);
+                            m = v < m ? v : m;
+                        }
+                }
+                best[i] = m;   // the gain above has read best (barrier before the exchange)
+            }
         }
         // every workgroup of the level reads the level's slots of this step
         if (t < SEL_G) {
@@ -565,9 +626,25 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
             ly[slot] = pair >> 6;
             lc[slot] = pair & 63;
             s_dist = s_sum - (key >> 12);
+            s_wide = 0;
         }
         gstep++;
         __syncthreads();
+        if (s + 1 < steps) {   // fold the new pair into the best costs of the next step
+            const int sk = 4095 - (int)(s_key & 4095ull);
+            const bool fresh = next_refines && slot == 1;   // the next step keeps only the new pair
+            const auto r = fold_new(sk >> 6, sk & 63, fresh);
+            unsigned long long psum = r.sum;
+            if (r.wide) s_wide = 1;
+            for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
+            if (lane == 0) part[wv][0] = psum;
+            __syncthreads();
+            if (t == 0) {
+                unsigned long long v = 0;
+                for (int w = 0; w < 8; w++) v += part[w][0];
+                s_sum = v;
+            }
+        }
     }
     // ---- level result, then the last level to finish decides
     if (b != 0 && gw != 0) return;
