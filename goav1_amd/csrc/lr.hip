@@ -108,7 +108,8 @@ __device__ __forceinline__ av1pf_lr_unit lr_tile_unit(const LrPlaneDesc &g, int 
 
 template <typename T>
 __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp, const LrPlaneDesc &g, int bd,
-                                        int k, int t, uint16_t *tile, uint16_t *tmp, uint16_t *Ab, int32_t *Bb)
+                                        int k, int t, uint16_t *tile, uint16_t *tmp, uint16_t *Ab0, int32_t *Bb0,
+                                        uint16_t *Ab1, int32_t *Bb1)
 {
     if (k >= g.xtiles || t >= g.stripes) return;
     const int x0 = k * g.procw;
@@ -180,22 +181,26 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
             store4<T>(row_ptr<T>(dp, y0 + y), x0 + x, o, n, dvec);
         });
     } else {
-        sgr_tile(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab, Bb,
-                 [&](int y, int x, int o) { row_ptr<T>(dp, y0 + y)[x0 + x] = (T)o; });
+        sgr_tile_parts(O, w, h, bd, u.sgr_set, u.sgr_xqd, Ab0, Bb0, Ab1, Bb1,
+                       [&](int y, int x, int o) { row_ptr<T>(dp, y0 + y)[x0 + x] = (T)o; });
     }
 }
 
-// 4 workgroups per CU: 36.2 KB of LDS each, and at most 128 VGPRs
-__global__ __launch_bounds__(256, 4) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
+// 6 workgroups per CU: 23.8 KB of LDS each, and at most 80 VGPRs
+__global__ __launch_bounds__(256, 6) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
                                                  int total, StageLaunch L)
 {
     __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
-    // Wiener intermediate and self-guided A/B planes share one buffer
-    __shared__ __align__(16) char work[LR_AB * LR_AB * 6];
-    int32_t *Bb = reinterpret_cast<int32_t *>(work);
-    uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
+    // Wiener intermediate and the self-guided A/B planes of both passes for
+    // a part of the tile share one buffer
+    constexpr int AB = LR_AB * (LR_SGR_ROWS + 2);
+    __shared__ __align__(16) char work[AB * 12];
+    int32_t *Bb0 = reinterpret_cast<int32_t *>(work);
+    int32_t *Bb1 = Bb0 + AB;
+    uint16_t *Ab0 = reinterpret_cast<uint16_t *>(work + AB * 8);
+    uint16_t *Ab1 = Ab0 + AB;
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
-    static_assert(LR_TH * 64 * 2 <= LR_AB * LR_AB * 6, "Wiener intermediate fits the shared buffer");
+    static_assert(LR_TH * 64 * 2 <= AB * 12, "Wiener intermediate fits the shared buffer");
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
         const FrameDesc &D = descs[sg.frame];
@@ -205,9 +210,9 @@ __global__ __launch_bounds__(256, 4) void lr_kernel(const FrameDesc *__restrict_
         const FrameView &dst = D.v[D.lr_dst];
         const int pl = sg.plane;
         if (src.bd == 8)
-            lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab, Bb);
+            lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab0, Bb0, Ab1, Bb1);
         else
-            lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab, Bb);
+            lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab0, Bb0, Ab1, Bb1);
     });
 }
 

@@ -25,6 +25,7 @@ namespace pf {
 #define LR_TH 70          // tile rows: 64 + 6
 #define LR_OX 4           // tile column of sample x = 0
 #define LR_AB 66          // 64 + 2
+#define LR_SGR_ROWS 16    // rows of a self-guided quarter tile (the decoder kernel)
 
 __device__ __forceinline__ int32_t r2s(int32_t v, int n) { return n <= 0 ? v : (v + (1 << (n - 1))) >> n; }
 __device__ __forceinline__ uint32_t r2u(uint32_t v, int n) { return n <= 0 ? v : (v + (1u << (n - 1))) >> n; }
@@ -216,7 +217,7 @@ __device__ __forceinline__ LaneRows lane_rows(int w, int h)
 //   r = 1 (every row): 4 (cross) + 3 (diagonals) = 4 H(y) + Q(y - 1) + Q(y + 1),
 //     Q = 3 H + C;
 // the same integer sums as the stencils, regrouped.
-template <typename Use>
+template <int PPT = SGR_PPT, typename Use>
 __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int bd, int set, uint16_t *Ab, int32_t *Bb,
                                            Use use)
 {
@@ -253,7 +254,7 @@ __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int 
             row(L.ry - 1, ha, ca, hb, cb);
             int32_t pa = 5 * ha + ca, pb = 5 * hb + cb;
 #pragma unroll
-            for (int j = 0; j < SGR_PPT; j++) {
+            for (int j = 0; j < PPT; j++) {
                 const int y = L.ry + j;
                 if (y >= L.end) break;
                 const int32_t uu = O[y * LR_TW + x];
@@ -273,7 +274,7 @@ __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int 
             row(L.ry, ha1, ca1, hb1, cb1);
             int32_t qa0 = 3 * ha0 + ca0, qb0 = 3 * hb0 + cb0;
 #pragma unroll
-            for (int j = 0; j < SGR_PPT; j++) {
+            for (int j = 0; j < PPT; j++) {
                 const int y = L.ry + j;
                 if (y >= L.end) break;
                 int32_t ha2, ca2, hb2, cb2;
@@ -294,12 +295,13 @@ __device__ __forceinline__ void sgr_passes(const uint16_t *O, int w, int h, int 
 
 // The two self-guided outputs of set `set` for this lane's pixels (Q4; zero
 // for a pass of radius 0): f0 from the r = 2 pass, f1 from the r = 1 pass.
+template <int PPT = SGR_PPT>
 __device__ __forceinline__ void sgr_filters(const uint16_t *O, int w, int h, int bd, int set, uint16_t *Ab, int32_t *Bb,
-                                            int32_t (&f0)[SGR_PPT], int32_t (&f1)[SGR_PPT])
+                                            int32_t (&f0)[PPT], int32_t (&f1)[PPT])
 {
 #pragma unroll
-    for (int j = 0; j < SGR_PPT; j++) { f0[j] = 0; f1[j] = 0; }
-    sgr_passes(O, w, h, bd, set, Ab, Bb, [&](int pass, int j, int32_t, int32_t f) {
+    for (int j = 0; j < PPT; j++) { f0[j] = 0; f1[j] = 0; }
+    sgr_passes<PPT>(O, w, h, bd, set, Ab, Bb, [&](int pass, int j, int32_t, int32_t f) {
         if (pass == 0) f0[j] = f;
         else f1[j] = f;
     });
@@ -325,19 +327,21 @@ __device__ __forceinline__ int sgr_project(int u, int32_t f0, int32_t f1, int xq
     return min(max(wv, 0), mx);
 }
 
-template <typename Sink>
+// PPT: rows per lane, at least the rows lane_rows gives a lane for h (16 for
+// h <= 64, 8 for h <= 32).
+template <int PPT = SGR_PPT, typename Sink>
 __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd, int set, const int8_t *xqd,
                                          uint16_t *Ab, int32_t *Bb, Sink sink)
 {
     int xq0, xq1;
     sgr_xq(set, xqd, xq0, xq1);
-    int32_t f0[SGR_PPT], f1[SGR_PPT];
-    sgr_filters(O, w, h, bd, set, Ab, Bb, f0, f1);
+    int32_t f0[PPT], f1[PPT];
+    sgr_filters<PPT>(O, w, h, bd, set, Ab, Bb, f0, f1);
     const LaneRows L = lane_rows(w, h);
     const int32_t mx = (1 << bd) - 1;
     if (L.x < w) {
 #pragma unroll
-        for (int j = 0; j < SGR_PPT; j++) {
+        for (int j = 0; j < PPT; j++) {
             const int y = L.ry + j;
             if (y >= L.end) break;
             sink(y, L.x, sgr_project(O[y * LR_TW + L.x], f0[j], f1[j], xq0, xq1, mx));
@@ -345,5 +349,95 @@ __device__ __forceinline__ void sgr_tile(const uint16_t *O, int w, int h, int bd
     }
 }
 
+
+// The decoder's self-guided filter, LR_SGR_ROWS rows at a time: both
+// passes' A and B planes of the rows (Ab0/Bb0 for r = 2, Ab1/Bb1 for r = 1,
+// each (LR_SGR_ROWS + 2) x LR_AB) are built first, then one walk per lane
+// forms both outputs of a pixel from the same per-row sums as sgr_passes and
+// projects it at once: no per-lane output arrays, and the A/B planes of a
+// part of the tile only (a position's A and B depend only on the tile's
+// samples, so the rows either side of a seam come out the same).
+template <typename Sink>
+__device__ __forceinline__ void sgr_tile_parts(const uint16_t *O, int w, int h, int bd, int set, const int8_t *xqd,
+                                               uint16_t *Ab0, int32_t *Bb0, uint16_t *Ab1, int32_t *Bb1, Sink sink)
+{
+    int xq0, xq1;
+    sgr_xq(set, xqd, xq0, xq1);
+    const int r0 = kSgrR[set][0], r1 = kSgrR[set][1];
+    const uint32_t s0 = (uint32_t)kSgrS[set][0], s1 = (uint32_t)kSgrS[set][1];
+    const int32_t mx = (1 << bd) - 1;
+#pragma unroll 1
+    for (int ya = 0; ya < h; ya += LR_SGR_ROWS) {
+        const int hh = min(LR_SGR_ROWS, h - ya);
+        const uint16_t *Oq = O + ya * LR_TW;
+        __syncthreads();   // the previous part's walk has read its planes
+        if (r0) sgr_ab<2, 2>(Oq, w, hh, bd, s0, Ab0, Bb0);
+        if (r1) sgr_ab<1, 1>(Oq, w, hh, bd, s1, Ab1, Bb1);
+        __syncthreads();
+        const LaneRows L = lane_rows(w, hh);   // at most 4 rows per lane, ry even
+        if (L.x >= w || L.ry >= L.end) continue;
+        const int x = L.x;
+        int oz = 0;   // opaque zero: keeps the three 16-bit reads of a row separate (see sgr_passes)
+        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+        auto row = [&](const uint16_t *Ab, const int32_t *Bb, int y, int32_t &ha, int32_t &ca, int32_t &hb,
+                       int32_t &cb) {
+            const int i = (y + 1) * LR_AB + x + 1;
+            const uint16_t *A = Ab + i;
+            const int32_t *B = Bb + i;
+            ca = (Ab + oz)[i];
+            ha = (int32_t)A[-1] + ca + A[1];
+            cb = B[0];
+            hb = B[-1] + B[0] + B[1];
+        };
+        // r = 2: P = 5 H + C of the odd rows; r = 1: H of the current row and
+        // Q = 3 H + C of the rows above and below
+        int32_t pa = 0, pb = 0, ha1 = 0, ca1 = 0, hb1 = 0, cb1 = 0, qa0 = 0, qb0 = 0;
+        if (r0) {
+            int32_t ha, ca, hb, cb;
+            row(Ab0, Bb0, L.ry - 1, ha, ca, hb, cb);
+            pa = 5 * ha + ca;
+            pb = 5 * hb + cb;
+        }
+        if (r1) {
+            int32_t ha0, ca0, hb0, cb0;
+            row(Ab1, Bb1, L.ry - 1, ha0, ca0, hb0, cb0);
+            row(Ab1, Bb1, L.ry, ha1, ca1, hb1, cb1);
+            qa0 = 3 * ha0 + ca0;
+            qb0 = 3 * hb0 + cb0;
+        }
+#pragma unroll
+        for (int j = 0; j < LR_SGR_ROWS / 4; j++) {
+            const int y = L.ry + j;
+            if (y >= L.end) break;
+            const int32_t uu = Oq[y * LR_TW + x];
+            int32_t f0 = 0, f1 = 0;
+            if (r0) {
+                if ((j & 1) == 0) {
+                    int32_t ha, ca, hb, cb;
+                    row(Ab0, Bb0, y + 1, ha, ca, hb, cb);
+                    const int32_t na = 5 * ha + ca, nb = 5 * hb + cb;
+                    f0 = r2s((pa + na) * uu + (pb + nb), 9);
+                    pa = na;
+                    pb = nb;
+                } else {
+                    f0 = r2s(pa * uu + pb, 8);
+                }
+            }
+            if (r1) {
+                int32_t ha2, ca2, hb2, cb2;
+                row(Ab1, Bb1, y + 1, ha2, ca2, hb2, cb2);
+                const int32_t qa2 = 3 * ha2 + ca2, qb2 = 3 * hb2 + cb2;
+                f1 = r2s((4 * ha1 + qa0 + qa2) * uu + (4 * hb1 + qb0 + qb2), 9);
+                qa0 = 3 * ha1 + ca1;
+                qb0 = 3 * hb1 + cb1;
+                ha1 = ha2;
+                ca1 = ca2;
+                hb1 = hb2;
+                cb1 = cb2;
+            }
+            sink(ya + y, x, sgr_project(uu, f0, f1, xq0, xq1, mx));
+        }
+    }
+}
 
 }  // namespace pf
