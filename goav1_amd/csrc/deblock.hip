@@ -341,9 +341,16 @@ __device__ __forceinline__ void deblock_dir(const FrameDesc *__restrict__ descs,
     }
     __syncthreads();
     const PlaneView &pv = f.p[pl];
-    for (int c = 0; c < 3; c++) {
+    // the three lists laid end to end from wave boundaries: one pass covers
+    // all of them when they fit the workgroup's lanes, and every wave still
+    // filters a single width
+    const int n0 = cnt[0], n1 = cnt[1], n2 = cnt[2];
+    const int o1 = (n0 + 63) & ~63, o2 = o1 + ((n1 + 63) & ~63), tot = o2 + ((n2 + 63) & ~63);
+    for (int i0 = tid; i0 < tot; i0 += 256) {
+        const int c = i0 < o1 ? 0 : i0 < o2 ? 1 : 2;   // uniform over a wave
+        const int i = i0 - (c == 0 ? 0 : c == 1 ? o1 : o2);
         const int w = c == 0 ? 16 : c == 1 ? (P.luma ? 8 : 6) : 4;
-        for (int i = tid; i < cnt[c]; i += 256) {
+        if (i < (c == 0 ? n0 : c == 1 ? n1 : n2)) {
             const uint32_t e = list[c][i];
             const int ex = tx * 64 + (int)(e & 63), ey = ty * DB_ROWS + (int)((e >> 6) & 15), el = (int)(e >> 10);
             int E, I;
