@@ -273,6 +273,18 @@ def batch_leg(runner, steps, nframes):
     runner.set_graphs(False)
     run(1, per_frame=True)
     ms_frames, _ = run(max(1, steps // 2), per_frame=True)
+    # the fused pipeline: one persistent launch for deblock, CDEF and LR
+    from goav1_amd.lib import PIPELINE_FUSED, PIPELINE_STAGED
+    runner.set_pipeline(PIPELINE_FUSED)
+    run(1)
+    ms_fused, _ = run(steps)
+    runner.set_graphs(True)
+    run(2)
+    ms_fused_graph, _ = run(steps)
+    runner.set_graphs(False)
+    run(1, per_frame=True)
+    ms_fused_frames, _ = run(max(1, steps // 2), per_frame=True)
+    runner.set_pipeline(PIPELINE_STAGED)
     luma = sum(fr.w * fr.h for fr in frames)
     samples = sum(fr.samples for fr in frames)
     stage_bytes = sum(2 * fr.samples * (1 if fr.bd == 8 else 2) for fr in frames)
@@ -287,6 +299,9 @@ def batch_leg(runner, steps, nframes):
             "graph_tuned_mpx_s": round(luma / (ms_graph / 1e3) / 1e6, 1),
             "one_call_per_frame_mpx_s": round(luma / (ms_frames / 1e3) / 1e6, 1),
             "one_call_per_frame_graph_mpx_s": round(luma / (ms_frames_graph / 1e3) / 1e6, 1),
+            "fused_mpx_s": round(luma / (ms_fused / 1e3) / 1e6, 1),
+            "fused_graph_mpx_s": round(luma / (ms_fused_graph / 1e3) / 1e6, 1),
+            "fused_one_call_per_frame_mpx_s": round(luma / (ms_fused_frames / 1e3) / 1e6, 1),
             "algorithmic_bytes_per_stage_pass": stage_bytes, "stages": stages,
             "generation_s": round(gen_s, 1)}
 
@@ -435,7 +450,7 @@ def main():
         return
     import torch
     from goav1_amd import abi, synth
-    from goav1_amd.lib import DeviceFrame, DeviceSideData, PostFilterRunner
+    from goav1_amd.lib import PIPELINE_FUSED, PIPELINE_STAGED, DeviceFrame, DeviceSideData, PostFilterRunner
 
     frames = synth.sequence_1080p10(seed=1, nframes=args.frames)
     runner = PostFilterRunner()
@@ -509,6 +524,24 @@ def main():
     # the same frames one apply per frame (decoder order, no batching), for reference
     frame_ms = timed(pristine(args.steps), False, False)
 
+    # the fused pipeline (deblock, LR save, CDEF and LR of all frames in one
+    # persistent launch with per-band dependency counters) on the same steps
+    def median_ms(batched):
+        res = []
+        for _ in range(3):
+            res.append(timed(pristine(args.steps), batched, False))
+        return sorted(res)[1]
+
+    runner.set_pipeline(PIPELINE_FUSED)
+    timed(pristine(args.warmup), True, False)
+    fused_ms = median_ms(True)
+    timed(pristine(1), False, False)
+    fused_frame_ms = median_ms(False)
+    timed(pristine(args.steps), True, True)
+    fprof = runner.profile_read()
+    runner.profile(False)
+    runner.set_pipeline(PIPELINE_STAGED)
+
     ms_per_step = gpu_ms / args.steps
     luma_px = sum(fr.w * fr.h for fr in frames)
     samples = sum(fr.samples for fr in frames)
@@ -540,6 +573,11 @@ def main():
         "msamples_per_s": round(samples * args.steps / (gpu_ms / 1e3) / 1e6, 2),
         "per_frame_apply_mpx_s": round(luma_px * args.steps / (frame_ms / 1e3) / 1e6, 2),
         "autotuned_modes": tuned_modes,
+        "fused_pipeline": {
+            "mpx_s": round(luma_px * args.steps / (fused_ms / 1e3) / 1e6, 2),
+            "per_frame_apply_mpx_s": round(luma_px * args.steps / (fused_frame_ms / 1e3) / 1e6, 2),
+            "stages": {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1]} for k, v in fprof.items()},
+        },
         "profiled_pass_ms_per_step": round(prof_ms / args.steps, 4),
         "timed_regions_ms": [round(r[0], 3) for r in regions],
         "wall_s_timed": round(wall, 4),

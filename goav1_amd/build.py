@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libav1pf.so")
-SOURCES = ["av1pf_api.hip", "deblock.hip", "cdef.hip", "lr.hip", "superres.hip", "border.hip", "lfbuild.hip", "grain.hip", "search.hip",
+# chain.hip compiles deblock.hip, cdef.hip, lr.hip and superres.hip into its
+# own translation unit (the fused chain kernel inlines their tile bodies)
+SOURCES = ["av1pf_api.hip", "chain.hip", "border.hip", "lfbuild.hip", "grain.hip", "search.hip",
            "lr_search.hip", "nn.hip", "search_finish.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

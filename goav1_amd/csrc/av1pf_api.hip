@@ -135,6 +135,7 @@ struct av1pf_ctx {
     std::map<std::vector<int>, GraphClass> gclass;
     bool graphs = false;
     int forced[kTunable] = { -1, -1, -1 };   // test override of the stage modes
+    int pipeline = AV1PF_PIPELINE_STAGED;    // staged launches or the fused chain launch
     hipStream_t cap = nullptr;               // capture stream of the graphs
     hipEvent_t g_fork = nullptr, g_up = nullptr, g_join = nullptr;
     void drop_graphs(const std::vector<int> &key)
@@ -176,7 +177,7 @@ namespace {
 
 enum { ST_DEBLOCK = 0, ST_LR_SAVE, ST_CDEF, ST_LR, ST_COPY, ST_GRAIN_T, ST_GRAIN_A, ST_SEARCH, ST_SEARCH_SELECT,
        ST_LR_SEARCH, ST_NN_FORWARD, ST_NN_TRAIN, ST_SEARCH_LEARNED, ST_LR_SEARCH_EXH, ST_SUPERRES,
-       ST_DEBLOCK_BUILD, ST_EXTEND };
+       ST_DEBLOCK_BUILD, ST_EXTEND, ST_CHAIN };
 
 int fail(av1pf_ctx *c, int code, const char *msg)
 {
@@ -766,6 +767,104 @@ int plan_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reque
     return AV1PF_OK;
 }
 
+// The fused launch's queue: the tiles of the deblock, context-save, CDEF,
+// super-resolution and LR stage lists (the same tiles the stage launches
+// cover), placed in steps by band (deblock-V of band p in step p, deblock-H
+// p+3, save t+5, CDEF u+6, super-resolution p+8, LR t+10: every counter a tile
+// waits for belongs to tiles at least two steps earlier), and the tile count of
+// every counter.  The queue is stored as one part per XCD: the tiles of a
+// band row of a stage go to the parts in contiguous column chunks, and each
+// part is in step order.
+struct ChainPlan {
+    std::vector<ChainItem> items;
+    std::vector<ChainFrame> frames;
+    std::vector<uint32_t> targets;
+    uint32_t qbase[kXcds + 1];   // the queue's part per XCD
+};
+
+int plan_chain(const BatchPlan &P, int n, ChainPlan &C)
+{
+    static const int kLag[CS_N] = { 0, 3, 5, 6, 8, 10 };
+    const SegList *lists[CS_N] = { &P.dbv, &P.dbh, &P.save, &P.cdef, &P.sr, &P.lr };
+    C.frames.assign(n, ChainFrame{});
+    // bands per frame: the luma rows of the frame and of its LR / upscaled view
+    for (int i = 0; i < n; i++) {
+        const FrameDesc &D = P.descs[i];
+        const int h = std::max(D.v[VIEW_A].p[0].ah, D.v[VIEW_U].p[0].ah);
+        C.frames[i].rows = std::max(1, (h + 63) / 64);
+    }
+    auto tile_row = [&](int st, int fr, int pl, int ty) {
+        const int sy = pl > 0 ? P.descs[fr].v[VIEW_A].ssy : 0;
+        if (st == CS_DBV || st == CS_DBH) return (ty * DB_ROWS * 4) >> (6 - sy);
+        if (st == CS_SR) return (ty * SR_ROWS * SR_CHUNKS) >> (6 - sy);
+        return ty;
+    };
+    // rows of every frame must hold its tiles (the save and LR stripes may
+    // use one more: counters are [rows + 1] per stage)
+    for (int st = 0; st < CS_N; st++) {
+        const SegList &L = *lists[st];
+        for (size_t k = 0; k < L.v.size(); k++) {
+            const Seg &g = L.v[k];
+            const int end = k + 1 < L.v.size() ? L.v[k + 1].base : L.total;
+            const int ny = (end - g.base) / g.nx;
+            const int last = tile_row(st, g.frame, g.plane, ny - 1);
+            ChainFrame &F = C.frames[g.frame];
+            if (st == CS_SAVE || st == CS_LR) {
+                if (last > F.rows) return AV1PF_ERR_INVALID_ARG;
+            } else if (last >= F.rows) {
+                F.rows = last + 1;
+            }
+        }
+    }
+    int ncnt = 0;
+    for (int i = 0; i < n; i++) {
+        ChainFrame &F = C.frames[i];
+        const FrameDesc &D = P.descs[i];
+        F.cbase = ncnt;
+        ncnt += CS_N * (F.rows + 1);
+        F.sr_src = D.sr_src == VIEW_B ? CS_CDEF : CS_DBH;
+        F.lr_src = D.lr_src == VIEW_S ? CS_SR : D.lr_src == VIEW_B ? CS_CDEF : CS_DBH;
+    }
+    C.targets.assign(ncnt, 0);
+    // counting sort by (part, step); inside a step, stage then frame then
+    // tile order
+    int maxstep = 0;
+    for (int i = 0; i < n; i++) maxstep = std::max(maxstep, C.frames[i].rows + 1 + kLag[CS_LR]);
+    const int nsteps = maxstep + 1;
+    std::vector<int> at(kXcds * nsteps + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        for (int st = 0; st < CS_N; st++) {
+            const SegList &L = *lists[st];
+            for (size_t k = 0; k < L.v.size(); k++) {
+                const Seg &g = L.v[k];
+                const int end = k + 1 < L.v.size() ? L.v[k + 1].base : L.total;
+                const int ny = (end - g.base) / g.nx;
+                const ChainFrame &F = C.frames[g.frame];
+                for (int ty = 0; ty < ny; ty++) {
+                    const int row = tile_row(st, g.frame, g.plane, ty);
+                    const int step = row + kLag[st];
+                    if (pass == 0 && st != CS_LR) C.targets[F.cbase + st * (F.rows + 1) + row] += g.nx;
+                    for (int tx = 0; tx < g.nx; tx++) {
+                        const int slot = (tx * kXcds / g.nx) * nsteps + step;
+                        if (pass == 0)
+                            at[slot + 1]++;
+                        else
+                            C.items[at[slot]++] = ChainItem{ (uint16_t)g.frame, (uint8_t)st,
+                                                             (uint8_t)(g.plane < 0 ? 0 : g.plane), (uint16_t)tx,
+                                                             (uint16_t)ty };
+                    }
+                }
+            }
+        }
+        if (pass == 0) {
+            for (size_t k = 1; k < at.size(); k++) at[k] += at[k - 1];
+            C.items.resize(at.back());
+            for (int q = 0; q <= kXcds; q++) C.qbase[q] = (uint32_t)at[q * nsteps];
+        }
+    }
+    return AV1PF_OK;
+}
+
 int ensure(av1pf_ctx *c, void **buf, size_t *cap, size_t need)
 {
     if (need <= *cap) return AV1PF_OK;
@@ -813,6 +912,27 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
     }
     const size_t qoff = off;   // work-queue counters of the tunable stages (one per XCD), zero at upload
     off = align256(off + kTunable * kXcds * sizeof(unsigned int));
+    // fused chain: queue, per-frame records, counter targets, then the counters
+    // and the queue head (zero at upload)
+    ChainPlan CP;
+    const bool fused = c->pipeline == AV1PF_PIPELINE_FUSED &&
+                       (P.dbv.total || P.dbh.total || P.save.total || P.cdef.total || P.sr.total || P.lr.total);
+    size_t ch_items = 0, ch_frames = 0, ch_tgt = 0, ch_cnt = 0, ch_heads = 0, ch_zero = 0;
+    if (fused) {
+        if (n > 65535) return fail(c, AV1PF_ERR_INVALID_ARG, "the fused chain takes at most 65535 frames per call");
+        if ((rc = plan_chain(P, n, CP))) return fail(c, rc, "fused chain: a stage's tiles exceed the frame's bands");
+        ch_items = off;
+        off = align256(off + sizeof(ChainItem) * CP.items.size());
+        ch_frames = off;
+        off = align256(off + sizeof(ChainFrame) * n);
+        ch_tgt = off;   // targets, then the part bases
+        off = align256(off + sizeof(uint32_t) * (CP.targets.size() + kXcds + 1));
+        // counters, then (in lines of their own) the part heads and the abort word
+        ch_cnt = off;
+        ch_heads = align256(off + sizeof(uint32_t) * CP.targets.size());
+        ch_zero = ch_heads + 256 - ch_cnt;
+        off = align256(off + ch_zero);
+    }
 
     const size_t tsz = GR_FRAME_BYTES;
     const bool side = P.tl.total > 0;
@@ -887,6 +1007,13 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             memcpy(h + offs[k] + sizeof(Seg) * lists[k]->v.size(), &sentinel, sizeof(Seg));
         }
         memset(h + qoff, 0, kTunable * kXcds * sizeof(unsigned int));
+        if (fused) {
+            memcpy(h + ch_items, CP.items.data(), sizeof(ChainItem) * CP.items.size());
+            memcpy(h + ch_frames, CP.frames.data(), sizeof(ChainFrame) * n);
+            memcpy(h + ch_tgt, CP.targets.data(), sizeof(uint32_t) * CP.targets.size());
+            memcpy(h + ch_tgt + sizeof(uint32_t) * CP.targets.size(), CP.qbase, sizeof(CP.qbase));
+            memset(h + ch_cnt, 0, ch_zero);
+        }
     };
     // shape class: everything a launch's arguments depend on besides data
     std::vector<int> key;
@@ -896,6 +1023,8 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         key.push_back(lists[k]->total);
         key.push_back((int)lists[k]->v.size());
     }
+    key.push_back(fused ? 1 : 0);
+    key.push_back((int)CP.targets.size());
     StageModes modes;
     if (auto it = c->tuned.find(key); it != c->tuned.end()) modes = it->second;
     for (int i = 0; i < kTunable; i++)
@@ -977,16 +1106,25 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
         } else {
             HIPCHK(c, hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
         }
-        if (P.dbv.total || P.dbh.total) {
+        int rc2;
+        if (fused) {
+            StageTimer t(c, ST_CHAIN, st, P.bytes_db + P.bytes_cdef + P.bytes_sr + P.bytes_lr);
+            unsigned *cnt = reinterpret_cast<unsigned *>(d + ch_cnt);
+            const unsigned *tgt = reinterpret_cast<const unsigned *>(d + ch_tgt);
+            HIPCHK(c, launch_chain(dd, reinterpret_cast<const ChainFrame *>(d + ch_frames),
+                                   reinterpret_cast<const ChainItem *>(d + ch_items), (int)CP.items.size(),
+                                   tgt + CP.targets.size(), reinterpret_cast<unsigned *>(d + ch_heads), cnt, tgt,
+                                   st));
+        }
+        if (!fused && (P.dbv.total || P.dbh.total)) {
             StageTimer t(c, ST_DEBLOCK, st, P.bytes_db);
             HIPCHK(c, launch_deblock(dd, segp(0), nseg(0), P.dbv.total, segp(1), nseg(1), P.dbh.total, st));
         }
-        if (P.save.total) {
+        if (!fused && P.save.total) {
             StageTimer t(c, ST_LR_SAVE, st, 0);
             HIPCHK(c, launch_lr_save(dd, segp(2), nseg(2), P.save.total, st));
         }
-        int rc2;
-        if (P.cdef.total) {
+        if (!fused && P.cdef.total) {
             {
                 StageTimer t(c, ST_CDEF, st, P.bytes_cdef);
                 HIPCHK(c, launch_cdef(dd, segp(3), nseg(3), P.cdef.total, launch_of(0), st));
@@ -994,11 +1132,11 @@ int run_batch(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_reques
             if ((rc2 = tune(0, [&](const StageLaunch &L) { return launch_cdef(dd, segp(3), nseg(3), P.cdef.total, L, st); })))
                 return rc2;
         }
-        if (P.sr.total) {
+        if (!fused && P.sr.total) {
             StageTimer t(c, ST_SUPERRES, st, P.bytes_sr);
             HIPCHK(c, launch_superres(dd, segp(10), nseg(10), P.sr.total, st));
         }
-        if (P.lr.total) {
+        if (!fused && P.lr.total) {
             {
                 StageTimer t(c, ST_LR, st, P.bytes_lr);
                 HIPCHK(c, launch_lr(dd, segp(4), nseg(4), P.lr.total, launch_of(1), st));
@@ -1260,6 +1398,15 @@ int av1pf_autotune(av1pf_ctx *c, int n, const av1pf_frame *frames, const av1pf_r
     return AV1PF_OK;
 }
 
+int av1pf_set_pipeline(av1pf_ctx *c, int mode)
+{
+    if (!c) return AV1PF_ERR_INVALID_ARG;
+    if (mode != AV1PF_PIPELINE_STAGED && mode != AV1PF_PIPELINE_FUSED)
+        return fail(c, AV1PF_ERR_INVALID_ARG, "pipeline mode is AV1PF_PIPELINE_STAGED or AV1PF_PIPELINE_FUSED");
+    c->pipeline = mode;
+    return AV1PF_OK;
+}
+
 int av1pf_set_stage_mode(av1pf_ctx *c, uint32_t stage, int mode)
 {
     if (!c || mode < -1 || mode > 8) return AV1PF_ERR_INVALID_ARG;
@@ -1287,6 +1434,7 @@ int av1pf_sync(av1pf_ctx *c, void *stream)
     if (flag) {
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), c->dev_err);
         HIPCHK(c, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+        if (flag & 4) return fail(c, AV1PF_ERR_DEVICE, "fused chain: a tile's dependency wait did not complete");
         if (flag & 2) {
             // an aborted selection leaves its barrier counters behind: start over
             HIPCHK(c, hipMemset(c->finish_state, 0, finish_state_bytes()));

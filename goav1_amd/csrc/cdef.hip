@@ -546,24 +546,34 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
     }
 }
 
+// LDS of a CDEF unit: the tile, then its copy at TSHIFT; per 8x8 block the
+// direction, variance, direction costs and filter record.
+struct __align__(16) CdefLds {
+    uint16_t tile[TSHIFT + TROWS * TPITCH + 1];
+    uint8_t bdir[64];
+    int32_t bvar[64];
+    uint32_t dcost[64][8];
+    BlockRec brec[64];
+};
+
+// Unit (uc, ur) of a frame: the deblocked frame A into its scratch B.
+__device__ __forceinline__ void cdef_tile(const FrameDesc &D, int uc, int ur, CdefLds &S)
+{
+    if (D.v[VIEW_A].bd == 8)
+        cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, S.tile, S.bdir, S.bvar, S.dcost, S.brec);
+    else
+        cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, S.tile, S.bdir, S.bvar, S.dcost, S.brec);
+}
+
 // 6 workgroups per CU: 26.7 KB of LDS each, and at most 80 VGPRs
 __global__ __launch_bounds__(256, 6) void cdef_kernel(const FrameDesc *__restrict__ descs,
                                                    const Seg *__restrict__ segs, int nseg, int total, StageLaunch L)
 {
-    __shared__ __align__(16) uint16_t tile[TSHIFT + TROWS * TPITCH + 1];   // the tile, then its copy at TSHIFT
-    __shared__ uint8_t bdir[64];
-    __shared__ int32_t bvar[64];
-    __shared__ uint32_t dcost[64][8];   // direction costs per block
-    __shared__ BlockRec brec[64];
+    __shared__ CdefLds S;
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
-        const FrameDesc &D = descs[sg.frame];
         const int t = u - sg.base;
-        const int uc = t % sg.nx, ur = t / sg.nx;
-        if (D.v[VIEW_A].bd == 8)
-            cdef_unit<uint8_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
-        else
-            cdef_unit<uint16_t>(D.v[VIEW_A], D.v[VIEW_B], D.cd, D.err, uc, ur, tile, bdir, bvar, dcost, brec);
+        cdef_tile(descs[sg.frame], t % sg.nx, t / sg.nx, S);
     });
 }
 

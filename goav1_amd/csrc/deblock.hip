@@ -268,6 +268,12 @@ __device__ __forceinline__ int unit_edge(const DbDesc &db, const DbPlaneK &P, in
     return unit_wd(mask_bit(w0, n, lane), mask_bit(w1, n, lane), P.luma ? mask_bit(w2, n, lane) : 0, P.luma);
 }
 
+// LDS of a deblock tile: the edge units of each width class, and their counts.
+struct DbLds {
+    uint32_t list[3][64 * DB_ROWS];
+    int cnt[3];
+};
+
 // Append one entry per lane with p set to an LDS list: one LDS atomic per
 // wave, each lane's slot from the count of set lanes below it.  Returns the
 // lane's slot (-1 when p is false).
@@ -292,18 +298,15 @@ __device__ __forceinline__ int wave_append(int *cnt, bool p)
 // LDS list per filter width; then the lists are filtered width by width, a
 // thread per unit, so every lane has an edge and the lanes of a wave share the
 // width (same loads, same filter branches).
+// The LDS the tile uses (DbLds) is passed in, so the stage kernels and the
+// chain kernel share the body.
 template <int DIR>
-__device__ __forceinline__ void deblock_dir(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+__device__ __forceinline__ void deblock_tile(const FrameDesc &D, int pl, int tx, int ty, DbLds &S)
 {
-    __shared__ uint32_t list[3][64 * DB_ROWS];
-    __shared__ int cnt[3];
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
+    uint32_t (*list)[64 * DB_ROWS] = S.list;
+    int *cnt = S.cnt;
     const FrameView &f = D.v[VIEW_A];
-    const int pl = sg.plane;
     const DbPlaneK P = db_plane(f, pl);
-    const int t = blockIdx.x - sg.base;
-    const int tx = t % sg.nx, ty = t / sg.nx;
     const int tid = threadIdx.x;
     if (tid < 3) cnt[tid] = 0;
     __syncthreads();
@@ -366,6 +369,15 @@ __device__ __forceinline__ void deblock_dir(const FrameDesc *__restrict__ descs,
             }
         }
     }
+}
+
+template <int DIR>
+__device__ __forceinline__ void deblock_dir(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
+{
+    __shared__ DbLds S;
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const int t = blockIdx.x - sg.base;
+    deblock_tile<DIR>(descs[sg.frame], sg.plane, t % sg.nx, t / sg.nx, S);
 }
 
 __global__ __launch_bounds__(256) void deblock_v_kernel(const FrameDesc *__restrict__ descs,

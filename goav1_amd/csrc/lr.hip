@@ -186,33 +186,41 @@ __device__ __forceinline__ void lr_tile(const PlaneView &sp, const PlaneView &dp
     }
 }
 
+// LDS of an LR tile: the stripe tile with its halo, and one buffer that holds
+// the Wiener intermediate or the self-guided A/B planes of both passes for a
+// part of the tile.
+constexpr int LR_ABN = LR_AB * (LR_SGR_ROWS + 2);
+struct __align__(16) LrLds {
+    uint16_t tile[LR_TW * LR_TH];
+    char work[LR_ABN * 12];
+};
+static_assert(LR_TH * 64 * 2 <= LR_ABN * 12, "Wiener intermediate fits the shared buffer");
+
+// Processing unit k of stripe st of plane pl of a frame.
+__device__ __forceinline__ void lr_tile_of(const FrameDesc &D, int pl, int k, int st, LrLds &S)
+{
+    int32_t *Bb0 = reinterpret_cast<int32_t *>(S.work);
+    int32_t *Bb1 = Bb0 + LR_ABN;
+    uint16_t *Ab0 = reinterpret_cast<uint16_t *>(S.work + LR_ABN * 8);
+    uint16_t *Ab1 = Ab0 + LR_ABN;
+    uint16_t *tmp = reinterpret_cast<uint16_t *>(S.work);
+    const FrameView &src = D.v[D.lr_src];
+    const FrameView &dst = D.v[D.lr_dst];
+    if (src.bd == 8)
+        lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, S.tile, tmp, Ab0, Bb0, Ab1, Bb1);
+    else
+        lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, S.tile, tmp, Ab0, Bb0, Ab1, Bb1);
+}
+
 // 6 workgroups per CU: 23.8 KB of LDS each, and at most 80 VGPRs
 __global__ __launch_bounds__(256, 6) void lr_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg,
                                                  int total, StageLaunch L)
 {
-    __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
-    // Wiener intermediate and the self-guided A/B planes of both passes for
-    // a part of the tile share one buffer
-    constexpr int AB = LR_AB * (LR_SGR_ROWS + 2);
-    __shared__ __align__(16) char work[AB * 12];
-    int32_t *Bb0 = reinterpret_cast<int32_t *>(work);
-    int32_t *Bb1 = Bb0 + AB;
-    uint16_t *Ab0 = reinterpret_cast<uint16_t *>(work + AB * 8);
-    uint16_t *Ab1 = Ab0 + AB;
-    uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
-    static_assert(LR_TH * 64 * 2 <= AB * 12, "Wiener intermediate fits the shared buffer");
+    __shared__ LrLds S;
     for_each_unit(total, L, [&](int u) {
         const Seg sg = segs[find_seg(segs, nseg, u)];
-        const FrameDesc &D = descs[sg.frame];
         const int t = u - sg.base;
-        const int k = t % sg.nx, st = t / sg.nx;
-        const FrameView &src = D.v[D.lr_src];
-        const FrameView &dst = D.v[D.lr_dst];
-        const int pl = sg.plane;
-        if (src.bd == 8)
-            lr_tile<uint8_t>(src.p[pl], dst.p[pl], D.lr[pl], 8, k, st, tile, tmp, Ab0, Bb0, Ab1, Bb1);
-        else
-            lr_tile<uint16_t>(src.p[pl], dst.p[pl], D.lr[pl], src.bd, k, st, tile, tmp, Ab0, Bb0, Ab1, Bb1);
+        lr_tile_of(descs[sg.frame], sg.plane, t % sg.nx, t / sg.nx, S);
     });
 }
 
@@ -238,18 +246,23 @@ __device__ __forceinline__ void lr_save_cols(const PlaneView &pv, const LrPlaneD
     }
 }
 
+// 256 columns (chunk xc) of the context rows of stripe st of plane pl.
+__device__ __forceinline__ void lr_save_tile(const FrameDesc &D, int pl, int xc, int st)
+{
+    const int x = xc * 256 + threadIdx.x;
+    const FrameView &f = D.v[VIEW_A];
+    const LrPlaneDesc &g = D.lr[pl];
+    const SrPlane *sr = g.sr ? &D.sr[pl] : nullptr;
+    const int maxv = (1 << f.bd) - 1;
+    if (f.bd == 8) lr_save_cols<uint8_t>(f.p[pl], g, sr, maxv, x, st);
+    else lr_save_cols<uint16_t>(f.p[pl], g, sr, maxv, x, st);
+}
+
 __global__ __launch_bounds__(256) void lr_save_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs, int nseg)
 {
     const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
     const int t = blockIdx.x - sg.base;
-    const int x = (t % sg.nx) * 256 + threadIdx.x, st = t / sg.nx;
-    const FrameView &f = D.v[VIEW_A];
-    const LrPlaneDesc &g = D.lr[sg.plane];
-    const SrPlane *sr = g.sr ? &D.sr[sg.plane] : nullptr;
-    const int maxv = (1 << f.bd) - 1;
-    if (f.bd == 8) lr_save_cols<uint8_t>(f.p[sg.plane], g, sr, maxv, x, st);
-    else lr_save_cols<uint16_t>(f.p[sg.plane], g, sr, maxv, x, st);
+    lr_save_tile(descs[sg.frame], sg.plane, t % sg.nx, t / sg.nx);
 }
 
 hipError_t launch_lr_save(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)

@@ -189,7 +189,7 @@ __device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, B
             const int u = u0 + k;
             if (u >= total) return;
             if (k) __syncthreads();   // the previous unit is done with the LDS
-            body(u);
+            [[clang::always_inline]] body(u);   // inlined even when the tile body has other callers
         }
     } else {
         for (int q = 0; q < kXcds; q++) {
@@ -201,7 +201,7 @@ __device__ __forceinline__ void for_each_unit(int total, const StageLaunch &L, B
                 __syncthreads();
                 const int u = next_unit;
                 if (u >= hi) break;
-                body(u);
+                [[clang::always_inline]] body(u);
             }
         }
     }
@@ -259,6 +259,31 @@ hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &re
 hipError_t launch_lr_mask(const float *logits, int nhfb, int nvfb, int hz, int vt, float thr, uint8_t *mask,
                           hipStream_t s);
 constexpr int kMaxSearchBlocks = 4096;   // filter blocks the device selection holds
+
+// ---- fused chain (chain.hip) ------------------------------------------------
+// Stages of the fused launch, in dependency order; every stage but LR keeps a
+// completion counter per frame and 64-luma-row band.
+enum { CS_DBV = 0, CS_DBH, CS_SAVE, CS_CDEF, CS_SR, CS_LR, CS_N };
+// One tile of the queue: the stage's tile (x, y) of plane `plane` (deblock
+// band, context-row chunk and stripe, CDEF unit column and row,
+// super-resolution tile, LR processing unit and stripe).
+struct ChainItem {
+    uint16_t frame;
+    uint8_t stage, plane;
+    uint16_t x, y;
+};
+// Per frame: band count, the first of its counters ([CS_N][rows + 1], stage
+// major) and the stages whose output super-resolution and LR read (-1: an
+// input of the call).
+struct ChainFrame {
+    int rows, cbase;
+    int sr_src, lr_src;
+};
+// The queue is kXcds parts, part q = items[qbase[q] .. qbase[q + 1]); head
+// holds a take counter per part, then the abort word, all zero at launch, as
+// are the counters cnt (targets tgt, same layout).
+hipError_t launch_chain(const FrameDesc *d, const ChainFrame *frames, const ChainItem *items, int nitems,
+                        const unsigned *qbase, unsigned *head, unsigned *cnt, const unsigned *tgt, hipStream_t s);
 
 // tile geometry helpers shared by host and device
 __host__ __device__ inline int lr_stripes(int ph, int sh, int off) { return (ph + off + sh - 1) / sh; }

@@ -145,19 +145,28 @@ __device__ __forceinline__ void sr_tile(const PlaneView &s, const PlaneView &d, 
     }
 }
 
-__global__ __launch_bounds__(256) void superres_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
-                                                       int nseg)
+// LDS of a super-resolution tile: its source rows.
+struct __align__(16) SrLds {
+    uint16_t rows[SR_TROWS][SR_SPAN];
+};
+
+// Tile (tx, ty) of plane pl of a frame.
+__device__ __forceinline__ void sr_tile_of(const FrameDesc &D, int pl, int tx, int ty, SrLds &L)
 {
-    __shared__ __align__(16) uint16_t lds[SR_TROWS][SR_SPAN];
-    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const FrameDesc &D = descs[sg.frame];
-    const int t = blockIdx.x - sg.base;
-    const int pl = sg.plane;
     const FrameView &S = D.v[D.sr_src];
     const FrameView &O = D.v[D.sr_dst];
     const int maxv = (1 << S.bd) - 1;
-    if (S.bd == 8) sr_tile<uint8_t>(S.p[pl], O.p[pl], D.sr[pl], maxv, t % sg.nx, t / sg.nx, lds);
-    else sr_tile<uint16_t>(S.p[pl], O.p[pl], D.sr[pl], maxv, t % sg.nx, t / sg.nx, lds);
+    if (S.bd == 8) sr_tile<uint8_t>(S.p[pl], O.p[pl], D.sr[pl], maxv, tx, ty, L.rows);
+    else sr_tile<uint16_t>(S.p[pl], O.p[pl], D.sr[pl], maxv, tx, ty, L.rows);
+}
+
+__global__ __launch_bounds__(256) void superres_kernel(const FrameDesc *__restrict__ descs, const Seg *__restrict__ segs,
+                                                       int nseg)
+{
+    __shared__ SrLds L;
+    const Seg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const int t = blockIdx.x - sg.base;
+    sr_tile_of(descs[sg.frame], sg.plane, t % sg.nx, t / sg.nx, L);
 }
 
 hipError_t launch_superres(const FrameDesc *d, const Seg *segs, int nseg, int total, hipStream_t s)

@@ -29,11 +29,12 @@ EXPORTS = ["av1pf_create", "av1pf_destroy", "av1pf_version", "av1pf_last_error",
            "av1pf_lr_search_masked", "av1pf_cdef_search_shortlist", "av1pf_search_learned", "av1pf_search_shortlist_size",
            "av1pf_profile", "av1pf_profile_read", "av1pf_set_graphs", "av1pf_autotune", "av1pf_set_stage_mode",
            "av1pf_lr_search_exhaustive", "av1pf_superres", "av1pf_deblock_build",
-           "av1pf_cdef_search_apply", "av1pf_apply_noalloc"]
+           "av1pf_cdef_search_apply", "av1pf_apply_noalloc", "av1pf_set_pipeline"]
 STAGE_NAMES = ["deblock", "lr_save", "cdef", "lr", "copy", "grain_template", "grain_apply", "cdef_search",
                "cdef_search_select", "lr_search", "nn_forward", "nn_train", "cdef_search_learned",
-               "lr_search_exhaustive", "superres", "deblock_build", "extend"]
-PROFILE_STAGES = 17
+               "lr_search_exhaustive", "superres", "deblock_build", "extend", "chain"]
+PROFILE_STAGES = 18
+PIPELINE_STAGED, PIPELINE_FUSED = 0, 1
 
 
 class Av1pfError(RuntimeError):
@@ -66,6 +67,7 @@ def load():
     lib.av1pf_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.Frame), C.POINTER(abi.Request),
                                    C.POINTER(C.c_int32), C.c_void_p]
     lib.av1pf_set_stage_mode.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
+    lib.av1pf_set_pipeline.argtypes = [C.c_void_p, C.c_int]
     lib.av1pf_deblock.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.DeblockParams), C.c_void_p]
     lib.av1pf_cdef.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.CdefParams), C.c_void_p]
     lib.av1pf_lr_save_boundaries.argtypes = [C.c_void_p, C.POINTER(abi.Frame), C.POINTER(abi.LRParams), C.c_int, C.c_void_p]
@@ -335,6 +337,12 @@ class PostFilterRunner:
         """Force a stage's work distribution (abi.STAGE_CDEF / _LR / _FILMGRAIN;
         1, 2, 4 tiles per workgroup, 0 = persistent queue, -1 = automatic)."""
         self._check(self.lib.av1pf_set_stage_mode(self.h, int(stage), int(mode)))
+
+    def set_pipeline(self, mode):
+        """PIPELINE_STAGED (one launch per stage) or PIPELINE_FUSED (deblock,
+        LR context save, CDEF, super-resolution and LR in one persistent
+        launch with per-band dependency counters); same results."""
+        self._check(self.lib.av1pf_set_pipeline(self.h, int(mode)))
 
     # stage-level entry points (the stage functions the chain composes)
     def deblock(self, frame, side, stream=None):

@@ -302,6 +302,23 @@ int  av1pf_autotune(av1pf_ctx *ctx, int n, const av1pf_frame *frames, const av1p
                     void *stream);
 int  av1pf_set_stage_mode(av1pf_ctx *ctx, uint32_t stage, int mode);
 
+/* ---- pipeline mode (no counterpart in the reference's GPU-less design; the
+ * counterpart of its superblock-row pipelines, decoder/postfilter_sbrow.go:9-37
+ * and postfilter_rowpipe.go:6-27) -------------------------------------------
+ * AV1PF_PIPELINE_STAGED (default): one launch per stage over all frames of a
+ * call, ordered by the stream.
+ * AV1PF_PIPELINE_FUSED: deblocking, the LR context-row save, CDEF,
+ * super-resolution and LR of all frames of a call run in ONE persistent
+ * launch: the stages' tiles form a device work queue in dependency order and
+ * each frame keeps a completion counter per stage and 64-row band, so a band
+ * is CDEF-filtered as soon as the deblocked rows it reads are final, and
+ * restored as soon as its CDEF output and context rows are.  Same results
+ * bit for bit.  Applies to every later call of the handle (graphs included);
+ * film grain, copies and border extension stay separate launches.          */
+#define AV1PF_PIPELINE_STAGED 0
+#define AV1PF_PIPELINE_FUSED  1
+int  av1pf_set_pipeline(av1pf_ctx *ctx, int mode);
+
 /* Wait for all work the handle has queued on `stream`.                      */
 int  av1pf_sync(av1pf_ctx *ctx, void *stream);
 
@@ -487,8 +504,9 @@ int  av1pf_search_learned(av1pf_ctx *ctx, av1pf_nn *nn, const av1pf_frame *src, 
  * Stage ids: 0 deblock, 1 lr_save, 2 cdef, 3 lr, 4 copy, 5 grain_template,
  * 6 grain_apply, 7 cdef_search, 8 cdef_search_select, 9 lr_search,
  * 10 nn_forward, 11 nn_train, 12 cdef_search_learned, 13 lr_search_exhaustive,
- * 14 superres, 15 deblock_build, 16 border extension.                      */
-#define AV1PF_PROFILE_STAGES 17
+ * 14 superres, 15 deblock_build, 16 border extension, 17 fused chain
+ * (AV1PF_PIPELINE_FUSED: deblock + LR save + CDEF + superres + LR).        */
+#define AV1PF_PROFILE_STAGES 18
 int  av1pf_profile(av1pf_ctx *ctx, int enable);
 int  av1pf_profile_read(av1pf_ctx *ctx, double ms[AV1PF_PROFILE_STAGES], int count[AV1PF_PROFILE_STAGES],
                         double bytes[AV1PF_PROFILE_STAGES]);

@@ -86,10 +86,14 @@ def test_oracle_superres_extends_the_upscaled_frame():
 
 
 # ---------------------------------------------------------------- GPU parity
-@pytest.fixture(scope="module")
-def runner():
-    from goav1_amd.lib import PostFilterRunner
+@pytest.fixture(scope="module", params=["staged", "fused"])
+def runner(request):
+    """A handle per pipeline mode: every test of the module runs once with one
+    launch per stage and once with the fused chain launch."""
+    from goav1_amd.lib import PIPELINE_FUSED, PostFilterRunner
     r = PostFilterRunner()
+    if request.param == "fused":
+        r.set_pipeline(PIPELINE_FUSED)
     yield r
     r.close()
 

@@ -120,10 +120,14 @@ def _assert_same(a, b, what, full=True):
         raise AssertionError(f"{what}: " + "; ".join(msgs))
 
 
-@pytest.fixture(scope="module")
-def runner():
-    from goav1_amd.lib import PostFilterRunner
+@pytest.fixture(scope="module", params=["staged", "fused"])
+def runner(request):
+    """A handle per pipeline mode: every test of the module runs once with one
+    launch per stage and once with the fused chain launch."""
+    from goav1_amd.lib import PIPELINE_FUSED, PostFilterRunner
     r = PostFilterRunner()
+    if request.param == "fused":
+        r.set_pipeline(PIPELINE_FUSED)
     yield r
     r.close()
 
@@ -201,11 +205,13 @@ def test_superres_chain(runner, g):
 
 
 @pytest.mark.gpu
-def test_superres_chain_1080p10():
+@pytest.mark.parametrize("pipeline", [0, 1], ids=["staged", "fused"])
+def test_superres_chain_1080p10(pipeline):
     from goav1_amd.lib import PostFilterRunner
     fr = _frame_sr(303, 1920, 1080, 12, 10, (1, 1), grain=True)
-    print(f"superres chain 1080p10 {fr.w}->1920", flush=True)
+    print(f"superres chain 1080p10 {fr.w}->1920 pipeline {pipeline}", flush=True)
     r = PostFilterRunner()
+    r.set_pipeline(pipeline)
     try:
         _check([fr], _gpu_chain(r, [fr], batch=False))
     finally:
@@ -237,14 +243,16 @@ def test_superres_mixed_batch(runner):
 
 
 @pytest.mark.gpu
-def test_superres_graph_replays():
+@pytest.mark.parametrize("pipeline", [0, 1], ids=["staged", "fused"])
+def test_superres_graph_replays(pipeline):
     from goav1_amd.lib import PostFilterRunner
     frames = _mixed_sr_frames()
     r = PostFilterRunner()
+    r.set_pipeline(pipeline)
     try:
         r.set_graphs(True)
         for rep in range(3):
-            print(f"superres graph replay {rep}", flush=True)
+            print(f"superres graph replay {rep} pipeline {pipeline}", flush=True)
             _check(frames, _gpu_chain(r, frames, batch=True))
     finally:
         r.close()

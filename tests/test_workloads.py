@@ -63,10 +63,14 @@ def _check(frames, wants, devs, disps, what):
             _same(disps[i].to_numpy(), wdisp, tag + " display", full=False)
 
 
-@pytest.fixture(scope="module")
-def runner():
-    from goav1_amd.lib import PostFilterRunner
+@pytest.fixture(scope="module", params=["staged", "fused"])
+def runner(request):
+    """A handle per pipeline mode: every test of the module runs once with one
+    launch per stage and once with the fused chain launch."""
+    from goav1_amd.lib import PIPELINE_FUSED, PostFilterRunner
     r = PostFilterRunner()
+    if request.param == "fused":
+        r.set_pipeline(PIPELINE_FUSED)
     yield r
     r.close()
 
