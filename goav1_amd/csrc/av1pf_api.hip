@@ -1603,10 +1603,11 @@ int av1pf_cdef_search(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame *r
     int rc = validate_frame(c, src);
     if (rc) return rc;
     if ((rc = validate_frame(c, recon))) return rc;
-    if (src->width != recon->width || src->height != recon->height || recon->bit_depth != 8 ||
-        src->bit_depth != 8 || recon->ss_x != 1 || recon->ss_y != 1 || src->ss_x != 1 || src->ss_y != 1 ||
-        recon->mono || (recon->width & 7) || (recon->height & 7))
-        return fail(c, AV1PF_ERR_INVALID_ARG, "search needs matching 8-bit 4:2:0 frames with sizes a multiple of 8");
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != recon->bit_depth ||
+        (recon->bit_depth != 8 && recon->bit_depth != 10) || recon->ss_x != 1 || recon->ss_y != 1 || src->ss_x != 1 ||
+        src->ss_y != 1 || recon->mono || (recon->width & 7) || (recon->height & 7))
+        return fail(c, AV1PF_ERR_INVALID_ARG,
+                    "search needs matching 8-bit or 10-bit 4:2:0 frames with sizes a multiple of 8");
     if (damping < 3 || damping > 6) return fail(c, AV1PF_ERR_INVALID_ARG, "damping out of range");
     HIPCHK(c, hipSetDevice(c->device));
     const int mi_cols = ((recon->width + 7) >> 3) << 1, mi_rows = ((recon->height + 7) >> 3) << 1;
@@ -1647,8 +1648,9 @@ int av1pf_lr_search_masked(av1pf_ctx *c, const av1pf_frame *src, const av1pf_fra
     int rc = validate_frame(c, src);
     if (rc) return rc;
     if ((rc = validate_frame(c, recon))) return rc;
-    if (src->width != recon->width || src->height != recon->height || src->bit_depth != 8 || recon->bit_depth != 8)
-        return fail(c, AV1PF_ERR_INVALID_ARG, "lr search needs two 8-bit frames of one size");
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != recon->bit_depth ||
+        (recon->bit_depth != 8 && recon->bit_depth != 10))
+        return fail(c, AV1PF_ERR_INVALID_ARG, "lr search needs two 8-bit or two 10-bit frames of one size");
     if (unit_size != 32 && unit_size != 64 && unit_size != 128 && unit_size != 256)
         return fail(c, AV1PF_ERR_INVALID_ARG, "lr unit size must be 32, 64, 128 or 256");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1657,7 +1659,7 @@ int av1pf_lr_search_masked(av1pf_ctx *c, const av1pf_frame *src, const av1pf_fra
     const FrameView sv = view_of(src), rv = view_of(recon);
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_LR_SEARCH, s, 2.0 * W * H);
-    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], unit_size, hz, vt, unit_mask, units, sse, s));
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], rv.bd, unit_size, hz, vt, unit_mask, units, sse, s));
     return AV1PF_OK;
 }
 
@@ -1826,13 +1828,14 @@ int av1pf_nn_infer(av1pf_nn *nn, const av1pf_frame *src, const av1pf_frame *reco
     int rc = validate_frame(c, src);
     if (rc) return rc;
     if ((rc = validate_frame(c, recon))) return rc;
-    if (src->width != recon->width || src->height != recon->height || src->bit_depth != 8 || recon->bit_depth != 8)
-        return fail(c, AV1PF_ERR_INVALID_ARG, "fused inference needs 8-bit frames of one geometry");
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != recon->bit_depth ||
+        (recon->bit_depth != 8 && recon->bit_depth != 10))
+        return fail(c, AV1PF_ERR_INVALID_ARG, "fused inference needs two 8-bit or two 10-bit frames of one geometry");
     HIPCHK(c, hipSetDevice(c->device));
     const int nhfb = (recon->width + 63) / 64, nvfb = (recon->height + 63) / 64;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_NN_FORWARD, s, 0);
-    HIPCHK(c, nn_infer(infer_args(nn), view_of(src).p[0], view_of(recon).p[0], nhfb, nhfb * nvfb, logits, nullptr, s));
+    HIPCHK(c, nn_infer(infer_args(nn), view_of(src).p[0], view_of(recon).p[0], recon->bit_depth, nhfb, nhfb * nvfb, logits, nullptr, s));
     return AV1PF_OK;
 }
 
@@ -1881,10 +1884,11 @@ int validate_search_pair(av1pf_ctx *c, const av1pf_frame *src, const av1pf_frame
     int rc = validate_frame(c, src);
     if (rc) return rc;
     if ((rc = validate_frame(c, recon))) return rc;
-    if (src->width != recon->width || src->height != recon->height || recon->bit_depth != 8 ||
-        src->bit_depth != 8 || recon->ss_x != 1 || recon->ss_y != 1 || src->ss_x != 1 || src->ss_y != 1 ||
-        recon->mono || (recon->width & 7) || (recon->height & 7))
-        return fail(c, AV1PF_ERR_INVALID_ARG, "search needs matching 8-bit 4:2:0 frames with sizes a multiple of 8");
+    if (src->width != recon->width || src->height != recon->height || src->bit_depth != recon->bit_depth ||
+        (recon->bit_depth != 8 && recon->bit_depth != 10) || recon->ss_x != 1 || recon->ss_y != 1 || src->ss_x != 1 ||
+        src->ss_y != 1 || recon->mono || (recon->width & 7) || (recon->height & 7))
+        return fail(c, AV1PF_ERR_INVALID_ARG,
+                    "search needs matching 8-bit or 10-bit 4:2:0 frames with sizes a multiple of 8");
     return AV1PF_OK;
 }
 }  // namespace
@@ -1941,7 +1945,7 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     prop.probs = probs;
     prop.cand = cand;
     prop.conf = conf ? conf : conf_tmp;
-    HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], nhfb, nfb, logits, &prop, s));
+    HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], rv.bd, nhfb, nfb, logits, &prop, s));
     // the LR search depends only on the logits: it runs on the side stream
     // beside the CDEF shortlist search and selection
     HIPCHK(c, hipEventRecord(c->learn_fork, s));
@@ -1949,7 +1953,7 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     const int W = recon->width, H = recon->height;
     const int hz = std::max((W + 32) / 64, 1), vt = std::max((H + 32) / 64, 1);
     HIPCHK(c, launch_lr_mask(logits, nhfb, nvfb, hz, vt, lr_none_threshold, mask, c->side));
-    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], 64, hz, vt, mask, lr_units, nullptr, c->side));
+    HIPCHK(c, launch_lr_search(sv.p[0], rv.p[0], rv.bd, 64, hz, vt, mask, lr_units, nullptr, c->side));
     HIPCHK(c, hipEventRecord(c->learn_join, c->side));
     HIPCHK(c, launch_search_shortlist(sv, rv, noskip, damping, lambda, cand, msub, decision, fb_index, c->small_state,
                                       s));

@@ -1,6 +1,9 @@
 // Encoder loop-restoration search on gfx950: the curated search of goav1's
 // keyframe encoder, luma, 8-bit (encoder/restoration.go:149-262,
-// encoder/keyframe_lossy.go:2310-2386):
+// encoder/keyframe_lossy.go:2310-2386), and the same search on 10-bit frames
+// (an extension with no reference counterpart: estimates clipped to the
+// 10-bit range, the decoder filters at 10 bits, the acceptance margin w*h
+// scaled by 2^(2 * (bd - 8))):
 //   - units of the luma restoration grid (8-row offset, the last unit absorbs
 //     the remainder, tile/restoration_geometry.go:54-93);
 //   - baseline SSE; six Wiener tap triples scored with an H-only 7-tap sum
@@ -37,11 +40,12 @@ __device__ __forceinline__ uint64_t wg_sum(uint64_t v, uint64_t *red)
     return red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
+template <typename T>
+__global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R, int bd, int us, int hz, int vt,
                                                         const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse_out)
 {
     __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
-    __shared__ uint8_t srcT[64 * 64];
+    __shared__ uint16_t srcT[64 * 64];
     __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
     __shared__ uint64_t red[4];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
@@ -56,6 +60,8 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
     if (y0 > 0) y0 -= 8;
     if (y1 < H) y1 -= 8;
     const int w = x1 - x0, h = y1 - y0;
+    const int maxv = (1 << bd) - 1;
+    const uint64_t margin = (uint64_t)(w * h) << (2 * (bd - 8));   // a candidate must beat the best by this
     uint64_t e[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) e[i] = ~0ull;
@@ -74,8 +80,8 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
     uint64_t b = 0, a[6] = { 0, 0, 0, 0, 0, 0 };
     for (int i = tid; i < w * h; i += 256) {
         const int y = y0 + i / w, x = x0 + i % w;
-        const uint8_t *rr = row_ptr<uint8_t>(R, y);
-        const int sv = row_ptr<uint8_t>(S, y)[x];
+        const T *rr = row_ptr<T>(R, y);
+        const int sv = row_ptr<T>(S, y)[x];
         int px[7];
 #pragma unroll
         for (int k = 0; k < 7; k++) px[k] = rr[min(max(x + k - 3, 0), W - 1)];
@@ -85,7 +91,7 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
         for (int c = 0; c < 6; c++) {
             const int t0 = kWienerCand[c][0], t1 = kWienerCand[c][1], t2 = kWienerCand[c][2];
             const int hs = t0 * (px[0] + px[6]) + t1 * (px[1] + px[5]) + t2 * (px[2] + px[4]) - 2 * (t0 + t1 + t2) * px[3];
-            const int pred = min(max((hs + 64) >> 7, 0), 255);
+            const int pred = min(max((hs + 64) >> 7, 0), maxv);
             const int d = sv - pred;
             a[c] += (uint64_t)(d * d);
         }
@@ -101,7 +107,7 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
 #pragma unroll
         for (int c = 0; c < 6; c++) {
             e[1 + c] = wg_sum(a[c], red);
-            if (e[1 + c] + (uint64_t)(w * h) < best) { best = e[1 + c]; wbest = c; }
+            if (e[1 + c] + margin < best) { best = e[1 + c]; wbest = c; }
         }
         if (w <= 64 && h <= 64) {
             // ---- the unit with its 3-sample border, edge-clamped at the frame
@@ -109,11 +115,11 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
             for (int i = tid; i < tw * th; i += 256) {
                 const int r = i / tw, c = i - r * tw;
                 const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
-                tile[r * LR_TW + c + LR_OX - 3] = row_ptr<uint8_t>(R, y)[x];
+                tile[r * LR_TW + c + LR_OX - 3] = row_ptr<T>(R, y)[x];
             }
             for (int i = tid; i < w * h; i += 256) {
                 const int r = i / w, c = i - r * w;
-                srcT[r * 64 + c] = row_ptr<uint8_t>(S, y0 + r)[x0 + c];
+                srcT[r * 64 + c] = row_ptr<T>(S, y0 + r)[x0 + c];
             }
             __syncthreads();
             const uint16_t *O = tile + 3 * LR_TW + LR_OX;
@@ -127,7 +133,7 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
             };
             uint64_t best_sse = ~0ull;
             if (wbest >= 0) {
-                wiener_tile(O, w, h, 8, kWienerCand[wbest], kWienerCand[wbest], tmp, err4);
+                wiener_tile(O, w, h, bd, kWienerCand[wbest], kWienerCand[wbest], tmp, err4);
                 e[7] = best_sse = wg_sum(acc, red);
                 res.type = AV1PF_LR_WIENER;
                 for (int k = 0; k < 3; k++) res.wiener_v[k] = res.wiener_h[k] = kWienerCand[wbest][k];
@@ -138,9 +144,9 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
             for (int p = 0; p < 4; p++) {
                 acc = 0;
                 __syncthreads();
-                sgr_tile(O, w, h, 8, kSgrPreset[p], kSgrXqd[p], Ab, Bb, err);
+                sgr_tile(O, w, h, bd, kSgrPreset[p], kSgrXqd[p], Ab, Bb, err);
                 e[8 + p] = wg_sum(acc, red);
-                if (e[8 + p] + (uint64_t)(w * h) < sbest) { sbest = e[8 + p]; pbest = p; }
+                if (e[8 + p] + margin < sbest) { sbest = e[8 + p]; pbest = p; }
             }
             if (pbest >= 0 && sbest < best_sse) {
                 for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
@@ -416,10 +422,15 @@ hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &re
     return hipGetLastError();
 }
 
-hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
-                            av1pf_lr_unit *out, uint64_t *sse, hipStream_t s)
+hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int bd, int us, int hz, int vt,
+                            const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse, hipStream_t s)
 {
-    hipLaunchKernelGGL(lr_search_kernel, dim3(hz * vt), dim3(256), 0, s, src, rec, us, hz, vt, mask, out, sse);
+    if (bd == 8)
+        hipLaunchKernelGGL(lr_search_kernel<uint8_t>, dim3(hz * vt), dim3(256), 0, s, src, rec, bd, us, hz, vt, mask,
+                           out, sse);
+    else
+        hipLaunchKernelGGL(lr_search_kernel<uint16_t>, dim3(hz * vt), dim3(256), 0, s, src, rec, bd, us, hz, vt, mask,
+                           out, sse);
     return hipGetLastError();
 }
 

@@ -348,20 +348,20 @@ __device__ __forceinline__ v4s ld_frag(const uint16_t *p)
     return (v4s){ (short)(u.x & 0xffff), (short)(u.x >> 16), (short)(u.y & 0xffff), (short)(u.y >> 16) };
 }
 
-template <bool PROPOSE>
-__global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R, int nhfb, NNInfer p, float *logits,
-                                                       NNPropose pr)
+template <bool PROPOSE, typename T>
+__global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R, int bd, int nhfb, NNInfer p,
+                                                       float *logits, NNPropose pr)
 {
     __shared__ __align__(16) uint16_t bufA[64 * 64 * 2];    // tile [64][64][2], then conv2 out [256][32]
     __shared__ __align__(16) uint16_t bufB[32 * 32 * 16];   // conv1 out [1024][16], then conv3 out [64][64]
     __shared__ float pool[64], fc1[128];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, g = lane >> 4, l16 = lane & 15;
     const int fb = blockIdx.x, fx = (fb % nhfb) * 64, fy = (fb / nhfb) * 64;
-    const float inv = 1.f / 255.f;
+    const float inv = 1.f / (float)((1 << bd) - 1);
     uint32_t *tile = reinterpret_cast<uint32_t *>(bufA);
     for (int i = t; i < 4096; i += 256) {
         const int y = min(fy + (i >> 6), R.h - 1), x = min(fx + (i & 63), R.w - 1);
-        const float sv = row_ptr<uint8_t>(S, y)[x], rv = row_ptr<uint8_t>(R, y)[x];
+        const float sv = row_ptr<T>(S, y)[x], rv = row_ptr<T>(R, y)[x];
         tile[i] = (uint32_t)f2bf(rv * inv) | ((uint32_t)f2bf((sv - rv) * 16.f * inv) << 16);
     }
     __syncthreads();
@@ -535,14 +535,20 @@ __global__ __launch_bounds__(1024) void propose_kernel(const float *probs, int n
     if (t == 0) *conf = sums[129] > 0.f ? sums[128] / sums[129] : 1.f;
 }
 
-hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
+hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, float *logits,
                     const NNPropose *prop, hipStream_t s)
 {
-    if (prop) {
-        hipLaunchKernelGGL(nn_infer_kernel<true>, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits, *prop);
-        hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->probs, nfb, prop->cand, prop->conf);
-    } else
-        hipLaunchKernelGGL(nn_infer_kernel<false>, dim3(nfb), dim3(256), 0, s, S, R, nhfb, p, logits, NNPropose{});
+    const NNPropose none{};
+    const NNPropose &pp = prop ? *prop : none;
+    if (prop && bd == 8)
+        hipLaunchKernelGGL((nn_infer_kernel<true, uint8_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
+    else if (prop)
+        hipLaunchKernelGGL((nn_infer_kernel<true, uint16_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
+    else if (bd == 8)
+        hipLaunchKernelGGL((nn_infer_kernel<false, uint8_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
+    else
+        hipLaunchKernelGGL((nn_infer_kernel<false, uint16_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
+    if (prop) hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->probs, nfb, prop->cand, prop->conf);
     return hipGetLastError();
 }
 

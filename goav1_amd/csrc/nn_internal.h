@@ -93,7 +93,7 @@ struct NNPropose {
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 hipError_t nn_pack_frag(const NNWeights &w, uint16_t *frag, hipStream_t s);
-hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int nhfb, int nfb, float *logits,
+hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, float *logits,
                     const NNPropose *prop, hipStream_t s);
 hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s);
 hipError_t nn_forward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, float *logits, hipStream_t s);

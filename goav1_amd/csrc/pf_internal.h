@@ -252,8 +252,8 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
                               uint64_t lambda, void *state, unsigned int epoch, av1pf_cdef_decision *dec,
                               uint8_t *fb_index, int *err, hipStream_t s);
 size_t finish_state_bytes();
-hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt, const uint8_t *mask,
-                            av1pf_lr_unit *out, uint64_t *sse, hipStream_t s);
+hipError_t launch_lr_search(const PlaneView &src, const PlaneView &rec, int bd, int us, int hz, int vt,
+                            const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse, hipStream_t s);
 hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt,
                                        av1pf_lr_unit *out, uint64_t *sse, int8_t *fit, hipStream_t s);
 hipError_t launch_lr_mask(const float *logits, int nhfb, int nvfb, int hz, int vt, float thr, uint8_t *mask,

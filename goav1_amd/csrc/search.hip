@@ -20,6 +20,7 @@
 // of cdef/filter_search_batch.go); the accumulators are reduced across the
 // workgroup with wave shuffles and a 4-way LDS sum.  The work is integer-VALU
 // bound (64 trials x ~70 ops per pixel).
+#include <type_traits>
 #include "pf_internal.h"
 #include "nn_internal.h"
 
@@ -52,6 +53,8 @@ __device__ __forceinline__ int s_constrain(int diff, int t, int sh)
 }
 __device__ __forceinline__ int s_ilog2(unsigned v) { return 31 - __clz(v); }
 
+// CS = bit depth - 8: the direction search reads samples shifted down by CS
+template <int CS>
 __device__ void s_find_dir(const uint16_t *img, int *dir, int32_t *var)
 {
     int32_t pa[8][15];
@@ -63,7 +66,7 @@ __device__ void s_find_dir(const uint16_t *img, int *dir, int32_t *var)
     for (int i = 0; i < 8; i++)
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int32_t x = (int32_t)img[i * SP + j] - 128;
+            const int32_t x = (int32_t)(img[i * SP + j] >> CS) - 128;
             pa[0][i + j] += x; pa[1][i + (j >> 1)] += x; pa[2][i] += x; pa[3][3 + i - (j >> 1)] += x;
             pa[4][7 + i - j] += x; pa[5][3 - (i >> 1) + j] += x; pa[6][j] += x; pa[7][(i >> 1) + j] += x;
         }
@@ -98,6 +101,7 @@ __device__ void s_find_dir(const uint16_t *img, int *dir, int32_t *var)
 }
 
 // Load a w x h region at (x0,y0) plus the 2/8 halo; VeryLarge outside [0,pw)x[0,ph).
+template <typename T>
 __device__ __forceinline__ void s_load(uint16_t *tile, const PlaneView &pv, int x0, int y0, int w, int h, int tid)
 {
     const int rows = h + 4, cols = w + 16;
@@ -105,13 +109,17 @@ __device__ __forceinline__ void s_load(uint16_t *tile, const PlaneView &pv, int 
         const int r = i / cols, c = i - r * cols;
         const int y = y0 - 2 + r, x = x0 - 8 + c;
         uint16_t v = SVL;
-        if (y >= 0 && y < pv.h && x >= 0 && x < pv.w) v = row_ptr<uint8_t>(pv, y)[x];
+        if (y >= 0 && y < pv.h && x >= 0 && x < pv.w) v = row_ptr<T>(pv, y)[x];
         tile[r * SP + c] = v;
     }
 }
 
 // Evaluate the 64 trials for one pixel and add its squared errors to acc.
-template <bool LUMA>
+// CS = bit depth - 8: the decoder's high-bit-depth CDEF scales both strengths
+// by 2^CS (the primary after the luma variance adjustment reads the scaled
+// strength), picks the primary tap set by the unscaled strength's parity, and
+// takes a damping (here passed in) CS larger.
+template <bool LUMA, int CS>
 __device__ __forceinline__ void s_pixel(uint32_t acc[64], const uint16_t *c, int srcv, int dir, int adj_i,
                                         bool has_var, int damping)
 {
@@ -137,16 +145,16 @@ __device__ __forceinline__ void s_pixel(uint32_t acc[64], const uint16_t *c, int
     for (int gi = 0; gi < 64; gi++) {
         const int packed = gi < 16 ? 4 * gi : 4 * ((gi - 16) / 3) + (gi - 16) % 3 + 1;   // compile-time
         const int pri = packed >> 2;
-        const int sec = (packed & 3) == 3 ? 4 : (packed & 3);
-        int st = pri;
-        if (LUMA && pri) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
+        const int sec = ((packed & 3) == 3 ? 4 : (packed & 3)) << CS;
+        int st = pri << CS;
+        if (LUMA && pri) st = has_var ? (((pri << CS) * (4 + adj_i) + 8) >> 4) : 0;
         int out = x;
         if (st || sec) {
             const int ssh = sec ? max(damping - s_ilog2((unsigned)sec), 0) : 0;
             int sum = 0;
             if (pri) {
                 const int psh = st ? max(damping - s_ilog2((unsigned)st), 0) : 0;
-                const int t0 = (st & 1) ? 3 : 4, t1 = (st & 1) ? 3 : 2;
+                const int t0 = ((st >> CS) & 1) ? 3 : 4, t1 = ((st >> CS) & 1) ? 3 : 2;
                 if (st)
                     sum = t0 * (s_constrain(a0 - x, st, psh) + s_constrain(a1 - x, st, psh)) +
                           t1 * (s_constrain(a2 - x, st, psh) + s_constrain(a3 - x, st, psh));
@@ -182,9 +190,13 @@ __device__ __forceinline__ void s_reduce(uint32_t acc[64], uint32_t (*part)[64],
     }
 }
 
+// CS = bit depth - 8 (0: 8-bit samples, 2: 10-bit); damping is the frame's
+// (the decoder damping of the trials is damping + CS, one less for chroma)
+template <int CS>
 __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
                                                           int damping, uint64_t *mse, int nhfb, int nvfb)
 {
+    using T = typename std::conditional<CS == 0, uint8_t, uint16_t>::type;
     __shared__ uint16_t tile[68 * SP];
     __shared__ uint8_t bdir[64];
     __shared__ int8_t badj[64];      // adjust index i, -1 = zero variance
@@ -218,13 +230,13 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     // ---- luma
     {
         const int x0 = fbc * 64, y0 = fbr * 64, w = nhb * 4, h = nvb * 4;
-        s_load(tile, rec.p[0], x0, y0, w, h, tid);
+        s_load<T>(tile, rec.p[0], x0, y0, w, h, tid);
         __syncthreads();
         if (tid < nblk) {
             const int b = blist[tid];
             int d;
             int32_t v;
-            s_find_dir(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
+            s_find_dir<CS>(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
             bdir[b] = (uint8_t)d;
             int i = -1;
             if (v != 0) {
@@ -242,8 +254,8 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
             const int b = blist[p >> 6], q = p & 63;
             const int by = b >> 3, bx = b & 7, yy = by * 8 + (q >> 3), xx = bx * 8 + (q & 7);
             const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
-            const int sv = row_ptr<uint8_t>(src.p[0], y0 + yy)[x0 + xx];
-            s_pixel<true>(acc, c, sv, bdir[b], badj[b], badj[b] >= 0, damping);
+            const int sv = row_ptr<T>(src.p[0], y0 + yy)[x0 + xx];
+            s_pixel<true, CS>(acc, c, sv, bdir[b], badj[b], badj[b] >= 0, damping + CS);
         }
         s_reduce(acc, part, tid);
         __syncthreads();
@@ -255,15 +267,15 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     for (int pl = 1; pl <= 2; pl++) {
         const int x0 = fbc * 32, y0 = fbr * 32, w = nhb * 2, h = nvb * 2;
         __syncthreads();
-        s_load(tile, rec.p[pl], x0, y0, w, h, tid);
+        s_load<T>(tile, rec.p[pl], x0, y0, w, h, tid);
         __syncthreads();
         const int npx = nblk * 16;
         for (int p = tid; p < npx; p += 256) {
             const int b = blist[p >> 4], q = p & 15;
             const int by = b >> 3, bx = b & 7, yy = by * 4 + (q >> 2), xx = bx * 4 + (q & 3);
             const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
-            const int sv = row_ptr<uint8_t>(src.p[pl], y0 + yy)[x0 + xx];
-            s_pixel<false>(acc, c, sv, bdir[b], 0, true, damping - 1);
+            const int sv = row_ptr<T>(src.p[pl], y0 + yy)[x0 + xx];
+            s_pixel<false, CS>(acc, c, sv, bdir[b], 0, true, damping - 1 + CS);
         }
     }
     __syncthreads();
@@ -751,7 +763,8 @@ __device__ __forceinline__ int s_taps_sum(const int (&ad)[N], const int (&sw)[N]
     return s;
 }
 
-template <bool LUMA>
+// CS and damping as in s_pixel (damping includes CS)
+template <bool LUMA, int CS>
 __device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t *c, int srcv, int dir, int adj_i,
                                             bool has_var, int damping, const int *pk, unsigned need)
 {
@@ -785,7 +798,7 @@ __device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t 
     int S[3] = { 0, 0, 0 }, Z[3] = { 0, 0, 0 };
 #pragma unroll
     for (int i = 0; i < 3; i++)
-        if ((need >> i) & 1) S[i] = s_taps_sum(as, ws, 1 << i, max(damping - i, 0));
+        if ((need >> i) & 1) S[i] = s_taps_sum(as, ws, (1 << i) << CS, max(damping - i - CS, 0));
     if (need & 0x38) {
         const int z[4] = { kSOff[4][0], kSOff[0][0], kSOff[4][1], kSOff[0][1] };
         int az[8], wz[8];
@@ -797,7 +810,7 @@ __device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t 
         }
 #pragma unroll
         for (int i = 0; i < 3; i++)
-            if ((need >> (3 + i)) & 1) Z[i] = s_taps_sum(az, wz, 1 << i, max(damping - i, 0));
+            if ((need >> (3 + i)) & 1) Z[i] = s_taps_sum(az, wz, (1 << i) << CS, max(damping - i - CS, 0));
     }
 #pragma unroll
     for (int q = 0; q < NCAND; q++) {
@@ -805,12 +818,12 @@ __device__ __forceinline__ void s_pixel_sub(uint32_t acc[NCAND], const uint16_t 
         const int pri = packed >> 2, sc = packed & 3;   // secondary code 0..3 = strength 0, 1, 2, 4
         int out;
         if (pri) {
-            int st = pri;
-            if (LUMA) st = has_var ? ((pri * (4 + adj_i) + 8) >> 4) : 0;
+            int st = pri << CS;
+            if (LUMA) st = has_var ? (((pri << CS) * (4 + adj_i) + 8) >> 4) : 0;
             int sum = sc == 0 ? 0 : sc == 1 ? S[0] : sc == 2 ? S[1] : S[2];
             if (st) {
                 const int psh = max(damping - s_ilog2((unsigned)st), 0);
-                const int t0 = (st & 1) ? 3 : 4, t1 = (st & 1) ? 3 : 2;
+                const int t0 = ((st >> CS) & 1) ? 3 : 4, t1 = ((st >> CS) & 1) ? 3 : 2;
                 const int m01 = __mul24(wp[0], s_clamp0(st - (ap[0] >> psh), ap[0])) +
                                 __mul24(wp[1], s_clamp0(st - (ap[1] >> psh), ap[1]));
                 const int m23 = __mul24(wp[2], s_clamp0(st - (ap[2] >> psh), ap[2])) +
@@ -839,10 +852,12 @@ __device__ __forceinline__ unsigned s_need(const int *pk)
     return need;
 }
 
+template <int CS>
 __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
                                                                  int damping, const int *cand, uint64_t *mse, int nhfb,
                                                                  int nvfb)
 {
+    using T = typename std::conditional<CS == 0, uint8_t, uint16_t>::type;
     __shared__ uint16_t tile[68 * SP];
     __shared__ uint8_t bdir[64];
     __shared__ int8_t badj[64];
@@ -888,14 +903,14 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
             for (int q = 0; q < NCAND; q++) acc[q] = 0;
         }
         __syncthreads();
-        s_load(tile, rec.p[pl], x0, y0, w, h, tid);
+        s_load<T>(tile, rec.p[pl], x0, y0, w, h, tid);
         __syncthreads();
         if (pl == 0) {
             if (tid < nblk) {
                 const int b = blist[tid];
                 int d;
                 int32_t v;
-                s_find_dir(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
+                s_find_dir<CS>(tile + (2 + (b >> 3) * 8) * SP + 8 + (b & 7) * 8, &d, &v);
                 bdir[b] = (uint8_t)d;
                 int i = -1;
                 if (v != 0) {
@@ -912,15 +927,15 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
                 const int b = __builtin_amdgcn_readfirstlane(blist[p >> 6]), q = p & 63;
                 const int d = __builtin_amdgcn_readfirstlane(bdir[b]), adj = __builtin_amdgcn_readfirstlane(badj[b]);
                 const int yy = (b >> 3) * 8 + (q >> 3), xx = (b & 7) * 8 + (q & 7);
-                s_pixel_sub<true>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[0], y0 + yy)[x0 + xx],
-                                  d, adj, adj >= 0, damping, pk[0], need[0]);
+                s_pixel_sub<true, CS>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<T>(src.p[0], y0 + yy)[x0 + xx],
+                                      d, adj, adj >= 0, damping + CS, pk[0], need[0]);
             }
         } else {
             for (int p = tid; p < nblk * 16; p += 256) {
                 const int b = blist[p >> 4], q = p & 15;
                 const int yy = (b >> 3) * 4 + (q >> 2), xx = (b & 7) * 4 + (q & 3);
-                s_pixel_sub<false>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<uint8_t>(src.p[pl], y0 + yy)[x0 + xx],
-                                   bdir[b], 0, true, damping - 1, pk[1], need[1]);
+                s_pixel_sub<false, CS>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<T>(src.p[pl], y0 + yy)[x0 + xx],
+                                       bdir[b], 0, true, damping - 1 + CS, pk[1], need[1]);
             }
         }
         if (pl == 0 || pl == 2) {
@@ -1211,16 +1226,208 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     }
 }
 
+// The same greedy selection over the shortlist pairs with 64-bit costs, for
+// tables whose SSEs need more than 32 bits (10-bit frames: a 64x64 luma
+// block's SSE reaches 4096 * 1023^2).  Same steps, the same first-maximum
+// and first-minimum rules, the same RD cost and ticket as fin_small_kernel;
+// thread t keeps the active blocks t, t + SMALL_T, ... (NPER of them) with
+// their shortlist SSEs in registers and sums every pair's gain over them.
+template <int NPER>
+__global__ __launch_bounds__(SMALL_T) void fin_small_wide_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
+                                                                 int nvfb, int mi_cols, int mi_rows,
+                                                                 unsigned long long lambda, const int *cand,
+                                                                 SmallState *ss, av1pf_cdef_decision *dec,
+                                                                 uint8_t *fb_index)
+{
+    constexpr int NW = SMALL_T / 64;
+    constexpr int NP = NCAND * NCAND;
+    __shared__ uint16_t act[SMALL_MAX_FB];
+    __shared__ uint8_t flag[SMALL_MAX_FB];
+    __shared__ unsigned long long wpart[NW][NP + 1];   // per-wave pair gains (or column sums) and base sum
+    __shared__ int pick_s, nact_s, bits_s;
+    __shared__ unsigned sel_s[2];
+    __shared__ bool last;
+    __shared__ unsigned long long dist_s;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nfb = nhfb * nvfb;
+    for (int fb = t; fb < nfb; fb += SMALL_T) flag[fb] = fb_active(noskip, fb, nhfb, mi_cols, mi_rows) ? 1 : 0;
+    __syncthreads();
+    if (t < 64) {
+        int n = 0;
+        for (int base = 0; base < nfb; base += 64) {
+            const bool a = base + lane < nfb && flag[base + lane];
+            const uint64_t vote = __ballot(a);
+            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = (uint16_t)(base + lane);
+            n += __popcll(vote);
+        }
+        if (lane == 0) nact_s = n;
+    }
+    __syncthreads();
+    const int nact = nact_s;
+    // blocks past nact hold zeros: best cost 0, no gain, nothing in a column sum
+    unsigned long long Y[NPER][NCAND], U[NPER][NCAND];
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        const size_t fb = i < nact ? act[i] : 0;
+#pragma unroll
+        for (int q = 0; q < NCAND; q++) {
+            Y[r][q] = i < nact ? mse[fb * NCAND + q] : 0;
+            U[r][q] = i < nact ? mse[((size_t)nfb + fb) * NCAND + q] : 0;
+        }
+    }
+    const int bits = blockIdx.x;
+    const int nb = 1 << bits;
+    unsigned sy = 0, su = 0;
+    for (int step = 0; step < 5 * nb; step++) {
+        const int slot = step < nb ? step : nb - 1;
+        const bool final_step = step == 5 * nb - 1;
+        if (slot == 0) {
+#pragma unroll
+            for (int q = 0; q < 2 * NCAND; q++) {
+                unsigned long long v = 0;
+#pragma unroll
+                for (int r = 0; r < NPER; r++) v += q < NCAND ? Y[r][q] : U[r][q - NCAND];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) wpart[wv][q] = v;
+            }
+            __syncthreads();
+            if (t == 0) {
+                unsigned long long cs[2 * NCAND];
+                for (int q = 0; q < 2 * NCAND; q++) {
+                    cs[q] = 0;
+                    for (int w = 0; w < NW; w++) cs[q] += wpart[w][q];
+                }
+                int by = 0, bc = 0;
+                for (int q = 1; q < NCAND; q++) {
+                    if (cs[q] < cs[by]) by = q;
+                    if (cs[NCAND + q] < cs[NCAND + bc]) bc = q;
+                }
+                pick_s = by * NCAND + bc;
+                dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
+            }
+        } else {
+            unsigned long long b[NPER], bsum = 0;
+#pragma unroll
+            for (int r = 0; r < NPER; r++) {
+                b[r] = Y[r][sy & 15] + U[r][su & 15];
+                for (int g = 1; g < slot; g++)
+                    b[r] = min(b[r], Y[r][(sy >> (4 * g)) & 15] + U[r][(su >> (4 * g)) & 15]);
+                bsum += b[r];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                unsigned long long g = 0;
+#pragma unroll
+                for (int r = 0; r < NPER; r++) {
+                    const unsigned long long c = Y[r][p / NCAND] + U[r][p % NCAND];
+                    g += b[r] > c ? b[r] - c : 0;
+                }
+                for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+                if (lane == 0) wpart[wv][p] = g;
+            }
+            if (final_step) {
+                for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
+                if (lane == 0) wpart[wv][NP] = bsum;
+            }
+            __syncthreads();
+            if (t == 0) {   // first maximum over the pairs
+                unsigned long long bestg = 0;
+                int bp = 0;
+                for (int p = 0; p < NP; p++) {
+                    unsigned long long g = 0;
+                    for (int w = 0; w < NW; w++) g += wpart[w][p];
+                    if (p == 0 || g > bestg) { bestg = g; bp = p; }
+                }
+                pick_s = bp;
+                if (final_step) {
+                    unsigned long long base = 0;
+                    for (int w = 0; w < NW; w++) base += wpart[w][NP];
+                    dist_s = base - bestg;
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
+        sy = (sy & ~(15u << (4 * slot))) | ((pk / NCAND) << (4 * slot));
+        su = (su & ~(15u << (4 * slot))) | ((pk % NCAND) << (4 * slot));
+        if (final_step) {
+            if (t == 0) {
+                const long long rate = (long long)(nact * bits + nb * 12) << 9;
+                SmallLevel &L = ss->lv[bits];
+                L.cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((dist_s * 16) << 7);
+                for (int i = 0; i < 8; i++) {
+                    L.ys[i] = i < nb ? (int)((sy >> (4 * i)) & 15) : 0;
+                    L.us[i] = i < nb ? (int)((su >> (4 * i)) & 15) : 0;
+                }
+            }
+        } else if (step + 1 >= nb) {
+            sy >>= 4;
+            su >>= 4;
+        }
+        __syncthreads();   // pick_s and the partials are rewritten by the next step
+    }
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&ss->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    if (t == 0) {
+        __threadfence();
+        volatile SmallLevel *V = ss->lv;
+        int best_bits = 0;
+        unsigned long long best_cost = V[0].cost;
+        for (int b = 1; b < 4; b++)
+            if (V[b].cost < best_cost) { best_cost = V[b].cost; best_bits = b; }
+        const int n = 1 << best_bits;
+        unsigned a = 0, c = 0;
+        for (int i = 0; i < n; i++) { a |= (unsigned)V[best_bits].ys[i] << (4 * i); c |= (unsigned)V[best_bits].us[i] << (4 * i); }
+        ss->ticket = 0;
+        dec->cdef_bits = best_bits;
+        for (int i = 0; i < 8; i++) {
+            dec->y_strength[i] = i < n ? kLevel1[cand[V[best_bits].ys[i]]] : 0;
+            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + V[best_bits].us[i]]] : 0;
+        }
+        bits_s = best_bits;
+        sel_s[0] = a;
+        sel_s[1] = c;
+    }
+    for (int fb = t; fb < nfb; fb += SMALL_T) fb_index[fb] = 0;
+    __syncthreads();
+    const int n = 1 << bits_s;
+    const unsigned fy = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[0]), fu = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[1]);
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        if (i >= nact) continue;
+        unsigned long long bm = Y[r][fy & 15] + U[r][fu & 15];
+        int bg = 0;
+        for (int g = 1; g < n; g++) {
+            const unsigned long long c = Y[r][(fy >> (4 * g)) & 15] + U[r][(fu >> (4 * g)) & 15];
+            if (c < bm) { bm = c; bg = g; }
+        }
+        fb_index[act[i]] = (uint8_t)bg;
+    }
+}
+
 hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
                                    uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
                                    uint8_t *fb_index, void *small_state, hipStream_t s)
 {
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
-    hipLaunchKernelGGL(cdef_search_subset_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, cand,
-                       mse_sub, nhfb, nvfb);
+    if (rec.bd == 8)
+        hipLaunchKernelGGL(cdef_search_subset_kernel<0>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping,
+                           cand, mse_sub, nhfb, nvfb);
+    else
+        hipLaunchKernelGGL(cdef_search_subset_kernel<2>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping,
+                           cand, mse_sub, nhfb, nvfb);
     const int nfb = nhfb * nvfb;
-    auto fin = nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2> : fin_small_kernel<4>;
+    auto fin = rec.bd == 8 ? (nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2>
+                                                                                        : fin_small_kernel<4>)
+                           : (nfb <= SMALL_T ? fin_small_wide_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_wide_kernel<2>
+                                                                                             : fin_small_wide_kernel<4>);
     hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
                        (unsigned long long)lambda, cand, static_cast<SmallState *>(small_state), dec, fb_index);
     return hipGetLastError();
@@ -1229,7 +1436,12 @@ hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, c
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s)
 {
-    hipLaunchKernelGGL(cdef_search_kernel, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse, nhfb, nvfb);
+    if (rec.bd == 8)
+        hipLaunchKernelGGL(cdef_search_kernel<0>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse,
+                           nhfb, nvfb);
+    else
+        hipLaunchKernelGGL(cdef_search_kernel<2>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse,
+                           nhfb, nvfb);
     return hipGetLastError();
 }
 

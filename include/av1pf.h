@@ -360,8 +360,11 @@ int  av1pf_superres(av1pf_ctx *ctx, const av1pf_frame *coded, const av1pf_frame 
  * For every 64x64 filter block that has a non-skip 8x8 block, evaluate the 64
  * (primary, secondary) strength pairs of search level 1 on luma and on chroma
  * (U+V summed) and write the SSE (uint64, times the subsampling factor 1) to
- * mse[2][nfb][64] (plane 0 = luma, 1 = chroma).  src and recon are frames of
- * identical geometry; damping = 3 + (qindex >> 6) (cdef_search.go:588).    */
+ * mse[2][nfb][64] (plane 0 = luma, 1 = chroma).  src and recon are 4:2:0
+ * frames of identical geometry, sizes a multiple of 8, 8-bit as goav1's
+ * encoder searches, or 10-bit (an extension with no reference counterpart:
+ * every trial is the decoder's 10-bit CDEF, strengths and damping scaled as
+ * the decoder scales them); damping = 3 + (qindex >> 6) (cdef_search.go:588). */
 int  av1pf_cdef_search(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon,
                        const uint64_t *noskip, int damping, uint64_t *mse, void *stream);
 
@@ -406,6 +409,9 @@ uint64_t av1pf_cdef_lambda(int dc_quant, int key_frame, int gf_update, int tl_fa
 /* ---- encoder loop-restoration search ------------------------------------
  * The keyframe encoder's curated search on the luma plane of 8-bit frames
  * (SearchWienerUnitU8 / SearchSGRProjUnitU8, encoder/restoration.go:149-262,
+ * and, as an extension with no reference counterpart, of 10-bit frames: the
+ * estimates clip to the 10-bit range, the exact scores use the decoder's
+ * 10-bit filters and the acceptance margin w*h is scaled by 16)
  * and their driver, encoder/keyframe_lossy.go:2310-2386): per restoration
  * unit of size unit_size (32..256; units offset 8 rows as the decoder's),
  * six Wiener tap triples are screened with the reference's H-only estimate,
@@ -457,7 +463,7 @@ int  av1pf_nn_set_params(av1pf_nn *nn, const float *params);     /* host FP32; r
 int  av1pf_nn_get_params(av1pf_nn *nn, float *params);           /* host FP32 */
 int  av1pf_nn_tiles(av1pf_ctx *ctx, const av1pf_frame *src, const av1pf_frame *recon, uint16_t *tiles, void *stream);
 int  av1pf_nn_forward(av1pf_nn *nn, const uint16_t *tiles, int n, float *logits, void *stream);
-/* Inference straight from an 8-bit frame pair: logits [nfb][131] for every
+/* Inference straight from an 8-bit or 10-bit frame pair: logits [nfb][131] for every
  * 64x64 filter block, one fused launch (tiles built on chip, the same
  * rounding points as av1pf_nn_tiles + av1pf_nn_forward, FP32 sums in another
  * order).  What av1pf_search_learned runs.                                  */
@@ -475,7 +481,8 @@ int  av1pf_nn_adam(av1pf_nn *nn, const float *grads, float lr, float weight_deca
  * within each half): their exact SSE for every filter block
  * (mse_sub[2][nfb][AV1PF_SHORTLIST], same rules as av1pf_cdef_search), then
  * the same greedy strength selection over the AV1PF_SHORTLIST^2 shortlist
- * pairs, in one launch.  8-bit 4:2:0 frames up to 2048 filter blocks (4K).
+ * pairs, in one launch.  8-bit or 10-bit 4:2:0 frames up to 2048 filter
+ * blocks (4K).
  * Device pointers, asynchronous.                                            */
 #define AV1PF_SHORTLIST 4
 int  av1pf_search_shortlist_size(void);   /* AV1PF_SHORTLIST of the built library */

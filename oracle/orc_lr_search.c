@@ -1,6 +1,10 @@
 /*
- * CPU oracle: the encoder's curated loop-restoration search (luma, 8-bit).
+ * CPU oracle: the encoder's curated loop-restoration search (luma).
  * Test infrastructure only (see orc.h).
+ * The reference searches 8-bit frames; 10-bit frames are an extension with no
+ * reference counterpart: the estimates clip to the bit depth's range, the
+ * exact scores use the decoder's filters at that depth, and the acceptance
+ * margin w*h scales with the squared sample range (w*h << 2(bd - 8)).
  *
  * Restates:
  *   encoder/restoration.go:149-210      SearchWienerUnitU8: baseline SSE; 6 tap
@@ -55,11 +59,11 @@ static int wiener_pick(const uint64_t e[12], uint64_t wh)
  *   - the SGR presets run in order from the baseline with the same
  *     "sse + w*h < running best" rule; the found preset replaces the Wiener
  *     choice only when its SSE is strictly below the best so far. */
-void orc_lr_decide(const uint64_t e[12], int w, int h, av1pf_lr_unit *out)
+static void lr_decide(const uint64_t e[12], int w, int h, int cs, av1pf_lr_unit *out)
 {
     memset(out, 0, sizeof(*out));
     out->type = AV1PF_LR_NONE;
-    const uint64_t base = e[0], wh = (uint64_t)w * (uint64_t)h;
+    const uint64_t base = e[0], wh = ((uint64_t)w * (uint64_t)h) << (2 * cs);
     if (base == 0 || w <= 0 || h <= 0 || w > 64 || h > 64) return;
     uint64_t best_sse = UINT64_MAX;
     const int wbest = wiener_pick(e, wh);
@@ -82,6 +86,8 @@ void orc_lr_decide(const uint64_t e[12], int w, int h, av1pf_lr_unit *out)
     }
 }
 
+void orc_lr_decide(const uint64_t e[12], int w, int h, av1pf_lr_unit *out) { lr_decide(e, w, h, 0, out); }
+
 int orc_lr_search_units(int pw, int ph, int us, int *hz, int *vt)
 {
     *hz = (pw + (us >> 1)) / us;
@@ -93,9 +99,10 @@ int orc_lr_search_units(int pw, int ph, int us, int *hz, int *vt)
 
 int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_unit *out, uint64_t *sse)
 {
-    if (!src || !rec || rec->bd != 8 || src->w != rec->w || src->h != rec->h || us < 32 || us > 256)
+    if (!src || !rec || (rec->bd != 8 && rec->bd != 10) || src->bd != rec->bd || src->w != rec->w ||
+        src->h != rec->h || us < 32 || us > 256)
         return AV1PF_ERR_INVALID_ARG;
-    const int W = rec->w, H = rec->h, pad = 3;
+    const int W = rec->w, H = rec->h, pad = 3, bd = rec->bd, cs = bd - 8, maxv = (1 << bd) - 1;
     const int bs = W + 2 * pad;
     uint16_t *bord = malloc(sizeof(uint16_t) * (size_t)bs * (H + 2 * pad));
     uint16_t *tmp = malloc(sizeof(uint16_t) * 64 * 64);
@@ -147,17 +154,17 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
                             hs += f[k + 3] * (int)R[(size_t)y * rs + cx];
                         }
                         int pred = (hs + 64) >> 7;
-                        pred = pred < 0 ? 0 : (pred > 255 ? 255 : pred);
+                        pred = pred < 0 ? 0 : (pred > maxv ? maxv : pred);
                         const int d = (int)S[(size_t)y * ss + x] - pred;
                         s += (uint64_t)(d * d);
                     }
                 e[1 + c] = s;
             }
-            const int wbest = wiener_pick(e, (uint64_t)(w * h));
+            const int wbest = wiener_pick(e, (uint64_t)(w * h) << (2 * cs));
             const int valid = w > 0 && h > 0 && w <= 64 && h <= 64;
             const uint16_t *org = bord + (size_t)(y0 + pad) * bs + x0 + pad;
             if (wbest >= 0 && valid) {
-                orc_wiener_unit(org, bs, tmp, 64, w, h, kWienerCand[wbest], kWienerCand[wbest], 8);
+                orc_wiener_unit(org, bs, tmp, 64, w, h, kWienerCand[wbest], kWienerCand[wbest], bd);
                 uint64_t s = 0;
                 for (int y = 0; y < h; y++)
                     for (int x = 0; x < w; x++) {
@@ -168,7 +175,7 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
             }
             if (!valid) continue;
             for (int p = 0; p < 4; p++) {
-                orc_sgr_unit(org, bs, tmp, 64, w, h, kSgrPreset[p], kSgrXqd[p], 8);
+                orc_sgr_unit(org, bs, tmp, 64, w, h, kSgrPreset[p], kSgrXqd[p], bd);
                 uint64_t s = 0;
                 for (int y = 0; y < h; y++)
                     for (int x = 0; x < w; x++) {
@@ -177,7 +184,7 @@ int orc_lr_search(const orc_frame *src, const orc_frame *rec, int us, av1pf_lr_u
                     }
                 e[8 + p] = s;
             }
-            orc_lr_decide(e, w, h, &out[u]);
+            lr_decide(e, w, h, cs, &out[u]);
         }
     }
     free(bord);

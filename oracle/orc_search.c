@@ -15,8 +15,13 @@
  *   cdef/filter_search.go:20-74         FilterBlocksU8SubsampledTrusted (luma adjust only when the
  *                                       trial primary is nonzero; chroma takes luma directions)
  *   encoder/cdef_search.go:780-945      searchOneDual / jointStrengthSearchDual / finish
- * Only 8-bit 4:2:0 frames whose size is a multiple of 8 are searched, like the
- * reference encoder path (8-bit byte planes).
+ * 4:2:0 frames whose size is a multiple of 8 are searched.  The reference
+ * encoder searches 8-bit frames only; 10-bit frames are an extension with no
+ * reference counterpart: the trials are the decoder's high-bit-depth CDEF
+ * (strengths and damping shifted by bd - 8, directions from samples shifted
+ * down by bd - 8, the filter tap set chosen by the unshifted primary), which
+ * orc_cdef.c restates from postfilter_cdef.go, so every table entry is the
+ * SSE of the decoder CDEF at that strength.
  */
 #include "orc.h"
 #include <stdlib.h>
@@ -43,9 +48,10 @@ static void load_input(uint16_t *in, const uint16_t *plane, int stride, int pw, 
 int orc_cdef_search_mse(const orc_frame *src, const orc_frame *rec, const uint64_t *noskip,
                         int damping, uint64_t *mse)
 {
-    if (!src || !rec || !mse || rec->bd != 8 || rec->ssx != 1 || rec->ssy != 1 || rec->mono ||
-        (rec->w & 7) || (rec->h & 7))
+    if (!src || !rec || !mse || (rec->bd != 8 && rec->bd != 10) || src->bd != rec->bd || rec->ssx != 1 ||
+        rec->ssy != 1 || rec->mono || (rec->w & 7) || (rec->h & 7))
         return AV1PF_ERR_INVALID_ARG;
+    const int cs = rec->bd - 8;
     const int mi_cols = ((rec->w + 7) >> 3) << 1, mi_rows = ((rec->h + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16, nfb = nhfb * nvfb;
     uint16_t *in = malloc(BSTRIDE * 132 * sizeof(uint16_t));
@@ -82,15 +88,16 @@ int orc_cdef_search_mse(const orc_frame *src, const orc_frame *rec, const uint64
                     for (int b = 0; b < nblk; b++) {
                         int d;
                         int32_t v;
-                        orc_cdef_find_dir(rp + (size_t)(y0 + (blk[b][0] << 3)) * rs + x0 + (blk[b][1] << 3), rs, 0, &d, &v);
+                        orc_cdef_find_dir(rp + (size_t)(y0 + (blk[b][0] << 3)) * rs + x0 + (blk[b][1] << 3), rs, cs, &d, &v);
                         dirs[blk[b][0]][blk[b][1]] = (uint8_t)d;
                         vars[blk[b][0]][blk[b][1]] = v;
                     }
-                const int pd = damping - (pli ? 1 : 0);
+                const int pd = damping + cs - (pli ? 1 : 0);
                 for (int gi = 0; gi < 64; gi++) {
                     int packed = level1_packed(gi);
-                    int pri = packed >> 2, sec = packed & 3;
+                    int pri = (packed >> 2) << cs, sec = packed & 3;
                     if (sec == 3) sec = 4;
+                    sec <<= cs;
                     uint64_t sum = 0;
                     for (int b = 0; b < nblk; b++) {
                         int by = blk[b][0], bx = blk[b][1];
@@ -100,7 +107,7 @@ int orc_cdef_search_mse(const orc_frame *src, const orc_frame *rec, const uint64
                             for (int r = 0; r < bs; r++)
                                 for (int c = 0; c < bs; c++) tmp[r * bs + c] = org[r * BSTRIDE + c];
                         } else {
-                            orc_cdef_filter_block(tmp, bs, org, st, sec, pri ? dirs[by][bx] : 0, pd, pd, 0, bs, bs);
+                            orc_cdef_filter_block(tmp, bs, org, st, sec, pri ? dirs[by][bx] : 0, pd, pd, cs, bs, bs);
                         }
                         for (int r = 0; r < bs; r++)
                             for (int c = 0; c < bs; c++) {

@@ -34,19 +34,20 @@ def unit_rects(w, h, us=64):
     return out
 
 
-def forced_wiener_pair(w, h, seed=5):
+def forced_wiener_pair(w, h, seed=5, bd=8):
     """A source of zeros against a smooth reconstruction: the reference's
     H-only estimate (no identity tap) then beats the baseline, so the exact
     Wiener re-score runs."""
     rng = np.random.default_rng(seed)
-    src = synth.FrameData(w, h, 8, 1, 1)
-    rec = synth.FrameData(w, h, 8, 1, 1)
+    src = synth.FrameData(w, h, bd, 1, 1)
+    rec = synth.FrameData(w, h, bd, 1, 1)
+    k = 1 << (bd - 8)
     yy, xx = np.mgrid[0:rec.planes[0].alloc_h, 0:rec.planes[0].alloc_w]
-    ramp = 60 + 0.2 * xx + 0.1 * yy + rng.normal(0, 1.0, xx.shape)
-    ramp[:, : xx.shape[1] // 3] = 100          # a flat third: SGR cannot beat the baseline there
-    rec.planes[0].pix[:] = np.clip(ramp, 0, 255).astype(np.uint16)
+    ramp = k * (60 + 0.2 * xx + 0.1 * yy + rng.normal(0, 1.0, xx.shape))
+    ramp[:, : xx.shape[1] // 3] = 100 * k       # a flat third: SGR cannot beat the baseline there
+    rec.planes[0].pix[:] = np.clip(ramp, 0, 255 * k).astype(np.uint16)
     for p in rec.planes[1:]:
-        p.pix[:] = 128
+        p.pix[:] = 128 * k
     return src, rec
 
 
@@ -96,9 +97,10 @@ def sse(a, b):
     return int((d * d).sum())
 
 
+@pytest.mark.parametrize("bd", [8, 10])
 @pytest.mark.parametrize("size", [(256, 192), (200, 136)])
-def test_sgr_scores_equal_decoder_lr(size):
-    src, rec, _, _ = synth.search_pair(seed=7, w=size[0], h=size[1])
+def test_sgr_scores_equal_decoder_lr(size, bd):
+    src, rec, _, _ = synth.search_pair(seed=7, w=size[0], h=size[1], bd=bd)
     seam_safe(rec)
     units, e = orc.lr_search(src, rec, 64)
     S = src.planes[0].pix
@@ -111,8 +113,11 @@ def test_sgr_scores_equal_decoder_lr(size):
             assert int(e[i, 8 + p]) == sse(S[y0:y1, x0:x1], out[y0:y1, x0:x1]), f"unit {i} preset {p}"
 
 
-def test_wiener_exact_score_equals_decoder_lr():
-    src, rec = forced_wiener_pair(192, 128)
+@pytest.mark.parametrize("bd", [8, 10])
+def test_wiener_exact_score_equals_decoder_lr(bd):
+    """10-bit: the margin a candidate must beat the running best by is w*h
+    scaled by the squared sample range, 16 * w*h."""
+    src, rec = forced_wiener_pair(192, 128, bd=bd)
     seam_safe(rec)
     units, e = orc.lr_search(src, rec, 64)
     assert (e[:, 7] != NONE_SSE).any(), "the forced case must re-score a Wiener winner"
@@ -122,7 +127,7 @@ def test_wiener_exact_score_equals_decoder_lr():
             continue
         best, c = int(e[i, 0]), None     # the estimate's winner: first strict improvement by w*h
         for k in range(6):
-            if int(e[i, 1 + k]) + (x1 - x0) * (y1 - y0) < best:
+            if int(e[i, 1 + k]) + ((x1 - x0) * (y1 - y0) << (2 * (bd - 8))) < best:
                 best, c = int(e[i, 1 + k]), k
         assert c is not None
         if units[i, 0] == abi.LR_WIENER:
@@ -131,9 +136,10 @@ def test_wiener_exact_score_equals_decoder_lr():
         assert int(e[i, 7]) == sse(S[y0:y1, x0:x1], out[y0:y1, x0:x1]), f"unit {i}"
 
 
+@pytest.mark.parametrize("bd", [8, 10])
 @pytest.mark.parametrize("size", [(256, 192), (203, 141)])
-def test_wiener_estimate_numpy(size):
-    src, rec, _, _ = synth.search_pair(seed=8, w=size[0], h=size[1])
+def test_wiener_estimate_numpy(size, bd):
+    src, rec, _, _ = synth.search_pair(seed=8, w=size[0], h=size[1], bd=bd)
     _, e = orc.lr_search(src, rec, 64)
     R = rec.planes[0].pix[:rec.h, :rec.w].astype(np.int64)
     S = src.planes[0].pix[:rec.h, :rec.w].astype(np.int64)
@@ -141,7 +147,7 @@ def test_wiener_estimate_numpy(size):
     for c, (t0, t1, t2) in enumerate(WIENER):
         f = np.array([t0, t1, t2, -2 * (t0 + t1 + t2), t2, t1, t0])
         hs = sum(f[k] * padded[:, k:k + rec.w] for k in range(7))
-        pred = np.clip((hs + 64) >> 7, 0, 255)
+        pred = np.clip((hs + 64) >> 7, 0, (1 << bd) - 1)
         d2 = (S - pred) ** 2
         for i, (x0, x1, y0, y1) in enumerate(unit_rects(rec.w, rec.h)):
             if e[i, 0] == 0:
@@ -149,11 +155,12 @@ def test_wiener_estimate_numpy(size):
             assert int(e[i, 1 + c]) == int(d2[y0:y1, x0:x1].sum())
 
 
-def test_decisions_follow_scores():
-    src, rec, _, _ = synth.search_pair(seed=9, w=320, h=256)
+@pytest.mark.parametrize("bd", [8, 10])
+def test_decisions_follow_scores(bd):
+    src, rec, _, _ = synth.search_pair(seed=9, w=320, h=256, bd=bd)
     units, e = orc.lr_search(src, rec, 64)
     for i, (x0, x1, y0, y1) in enumerate(unit_rects(rec.w, rec.h)):
-        wh = (x1 - x0) * (y1 - y0)
+        wh = ((x1 - x0) * (y1 - y0)) << (2 * (bd - 8))
         t = units[i, 0]
         if t == abi.LR_SGRPROJ:
             p = [s for s, _ in SGR].index(units[i, 1])
@@ -188,14 +195,16 @@ def gpu_lr_search(src, rec, us):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", [("search", 1280, 720, 64), ("search", 200, 136, 64), ("search", 256, 192, 32),
-                                  ("search", 640, 360, 128), ("wiener", 192, 128, 64)])
+@pytest.mark.parametrize("case", [("search", 1280, 720, 64, 8), ("search", 200, 136, 64, 8), ("search", 256, 192, 32, 8),
+                                  ("search", 640, 360, 128, 8), ("wiener", 192, 128, 64, 8),
+                                  ("search", 1280, 720, 64, 10), ("search", 200, 136, 64, 10),
+                                  ("wiener", 192, 128, 64, 10)], ids=lambda c: f"{c[0]}_{c[1]}x{c[2]}_u{c[3]}_{c[4]}b")
 def test_gpu_lr_search_parity(case):
-    kind, w, h, us = case
+    kind, w, h, us, bd = case
     if kind == "search":
-        src, rec, _, _ = synth.search_pair(seed=50, w=w, h=h)
+        src, rec, _, _ = synth.search_pair(seed=50, w=w, h=h, bd=bd)
     else:
-        src, rec = forced_wiener_pair(w, h)
+        src, rec = forced_wiener_pair(w, h, bd=bd)
     print(f"lr search {case}", flush=True)
     gu, ge = gpu_lr_search(src, rec, us)
     wu, we = orc.lr_search(src, rec, us)

@@ -144,14 +144,16 @@ def test_training_reduces_loss(net):
 
 
 @pytest.mark.gpu
-def test_tiles_match_reference(net):
-    """Tiles of a ragged 8-bit frame pair: channel 0 the reconstruction,
-    channel 1 the amplified coding error, edges repeated past the frame."""
+@pytest.mark.parametrize("bd", [8, 10])
+def test_tiles_match_reference(net, bd):
+    """Tiles of a ragged frame pair: channel 0 the reconstruction, channel 1
+    the amplified coding error, both over 2^bd - 1, edges repeated past the
+    frame."""
     import torch
     from goav1_amd import synth
     from goav1_amd.lib import DeviceFrame
-    src, rec, _, _ = synth.search_pair(seed=21, w=200, h=136)
-    print("nn tiles 200x136", flush=True)
+    src, rec, _, _ = synth.search_pair(seed=21, w=200, h=136, bd=bd)
+    print(f"nn tiles 200x136 {bd}-bit", flush=True)
     got = net.tiles(DeviceFrame(src), DeviceFrame(rec)).cpu().numpy().view(np.uint16)
     torch.cuda.synchronize()
     want = np_tiles(src, rec)
@@ -160,20 +162,21 @@ def test_tiles_match_reference(net):
 
 
 def np_tiles(src, rec):
-    """BF16-valued FP32 tiles [nfb][64][64][2] of an 8-bit frame pair."""
+    """BF16-valued FP32 tiles [nfb][64][64][2] of a frame pair."""
     S = src.planes[0].pix[:rec.h, :rec.w].astype(np.float32)
     R = rec.planes[0].pix[:rec.h, :rec.w].astype(np.float32)
     nh, nv = (rec.w + 63) // 64, (rec.h + 63) // 64
     yy = np.minimum(np.arange(nv * 64), rec.h - 1)
     xx = np.minimum(np.arange(nh * 64), rec.w - 1)
-    inv = np.float32(1.0 / 255.0)
+    inv = np.float32(1.0 / ((1 << rec.bd) - 1))
     ch = [R[yy][:, xx] * inv, (S[yy][:, xx] - R[yy][:, xx]) * np.float32(16) * inv]
     want = np.stack([nn_ref.bf(c) for c in ch], axis=-1)            # [nv*64][nh*64][2]
     return np.ascontiguousarray(want.reshape(nv, 64, nh, 64, 2).transpose(0, 2, 1, 3, 4).reshape(nv * nh, 64, 64, 2))
 
 
 @pytest.mark.gpu
-def test_fused_inference_matches_reference(net):
+@pytest.mark.parametrize("bd", [8, 10])
+def test_fused_inference_matches_reference(net, bd):
     """The one-launch inference from frames against the reference forward of
     the same tiles, and against the layer-by-layer device pass, before and
     after an optimizer step (which repacks the convolution fragments)."""
@@ -184,11 +187,11 @@ def test_fused_inference_matches_reference(net):
     P = nn_ref.init_params(rng)
     P[nn_ref.param_count() - 131:] = rng.standard_normal(131) * 0.1
     net.set_params(P)
-    src, rec, _, _ = synth.search_pair(seed=22, w=328, h=200)
+    src, rec, _, _ = synth.search_pair(seed=22, w=328, h=200, bd=bd)
     ds, dr = DeviceFrame(src), DeviceFrame(rec)
     tiles = np_tiles(src, rec)
     for rnd in range(2):
-        print(f"nn fused inference 328x200 round {rnd}", flush=True)
+        print(f"nn fused inference 328x200 {bd}-bit round {rnd}", flush=True)
         fused = net.infer(ds, dr).cpu().numpy()
         layered = net.forward(net.tiles(ds, dr)).cpu().numpy()
         torch.cuda.synchronize()

@@ -54,20 +54,23 @@ def active_blocks(fr, noskip):
     return np.array([1 if block_list(fr, noskip, fb) else 0 for fb in range(noskip.shape[0])], np.uint8)
 
 
+@pytest.mark.parametrize("bd", [8, 10])
 @pytest.mark.parametrize("size", [(128, 128), (200, 136)])
-def test_unfiltered_trial_is_block_sse(size):
-    src, rec, noskip, damping = synth.search_pair(seed=11, w=size[0], h=size[1])
+def test_unfiltered_trial_is_block_sse(size, bd):
+    src, rec, noskip, damping = synth.search_pair(seed=11, w=size[0], h=size[1], bd=bd)
     mse = orc.cdef_search_mse(src, rec, noskip, damping)
     for fb in range(mse.shape[1]):
         sy, sc = block_sse(src, rec, src, block_list(src, noskip, fb))
         assert (int(mse[0, fb, 0]), int(mse[1, fb, 0])) == (sy, sc), f"fb {fb}"
 
 
+@pytest.mark.parametrize("bd", [8, 10])
 @pytest.mark.parametrize("size", [(128, 128), (200, 136)])
-def test_every_trial_equals_decoder_cdef_sse(size):
+def test_every_trial_equals_decoder_cdef_sse(size, bd):
     """The search's trial gi is the decoder's CDEF with strength LEVEL1[gi] on
-    every filter block, measured over the non-skip blocks."""
-    src, rec, noskip, damping = synth.search_pair(seed=12, w=size[0], h=size[1])
+    every filter block, measured over the non-skip blocks (10-bit: the
+    decoder's high-bit-depth CDEF, which the chain parity tests pin)."""
+    src, rec, noskip, damping = synth.search_pair(seed=12, w=size[0], h=size[1], bd=bd)
     mse = orc.cdef_search_mse(src, rec, noskip, damping)
     ucols, urows = (rec.w + 63) // 64, (rec.h + 63) // 64
     assert mse.shape[1] == ucols * urows
@@ -156,9 +159,10 @@ def gpu_search(src, rec, noskip, damping):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("size", [(1280, 720), (200, 136), (64, 64)])
+@pytest.mark.parametrize("size", [(1280, 720, 8), (200, 136, 8), (64, 64, 8), (1280, 720, 10), (200, 136, 10)],
+                         ids=lambda s: f"{s[0]}x{s[1]}_{s[2]}b")
 def test_gpu_search_parity(size):
-    src, rec, noskip, damping = synth.search_pair(seed=21, w=size[0], h=size[1])
+    src, rec, noskip, damping = synth.search_pair(seed=21, w=size[0], h=size[1], bd=size[2])
     print(f"search {size} damping {damping}", flush=True)
     got = gpu_search(src, rec, noskip, damping)
     want = orc.cdef_search_mse(src, rec, noskip, damping)
@@ -247,16 +251,19 @@ def test_gpu_select_random_tables_with_ties(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", [(1280, 720, 160, 0), (200, 136, 100, 1), (1920, 1080, 220, 2), (2048, 1152, 120, 3),
-                                  (2304, 1856, 200, 4)])
+@pytest.mark.parametrize("case", [(1280, 720, 160, 0, 8), (200, 136, 100, 1, 8), (1920, 1080, 220, 2, 8),
+                                  (2048, 1152, 120, 3, 8), (2304, 1856, 200, 4, 8), (1280, 720, 160, 5, 10),
+                                  (200, 136, 60, 6, 10), (2304, 1856, 200, 7, 10)],
+                         ids=lambda c: f"{c[0]}x{c[1]}_q{c[2]}_{c[4]}b")
 def test_gpu_shortlist_equals_masked_host_finish(case):
     """The shortlist search's table equals the full table's shortlisted
     columns, and its one-launch selection equals the host finish over the full
-    table with every other strength priced out (shortlists ascending, with 0)."""
+    table with every other strength priced out (shortlists ascending, with 0).
+    10-bit tables take the 64-bit selection."""
     import torch
     from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_lambda, cdef_search_finish
-    w, h, q, seed = case
-    src, rec, noskip, damping = synth.search_pair(seed=60 + seed, w=w, h=h, q=q)
+    w, h, q, seed, bd = case
+    src, rec, noskip, damping = synth.search_pair(seed=60 + seed, w=w, h=h, q=q, bd=bd)
     rng = np.random.default_rng(seed)
     cand = np.concatenate([np.sort(np.r_[0, rng.choice(np.arange(1, 64), K - 1, replace=False)]) for _ in range(2)])
     print(f"shortlist {w}x{h} q{q} cand {cand.tolist()}", flush=True)
