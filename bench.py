@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-search", action="store_true", help="skip the encoder-search legs")
     ap.add_argument("--no-superres", action="store_true", help="skip the super-resolution leg")
-    ap.add_argument("--learned-frames", type=int, default=20, help="held-out frames of the learned-search leg")
+    ap.add_argument("--learned-frames", type=int, default=3, help="held-out frames per cell of the learned-search leg")
     ap.add_argument("--batch-frames", type=int, default=64,
                     help="frames in the mixed-batch leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -177,41 +177,40 @@ def search_leg(runner, steps, cpu):
     return out
 
 
-def learned_leg(runner, nframes, width=1920, height=1080):
+def learned_leg(runner, nframes):
     """Learned-proposal search against the exhaustive search on held-out
-    moving-rectangle frames (8-bit 4:2:0, q 60..220): the network (committed
-    weights) proposes a shortlist of luma and of chroma strengths (the
-    library's av1pf_search_shortlist_size() each) and the LR units worth
+    moving-rectangle frames (4:2:0, q 60..220) of the four search cells
+    (1920x1080 and 3840x2160, 8- and 10-bit), `nframes` per cell: the network
+    (committed weights) proposes a shortlist of luma and of chroma strengths
+    (the library's av1pf_search_shortlist_size() each) and the LR units worth
     searching; the shortlist is evaluated exactly on the device.  Both
     searches run the LR search on a second stream beside the CDEF work.
     PSNR-Y loss is measured against the exhaustive decision through the full
-    MSE table (CDEF) and the exact LR scores."""
+    MSE table (CDEF) and the exact LR scores; on 8-bit frames also against
+    the exhaustive superset LR mode."""
     import numpy as np
     from goav1_amd import learned
-    from goav1_amd.lib import ProposalNet, cdef_lambda
+    from goav1_amd.lib import ProposalNet
     dc = json.load(open(os.path.join(ROOT, "tests", "golden", "dc_qlookup_8bit.json")))["dc"]
     meta = json.load(open(os.path.splitext(learned.DEFAULT_WEIGHTS)[0] + ".json"))
     net = ProposalNet(runner, max_batch=64)
+    cells = []
     try:
         net.set_params(np.load(learned.DEFAULT_WEIGHTS))
-        frames = learned.held_out(nframes, width, height)
-        res = learned.evaluate(runner, net, frames, lambda q: cdef_lambda(dc[q]),
-                               np.array(meta["static_shortlist"], np.int32))
+        rank = np.argsort(-np.asarray(meta["label_counts"][0]), kind="stable")
+        for w, h, bd in learned.CELLS:
+            frames = learned.held_out(nframes, w, h, bd)
+            res = learned.evaluate(runner, net, frames, lambda q, b: learned.lambda_for(dc, q, b),
+                                   np.array(meta["static_shortlist"], np.int32), static_rank=rank)
+            cells.append(learned.cell_summary(res, w, h, bd))
     finally:
         net.close()
-    px = width * height
-    return {"workload": f"{nframes} held-out {width}x{height} 8-bit 4:2:0 frames, q 60..220: CDEF (64 luma x 64 "
-                        "chroma strengths) + curated luma LR search",
-            "exhaustive_mpx_s": round(px / res["ms_exhaustive"] / 1e3, 1),
-            "learned_mpx_s": round(px / res["ms_learned"] / 1e3, 1),
-            "static_shortlist_mpx_s": round(px / res["ms_static"] / 1e3, 1),
-            "ms_exhaustive": round(res["ms_exhaustive"], 4), "ms_learned": round(res["ms_learned"], 4),
-            "speedup": round(res["speedup_learned"], 2),
-            "cdef_psnr_loss_db": round(res["cdef_psnr_loss_learned"], 5),
-            "lr_psnr_loss_db": round(res["lr_psnr_loss_learned"], 5),
-            "cdef_rate_delta_bits": round(res["cdef_rate_delta_learned"], 2),
-            "static_cdef_psnr_loss_db": round(res["cdef_psnr_loss_static"], 5),
-            "static_lr_psnr_loss_db": round(res["lr_psnr_loss_static"], 5)}
+    return {"workload": f"{nframes} held-out frames per cell, 1920x1080 and 3840x2160, 8- and 10-bit 4:2:0, "
+                        "q 60..220: CDEF (64 luma x 64 chroma strengths) + curated luma LR search",
+            "speedup_min": min(c["speedup"] for c in cells),
+            "cdef_psnr_loss_db_max": max(c["cdef_psnr_loss_db"] for c in cells),
+            "lr_psnr_loss_db_max": max(c["lr_psnr_loss_db"] for c in cells),
+            "cells": cells}
 
 
 def batch_leg(runner, steps, nframes):

@@ -1926,13 +1926,16 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     HIPCHK(c, hipSetDevice(c->device));
     const size_t logits_b = (size_t)nfb * NN_OUT * 4, probs_b = (size_t)nfb * NN_PROP_STRIDE * 4;
     const size_t msub_b = (size_t)2 * nfb * NCAND * 8, mask_b = (size_t)nfb + 256;
-    const size_t need = align256(logits_b) + align256(probs_b) + align256(msub_b) + align256(mask_b) + 256;
+    const size_t part_b = (size_t)((nfb + 63) / 64) * NN_PROP_STRIDE * 4;
+    const size_t need = align256(logits_b) + align256(probs_b) + align256(msub_b) + align256(mask_b) +
+                        align256(part_b) + 256;
     if ((rc = ensure(c, (void **)&c->learn_buf, &c->learn_cap, need))) return rc;
     char *p = c->learn_buf;
     float *logits = (float *)p; p += align256(logits_b);
     float *probs = (float *)p; p += align256(probs_b);
     uint64_t *msub = (uint64_t *)p; p += align256(msub_b);
     uint8_t *mask = (uint8_t *)p; p += align256(mask_b);
+    float *partial = (float *)p; p += align256(part_b);
     float *conf_tmp = (float *)p;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StageTimer t(c, ST_SEARCH_LEARNED, s, 0);
@@ -1943,6 +1946,7 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     prop.mi_cols = ((recon->width + 7) >> 3) << 1;
     prop.mi_rows = ((recon->height + 7) >> 3) << 1;
     prop.probs = probs;
+    prop.partial = partial;
     prop.cand = cand;
     prop.conf = conf ? conf : conf_tmp;
     HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], rv.bd, nhfb, nfb, logits, &prop, s));

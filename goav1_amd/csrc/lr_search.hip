@@ -76,23 +76,61 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
         return;
     }
 
+    // a unit of at most 64 x 64 (the exact filters' limit) is staged once,
+    // with its 3-sample border edge-clamped at the frame, and the source
+    // beside it, every load in flight before the first LDS store; the
+    // baseline and the estimates then read LDS (the border columns are the
+    // frame-clamped columns the estimate reads)
+    const bool small = w <= 64 && h <= 64;
+    if (small) {
+        const int tw = w + 6, th = h + 6;
+        constexpr int KT = (LR_TH * 70 + 255) / 256, KS = 64 * 64 / 256;
+        uint16_t v[KT], sv[KS];
+#pragma unroll
+        for (int k = 0; k < KT; k++) {
+            const int i = tid + k * 256, r = i / tw, c = i - r * tw;
+            const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
+            v[k] = i < tw * th ? row_ptr<T>(R, y)[x] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int i = tid + k * 256, r = i >> 6, c = i & 63;
+            sv[k] = r < h && c < w ? row_ptr<T>(S, y0 + r)[x0 + c] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < KT; k++) {
+            const int i = tid + k * 256, r = i / tw, c = i - r * tw;
+            if (i < tw * th) tile[r * LR_TW + c + LR_OX - 3] = v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++) srcT[tid + k * 256] = sv[k];
+        __syncthreads();
+    }
+
     // ---- baseline and the H-only Wiener approximations (any unit size)
     uint64_t b = 0, a[6] = { 0, 0, 0, 0, 0, 0 };
     for (int i = tid; i < w * h; i += 256) {
-        const int y = y0 + i / w, x = x0 + i % w;
-        const T *rr = row_ptr<T>(R, y);
-        const int sv = row_ptr<T>(S, y)[x];
-        int px[7];
+        const int yy = i / w, xx = i % w;
+        int svv, px[7];
+        if (small) {
+            const uint16_t *rr = tile + (yy + 3) * LR_TW + LR_OX + xx;
+            svv = srcT[yy * 64 + xx];
 #pragma unroll
-        for (int k = 0; k < 7; k++) px[k] = rr[min(max(x + k - 3, 0), W - 1)];
-        const int d0 = sv - px[3];
+            for (int k = 0; k < 7; k++) px[k] = rr[k - 3];
+        } else {
+            const T *rr = row_ptr<T>(R, y0 + yy);
+            svv = row_ptr<T>(S, y0 + yy)[x0 + xx];
+#pragma unroll
+            for (int k = 0; k < 7; k++) px[k] = rr[min(max(x0 + xx + k - 3, 0), W - 1)];
+        }
+        const int d0 = svv - px[3];
         b += (uint64_t)(d0 * d0);
 #pragma unroll
         for (int c = 0; c < 6; c++) {
             const int t0 = kWienerCand[c][0], t1 = kWienerCand[c][1], t2 = kWienerCand[c][2];
             const int hs = t0 * (px[0] + px[6]) + t1 * (px[1] + px[5]) + t2 * (px[2] + px[4]) - 2 * (t0 + t1 + t2) * px[3];
             const int pred = min(max((hs + 64) >> 7, 0), maxv);
-            const int d = sv - pred;
+            const int d = svv - pred;
             a[c] += (uint64_t)(d * d);
         }
     }
@@ -109,19 +147,7 @@ __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R
             e[1 + c] = wg_sum(a[c], red);
             if (e[1 + c] + margin < best) { best = e[1 + c]; wbest = c; }
         }
-        if (w <= 64 && h <= 64) {
-            // ---- the unit with its 3-sample border, edge-clamped at the frame
-            const int tw = w + 6, th = h + 6;
-            for (int i = tid; i < tw * th; i += 256) {
-                const int r = i / tw, c = i - r * tw;
-                const int y = min(max(y0 + r - 3, 0), H - 1), x = min(max(x0 + c - 3, 0), W - 1);
-                tile[r * LR_TW + c + LR_OX - 3] = row_ptr<T>(R, y)[x];
-            }
-            for (int i = tid; i < w * h; i += 256) {
-                const int r = i / w, c = i - r * w;
-                srcT[r * 64 + c] = row_ptr<T>(S, y0 + r)[x0 + c];
-            }
-            __syncthreads();
+        if (small) {
             const uint16_t *O = tile + 3 * LR_TW + LR_OX;
             uint64_t acc = 0;
             auto err = [&](int y, int x, int v) {

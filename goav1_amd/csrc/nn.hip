@@ -359,10 +359,21 @@ __global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R,
     const int fb = blockIdx.x, fx = (fb % nhfb) * 64, fy = (fb / nhfb) * 64;
     const float inv = 1.f / (float)((1 << bd) - 1);
     uint32_t *tile = reinterpret_cast<uint32_t *>(bufA);
-    for (int i = t; i < 4096; i += 256) {
-        const int y = min(fy + (i >> 6), R.h - 1), x = min(fx + (i & 63), R.w - 1);
-        const float sv = row_ptr<T>(S, y)[x], rv = row_ptr<T>(R, y)[x];
-        tile[i] = (uint32_t)f2bf(rv * inv) | ((uint32_t)f2bf((sv - rv) * 16.f * inv) << 16);
+    {
+        // every load of the lane in flight before the first conversion
+        T sv[16], rv[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = t + k * 256;
+            const int y = min(fy + (i >> 6), R.h - 1), x = min(fx + (i & 63), R.w - 1);
+            sv[k] = row_ptr<T>(S, y)[x];
+            rv[k] = row_ptr<T>(R, y)[x];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const float s = sv[k], r = rv[k];
+            tile[t + k * 256] = (uint32_t)f2bf(r * inv) | ((uint32_t)f2bf((s - r) * 16.f * inv) << 16);
+        }
     }
     __syncthreads();
     // conv1: M 1024 (64 row blocks, one output half-row each), N 16, K 18 in two k-steps
@@ -491,32 +502,31 @@ __global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R,
     }
 }
 
-// Shortlist from the per-block probabilities: thread (column c, part k) sums
-// the blocks f = k, k + 8, ...; the parts are added in order, so the sums do
-// not depend on timing.  Then the NCAND strengths with the largest sums per
-// plane (ascending strength order) and the mean largest luma probability.
-__global__ __launch_bounds__(1024) void propose_kernel(const float *probs, int nfb, int *cand, float *conf)
+// Shortlist from the per-block probabilities: the column sums in two passes
+// with a fixed order (so the ranking does not depend on timing): workgroup g
+// sums the blocks 64 g .. 64 g + 63 of every column, then one workgroup adds
+// the partial sums in g order and takes the NCAND strengths with the largest
+// sums per plane (ascending strength order) and the mean largest luma
+// probability.
+constexpr int PROP_CHUNK = 64;
+__global__ __launch_bounds__(256) void propose_partial_kernel(const float *probs, int nfb, float *partial)
 {
-    __shared__ float part[8][NN_PROP_STRIDE];
-    __shared__ float sums[NN_PROP_STRIDE];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    {
-        const int c = t & 127, k = t >> 7;
+    const int g = blockIdx.x, f0 = g * PROP_CHUNK, f1 = min(nfb, f0 + PROP_CHUNK);
+    for (int c = threadIdx.x; c < NN_PROP_STRIDE; c += 256) {
         float acc = 0.f;
 #pragma unroll 8
-        for (int f = k; f < nfb; f += 8) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
-        part[k][c] = acc;
+        for (int f = f0; f < f1; f++) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
+        partial[(size_t)g * NN_PROP_STRIDE + c] = acc;
     }
-    if (t < 16) {
-        const int c = 128 + (t & 1), k = t >> 1;
-        float acc = 0.f;
-        for (int f = k; f < nfb; f += 8) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
-        part[k][c] = acc;
-    }
-    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void propose_kernel(const float *partial, int ng, int *cand, float *conf)
+{
+    __shared__ float sums[NN_PROP_STRIDE];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t < NN_PROP_STRIDE) {
         float v = 0.f;
-        for (int k = 0; k < 8; k++) v += part[k][t];
+        for (int g = 0; g < ng; g++) v += partial[(size_t)g * NN_PROP_STRIDE + t];
         sums[t] = v;
     }
     __syncthreads();
@@ -548,7 +558,11 @@ hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, in
         hipLaunchKernelGGL((nn_infer_kernel<false, uint8_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
     else
         hipLaunchKernelGGL((nn_infer_kernel<false, uint16_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
-    if (prop) hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->probs, nfb, prop->cand, prop->conf);
+    if (prop) {
+        const int ng = (nfb + PROP_CHUNK - 1) / PROP_CHUNK;
+        hipLaunchKernelGGL(propose_partial_kernel, dim3(ng), dim3(256), 0, s, prop->probs, nfb, prop->partial);
+        hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->partial, ng, prop->cand, prop->conf);
+    }
     return hipGetLastError();
 }
 

@@ -87,6 +87,7 @@ struct NNPropose {
     const uint64_t *noskip;
     int mi_cols, mi_rows;
     float *probs;            // [nfb][NN_PROP_STRIDE] scratch
+    float *partial;          // [ceil(nfb / 64)][NN_PROP_STRIDE] scratch: column sums of 64 blocks each
     int *cand;               // [2 * NCAND], ascending per plane
     float *conf;             // mean largest luma probability over the active blocks
 };

@@ -100,17 +100,61 @@ __device__ void s_find_dir(const uint16_t *img, int *dir, int32_t *var)
     *var = (int32_t)((uint32_t)bc - c[(best + 4) & 7]) >> 10;
 }
 
-// Load a w x h region at (x0,y0) plus the 2/8 halo; VeryLarge outside [0,pw)x[0,ph).
-template <typename T>
-__device__ __forceinline__ void s_load(uint16_t *tile, const PlaneView &pv, int x0, int y0, int w, int h, int tid)
+// The reconstruction tile (ROWS rows of SP columns: the block with its
+// 2-row / 8-column halo, VeryLarge outside [0,pw)x[0,ph); the whole tile is
+// staged whatever the block's size, so a lane's positions are fixed) and the
+// filter block's source samples (w x h of at most S x S) into srcT (pitch
+// 64).  BATCH: every load in flight before the first LDS store (for the
+// kernel whose occupancy its registers already set); otherwise a plain loop
+// that keeps the register count low.
+template <typename T, int ROWS, int S, bool BATCH = true>
+__device__ __forceinline__ void s_stage(uint16_t *tile, const PlaneView &rp, uint16_t *srcT, const PlaneView &sp,
+                                        int x0, int y0, int w, int h, int tid)
 {
-    const int rows = h + 4, cols = w + 16;
-    for (int i = tid; i < rows * cols; i += 256) {
-        const int r = i / cols, c = i - r * cols;
+    constexpr int N = ROWS * SP, K = (N + 255) / 256, KS = S * S / 256;
+    auto rec_at = [&](int i) -> uint16_t {
+        const int r = i / SP, c = i - r * SP;
         const int y = y0 - 2 + r, x = x0 - 8 + c;
-        uint16_t v = SVL;
-        if (y >= 0 && y < pv.h && x >= 0 && x < pv.w) v = row_ptr<T>(pv, y)[x];
-        tile[r * SP + c] = v;
+        return (y >= 0 && y < rp.h && x >= 0 && x < rp.w) ? (uint16_t)row_ptr<T>(rp, y)[x] : (uint16_t)SVL;
+    };
+    auto src_at = [&](int i) -> uint16_t {
+        const int r = i / S, c = i % S;
+        return (r < h && c < w) ? (uint16_t)row_ptr<T>(sp, y0 + r)[x0 + c] : (uint16_t)0;
+    };
+    if constexpr (BATCH) {
+        uint16_t v[K], u[KS];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * 256;
+            v[k] = i < N ? rec_at(i) : (uint16_t)SVL;
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++) u[k] = src_at(tid + k * 256);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * 256;
+            if (i < N) tile[i] = v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int i = tid + k * 256;
+            srcT[(i / S) * 64 + i % S] = u[k];
+        }
+    } else {
+        for (int i = tid; i < N; i += 256) tile[i] = rec_at(i);
+        for (int i = tid; i < S * S; i += 256) srcT[(i / S) * 64 + i % S] = src_at(i);
+    }
+}
+
+// The 8x8 luma blocks of the filter block that have a non-skip MI, in raster
+// order (one wave: a lane per block position, compacted by ballot).
+__device__ __forceinline__ void s_block_list(uint64_t m, int nhb, int nvb, uint8_t *blist, int *nblk, int tid)
+{
+    if (tid < 64) {
+        const bool on = 2 * (tid >> 3) < nvb && 2 * (tid & 7) < nhb && ((m >> tid) & 1);
+        const uint64_t vote = __ballot(on);
+        if (on) blist[__popcll(vote & ((1ull << tid) - 1))] = (uint8_t)tid;
+        if (tid == 0) *nblk = __popcll(vote);
     }
 }
 
@@ -198,6 +242,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
 {
     using T = typename std::conditional<CS == 0, uint8_t, uint16_t>::type;
     __shared__ uint16_t tile[68 * SP];
+    __shared__ uint16_t srcT[64 * 64];
     __shared__ uint8_t bdir[64];
     __shared__ int8_t badj[64];      // adjust index i, -1 = zero variance
     __shared__ uint8_t blist[64];
@@ -210,15 +255,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
     const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
     const uint64_t m = noskip ? noskip[fb] : ~0ull;
-    if (tid == 0) {
-        int n = 0;
-        for (int r = 0; r < nvb; r += 2)
-            for (int c = 0; c < nhb; c += 2) {
-                const int b = (r >> 1) * 8 + (c >> 1);
-                if ((m >> b) & 1) blist[n++] = (uint8_t)b;
-            }
-        nblk_s = n;
-    }
+    s_block_list(m, nhb, nvb, blist, &nblk_s, tid);
     __syncthreads();
     const int nblk = nblk_s;
     uint64_t *m0 = mse + (size_t)fb * 64, *m1 = mse + ((size_t)nfb + fb) * 64;
@@ -230,7 +267,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     // ---- luma
     {
         const int x0 = fbc * 64, y0 = fbr * 64, w = nhb * 4, h = nvb * 4;
-        s_load<T>(tile, rec.p[0], x0, y0, w, h, tid);
+        s_stage<T, 68, 64>(tile, rec.p[0], srcT, src.p[0], x0, y0, w, h, tid);
         __syncthreads();
         if (tid < nblk) {
             const int b = blist[tid];
@@ -254,8 +291,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
             const int b = blist[p >> 6], q = p & 63;
             const int by = b >> 3, bx = b & 7, yy = by * 8 + (q >> 3), xx = bx * 8 + (q & 7);
             const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
-            const int sv = row_ptr<T>(src.p[0], y0 + yy)[x0 + xx];
-            s_pixel<true, CS>(acc, c, sv, bdir[b], badj[b], badj[b] >= 0, damping + CS);
+            s_pixel<true, CS>(acc, c, srcT[yy * 64 + xx], bdir[b], badj[b], badj[b] >= 0, damping + CS);
         }
         s_reduce(acc, part, tid);
         __syncthreads();
@@ -267,15 +303,14 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     for (int pl = 1; pl <= 2; pl++) {
         const int x0 = fbc * 32, y0 = fbr * 32, w = nhb * 2, h = nvb * 2;
         __syncthreads();
-        s_load<T>(tile, rec.p[pl], x0, y0, w, h, tid);
+        s_stage<T, 36, 32>(tile, rec.p[pl], srcT, src.p[pl], x0, y0, w, h, tid);
         __syncthreads();
         const int npx = nblk * 16;
         for (int p = tid; p < npx; p += 256) {
             const int b = blist[p >> 4], q = p & 15;
             const int by = b >> 3, bx = b & 7, yy = by * 4 + (q >> 2), xx = bx * 4 + (q & 3);
             const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
-            const int sv = row_ptr<T>(src.p[pl], y0 + yy)[x0 + xx];
-            s_pixel<false, CS>(acc, c, sv, bdir[b], 0, true, damping - 1 + CS);
+            s_pixel<false, CS>(acc, c, srcT[yy * 64 + xx], bdir[b], 0, true, damping - 1 + CS);
         }
     }
     __syncthreads();
@@ -859,6 +894,7 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
 {
     using T = typename std::conditional<CS == 0, uint8_t, uint16_t>::type;
     __shared__ uint16_t tile[68 * SP];
+    __shared__ uint16_t srcT[64 * 64];
     __shared__ uint8_t bdir[64];
     __shared__ int8_t badj[64];
     __shared__ uint8_t blist[64];
@@ -878,15 +914,7 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
         pk[1][q] = kLevel1[cand[NCAND + q]];
     }
     const unsigned need[2] = { s_need(pk[0]), s_need(pk[1]) };
-    if (tid == 0) {
-        int n = 0;
-        for (int r = 0; r < nvb; r += 2)
-            for (int c = 0; c < nhb; c += 2) {
-                const int b = (r >> 1) * 8 + (c >> 1);
-                if ((m >> b) & 1) blist[n++] = (uint8_t)b;
-            }
-        nblk_s = n;
-    }
+    s_block_list(m, nhb, nvb, blist, &nblk_s, tid);
     __syncthreads();
     const int nblk = nblk_s;
     uint64_t *m0 = mse + (size_t)fb * NCAND, *m1 = mse + ((size_t)nfb + fb) * NCAND;
@@ -903,7 +931,8 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
             for (int q = 0; q < NCAND; q++) acc[q] = 0;
         }
         __syncthreads();
-        s_load<T>(tile, rec.p[pl], x0, y0, w, h, tid);
+        if (pl == 0) s_stage<T, 68, 64, false>(tile, rec.p[pl], srcT, src.p[pl], x0, y0, w, h, tid);
+        else s_stage<T, 36, 32, false>(tile, rec.p[pl], srcT, src.p[pl], x0, y0, w, h, tid);
         __syncthreads();
         if (pl == 0) {
             if (tid < nblk) {
@@ -927,15 +956,15 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
                 const int b = __builtin_amdgcn_readfirstlane(blist[p >> 6]), q = p & 63;
                 const int d = __builtin_amdgcn_readfirstlane(bdir[b]), adj = __builtin_amdgcn_readfirstlane(badj[b]);
                 const int yy = (b >> 3) * 8 + (q >> 3), xx = (b & 7) * 8 + (q & 7);
-                s_pixel_sub<true, CS>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<T>(src.p[0], y0 + yy)[x0 + xx],
-                                      d, adj, adj >= 0, damping + CS, pk[0], need[0]);
+                s_pixel_sub<true, CS>(acc, tile + (2 + yy) * SP + 8 + xx, srcT[yy * 64 + xx], d, adj, adj >= 0,
+                                      damping + CS, pk[0], need[0]);
             }
         } else {
             for (int p = tid; p < nblk * 16; p += 256) {
                 const int b = blist[p >> 4], q = p & 15;
                 const int yy = (b >> 3) * 4 + (q >> 2), xx = (b & 7) * 4 + (q & 3);
-                s_pixel_sub<false, CS>(acc, tile + (2 + yy) * SP + 8 + xx, row_ptr<T>(src.p[pl], y0 + yy)[x0 + xx],
-                                       bdir[b], 0, true, damping - 1 + CS, pk[1], need[1]);
+                s_pixel_sub<false, CS>(acc, tile + (2 + yy) * SP + 8 + xx, srcT[yy * 64 + xx], bdir[b], 0, true,
+                                       damping - 1 + CS, pk[1], need[1]);
             }
         }
         if (pl == 0 || pl == 2) {
@@ -977,6 +1006,7 @@ struct SmallLevel {
 struct SmallState {
     SmallLevel lv[4];
     unsigned int ticket;
+    unsigned int wide;     // the 32-bit selection found an SSE of 2^28 or more: the 64-bit one runs
 };
 static_assert(sizeof(SmallState) <= 1024, "small selection state");
 #define SMALL_MAX_FB 2048
@@ -1014,7 +1044,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return max(v, xchg64<32>(v));
 }
 
-template <int NPER>
+template <int NPER, bool WIDE_CHECK>
 __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
                                                             int nvfb, int mi_cols, int mi_rows,
                                                             unsigned long long lambda, const int *cand,
@@ -1058,6 +1088,25 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
         for (int q = 0; q < NCAND; q++) {
             Y[r][q] = i < nact ? (int)mse[fb * NCAND + q] : 0;
             U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
+        }
+    }
+    // the 32-bit arithmetic below is exact when every SSE is below 2^28 (a
+    // pair's cost below 2^29, eight gains below 2^32): always for 8-bit
+    // frames; a 10-bit table with a larger value is left to the 64-bit
+    // selection (fin_small_wide_kernel, launched after this one)
+    if (WIDE_CHECK) {
+        bool big = false;
+#pragma unroll
+        for (int r = 0; r < NPER; r++) {
+            const int i = t + r * SMALL_T;
+            const size_t fb = i < nact ? act[i] : 0;
+#pragma unroll
+            for (int q = 0; q < NCAND; q++)
+                big |= i < nact && ((mse[fb * NCAND + q] | mse[((size_t)nfb + fb) * NCAND + q]) >> 28) != 0;
+        }
+        if (__syncthreads_or(big)) {
+            if (t == 0 && blockIdx.x == 0) ss->wide = 1;
+            return;
         }
     }
     // pair lanes: lane l of wave w serves pair pr = 8 j + k (j, k < NCAND) for
@@ -1241,6 +1290,8 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_wide_kernel(const uint64_t 
 {
     constexpr int NW = SMALL_T / 64;
     constexpr int NP = NCAND * NCAND;
+    // runs only when the 32-bit selection before it gave up on this table
+    if (!*reinterpret_cast<volatile unsigned *>(&ss->wide)) return;
     __shared__ uint16_t act[SMALL_MAX_FB];
     __shared__ uint8_t flag[SMALL_MAX_FB];
     __shared__ unsigned long long wpart[NW][NP + 1];   // per-wave pair gains (or column sums) and base sum
@@ -1384,6 +1435,7 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_wide_kernel(const uint64_t 
         unsigned a = 0, c = 0;
         for (int i = 0; i < n; i++) { a |= (unsigned)V[best_bits].ys[i] << (4 * i); c |= (unsigned)V[best_bits].us[i] << (4 * i); }
         ss->ticket = 0;
+        ss->wide = 0;
         dec->cdef_bits = best_bits;
         for (int i = 0; i < 8; i++) {
             dec->y_strength[i] = i < n ? kLevel1[cand[V[best_bits].ys[i]]] : 0;
@@ -1424,12 +1476,24 @@ hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, c
         hipLaunchKernelGGL(cdef_search_subset_kernel<2>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping,
                            cand, mse_sub, nhfb, nvfb);
     const int nfb = nhfb * nvfb;
-    auto fin = rec.bd == 8 ? (nfb <= SMALL_T ? fin_small_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_kernel<2>
-                                                                                        : fin_small_kernel<4>)
-                           : (nfb <= SMALL_T ? fin_small_wide_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_wide_kernel<2>
-                                                                                             : fin_small_wide_kernel<4>);
-    hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows,
-                       (unsigned long long)lambda, cand, static_cast<SmallState *>(small_state), dec, fb_index);
+    SmallState *ss = static_cast<SmallState *>(small_state);
+    const unsigned long long lam = lambda;
+    if (rec.bd == 8) {
+        auto fin = nfb <= SMALL_T ? fin_small_kernel<1, false> : nfb <= 2 * SMALL_T ? fin_small_kernel<2, false>
+                                                                                    : fin_small_kernel<4, false>;
+        hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam, cand,
+                           ss, dec, fb_index);
+    } else {
+        // 10-bit: the 32-bit selection when every SSE allows it, else the 64-bit one
+        auto fin = nfb <= SMALL_T ? fin_small_kernel<1, true> : nfb <= 2 * SMALL_T ? fin_small_kernel<2, true>
+                                                                                   : fin_small_kernel<4, true>;
+        auto wide = nfb <= SMALL_T ? fin_small_wide_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_wide_kernel<2>
+                                                                                    : fin_small_wide_kernel<4>;
+        hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam, cand,
+                           ss, dec, fb_index);
+        hipLaunchKernelGGL(wide, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam,
+                           cand, ss, dec, fb_index);
+    }
     return hipGetLastError();
 }
 

@@ -15,12 +15,18 @@ predicted NONE.
 
     python -m goav1_amd.learned train --frames 300 --out goav1_amd/models/proposal_net.npy
 
-Frames: moving-rectangle 8-bit 4:2:0 sources (the 1080p sequence recipe at
-8 bits) with reconstructions from 8x8 DCT dead-zone quantisation at a random
-qindex in 60..220.
+Frames: moving-rectangle 4:2:0 sources (the 1080p sequence recipe) at
+1920x1080 and 3840x2160, 8- and 10-bit, with reconstructions from 8x8 DCT
+dead-zone quantisation at a random qindex in 60..220.  One network serves
+both depths (its inputs are divided by 2^bd - 1).  10-bit searches are an
+extension of goav1's 8-bit encoder: the RD lambda is the 8-bit one times 16
+(SSE grows with the squared sample range).
+
+    python -m goav1_amd.learned eval --dc-table tests/golden/dc_qlookup_8bit.json --eval-frames 50
 """
 import argparse
 import json
+import multiprocessing
 import os
 import time
 
@@ -48,12 +54,68 @@ def init_params(rng):
     return np.concatenate(P)
 
 
-def frame_pair(seed, w, h, q):
-    """(src, rec, noskip, damping) for a moving-rectangle 8-bit source."""
+# the search workloads: (width, height, bit depth)
+CELLS = [(1920, 1080, 8), (1920, 1080, 10), (3840, 2160, 8), (3840, 2160, 10)]
+
+
+def frame_pair(seed, w, h, q, bd=8):
+    """(src, rec, noskip, damping) for a moving-rectangle source."""
     rng = np.random.default_rng(seed)
-    src = synth.FrameData(w, h, 8, 1, 1)
+    src = synth.FrameData(w, h, bd, 1, 1)
     synth.fill_rectangles(src, rng, synth.make_rects(rng, w, h), int(rng.integers(0, 8)))
     return synth.search_pair(seed=seed + 1, w=w, h=h, q=q, src=src, model="dct")
+
+
+def lambda_for(dc, q, bd):
+    """RD lambda of the search at qindex q: goav1's full-search lambda from the
+    8-bit DC quantiser, times 16 for 10-bit frames (whose SSEs are 16 times
+    larger for the same picture)."""
+    from .lib import cdef_lambda
+    return cdef_lambda(dc[q] if dc else 223) << (2 * (bd - 8))
+
+
+def _make(spec):
+    seed, w, h, q, bd = spec
+    src, rec, noskip, damping = frame_pair(seed, w, h, q, bd)
+    return src, rec, noskip, damping, q
+
+
+def pairs(specs, workers=None):
+    """Frame pairs of (seed, w, h, q, bd) specs, in order.  With workers > 1 a
+    process pool makes them; call it before this process touches the GPU (the
+    pool forks) and consume the iterator afterwards."""
+    if workers and workers > 1:
+        pool = multiprocessing.get_context("fork").Pool(workers)
+        it = pool.imap(_make, specs, chunksize=1)
+
+        def gen():
+            try:
+                for x in it:
+                    yield x
+            finally:
+                pool.terminate()
+        return gen()
+    return (_make(sp) for sp in specs)
+
+
+def train_specs(n_frames, seed=1000):
+    """Training frames over the four cells: 1080p and 4K in proportion 4 : 1
+    (about as many filter blocks of each size), 8- and 10-bit alternately."""
+    specs = []
+    for f in range(n_frames):
+        w, h = (3840, 2160) if f % 5 == 4 else (1920, 1080)
+        bd = 10 if ((f // 5) if w == 3840 else f) % 2 else 8
+        sd = seed + 7 * f
+        specs.append((sd, w, h, int(np.random.default_rng(sd).integers(60, 221)), bd))
+    return specs
+
+
+def held_out_specs(n, w, h, bd, seed=900000):
+    out = []
+    for f in range(n):
+        sd = seed + 11 * f + (bd << 20) + (w << 8)
+        out.append((sd, w, h, int(np.random.default_rng(sd).integers(60, 221)), bd))
+    return out
 
 
 def geometry(w, h):
@@ -155,13 +217,13 @@ def targets_for(mse, lr_units, w, h, act, temp=TEMP):
     return fbs, T
 
 
-def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None, temp=TEMP):
-    """Tiles and targets of n_frames frames from the exhaustive device search."""
+def build_dataset(runner, net, frames, log=None, temp=TEMP):
+    """Tiles and targets of frame pairs (src, rec, noskip, damping, q) from
+    the exhaustive device search."""
     import torch
     tiles, tg, counts = [], [], np.zeros((2, 64), np.int64)
-    for f in range(n_frames):
-        q = int(np.random.default_rng(seed + 7 * f).integers(60, 221))
-        src, rec, noskip, damping = frame_pair(seed + 7 * f, w, h, q)
+    for f, (src, rec, noskip, damping, q) in enumerate(frames):
+        w, h = rec.w, rec.h
         d = Device(src, rec, noskip)
         run_exhaustive(runner, d, damping, 1000, with_sse=False)
         t = net.tiles(d.ds, d.dr)
@@ -175,7 +237,7 @@ def build_dataset(runner, net, n_frames, seed=1000, w=1280, h=720, log=None, tem
         for p in range(2):
             counts[p] += np.bincount(T[:, 64 * p:64 * p + 64].argmax(axis=1), minlength=64)
         if log and (f + 1) % 25 == 0:
-            log(f"dataset: {f + 1}/{n_frames} frames, {sum(x.shape[0] for x in tiles)} tiles")
+            log(f"dataset: {f + 1} frames, {sum(x.shape[0] for x in tiles)} tiles")
     return torch.cat(tiles), torch.cat(tg), counts
 
 
@@ -213,8 +275,8 @@ def train(runner, net, tiles, targets, epochs=6, batch=256, lr=2e-3, seed=0, log
     return history
 
 
-def psnr(sse, npx):
-    return 10 * np.log10(255.0 ** 2 * npx / max(sse, 1))
+def psnr(sse, npx, bd=8):
+    return 10 * np.log10(float((1 << bd) - 1) ** 2 * npx / max(sse, 1))
 
 
 def cdef_luma_sse(full_mse, dec, idx, act, base_sse):
@@ -246,16 +308,49 @@ def topk_gain(lm, order, ks=(1, 2, 4, 8)):
     return {k: float((lm[:, 0] - lm[:, np.asarray(order[:k])].min(axis=1)).sum()) for k in ks}
 
 
+def lr_exhaustive_mode_sse(runner, d, base):
+    """Frame luma SSE after LR with the exhaustive superset mode's decisions
+    (8-bit): per unit the running best of its 28 scores under the curated
+    rule (a candidate replaces the best when sse + w*h is below it)."""
+    import torch
+    n = d.hz * d.vt
+    units = torch.zeros((n, 16), dtype=torch.uint8, device="cuda")
+    e = torch.zeros((n, 28), dtype=torch.int64, device="cuda")
+    fit = torch.zeros((n, 38), dtype=torch.int8, device="cuda")
+    runner.lr_search_exhaustive(d.ds, d.dr, 64, units.data_ptr(), e.data_ptr(), fit.data_ptr())
+    runner.sync()
+    torch.cuda.synchronize()
+    E = e.cpu().numpy().view(np.uint64)
+    W, H = d.rec.w, d.rec.h
+    s = base
+    for u in range(n):
+        col, row = u % d.hz, u // d.hz
+        x0, x1 = col * 64, (W if col == d.hz - 1 else col * 64 + 64)
+        y0, y1 = row * 64, (H if row == d.vt - 1 else row * 64 + 64)
+        y0 -= 8 if y0 > 0 else 0
+        y1 -= 8 if y1 < H else 0
+        wh = (x1 - x0) * (y1 - y0)
+        best = int(E[u, 0])
+        for c in range(1, 28):
+            if E[u, c] != np.uint64(2 ** 64 - 1) and int(E[u, c]) + wh < best:
+                best = int(E[u, c])
+        s += best - int(E[u, 0])
+    return s
+
+
 def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=None, static_rank=None):
     """PSNR-Y loss, CDEF rate and time of the learned and static-shortlist
-    searches against the exhaustive search on held-out frames."""
+    searches against the exhaustive search on held-out frames (lam_for(q, bd)
+    gives the RD lambda).  On 8-bit frames the learned LR decisions are also
+    compared with the exhaustive superset LR mode."""
     import torch
     stream = torch.cuda.current_stream()
     cand_static = torch.from_numpy(static_cand).cuda()
     rows = []
     t_ex = t_le = t_st = 0.0
     for fi, (src, rec, noskip, damping, q) in enumerate(frames):
-        lam = lam_for(q)
+        bd = rec.bd
+        lam = lam_for(q, bd)
         d = Device(src, rec, noskip)
         act = active_blocks(rec.w, rec.h, noskip)
         S, R = src.planes[0].pix[:rec.h, :rec.w].astype(np.int64), rec.planes[0].pix[:rec.h, :rec.w].astype(np.int64)
@@ -309,13 +404,15 @@ def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=
                        "frame_counts": topk_gain(lm, np.argsort(-counts, kind="stable"))}
         if static_rank is not None:
             row["topk"]["static"] = topk_gain(lm, static_rank)
-        row["unfiltered_psnr"] = psnr(base, npx)
+        row["unfiltered_psnr"] = psnr(base, npx, bd)
         for name, dec, idx, lru in (("exhaustive", ex_dec, ex_idx, ex_lru), ("learned", le_dec, le_idx, le_lru),
                                     ("static", st_dec, st_idx, st_lru)):
             sse, nb = cdef_luma_sse(full, dec, idx, act, base)
             bits = int(np.frombuffer(bytes(dec[:4]), np.int32)[0])
-            row[name] = {"cdef_psnr": psnr(sse, npx), "cdef_rate_bits": nact * bits + nb * 12,
-                         "lr_psnr": psnr(lr_luma_sse(lse, lru, base), npx)}
+            row[name] = {"cdef_psnr": psnr(sse, npx, bd), "cdef_rate_bits": nact * bits + nb * 12,
+                         "lr_psnr": psnr(lr_luma_sse(lse, lru, base), npx, bd)}
+        if bd == 8:
+            row["lr_exhaustive_mode_psnr"] = psnr(lr_exhaustive_mode_sse(runner, d, base), npx, bd)
         rows.append(row)
         if log:
             log(f"eval frame {fi}: q {q} exh {row['exhaustive']['cdef_psnr']:.3f} "
@@ -332,6 +429,11 @@ def evaluate(runner, net, frames, lam_for, static_cand, lr_thr=0.9, reps=3, log=
     out["topk_gain_kept"] = {name: {k: round(sum(r["topk"][name][k] for r in rows) / max(room, 1.0), 4)
                                     for k in (1, 2, 4, 8)} for name in rows[0]["topk"]}
     out["cdef_gain_db_exhaustive"] = float(np.mean([r["exhaustive"]["cdef_psnr"] - r["unfiltered_psnr"] for r in rows]))
+    if "lr_exhaustive_mode_psnr" in rows[0]:
+        out["lr_psnr_loss_vs_exhaustive_mode"] = float(np.mean([r["lr_exhaustive_mode_psnr"] - r["learned"]["lr_psnr"]
+                                                                for r in rows]))
+        out["lr_psnr_loss_curated_vs_exhaustive_mode"] = float(np.mean(
+            [r["lr_exhaustive_mode_psnr"] - r["exhaustive"]["lr_psnr"] for r in rows]))
     for name in ("learned", "static"):
         out[f"cdef_psnr_loss_{name}"] = float(np.mean([r["exhaustive"]["cdef_psnr"] - r[name]["cdef_psnr"] for r in rows]))
         out[f"lr_psnr_loss_{name}"] = float(np.mean([r["exhaustive"]["lr_psnr"] - r[name]["lr_psnr"] for r in rows]))
@@ -344,13 +446,30 @@ def torch_log(t):
     return t.clamp_min(1e-30).log()
 
 
-def held_out(n, w, h, seed=900000):
-    frames = []
-    for f in range(n):
-        q = int(np.random.default_rng(seed + 11 * f).integers(60, 221))
-        src, rec, noskip, damping = frame_pair(seed + 11 * f, w, h, q)
-        frames.append((src, rec, noskip, damping, q))
-    return frames
+def held_out(n, w, h, bd=8, seed=900000):
+    return list(pairs(held_out_specs(n, w, h, bd, seed)))
+
+
+def cell_summary(res, w, h, bd):
+    """The reported figures of one cell's evaluation."""
+    px = w * h
+    out = {"frames": res["frames"], "width": w, "height": h, "bit_depth": bd,
+           "exhaustive_mpx_s": round(px / res["ms_exhaustive"] / 1e3, 1),
+           "learned_mpx_s": round(px / res["ms_learned"] / 1e3, 1),
+           "static_shortlist_mpx_s": round(px / res["ms_static"] / 1e3, 1),
+           "ms_exhaustive": round(res["ms_exhaustive"], 4), "ms_learned": round(res["ms_learned"], 4),
+           "speedup": round(res["speedup_learned"], 2),
+           "cdef_psnr_loss_db": round(res["cdef_psnr_loss_learned"], 5),
+           "lr_psnr_loss_db": round(res["lr_psnr_loss_learned"], 5),
+           "cdef_rate_delta_bits": round(res["cdef_rate_delta_learned"], 2),
+           "static_cdef_psnr_loss_db": round(res["cdef_psnr_loss_static"], 5),
+           "static_lr_psnr_loss_db": round(res["lr_psnr_loss_static"], 5),
+           "cdef_gain_db_exhaustive": round(res["cdef_gain_db_exhaustive"], 4),
+           "luma_block_gain_kept": res["luma_block_gain_kept"], "topk_gain_kept": res["topk_gain_kept"]}
+    if "lr_psnr_loss_vs_exhaustive_mode" in res:
+        out["lr_psnr_loss_vs_exhaustive_lr_mode_db"] = round(res["lr_psnr_loss_vs_exhaustive_mode"], 5)
+        out["curated_lr_psnr_loss_vs_exhaustive_lr_mode_db"] = round(res["lr_psnr_loss_curated_vs_exhaustive_mode"], 5)
+    return out
 
 
 def main(argv=None):
@@ -358,44 +477,58 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--dc-table", default=None, help="JSON with the 8-bit DC quantizer per qindex ('dc')")
-    ap.add_argument("--eval-frames", type=int, default=20)
-    ap.add_argument("--eval-width", type=int, default=1920)
-    ap.add_argument("--eval-height", type=int, default=1080)
+    ap.add_argument("--eval-frames", type=int, default=50, help="held-out frames per cell")
+    ap.add_argument("--eval-out", default=None, help="JSON file for the per-cell evaluation")
     ap.add_argument("--frames", type=int, default=300)
-    ap.add_argument("--width", type=int, default=1280)
-    ap.add_argument("--height", type=int, default=720)
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--temp", type=float, default=TEMP, help="label temperature (relative to the block's best SSE)")
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="processes making the frames (before the GPU is used)")
     args = ap.parse_args(argv)
-    from .lib import PostFilterRunner, ProposalNet
 
     def log(m):
         print(m, flush=True)
 
+    # the frames are made by a process pool that forks before this process
+    # touches the GPU, and consumed as they arrive
+    if args.cmd == "eval":
+        cells = [(w, h, bd, held_out_specs(args.eval_frames, w, h, bd)) for (w, h, bd) in CELLS]
+        stream = pairs([sp for c in cells for sp in c[3]], args.workers)
+    else:
+        stream = pairs(train_specs(args.frames, args.seed), args.workers)
+    from .lib import PostFilterRunner, ProposalNet
     runner = PostFilterRunner()
     net = ProposalNet(runner, max_batch=1024 if args.cmd == "eval" else 256)
+    dc = json.load(open(args.dc_table))["dc"] if args.dc_table else None
     if args.cmd == "eval":
-        from .lib import cdef_lambda
-        dc = json.load(open(args.dc_table))["dc"] if args.dc_table else None
         meta = json.load(open(os.path.splitext(args.weights)[0] + ".json"))
         net.set_params(np.load(args.weights))
-        frames = held_out(args.eval_frames, args.eval_width, args.eval_height)
         rank = None
         if "label_counts" in meta:
             rank = np.argsort(-np.asarray(meta["label_counts"][0]), kind="stable")
-        res = evaluate(runner, net, frames, lambda q: cdef_lambda(dc[q] if dc else 223),
-                       np.array(meta["static_shortlist"], np.int32), log=log, static_rank=rank)
-        log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
+        report = {"weights": os.path.basename(args.weights), "frames_per_cell": args.eval_frames, "cells": []}
+        # every frame is made before the first timed search, so the pool's
+        # processes do not compete with the timed launches for the host
+        made = list(stream)
+        log(f"{len(made)} held-out frames made")
+        for ci, (w, h, bd, specs) in enumerate(cells):
+            frames = made[ci * args.eval_frames:(ci + 1) * args.eval_frames]
+            res = evaluate(runner, net, frames, lambda q, b: lambda_for(dc, q, b),
+                           np.array(meta["static_shortlist"], np.int32), log=log, static_rank=rank)
+            summary = cell_summary(res, w, h, bd)
+            report["cells"].append(summary)
+            log(json.dumps(summary))
+        if args.eval_out:
+            json.dump(report, open(args.eval_out, "w"), indent=1)
         net.close()
         runner.close()
         return
     net.set_params(init_params(np.random.default_rng(args.seed)))
     t0 = time.time()
-    tiles, targets, counts = build_dataset(runner, net, args.frames, seed=args.seed, w=args.width, h=args.height,
-                                           log=log, temp=args.temp)
+    tiles, targets, counts = build_dataset(runner, net, stream, log=log, temp=args.temp)
     ent = float(-(targets * torch_log(targets)).sum(dim=1).mean())
     log(f"dataset {tiles.shape[0]} tiles in {time.time() - t0:.1f} s; target entropy {ent:.4f} (the loss floor)")
     t0 = time.time()
@@ -404,8 +537,8 @@ def main(argv=None):
     P = net.get_params()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, P)
-    meta = {"frames": args.frames, "size": [args.width, args.height], "epochs": args.epochs, "loss": hist,
-            "temp": args.temp, "lr": args.lr,
+    meta = {"frames": args.frames, "cells": [list(c) for c in CELLS], "epochs": args.epochs, "loss": hist,
+            "tiles": int(tiles.shape[0]), "target_entropy": ent, "temp": args.temp, "lr": args.lr,
             "static_shortlist": static_shortlist(counts).tolist(), "label_counts": counts.tolist(), "seed": args.seed}
     json.dump(meta, open(os.path.splitext(args.out)[0] + ".json", "w"))
     log(json.dumps({k: v for k, v in meta.items() if k not in ("loss", "label_counts")}))

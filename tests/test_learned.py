@@ -20,9 +20,10 @@ import pytest
 
 
 @pytest.mark.gpu
-def test_trained_network_beats_static_shortlist():
+@pytest.mark.parametrize("bd", [8, 10])
+def test_trained_network_beats_static_shortlist(bd):
     from goav1_amd import learned
-    from goav1_amd.lib import PostFilterRunner, ProposalNet, cdef_lambda
+    from goav1_amd.lib import PostFilterRunner, ProposalNet
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     dc = json.load(open(os.path.join(root, "tests", "golden", "dc_qlookup_8bit.json")))["dc"]
     meta = json.load(open(os.path.splitext(learned.DEFAULT_WEIGHTS)[0] + ".json"))
@@ -32,9 +33,10 @@ def test_trained_network_beats_static_shortlist():
     net = ProposalNet(runner, max_batch=64)
     try:
         net.set_params(np.load(learned.DEFAULT_WEIGHTS))
-        frames = learned.held_out(6, 1280, 720, seed=770000)
-        print("learned vs static shortlist on 6 held-out 1280x720 frames", flush=True)
-        res = learned.evaluate(runner, net, frames, lambda q: cdef_lambda(dc[q]), static, reps=1, static_rank=rank)
+        frames = learned.held_out(6, 1920, 1080, bd, seed=770000)
+        print(f"learned vs static shortlist on 6 held-out 1920x1080 {bd}-bit frames", flush=True)
+        res = learned.evaluate(runner, net, frames, lambda q, b: learned.lambda_for(dc, q, b), static, reps=1,
+                               static_rank=rank)
     finally:
         net.close()
         runner.close()

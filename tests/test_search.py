@@ -253,17 +253,26 @@ def test_gpu_select_random_tables_with_ties(case):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", [(1280, 720, 160, 0, 8), (200, 136, 100, 1, 8), (1920, 1080, 220, 2, 8),
                                   (2048, 1152, 120, 3, 8), (2304, 1856, 200, 4, 8), (1280, 720, 160, 5, 10),
-                                  (200, 136, 60, 6, 10), (2304, 1856, 200, 7, 10)],
+                                  (200, 136, 60, 6, 10), (2304, 1856, 200, 7, 10), (640, 360, 0, 8, 10),
+                                  (2304, 1856, 0, 9, 10)],
                          ids=lambda c: f"{c[0]}x{c[1]}_q{c[2]}_{c[4]}b")
 def test_gpu_shortlist_equals_masked_host_finish(case):
     """The shortlist search's table equals the full table's shortlisted
     columns, and its one-launch selection equals the host finish over the full
     table with every other strength priced out (shortlists ascending, with 0).
-    10-bit tables take the 64-bit selection."""
+    10-bit tables of realistic frames take the 32-bit selection, noise frames
+    (q 0 here) the 64-bit one."""
     import torch
     from goav1_amd.lib import DeviceFrame, PostFilterRunner, cdef_lambda, cdef_search_finish
     w, h, q, seed, bd = case
     src, rec, noskip, damping = synth.search_pair(seed=60 + seed, w=w, h=h, q=q, bd=bd)
+    if q == 0:
+        # independent uniform noise: block SSEs near 4096 * 1023^2 / 6, past the
+        # 32-bit selection's range, so the 64-bit selection decides
+        rng = np.random.default_rng(seed)
+        for fr in (src, rec):
+            for p in fr.planes:
+                p.pix[:] = rng.integers(0, 1 << bd, size=p.pix.shape).astype(np.uint16)
     rng = np.random.default_rng(seed)
     cand = np.concatenate([np.sort(np.r_[0, rng.choice(np.arange(1, 64), K - 1, replace=False)]) for _ in range(2)])
     print(f"shortlist {w}x{h} q{q} cand {cand.tolist()}", flush=True)

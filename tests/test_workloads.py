@@ -135,16 +135,17 @@ def test_mixed_batch_64_frames_tuned_and_graph_replays(runner):
         runner.set_graphs(False)
 
 
-@pytest.mark.parametrize("size", [(1920, 1080), (3840, 2160)], ids=["1080p", "4k"])
+@pytest.mark.parametrize("size", [(1920, 1080, 8), (3840, 2160, 8), (1920, 1080, 10), (3840, 2160, 10)],
+                         ids=["1080p8", "4k8", "1080p10", "4k10"])
 def test_learned_search_at_size(size):
     import torch
     from goav1_amd import learned
-    from goav1_amd.lib import PostFilterRunner, ProposalNet, cdef_lambda
-    w, h = size
+    from goav1_amd.lib import PostFilterRunner, ProposalNet
+    w, h, bd = size
     dc = json.load(open(os.path.join(ROOT, "tests", "golden", "dc_qlookup_8bit.json")))["dc"]
     q = 200
-    src, rec, noskip, damping = learned.frame_pair(31000 + w, w, h, q)
-    lam = cdef_lambda(dc[q])
+    src, rec, noskip, damping = learned.frame_pair(31000 + w, w, h, q, bd)
+    lam = learned.lambda_for(dc, q, bd)
     K = abi.SHORTLIST
     r = PostFilterRunner()
     net = ProposalNet(r, max_batch=64)
@@ -152,7 +153,7 @@ def test_learned_search_at_size(size):
         net.set_params(np.load(learned.DEFAULT_WEIGHTS))
         d = learned.Device(src, rec, noskip)
         assert d.nfb == ((w + 63) // 64) * ((h + 63) // 64)
-        print(f"learned search {w}x{h} ({d.nfb} filter blocks)", flush=True)
+        print(f"learned search {w}x{h} {bd}-bit ({d.nfb} filter blocks)", flush=True)
         learned.run_learned(r, net, d, damping, lam, 0.9)
         r.sync()
         torch.cuda.synchronize()
