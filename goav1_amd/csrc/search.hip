@@ -1006,7 +1006,6 @@ struct SmallLevel {
 struct SmallState {
     SmallLevel lv[4];
     unsigned int ticket;
-    unsigned int wide;     // the 32-bit selection found an SSE of 2^28 or more: the 64-bit one runs
 };
 static_assert(sizeof(SmallState) <= 1024, "small selection state");
 #define SMALL_MAX_FB 2048
@@ -1042,6 +1041,190 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     v = max(v, xchg64<8>(v));
     v = max(v, xchg64<16>(v));
     return max(v, xchg64<32>(v));
+}
+
+// The greedy selection over the shortlist pairs with 64-bit costs, for
+// tables whose SSEs need more than 32 bits (10-bit frames: a 64x64 luma
+// block's SSE reaches 4096 * 1023^2).  Same steps, the same first-maximum
+// and first-minimum rules, the same RD cost and ticket as fin_small_kernel,
+// which runs it instead of its own steps when a table needs it;
+// thread t keeps the active blocks t, t + SMALL_T, ... (NPER of them) with
+// their shortlist SSEs in registers and sums every pair's gain over them.
+template <int NPER>
+__device__ __noinline__ void fin_small_wide(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb,
+                                               int mi_cols, int mi_rows, unsigned long long lambda, const int *cand,
+                                               SmallState *ss, av1pf_cdef_decision *dec, uint8_t *fb_index)
+{
+    constexpr int NW = SMALL_T / 64;
+    constexpr int NP = NCAND * NCAND;
+    __shared__ uint16_t act[SMALL_MAX_FB];
+    __shared__ uint8_t flag[SMALL_MAX_FB];
+    __shared__ unsigned long long wpart[NW][NP + 1];   // per-wave pair gains (or column sums) and base sum
+    __shared__ int pick_s, nact_s, bits_s;
+    __shared__ unsigned sel_s[2];
+    __shared__ bool last;
+    __shared__ unsigned long long dist_s;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nfb = nhfb * nvfb;
+    for (int fb = t; fb < nfb; fb += SMALL_T) flag[fb] = fb_active(noskip, fb, nhfb, mi_cols, mi_rows) ? 1 : 0;
+    __syncthreads();
+    if (t < 64) {
+        int n = 0;
+        for (int base = 0; base < nfb; base += 64) {
+            const bool a = base + lane < nfb && flag[base + lane];
+            const uint64_t vote = __ballot(a);
+            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = (uint16_t)(base + lane);
+            n += __popcll(vote);
+        }
+        if (lane == 0) nact_s = n;
+    }
+    __syncthreads();
+    const int nact = nact_s;
+    // blocks past nact hold zeros: best cost 0, no gain, nothing in a column sum
+    unsigned long long Y[NPER][NCAND], U[NPER][NCAND];
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        const size_t fb = i < nact ? act[i] : 0;
+#pragma unroll
+        for (int q = 0; q < NCAND; q++) {
+            Y[r][q] = i < nact ? mse[fb * NCAND + q] : 0;
+            U[r][q] = i < nact ? mse[((size_t)nfb + fb) * NCAND + q] : 0;
+        }
+    }
+    const int bits = blockIdx.x;
+    const int nb = 1 << bits;
+    unsigned sy = 0, su = 0;
+    for (int step = 0; step < 5 * nb; step++) {
+        const int slot = step < nb ? step : nb - 1;
+        const bool final_step = step == 5 * nb - 1;
+        if (slot == 0) {
+#pragma unroll
+            for (int q = 0; q < 2 * NCAND; q++) {
+                unsigned long long v = 0;
+#pragma unroll
+                for (int r = 0; r < NPER; r++) v += q < NCAND ? Y[r][q] : U[r][q - NCAND];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) wpart[wv][q] = v;
+            }
+            __syncthreads();
+            if (t == 0) {
+                unsigned long long cs[2 * NCAND];
+                for (int q = 0; q < 2 * NCAND; q++) {
+                    cs[q] = 0;
+                    for (int w = 0; w < NW; w++) cs[q] += wpart[w][q];
+                }
+                int by = 0, bc = 0;
+                for (int q = 1; q < NCAND; q++) {
+                    if (cs[q] < cs[by]) by = q;
+                    if (cs[NCAND + q] < cs[NCAND + bc]) bc = q;
+                }
+                pick_s = by * NCAND + bc;
+                dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
+            }
+        } else {
+            unsigned long long b[NPER], bsum = 0;
+#pragma unroll
+            for (int r = 0; r < NPER; r++) {
+                b[r] = Y[r][sy & 15] + U[r][su & 15];
+                for (int g = 1; g < slot; g++)
+                    b[r] = min(b[r], Y[r][(sy >> (4 * g)) & 15] + U[r][(su >> (4 * g)) & 15]);
+                bsum += b[r];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                unsigned long long g = 0;
+#pragma unroll
+                for (int r = 0; r < NPER; r++) {
+                    const unsigned long long c = Y[r][p / NCAND] + U[r][p % NCAND];
+                    g += b[r] > c ? b[r] - c : 0;
+                }
+                for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+                if (lane == 0) wpart[wv][p] = g;
+            }
+            if (final_step) {
+                for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
+                if (lane == 0) wpart[wv][NP] = bsum;
+            }
+            __syncthreads();
+            if (t == 0) {   // first maximum over the pairs
+                unsigned long long bestg = 0;
+                int bp = 0;
+                for (int p = 0; p < NP; p++) {
+                    unsigned long long g = 0;
+                    for (int w = 0; w < NW; w++) g += wpart[w][p];
+                    if (p == 0 || g > bestg) { bestg = g; bp = p; }
+                }
+                pick_s = bp;
+                if (final_step) {
+                    unsigned long long base = 0;
+                    for (int w = 0; w < NW; w++) base += wpart[w][NP];
+                    dist_s = base - bestg;
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
+        sy = (sy & ~(15u << (4 * slot))) | ((pk / NCAND) << (4 * slot));
+        su = (su & ~(15u << (4 * slot))) | ((pk % NCAND) << (4 * slot));
+        if (final_step) {
+            if (t == 0) {
+                const long long rate = (long long)(nact * bits + nb * 12) << 9;
+                SmallLevel &L = ss->lv[bits];
+                L.cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((dist_s * 16) << 7);
+                for (int i = 0; i < 8; i++) {
+                    L.ys[i] = i < nb ? (int)((sy >> (4 * i)) & 15) : 0;
+                    L.us[i] = i < nb ? (int)((su >> (4 * i)) & 15) : 0;
+                }
+            }
+        } else if (step + 1 >= nb) {
+            sy >>= 4;
+            su >>= 4;
+        }
+        __syncthreads();   // pick_s and the partials are rewritten by the next step
+    }
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&ss->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    if (t == 0) {
+        __threadfence();
+        volatile SmallLevel *V = ss->lv;
+        int best_bits = 0;
+        unsigned long long best_cost = V[0].cost;
+        for (int b = 1; b < 4; b++)
+            if (V[b].cost < best_cost) { best_cost = V[b].cost; best_bits = b; }
+        const int n = 1 << best_bits;
+        unsigned a = 0, c = 0;
+        for (int i = 0; i < n; i++) { a |= (unsigned)V[best_bits].ys[i] << (4 * i); c |= (unsigned)V[best_bits].us[i] << (4 * i); }
+        ss->ticket = 0;
+        dec->cdef_bits = best_bits;
+        for (int i = 0; i < 8; i++) {
+            dec->y_strength[i] = i < n ? kLevel1[cand[V[best_bits].ys[i]]] : 0;
+            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + V[best_bits].us[i]]] : 0;
+        }
+        bits_s = best_bits;
+        sel_s[0] = a;
+        sel_s[1] = c;
+    }
+    for (int fb = t; fb < nfb; fb += SMALL_T) fb_index[fb] = 0;
+    __syncthreads();
+    const int n = 1 << bits_s;
+    const unsigned fy = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[0]), fu = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[1]);
+#pragma unroll
+    for (int r = 0; r < NPER; r++) {
+        const int i = t + r * SMALL_T;
+        if (i >= nact) continue;
+        unsigned long long bm = Y[r][fy & 15] + U[r][fu & 15];
+        int bg = 0;
+        for (int g = 1; g < n; g++) {
+            const unsigned long long c = Y[r][(fy >> (4 * g)) & 15] + U[r][(fu >> (4 * g)) & 15];
+            if (c < bm) { bm = c; bg = g; }
+        }
+        fb_index[act[i]] = (uint8_t)bg;
+    }
 }
 
 template <int NPER, bool WIDE_CHECK>
@@ -1092,8 +1275,7 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     }
     // the 32-bit arithmetic below is exact when every SSE is below 2^28 (a
     // pair's cost below 2^29, eight gains below 2^32): always for 8-bit
-    // frames; a 10-bit table with a larger value is left to the 64-bit
-    // selection (fin_small_wide_kernel, launched after this one)
+    // frames; a 10-bit table with a larger value takes the 64-bit steps
     if (WIDE_CHECK) {
         bool big = false;
 #pragma unroll
@@ -1105,7 +1287,7 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
                 big |= i < nact && ((mse[fb * NCAND + q] | mse[((size_t)nfb + fb) * NCAND + q]) >> 28) != 0;
         }
         if (__syncthreads_or(big)) {
-            if (t == 0 && blockIdx.x == 0) ss->wide = 1;
+            fin_small_wide<NPER>(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, cand, ss, dec, fb_index);
             return;
         }
     }
@@ -1275,194 +1457,6 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     }
 }
 
-// The same greedy selection over the shortlist pairs with 64-bit costs, for
-// tables whose SSEs need more than 32 bits (10-bit frames: a 64x64 luma
-// block's SSE reaches 4096 * 1023^2).  Same steps, the same first-maximum
-// and first-minimum rules, the same RD cost and ticket as fin_small_kernel;
-// thread t keeps the active blocks t, t + SMALL_T, ... (NPER of them) with
-// their shortlist SSEs in registers and sums every pair's gain over them.
-template <int NPER>
-__global__ __launch_bounds__(SMALL_T) void fin_small_wide_kernel(const uint64_t *mse, const uint64_t *noskip, int nhfb,
-                                                                 int nvfb, int mi_cols, int mi_rows,
-                                                                 unsigned long long lambda, const int *cand,
-                                                                 SmallState *ss, av1pf_cdef_decision *dec,
-                                                                 uint8_t *fb_index)
-{
-    constexpr int NW = SMALL_T / 64;
-    constexpr int NP = NCAND * NCAND;
-    // runs only when the 32-bit selection before it gave up on this table
-    if (!*reinterpret_cast<volatile unsigned *>(&ss->wide)) return;
-    __shared__ uint16_t act[SMALL_MAX_FB];
-    __shared__ uint8_t flag[SMALL_MAX_FB];
-    __shared__ unsigned long long wpart[NW][NP + 1];   // per-wave pair gains (or column sums) and base sum
-    __shared__ int pick_s, nact_s, bits_s;
-    __shared__ unsigned sel_s[2];
-    __shared__ bool last;
-    __shared__ unsigned long long dist_s;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int nfb = nhfb * nvfb;
-    for (int fb = t; fb < nfb; fb += SMALL_T) flag[fb] = fb_active(noskip, fb, nhfb, mi_cols, mi_rows) ? 1 : 0;
-    __syncthreads();
-    if (t < 64) {
-        int n = 0;
-        for (int base = 0; base < nfb; base += 64) {
-            const bool a = base + lane < nfb && flag[base + lane];
-            const uint64_t vote = __ballot(a);
-            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = (uint16_t)(base + lane);
-            n += __popcll(vote);
-        }
-        if (lane == 0) nact_s = n;
-    }
-    __syncthreads();
-    const int nact = nact_s;
-    // blocks past nact hold zeros: best cost 0, no gain, nothing in a column sum
-    unsigned long long Y[NPER][NCAND], U[NPER][NCAND];
-#pragma unroll
-    for (int r = 0; r < NPER; r++) {
-        const int i = t + r * SMALL_T;
-        const size_t fb = i < nact ? act[i] : 0;
-#pragma unroll
-        for (int q = 0; q < NCAND; q++) {
-            Y[r][q] = i < nact ? mse[fb * NCAND + q] : 0;
-            U[r][q] = i < nact ? mse[((size_t)nfb + fb) * NCAND + q] : 0;
-        }
-    }
-    const int bits = blockIdx.x;
-    const int nb = 1 << bits;
-    unsigned sy = 0, su = 0;
-    for (int step = 0; step < 5 * nb; step++) {
-        const int slot = step < nb ? step : nb - 1;
-        const bool final_step = step == 5 * nb - 1;
-        if (slot == 0) {
-#pragma unroll
-            for (int q = 0; q < 2 * NCAND; q++) {
-                unsigned long long v = 0;
-#pragma unroll
-                for (int r = 0; r < NPER; r++) v += q < NCAND ? Y[r][q] : U[r][q - NCAND];
-                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (lane == 0) wpart[wv][q] = v;
-            }
-            __syncthreads();
-            if (t == 0) {
-                unsigned long long cs[2 * NCAND];
-                for (int q = 0; q < 2 * NCAND; q++) {
-                    cs[q] = 0;
-                    for (int w = 0; w < NW; w++) cs[q] += wpart[w][q];
-                }
-                int by = 0, bc = 0;
-                for (int q = 1; q < NCAND; q++) {
-                    if (cs[q] < cs[by]) by = q;
-                    if (cs[NCAND + q] < cs[NCAND + bc]) bc = q;
-                }
-                pick_s = by * NCAND + bc;
-                dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
-            }
-        } else {
-            unsigned long long b[NPER], bsum = 0;
-#pragma unroll
-            for (int r = 0; r < NPER; r++) {
-                b[r] = Y[r][sy & 15] + U[r][su & 15];
-                for (int g = 1; g < slot; g++)
-                    b[r] = min(b[r], Y[r][(sy >> (4 * g)) & 15] + U[r][(su >> (4 * g)) & 15]);
-                bsum += b[r];
-            }
-#pragma unroll
-            for (int p = 0; p < NP; p++) {
-                unsigned long long g = 0;
-#pragma unroll
-                for (int r = 0; r < NPER; r++) {
-                    const unsigned long long c = Y[r][p / NCAND] + U[r][p % NCAND];
-                    g += b[r] > c ? b[r] - c : 0;
-                }
-                for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
-                if (lane == 0) wpart[wv][p] = g;
-            }
-            if (final_step) {
-                for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
-                if (lane == 0) wpart[wv][NP] = bsum;
-            }
-            __syncthreads();
-            if (t == 0) {   // first maximum over the pairs
-                unsigned long long bestg = 0;
-                int bp = 0;
-                for (int p = 0; p < NP; p++) {
-                    unsigned long long g = 0;
-                    for (int w = 0; w < NW; w++) g += wpart[w][p];
-                    if (p == 0 || g > bestg) { bestg = g; bp = p; }
-                }
-                pick_s = bp;
-                if (final_step) {
-                    unsigned long long base = 0;
-                    for (int w = 0; w < NW; w++) base += wpart[w][NP];
-                    dist_s = base - bestg;
-                }
-            }
-        }
-        __syncthreads();
-        const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
-        sy = (sy & ~(15u << (4 * slot))) | ((pk / NCAND) << (4 * slot));
-        su = (su & ~(15u << (4 * slot))) | ((pk % NCAND) << (4 * slot));
-        if (final_step) {
-            if (t == 0) {
-                const long long rate = (long long)(nact * bits + nb * 12) << 9;
-                SmallLevel &L = ss->lv[bits];
-                L.cost = (unsigned long long)((rate * (long long)lambda + (1 << 8)) >> 9) + ((dist_s * 16) << 7);
-                for (int i = 0; i < 8; i++) {
-                    L.ys[i] = i < nb ? (int)((sy >> (4 * i)) & 15) : 0;
-                    L.us[i] = i < nb ? (int)((su >> (4 * i)) & 15) : 0;
-                }
-            }
-        } else if (step + 1 >= nb) {
-            sy >>= 4;
-            su >>= 4;
-        }
-        __syncthreads();   // pick_s and the partials are rewritten by the next step
-    }
-    if (t == 0) {
-        __threadfence();
-        last = atomicAdd(&ss->ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    if (t == 0) {
-        __threadfence();
-        volatile SmallLevel *V = ss->lv;
-        int best_bits = 0;
-        unsigned long long best_cost = V[0].cost;
-        for (int b = 1; b < 4; b++)
-            if (V[b].cost < best_cost) { best_cost = V[b].cost; best_bits = b; }
-        const int n = 1 << best_bits;
-        unsigned a = 0, c = 0;
-        for (int i = 0; i < n; i++) { a |= (unsigned)V[best_bits].ys[i] << (4 * i); c |= (unsigned)V[best_bits].us[i] << (4 * i); }
-        ss->ticket = 0;
-        ss->wide = 0;
-        dec->cdef_bits = best_bits;
-        for (int i = 0; i < 8; i++) {
-            dec->y_strength[i] = i < n ? kLevel1[cand[V[best_bits].ys[i]]] : 0;
-            dec->uv_strength[i] = i < n ? kLevel1[cand[NCAND + V[best_bits].us[i]]] : 0;
-        }
-        bits_s = best_bits;
-        sel_s[0] = a;
-        sel_s[1] = c;
-    }
-    for (int fb = t; fb < nfb; fb += SMALL_T) fb_index[fb] = 0;
-    __syncthreads();
-    const int n = 1 << bits_s;
-    const unsigned fy = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[0]), fu = (unsigned)__builtin_amdgcn_readfirstlane(sel_s[1]);
-#pragma unroll
-    for (int r = 0; r < NPER; r++) {
-        const int i = t + r * SMALL_T;
-        if (i >= nact) continue;
-        unsigned long long bm = Y[r][fy & 15] + U[r][fu & 15];
-        int bg = 0;
-        for (int g = 1; g < n; g++) {
-            const unsigned long long c = Y[r][(fy >> (4 * g)) & 15] + U[r][(fu >> (4 * g)) & 15];
-            if (c < bm) { bm = c; bg = g; }
-        }
-        fb_index[act[i]] = (uint8_t)bg;
-    }
-}
-
 hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
                                    uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
                                    uint8_t *fb_index, void *small_state, hipStream_t s)
@@ -1484,15 +1478,11 @@ hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, c
         hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam, cand,
                            ss, dec, fb_index);
     } else {
-        // 10-bit: the 32-bit selection when every SSE allows it, else the 64-bit one
+        // 10-bit: the 32-bit steps when every SSE allows them, else the 64-bit ones
         auto fin = nfb <= SMALL_T ? fin_small_kernel<1, true> : nfb <= 2 * SMALL_T ? fin_small_kernel<2, true>
                                                                                    : fin_small_kernel<4, true>;
-        auto wide = nfb <= SMALL_T ? fin_small_wide_kernel<1> : nfb <= 2 * SMALL_T ? fin_small_wide_kernel<2>
-                                                                                    : fin_small_wide_kernel<4>;
         hipLaunchKernelGGL(fin, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam, cand,
                            ss, dec, fb_index);
-        hipLaunchKernelGGL(wide, dim3(4), dim3(SMALL_T), 0, s, mse_sub, noskip, nhfb, nvfb, mi_cols, mi_rows, lam,
-                           cand, ss, dec, fb_index);
     }
     return hipGetLastError();
 }
