@@ -320,193 +320,349 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
 }
 
 // ---------------------------------------------------------------------------
+// Wave exchanges and reductions (DPP and swizzle, no LDS round trip)
+template <int X>
+__device__ __forceinline__ uint32_t xchg32(uint32_t v)
+{
+    if constexpr (X == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false);
+    else if constexpr (X == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false);
+    else if constexpr (X == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
+    else if constexpr (X == 8) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
+    else if constexpr (X == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+    else return (uint32_t)__shfl_xor((int)v, 32, 64);
+}
+template <int X>
+__device__ __forceinline__ uint64_t xchg64(uint64_t v)
+{
+    return (uint64_t)xchg32<X>((uint32_t)v) | ((uint64_t)xchg32<X>((uint32_t)(v >> 32)) << 32);
+}
+// Largest 64-bit value over the wave (every lane ends with it).
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+    v = max(v, xchg64<1>(v));
+    v = max(v, xchg64<2>(v));
+    v = max(v, xchg64<4>(v));
+    v = max(v, xchg64<8>(v));
+    v = max(v, xchg64<16>(v));
+    return max(v, xchg64<32>(v));
+}
+
+// ---------------------------------------------------------------------------
 // Strength selection on the device in one launch (finish /
 // jointStrengthSearchDual / searchOneDual, cdef_search.go:780-945).
 //
 // The four cdef_bits levels are independent searches, so they run
 // concurrently: level 0 on workgroup 0, level L >= 1 on SEL_G workgroups,
-// one per luma strength j.  A level's step sequence is fixed (nb =
-// 1 << L greedy steps filling slots 0..nb-1, then 4 nb refinements that drop
-// the oldest pair and fill the last slot again); only the chosen pairs depend
-// on the data.
+// one per luma strength j.  A level's step sequence is fixed (nb = 1 << L
+// greedy steps filling slots 0..nb-1, then 4 nb refinements that drop the
+// oldest pair and fill the last slot again); only the chosen pairs depend on
+// the data.
 //  - A step with no prior pair (slot 0) chooses luma and chroma separately
 //    (first minimum of the column sums over the active blocks); every
 //    workgroup of the level computes it, so it needs no exchange.
-//  - Any other step: every workgroup computes the active blocks' best cost
-//    over the prior pairs (best_i); a wave sums, for the pairs (j, k = lane),
-//    the savings max(0, best_i - a_ij - b_ik) over an eighth of the blocks
-//    (a block with a_ij >= best_i saves nothing for any k, as the reference
-//    skips it; with the rows in LDS and every best_i below 2^32 the savings
-//    are saturating 32-bit differences, summed in 64 bits); the workgroup's best pair
-//    becomes a key (gain << 12 | 4095 - (64 j + k): the largest gain, then the
-//    first pair in (j, k) order) that it publishes in its own slot of the
-//    step, and every workgroup reads the level's SEL_G slots and keeps the
-//    largest key, so all of them hold the same lists.
+//  - Any other step: every workgroup holds the active blocks' best cost over
+//    the prior pairs (best_i).  Wave w takes a contiguous eighth of the
+//    blocks; a lane per block forms the room max(0, best_i - a_ij) (a block
+//    with no room saves nothing for any k, as the reference skips it), and
+//    the lanes k sum the savings max(0, room_i - b_ik) over the blocks (with
+//    every value below 2^32 as saturating 32-bit differences, summed in 64
+//    bits).  The workgroup's best pair becomes a
+//    key (gain << 12 | 4095 - (64 j + k): the largest gain, then the first
+//    pair in (j, k) order) that it publishes in its own slot of the step, and
+//    every workgroup reads the level's SEL_G slots and keeps the largest key,
+//    so all of them hold the same lists.
+//  - Each chosen pair's cost column (a_ij + b_ik of every block) stays in LDS
+//    when it fits, so the best cost over the pairs the next refinement keeps
+//    is a minimum over LDS columns, computed while the exchange is in flight.
 //  - The last level to finish (a ticket) takes the cheapest level by the RD
 //    cost and writes the decision and each block's index.
-// The MSE rows of the active blocks are staged in LDS as 32-bit values when
-// they fit (TABLES = 2: luma and chroma, 1: chroma; values of 2^32 or more
-// switch the workgroup -- every workgroup alike -- to reading device memory).
-// The slots are written and read only by device-scope atomic operations,
-// which are performed where every XCD sees them, so the exchange needs no
-// cache write-back or invalidation; the keys are the only data exchanged.
-// The slots alternate between two tables by launch parity, each launch
-// clearing the table the next one uses; the ticket counter only grows (4
-// arrivals per launch).  A slot that stays empty past a bounded spin marks
-// the launch aborted (every workgroup then leaves) and sets bit 1 of the
-// error flag.
+// The chroma rows of the active blocks are read as 32-bit values: from LDS
+// when they fit there (TABLES >= 1; TABLES = 2 also holds the luma rows),
+// else from a 32-bit copy in the state buffer that the workgroups write at
+// the start, behind a grid-wide barrier.  A value of 2^32 or more switches a
+// workgroup to the 64-bit rows in device memory.
+// The slots are written by device-scope atomic stores and polled by
+// device-scope atomic loads, which are performed where every XCD sees them,
+// so the exchange needs no cache write-back or invalidation; the keys are the
+// only data exchanged.  Each member's slots sit in their own 4 KiB page, so a
+// level's polls spread over the memory channels instead of queueing on a few
+// cache lines.  The slots alternate between two halves by launch parity, each
+// launch clearing the half the next one uses; the ticket and barrier
+// counters only grow.  A wait that runs past a bounded spin marks the launch
+// aborted (every workgroup then leaves) and sets bit 1 of the error flag.
 constexpr int SEL_G = 64;                  // workgroups per level >= 1 (one per luma strength)
 static_assert(SEL_G == 64, "one wave reads a level's slots, one lane each");
 constexpr int SEL_T = 512;                 // threads per workgroup (8 waves)
+constexpr int SEL_W = SEL_T / 64;
 constexpr int SEL_GRID = 1 + 3 * SEL_G;    // level 0 on workgroup 0
 constexpr int SEL_STEPS = 39;              // exchanged steps of level 3 (8 greedy + 32 refinements, less the first)
-constexpr int SEL_T2_MAX = 280;            // filter blocks for both tables in LDS
-constexpr int SEL_T1_MAX = 560;            // filter blocks for the chroma table in LDS
+constexpr int SEL_HALF = 3 * SEL_STEPS;    // slots of one launch parity in a member's page
+constexpr int SEL_PAGE = 512;              // 64-bit words per member page (4 KiB)
+static_assert(2 * SEL_HALF <= SEL_PAGE, "a member's slots fit its page");
 constexpr int SEL_TP = 65;                 // LDS row pitch of the tables (words): lanes reading one
                                            // column of consecutive rows hit distinct banks
+constexpr size_t SEL_LDS_BUDGET = 160 * 1024 - 6144;   // dynamic LDS; the kernel's own variables take < 6 KiB
 
 struct FinishState {
+    unsigned long long slot[SEL_G][SEL_PAGE];   // published keys, bit 63 = present
+    uint32_t c32[4096 * 64];               // 32-bit chroma rows of the active blocks (TABLES == 0)
     unsigned int done;                     // levels finished (monotonic, 4 per launch)
+    unsigned int conv;                     // workgroups past the chroma copy (monotonic, SEL_GRID per launch
+                                           // of any table variant)
+    unsigned int c32bad;                   // epoch whose copy holds a value of 2^32 or more
     unsigned int abort;                    // epoch of an aborted launch
-    unsigned long long slot[2][3][SEL_STEPS][SEL_G];   // published keys, bit 63 = present
     int ly[4][8], lc[4][8];                // each level's final pairs
     unsigned long long cost[4];
 };
 
-// LDS: active block list, each block's best cost, the workgroup's own luma
-// column (when the luma table is not in LDS), then the chroma and luma tables.
-__host__ __device__ inline size_t sel_lds_bytes(int nfb, int tables)
+// LDS: active block list, each block's best cost, the member's luma column,
+// the chosen pairs' cost columns (when they fit), then the chroma and luma
+// tables.
+struct SelLayout {
+    size_t act, best, lcol, cc, ct, lt, total;
+};
+
+__host__ __device__ inline SelLayout sel_layout(int nfb, int tables, bool cc)
 {
-    const size_t base = ((size_t)nfb * 4 + 15) / 16 * 16 + (size_t)nfb * 8 + (tables < 2 ? (size_t)nfb * 8 : 0);
-    return base + (size_t)nfb * SEL_TP * 4 * (size_t)tables;
+    SelLayout L;
+    size_t o = 0;
+    L.act = o;
+    o += ((size_t)nfb * 4 + 15) / 16 * 16;
+    L.best = o;
+    o += (size_t)nfb * 8;
+    L.lcol = o;
+    o += ((size_t)nfb * 4 + 15) / 16 * 16;
+    L.cc = o;
+    o += cc ? ((size_t)8 * nfb * 4 + 15) / 16 * 16 : 0;
+    L.ct = o;
+    o += tables >= 1 ? (size_t)nfb * SEL_TP * 4 : 0;
+    L.lt = o;
+    o += tables == 2 ? (size_t)nfb * SEL_TP * 4 : 0;
+    L.total = o;
+    return L;
+}
+
+// bounded wait on a monotonic counter; false when the launch was aborted
+__device__ inline bool sel_wait_count(unsigned int *cnt, unsigned int target, FinishState *st, unsigned int epoch,
+                                      int *err)
+{
+    for (int spin = 0;; spin++) {
+        if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch || spin > (1 << 21)) {
+            atomicExch(&st->abort, epoch);
+            atomicOr(err, 2);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 template <int TABLES>
 __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__restrict__ mse, const uint64_t *noskip,
                                                             int nhfb, int nvfb, int mi_cols, int mi_rows,
                                                             unsigned long long lambda, FinishState *st,
-                                                            unsigned int epoch, av1pf_cdef_decision *dec,
+                                                            unsigned int epoch, int use_cc, av1pf_cdef_decision *dec,
                                                             uint8_t *fb_index, int *err)
 {
     extern __shared__ __align__(16) char smem[];
-    __shared__ unsigned long long part[8][64];
-    __shared__ unsigned long long sums[128];
-    __shared__ unsigned long long s_key;
-    __shared__ int ly[8], lc[8];
+    __shared__ unsigned long long part[SEL_W][64];
+    __shared__ unsigned long long amask[64];
+    __shared__ int aoff[64];
+    __shared__ unsigned long long s_key, s_dist, s_min[2];
+    __shared__ int ly[8], lc[8], rp[8];
     __shared__ int s_nact, s_bad, s_abort, s_wide;
-    __shared__ unsigned long long s_dist, s_sum;
     __shared__ bool s_last;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfb = nhfb * nvfb;
     const int b = blockIdx.x;
     const int level = b == 0 ? 0 : 1 + (b - 1) / SEL_G;
-    const int gw = b == 0 ? 0 : (b - 1) % SEL_G;
-    int *act = reinterpret_cast<int *>(smem);
-    unsigned long long *best = reinterpret_cast<unsigned long long *>(smem + ((size_t)nfb * 4 + 15) / 16 * 16);
-    unsigned long long *lcol = best + nfb;                      // a_ij of this workgroup's j (TABLES < 2)
-    uint32_t *ct = reinterpret_cast<uint32_t *>(best + nfb + (TABLES < 2 ? nfb : 0));   // chroma rows
-    uint32_t *lt = ct + (size_t)nfb * SEL_TP;                   // luma rows (TABLES == 2)
+    const int j = b == 0 ? 0 : (b - 1) % SEL_G;
+    const bool cc = use_cc != 0;
+    const SelLayout Lo = sel_layout(nfb, TABLES, cc);
+    int *act = reinterpret_cast<int *>(smem + Lo.act);
+    unsigned long long *best = reinterpret_cast<unsigned long long *>(smem + Lo.best);
+    uint32_t *lcol = reinterpret_cast<uint32_t *>(smem + Lo.lcol);   // a_ij, 32-bit
+    uint32_t *costc = reinterpret_cast<uint32_t *>(smem + Lo.cc);    // [8][nfb], saturated at 2^32 - 1
+    uint32_t *ct = reinterpret_cast<uint32_t *>(smem + Lo.ct);       // chroma rows
+    uint32_t *lt = reinterpret_cast<uint32_t *>(smem + Lo.lt);       // luma rows (TABLES == 2)
     const uint64_t *Lm = mse, *Cm = mse + (size_t)nfb * 64;
-    // ---- active blocks (a non-skip 8x8 block inside the frame), ascending
-    if (t < 64) {
-        int n = 0;
-        for (int base = 0; base < nfb; base += 64) {
-            const int fb = base + lane;
-            bool a = false;
-            if (fb < nfb) {
-                const int fbr = fb / nhfb, fbc = fb % nhfb;
-                const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
-                uint64_t valid = 0;
-                for (int r = 0; r < nvb; r += 2)
-                    for (int c = 0; c < nhb; c += 2) valid |= 1ull << ((r >> 1) * 8 + (c >> 1));
-                a = ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
-            }
-            const uint64_t vote = __ballot(a);
-            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = fb;
-            n += __popcll(vote);
+    // ---- active blocks (a non-skip 8x8 block inside the frame), ascending:
+    // a ballot per 64 blocks, the chunks' offsets by one wave's scan, then
+    // every block writes its own entry
+    for (int c0 = wv; c0 * 64 < nfb; c0 += SEL_W) {
+        const int fb = c0 * 64 + lane;
+        bool a = false;
+        if (fb < nfb) {
+            const int fbr = fb / nhfb, fbc = fb % nhfb;
+            const int nhb = min(16, mi_cols - 16 * fbc), nvb = min(16, mi_rows - 16 * fbr);
+            // the 8x8 blocks inside the frame: (nhb + 1) / 2 columns of the
+            // first (nvb + 1) / 2 rows of the 8 x 8 bit grid
+            const uint64_t cols = (1ull << ((nhb + 1) >> 1)) - 1;
+            const int rows = (nvb + 1) >> 1;
+            const uint64_t valid = (cols * 0x0101010101010101ull) & (rows >= 8 ? ~0ull : (1ull << (8 * rows)) - 1);
+            a = ((noskip ? noskip[fb] : ~0ull) & valid) != 0;
         }
-        if (lane == 0) {
-            s_nact = n;
+        const uint64_t vote = __ballot(a);
+        if (lane == 0) amask[c0] = vote;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const int nch = (nfb + 63) >> 6;   // at most 64 (4096 blocks)
+        const int cnt = lane < nch ? __popcll(amask[lane]) : 0;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        aoff[lane] = incl - cnt;
+        if (lane == 63) {
+            s_nact = incl;
             s_bad = 0;
             s_abort = 0;
+            s_wide = 0;
         }
+    }
+    __syncthreads();
+    for (int c0 = wv; c0 * 64 < nfb; c0 += SEL_W) {
+        const uint64_t m = amask[c0];
+        if ((m >> lane) & 1) act[aoff[c0] + __popcll(m & ((1ull << lane) - 1))] = c0 * 64 + lane;
     }
     // the next launch's slots start empty
     if (b == 0)
-        for (int i = t; i < 3 * SEL_STEPS * SEL_G; i += SEL_T) (&st->slot[(epoch + 1) & 1][0][0][0])[i] = 0ull;
+        for (int i = t; i < SEL_G * SEL_HALF; i += SEL_T)
+            st->slot[i / SEL_HALF][((epoch + 1) & 1) * SEL_HALF + i % SEL_HALF] = 0ull;
     __syncthreads();
     const int nact = s_nact;
-    // ---- the MSE rows of the active blocks in LDS
-    if (TABLES >= 1 && b != 0) {
-        // eight rows' loads in flight per thread before any store
-        bool bad = false;
-        for (int e0 = t; e0 < nact * 64; e0 += 8 * SEL_T) {
-            uint64_t c[8], l[8];
+    bool bad = false;
+    if (TABLES >= 1) {
+        // the MSE rows of the active blocks in LDS, eight rows' loads in
+        // flight per thread before any store
+        if (b != 0)
+            for (int e0 = t; e0 < nact * 64; e0 += 8 * SEL_T) {
+                uint64_t c[8], l[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int e = e0 + u * SEL_T;
-                const int fb = act[min(e, nact * 64 - 1) >> 6], g = e & 63;
-                c[u] = Cm[(size_t)fb * 64 + g];
-                l[u] = TABLES == 2 ? Lm[(size_t)fb * 64 + g] : 0;
+                for (int u = 0; u < 8; u++) {
+                    const int e = e0 + u * SEL_T;
+                    const int fb = act[min(e, nact * 64 - 1) >> 6], g = e & 63;
+                    c[u] = Cm[(size_t)fb * 64 + g];
+                    l[u] = TABLES == 2 ? Lm[(size_t)fb * 64 + g] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int e = e0 + u * SEL_T;
+                    if (e < nact * 64) {
+                        ct[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)c[u];
+                        if (TABLES == 2) lt[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)l[u];
+                        bad |= ((c[u] | l[u]) >> 32) != 0;
+                    }
+                }
+            }
+        // every launch arrives at the barrier counter, so its target is
+        // SEL_GRID per launch whichever kernel ran before
+        if (t == 0) __hip_atomic_fetch_add(&st->conv, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        // the 32-bit chroma copy: every workgroup writes its share, then all
+        // wait for all (the copy is read from every XCD: a release of this
+        // XCD's L2 before the arrival, an acquire after the wait)
+        bool cbad = false;
+        for (int e0 = b * SEL_T + t; e0 < nact * 64; e0 += 4 * SEL_GRID * SEL_T) {
+            uint64_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = min(e0 + u * SEL_GRID * SEL_T, nact * 64 - 1);
+                c[u] = Cm[(size_t)act[e >> 6] * 64 + (e & 63)];
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int e = e0 + u * SEL_T;
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * SEL_GRID * SEL_T;
                 if (e < nact * 64) {
-                    ct[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)c[u];
-                    if (TABLES == 2) lt[(e >> 6) * SEL_TP + (e & 63)] = (uint32_t)l[u];
-                    bad |= ((c[u] | l[u]) >> 32) != 0;
+                    st->c32[e] = (uint32_t)c[u];
+                    cbad |= (c[u] >> 32) != 0;
                 }
             }
         }
-        if (bad) s_bad = 1;
+        if (cbad) atomicExch(&st->c32bad, epoch);
+        // each wave's stores complete before the barrier (it does not wait
+        // for them), so the one write-back below covers them
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (t == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            // the write-back must be complete before the arrival (the
+            // compiler does not count it as an outstanding store)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __hip_atomic_fetch_add(&st->conv, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!sel_wait_count(&st->conv, SEL_GRID * epoch, st, epoch, err)) s_abort = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (__hip_atomic_load(&st->c32bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) s_bad = 1;
+        }
     }
+    // the member's luma column, and every best cost "none yet"
+    if (b != 0)
+        for (int i0 = t; i0 < nact; i0 += 4 * SEL_T) {
+            uint64_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = Lm[(size_t)act[min(i0 + u * SEL_T, nact - 1)] * 64 + j];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * SEL_T;
+                if (i < nact) {
+                    lcol[i] = (uint32_t)v[u];
+                    best[i] = ~0ull;
+                    bad |= (v[u] >> 32) != 0;
+                }
+            }
+        }
+    if (bad) s_bad = 1;
     __syncthreads();
-    const bool ctab = TABLES >= 1 && b != 0 && !s_bad, ltab = TABLES == 2 && b != 0 && !s_bad;
+    const bool lok = b != 0 && !s_bad;   // lcol and the 32-bit chroma rows hold exact values
+    const bool ctab = TABLES >= 1 && lok, ltab = TABLES == 2 && lok;
+    const uint32_t *c32 = st->c32;
     auto luma = [&](int i, int g) -> unsigned long long {
         return ltab ? (unsigned long long)lt[i * SEL_TP + g] : Lm[(size_t)act[i] * 64 + g];
     };
     auto chroma = [&](int i, int k) -> unsigned long long {
-        return ctab ? (unsigned long long)ct[i * SEL_TP + k] : Cm[(size_t)act[i] * 64 + k];
+        return ctab ? (unsigned long long)ct[i * SEL_TP + k]
+             : (TABLES == 0 && lok) ? (unsigned long long)c32[i * 64 + k]
+                                    : Cm[(size_t)act[i] * 64 + k];
     };
-    const int nb = 1 << level;
-    const int steps = level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: compute it once
-    const int j = gw;
-    // a_ij of this workgroup's luma strength for every active block
-    if (TABLES < 2 && b != 0) {
-        for (int i0 = t; i0 < nact; i0 += 8 * SEL_T) {
-            unsigned long long v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = Lm[(size_t)act[min(i0 + u * SEL_T, nact - 1)] * 64 + j];
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (i0 + u * SEL_T < nact) lcol[i0 + u * SEL_T] = v[u];
-        }
-    }
-    auto lj = [&](int i) -> unsigned long long { return ltab ? (unsigned long long)lt[i * SEL_TP + j] : lcol[i]; };
-    // best_i, the cheapest prior pair of every active block, is kept across
-    // steps: a new pair only lowers it (min with the pair's cost), and a
-    // refinement's dropped pair is handled by recomputing, while the step's
-    // exchange is in flight, the minimum over the pairs the next step keeps
-    // but the new one (then the new pair's cost joins it)
-    auto fold_new = [&](int py_new, int pc_new, bool fresh) {
-        unsigned long long ps = 0;
+    // a new pair: its cost column, and every block's best cost lowered to it
+    // (best_i is "none yet" before the first pair, and before a refinement's
+    // new pair the minimum over the pairs that refinement keeps)
+    auto fold = [&](int py, int pc, int pos) {
         bool wide = false;
         for (int i = t; i < nact; i += SEL_T) {
-            const unsigned long long c = luma(i, py_new) + chroma(i, pc_new);
-            const unsigned long long m = fresh ? c : (c < best[i] ? c : best[i]);
+            const unsigned long long c = luma(i, py) + chroma(i, pc);
+            const unsigned long long m = c < best[i] ? c : best[i];
             best[i] = m;
-            ps += m;
-            wide |= (m >> 32) != 0;
+            if (cc) costc[pos * nfb + i] = (c >> 32) ? 0xffffffffu : (uint32_t)c;
+            wide |= (c >> 32) != 0;
         }
-        struct { unsigned long long sum; bool wide; } r = { ps, wide };
-        return r;
+        // sticky: a saturated cost column stays in the lists for later steps
+        if (wide) s_wide = 1;
     };
+    const int nb = 1 << level;
+    const int steps = s_abort ? 0 : level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: once
+    const int per = (nact + SEL_W - 1) / SEL_W;                // blocks per wave
+    const int lo = min(wv * per, nact), hi = min(lo + per, nact);
     int gstep = 0;
     for (int s = 0; s < steps; s++) {
         const int slot = s < nb ? s : nb - 1;
-        if (s >= nb && t == 0)   // refinement: drop the oldest pair
-            for (int i = 0; i + 1 < nb; i++) { ly[i] = ly[i + 1]; lc[i] = lc[i + 1]; }
+        if (wv == 0) {
+            // refinement: drop the oldest pair (the new one reuses its cost column)
+            if (s >= nb) {
+                const int y = ly[min(lane + 1, 7)], c = lc[min(lane + 1, 7)];
+                const int r = lane == nb - 1 ? rp[0] : rp[min(lane + 1, 7)];
+                asm volatile("" ::: "memory");   // every lane reads before any lane writes
+                if (lane < nb) {
+                    ly[lane] = y;
+                    lc[lane] = c;
+                    rp[lane] = r;
+                }
+            } else if (lane == 0) {
+                rp[s] = s;
+            }
+        }
         __syncthreads();
         if (slot == 0) {
             // column sums: 4 threads per column over interleaved blocks
@@ -524,130 +680,91 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
                 for (int o = 32; o > 0; o >>= 1) {
                     const unsigned long long ov = __shfl_xor(v, o);
                     const int ok = __shfl_xor(k, o);
-                    if (ov < v || (ov == v && ok < k)) { v = ov; k = ok; }
+                    if (ov < v || (ov == v && ok < k)) {
+                        v = ov;
+                        k = ok;
+                    }
                 }
                 if (lane == 0) {
-                    sums[wv] = v;
+                    s_min[wv] = v;
                     if (wv == 0) ly[0] = nact ? k : 0;
                     else lc[0] = nact ? k : 0;
                 }
             }
             __syncthreads();
-            if (t == 0) s_dist = nact ? sums[0] + sums[1] : 0ull;
-            if (s + 1 < steps) {   // the blocks' best costs for the next step: pair 0
-                if (t == 0) s_wide = 0;
-                __syncthreads();
-                const auto r = fold_new(ly[0], lc[0], true);
-                unsigned long long psum = r.sum;
-                if (r.wide) s_wide = 1;
-                for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
-                if (lane == 0) part[wv][0] = psum;
-                __syncthreads();
-                if (t == 0) {
-                    unsigned long long v = 0;
-                    for (int w = 0; w < 8; w++) v += part[w][0];
-                    s_sum = v;
-                }
+            if (level == 0) {
+                if (t == 0) s_dist = nact ? s_min[0] + s_min[1] : 0ull;
+                continue;
             }
+            fold(ly[0], lc[0], 0);
+            __syncthreads();
             continue;
         }
-        // savings of the pairs (j, lane) over this wave's eighth of the
-        // blocks, eight blocks' loads in flight at a time (a block whose
-        // a_ij is not below best_i has no room and saves nothing)
+        // savings of the pairs (j, lane) over this wave's blocks: a lane per
+        // block forms its room, then the rows are read eight at a time (one
+        // address, immediate offsets) and each row's room is read from its lane
         unsigned long long gain = 0;
-        if (ltab && !s_wide) {
-            int i = wv;
-            for (; i + 56 < nact; i += 64) {
-                uint32_t bi[8], a[8], c[8];
+        if (lok && !s_wide) {
+            for (int base = lo; base < hi; base += 64) {
+                const int n = min(64, hi - base);
+                const uint32_t room =
+                    lane < n ? __builtin_elementwise_sub_sat((uint32_t)best[base + lane], lcol[base + lane]) : 0u;
+                int u0 = 0;
+                for (; u0 + 8 <= n; u0 += 8) {
+                    uint32_t c[8];
+                    if constexpr (TABLES >= 1) {
+                        const uint32_t *row = ct + (base + u0) * SEL_TP + lane;
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    bi[u] = (uint32_t)best[i + 8 * u];
-                    a[u] = lt[(i + 8 * u) * SEL_TP + j];
-                    c[u] = ct[(i + 8 * u) * SEL_TP + lane];
+                        for (int u = 0; u < 8; u++) c[u] = row[u * SEL_TP];
+                    } else {
+                        const uint32_t *row = c32 + (size_t)(base + u0) * 64 + lane;
+#pragma unroll
+                        for (int u = 0; u < 8; u++) c[u] = row[u * 64];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        gain += __builtin_elementwise_sub_sat(
+                            (uint32_t)__builtin_amdgcn_readlane((int)room, u0 + u), c[u]);
                 }
-#pragma unroll
-                for (int u = 0; u < 8; u++)
-                    gain += __builtin_elementwise_sub_sat(__builtin_elementwise_sub_sat(bi[u], a[u]), c[u]);
+                for (; u0 < n; u0++) {
+                    const uint32_t c = TABLES >= 1 ? ct[(base + u0) * SEL_TP + lane] : c32[(size_t)(base + u0) * 64 + lane];
+                    gain += __builtin_elementwise_sub_sat((uint32_t)__builtin_amdgcn_readlane((int)room, u0), c);
+                }
             }
-            for (; i < nact; i += 8)
-                gain += __builtin_elementwise_sub_sat(
-                    __builtin_elementwise_sub_sat((uint32_t)best[i], lt[i * SEL_TP + j]), ct[i * SEL_TP + lane]);
         } else {
             auto save = [&](unsigned long long bi, unsigned long long a, unsigned long long c) {
                 const unsigned long long room = a < bi ? bi - a : 0ull;
                 return c < room ? room - c : 0ull;
             };
-            int i = wv;
-            for (; i + 56 < nact; i += 64) {
+            for (int i = lo; i < hi; i += 8) {
                 unsigned long long bi[8], a[8], c[8];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    bi[u] = best[i + 8 * u];
-                    a[u] = lj(i + 8 * u);
-                    c[u] = chroma(i + 8 * u, lane);
+                    const int ii = min(i + u, hi - 1);
+                    bi[u] = best[ii];
+                    a[u] = lok ? (unsigned long long)lcol[ii] : Lm[(size_t)act[ii] * 64 + j];
+                    c[u] = chroma(ii, lane);
                 }
 #pragma unroll
-                for (int u = 0; u < 8; u++) gain += save(bi[u], a[u], c[u]);
+                for (int u = 0; u < 8; u++)
+                    if (i + u < hi) gain += save(bi[u], a[u], c[u]);
             }
-            for (; i < nact; i += 8) gain += save(best[i], lj(i), chroma(i, lane));
         }
-        __syncthreads();
         part[wv][lane] = gain;
         __syncthreads();
         if (wv == 0) {
             unsigned long long g = 0;
 #pragma unroll
-            for (int w = 0; w < 8; w++) g += part[w][lane];
-            unsigned long long key = (g << 12) | (unsigned long long)(4095 - (j * 64 + lane));
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long other = __shfl_xor(key, o);
-                key = other > key ? other : key;
-            }
-            // a store that is not waited for: the workgroup goes on to poll
+            for (int w = 0; w < SEL_W; w++) g += part[w][lane];
+            unsigned long long key = wave_max_u64((g << 12) | (unsigned long long)(4095 - (j * 64 + lane)));
+            const int at = (epoch & 1) * SEL_HALF + (level - 1) * SEL_STEPS + gstep;
+            // a store that is not waited for: the wave goes on to poll
             if (lane == 0)
-                __hip_atomic_store(&st->slot[epoch & 1][level - 1][gstep][gw], key | (1ull << 63), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // while the exchange is in flight: the next step's best costs over
-        // the pairs it keeps, less the one this step adds.  A greedy next
-        // step keeps every pair (best_i stays); a refinement drops slot 0,
-        // keeping slots 1 .. slot-1 of the current lists.
-        const bool next_refines = s + 1 < steps && s + 1 >= nb;
-        if (next_refines && wv > 0) {
-            int py[8], pc[8];
-#pragma unroll
-            for (int g = 0; g < 8; g++) { py[g] = ly[g]; pc[g] = lc[g]; }
-            for (int i = t - 64; i < nact; i += SEL_T - 64) {
-                unsigned long long m = ~0ull;
-                if (ltab) {
-                    uint32_t a[8], c[8];
-#pragma unroll
-                    for (int g = 1; g < 8; g++) {
-                        a[g] = lt[i * SEL_TP + py[g]];
-                        c[g] = ct[i * SEL_TP + pc[g]];
-                    }
-#pragma unroll
-                    for (int g = 1; g < 8; g++)
-                        if (g < slot) {
-                            const unsigned long long v = (unsigned long long)a[g] + c[g];
-                            m = v < m ? v : m;
-                        }
-                } else {
-#pragma unroll
-                    for (int g = 1; g < 8; g++)
-                        if (g < slot) {
-                            const unsigned long long v = luma(i, py[g]) + chroma(i, pc[g]);
-                            m = v < m ? v : m;
-                        }
-                }
-                best[i] = m;   // the gain above has read best (barrier before the exchange)
-            }
-        }
-        // every workgroup of the level reads the level's slots of this step
-        if (t < SEL_G) {
+                __hip_atomic_store(&st->slot[j][at], key | (1ull << 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // every workgroup of the level reads the level's slots of this step
             unsigned long long v;
             for (int spin = 0;; spin++) {
-                v = atomicOr(&st->slot[epoch & 1][level - 1][gstep][t], 0ull);
+                v = __hip_atomic_load(&st->slot[lane][at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (v >> 63) break;
                 if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch ||
                     spin > (1 << 21)) {
@@ -658,43 +775,59 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            unsigned long long key = v & ~(1ull << 63);
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long other = __shfl_xor(key, o);
-                key = other > key ? other : key;
+            key = wave_max_u64(v & ~(1ull << 63));
+            if (lane == 0) s_key = key;
+        } else if (s + 1 < steps && s + 1 >= nb) {
+            // while the exchange is in flight: the next step (a refinement,
+            // which drops list entry 0) keeps entries 1 .. slot-1 besides the
+            // pair this step adds; their minimum per block (the gain above
+            // has read best: a barrier lies between)
+            for (int i = t - 64; i < nact; i += SEL_T - 64) {
+                unsigned long long m = ~0ull;
+                if (cc && !s_wide) {
+#pragma unroll
+                    for (int g = 1; g < 8; g++)
+                        if (g < slot) {
+                            const unsigned long long v = costc[rp[g] * nfb + i];
+                            m = v < m ? v : m;
+                        }
+                } else {
+#pragma unroll
+                    for (int g = 1; g < 8; g++)
+                        if (g < slot) {
+                            const unsigned long long v = luma(i, ly[g]) + chroma(i, lc[g]);
+                            m = v < m ? v : m;
+                        }
+                }
+                best[i] = m;
             }
-            if (t == 0) s_key = key;
         }
         __syncthreads();
         if (s_abort) break;
+        const int pair = 4095 - (int)(s_key & 4095ull);
         if (t == 0) {
-            const unsigned long long key = s_key;
-            const int pair = 4095 - (int)(key & 4095ull);
             ly[slot] = pair >> 6;
             lc[slot] = pair & 63;
-            s_dist = s_sum - (key >> 12);
-            s_wide = 0;
         }
         gstep++;
+        fold(pair >> 6, pair & 63, rp[slot]);
         __syncthreads();
-        if (s + 1 < steps) {   // fold the new pair into the best costs of the next step
-            const int sk = 4095 - (int)(s_key & 4095ull);
-            const bool fresh = next_refines && slot == 1;   // the next step keeps only the new pair
-            const auto r = fold_new(sk >> 6, sk & 63, fresh);
-            unsigned long long psum = r.sum;
-            if (r.wide) s_wide = 1;
-            for (int o = 32; o > 0; o >>= 1) psum += __shfl_down(psum, o);
-            if (lane == 0) part[wv][0] = psum;
-            __syncthreads();
-            if (t == 0) {
-                unsigned long long v = 0;
-                for (int w = 0; w < 8; w++) v += part[w][0];
-                s_sum = v;
-            }
-        }
     }
     // ---- level result, then the last level to finish decides
-    if (b != 0 && gw != 0) return;
+    if (b != 0 && j != 0) return;
+    if (b != 0) {
+        // the level's distortion: every block at its best listed pair
+        unsigned long long ps = 0;
+        for (int i = t; i < nact; i += SEL_T) ps += best[i];
+        for (int o = 32; o > 0; o >>= 1) ps += __shfl_down(ps, o);
+        if (lane == 0) part[wv][0] = ps;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long v = 0;
+            for (int w = 0; w < SEL_W; w++) v += part[w][0];
+            s_dist = nact ? v : 0ull;
+        }
+    }
     if (t == 0) {
         if (!s_abort) {
             const long long rate = (long long)(nact * level + nb * 12) << 9;
@@ -718,7 +851,10 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
         for (int l = 1; l < 4; l++)
             if (st->cost[l] < st->cost[bb]) bb = l;
         bits = bb;
-        for (int i = 0; i < 8; i++) { ys[i] = st->ly[bb][i]; us[i] = st->lc[bb][i]; }
+        for (int i = 0; i < 8; i++) {
+            ys[i] = st->ly[bb][i];
+            us[i] = st->lc[bb][i];
+        }
     }
     __syncthreads();
     const int nbb = 1 << bits;
@@ -730,7 +866,10 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
         int bg = 0;
         for (int g = 0; g < nbb; g++) {
             const unsigned long long c = Lm[(size_t)fb * 64 + ys[g]] + Cm[(size_t)fb * 64 + us[g]];
-            if (c < bm) { bm = c; bg = g; }
+            if (c < bm) {
+                bm = c;
+                bg = g;
+            }
         }
         fb_index[fb] = (uint8_t)bg;
     }
@@ -753,17 +892,35 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
     FinishState *st = static_cast<FinishState *>(state);
     unsigned long long lam = lambda;
     const int nfb = nhfb * nvfb;
-    const int tables = nfb <= SEL_T2_MAX ? 2 : nfb <= SEL_T1_MAX ? 1 : 0;
+    if (nfb > 4096) return hipErrorInvalidValue;
+    // the most that fits in LDS: both tables, the chroma table, neither; the
+    // cost columns when they still fit
+    int tables = 0, use_cc = 0;
+    bool found = false;
+    for (int tb = 2; tb >= 0 && !found; tb--)
+        for (int c = 1; c >= 0 && !found; c--)
+            if (sel_layout(nfb, tb, c != 0).total <= SEL_LDS_BUDGET) {
+                tables = tb;
+                use_cc = c;
+                found = true;
+            }
+    if (!found) return hipErrorInvalidValue;
     const void *fn = tables == 2 ? (const void *)cdef_select_kernel<2>
                    : tables == 1 ? (const void *)cdef_select_kernel<1>
                                  : (const void *)cdef_select_kernel<0>;
-    const size_t lds = sel_lds_bytes(nfb, tables);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const size_t lds = sel_layout(nfb, tables, use_cc != 0).total;
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    if (fa.sharedSizeBytes + SEL_LDS_BUDGET > 160 * 1024) return hipErrorInvalidValue;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void *args[] = { (void *)&mse, (void *)&noskip, (void *)&nhfb, (void *)&nvfb, (void *)&mi_cols, (void *)&mi_rows,
-                     (void *)&lam, (void *)&st, (void *)&epoch, (void *)&dec, (void *)&fb_index, (void *)&err };
+                     (void *)&lam, (void *)&st, (void *)&epoch, (void *)&use_cc, (void *)&dec, (void *)&fb_index,
+                     (void *)&err };
     // a cooperative launch: every workgroup resident at once (a workgroup
-    // waits for the slots of the other workgroups of its level)
+    // waits for the slots of the other workgroups of its level, and for the
+    // chroma copy of all)
     return hipLaunchCooperativeKernel(fn, dim3(SEL_GRID), dim3(SEL_T), args, (unsigned int)lds, s);
 }
 
@@ -1013,35 +1170,10 @@ static_assert(sizeof(SmallState) <= 1024, "small selection state");
 
 // Lane exchanges (lane l reads lane l ^ X): DPP quad permutations for
 // distances 1 and 2, swizzles within 32 lanes for 4..16, a permute for 32.
-template <int X>
-__device__ __forceinline__ uint32_t xchg32(uint32_t v)
-{
-    if constexpr (X == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false);
-    else if constexpr (X == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false);
-    else if constexpr (X == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
-    else if constexpr (X == 8) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
-    else if constexpr (X == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
-    else return (uint32_t)__shfl_xor((int)v, 32, 64);
-}
-template <int X>
-__device__ __forceinline__ uint64_t xchg64(uint64_t v)
-{
-    return (uint64_t)xchg32<X>((uint32_t)v) | ((uint64_t)xchg32<X>((uint32_t)(v >> 32)) << 32);
-}
 // A block's NCAND shortlist SSEs of one plane as one register vector:
 // indexing it with a wave-uniform (scalar) index stays in registers.
 typedef int cand8 __attribute__((ext_vector_type(NCAND)));
 
-// Largest 64-bit value over the wave (every lane ends with it).
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
-{
-    v = max(v, xchg64<1>(v));
-    v = max(v, xchg64<2>(v));
-    v = max(v, xchg64<4>(v));
-    v = max(v, xchg64<8>(v));
-    v = max(v, xchg64<16>(v));
-    return max(v, xchg64<32>(v));
-}
 
 // The greedy selection over the shortlist pairs with 64-bit costs, for
 // tables whose SSEs need more than 32 bits (10-bit frames: a 64x64 luma
