@@ -20,6 +20,8 @@
 // of cdef/filter_search_batch.go); the accumulators are reduced across the
 // workgroup with wave shuffles and a 4-way LDS sum.  The work is integer-VALU
 // bound (64 trials x ~70 ops per pixel).
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include "pf_internal.h"
 #include "nn_internal.h"
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
     __shared__ int aoff[64];
     __shared__ unsigned long long s_key, s_dist, s_min[2];
     __shared__ int ly[8], lc[8], rp[8];
-    __shared__ int s_nact, s_bad, s_abort, s_wide;
+    __shared__ int s_nact, s_bad, s_abort, s_wide, s_pos, s_fpos;
     __shared__ bool s_last;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfb = nhfb * nvfb;
@@ -626,44 +628,31 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
              : (TABLES == 0 && lok) ? (unsigned long long)c32[i * 64 + k]
                                     : Cm[(size_t)act[i] * 64 + k];
     };
-    // a new pair: its cost column, and every block's best cost lowered to it
-    // (best_i is "none yet" before the first pair, and before a refinement's
-    // new pair the minimum over the pairs that refinement keeps)
+    const int nb = 1 << level;
+    const int steps = s_abort ? 0 : level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: once
+    // wave w owns the blocks [lo, hi) (a multiple of 8 blocks each): it folds
+    // their best costs, forms their rooms and sums their savings, so the waves
+    // meet only around the exchange
+    const int per = ((nact + SEL_W - 1) / SEL_W + 7) & ~7;
+    const int lo = min(wv * per, nact), hi = min(lo + per, nact);
+    bool wwide = false;   // a cost of 2^32 or more among the wave's blocks (sticky; s_wide: any block)
+    // a new pair: its cost column, and the wave's blocks' best costs lowered
+    // to it (best_i is "none yet" before the first pair, and before a
+    // refinement's new pair the minimum over the pairs that refinement keeps)
     auto fold = [&](int py, int pc, int pos) {
-        bool wide = false;
-        for (int i = t; i < nact; i += SEL_T) {
+        for (int i = lo + lane; i < hi; i += 64) {
             const unsigned long long c = luma(i, py) + chroma(i, pc);
             const unsigned long long m = c < best[i] ? c : best[i];
             best[i] = m;
             if (cc) costc[pos * nfb + i] = (c >> 32) ? 0xffffffffu : (uint32_t)c;
-            wide |= (c >> 32) != 0;
+            wwide |= (c >> 32) != 0;
         }
-        // sticky: a saturated cost column stays in the lists for later steps
-        if (wide) s_wide = 1;
+        wwide = __ballot(wwide) != 0;
+        if (wwide) s_wide = 1;   // for the recompute, which reads every wave's blocks
     };
-    const int nb = 1 << level;
-    const int steps = s_abort ? 0 : level == 0 ? 1 : 5 * nb;   // level 0 repeats one independent choice: once
-    const int per = (nact + SEL_W - 1) / SEL_W;                // blocks per wave
-    const int lo = min(wv * per, nact), hi = min(lo + per, nact);
     int gstep = 0;
     for (int s = 0; s < steps; s++) {
         const int slot = s < nb ? s : nb - 1;
-        if (wv == 0) {
-            // refinement: drop the oldest pair (the new one reuses its cost column)
-            if (s >= nb) {
-                const int y = ly[min(lane + 1, 7)], c = lc[min(lane + 1, 7)];
-                const int r = lane == nb - 1 ? rp[0] : rp[min(lane + 1, 7)];
-                asm volatile("" ::: "memory");   // every lane reads before any lane writes
-                if (lane < nb) {
-                    ly[lane] = y;
-                    lc[lane] = c;
-                    rp[lane] = r;
-                }
-            } else if (lane == 0) {
-                rp[s] = s;
-            }
-        }
-        __syncthreads();
         if (slot == 0) {
             // column sums: 4 threads per column over interleaved blocks
             const int col = t & 127, sub = t >> 7;
@@ -696,21 +685,29 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
                 if (t == 0) s_dist = nact ? s_min[0] + s_min[1] : 0ull;
                 continue;
             }
-            fold(ly[0], lc[0], 0);
+            const int py = ly[0], pc = lc[0];
+            // the list holds pair 0 in cost column 0; the next step's pair goes to column 1
             __syncthreads();
+            if (t == 0) {
+                rp[0] = 0;
+                s_pos = 1;
+            }
+            fold(py, pc, 0);
             continue;
         }
         // savings of the pairs (j, lane) over this wave's blocks: a lane per
         // block forms its room, then the rows are read eight at a time (one
-        // address, immediate offsets) and each row's room is read from its lane
+        // address, immediate offsets; rows past the wave's own up to the
+        // table's end carry no room) and each row's room is read from its lane
         unsigned long long gain = 0;
-        if (lok && !s_wide) {
+        if (lok && !wwide) {
             for (int base = lo; base < hi; base += 64) {
                 const int n = min(64, hi - base);
                 const uint32_t room =
                     lane < n ? __builtin_elementwise_sub_sat((uint32_t)best[base + lane], lcol[base + lane]) : 0u;
+                const int n8 = base + ((n + 7) & ~7) <= nact ? (n + 7) & ~7 : n & ~7;
                 int u0 = 0;
-                for (; u0 + 8 <= n; u0 += 8) {
+                for (; u0 < n8; u0 += 8) {
                     uint32_t c[8];
                     if constexpr (TABLES >= 1) {
                         const uint32_t *row = ct + (base + u0) * SEL_TP + lane;
@@ -751,6 +748,7 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
             }
         }
         part[wv][lane] = gain;
+        const bool next_refines = s + 1 < steps && s + 1 >= nb;
         __syncthreads();
         if (wv == 0) {
             unsigned long long g = 0;
@@ -761,27 +759,44 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
             // a store that is not waited for: the wave goes on to poll
             if (lane == 0)
                 __hip_atomic_store(&st->slot[j][at], key | (1ull << 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // every workgroup of the level reads the level's slots of this step
-            unsigned long long v;
+            // every workgroup of the level reads the level's slots of this
+            // step; two reads of a slot in flight (alternating registers), so
+            // a key is seen about one memory latency after it lands (the
+            // abort word every 64 rounds)
+            const unsigned long long *sp = &st->slot[lane][at];
+            unsigned long long v, pa = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long pb = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (int spin = 0;; spin++) {
-                v = __hip_atomic_load(&st->slot[lane][at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (v >> 63) break;
-                if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch ||
-                    spin > (1 << 21)) {
+                if (pa >> 63) {
+                    v = pa;
+                    break;
+                }
+                pa = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (pb >> 63) {
+                    v = pb;
+                    break;
+                }
+                pb = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((spin & 63) == 63 &&
+                    (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch ||
+                     spin > (1 << 21))) {
                     atomicExch(&st->abort, epoch);
                     atomicOr(err, 2);
                     s_abort = 1;
+                    v = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
             key = wave_max_u64(v & ~(1ull << 63));
-            if (lane == 0) s_key = key;
-        } else if (s + 1 < steps && s + 1 >= nb) {
+            if (lane == 0) {
+                s_key = key;
+                s_fpos = s_pos;
+            }
+        } else if (next_refines) {
             // while the exchange is in flight: the next step (a refinement,
             // which drops list entry 0) keeps entries 1 .. slot-1 besides the
-            // pair this step adds; their minimum per block (the gain above
-            // has read best: a barrier lies between)
+            // pair this step adds; their minimum per block (every gain has
+            // read best: a barrier lies between)
             for (int i = t - 64; i < nact; i += SEL_T - 64) {
                 unsigned long long m = ~0ull;
                 if (cc && !s_wide) {
@@ -805,14 +820,32 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
         __syncthreads();
         if (s_abort) break;
         const int pair = 4095 - (int)(s_key & 4095ull);
-        if (t == 0) {
-            ly[slot] = pair >> 6;
-            lc[slot] = pair & 63;
+        if (wv == SEL_W - 1) {
+            // the lists after this step (the new pair in `slot`, its cost in
+            // column s_fpos), then, when the next step refines, without entry
+            // 0, whose column the next pair reuses; read by the next step's
+            // recompute, behind its first barrier (the last wave has the
+            // fewest blocks to fold)
+            const int e = next_refines ? min(lane + 1, 7) : min(lane, 7);
+            int y = ly[e], c = lc[e], r = rp[e];
+            const int r0 = rp[0];
+            if (e == slot) {
+                y = pair >> 6;
+                c = pair & 63;
+                r = s_fpos;
+            }
+            asm volatile("" ::: "memory");   // every lane reads before any lane writes
+            if (next_refines ? lane < nb - 1 : lane == slot) {
+                ly[lane] = y;
+                lc[lane] = c;
+                rp[lane] = r;
+            }
+            if (lane == 0) s_pos = next_refines ? (slot == 0 ? s_fpos : r0) : s + 1;
         }
         gstep++;
-        fold(pair >> 6, pair & 63, rp[slot]);
-        __syncthreads();
+        fold(pair >> 6, pair & 63, s_fpos);
     }
+    __syncthreads();
     // ---- level result, then the last level to finish decides
     if (b != 0 && j != 0) return;
     if (b != 0) {
@@ -909,12 +942,25 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
                    : tables == 1 ? (const void *)cdef_select_kernel<1>
                                  : (const void *)cdef_select_kernel<0>;
     const size_t lds = sel_layout(nfb, tables, use_cc != 0).total;
-    hipFuncAttributes fa;
-    hipError_t e = hipFuncGetAttributes(&fa, fn);
+    // the attribute calls once per device and variant (they cost host time
+    // between the launches of a frame's search)
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, bool> ready;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    if (fa.sharedSizeBytes + SEL_LDS_BUDGET > 160 * 1024) return hipErrorInvalidValue;
-    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!ready[{ dev, tables }]) {
+            hipFuncAttributes fa;
+            e = hipFuncGetAttributes(&fa, fn);
+            if (e != hipSuccess) return e;
+            if (fa.sharedSizeBytes + SEL_LDS_BUDGET > 160 * 1024) return hipErrorInvalidValue;
+            e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SEL_LDS_BUDGET);
+            if (e != hipSuccess) return e;
+            ready[{ dev, tables }] = true;
+        }
+    }
     void *args[] = { (void *)&mse, (void *)&noskip, (void *)&nhfb, (void *)&nvfb, (void *)&mi_cols, (void *)&mi_rows,
                      (void *)&lam, (void *)&st, (void *)&epoch, (void *)&use_cc, (void *)&dec, (void *)&fb_index,
                      (void *)&err };
