@@ -363,7 +363,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 //    (first minimum of the column sums over the active blocks); every
 //    workgroup of the level computes it, so it needs no exchange.
 //  - Any other step: every workgroup holds the active blocks' best cost over
-//    the prior pairs (best_i).  Wave w takes a contiguous eighth of the
+//    the prior pairs (best_i).  Wave w takes a contiguous share of the
 //    blocks; a lane per block forms the room max(0, best_i - a_ij) (a block
 //    with no room saves nothing for any k, as the reference skips it), and
 //    the lanes k sum the savings max(0, room_i - b_ik) over the blocks (with
@@ -394,8 +394,9 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 // aborted (every workgroup then leaves) and sets bit 1 of the error flag.
 constexpr int SEL_G = 64;                  // workgroups per level >= 1 (one per luma strength)
 static_assert(SEL_G == 64, "one wave reads a level's slots, one lane each");
-constexpr int SEL_T = 512;                 // threads per workgroup (8 waves)
-constexpr int SEL_W = SEL_T / 64;
+// threads per workgroup: 8 waves when a table sits in LDS, else 16 (the
+// chroma rows come from device memory: more waves hide its latency)
+__host__ __device__ constexpr int sel_threads(int tables) { return tables == 0 ? 1024 : 512; }
 constexpr int SEL_GRID = 1 + 3 * SEL_G;    // level 0 on workgroup 0
 constexpr int SEL_STEPS = 39;              // exchanged steps of level 3 (8 greedy + 32 refinements, less the first)
 constexpr int SEL_HALF = 3 * SEL_STEPS;    // slots of one launch parity in a member's page
@@ -403,7 +404,8 @@ constexpr int SEL_PAGE = 512;              // 64-bit words per member page (4 Ki
 static_assert(2 * SEL_HALF <= SEL_PAGE, "a member's slots fit its page");
 constexpr int SEL_TP = 65;                 // LDS row pitch of the tables (words): lanes reading one
                                            // column of consecutive rows hit distinct banks
-constexpr size_t SEL_LDS_BUDGET = 160 * 1024 - 6144;   // dynamic LDS; the kernel's own variables take < 6 KiB
+// dynamic LDS; the kernel's own variables take less than the rest of 160 KiB
+__host__ __device__ constexpr size_t sel_lds_budget(int tables) { return 160 * 1024 - (tables == 0 ? 10240 : 6144); }
 
 struct FinishState {
     unsigned long long slot[SEL_G][SEL_PAGE];   // published keys, bit 63 = present
@@ -460,12 +462,13 @@ __device__ inline bool sel_wait_count(unsigned int *cnt, unsigned int target, Fi
 }
 
 template <int TABLES>
-__global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__restrict__ mse, const uint64_t *noskip,
+__global__ __launch_bounds__(sel_threads(TABLES)) void cdef_select_kernel(const uint64_t *__restrict__ mse, const uint64_t *noskip,
                                                             int nhfb, int nvfb, int mi_cols, int mi_rows,
                                                             unsigned long long lambda, FinishState *st,
                                                             unsigned int epoch, int use_cc, av1pf_cdef_decision *dec,
                                                             uint8_t *fb_index, int *err)
 {
+    constexpr int SEL_T = sel_threads(TABLES), SEL_W = SEL_T / 64;
     extern __shared__ __align__(16) char smem[];
     __shared__ unsigned long long part[SEL_W][64];
     __shared__ unsigned long long amask[64];
@@ -654,17 +657,19 @@ __global__ __launch_bounds__(SEL_T) void cdef_select_kernel(const uint64_t *__re
     for (int s = 0; s < steps; s++) {
         const int slot = s < nb ? s : nb - 1;
         if (slot == 0) {
-            // column sums: 4 threads per column over interleaved blocks
+            // column sums: SEL_T / 128 threads per column over interleaved blocks
             const int col = t & 127, sub = t >> 7;
             unsigned long long cs = 0;
-            for (int i = sub; i < nact; i += 4) cs += col < 64 ? luma(i, col) : chroma(i, col - 64);
-            reinterpret_cast<unsigned long long *>(part)[sub * 128 + col] = cs;   // part as [4][128]
+            for (int i = sub; i < nact; i += SEL_T / 128) cs += col < 64 ? luma(i, col) : chroma(i, col - 64);
+            reinterpret_cast<unsigned long long *>(part)[sub * 128 + col] = cs;   // part as [SEL_T / 128][128]
             __syncthreads();
             if (t < 128) {
                 // wave 0 holds the luma sums, wave 1 the chroma sums, lane k
                 // strength k: the first minimum by a shuffle reduction
                 const unsigned long long *pp = reinterpret_cast<const unsigned long long *>(part);
-                unsigned long long v = pp[t] + pp[128 + t] + pp[256 + t] + pp[384 + t];
+                unsigned long long v = 0;
+#pragma unroll
+                for (int q = 0; q < SEL_T / 128; q++) v += pp[128 * q + t];
                 int k = lane;
                 for (int o = 32; o > 0; o >>= 1) {
                     const unsigned long long ov = __shfl_xor(v, o);
@@ -932,7 +937,7 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
     bool found = false;
     for (int tb = 2; tb >= 0 && !found; tb--)
         for (int c = 1; c >= 0 && !found; c--)
-            if (sel_layout(nfb, tb, c != 0).total <= SEL_LDS_BUDGET) {
+            if (sel_layout(nfb, tb, c != 0).total <= sel_lds_budget(tb)) {
                 tables = tb;
                 use_cc = c;
                 found = true;
@@ -955,8 +960,8 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
             hipFuncAttributes fa;
             e = hipFuncGetAttributes(&fa, fn);
             if (e != hipSuccess) return e;
-            if (fa.sharedSizeBytes + SEL_LDS_BUDGET > 160 * 1024) return hipErrorInvalidValue;
-            e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SEL_LDS_BUDGET);
+            if (fa.sharedSizeBytes + sel_lds_budget(tables) > 160 * 1024) return hipErrorInvalidValue;
+            e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds_budget(tables));
             if (e != hipSuccess) return e;
             ready[{ dev, tables }] = true;
         }
@@ -967,7 +972,7 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
     // a cooperative launch: every workgroup resident at once (a workgroup
     // waits for the slots of the other workgroups of its level, and for the
     // chroma copy of all)
-    return hipLaunchCooperativeKernel(fn, dim3(SEL_GRID), dim3(SEL_T), args, (unsigned int)lds, s);
+    return hipLaunchCooperativeKernel(fn, dim3(SEL_GRID), dim3(sel_threads(tables)), args, (unsigned int)lds, s);
 }
 
 // ---------------------------------------------------------------------------
