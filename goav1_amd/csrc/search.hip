@@ -1421,26 +1421,43 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     constexpr int NP = NCAND * NCAND, BPL = NP;   // pairs; blocks per lane = 64 / (64 / NP) lanes per pair
     static_assert(64 % NP == 0 && BPL % 8 == 0, "pair lanes need NCAND^2 to divide 64 and 8 | NCAND^2");
     __shared__ uint16_t act[SMALL_MAX_FB];
-    __shared__ uint8_t flag[SMALL_MAX_FB];
-    __shared__ unsigned long long wpart[NW][65];   // per-wave pair gains (or column sums) and base sum
+    __shared__ unsigned long long amask[SMALL_MAX_FB / 64];
+    __shared__ int aoff[SMALL_MAX_FB / 64];
+    // per-wave pair gains (or column sums) and base sum, by step parity: every
+    // wave reads all of them after the step's one barrier, and a wave can be
+    // one step ahead of another
+    __shared__ unsigned long long wpart[2][NW][65];
     __shared__ __align__(16) uint32_t bw[NPER * SMALL_T];   // each block's best cost, for the pair lanes
+    __shared__ __align__(16) uint32_t yu[NPER * SMALL_T][2 * NCAND];   // the blocks' shortlist SSEs
     __shared__ int pick_s, nact_s, bits_s;
     __shared__ unsigned sel_s[2];
     __shared__ bool last;
     __shared__ unsigned long long dist_s;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int nfb = nhfb * nvfb;
-    for (int fb = t; fb < nfb; fb += SMALL_T) flag[fb] = fb_active(noskip, fb, nhfb, mi_cols, mi_rows) ? 1 : 0;
+    // ascending compaction: a ballot per 64 blocks, the chunks' offsets by one
+    // wave's scan, then every block writes its own entry
+    for (int c0 = wv; c0 * 64 < nfb; c0 += NW) {
+        const int fb = c0 * 64 + lane;
+        const uint64_t vote = __ballot(fb < nfb && fb_active(noskip, fb, nhfb, mi_cols, mi_rows));
+        if (lane == 0) amask[c0] = vote;
+    }
     __syncthreads();
-    if (t < 64) {   // ascending compaction by one wave
-        int n = 0;
-        for (int base = 0; base < nfb; base += 64) {
-            const bool a = base + lane < nfb && flag[base + lane];
-            const uint64_t vote = __ballot(a);
-            if (a) act[n + __popcll(vote & ((1ull << lane) - 1))] = (uint16_t)(base + lane);
-            n += __popcll(vote);
+    if (wv == 0) {
+        const int nch = (nfb + 63) >> 6;
+        const int cnt = lane < nch ? __popcll(amask[lane]) : 0;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
         }
-        if (lane == 0) nact_s = n;
+        if (lane < SMALL_MAX_FB / 64) aoff[lane] = incl - cnt;
+        if (lane == 63) nact_s = incl;
+    }
+    __syncthreads();
+    for (int c0 = wv; c0 * 64 < nfb; c0 += NW) {
+        const uint64_t m = amask[c0];
+        if ((m >> lane) & 1) act[aoff[c0] + __popcll(m & ((1ull << lane) - 1))] = (uint16_t)(c0 * 64 + lane);
     }
     __syncthreads();
     const int nact = nact_s;
@@ -1454,6 +1471,11 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
         for (int q = 0; q < NCAND; q++) {
             Y[r][q] = i < nact ? (int)mse[fb * NCAND + q] : 0;
             U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < NCAND; q++) {
+            yu[i][q] = (uint32_t)Y[r][q];
+            yu[i][NCAND + q] = (uint32_t)U[r][q];
         }
     }
     // the 32-bit arithmetic below is exact when every SSE is below 2^28 (a
@@ -1478,15 +1500,14 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     // the blocks w * 64 + part * BPL + m (+ 512 r), m < BPL, and holds their
     // y_j + uv_k
     const int pr = lane % NP, part = lane / NP;
+    __syncthreads();   // the shortlist SSEs of every block are in LDS
     uint32_t YU[NPER][BPL];
 #pragma unroll
     for (int r = 0; r < NPER; r++)
 #pragma unroll
         for (int m = 0; m < BPL; m++) {
             const int i = r * SMALL_T + wv * 64 + part * BPL + m;
-            const size_t fb = i < nact ? act[i] : 0;
-            YU[r][m] = i < nact ? (uint32_t)(mse[fb * NCAND + pr / NCAND] + mse[((size_t)nfb + fb) * NCAND + pr % NCAND])
-                                : 0u;
+            YU[r][m] = yu[i][pr / NCAND] + yu[i][NCAND + pr % NCAND];   // zeros past nact
         }
     const int bits = blockIdx.x;
     const int nb = 1 << bits;
@@ -1503,14 +1524,14 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
 #pragma unroll
                 for (int r = 0; r < NPER; r++) v += (unsigned)(q < NCAND ? Y[r][q] : U[r][q - NCAND]);
                 for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (lane == 0) wpart[wv][q] = v;
+                if (lane == 0) wpart[0][wv][q] = v;
             }
             __syncthreads();
             if (t == 0) {
                 unsigned long long cs[2 * NCAND];
                 for (int q = 0; q < 2 * NCAND; q++) {
                     cs[q] = 0;
-                    for (int w = 0; w < NW; w++) cs[q] += wpart[w][q];
+                    for (int w = 0; w < NW; w++) cs[q] += wpart[0][w][q];
                 }
                 int by = 0, bc = 0;
                 for (int q = 1; q < NCAND; q++) {
@@ -1521,13 +1542,20 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
                 dist_s = nact ? cs[by] + cs[NCAND + bc] : 0;
             }
         } else {
+            // the prior pairs as a wave-uniform bit set over the NP pairs;
+            // every block's best cost is the minimum over that set, read
+            // with constant register indices
+            unsigned pm = 0;
+            for (int g = 0; g < slot; g++) pm |= 1u << (((sy >> (4 * g)) & 15) * NCAND + ((su >> (4 * g)) & 15));
+            pm = (unsigned)__builtin_amdgcn_readfirstlane((int)pm);
             unsigned b[NPER];
             unsigned long long bsum = 0;
 #pragma unroll
             for (int r = 0; r < NPER; r++) {
-                b[r] = (unsigned)(Y[r][sy & 15] + U[r][su & 15]);
-                for (int g = 1; g < slot; g++)
-                    b[r] = min(b[r], (unsigned)(Y[r][(sy >> (4 * g)) & 15] + U[r][(su >> (4 * g)) & 15]));
+                b[r] = ~0u;
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+                    if ((pm >> q) & 1) b[r] = min(b[r], (unsigned)(Y[r][q / NCAND] + U[r][q % NCAND]));
                 bsum += b[r];
             }
 #pragma unroll
@@ -1556,30 +1584,42 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
             if constexpr (NP <= 8) gain += xchg64<8>(gain);
             if constexpr (NP <= 16) gain += xchg64<16>(gain);
             if constexpr (NP <= 32) gain += xchg64<32>(gain);
+            const int ph = step & 1;
             if (final_step) {   // the frame distortion is needed at the last step only
                 for (int o = 32; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 64);
-                if (lane == 0) wpart[wv][64] = bsum;
+                if (lane == 0) wpart[ph][wv][64] = bsum;
             }
-            wpart[wv][lane] = gain;
+            wpart[ph][wv][lane] = gain;
             __syncthreads();
-            if (wv == 0) {   // first maximum over the pairs: the largest (gain, 63 - pair) key
-                unsigned long long v = 0;
-                for (int w = 0; w < NW; w++) v += wpart[w][lane];
-                const unsigned long long key = wave_max_u64((v << 6) | (unsigned long long)(63 - pr));
-                if (lane == 0) {
-                    pick_s = 63 - (int)(key & 63);
-                    if (final_step) {
-                        unsigned long long base = 0;
-                        for (int w = 0; w < NW; w++) base += wpart[w][64];
-                        dist_s = base - (key >> 6);
-                    }
-                }
+            // every wave: the first maximum over the pairs, the largest
+            // (gain, 63 - pair) key
+            unsigned long long v = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) v += wpart[ph][w][lane];
+            // the lanes of a pair hold the same total: the maximum over the
+            // first NP lanes' neighbourhoods is the wave's
+            unsigned long long key = (v << 6) | (unsigned long long)(63 - pr);
+            key = max(key, xchg64<1>(key));
+            key = max(key, xchg64<2>(key));
+            if constexpr (NP > 4) key = max(key, xchg64<4>(key));
+            if constexpr (NP > 8) key = max(key, xchg64<8>(key));
+            if constexpr (NP > 16) key = max(key, xchg64<16>(key));
+            if constexpr (NP > 32) key = max(key, xchg64<32>(key));
+            if (t == 0 && final_step) {
+                unsigned long long base = 0;
+                for (int w = 0; w < NW; w++) base += wpart[ph][w][64];
+                dist_s = base - (key >> 6);
             }
+            const unsigned pk0 = 63 - (unsigned)(__builtin_amdgcn_readfirstlane((int)(uint32_t)key) & 63);
+            sy = (sy & ~(15u << (4 * slot))) | ((pk0 / NCAND) << (4 * slot));
+            su = (su & ~(15u << (4 * slot))) | ((pk0 % NCAND) << (4 * slot));
         }
-        __syncthreads();
-        const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
-        sy = (sy & ~(15u << (4 * slot))) | ((pk / NCAND) << (4 * slot));
-        su = (su & ~(15u << (4 * slot))) | ((pk % NCAND) << (4 * slot));
+        if (slot == 0) {
+            __syncthreads();   // pick_s of the column sums
+            const unsigned pk = (unsigned)__builtin_amdgcn_readfirstlane(pick_s);
+            sy = (sy & ~15u) | (pk / NCAND);
+            su = (su & ~15u) | (pk % NCAND);
+        }
         if (final_step) {
             if (t == 0) {
                 const long long rate = (long long)(nact * bits + nb * 12) << 9;
@@ -1594,8 +1634,8 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
             sy >>= 4;
             su >>= 4;
         }
-        // no barrier needed here: wave 0 rewrites pick_s and the partials only
-        // after the next step's first barrier, which every reader of them reaches
+        // the partials alternate by step parity, and a wave passes the next
+        // step's barrier only when every wave has read this step's
     }
     if (t == 0) {
         __threadfence();
