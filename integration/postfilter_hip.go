@@ -537,6 +537,15 @@ type hipPostFilterRunner struct {
 	source     *Frame // the frame the display frame was made from
 	output     *Frame
 	published  int
+
+	// Result of the last Apply (FrameWorkPostFilterResult,
+	// internal/av1/decoder/postfilter.go:309-316), as the Go runner keeps it
+	// (decoder_postfilter_bind.go:237,361): the stages the call completed
+	// and the loop filter's plan extent (Active, MICols, MIRows, Cells, as
+	// the mask path fills them, postfilter_loopfilter_pool.go:84-93).  The
+	// Go walkers' per-edge, per-unit and per-stripe counts are bookkeeping
+	// of their own scalar loops and are left zero.
+	Result internaldecoder.FrameWorkPostFilterResult
 }
 
 func newHIPPostFilterRunner(device int, outputPool *internalframe.Pool) (*hipPostFilterRunner, error) {
@@ -587,6 +596,7 @@ func (r *hipPostFilterRunner) Apply(ctx internaldecoder.FrameWorkPostFilterConte
 		return internaldecoder.ErrInvalidFrameWorkState
 	}
 	r.output, r.source, r.published = nil, nil, -1
+	r.Result = internaldecoder.FrameWorkPostFilterResult{}
 	cl := r.call
 	*cl = C.hip_pf_call{}
 	remaining := ctx.RemainingPostFilters()
@@ -697,6 +707,13 @@ func (r *hipPostFilterRunner) Apply(ctx internaldecoder.FrameWorkPostFilterConte
 	if err := r.dev.output.download(out); err != nil {
 		release()
 		return err
+	}
+	r.Result.Completed = stages
+	if stages.Has(internaldecoder.FrameWorkPostFilterLoopFilter) {
+		lf := &r.Result.LoopFilter
+		lf.Active, lf.Plan.Active = true, true
+		lf.Plan.MICols, lf.Plan.MIRows = uint16(cl.deblock.mi_cols), uint16(cl.deblock.mi_rows)
+		lf.Plan.Cells = int32(cl.deblock.mi_cols) * int32(cl.deblock.mi_rows)
 	}
 	r.output = out
 	if local >= 0 {
