@@ -1462,35 +1462,33 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
     __syncthreads();
     const int nact = nact_s;
     // blocks past nact hold zeros: best cost 0, no gain, nothing in a column sum
+    // every block's shortlist SSEs, a block's loads in flight before use
     cand8 Y[NPER], U[NPER];
+    bool big = false;
 #pragma unroll
     for (int r = 0; r < NPER; r++) {
         const int i = t + r * SMALL_T;
         const size_t fb = i < nact ? act[i] : 0;
+        uint64_t my[NCAND], mu[NCAND];
 #pragma unroll
         for (int q = 0; q < NCAND; q++) {
-            Y[r][q] = i < nact ? (int)mse[fb * NCAND + q] : 0;
-            U[r][q] = i < nact ? (int)mse[((size_t)nfb + fb) * NCAND + q] : 0;
+            my[q] = mse[fb * NCAND + q];
+            mu[q] = mse[((size_t)nfb + fb) * NCAND + q];
         }
 #pragma unroll
         for (int q = 0; q < NCAND; q++) {
-            yu[i][q] = (uint32_t)Y[r][q];
-            yu[i][NCAND + q] = (uint32_t)U[r][q];
+            const uint64_t a = i < nact ? my[q] : 0, c = i < nact ? mu[q] : 0;
+            Y[r][q] = (int)a;
+            U[r][q] = (int)c;
+            big |= ((a | c) >> 28) != 0;
+            yu[i][q] = (uint32_t)a;
+            yu[i][NCAND + q] = (uint32_t)c;
         }
     }
     // the 32-bit arithmetic below is exact when every SSE is below 2^28 (a
     // pair's cost below 2^29, eight gains below 2^32): always for 8-bit
     // frames; a 10-bit table with a larger value takes the 64-bit steps
     if (WIDE_CHECK) {
-        bool big = false;
-#pragma unroll
-        for (int r = 0; r < NPER; r++) {
-            const int i = t + r * SMALL_T;
-            const size_t fb = i < nact ? act[i] : 0;
-#pragma unroll
-            for (int q = 0; q < NCAND; q++)
-                big |= i < nact && ((mse[fb * NCAND + q] | mse[((size_t)nfb + fb) * NCAND + q]) >> 28) != 0;
-        }
         if (__syncthreads_or(big)) {
             fin_small_wide<NPER>(mse, noskip, nhfb, nvfb, mi_cols, mi_rows, lambda, cand, ss, dec, fb_index);
             return;
