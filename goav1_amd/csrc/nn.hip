@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R,
 {
     __shared__ __align__(16) uint16_t bufA[64 * 64 * 2];    // tile [64][64][2], then conv2 out [256][32]
     __shared__ __align__(16) uint16_t bufB[32 * 32 * 16];   // conv1 out [1024][16], then conv3 out [64][64]
-    __shared__ float pool[64], fc1[128];
+    __shared__ float pool[64], fc1[128], fcp[4][NN_OUT];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, g = lane >> 4, l16 = lane & 15;
     const int fb = blockIdx.x, fx = (fb % nhfb) * 64, fy = (fb / nhfb) * 64;
     const float inv = 1.f / (float)((1 << bd) - 1);
@@ -459,23 +459,35 @@ __global__ __launch_bounds__(256) void nn_infer_kernel(PlaneView S, PlaneView R,
     __syncthreads();
     if (t < 64) {   // average pool, in pixel order as the layer-by-layer pass
         float sum = 0.f;
+#pragma unroll 16
         for (int q = 0; q < 64; q++) sum += bf2f(bufB[q * 64 + t]);
         pool[t] = bf2f(f2bf(sum / 64));
     }
     __syncthreads();
-    if (t < 128) {
+    // fc1 and the heads: the inputs split in parts summed by separate threads
+    // (every weight load of a part in flight at once), then the parts added
+    // in order
+    {
+        const int o = t & 127, h = t >> 7;   // fc1: two halves of the 64 inputs
         float sum = 0.f;
-#pragma unroll 8
-        for (int c = 0; c < 64; c++) sum += pool[c] * bf2f(p.w3[c * 128 + t]);
-        fc1[t] = bf2f(f2bf(fmaxf(sum + p.b3[t], 0.f)));
+#pragma unroll
+        for (int c = 0; c < 32; c++) sum += pool[32 * h + c] * bf2f(p.w3[(32 * h + c) * 128 + o]);
+        fcp[h][o] = sum;
+    }
+    __syncthreads();
+    if (t < 128) fc1[t] = bf2f(f2bf(fmaxf(fcp[0][t] + fcp[1][t] + p.b3[t], 0.f)));
+    __syncthreads();
+    for (int it = t; it < 4 * NN_OUT; it += 256) {   // heads: four quarters of the 128 inputs
+        const int o = it % NN_OUT, q = it / NN_OUT;
+        float sum = 0.f;
+#pragma unroll
+        for (int j = 0; j < 32; j++) sum += fc1[32 * q + j] * bf2f(p.w4[(32 * q + j) * NN_OUT + o]);
+        fcp[q][o] = sum;
     }
     __syncthreads();
     float lg = 0.f;
     if (t < NN_OUT) {
-        float sum = 0.f;
-#pragma unroll 8
-        for (int j = 0; j < 128; j++) sum += fc1[j] * bf2f(p.w4[j * NN_OUT + t]);
-        lg = sum + p.b4[t];
+        lg = fcp[0][t] + fcp[1][t] + fcp[2][t] + fcp[3][t] + p.b4[t];
         logits[(int64_t)fb * NN_OUT + t] = lg;
     }
     if constexpr (PROPOSE) {
