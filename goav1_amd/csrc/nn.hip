@@ -526,7 +526,7 @@ __global__ __launch_bounds__(256) void propose_partial_kernel(const float *probs
     const int g = blockIdx.x, f0 = g * PROP_CHUNK, f1 = min(nfb, f0 + PROP_CHUNK);
     for (int c = threadIdx.x; c < NN_PROP_STRIDE; c += 256) {
         float acc = 0.f;
-#pragma unroll 8
+#pragma unroll 16
         for (int f = f0; f < f1; f++) acc += probs[(size_t)f * NN_PROP_STRIDE + c];
         partial[(size_t)g * NN_PROP_STRIDE + c] = acc;
     }
@@ -538,6 +538,7 @@ __global__ __launch_bounds__(1024) void propose_kernel(const float *partial, int
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t < NN_PROP_STRIDE) {
         float v = 0.f;
+#pragma unroll 16
         for (int g = 0; g < ng; g++) v += partial[(size_t)g * NN_PROP_STRIDE + t];
         sums[t] = v;
     }
@@ -570,11 +571,14 @@ hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, in
         hipLaunchKernelGGL((nn_infer_kernel<false, uint8_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
     else
         hipLaunchKernelGGL((nn_infer_kernel<false, uint16_t>), dim3(nfb), dim3(256), 0, s, S, R, bd, nhfb, p, logits, pp);
-    if (prop) {
-        const int ng = (nfb + PROP_CHUNK - 1) / PROP_CHUNK;
-        hipLaunchKernelGGL(propose_partial_kernel, dim3(ng), dim3(256), 0, s, prop->probs, nfb, prop->partial);
-        hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop->partial, ng, prop->cand, prop->conf);
-    }
+    return hipGetLastError();
+}
+
+hipError_t nn_propose(const NNPropose &prop, int nfb, hipStream_t s)
+{
+    const int ng = (nfb + PROP_CHUNK - 1) / PROP_CHUNK;
+    hipLaunchKernelGGL(propose_partial_kernel, dim3(ng), dim3(256), 0, s, prop.probs, nfb, prop.partial);
+    hipLaunchKernelGGL(propose_kernel, dim3(1), dim3(1024), 0, s, prop.partial, ng, prop.cand, prop.conf);
     return hipGetLastError();
 }
 

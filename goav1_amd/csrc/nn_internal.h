@@ -96,6 +96,9 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 hipError_t nn_pack_frag(const NNWeights &w, uint16_t *frag, hipStream_t s);
 hipError_t nn_infer(const NNInfer &p, const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, float *logits,
                     const NNPropose *prop, hipStream_t s);
+// The frame shortlist from the probabilities nn_infer wrote with a proposal
+// (per-strength sums in a fixed order, then the NCAND largest per plane).
+hipError_t nn_propose(const NNPropose &prop, int nfb, hipStream_t s);
 hipError_t launch_tiles(const PlaneView &S, const PlaneView &R, int bd, int nhfb, int nfb, uint16_t *tiles, hipStream_t s);
 hipError_t nn_forward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, float *logits, hipStream_t s);
 hipError_t nn_backward(const NNWeights &w, const NNWork &ws, const uint16_t *tiles, int n, const float *target,
