@@ -1950,14 +1950,14 @@ int av1pf_search_learned(av1pf_ctx *c, av1pf_nn *nn, const av1pf_frame *src, con
     prop.cand = cand;
     prop.conf = conf ? conf : conf_tmp;
     HIPCHK(c, nn_infer(infer_args(nn), sv.p[0], rv.p[0], rv.bd, nhfb, nfb, logits, &prop, s));
-    // the LR search depends only on the logits; it runs on the side stream
-    // beside the shortlist selection, whose four workgroups leave the GPU to
-    // it (beside the shortlist SSE kernel, which fills the GPU, it would only
-    // slow that kernel down).  The main stream's launches are issued first,
-    // so nothing on it waits for the host to issue the side stream's.
+    // the LR search depends only on the logits: it runs on the side stream
+    // beside the proposal, the CDEF shortlist search and the selection (the
+    // main stream's launches are issued first, so nothing on it waits for
+    // the host to issue the side stream's)
+    HIPCHK(c, hipEventRecord(c->learn_fork, s));
     HIPCHK(c, nn_propose(prop, nfb, s));
     HIPCHK(c, launch_search_shortlist(sv, rv, noskip, damping, lambda, cand, msub, decision, fb_index, c->small_state,
-                                      s, c->learn_fork));
+                                      s));
     HIPCHK(c, hipStreamWaitEvent(c->side, c->learn_fork, 0));
     const int W = recon->width, H = recon->height;
     const int hz = std::max((W + 32) / 64, 1), vt = std::max((H + 32) / 64, 1);

@@ -111,8 +111,7 @@ size_t nn_work_bytes(int max_batch);
 // the shortlist pairs.
 hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
                                    uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
-                                   uint8_t *fb_index, void *small_state, hipStream_t s,
-                                   hipEvent_t table_done = nullptr);
+                                   uint8_t *fb_index, void *small_state, hipStream_t s);
 // Device state of the shortlist selection (zeroed once): per-level results
 // and the ticket.
 size_t small_state_bytes();

@@ -1680,7 +1680,7 @@ __global__ __launch_bounds__(SMALL_T) void fin_small_kernel(const uint64_t *mse,
 
 hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, const uint64_t *noskip, int damping,
                                    uint64_t lambda, const int *cand, uint64_t *mse_sub, av1pf_cdef_decision *dec,
-                                   uint8_t *fb_index, void *small_state, hipStream_t s, hipEvent_t table_done)
+                                   uint8_t *fb_index, void *small_state, hipStream_t s)
 {
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
     const int nhfb = (mi_cols + 15) / 16, nvfb = (mi_rows + 15) / 16;
@@ -1690,10 +1690,6 @@ hipError_t launch_search_shortlist(const FrameView &src, const FrameView &rec, c
     else
         hipLaunchKernelGGL(cdef_search_subset_kernel<2>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping,
                            cand, mse_sub, nhfb, nvfb);
-    if (table_done) {
-        const hipError_t e = hipEventRecord(table_done, s);
-        if (e != hipSuccess) return e;
-    }
     const int nfb = nhfb * nvfb;
     SmallState *ss = static_cast<SmallState *>(small_state);
     const unsigned long long lam = lambda;
