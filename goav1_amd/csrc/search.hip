@@ -322,7 +322,9 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
 }
 
 // ---------------------------------------------------------------------------
-// Wave exchanges and reductions (DPP and swizzle, no LDS round trip)
+// Wave exchanges and reductions.  Lane exchanges (lane l reads lane l ^ X):
+// DPP quad permutations for distances 1 and 2, swizzles within 32 lanes for
+// 4..16, a permute for 32.
 template <int X>
 __device__ __forceinline__ uint32_t xchg32(uint32_t v)
 {
@@ -1202,8 +1204,9 @@ __global__ __launch_bounds__(256) void cdef_search_subset_kernel(FrameView src, 
 // are summed by pair lanes: lane l of a wave serves pair l % NP for the
 // (64 / NP)-th part l / NP of the wave's 64 blocks and holds y_j + uv_k of
 // those blocks, the blocks' best costs come through LDS (broadcast reads),
-// and the parts are added across lanes.  Wave 0 adds the waves' totals and
-// takes the first maximum as one 64-bit key (gain, 63 - pair).  Each level's
+// and the parts are added across lanes.  Every wave adds the waves' totals
+// (written by step parity, one barrier per step) and takes the first maximum
+// as one 64-bit key (gain, 63 - pair).  Each level's
 // cost and pairs go to `lv`; the last workgroup to finish (ticket) keeps the
 // cheapest level (the lowest on ties, as a sequential scan with a strict
 // comparison would) and writes the decision.
@@ -1219,8 +1222,6 @@ static_assert(sizeof(SmallState) <= 1024, "small selection state");
 #define SMALL_MAX_FB 2048
 #define SMALL_T 512
 
-// Lane exchanges (lane l reads lane l ^ X): DPP quad permutations for
-// distances 1 and 2, swizzles within 32 lanes for 4..16, a permute for 32.
 // A block's NCAND shortlist SSEs of one plane as one register vector:
 // indexing it with a wave-uniform (scalar) index stays in registers.
 typedef int cand8 __attribute__((ext_vector_type(NCAND)));
