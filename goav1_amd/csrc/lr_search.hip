@@ -265,8 +265,16 @@ __device__ void sgr_fit(int set, const int64_t *st, int8_t *xqd)
     }
 }
 
+// The exhaustive mode in LR_EXH_PARTS workgroups per unit, each scoring its
+// own candidates (the scores and fits do not depend on each other): part 0
+// the Wiener least-squares fit and its score, part 1 the six curated Wiener
+// triples, part 2 + s SGR set s (its preset, when it has one, then its fit).
+// lr_exh_decide_kernel then takes the decision from the scores in the
+// sequential order (lr_search_decide).
+constexpr int LR_EXH_PARTS = 18;
+
 __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneView R, int us, int hz, int vt,
-                                                            av1pf_lr_unit *out, uint64_t *sse_out, int8_t *fit_out)
+                                                            uint64_t *sse_out, int8_t *fit_out)
 {
     __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
     __shared__ uint8_t srcT[64 * 64];
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
     uint16_t *Ab = reinterpret_cast<uint16_t *>(work + LR_AB * LR_AB * 4);
     uint16_t *tmp = reinterpret_cast<uint16_t *>(work);
     const int tid = threadIdx.x;
-    const int u = blockIdx.x;
+    const int u = blockIdx.x, part = blockIdx.y;
     const int col = u % hz, row = u / hz;
     const int W = R.w, H = R.h;
     const int x0 = col * us, x1 = col == hz - 1 ? W : x0 + us;
@@ -288,8 +296,21 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
     const int w = x1 - x0, h = y1 - y0;
     uint64_t *e = sse_out + (size_t)u * 28;
     int8_t *ft = fit_out + (size_t)u * 38;
-    for (int i = tid; i < 28; i += 256) e[i] = ~0ull;
-    for (int i = tid; i < 38; i += 256) ft[i] = 0;
+    // each part's own slots: no score (~0) and no fit (0) until it finds them
+    if (tid == 0) {
+        if (part == 0) {
+            e[7] = ~0ull;
+            for (int k = 0; k < 6; k++) ft[k] = 0;
+        } else if (part == 1) {
+            for (int c = 0; c < 6; c++) e[1 + c] = ~0ull;
+        } else {
+            const int set = part - 2;
+            e[12 + set] = ~0ull;
+            for (int q = 0; q < 4; q++)
+                if (kSgrPreset[q] == set) e[8 + q] = ~0ull;
+            ft[6 + 2 * set] = ft[7 + 2 * set] = 0;
+        }
+    }
     uint64_t b = 0;
     for (int i = tid; i < w * h; i += 256) {
         const int y = y0 + i / w, x = x0 + i % w;
@@ -297,9 +318,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
         b += (uint64_t)(d * d);
     }
     const uint64_t base = wg_sum(b, red);
-    av1pf_lr_unit res;
-    for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
-    res.type = AV1PF_LR_NONE;
+    if (part == 0 && tid == 0) e[0] = base;
     if (base != 0 && w <= 64 && h <= 64) {
         const int tw = w + 6, th = h + 6;
         for (int i = tid; i < tw * th; i += 256) {
@@ -314,11 +333,10 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
         __syncthreads();
         const uint16_t *O = tile + 3 * LR_TW + LR_OX;
         const uint64_t wh = (uint64_t)(w * h);
-        uint64_t best = base;
         const LaneRows L = lane_rows(w, h);
         // ---- Wiener fit: alternating least squares over this lane's pixels
         int8_t fv[3] = { 0, 0, 0 }, fh[3] = { 0, 0, 0 };
-        for (int half = 0; half < 4; half++) {
+        for (int half = 0; part == 0 && half < 4; half++) {
             const bool horiz = (half & 1) == 0;   // solve H given V, then V given H
             const int8_t *fx = horiz ? fv : fh;
             const int ftp[7] = { fx[0], fx[1], fx[2], 128 - 2 * (fx[0] + fx[1] + fx[2]), fx[2], fx[1], fx[0] };
@@ -354,7 +372,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             for (int k = 0; k < 3; k++) dstt[k] = taps_s[horiz ? 1 : 0][k];
             __syncthreads();
         }
-        if (tid == 0)
+        if (part == 0 && tid == 0)
             for (int k = 0; k < 3; k++) { ft[k] = fv[k]; ft[3 + k] = fh[k]; }
         // ---- Wiener: six curated triples, then the fit, all exact
         uint64_t acc = 0;
@@ -365,7 +383,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
         auto err4 = [&](int y, int x, const int *v, int n) {
             for (int k = 0; k < n; k++) err(y, x + k, v[k]);
         };
-        for (int c = 0; c < 7; c++) {
+        for (int c = part == 0 ? 6 : 0; part <= 1 && c < (part == 0 ? 7 : 6); c++) {
             const int8_t *vt = c < 6 ? kWienerCand[c] : fv;
             const int8_t *ht = c < 6 ? kWienerCand[c] : fh;
             acc = 0;
@@ -373,15 +391,9 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             wiener_tile(O, w, h, 8, vt, ht, tmp, err4);
             const uint64_t sc = wg_sum(acc, red);
             if (tid == 0) e[1 + c] = sc;
-            if (sc + wh < best) {
-                best = sc;
-                for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
-                res.type = AV1PF_LR_WIENER;
-                for (int k = 0; k < 3; k++) { res.wiener_v[k] = vt[k]; res.wiener_h[k] = ht[k]; }
-            }
         }
         // ---- SGR: per set the curated preset (when it has one), then the fit
-        for (int set = 0; set < 16; set++) {
+        for (int set = part - 2; set >= 0 && set == part - 2; set++) {
             int32_t f0[SGR_PPT], f1[SGR_PPT];
             __syncthreads();
             sgr_filters(O, w, h, 8, set, Ab, Bb, f0, f1);
@@ -424,27 +436,74 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
                 }
                 const uint64_t sc = wg_sum(a2, red);
                 if (tid == 0) e[k == 0 ? 8 + p : 12 + set] = sc;
+            }
+        }
+    }
+}
+
+// The decision of the exhaustive mode from its 28 scores, in the sequential
+// order of the single-pass search: base, the six curated Wiener triples, the
+// Wiener fit, then per SGR set its preset (when it has one) and its fit; a
+// candidate replaces the running best when its SSE plus w * h is below it.
+__global__ void lr_exh_decide_kernel(int W, int H, int us, int hz, int vt, const uint64_t *sse, const int8_t *fit,
+                                     av1pf_lr_unit *out)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= hz * vt) return;
+    const int col = u % hz, row = u / hz;
+    const int x0 = col * us, x1 = col == hz - 1 ? W : x0 + us;
+    int y0 = row * us, y1 = row == vt - 1 ? H : y0 + us;
+    if (y0 > 0) y0 -= 8;
+    if (y1 < H) y1 -= 8;
+    const int w = x1 - x0, h = y1 - y0;
+    const uint64_t *e = sse + (size_t)u * 28;
+    const int8_t *ft = fit + (size_t)u * 38;
+    av1pf_lr_unit res;
+    for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
+    res.type = AV1PF_LR_NONE;
+    const uint64_t base = e[0];
+    if (base != 0 && w <= 64 && h <= 64) {
+        const uint64_t wh = (uint64_t)(w * h);
+        uint64_t best = base;
+        for (int c = 0; c < 7; c++) {
+            const uint64_t sc = e[1 + c];
+            if (sc + wh < best) {
+                best = sc;
+                for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
+                res.type = AV1PF_LR_WIENER;
+                for (int k = 0; k < 3; k++) {
+                    res.wiener_v[k] = c < 6 ? kWienerCand[c][k] : ft[k];
+                    res.wiener_h[k] = c < 6 ? kWienerCand[c][k] : ft[3 + k];
+                }
+            }
+        }
+        for (int set = 0; set < 16; set++) {
+            int p = -1;
+            for (int q = 0; q < 4; q++)
+                if (kSgrPreset[q] == set) p = q;
+            for (int k = p >= 0 ? 0 : 1; k < 2; k++) {
+                const uint64_t sc = e[k == 0 ? 8 + p : 12 + set];
                 if (sc + wh < best) {
                     best = sc;
                     for (int i = 0; i < 16; i++) reinterpret_cast<uint8_t *>(&res)[i] = 0;
                     res.type = AV1PF_LR_SGRPROJ;
                     res.sgr_set = (uint8_t)set;
-                    res.sgr_xqd[0] = xq[0];
-                    res.sgr_xqd[1] = xq[1];
+                    res.sgr_xqd[0] = k == 0 ? kSgrXqd[p][0] : ft[6 + 2 * set];
+                    res.sgr_xqd[1] = k == 0 ? kSgrXqd[p][1] : ft[7 + 2 * set];
                 }
             }
         }
     }
-    if (tid == 0) {
-        out[u] = res;
-        e[0] = base;
-    }
+    out[u] = res;
 }
 
 hipError_t launch_lr_search_exhaustive(const PlaneView &src, const PlaneView &rec, int us, int hz, int vt,
                                        av1pf_lr_unit *out, uint64_t *sse, int8_t *fit, hipStream_t s)
 {
-    hipLaunchKernelGGL(lr_search_exh_kernel, dim3(hz * vt), dim3(256), 0, s, src, rec, us, hz, vt, out, sse, fit);
+    hipLaunchKernelGGL(lr_search_exh_kernel, dim3(hz * vt, LR_EXH_PARTS), dim3(256), 0, s, src, rec, us, hz, vt, sse,
+                       fit);
+    hipLaunchKernelGGL(lr_exh_decide_kernel, dim3((hz * vt + 63) / 64), dim3(64), 0, s, rec.w, rec.h, us, hz, vt, sse,
+                       fit, out);
     return hipGetLastError();
 }
 
