@@ -345,22 +345,26 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
                 for (int j = 0; j < SGR_PPT; j++) {
                     const int y = L.ry + j, x = L.x;
                     if (y >= L.end) break;
-                    int64_t Y[7];
+                    // 32-bit sums are exact: 8-bit samples and taps within
+                    // the fit's clamps (outer taps -5..10, -23..8, -17..46,
+                    // centre 0..218: 376 in absolute sum), so |Y| < 2^17,
+                    // |z| < 2^19 and |t| < 2^24; the products need 64 bits
+                    int32_t Y[7];
 #pragma unroll
                     for (int k = 0; k < 7; k++) {
-                        int64_t acc = 0;
+                        int32_t acc = 0;
 #pragma unroll
                         for (int q = 0; q < 7; q++) {
                             const int yy = horiz ? y + q - 3 : y + k - 3, xx = horiz ? x + k - 3 : x + q - 3;
-                            acc += (int64_t)ftp[q] * O[yy * LR_TW + xx];
+                            acc += ftp[q] * (int32_t)O[yy * LR_TW + xx];
                         }
                         Y[k] = acc;
                     }
-                    const int64_t z0 = Y[0] + Y[6] - 2 * Y[3], z1 = Y[1] + Y[5] - 2 * Y[3], z2 = Y[2] + Y[4] - 2 * Y[3];
-                    const int64_t t = ((int64_t)srcT[y * 64 + x] << 14) - 128 * Y[3];
-                    st[0] += z0 * z0; st[1] += z0 * z1; st[2] += z0 * z2;
-                    st[3] += z1 * z1; st[4] += z1 * z2; st[5] += z2 * z2;
-                    st[6] += z0 * t; st[7] += z1 * t; st[8] += z2 * t;
+                    const int32_t z0 = Y[0] + Y[6] - 2 * Y[3], z1 = Y[1] + Y[5] - 2 * Y[3], z2 = Y[2] + Y[4] - 2 * Y[3];
+                    const int32_t t = ((int32_t)srcT[y * 64 + x] << 14) - 128 * Y[3];
+                    st[0] += (int64_t)z0 * z0; st[1] += (int64_t)z0 * z1; st[2] += (int64_t)z0 * z2;
+                    st[3] += (int64_t)z1 * z1; st[4] += (int64_t)z1 * z2; st[5] += (int64_t)z2 * z2;
+                    st[6] += (int64_t)z0 * t; st[7] += (int64_t)z1 * t; st[8] += (int64_t)z2 * t;
                 }
             }
             int64_t tot[9];
