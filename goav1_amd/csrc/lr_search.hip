@@ -40,6 +40,26 @@ __device__ __forceinline__ uint64_t wg_sum(uint64_t v, uint64_t *red)
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// N sums over the workgroup at once: the waves' shuffle reductions of the N
+// values overlap, and one pair of barriers serves all of them (red: 4 N words)
+template <int N>
+__device__ __forceinline__ void wg_sum_n(int64_t (&v)[N], uint64_t *red)
+{
+    uint64_t u[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) u[k] = (uint64_t)v[k];
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < N; k++) u[k] += __shfl_xor(u[k], o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < N; k++) red[(threadIdx.x >> 6) * N + k] = u[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = (int64_t)(red[k] + red[N + k] + red[2 * N + k] + red[3 * N + k]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void lr_search_kernel(PlaneView S, PlaneView R, int bd, int us, int hz, int vt,
                                                         const uint8_t *mask, av1pf_lr_unit *out, uint64_t *sse_out)
@@ -279,7 +299,7 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
     __shared__ __align__(16) uint16_t tile[LR_TW * LR_TH];
     __shared__ uint8_t srcT[64 * 64];
     __shared__ __align__(16) char work[LR_AB * LR_AB * 6];   // Wiener intermediate, then SGR A/B
-    __shared__ uint64_t red[4];
+    __shared__ uint64_t red[4 * 9];
     __shared__ int8_t taps_s[2][3];
     __shared__ int8_t xqd_s[2];
     int32_t *Bb = reinterpret_cast<int32_t *>(work);
@@ -369,7 +389,8 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             }
             int64_t tot[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) tot[k] = (int64_t)wg_sum((uint64_t)st[k], red);
+            for (int k = 0; k < 9; k++) tot[k] = st[k];
+            wg_sum_n(tot, red);
             if (tid == 0) solve_taps(tot, tot + 6, taps_s[horiz ? 1 : 0]);
             __syncthreads();
             int8_t *dstt = horiz ? fh : fv;
@@ -416,7 +437,8 @@ __global__ __launch_bounds__(256) void lr_search_exh_kernel(PlaneView S, PlaneVi
             }
             int64_t tot[5];
 #pragma unroll
-            for (int k = 0; k < 5; k++) tot[k] = (int64_t)wg_sum((uint64_t)st[k], red);
+            for (int k = 0; k < 5; k++) tot[k] = st[k];
+            wg_sum_n(tot, red);
             if (tid == 0) sgr_fit(set, tot, xqd_s);
             __syncthreads();
             int8_t fx[2] = { xqd_s[0], xqd_s[1] };
