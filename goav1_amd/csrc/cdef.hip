@@ -470,16 +470,20 @@ __device__ __forceinline__ void cdef_unit(const FrameView &src, const FrameView 
         // immediate row offset.  Packed 16-bit arithmetic (values stay below
         // 2^15 in magnitude: samples < 2^12, the sentinel -2^14, the weighted
         // sum below 2^12).  Lanes of a wave cover whole rows of a block row,
-        // so the stores coalesce.
+        // so the stores coalesce.  Blocks 4 wide (chroma subsampled across)
+        // give 128 items: two lanes take each, one half of its rows apiece.
         const int P = bw >> 1;               // pairs per block row: 2 or 4
         const int per_brow = 8 * P;          // items per block row (8 block columns)
-        if (tid < brows * per_brow) {
-            const int br = tid / per_brow, c = tid - br * per_brow;
+        const int hr = P == 2 ? bh >> 1 : bh;                  // rows per lane
+        const int it = P == 2 ? (tid & 127) : tid;
+        const int r0 = P == 2 ? (tid >> 7) * hr : 0;
+        if (it < brows * per_brow) {
+            const int br = it / per_brow, c = it - br * per_brow;
             const int bc = c / P;
             const int x = bc * bw + 2 * (c - bc * P);
-            const int y0 = br * bh;
-            const int rows = min(bh, uh - y0);
-            if (x < uw) {
+            const int y0 = br * bh + r0;
+            const int rows = min(hr, uh - y0);
+            if (x < uw && rows > 0) {
                 const BlockRec r = brec[br * 8 + bc];
                 const uint32_t *Wt = reinterpret_cast<const uint32_t *>(tile);
                 const int w0 = (int)((org - tile + y0 * TPITCH + x) >> 1);   // word of the pair in row y0
