@@ -245,7 +245,8 @@ hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t s);
 hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const uint64_t *noskip,
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s);
 
-// One launch (cooperative: every workgroup resident); epoch counts the
+// One launch (every workgroup resident; refused when the grid does not fit the
+// device); epoch counts the
 // launches on this state from 1 (the state is zero before the first); bit 1
 // of err is set if a barrier did not complete.
 hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int nhfb, int nvfb, int mi_cols, int mi_rows,

@@ -965,16 +965,27 @@ hipError_t launch_cdef_select(const uint64_t *mse, const uint64_t *noskip, int n
             if (fa.sharedSizeBytes + sel_lds_budget(tables) > 160 * 1024) return hipErrorInvalidValue;
             e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sel_lds_budget(tables));
             if (e != hipSuccess) return e;
+            // every workgroup must be resident at once (a workgroup waits for
+            // the slots of the other workgroups of its level, and for the
+            // chroma copy of all): the device holds the grid at the largest
+            // LDS request, or the launch is refused here
+            int cus = 0, per_cu = 0;
+            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e != hipSuccess) return e;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sel_threads(tables), sel_lds_budget(tables));
+            if (e != hipSuccess) return e;
+            if ((long)cus * per_cu < SEL_GRID) return hipErrorCooperativeLaunchTooLarge;
             ready[{ dev, tables }] = true;
         }
     }
     void *args[] = { (void *)&mse, (void *)&noskip, (void *)&nhfb, (void *)&nvfb, (void *)&mi_cols, (void *)&mi_rows,
                      (void *)&lam, (void *)&st, (void *)&epoch, (void *)&use_cc, (void *)&dec, (void *)&fb_index,
                      (void *)&err };
-    // a cooperative launch: every workgroup resident at once (a workgroup
-    // waits for the slots of the other workgroups of its level, and for the
-    // chroma copy of all)
-    return hipLaunchCooperativeKernel(fn, dim3(SEL_GRID), dim3(sel_threads(tables)), args, (unsigned int)lds, s);
+    // an ordinary launch (the grid fits the device, checked above): the
+    // cooperative queue's hand-off cost 15 us per call.  Workgroups that other
+    // kernels keep from starting are waited for; every wait is bounded and
+    // flags the launch aborted (bit 1 of the error flag) when it runs out.
+    return hipLaunchKernel(fn, dim3(SEL_GRID), dim3(sel_threads(tables)), args, lds, s);
 }
 
 // ---------------------------------------------------------------------------
