@@ -237,7 +237,9 @@ __device__ __forceinline__ void s_reduce(uint32_t acc[64], uint32_t (*part)[64],
 }
 
 // CS = bit depth - 8 (0: 8-bit samples, 2: 10-bit); damping is the frame's
-// (the decoder damping of the trials is damping + CS, one less for chroma)
+// (the decoder damping of the trials is damping + CS, one less for chroma).
+// Two workgroups per filter block (blockIdx.z): 0 the luma row of the table,
+// 1 the chroma row; both search the luma directions, which chroma filters with.
 template <int CS>
 __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameView rec, const uint64_t *noskip,
                                                           int damping, uint64_t *mse, int nhfb, int nvfb)
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     __shared__ int nblk_s;
     __shared__ uint32_t part[4][64];
     const int tid = threadIdx.x;
-    const int fbc = blockIdx.x, fbr = blockIdx.y;
+    const int fbc = blockIdx.x, fbr = blockIdx.y, chroma = blockIdx.z;
     const int fb = fbr * nhfb + fbc;
     const int nfb = nhfb * nvfb;
     const int mi_cols = ((rec.w + 7) >> 3) << 1, mi_rows = ((rec.h + 7) >> 3) << 1;
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
     const int nblk = nblk_s;
     uint64_t *m0 = mse + (size_t)fb * 64, *m1 = mse + ((size_t)nfb + fb) * 64;
     if (nblk == 0) {
-        if (tid < 64) { m0[tid] = 0; m1[tid] = 0; }
+        if (tid < 64) (chroma ? m1 : m0)[tid] = 0;
         return;
     }
     uint32_t acc[64];
@@ -286,18 +288,21 @@ __global__ __launch_bounds__(256) void cdef_search_kernel(FrameView src, FrameVi
             badj[b] = (int8_t)i;
         }
         __syncthreads();
+        if (!chroma) {
 #pragma unroll
-        for (int gi = 0; gi < 64; gi++) acc[gi] = 0;
-        const int npx = nblk * 64;
-        for (int p = tid; p < npx; p += 256) {
-            const int b = blist[p >> 6], q = p & 63;
-            const int by = b >> 3, bx = b & 7, yy = by * 8 + (q >> 3), xx = bx * 8 + (q & 7);
-            const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
-            s_pixel<true, CS>(acc, c, srcT[yy * 64 + xx], bdir[b], badj[b], badj[b] >= 0, damping + CS);
+            for (int gi = 0; gi < 64; gi++) acc[gi] = 0;
+            const int npx = nblk * 64;
+            for (int p = tid; p < npx; p += 256) {
+                const int b = blist[p >> 6], q = p & 63;
+                const int by = b >> 3, bx = b & 7, yy = by * 8 + (q >> 3), xx = bx * 8 + (q & 7);
+                const uint16_t *c = tile + (2 + yy) * SP + 8 + xx;
+                s_pixel<true, CS>(acc, c, srcT[yy * 64 + xx], bdir[b], badj[b], badj[b] >= 0, damping + CS);
+            }
+            s_reduce(acc, part, tid);
+            __syncthreads();
+            if (tid < 64) m0[tid] = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+            return;
         }
-        s_reduce(acc, part, tid);
-        __syncthreads();
-        if (tid < 64) m0[tid] = (uint64_t)part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
     }
     // ---- chroma (U + V)
 #pragma unroll
@@ -1724,10 +1729,10 @@ hipError_t launch_cdef_search(const FrameView &src, const FrameView &rec, const 
                               int damping, uint64_t *mse, int nhfb, int nvfb, hipStream_t s)
 {
     if (rec.bd == 8)
-        hipLaunchKernelGGL(cdef_search_kernel<0>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse,
+        hipLaunchKernelGGL(cdef_search_kernel<0>, dim3(nhfb, nvfb, 2), dim3(256), 0, s, src, rec, noskip, damping, mse,
                            nhfb, nvfb);
     else
-        hipLaunchKernelGGL(cdef_search_kernel<2>, dim3(nhfb, nvfb), dim3(256), 0, s, src, rec, noskip, damping, mse,
+        hipLaunchKernelGGL(cdef_search_kernel<2>, dim3(nhfb, nvfb, 2), dim3(256), 0, s, src, rec, noskip, damping, mse,
                            nhfb, nvfb);
     return hipGetLastError();
 }
